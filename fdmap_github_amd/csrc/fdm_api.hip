@@ -1,0 +1,1032 @@
+// fdm_api.hip -- C ABI (include/fdmap.h) of the MI355X time-stepping
+// library: device-resident storage, kernel launches and host<->device
+// transfers.  One HIP stream per context.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fdm_bulk.hpp"
+
+namespace fdm {
+
+// mms_sin (mms.f90:52-234) on the device; same expression order as the
+// reference.  field: 0 vx,1 vy,2 sxx,3 sxy,4 syy,5 szz, 6..11 sources.
+__device__ double mms_sin(double x, double y, double t, int side, int field) {
+  const double G = 32.0, cs = 3.0, cp = 5.0;
+  double rho = G / (cs * cs);
+  double lambda = rho * (cp * cp) - 2.0 * G;
+  double Zs = rho * cs;
+  double pi = 4.0 * atan(1.0);
+  double kx = 2.0 * pi, ky = 2.0 * pi, mv = 2.0;
+  double w = (4.0 / 0.46) * pi;
+  double ft = cos(w * t);
+  double ft_t = -w * sin(w * t);
+  double mf = 0.1, kf = 1.0 * pi;
+  double ftpp = mf * kf * cos(kf * x);
+  double ftpp_x = -mf * (kf * kf) * sin(kf * x);
+  double nx = ftpp / sqrt(1.0 + ftpp * ftpp);
+  double ny = -1.0 / sqrt(1.0 + ftpp * ftpp);
+  double mx = ny, my = -nx;
+  double q = 1.0 + ftpp * ftpp;
+  double q3 = q * q * q;
+  double nx_x = ftpp_x / sqrt(q3);
+  double ny_x = ftpp * ftpp_x / sqrt(q3);
+  double mx_x = ny_x, my_x = -nx_x;
+  double nx_y = 0.0, ny_y = 0.0;
+  double mx_y = ny_y, my_y = -nx_y;
+  double sd = (double)side;
+  double amp = 2.0 * 12.6 / Zs;
+  double ckx = cos(kx * x), sky = sin(ky * y), cky = cos(ky * y), skx = sin(kx * x);
+  double vx = mv * (3.0 - 2.0 * sd) * amp * ckx * cky * ft;
+  double vx_t = mv * (3.0 - 2.0 * sd) * amp * ckx * cky * ft_t;
+  double vx_x = -mv * (3.0 - 2.0 * sd) * kx * amp * skx * cky * ft;
+  double vx_y = -mv * (3.0 - 2.0 * sd) * ky * amp * ckx * sky * ft;
+  double vy = amp * ckx * cky * ft;
+  double vy_t = amp * ckx * cky * ft_t;
+  double vy_x = -kx * amp * skx * cky * ft;
+  double vy_y = -ky * amp * ckx * sky * ft;
+  double syy = -63.0 * ckx * cky * ft - 126.0;
+  double syy_t = -63.0 * ckx * cky * ft_t;
+  double syy_x = kx * 63.0 * skx * cky * ft;
+  double syy_y = ky * 63.0 * ckx * sky * ft;
+  double sxy = 0.6 * 126.0 * ckx * cky * ft;
+  double sxy_t = 0.6 * 126.0 * ckx * cky * ft_t;
+  double sxy_x = -kx * 0.6 * 126.0 * skx * cky * ft;
+  double sxy_y = -ky * 0.6 * 126.0 * ckx * sky * ft;
+  double sxx = (2.0 * sd - 3.0) * 0.6 * 1260.0 * ckx * cky * ft;
+  double sxx_t = (2.0 * sd - 3.0) * 0.6 * 1260.0 * ckx * cky * ft_t;
+  double sxx_x = -(2.0 * sd - 3.0) * kx * 0.6 * 1260.0 * skx * cky * ft;
+  double sxx_y = -(2.0 * sd - 3.0) * ky * 0.6 * 1260.0 * ckx * sky * ft;
+  switch (field) {
+    case 5:
+      return 0.0;
+    case 0:
+      return ny * vx - my * vy;
+    case 1:
+      return mx * vy - nx * vx;
+    case 2:
+      return syy * (my * my) + sxx * (ny * ny) - 2.0 * sxy * my * ny;
+    case 4:
+      return syy * (mx * mx) + sxx * (nx * nx) - 2.0 * sxy * mx * nx;
+    case 3:
+      return -syy * my * mx - sxx * ny * nx + sxy * (ny * mx + nx * my);
+    case 6: {
+      double vxt = ny * vx_t - my * vy_t;
+      double sxxx = syy_x * (my * my) + sxx_x * (ny * ny) - 2.0 * sxy_x * my * ny +
+                    2.0 * syy * my * my_x + 2.0 * sxx * ny * ny_x -
+                    2.0 * sxy * (my_x * ny + my * ny_x);
+      double sxyy = -syy_y * my * mx - sxx_y * ny * nx + sxy_y * (ny * mx + nx * my) -
+                    syy * (my_y * mx + my * mx_y) - sxx * (ny_y * nx + ny * nx_y) +
+                    sxy * (ny_y * mx + ny * mx_y + nx_y * my + nx * my_y);
+      return vxt - (1.0 / rho) * (sxxx + sxyy);
+    }
+    case 7: {
+      double vyt = mx * vy_t - nx * vx_t;
+      double sxyx = -syy_x * my * mx - sxx_x * ny * nx + sxy_x * (ny * mx + nx * my) -
+                    syy * (my_x * mx + my * mx_x) - sxx * (ny_x * nx + ny * nx_x) +
+                    sxy * (ny_x * mx + ny * mx_x + nx_x * my + nx * my_x);
+      double syyy = syy_y * (mx * mx) + sxx_y * (nx * nx) - 2.0 * sxy_y * mx * nx +
+                    2.0 * syy * mx * mx_y + 2.0 * sxx * nx * nx_y -
+                    2.0 * sxy * (mx_y * nx + mx * nx_y);
+      return vyt - (1.0 / rho) * (sxyx + syyy);
+    }
+    case 8: {
+      double sxxt = syy_t * (my * my) + sxx_t * (ny * ny) - 2.0 * sxy_t * my * ny;
+      double vxx = ny_x * vx - my_x * vy + ny * vx_x - my * vy_x;
+      double vyy = mx_y * vy - nx_y * vx + mx * vy_y - nx * vx_y;
+      return sxxt - (lambda + 2.0 * G) * vxx - lambda * vyy;
+    }
+    case 10: {
+      double vxx = ny_x * vx - my_x * vy + ny * vx_x - my * vy_x;
+      double vyy = mx_y * vy - nx_y * vx + mx * vy_y - nx * vx_y;
+      double syyt = syy_t * (mx * mx) + sxx_t * (nx * nx) - 2.0 * sxy_t * mx * nx;
+      return syyt - lambda * vxx - (lambda + 2.0 * G) * vyy;
+    }
+    case 9: {
+      double vxy = ny_y * vx - my_y * vy + ny * vx_y - my * vy_y;
+      double vyx = mx_x * vy - nx_x * vx + mx * vy_x - nx * vx_x;
+      double sxyt = -syy_t * my * mx - sxx_t * ny * nx + sxy_t * (ny * mx + nx * my);
+      return sxyt - G * (vyx + vxy);
+    }
+    case 11: {
+      double szzt = 0.0;
+      double vxx = ny_x * vx - my_x * vy + ny * vx_x - my * vy_x;
+      double vyy = mx_y * vy - nx_y * vx + mx * vy_y - nx * vx_y;
+      return szzt - lambda * vxx - lambda * vyy;
+    }
+    default:
+      return 0.0;
+  }
+}
+
+}  // namespace fdm
+
+using namespace fdm;
+
+namespace {
+
+std::string g_create_error;
+
+struct EventPair {
+  hipEvent_t a, b;
+};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+struct fdm_ctx {
+  int nx = 0, ny = 0, ld = 0, nb = 0, ni = 0, nbx = 0, nby = 0, nstage = 0;
+  long long ps = 0;  // plane stride in elements
+  int precision = FDM_FP64;
+  size_t esz = 8;
+  double rkA[FDM_MAX_STAGES], rkB[FDM_MAX_STAGES], rkC[FDM_MAX_STAGES + 1];
+  hipStream_t stream = nullptr;
+  int device = 0;
+  // bulk storage (element type per precision)
+  void* F[2] = {nullptr, nullptr};
+  int cur = 0;
+  void* DF = nullptr;
+  void* met[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};  // xq xr yq yr Ji
+  double* X = nullptr;
+  double* Y = nullptr;
+  int* colb = nullptr;
+  int* rowb = nullptr;
+  int* blk = nullptr;
+  BlockDev* blocks = nullptr;
+  std::vector<BlockDev> hblocks;
+  // sides
+  int NS = 0;
+  double* sM = nullptr;
+  double* snormal = nullptr;
+  double* sF0 = nullptr;
+  double* sx = nullptr;
+  double* sy = nullptr;
+  double* sFhat = nullptr;
+  // interfaces
+  int NI = 0;
+  std::vector<IfaceDev> hifaces;
+  IfaceDev* ifaces = nullptr;
+  double* inhat = nullptr;
+  double* irs = nullptr;
+  double* isw = nullptr;
+  double* iS0b = nullptr;
+  double* iN0b = nullptr;
+  double* ildS = nullptr;
+  double* ildN = nullptr;
+  int* iout = nullptr;
+  double* ist = nullptr;
+  double* ist1 = nullptr;
+  bool any_friction = false;
+  std::vector<double> ld_T;  // per interface
+  // edge tasks
+  int ntasks = 0;
+  int2* tasks = nullptr;
+  // source
+  fdm_source src{};
+  SourcePlaceDev* dsrc = nullptr;
+  // plasticity
+  bool any_plastic = false;
+  double* gammap = nullptr;
+  double* lambda = nullptr;
+  int* fail = nullptr;
+  // profiling
+  bool prof = false;
+  std::vector<EventPair> pool;
+  size_t pool_used = 0;
+  double bulk_ms = 0.0, edge_ms = 0.0;
+  long long bulk_n = 0, edge_n = 0;
+  std::vector<int> pair_kind;
+  std::string err;
+  std::vector<void*> allocs;
+};
+
+namespace {
+
+int set_err(fdm_ctx* c, int code, const std::string& msg) {
+  if (c)
+    c->err = msg;
+  else
+    g_create_error = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                            \
+  do {                                                                               \
+    hipError_t e_ = (call);                                                          \
+    if (e_ != hipSuccess)                                                            \
+      return set_err((ctx), FDM_EDEVICE,                                             \
+                     std::string(#call) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+template <typename P>
+int dalloc(fdm_ctx* c, P** p, size_t bytes) {
+  void* q = nullptr;
+  if (bytes == 0) bytes = 8;
+  hipError_t e = hipMalloc(&q, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return set_err(c, FDM_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  c->allocs.push_back(q);
+  *p = (P*)q;
+  return FDM_OK;
+}
+
+template <typename P>
+int dupload(fdm_ctx* c, P** p, const void* host, size_t bytes) {
+  int rc = dalloc(c, p, bytes);
+  if (rc) return rc;
+  if (host && bytes) HIPCHK(c, hipMemcpy(*p, host, bytes, hipMemcpyHostToDevice));
+  return FDM_OK;
+}
+
+int side_len(const fdm_block& b, int s) { return s < 2 ? (b.py - b.my + 1) : (b.px - b.mx + 1); }
+
+// copy a [ny][nx] double plane into a padded [ny][ld] plane of type T
+template <typename T>
+void pack_plane(std::vector<T>& out, const double* in, int nx, int ny, int ld) {
+  out.assign((size_t)ld * ny, (T)0);
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) out[(size_t)j * ld + i] = (T)in[(size_t)j * nx + i];
+}
+
+// source time function (source.f90:212-223), evaluated on the host
+double stf_amplitude(const fdm_source& s, double t) {
+  const double pi = 3.14159265358979323846264338327950288419716939937510;
+  double tc = s.t_center, td = s.t_duration;
+  switch (s.stf) {
+    case FDM_STF_RAMP:
+      return std::fmax(std::fmin((t - tc) / td, 1.0), 0.0);
+    case FDM_STF_SINUSOID:
+      return std::sin(2.0 * pi * (t - tc) / td);
+    case FDM_STF_GAUSSIAN:
+      return std::exp(-0.5 * ((t - tc) * (t - tc)) / (td * td));
+    case FDM_STF_RICKER:
+      return (1.0 - 2.0 * (pi * pi) * ((t - tc) * (t - tc)) / (td * td)) *
+             std::exp(-(pi * pi) * ((t - tc) * (t - tc)) / (td * td));
+    default:
+      return 0.0;
+  }
+}
+
+// load_ramp (friction.f90:1455-1484), evaluated on the host
+double load_ramp(double t, double T) {
+  if (T <= 0.0) return (t < 0.0) ? 0.0 : 1.0;
+  return (t < 0.0) ? 0.0 : 1.0 - std::exp(-t / T);
+}
+
+EventPair* prof_begin(fdm_ctx* c, int kind) {
+  if (!c->prof) return nullptr;
+  if (c->pool_used == c->pool.size()) {
+    if (c->pool.size() < 65536) {
+      EventPair ep;
+      if (hipEventCreate(&ep.a) != hipSuccess || hipEventCreate(&ep.b) != hipSuccess) return nullptr;
+      c->pool.push_back(ep);
+      c->pair_kind.push_back(kind);
+    } else {
+      // drain: accumulate and recycle
+      (void)hipStreamSynchronize(c->stream);
+      for (size_t k = 0; k < c->pool_used; k++) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, c->pool[k].a, c->pool[k].b);
+        if (c->pair_kind[k] == 0) c->bulk_ms += ms; else c->edge_ms += ms;
+      }
+      c->pool_used = 0;
+    }
+  }
+  EventPair* ep = &c->pool[c->pool_used];
+  c->pair_kind[c->pool_used] = kind;
+  c->pool_used++;
+  (void)hipEventRecord(ep->a, c->stream);
+  return ep;
+}
+
+void prof_end(fdm_ctx* c, EventPair* ep, int kind) {
+  if (!ep) return;
+  (void)hipEventRecord(ep->b, c->stream);
+  if (kind == 0) c->bulk_n++; else c->edge_n++;
+}
+
+template <typename T>
+int launch_edges(fdm_ctx* c, double t, double Bdt, double A, int do_scale, int do_update,
+                 int snapshot, double Bload) {
+  if (c->ntasks == 0) return FDM_OK;
+  EdgeArgs<T> a;
+  a.F = (const T*)c->F[c->cur];
+  a.ld = c->ld;
+  a.ps = c->ps;
+  a.tasks = c->tasks;
+  a.ntasks = c->ntasks;
+  a.blocks = c->blocks;
+  a.sd = SidesDev{c->sM, c->snormal, c->sF0, c->sx, c->sy, c->sFhat, c->NS};
+  a.fi = IfacesDev{c->ifaces, c->inhat, c->irs, c->isw, c->iS0b, c->iN0b, c->ildS, c->ildN,
+                   c->iout, c->ist, c->ist1, c->NI};
+  a.t = t;
+  a.Bdt = Bdt;
+  a.A = A;
+  a.Bload = Bload;
+  a.do_scale = do_scale;
+  a.do_update = do_update;
+  a.snapshot = snapshot;
+  a.fail = c->fail;
+  dim3 grid((c->ntasks + 63) / 64), block(64);
+  EventPair* ep = prof_begin(c, 1);
+  hipLaunchKernelGGL(edge_kernel<T>, grid, block, 0, c->stream, a);
+  prof_end(c, ep, 1);
+  HIPCHK(c, hipGetLastError());
+  return FDM_OK;
+}
+
+template <typename T>
+int launch_bulk(fdm_ctx* c, double A, double Bdt, int do_update, double t, double srcA,
+                int do_plastic, double dt) {
+  BulkArgs<T> a;
+  a.Fin = (const T*)c->F[c->cur];
+  a.Fout = (T*)c->F[1 - c->cur];
+  a.DF = (T*)c->DF;
+  a.xq = (const T*)c->met[0];
+  a.xr = (const T*)c->met[1];
+  a.yq = (const T*)c->met[2];
+  a.yr = (const T*)c->met[3];
+  a.Ji = (const T*)c->met[4];
+  a.X = c->X;
+  a.Y = c->Y;
+  a.nx = c->nx;
+  a.ny = c->ny;
+  a.ld = c->ld;
+  a.ps = c->ps;
+  a.colb = c->colb;
+  a.rowb = c->rowb;
+  a.blk = c->blk;
+  a.nbx = c->nbx;
+  a.blocks = c->blocks;
+  a.sd = SidesDev{c->sM, c->snormal, c->sF0, c->sx, c->sy, c->sFhat, c->NS};
+  a.A = A;
+  a.a_zero = (A == 0.0);
+  a.Bdt = Bdt;
+  a.do_update = do_update;
+  a.t = t;
+  a.src = c->dsrc;
+  a.nplace = c->src.nplace;
+  a.srcA = srcA;
+  a.Fx = c->src.Fx;
+  a.Fy = c->src.Fy;
+  a.Mxx = c->src.Mxx;
+  a.Mxy = c->src.Mxy;
+  a.Myy = c->src.Myy;
+  a.do_plastic = do_plastic && c->any_plastic;
+  a.dt = dt;
+  a.gammap = c->gammap;
+  a.lambda = c->lambda;
+  dim3 block(64, 4);
+  dim3 grid((c->nx + 63) / 64, (c->ny + 3) / 4);
+  EventPair* ep = prof_begin(c, 0);
+  hipLaunchKernelGGL(bulk_kernel<T>, grid, block, 0, c->stream, a);
+  prof_end(c, ep, 0);
+  HIPCHK(c, hipGetLastError());
+  return FDM_OK;
+}
+
+template <typename T>
+int run_stage(fdm_ctx* c, int stage, double t0, double dt, int final_step, int snapshot) {
+  double A = c->rkA[stage - 1];
+  double Bdt = c->rkB[stage - 1] * dt;
+  double t = t0 + c->rkC[stage - 1] * dt;  // D%t during this stage (time_step.f90:137)
+  double Bload = c->ld_T.empty() ? 1.0 : load_ramp(t, c->ld_T[0]);
+  int rc = launch_edges<T>(c, t, Bdt, A, 1, final_step ? 0 : 1, snapshot, Bload);
+  if (rc) return rc;
+  double srcA = c->src.nplace ? stf_amplitude(c->src, t) : 0.0;
+  int last = (stage == c->nstage);
+  rc = launch_bulk<T>(c, A, Bdt, final_step ? 0 : 1, t, srcA, (!final_step && last) ? 1 : 0, dt);
+  if (rc) return rc;
+  if (!final_step) c->cur = 1 - c->cur;
+  return FDM_OK;
+}
+
+template <typename T>
+int step_impl(fdm_ctx* c, double t, double dt, long long n_first, int nsteps) {
+  for (int k = 0; k < nsteps; k++) {
+    long long n = n_first + k;
+    double t0 = t + (double)(n - 1) * dt;  // main.f90:176
+    for (int stage = 1; stage <= c->nstage; stage++) {
+      int rc = run_stage<T>(c, stage, t0, dt, 0, stage == 1);
+      if (rc) return rc;
+    }
+  }
+  return FDM_OK;
+}
+
+int check_fail(fdm_ctx* c) {
+  if (!c->fail) return FDM_OK;
+  int h = 0;
+  HIPCHK(c, hipMemcpyAsync(&h, c->fail, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (h) return set_err(c, FDM_ENEWTON, "friction Newton solve failed (friction.f90:1012-1036)");
+  return FDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fdm_last_error(const fdm_ctx* c) { return c ? c->err.c_str() : g_create_error.c_str(); }
+
+void fdm_destroy(fdm_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto& ep : c->pool) {
+    (void)hipEventDestroy(ep.a);
+    (void)hipEventDestroy(ep.b);
+  }
+  for (void* p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int fdm_create(const fdm_problem* P, fdm_ctx** out) {
+  if (!out) return set_err(nullptr, FDM_EINVAL, "fdm_create: null output pointer");
+  *out = nullptr;
+  if (!P || P->nx < 1 || P->ny < 1 || P->nblocks < 1 || P->nstage < 1 ||
+      P->nstage > FDM_MAX_STAGES || !P->blocks || (P->nifaces > 0 && !P->ifaces) || !P->xq ||
+      !P->xr || !P->yq || !P->yr || !P->Ji)
+    return set_err(nullptr, FDM_EINVAL, "fdm_create: malformed problem description");
+  if (P->precision != FDM_FP64 && P->precision != FDM_FP32_BULK)
+    return set_err(nullptr, FDM_EINVAL, "fdm_create: unknown precision");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return set_err(nullptr, FDM_EDEVICE, "fdm_create: no HIP device available");
+  if (P->source.nplace < 0 || P->source.nplace > FDM_MAX_SOURCE_PLACES)
+    return set_err(nullptr, FDM_EINVAL, "fdm_create: bad source placement count");
+
+  fdm_ctx* c = new fdm_ctx();
+  c->nx = P->nx;
+  c->ny = P->ny;
+  c->nb = P->nblocks;
+  c->ni = P->nifaces;
+  c->nstage = P->nstage;
+  c->precision = P->precision;
+  c->esz = P->precision == FDM_FP64 ? 8 : 4;
+  for (int s = 0; s < P->nstage; s++) {
+    c->rkA[s] = P->rkA[s];
+    c->rkB[s] = P->rkB[s];
+  }
+  for (int s = 0; s <= P->nstage; s++) c->rkC[s] = P->rkC[s];
+  // row pitch: a multiple of 64 elements so every row starts 256/512-byte aligned
+  c->ld = (P->nx + 63) / 64 * 64;
+  c->ps = (long long)c->ld * P->ny;
+  (void)hipGetDevice(&c->device);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return set_err(nullptr, FDM_EDEVICE, "fdm_create: hipStreamCreate failed");
+  }
+  auto fail = [&](int rc) {
+    g_create_error = c->err.empty() ? g_create_error : c->err;
+    fdm_destroy(c);
+    return rc;
+  };
+
+  // ---- block layout: blocks must tile the grid as a tensor product ----
+  std::vector<int> xs, ys;
+  for (int b = 0; b < c->nb; b++) {
+    const fdm_block& B = P->blocks[b];
+    if (B.mx < 0 || B.px >= P->nx || B.my < 0 || B.py >= P->ny || B.px - B.mx + 1 < 8 ||
+        B.py - B.my + 1 < 8) {
+      set_err(c, FDM_EINVAL, "fdm_create: block " + std::to_string(b) +
+                                 " out of range or with fewer than 8 points per direction");
+      return fail(FDM_EINVAL);
+    }
+    xs.push_back(B.mx);
+    ys.push_back(B.my);
+  }
+  std::sort(xs.begin(), xs.end());
+  xs.erase(std::unique(xs.begin(), xs.end()), xs.end());
+  std::sort(ys.begin(), ys.end());
+  ys.erase(std::unique(ys.begin(), ys.end()), ys.end());
+  c->nbx = (int)xs.size();
+  c->nby = (int)ys.size();
+  std::vector<int> blk((size_t)c->nbx * c->nby, -1), colb(P->nx, -1), rowb(P->ny, -1);
+  for (int b = 0; b < c->nb; b++) {
+    const fdm_block& B = P->blocks[b];
+    int cx = int(std::lower_bound(xs.begin(), xs.end(), B.mx) - xs.begin());
+    int cy = int(std::lower_bound(ys.begin(), ys.end(), B.my) - ys.begin());
+    if (blk[(size_t)cy * c->nbx + cx] != -1) {
+      set_err(c, FDM_EINVAL, "fdm_create: two blocks share a corner index");
+      return fail(FDM_EINVAL);
+    }
+    blk[(size_t)cy * c->nbx + cx] = b;
+    for (int i = B.mx; i <= B.px; i++) {
+      if (colb[i] != -1 && colb[i] != cx) {
+        set_err(c, FDM_EINVAL, "fdm_create: blocks overlap in x");
+        return fail(FDM_EINVAL);
+      }
+      colb[i] = cx;
+    }
+    for (int j = B.my; j <= B.py; j++) {
+      if (rowb[j] != -1 && rowb[j] != cy) {
+        set_err(c, FDM_EINVAL, "fdm_create: blocks overlap in y");
+        return fail(FDM_EINVAL);
+      }
+      rowb[j] = cy;
+    }
+  }
+  for (int v : blk)
+    if (v < 0) {
+      set_err(c, FDM_EINVAL, "fdm_create: blocks do not form a tensor-product layout");
+      return fail(FDM_EINVAL);
+    }
+  for (int v : colb)
+    if (v < 0) {
+      set_err(c, FDM_EINVAL, "fdm_create: grid columns not covered by blocks");
+      return fail(FDM_EINVAL);
+    }
+  for (int v : rowb)
+    if (v < 0) {
+      set_err(c, FDM_EINVAL, "fdm_create: grid rows not covered by blocks");
+      return fail(FDM_EINVAL);
+    }
+  // every block of one column spans the same x range (and rows the same y range)
+  for (int b = 0; b < c->nb; b++) {
+    const fdm_block& B = P->blocks[b];
+    for (int b2 = 0; b2 < c->nb; b2++) {
+      const fdm_block& B2 = P->blocks[b2];
+      if ((B.mx == B2.mx && B.px != B2.px) || (B.my == B2.my && B.py != B2.py)) {
+        set_err(c, FDM_EINVAL, "fdm_create: blocks do not form a tensor-product layout");
+        return fail(FDM_EINVAL);
+      }
+    }
+  }
+  int rc;
+  if ((rc = dupload(c, &c->blk, blk.data(), blk.size() * sizeof(int)))) return fail(rc);
+  if ((rc = dupload(c, &c->colb, colb.data(), colb.size() * sizeof(int)))) return fail(rc);
+  if ((rc = dupload(c, &c->rowb, rowb.data(), rowb.size() * sizeof(int)))) return fail(rc);
+
+  // ---- sides ----
+  c->hblocks.resize(c->nb);
+  int NS = 0;
+  for (int b = 0; b < c->nb; b++)
+    for (int s = 0; s < 4; s++) {
+      c->hblocks[b].side_off[s] = NS;
+      c->hblocks[b].side_n[s] = side_len(P->blocks[b], s);
+      NS += c->hblocks[b].side_n[s];
+    }
+  c->NS = NS;
+  std::vector<double> hM((size_t)5 * NS), hN((size_t)2 * NS), hF0((size_t)6 * NS),
+      hx(NS, 0.0), hy(NS, 0.0);
+  bool have_xy = false;
+  for (int b = 0; b < c->nb; b++) {
+    const fdm_block& B = P->blocks[b];
+    BlockDev& D = c->hblocks[b];
+    D.mx = B.mx;
+    D.px = B.px;
+    D.my = B.my;
+    D.py = B.py;
+    D.plastic = B.plastic;
+    D.mms = B.mms;
+    D.id = B.id;
+    D.pad = 0;
+    D.rho = B.rho;
+    D.G = B.G;
+    D.M = B.M;
+    D.gamma = B.gamma;
+    D.nu = B.nu;
+    // rates.f90:400-405: a(s) = DI(s)/rho, b(s) = DI(s)*G, c(s) = DI(s)*M
+    const double DIs[2] = {FDM_DI1, FDM_DI2};
+    for (int s = 0; s < 2; s++) {
+      D.cI[0][s] = DIs[s] / B.rho;
+      D.cI[1][s] = DIs[s] / B.rho;
+      D.cI[2][s] = DIs[s] * B.M;
+      D.cI[3][s] = DIs[s] * B.G;
+      D.cI[4][s] = DIs[s] * B.M;
+    }
+    D.mu = B.mu;
+    D.beta = B.beta;
+    D.b = B.b;
+    D.h = B.h;
+    D.eta = B.eta;
+    D.K = B.K;
+    for (int k = 0; k < 6; k++) D.s0[k] = B.s0[k];
+    if (B.plastic) c->any_plastic = true;
+    if (B.mms && (!P->x || !P->y)) {
+      set_err(c, FDM_EINVAL, "fdm_create: MMS source needs coordinates x, y");
+      return fail(FDM_EINVAL);
+    }
+    for (int s = 0; s < 4; s++) {
+      const fdm_side& S = B.side[s];
+      int n = D.side_n[s];
+      D.side_bc[s] = S.bc;
+      if (S.n != n || !S.normal || !S.M || !S.F0) {
+        set_err(c, FDM_EINVAL, "fdm_create: block " + std::to_string(b) + " side " +
+                                   std::to_string(s) + " arrays missing or wrong length");
+        return fail(FDM_EINVAL);
+      }
+      if (S.bc == FDM_BC_MMS_SIN_W && (!S.x || !S.y)) {
+        set_err(c, FDM_EINVAL, "fdm_create: MMS boundary data needs side coordinates");
+        return fail(FDM_EINVAL);
+      }
+      int o = D.side_off[s];
+      for (int k = 0; k < n; k++) {
+        for (int q = 0; q < 5; q++) hM[(size_t)q * NS + o + k] = S.M[(size_t)q * n + k];
+        for (int q = 0; q < 2; q++) hN[(size_t)q * NS + o + k] = S.normal[(size_t)q * n + k];
+        for (int q = 0; q < 6; q++) hF0[(size_t)q * NS + o + k] = S.F0[(size_t)q * n + k];
+        if (S.x && S.y) {
+          hx[o + k] = S.x[k];
+          hy[o + k] = S.y[k];
+          have_xy = true;
+        }
+      }
+    }
+  }
+  if ((rc = dupload(c, &c->sM, hM.data(), hM.size() * 8))) return fail(rc);
+  if ((rc = dupload(c, &c->snormal, hN.data(), hN.size() * 8))) return fail(rc);
+  if ((rc = dupload(c, &c->sF0, hF0.data(), hF0.size() * 8))) return fail(rc);
+  if (have_xy) {
+    if ((rc = dupload(c, &c->sx, hx.data(), hx.size() * 8))) return fail(rc);
+    if ((rc = dupload(c, &c->sy, hy.data(), hy.size() * 8))) return fail(rc);
+  }
+  {
+    std::vector<double> fh((size_t)6 * NS, 1e40);
+    if ((rc = dupload(c, &c->sFhat, fh.data(), fh.size() * 8))) return fail(rc);
+  }
+  if ((rc = dupload(c, &c->blocks, c->hblocks.data(), c->hblocks.size() * sizeof(BlockDev))))
+    return fail(rc);
+
+  // ---- interfaces ----
+  std::vector<int2> tasks;
+  for (int b = 0; b < c->nb; b++)
+    for (int s = 0; s < 4; s++)
+      if (P->blocks[b].side[s].bc != FDM_BC_NONE)
+        for (int k = 0; k < c->hblocks[b].side_n[s]; k++) tasks.push_back(make_int2(4 * b + s, k));
+  std::vector<std::vector<char>> covered(c->nb, std::vector<char>(4, 0));
+  int NI = 0;
+  c->hifaces.resize(c->ni);
+  for (int ii = 0; ii < c->ni; ii++) {
+    const fdm_iface& I = P->ifaces[ii];
+    IfaceDev& D = c->hifaces[ii];
+    int n = I.p - I.m + 1;
+    if (I.blockm < 0 || I.blockm >= c->nb || I.blockp < 0 || I.blockp >= c->nb || n < 1 ||
+        !I.nhat || (I.direction != 0 && I.direction != 1)) {
+      set_err(c, FDM_EINVAL, "fdm_create: interface " + std::to_string(ii) + " malformed");
+      return fail(FDM_EINVAL);
+    }
+    const fdm_block& Bm = P->blocks[I.blockm];
+    const fdm_block& Bp = P->blocks[I.blockp];
+    int sm = I.direction == 0 ? FDM_SIDE_R : FDM_SIDE_T;
+    int sp = I.direction == 0 ? FDM_SIDE_L : FDM_SIDE_B;
+    int lo_m = I.direction == 0 ? Bm.my : Bm.mx, hi_m = I.direction == 0 ? Bm.py : Bm.px;
+    int lo_p = I.direction == 0 ? Bp.my : Bp.mx, hi_p = I.direction == 0 ? Bp.py : Bp.px;
+    if (I.m < lo_m || I.p > hi_m || I.m < lo_p || I.p > hi_p) {
+      set_err(c, FDM_EINVAL, "fdm_create: interface " + std::to_string(ii) +
+                                 " range outside the adjacent block sides");
+      return fail(FDM_EINVAL);
+    }
+    if (I.coupling == FDM_COUPLING_FRICTION) {
+      c->any_friction = true;
+      if ((I.law >= FDM_LAW_SL && !I.rs) || (I.law == FDM_LAW_SW && !I.sw) || I.law < 0 ||
+          I.law > FDM_LAW_RAL) {
+        set_err(c, FDM_EINVAL, "fdm_create: interface " + std::to_string(ii) +
+                                   " friction parameters missing or unknown law");
+        return fail(FDM_EINVAL);
+      }
+    }
+    covered[I.blockm][sm] = 1;
+    covered[I.blockp][sp] = 1;
+    D.direction = I.direction;
+    D.bm = I.blockm;
+    D.bp = I.blockp;
+    D.m = I.m;
+    D.p = I.p;
+    D.coupling = I.coupling;
+    D.law = I.law;
+    D.reg = I.reg;
+    D.rup_field = I.rup_field;
+    D.node_off = NI;
+    D.n = n;
+    D.pad = 0;
+    D.rup_threshold = I.rup_threshold;
+    D.skempton = I.skempton;
+    D.flock = I.flock;
+    D.ld_T = I.ld_T;
+    c->ld_T.push_back(I.ld_T);
+    for (int k = 0; k < n; k++) tasks.push_back(make_int2(-(ii + 1), k));
+    NI += n;
+  }
+  // every side without a boundary condition must be an interface side
+  for (int b = 0; b < c->nb; b++)
+    for (int s = 0; s < 4; s++)
+      if (P->blocks[b].side[s].bc == FDM_BC_NONE && !covered[b][s]) {
+        set_err(c, FDM_EINVAL, "fdm_create: block " + std::to_string(b) + " side " +
+                                   std::to_string(s) + " has bc 'none' and no interface");
+        return fail(FDM_EINVAL);
+      }
+  // one load-ramp time constant is applied to all interfaces per stage
+  for (double T : c->ld_T)
+    if (T != c->ld_T[0]) {
+      set_err(c, FDM_EINVAL, "fdm_create: interfaces with different ld%T are not supported");
+      return fail(FDM_EINVAL);
+    }
+  c->NI = NI;
+  {
+    size_t NIs = NI > 0 ? NI : 1;
+    std::vector<double> nh(2 * NIs, 0.0), rs(7 * NIs, 1.0), sw(4 * NIs, 1.0), S0b(NIs, 0.0),
+        N0b(NIs, 0.0), ldS(NIs, 0.0), ldN(NIs, 0.0), st((size_t)FDM_IF_NFIELDS * NIs, 0.0);
+    std::vector<int> outv(NIs, 0);
+    for (int ii = 0; ii < c->ni; ii++) {
+      const fdm_iface& I = P->ifaces[ii];
+      int n = c->hifaces[ii].n, o = c->hifaces[ii].node_off;
+      for (int k = 0; k < n; k++) {
+        nh[o + k] = I.nhat[k];
+        nh[NIs + o + k] = I.nhat[(size_t)n + k];
+        if (I.rs)
+          for (int q = 0; q < 7; q++) rs[(size_t)q * NIs + o + k] = I.rs[(size_t)q * n + k];
+        if (I.sw)
+          for (int q = 0; q < 4; q++) sw[(size_t)q * NIs + o + k] = I.sw[(size_t)q * n + k];
+        if (I.S0base) S0b[o + k] = I.S0base[k];
+        if (I.N0base) N0b[o + k] = I.N0base[k];
+        if (I.ldS) ldS[o + k] = I.ldS[k];
+        if (I.ldN) ldN[o + k] = I.ldN[k];
+        if (I.outside) outv[o + k] = I.outside[k];
+        // initial values (friction.f90:217-249)
+        auto S = [&](int f) -> double& { return st[(size_t)f * NIs + o + k]; };
+        S(FDM_IF_D) = 0.0;
+        S(FDM_IF_PSI) = I.Psi0 ? I.Psi0[k] : 0.0;
+        S(FDM_IF_DPSI) = 1e40;
+        S(FDM_IF_TRUP) = 1e10;
+        S(FDM_IF_V) = 1e40;
+        S(FDM_IF_O) = 1e40;
+        S(FDM_IF_S) = 1e40;
+        S(FDM_IF_N) = 1e40;
+        S(FDM_IF_S0) = 1e40;
+        S(FDM_IF_N0) = 1e40;
+        S(FDM_IF_DS) = 0.0;
+        S(FDM_IF_DN) = 0.0;
+        S(FDM_IF_DDS) = 0.0;
+        S(FDM_IF_DDN) = 0.0;
+        S(FDM_IF_W) = 0.0;
+        S(FDM_IF_DW) = 1e40;
+        S(FDM_IF_NREG) = 1e40;
+        S(FDM_IF_DNREG) = 1e40;
+      }
+    }
+    // component strides use NIs (= NI, or 1 when there is no interface)
+    c->NI = (int)NIs;
+    for (auto& D : c->hifaces) (void)D;
+    if ((rc = dupload(c, &c->ifaces, c->hifaces.data(),
+                      std::max<size_t>(1, c->hifaces.size()) * sizeof(IfaceDev))))
+      return fail(rc);
+    if ((rc = dupload(c, &c->inhat, nh.data(), nh.size() * 8))) return fail(rc);
+    if ((rc = dupload(c, &c->irs, rs.data(), rs.size() * 8))) return fail(rc);
+    if ((rc = dupload(c, &c->isw, sw.data(), sw.size() * 8))) return fail(rc);
+    if ((rc = dupload(c, &c->iS0b, S0b.data(), S0b.size() * 8))) return fail(rc);
+    if ((rc = dupload(c, &c->iN0b, N0b.data(), N0b.size() * 8))) return fail(rc);
+    if ((rc = dupload(c, &c->ildS, ldS.data(), ldS.size() * 8))) return fail(rc);
+    if ((rc = dupload(c, &c->ildN, ldN.data(), ldN.size() * 8))) return fail(rc);
+    if ((rc = dupload(c, &c->iout, outv.data(), outv.size() * sizeof(int)))) return fail(rc);
+    if ((rc = dupload(c, &c->ist, st.data(), st.size() * 8))) return fail(rc);
+    if ((rc = dupload(c, &c->ist1, st.data(), st.size() * 8))) return fail(rc);
+  }
+  c->ntasks = (int)tasks.size();
+  if ((rc = dupload(c, &c->tasks, tasks.data(), std::max<size_t>(1, tasks.size()) * sizeof(int2))))
+    return fail(rc);
+
+  // ---- source ----
+  c->src = P->source;
+  {
+    std::vector<SourcePlaceDev> sp(std::max(1, c->src.nplace));
+    for (int k = 0; k < c->src.nplace; k++) {
+      const fdm_source_place& S = c->src.place[k];
+      if (S.block < 0 || S.block >= c->nb) {
+        set_err(c, FDM_EINVAL, "fdm_create: source placement block out of range");
+        return fail(FDM_EINVAL);
+      }
+      sp[k].block = S.block;
+      sp[k].i0 = S.i0;
+      sp[k].j0 = S.j0;
+      sp[k].pad = 0;
+      for (int q = 0; q < 5; q++) {
+        sp[k].dx[q] = S.dx[q];
+        sp[k].dy[q] = S.dy[q];
+      }
+    }
+    if ((rc = dupload(c, &c->dsrc, sp.data(), sp.size() * sizeof(SourcePlaceDev)))) return fail(rc);
+  }
+
+  // ---- bulk arrays ----
+  size_t pbytes = (size_t)c->ps * c->esz;
+  for (int q = 0; q < 2; q++)
+    if ((rc = dalloc(c, &c->F[q], 6 * pbytes))) return fail(rc);
+  if ((rc = dalloc(c, &c->DF, 6 * pbytes))) return fail(rc);
+  HIPCHK(c, hipMemset(c->F[0], 0, 6 * pbytes));
+  HIPCHK(c, hipMemset(c->F[1], 0, 6 * pbytes));
+  HIPCHK(c, hipMemset(c->DF, 0, 6 * pbytes));
+  const double* mets[5] = {P->xq, P->xr, P->yq, P->yr, P->Ji};
+  for (int q = 0; q < 5; q++) {
+    if (c->precision == FDM_FP64) {
+      std::vector<double> h;
+      pack_plane(h, mets[q], c->nx, c->ny, c->ld);
+      if ((rc = dupload(c, &c->met[q], h.data(), h.size() * 8))) return fail(rc);
+    } else {
+      std::vector<float> h;
+      pack_plane(h, mets[q], c->nx, c->ny, c->ld);
+      if ((rc = dupload(c, &c->met[q], h.data(), h.size() * 4))) return fail(rc);
+    }
+  }
+  bool any_mms = false;
+  for (int b = 0; b < c->nb; b++) any_mms |= (P->blocks[b].mms != 0);
+  if (any_mms) {
+    size_t np = (size_t)c->nx * c->ny;
+    if ((rc = dupload(c, &c->X, P->x, np * 8))) return fail(rc);
+    if ((rc = dupload(c, &c->Y, P->y, np * 8))) return fail(rc);
+  }
+  if (c->any_plastic) {
+    size_t np = (size_t)c->nx * c->ny;
+    std::vector<double> z(np, 0.0);
+    if ((rc = dupload(c, &c->gammap, z.data(), np * 8))) return fail(rc);
+    if ((rc = dupload(c, &c->lambda, z.data(), np * 8))) return fail(rc);
+  }
+  {
+    int z = 0;
+    if ((rc = dupload(c, &c->fail, &z, sizeof(int)))) return fail(rc);
+  }
+  *out = c;
+  return FDM_OK;
+}
+
+int fdm_set_fields(fdm_ctx* c, const double* F, const double* DF) {
+  if (!c || !F) return FDM_EINVAL;
+  size_t pe = (size_t)c->ps;
+  if (c->precision == FDM_FP64) {
+    std::vector<double> h(6 * pe, 0.0);
+    for (int l = 0; l < 6; l++)
+      for (int j = 0; j < c->ny; j++)
+        memcpy(&h[l * pe + (size_t)j * c->ld], F + ((size_t)l * c->ny + j) * c->nx, c->nx * 8);
+    HIPCHK(c, hipMemcpy(c->F[c->cur], h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    for (int l = 0; l < 6; l++)
+      for (int j = 0; j < c->ny; j++)
+        for (int i = 0; i < c->nx; i++)
+          h[l * pe + (size_t)j * c->ld + i] = DF ? DF[((size_t)l * c->ny + j) * c->nx + i] : 1e40;
+    HIPCHK(c, hipMemcpy(c->DF, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> h(6 * pe, 0.f);
+    for (int l = 0; l < 6; l++)
+      for (int j = 0; j < c->ny; j++)
+        for (int i = 0; i < c->nx; i++)
+          h[l * pe + (size_t)j * c->ld + i] = (float)F[((size_t)l * c->ny + j) * c->nx + i];
+    HIPCHK(c, hipMemcpy(c->F[c->cur], h.data(), h.size() * 4, hipMemcpyHostToDevice));
+    for (int l = 0; l < 6; l++)
+      for (int j = 0; j < c->ny; j++)
+        for (int i = 0; i < c->nx; i++) {
+          double v = DF ? DF[((size_t)l * c->ny + j) * c->nx + i] : 3.0e38;
+          h[l * pe + (size_t)j * c->ld + i] = (float)std::fmax(std::fmin(v, 3.0e38), -3.0e38);
+        }
+    HIPCHK(c, hipMemcpy(c->DF, h.data(), h.size() * 4, hipMemcpyHostToDevice));
+  }
+  return FDM_OK;
+}
+
+int fdm_get_fields(fdm_ctx* c, double* F, double* DF) {
+  if (!c) return FDM_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  size_t pe = (size_t)c->ps;
+  for (int which = 0; which < 2; which++) {
+    double* dst = which == 0 ? F : DF;
+    if (!dst) continue;
+    const void* src = which == 0 ? c->F[c->cur] : c->DF;
+    if (c->precision == FDM_FP64) {
+      std::vector<double> h(6 * pe);
+      HIPCHK(c, hipMemcpy(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost));
+      for (int l = 0; l < 6; l++)
+        for (int j = 0; j < c->ny; j++)
+          memcpy(dst + ((size_t)l * c->ny + j) * c->nx, &h[l * pe + (size_t)j * c->ld], c->nx * 8);
+    } else {
+      std::vector<float> h(6 * pe);
+      HIPCHK(c, hipMemcpy(h.data(), src, h.size() * 4, hipMemcpyDeviceToHost));
+      for (int l = 0; l < 6; l++)
+        for (int j = 0; j < c->ny; j++)
+          for (int i = 0; i < c->nx; i++)
+            dst[((size_t)l * c->ny + j) * c->nx + i] = (double)h[l * pe + (size_t)j * c->ld + i];
+    }
+  }
+  return check_fail(c);
+}
+
+int fdm_init_edges(fdm_ctx* c, double t) {
+  if (!c) return FDM_EINVAL;
+  // prepare_edges + enforce_edge_conditions(D,0d0) (domain.f90:346-356)
+  double Bload = c->ld_T.empty() ? 1.0 : load_ramp(t, c->ld_T[0]);
+  int rc = c->precision == FDM_FP64 ? launch_edges<double>(c, t, 0.0, 1.0, 0, 0, 0, Bload)
+                                    : launch_edges<float>(c, t, 0.0, 1.0, 0, 0, 0, Bload);
+  if (rc) return rc;
+  return check_fail(c);
+}
+
+int fdm_step(fdm_ctx* c, double t, double dt, int64_t n_first, int32_t nsteps) {
+  if (!c || nsteps < 0) return FDM_EINVAL;
+  return c->precision == FDM_FP64 ? step_impl<double>(c, t, dt, n_first, nsteps)
+                                  : step_impl<float>(c, t, dt, n_first, nsteps);
+}
+
+int fdm_stage1(fdm_ctx* c, double t, double dt, int64_t n) {
+  if (!c) return FDM_EINVAL;
+  double t0 = t + (double)(n - 1) * dt;
+  return c->precision == FDM_FP64 ? run_stage<double>(c, 1, t0, dt, 1, 1)
+                                  : run_stage<float>(c, 1, t0, dt, 1, 1);
+}
+
+static int iface_copy(fdm_ctx* c, int32_t ii, int32_t field, double* out, const double* in,
+                      double* base) {
+  if (!c || ii < 0 || ii >= c->ni || field < 0 || field >= FDM_IF_NFIELDS) return FDM_EINVAL;
+  const IfaceDev& I = c->hifaces[ii];
+  double* p = base + (size_t)field * c->NI + I.node_off;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (out) HIPCHK(c, hipMemcpy(out, p, I.n * 8, hipMemcpyDeviceToHost));
+  if (in) HIPCHK(c, hipMemcpy(p, in, I.n * 8, hipMemcpyHostToDevice));
+  return FDM_OK;
+}
+
+int fdm_get_iface(fdm_ctx* c, int32_t ii, int32_t field, double* out) {
+  if (!out) return FDM_EINVAL;
+  int rc = iface_copy(c, ii, field, out, nullptr, c ? c->ist : nullptr);
+  return rc ? rc : check_fail(c);
+}
+
+int fdm_set_iface(fdm_ctx* c, int32_t ii, int32_t field, const double* in) {
+  if (!in) return FDM_EINVAL;
+  return iface_copy(c, ii, field, nullptr, in, c ? c->ist : nullptr);
+}
+
+int fdm_get_iface_stage1(fdm_ctx* c, int32_t ii, int32_t field, double* out) {
+  if (!out) return FDM_EINVAL;
+  return iface_copy(c, ii, field, out, nullptr, c ? c->ist1 : nullptr);
+}
+
+int fdm_get_plastic(fdm_ctx* c, double* gammap, double* lambda) {
+  if (!c) return FDM_EINVAL;
+  size_t np = (size_t)c->nx * c->ny;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (!c->any_plastic) {
+    if (gammap) std::fill(gammap, gammap + np, 0.0);
+    if (lambda) std::fill(lambda, lambda + np, 0.0);
+    return FDM_OK;
+  }
+  if (gammap) HIPCHK(c, hipMemcpy(gammap, c->gammap, np * 8, hipMemcpyDeviceToHost));
+  if (lambda) HIPCHK(c, hipMemcpy(lambda, c->lambda, np * 8, hipMemcpyDeviceToHost));
+  return FDM_OK;
+}
+
+int fdm_max_slip_velocity(fdm_ctx* c, double* vmax) {
+  if (!c || !vmax) return FDM_EINVAL;
+  double v = 0.0;
+  for (int ii = 0; ii < c->ni; ii++) {
+    if (c->hifaces[ii].coupling != FDM_COUPLING_FRICTION) continue;
+    std::vector<double> h(c->hifaces[ii].n);
+    int rc = iface_copy(c, ii, FDM_IF_V, h.data(), nullptr, c->ist);
+    if (rc) return rc;
+    for (double x : h) v = std::max(v, x);
+  }
+  *vmax = v;
+  return FDM_OK;
+}
+
+int fdm_sync(fdm_ctx* c) {
+  if (!c) return FDM_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return check_fail(c);
+}
+
+int fdm_profile_enable(fdm_ctx* c, int32_t enable) {
+  if (!c) return FDM_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->prof = enable != 0;
+  c->pool_used = 0;
+  c->bulk_ms = c->edge_ms = 0.0;
+  c->bulk_n = c->edge_n = 0;
+  return FDM_OK;
+}
+
+int fdm_profile_read(fdm_ctx* c, fdm_profile_t* out) {
+  if (!c || !out) return FDM_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (size_t k = 0; k < c->pool_used; k++) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->pool[k].a, c->pool[k].b));
+    if (c->pair_kind[k] == 0) c->bulk_ms += ms; else c->edge_ms += ms;
+  }
+  c->pool_used = 0;
+  out->bulk_ms = c->bulk_ms;
+  out->bulk_launches = c->bulk_n;
+  out->edge_ms = c->edge_ms;
+  out->edge_launches = c->edge_n;
+  out->bulk_points = (int64_t)c->nx * c->ny;
+  return FDM_OK;
+}
+
+}  // extern "C"

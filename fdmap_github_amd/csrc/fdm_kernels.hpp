@@ -1,0 +1,266 @@
+// fdm_kernels.hpp -- device kernels of the time-stepping path.
+//
+//   edge_kernel  : one lane per boundary node or interface node.  Boundary
+//                  hats (set_bc_mode2), locked interface hats, and for
+//                  frictional interfaces the fault-rate scaling, the
+//                  rate-and-state solve, hats, state rates and the fault RK
+//                  update, all in FP64.
+//   bulk_kernel  : one lane per grid point.  DF <- A*DF, SAT penalties,
+//                  SBP4 stencils with curvilinear metrics, szz overwrite,
+//                  sources, F <- F + B*dt*DF and (last stage) the
+//                  Drucker-Prager return map.  Templated on the storage
+//                  type T of fields/rates/metrics (double, or float for the
+//                  FP32-bulk mode).
+#pragma once
+
+#include "fdm_physics.hpp"
+
+namespace fdm {
+
+struct BlockDev {
+  int mx, px, my, py;
+  int plastic, mms, id, pad;
+  double rho, G, M, gamma, nu;
+  double cI[5][2];  // interior coefficients DI(s)/rho, DI(s)*M, DI(s)*G (rates.f90:400-405)
+  double mu, beta, b, h, eta, K;
+  double s0[6];
+  int side_off[4], side_n[4], side_bc[4];
+};
+
+struct IfaceDev {
+  int direction, bm, bp, m, p, coupling, law, reg, rup_field, node_off, n, pad;
+  double rup_threshold, skempton, flock, ld_T;
+};
+
+struct SourcePlaceDev {
+  int block, i0, j0, pad;
+  double dx[5], dy[5];
+};
+
+// concatenated per-node arrays of all block sides; component-major, stride NS
+struct SidesDev {
+  const double* M;       // [5][NS]
+  const double* normal;  // [2][NS]
+  const double* F0;      // [6][NS]
+  const double* x;       // [NS] (may be null)
+  const double* y;
+  double* Fhat;          // [6][NS]
+  int NS;
+};
+
+// concatenated per-node arrays of all interfaces; component-major, stride NI
+struct IfacesDev {
+  const IfaceDev* tab;
+  const double* nhat;    // [2][NI]
+  const double* rs;      // [7][NI]
+  const double* sw;      // [4][NI]
+  const double* S0b;     // [NI]
+  const double* N0b;
+  const double* ldS;
+  const double* ldN;
+  const int* outside;
+  double* st;            // [18][NI] fault state (FDM_IF_* order)
+  double* st1;           // [18][NI] stage-1 snapshot
+  int NI;
+};
+
+template <typename T>
+struct EdgeArgs {
+  const T* F;
+  int ld;
+  long long ps;
+  const int2* tasks;  // .x: 4*block+side for a boundary node, -(1+iface) for an interface node
+  int ntasks;
+  const BlockDev* blocks;
+  SidesDev sd;
+  IfacesDev fi;
+  double t, Bdt, A, Bload;
+  int do_scale, do_update, snapshot;
+  int* fail;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(64) edge_kernel(EdgeArgs<T> a) {
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= a.ntasks) return;
+  int2 task = a.tasks[tid];
+  const int NS = a.sd.NS;
+  if (task.x >= 0) {
+    // boundary node (boundaries.f90:106-129)
+    int b = task.x >> 2, s = task.x & 3, k = task.y;
+    const BlockDev& B = a.blocks[b];
+    int i, j;
+    if (s == FDM_SIDE_L) { i = B.mx; j = B.my + k; }
+    else if (s == FDM_SIDE_R) { i = B.px; j = B.my + k; }
+    else if (s == FDM_SIDE_B) { i = B.mx + k; j = B.my; }
+    else { i = B.mx + k; j = B.py; }
+    int so = B.side_off[s] + k;
+    long long cc = (long long)j * a.ld + i;
+    double F[6], F0[6], Fh[6];
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      F[l] = (double)a.F[l * a.ps + cc];
+      F0[l] = a.sd.F0[l * NS + so];
+    }
+    double x = a.sd.x ? a.sd.x[so] : 0.0, y = a.sd.y ? a.sd.y[so] : 0.0;
+    set_bc_mode2(Fh, F, F0, a.sd.normal[so], a.sd.normal[NS + so], B.side_bc[s], a.sd.M[so],
+                 a.sd.M[NS + so], a.sd.M[2 * NS + so], x, y, a.t, B.id);
+#pragma unroll
+    for (int l = 0; l < 6; l++) a.sd.Fhat[l * NS + so] = Fh[l];
+    return;
+  }
+  // interface node (interfaces.f90:520-1012)
+  const IfaceDev& I = a.fi.tab[-task.x - 1];
+  int k = task.y;
+  int g = I.m + k;
+  int q = I.node_off + k;
+  const int NI = a.fi.NI;
+  const BlockDev& Bm = a.blocks[I.bm];
+  const BlockDev& Bp = a.blocks[I.bp];
+  int im, jm, ip, jp, km, kp, sm, sp;
+  if (I.direction == 0) {
+    sm = FDM_SIDE_R; sp = FDM_SIDE_L;
+    im = Bm.px; jm = g; ip = Bp.mx; jp = g;
+    km = g - Bm.my; kp = g - Bp.my;
+  } else {
+    sm = FDM_SIDE_T; sp = FDM_SIDE_B;
+    im = g; jm = Bm.py; ip = g; jp = Bp.my;
+    km = g - Bm.mx; kp = g - Bp.mx;
+  }
+  int som = Bm.side_off[sm] + km, sop = Bp.side_off[sp] + kp;
+  long long ccm = (long long)jm * a.ld + im, ccp = (long long)jp * a.ld + ip;
+  double Fm[6], Fp[6], F0m[6], F0p[6], Fhm[6], Fhp[6];
+#pragma unroll
+  for (int l = 0; l < 6; l++) {
+    Fm[l] = (double)a.F[l * a.ps + ccm];
+    Fp[l] = (double)a.F[l * a.ps + ccp];
+    F0m[l] = a.sd.F0[l * NS + som];
+    F0p[l] = a.sd.F0[l * NS + sop];
+  }
+  double n1 = a.fi.nhat[q], n2 = a.fi.nhat[NI + q];
+  double Zsm = a.sd.M[som], Zsp = a.sd.M[sop];
+  double Zpm = a.sd.M[NS + som], Zpp = a.sd.M[NS + sop];
+  double gm = a.sd.M[2 * NS + som], gp = a.sd.M[2 * NS + sop];
+  if (I.coupling == FDM_COUPLING_LOCKED) {
+    locked_interface_mode2(Fhm, Fhp, Fm, Fp, n1, n2, Zsm, Zsp, Zpm, Zpp, gm, gp);
+  } else {
+    double* st = a.fi.st;
+#define ST(f) st[(long long)(f) * NI + q]
+    if (a.do_scale) {
+      // scale_rates_friction (friction.f90:555-573)
+      ST(FDM_IF_DDS) = a.A * ST(FDM_IF_DDS);
+      ST(FDM_IF_DDN) = a.A * ST(FDM_IF_DDN);
+      ST(FDM_IF_DPSI) = a.A * ST(FDM_IF_DPSI);
+      ST(FDM_IF_DW) = a.A * ST(FDM_IF_DW);
+      ST(FDM_IF_DNREG) = a.A * ST(FDM_IF_DNREG);
+    }
+    // frictional_interface_mode2 (interfaces.f90:900-1012).  The reference
+    // passes the minus-side initial fields to its dummy "F0p" and the
+    // plus-side ones to "F0m" (interfaces.f90:795 vs :900); kept as is.
+    const double* dF0p = F0m;
+    const double* dF0m = F0p;
+    NT p0 = xy2nt(dF0p, n1, n2);
+    double dp[6], dm[6];
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      dp[l] = Fp[l] - dF0p[l];
+      dm[l] = Fm[l] - dF0m[l];
+    }
+    NT p = xy2nt(dp, n1, n2);
+    NT m = xy2nt(dm, n1, n2);
+    double Zsip = 1.0 / Zsp, Zsim = 1.0 / Zsm, Zpip = 1.0 / Zpp, Zpim = 1.0 / Zpm;
+    double etas = 1.0 / (Zsip + Zsim);
+    double phis = etas * (p.snt * Zsip + m.snt * Zsim + p.vt - m.vt);
+    double etap = 1.0 / (Zpip + Zpim);
+    double phip = etap * (p.snn * Zpip + m.snn * Zpim + p.vn - m.vn);
+    double S0 = p0.snt;
+    double N0 = -p0.snn - 0.0;
+    // load_stress (friction.f90:1335-1374) with the ramp B(t) precomputed
+    double S0l = a.fi.S0b[q] + a.fi.ldS[q] * a.Bload;
+    double N0l = a.fi.N0b[q] + a.fi.ldN[q] * a.Bload;
+    double fS0 = S0l + S0;
+    double fN0 = N0l + N0;
+    ST(FDM_IF_S0) = fS0;
+    ST(FDM_IF_N0) = fN0;
+    double rs[7], sw[4];
+    if (I.law >= FDM_LAW_SL) {
+#pragma unroll
+      for (int c = 0; c < 7; c++) rs[c] = a.fi.rs[(long long)c * NI + q];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 7; c++) rs[c] = 1.0;
+    }
+    if (I.law == FDM_LAW_SW) {
+#pragma unroll
+      for (int c = 0; c < 4; c++) sw[c] = a.fi.sw[(long long)c * NI + q];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 4; c++) sw[c] = 1.0;
+    }
+    double V = ST(FDM_IF_V), S = ST(FDM_IF_S), O = ST(FDM_IF_O), N = ST(FDM_IF_N),
+           Nreg = ST(FDM_IF_NREG);
+    double D = ST(FDM_IF_D), Psi = ST(FDM_IF_PSI);
+    bool fail = solve_friction_core(I.law, rs, sw, I.reg, a.fi.outside[q], I.flock, fS0, fN0,
+                                    phip, phis, etas, D, Psi, I.skempton, a.Bdt, V, S, O, N, Nreg);
+    if (fail) *a.fail = 1;
+    ST(FDM_IF_V) = V;
+    ST(FDM_IF_S) = S;
+    ST(FDM_IF_O) = O;
+    ST(FDM_IF_N) = N;
+    ST(FDM_IF_NREG) = Nreg;
+    double skm = 1.0 - I.skempton;
+    double snnp = -(N - fN0) / skm;
+    double snnm = -(N - fN0) / skm;
+    double vnp = p.vn + Zpip * (p.snn - snnp);
+    double vnm = m.vn - Zpim * (m.snn - snnm);
+    double sttp = p.stt - gp * (p.snn - snnp);
+    double sttm = m.stt - gm * (m.snn - snnm);
+    double szzp = p.szz - gp * (p.snn - snnp);
+    double szzm = m.szz - gm * (m.snn - snnm);
+    double sntp = S - fS0;
+    double sntm = S - fS0;
+    double vtp = p.vt + Zsip * (p.snt - sntp);
+    double vtm = m.vt - Zsim * (m.snt - sntm);
+    nt2xy(Fhp, n1, n2, vtp, vnp, sttp, sntp, snnp, szzp);
+    nt2xy(Fhm, n1, n2, vtm, vnm, sttm, sntm, snnm, szzm);
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      Fhp[l] = Fhp[l] + dF0p[l];
+      Fhm[l] = Fhm[l] + dF0m[l];
+    }
+    // set_rates_friction (friction.f90:1101-1124)
+    ST(FDM_IF_DDS) = ST(FDM_IF_DDS) + V;
+    ST(FDM_IF_DDN) = ST(FDM_IF_DDN) + O;
+    ST(FDM_IF_DPSI) = ST(FDM_IF_DPSI) + state_rate(I.law, rs, V, Psi);
+    if (I.reg) {
+      double absV = fabs(V);
+      double Nr = -(absV / rs[4]) * (Nreg - fmax(N, 0.0));
+      ST(FDM_IF_DNREG) = ST(FDM_IF_DNREG) + Nr;
+    }
+    ST(FDM_IF_DW) = ST(FDM_IF_DW) + S * V;
+    bool ruptured = I.rup_field == 1 ? (D > I.rup_threshold) : (fabs(V) > I.rup_threshold);
+    if (ruptured) ST(FDM_IF_TRUP) = fmin(ST(FDM_IF_TRUP), a.t);
+    if (a.snapshot) {
+      for (int f = 0; f < FDM_IF_NFIELDS; f++)
+        a.fi.st1[(long long)f * NI + q] = st[(long long)f * NI + q];
+    }
+    if (a.do_update) {
+      // update_fields_friction (friction.f90:576-590)
+      double Bdt = a.Bdt;
+      ST(FDM_IF_DS) = ST(FDM_IF_DS) + Bdt * ST(FDM_IF_DDS);
+      ST(FDM_IF_DN) = ST(FDM_IF_DN) + Bdt * ST(FDM_IF_DDN);
+      ST(FDM_IF_D) = ST(FDM_IF_D) + Bdt * fabs(ST(FDM_IF_DDS));
+      ST(FDM_IF_PSI) = ST(FDM_IF_PSI) + Bdt * ST(FDM_IF_DPSI);
+      ST(FDM_IF_W) = ST(FDM_IF_W) + Bdt * ST(FDM_IF_DW);
+      ST(FDM_IF_NREG) = ST(FDM_IF_NREG) + Bdt * ST(FDM_IF_DNREG);
+    }
+#undef ST
+  }
+#pragma unroll
+  for (int l = 0; l < 6; l++) {
+    a.sd.Fhat[l * NS + som] = Fhm[l];
+    a.sd.Fhat[l * NS + sop] = Fhp[l];
+  }
+}
+
+}  // namespace fdm

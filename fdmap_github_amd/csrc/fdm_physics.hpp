@@ -1,0 +1,345 @@
+// fdm_physics.hpp -- pointwise physics shared by the device kernels:
+// rotations, characteristic boundary hats, interface hats and the
+// rate-and-state friction solve.  All in FP64; every expression keeps the
+// reference's left-to-right evaluation order (the library is compiled with
+// -ffp-contract=off so that nothing is fused).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/fdmap.h"
+
+namespace fdm {
+
+// SBP4 interior coefficients DI(1), DI(2) (fd_coeff.f90:101-102)
+#define FDM_DI1 0.666666666666667
+#define FDM_DI2 (-0.083333333333333)
+
+// DL(s,r) (fd_coeff.f90:108-115); DR(-s,-r) = -DL(s,r) (fd_coeff.f90:242-246)
+__device__ constexpr double kDL[4][6] = {
+    {-1.411764705882353, 1.735294117647059, -0.235294117647059, -0.088235294117647,
+     0.000000000000000, 0.000000000000000},
+    {-0.500000000000000, 0.000000000000000, 0.500000000000000, 0.000000000000000,
+     0.000000000000000, 0.000000000000000},
+    {0.093023255813953, -0.686046511627907, 0.000000000000000, 0.686046511627907,
+     -0.093023255813953, 0.000000000000000},
+    {0.030612244897959, 0.000000000000000, -0.602040816326531, 0.000000000000000,
+     0.653061224489796, -0.081632653061224}};
+
+struct NT {
+  double vt, vn, stt, snt, snn, szz;
+};
+
+// rotate_fields_xy2nt_mode2 (fields.f90:1312-1325, geometry.f90:285-322)
+__device__ __forceinline__ NT xy2nt(const double* F, double n1, double n2) {
+  NT r;
+  r.vt = n2 * F[0] - n1 * F[1];
+  r.vn = n1 * F[0] + n2 * F[1];
+  r.stt = -2.0 * n1 * n2 * F[3] + n2 * n2 * F[2] + n1 * n1 * F[4];
+  r.snt = n1 * n2 * (F[2] - F[4]) + (n2 * n2 - n1 * n1) * F[3];
+  r.snn = 2.0 * n1 * n2 * F[3] + n1 * n1 * F[2] + n2 * n2 * F[4];
+  r.szz = F[5];
+  return r;
+}
+
+// rotate_fields_nt2xy_mode2 (fields.f90:1343-1356, geometry.f90:298-336)
+__device__ __forceinline__ void nt2xy(double* F, double n1, double n2, double vt, double vn,
+                                      double stt, double snt, double snn, double szz) {
+  F[0] = n2 * vt + n1 * vn;
+  F[1] = -n1 * vt + n2 * vn;
+  F[2] = 2.0 * n1 * n2 * snt + n2 * n2 * stt + n1 * n1 * snn;
+  F[3] = -n1 * n2 * (stt - snn) + (n2 * n2 - n1 * n1) * snt;
+  F[4] = -2.0 * n1 * n2 * snt + n1 * n1 * stt + n2 * n2 * snn;
+  F[5] = szz;
+}
+
+__device__ __forceinline__ double fsign(double a, double b) { return copysign(fabs(a), b); }
+
+// arcsinh (friction.f90:1293-1304)
+__device__ __forceinline__ double arcsinh_ref(double x) {
+  double y = fabs(x);
+  double f = log(y + sqrt(y * y + 1.0));
+  return fsign(f, x);
+}
+
+__device__ __forceinline__ double pow8(double x) {
+  double x2 = x * x;
+  double x4 = x2 * x2;
+  return x4 * x4;
+}
+
+// mms_sin (mms.f90:52-234).  field: 0..5 fields, 6..11 sources s_*.
+__device__ double mms_sin(double x, double y, double t, int side, int field);
+
+// set_bc_mode2 (boundaries.f90:216-377)
+__device__ __forceinline__ void set_bc_mode2(double* Fhat, const double* F, const double* F0,
+                                             double n1, double n2, int bc, double Zs, double Zp,
+                                             double gamma, double x, double y, double t,
+                                             int iblock) {
+  double Zsi = (Zs == 0.0) ? 0.0 : 1.0 / Zs;
+  double Zpi = 1.0 / Zp;
+  bool subtract = !(bc == FDM_BC_MMS_SIN_W || bc == FDM_BC_RIGID_0 || bc == FDM_BC_ABSORBING_0 ||
+                    bc == FDM_BC_FREE_0);
+  double Fd[6];
+#pragma unroll
+  for (int l = 0; l < 6; l++) Fd[l] = subtract ? F[l] - F0[l] : F[l];
+  NT fd = xy2nt(Fd, n1, n2);
+  double vt = 0.0, vn = 0.0, snt = 0.0, snn = 0.0;
+  switch (bc) {
+    case FDM_BC_MMS_SIN_W: {
+      double FEX[6];
+      for (int l = 0; l < 5; l++) FEX[l] = mms_sin(x, y, t, iblock, l);
+      FEX[5] = 0.0;
+      NT ex = xy2nt(FEX, n1, n2);
+      vt = 0.5 * (ex.vt + fd.vt + Zsi * (ex.snt - fd.snt));
+      snt = 0.5 * (ex.snt + fd.snt + Zs * (ex.vt - fd.vt));
+      vn = 0.5 * (ex.vn + fd.vn + Zpi * (ex.snn - fd.snn));
+      snn = 0.5 * (ex.snn + fd.snn + Zp * (ex.vn - fd.vn));
+      break;
+    }
+    case FDM_BC_ABSORBING:
+    case FDM_BC_ABSORBING_0:
+      vt = 0.5 * (fd.vt - Zsi * fd.snt);
+      snt = 0.5 * (fd.snt - Zs * fd.vt);
+      vn = 0.5 * (fd.vn - Zpi * fd.snn);
+      snn = 0.5 * (fd.snn - Zp * fd.vn);
+      break;
+    case FDM_BC_FREE:
+    case FDM_BC_FREE_0:
+      vt = fd.vt - Zsi * fd.snt;
+      snt = 0.0;
+      vn = fd.vn - Zpi * fd.snn;
+      snn = 0.0;
+      break;
+    case FDM_BC_RIGID:
+    case FDM_BC_RIGID_0:
+      vt = 0.0;
+      snt = fd.snt - Zs * fd.vt;
+      vn = 0.0;
+      snn = fd.snn - Zp * fd.vn;
+      break;
+    case FDM_BC_ABSORBING_VELOCITY:
+      vt = -Zsi * fd.snt;
+      snt = fd.snt;
+      vn = -Zpi * fd.snn;
+      snn = fd.snn;
+      break;
+    case FDM_BC_FREE_VELOCITY:
+      vt = fd.vt - Zsi * fd.snt;
+      snt = fd.snt;
+      vn = fd.vn - Zpi * fd.snn;
+      snn = fd.snn;
+      break;
+    default:
+      break;
+  }
+  double stt = fd.stt - gamma * (fd.snn - snn);
+  double szz = fd.szz - gamma * (fd.snn - snn);
+  nt2xy(Fhat, n1, n2, vt, vn, stt, snt, snn, szz);
+  if (subtract) {
+#pragma unroll
+    for (int l = 0; l < 6; l++) Fhat[l] = Fhat[l] + F0[l];
+  }
+}
+
+// locked_interface_mode2 (interfaces.f90:715-772)
+__device__ __forceinline__ void locked_interface_mode2(double* Fhatm, double* Fhatp,
+                                                       const double* Fm, const double* Fp,
+                                                       double n1, double n2, double Zsm,
+                                                       double Zsp, double Zpm, double Zpp,
+                                                       double gammam, double gammap) {
+  NT p = xy2nt(Fp, n1, n2);
+  NT m = xy2nt(Fm, n1, n2);
+  double Zsip = 1.0 / Zsp, Zsim = 1.0 / Zsm, Zpip = 1.0 / Zpp, Zpim = 1.0 / Zpm;
+  double etap = 1.0 / (Zpip + Zpim);
+  double phip = etap * (p.snn * Zpip + m.snn * Zpim + p.vn - m.vn);
+  double snnp = phip, snnm = phip;
+  double vnp = p.vn + Zpip * (p.snn - snnp);
+  double vnm = m.vn - Zpim * (m.snn - snnm);
+  double sttp = p.stt - gammap * (p.snn - snnp);
+  double sttm = m.stt - gammam * (m.snn - snnm);
+  double szzp = p.szz - gammap * (p.snn - snnp);
+  double szzm = m.szz - gammam * (m.snn - snnm);
+  double etas = 1.0 / (Zsip + Zsim);
+  double phis = etas * (p.snt * Zsip + m.snt * Zsim + p.vt - m.vt);
+  double sntp = phis, sntm = phis;
+  double vtp = p.vt + Zsip * (p.snt - sntp);
+  double vtm = m.vt - Zsim * (m.snt - sntm);
+  nt2xy(Fhatp, n1, n2, vtp, vnp, sttp, sntp, snnp, szzp);
+  nt2xy(Fhatm, n1, n2, vtm, vnm, sttm, sntm, snnm, szzm);
+}
+
+// strength (friction.f90:1041-1074)
+__device__ __forceinline__ void strength(int law, double a, double V0, double V, double Psi,
+                                         double N, double& S, double& dSdV) {
+  if (law == FDM_LAW_SL || law == FDM_LAW_FL) {
+    if (V == 0.0) {
+      S = 0.0;
+      dSdV = 1e10;
+    } else {
+      double absV = fabs(V);
+      double f = a * log(absV / V0) + Psi;
+      S = N * fsign(f, V);
+      dSdV = N * a / absV;
+    }
+  } else {
+    double O = 0.5 / V0 * exp(Psi / a);
+    S = a * N * arcsinh_ref(O * V);
+    dSdV = a * N * O / sqrt(1.0 + (V * O) * (V * O));
+  }
+}
+
+// estimate_V (friction.f90:1077-1098)
+__device__ __forceinline__ void estimate_V(int law, double a, double V0, double& V, double Slock,
+                                           double N, double Psi) {
+  double SaN = Slock / (a * N);
+  if (fabs(SaN) > 10.0) return;
+  if (law == FDM_LAW_SL || law == FDM_LAW_FL) {
+    V = fsign(V0 * exp(-Psi / a) * exp(fabs(SaN)), SaN);
+  } else if (law == FDM_LAW_RSL || law == FDM_LAW_RFL || law == FDM_LAW_RAL) {
+    V = 2.0 * V0 * exp(-Psi / a) * sinh(SaN);
+  }
+}
+
+// newton_solver (friction.f90:909-1038): returns true on failure
+__device__ __forceinline__ bool newton_solver(int law, double a, double V0, double& Vio,
+                                             double& Sio, double Slock, double N, double eta,
+                                             double Psi) {
+  const int imax = 99;
+  const double atolR = 2.220446049250313e-16 * 1e4;
+  double V = Vio, S = Sio;
+  double Vmin = fmin(0.0, Slock / eta);
+  double Vmax = fmax(0.0, Slock / eta);
+  estimate_V(law, a, V0, V, Slock, N, Psi);
+  if (V <= Vmin || Vmax <= V) V = 0.5 * (Vmin + Vmax);
+  int i;
+  for (i = 1; i <= imax; i++) {
+    S = Slock - eta * V;
+    double dSdV = -eta;
+    double Sf, dSfdV;
+    strength(law, a, V0, V, Psi, N, Sf, dSfdV);
+    double R = -S + Sf;
+    if (R > 0.0) Vmax = V;
+    if (R < 0.0) Vmin = V;
+    double dRdV = -dSdV + dSfdV;
+    double dV = -R / dRdV;
+    if (fabs(R) <= atolR) break;
+    if (i == imax) break;
+    V = V + dV;
+    if (V <= Vmin || Vmax <= V) {
+      if (Vmin == 0.0)
+        V = 0.1 * Vmax;
+      else if (Vmax == 0.0)
+        V = 0.1 * Vmin;
+      else
+        V = fsign(sqrt(Vmin * Vmax), Slock);
+      dV = Vmax - Vmin;
+    }
+    if (fabs(dV) <= 0.0) break;
+  }
+  Vio = V;
+  Sio = S;
+  return i == imax;
+}
+
+// state_rate (friction.f90:1147-1264)
+__device__ __forceinline__ double state_rate(int law, const double* rs, double V, double Psi) {
+  if (law == FDM_LAW_FRICTIONLESS || law == FDM_LAW_SW || law == FDM_LAW_LOCKED) return 0.0;
+  if (V == 0.0) return 0.0;
+  double a = rs[0], b = rs[1], V0 = rs[2], f0 = rs[3], L = rs[4], fw = rs[5], Vw = rs[6];
+  double absV = fabs(V);
+  double frs, fss, fLV, Psiss;
+  switch (law) {
+    case FDM_LAW_SL:
+      frs = a * log(absV / V0) + Psi;
+      fss = f0 - (b - a) * log(absV / V0);
+      return -(absV / L) * (frs - fss);
+    case FDM_LAW_FL:
+      frs = a * log(absV / V0) + Psi;
+      fLV = f0 - (b - a) * log(absV / V0);
+      fss = fw + (fLV - fw) / pow(1.0 + pow8(V / Vw), 0.125);
+      return -(absV / L) * (frs - fss);
+    case FDM_LAW_RSL:
+      fss = f0 - (b - a) * log(absV / V0);
+      frs = a * arcsinh_ref(0.5 * absV / V0 * exp(Psi / a));
+      return -(absV / L) * (frs - fss);
+    case FDM_LAW_RFL:
+      fLV = f0 - (b - a) * log(absV / V0);
+      fss = fw + (fLV - fw) / pow(1.0 + pow8(V / Vw), 0.125);
+      frs = a * arcsinh_ref(0.5 * absV / V0 * exp(Psi / a));
+      return (absV / L) * (fss - frs);
+    case FDM_LAW_RSF:
+      fLV = f0 - (b - a) * log(absV / V0);
+      fss = fw + (fLV - fw) / pow(1.0 + pow8(V / Vw), 0.125);
+      Psiss = a * log(2.0 * V0 / absV * sinh(fss / a));
+      return (absV / L) * (Psiss - Psi);
+    case FDM_LAW_RAL:
+      return (b * V0 / L) * (exp((f0 - Psi) / b) - absV / V0);
+    default:
+      return 0.0;
+  }
+}
+
+// solve_friction after load_stress (friction.f90:655-826).  Returns true on
+// a second Newton failure (the reference stops the run there).
+__device__ __forceinline__ bool solve_friction_core(int law, const double* rs, const double* sw,
+                                                    int reg, int outside, double flock,
+                                                    double S0i, double N0i, double phip,
+                                                    double phis, double eta, double D,
+                                                    double Psi, double skempton, double dt,
+                                                    double& V, double& S, double& O, double& N,
+                                                    double& Nreg) {
+  double Slock = S0i + phis;
+  double Nlock = N0i - (1.0 - skempton) * phip;
+  O = 0.0;
+  N = Nlock;
+  if (Nlock <= 0.0) {
+    S = 0.0;
+    V = Slock / eta;
+    return false;
+  }
+  if (dt == 0.0) Nreg = N;
+  if (!reg) Nreg = N;
+  if (outside) {
+    double Sv = flock * Nreg;
+    if (fabs(Slock) >= Sv) {
+      Sv = fsign(Sv, Slock);
+      V = (Slock - Sv) / eta;
+      S = Sv;
+    } else {
+      V = 0.0;
+      S = Slock;
+    }
+    return false;
+  }
+  switch (law) {
+    case FDM_LAW_FRICTIONLESS:
+      S = 0.0;
+      V = Slock / eta;
+      return false;
+    case FDM_LAW_LOCKED:
+      V = 0.0;
+      S = Slock;
+      return false;
+    case FDM_LAW_SW: {
+      double c0 = sw[0], fs = sw[1], fd = sw[2], Dc = sw[3];
+      double Sv = c0 + fmax(Nreg, 0.0) * ((fs - (fs - fd) * fmin(D, Dc) / Dc));
+      if (fabs(Slock) >= Sv) {
+        Sv = fsign(Sv, Slock);
+        V = (Slock - Sv) / eta;
+        S = Sv;
+      } else {
+        V = 0.0;
+        S = Slock;
+      }
+      return false;
+    }
+    default: {
+      bool fail = newton_solver(law, rs[0], rs[2], V, S, Slock, Nreg, eta, Psi);
+      if (fail) fail = newton_solver(law, rs[0], rs[2], V, S, Slock, Nreg, eta, Psi);
+      return fail;
+    }
+  }
+}
+
+}  // namespace fdm
