@@ -1,0 +1,173 @@
+"""FDMAP .in problem files for the benchmark configurations.
+
+Each function returns the text of an input file in the reference's format
+(Fortran namelists, '!---BLOCKn---' / '!---IFACEn---' sections and the
+free-text output list), parameterised by size so that the same problem can
+be run at full size (benchmark) and reduced size (parity tests).
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+
+RHO, CS, CP = 2.67, 3.464, 6.0
+
+
+def _f(v):
+    """Fortran double literal."""
+    return repr(float(v)).replace("e", "d") if "e" in repr(float(v)) else repr(float(v)) + "d0"
+
+
+def rs_state(a, V0, tau, sn, V):
+    """Psi such that the regularised law slips at V under tau, sn
+    (inverse of friction.f90:1066-1070)."""
+    x = tau / (a * sn)
+    # a*log(2*V0/V*sinh(x)) without overflow for large x
+    return a * (math.log(2.0 * V0 / V) + x + math.log(0.5 * (1.0 - math.exp(-2.0 * x))))
+
+
+def c1_pointsource(name="c1", n=401, h=0.025, nsteps=1000, outputs=True, ninfo=100):
+    """Single elastic block n x n, free surface on top, absorbing elsewhere,
+    isotropic moment-tensor source at the centre with a Gaussian source time
+    function of width 0.05 s, LSRK(5,4) at CFL 0.5."""
+    L = 0.5 * (n - 1) * h
+    out = ""
+    if outputs:
+        out = f"""B
+body
+vx vy
+0d0 1d10 {max(1, nsteps // 10)}
+"""
+    return f"""&problem_list
+ name = '{name}', nt = {nsteps + 1}, CFL = 0.5d0, RKorder = 4, ninfo = {ninfo} /
+
+&domain_list
+ mode = 2, FDmethod = 'SBP4', nblocks = 1, nx = {n}, ny = {n} /
+
+!---BLOCK1---
+
+&grid_list
+ nx = {n}, ny = {n},
+ LB = {_f(-L)},{_f(-L)}, LT = {_f(-L)},{_f(L)},
+ RB = {_f(L)},{_f(-L)}, RT = {_f(L)},{_f(L)} /
+
+&boundaries_list bcL = 'absorbing', bcR = 'absorbing', bcB = 'absorbing', bcT = 'free' /
+
+&material_list rho = {_f(RHO)}, cs = {_f(CS)}, cp = {_f(CP)} /
+
+&source_list
+ Mxx = 1d0, Myy = 1d0, x0 = 0d0, y0 = 0d0,
+ t_center = 0.2d0, t_duration = 0.05d0, source_time_function = 'gaussian' /
+
+!---begin:output_list---
+{out}!---end:output_list---
+"""
+
+
+def c2_planar_rs(name="c2", nx=1601, ny=801, h=0.025, nsteps=4000, outputs=True, ninfo=500,
+                 law="RFL", plastic=False, curve=None, tau0=70.0, sn0=120.0, overstress=25.0,
+                 R=1.5, Vinit=1e-12, a=0.010, b=0.014):
+    """Two blocks nx x ny coupled across a horizontal fault by SAT; slip law
+    with strong velocity weakening; Gaussian overstress at the fault centre.
+    curve: optional path of a fault-profile curve file (FP64 x, y, n1, n2),
+    used for the top edge of block 1 and the bottom edge of block 2."""
+    Lx = 0.5 * (nx - 1) * h
+    Ly = (ny - 1) * h
+    psi0 = rs_state(a, 1e-6, tau0, sn0, Vinit)
+    c1 = f"bndTfile = '{curve}'," if curve else ""
+    c2 = f"bndBfile = '{curve}'," if curve else ""
+    mat = f"rho = {_f(RHO)}, cs = {_f(CS)}, cp = {_f(CP)}"
+    if plastic:
+        mat = f"response = 'plastic', {mat}, mu = 0.6d0, b = 0d0, beta = 0d0"
+    out = ""
+    if outputs:
+        st = max(1, nsteps // 100)
+        out = f"""I
+iface1
+D V S N Psi
+0d0 1d10 {st}
+"""
+    return f"""&problem_list
+ name = '{name}', nt = {nsteps + 1}, CFL = 0.5d0, RKorder = 4, ninfo = {ninfo} /
+
+&domain_list
+ mode = 2, FDmethod = 'SBP4', nblocks_x = 1, nblocks_y = 2, nx = {nx}, ny = {2 * ny} /
+
+!---BLOCK1---
+
+&grid_list
+ nx = {nx}, ny = {ny}, {c1}
+ LB = {_f(-Lx)},{_f(-Ly)}, LT = {_f(-Lx)},0d0,
+ RB = {_f(Lx)},{_f(-Ly)}, RT = {_f(Lx)},0d0 /
+
+&boundaries_list bcL = 'absorbing', bcR = 'absorbing', bcB = 'absorbing', bcT = 'none' /
+
+!---BLOCK2---
+
+&grid_list
+ nx = {nx}, ny = {ny}, {c2}
+ LB = {_f(-Lx)},0d0, LT = {_f(-Lx)},{_f(Ly)},
+ RB = {_f(Lx)},0d0, RT = {_f(Lx)},{_f(Ly)} /
+
+&boundaries_list bcL = 'absorbing', bcR = 'absorbing', bcB = 'none', bcT = 'absorbing' /
+
+!---ALL BLOCKS---
+
+&fields_list sxx0 = {_f(-sn0)}, sxy0 = {_f(tau0)}, syy0 = {_f(-sn0)}, szz0 = {_f(-sn0)} /
+
+&material_list {mat} /
+
+!---IFACE1---
+
+&interface_list coupling = 'friction', iblockm = 1, iblockp = 2, direction = 'y' /
+
+&friction_list
+ friction_law = '{law}',
+ rs%a = {_f(a)}, rs%b = {_f(b)}, rs%V0 = 1d-6, rs%f0 = 0.6d0, rs%L = 0.4d0,
+ rs%fw = 0.1d0, rs%Vw = 0.1d0,
+ Psi0 = {_f(psi0)},
+ ld%shape = 'gaussian', ld%S = {_f(overstress)}, ld%R = {_f(R)}, ld%x0 = 0d0 /
+
+!---begin:output_list---
+{out}!---end:output_list---
+"""
+
+
+def fractal_profile(nx, h, alpha=1e-2, seed=0, lam_min_h=20):
+    """Seeded band-limited self-similar profile y(x) over nx points of
+    spacing h: power spectrum ~ k^-3 between wavelengths lam_min_h*h and the
+    profile length L, scaled so that rms(y) = alpha*L.  Returns x, y, and the
+    downward unit normal (n1, n2) = (y', -1)/sqrt(1+y'^2) of the curve-file
+    convention (grid.f90:686-695)."""
+    rng = np.random.default_rng(seed)
+    L = (nx - 1) * h
+    x = np.linspace(-0.5 * L, 0.5 * L, nx)
+    kmin = 2 * np.pi / L
+    kmax = 2 * np.pi / (lam_min_h * h)
+    nk = int(np.floor(kmax / kmin))
+    k = kmin * np.arange(1, nk + 1)
+    amp = k ** -1.5
+    ph = rng.uniform(0, 2 * np.pi, nk)
+    y = (amp[None, :] * np.cos(np.outer(x, k) + ph[None, :])).sum(axis=1)
+    yp = (-amp[None, :] * k[None, :] * np.sin(np.outer(x, k) + ph[None, :])).sum(axis=1)
+    s = alpha * L / np.sqrt(np.mean(y ** 2))
+    y *= s
+    yp *= s
+    nrm = np.sqrt(1.0 + yp ** 2)
+    return x, y, yp / nrm, -1.0 / nrm
+
+
+def write_curve(path, x, y, n1, n2):
+    """binary FP64 curve file: x(1:n), y(1:n), n1(1:n), n2(1:n)"""
+    with open(path, "wb") as f:
+        for a in (x, y, n1, n2):
+            np.ascontiguousarray(a, dtype="<f8").tofile(f)
+
+
+def write(path, text):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    with open(path, "w") as f:
+        f.write(text)
+    return path
