@@ -1,0 +1,179 @@
+#!/usr/bin/env python3
+"""Benchmark of the FDMAP time-stepping path on one MI355X.
+
+Workload: the planar rate-and-state rupture, two blocks of 1601 x 801
+points (h = 0.025 km) coupled by a frictional fault (slip law with strong
+velocity weakening), LSRK(5,4) at CFL 0.5, built from its .in file by the
+Fortran host.  A step is one full LSRK(5,4) step of the whole grid.
+
+Prints one JSON line: FP64 updates/s (value), the roofline of the bulk
+kernel measured with HIP events inside the timed region, the FP32-bulk
+throughput and its error against FP64, and the CPU oracle timed on the
+host cores as the CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("updates/s & %HBM peak, FP32-bulk vs FP64 at stated L2 err; adjoint/fwd time")
+HBM_PEAK_GBS = 8000.0
+# algorithmic bytes per grid point per bulk launch: read F(6) + DF(6), write
+# DF(6) + F(6), read 5 metric arrays; the first stage does not read DF
+# (A1 = 0).  FP64: 232 B (184 B at stage 1); FP32-bulk: half.
+BYTES_STAGE = {"fp64": (184.0, 232.0), "fp32": (92.0, 116.0)}
+
+
+def rel_l2(a, b):
+    d = np.linalg.norm(np.asarray(b))
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / (d if d > 0 else 1.0))
+
+
+def run_device(inpath, precision, steps, warmup):
+    from fdmap_github_amd import host
+    from fdmap_github_amd.device import Context
+
+    hp = host.HostProblem(inpath, precision=precision)
+    ctx = Context(hp.ptr)
+    npts = hp.nx * hp.ny
+    ctx.set_fields(hp.fields())
+    ctx.init_edges(hp.t)
+    ctx.step(hp.t, hp.dt, 1, warmup)
+    ctx.sync()
+    ctx.profile_enable(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(hp.t, hp.dt, warmup + 1, steps)
+    ctx.sync()
+    el = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    F, _ = ctx.get_fields()
+    fault = {f: ctx.get_iface(0, f) for f in ("D", "V", "S")}
+    nstage = hp.ptr.contents.nstage
+    ctx.close()
+    return dict(hp=hp, npts=npts, el=el, prof=prof, F=F, fault=fault, nstage=nstage)
+
+
+def cpu_baseline(inpath, sample_steps):
+    """Oracle (C restatement of the reference, OpenMP) on the same problem."""
+    from fdmap_github_amd import host
+    from oracle.binding import OracleContext
+
+    threads = min(os.cpu_count() or 1, 16)
+    hp = host.HostProblem(inpath, precision="fp64")
+    o = OracleContext(hp.ptr, threads=threads)
+    o.set_fields(hp.fields())
+    o.init_edges(hp.t)
+    o.step(hp.t, hp.dt, 1, 1)
+    t0 = time.perf_counter()
+    o.step(hp.t, hp.dt, 2, sample_steps)
+    el = time.perf_counter() - t0
+    return dict(value=hp.nx * hp.ny * sample_steps / el, unit="updates/s", cores=threads,
+                kind="port",
+                sample=f"{sample_steps} LSRK(5,4) steps of the same 2x(1601x801) problem "
+                       f"({el:.1f} s wall on {threads} threads)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--nx", type=int, default=1601)
+    ap.add_argument("--ny", type=int, default=801)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
+              "(blocks would shard across GPUs as described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from fdmap_github_amd import inputs
+
+    tmp = tempfile.mkdtemp(prefix="fdmap_bench_")
+    inpath = inputs.write(os.path.join(tmp, "c2.in"),
+                          inputs.c2_planar_rs(name=os.path.join(tmp, "c2"), nx=args.nx,
+                                              ny=args.ny, nsteps=args.steps + args.warmup,
+                                              outputs=False))
+    r64 = run_device(inpath, "fp64", args.steps, args.warmup)
+    npts = r64["npts"]
+    prof = r64["prof"]
+    ns = r64["nstage"]
+    b1, b2 = BYTES_STAGE["fp64"]
+    bytes_launch = npts * (b1 + (ns - 1) * b2) / ns
+    avg_ms = prof["bulk_ms"] / max(1, prof["bulk_launches"])
+    achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+    ups = npts * args.steps / r64["el"]
+    out = {
+        "metric": METRIC,
+        "value": ups,
+        "unit": "updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * r64["el"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (planar fault problem generated as an FDMAP .in file)",
+        "config": {
+            "workload": "planar rate-and-state rupture, 2 blocks x (1601x801), h=0.025 km, "
+                        "RFL slip law, LSRK(5,4) CFL 0.5, FP64",
+            "grid_points": npts,
+            "fault_nodes": args.nx,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "bulk_kernel<double> (SBP4 rates + SAT + RK update)",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "alg_bytes_per_launch": bytes_launch,
+            "avg_launch_ms": avg_ms,
+            "launches": prof["bulk_launches"],
+            "edge_kernel_avg_ms": prof["edge_ms"] / max(1, prof["edge_launches"]),
+            "hbm_frac_end_to_end": npts * (b1 + (ns - 1) * b2) * args.steps / r64["el"] / 1e9
+                                   / HBM_PEAK_GBS,
+        },
+    }
+    tr = os.path.join(ROOT, "profiles", "bulk_c2_fp64_traffic.json")
+    if os.path.exists(tr):
+        with open(tr) as f:
+            out["roofline"]["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+    if not args.no_fp32:
+        r32 = run_device(inpath, "fp32", args.steps, args.warmup)
+        ups32 = npts * args.steps / r32["el"]
+        p32 = r32["prof"]
+        c1, c2 = BYTES_STAGE["fp32"]
+        avg32 = p32["bulk_ms"] / max(1, p32["bulk_launches"])
+        out["fp32_bulk"] = {
+            "updates_per_s": ups32,
+            "speedup_vs_fp64": ups32 / ups,
+            "ms_per_step": 1e3 * r32["el"] / args.steps,
+            "bulk_avg_launch_ms": avg32,
+            "bulk_achieved_GBs": npts * (c1 + (ns - 1) * c2) / ns / (avg32 * 1e-3) / 1e9,
+            "rel_l2_fields_vs_fp64": rel_l2(r32["F"], r64["F"]),
+            "rel_l2_slip_vs_fp64": rel_l2(r32["fault"]["D"], r64["fault"]["D"]),
+            "rel_l2_slip_velocity_vs_fp64": rel_l2(r32["fault"]["V"], r64["fault"]["V"]),
+            "after_steps": args.steps + args.warmup,
+        }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(inpath, args.cpu_steps)
+        out["cpu_baseline"]["gpu_over_cpu"] = ups / out["cpu_baseline"]["value"]
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,78 @@
+"""Fortran host (CPU): .in parsing and setup against closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+from fdmap_github_amd import host, inputs
+
+H00I = 2.823529411764706
+
+
+def test_c2_setup_closed_forms(tmp_path):
+    h = 0.025
+    p = inputs.write(str(tmp_path / "c2.in"), inputs.c2_planar_rs(name="c2", nx=81, ny=41, nsteps=10))
+    hp = host.HostProblem(p)
+    assert (hp.nx, hp.ny, hp.nt) == (81, 82, 11)
+    # set_dt: dt = CFL*hmin/sqrt(cs^2+cp^2) (domain.f90:1511-1524)
+    assert math.isclose(hp.dt, 0.5 * h / math.sqrt(3.464 ** 2 + 6.0 ** 2), rel_tol=1e-13)
+    P = hp.ptr.contents
+    assert P.nblocks == 2 and P.nifaces == 1 and P.nstage == 5
+    b1, b2 = P.blocks[0], P.blocks[1]
+    assert (b1.mx, b1.px, b1.my, b1.py) == (0, 80, 0, 40)
+    assert (b2.mx, b2.px, b2.my, b2.py) == (0, 80, 41, 81)
+    assert [b1.side[s].bc for s in range(4)] == [1, 1, 1, 0]
+    assert [b2.side[s].bc for s in range(4)] == [1, 1, 0, 1]
+    # metrics of a Cartesian grid: xq = yr = h, xr = yq = 0, Ji = 1/h^2
+    for name, v in (("xq", h), ("yr", h), ("xr", 0.0), ("yq", 0.0), ("Ji", 1 / h ** 2)):
+        np.testing.assert_allclose(hp.coords(name), v, rtol=1e-12, atol=1e-14)
+    # init_edges coefficients on block 1's top side (domain.f90:1439-1457)
+    n = b1.side[3].n
+    M = np.ctypeslib.as_array(b1.side[3].M, shape=(5, n))
+    rho, cs, cp = 2.67, 3.464, 6.0
+    np.testing.assert_allclose(M[0], rho * cs, rtol=1e-15)
+    np.testing.assert_allclose(M[1], rho * cp, rtol=1e-15)
+    np.testing.assert_allclose(M[2], 1 - 2 * (cs / cp) ** 2, rtol=1e-15)
+    np.testing.assert_allclose(M[3], H00I * cs / h, rtol=1e-12)
+    np.testing.assert_allclose(M[4], H00I * cp / h, rtol=1e-12)
+    nrm = np.ctypeslib.as_array(b1.side[3].normal, shape=(2, n))
+    np.testing.assert_allclose(nrm[0], 0.0, atol=1e-15)
+    np.testing.assert_allclose(nrm[1], 1.0)
+    # fault: nhat from block 1 to 2, Psi0 steady state at 1e-12 m/s, overstress
+    I = P.ifaces[0]
+    assert (I.direction, I.m, I.p, I.coupling, I.law) == (1, 0, 80, 1, 6)
+    psi = np.ctypeslib.as_array(I.Psi0, shape=(81,))
+    a, V0 = 0.01, 1e-6
+    np.testing.assert_allclose(psi, a * math.log(2 * V0 / 1e-12 * math.sinh(70 / (a * 120))), rtol=1e-12)
+    ldS = np.ctypeslib.as_array(I.ldS, shape=(81,))
+    x = np.linspace(-1.0, 1.0, 81)
+    np.testing.assert_allclose(ldS, 25 * np.exp(-0.5 * (x / 1.5) ** 2), rtol=1e-12)
+    F = hp.fields()
+    for l, v in enumerate([0, 0, -120, 70, -120, -120]):
+        assert np.all(F[l] == v)
+
+
+def test_c1_source_placement(tmp_path):
+    p = inputs.write(str(tmp_path / "c1.in"), inputs.c1_pointsource(name="c1", n=41, nsteps=5))
+    hp = host.HostProblem(p)
+    s = hp.ptr.contents.source
+    assert s.nplace == 1 and s.stf == 2
+    pl = s.place[0]
+    assert (pl.block, pl.i0, pl.j0) == (0, 20, 20)
+    # a = 0: weights (-gam, beta+3gam-gap, 1-2beta-3gam+3gap, beta+gam-3gap, gap)/h
+    h = 0.025
+    beta, gam, gap = 0.5, -1 / 16, 1 / 16
+    want = np.array([-gam, beta + 3 * gam - gap, 1 - 2 * beta - 3 * gam + 3 * gap,
+                     beta + gam - 3 * gap, gap]) / h
+    np.testing.assert_allclose(np.array(pl.dx[:]), want, rtol=1e-14)
+    assert abs(sum(pl.dx[:]) * h - 1.0) < 1e-14  # discrete delta integrates to 1
+
+
+@pytest.mark.parametrize("text,msg", [
+    ("&problem_list nt = 2 /\n&domain_list mode = 3, FDmethod = 'SBP4' /\n", "mode = 2"),
+    ("&problem_list nt = 2 /\n&domain_list FDmethod = 'SBP6' /\n", "SBP4"),
+])
+def test_unsupported_options_fail_loudly(tmp_path, text, msg):
+    p = inputs.write(str(tmp_path / "bad.in"), text)
+    with pytest.raises(RuntimeError, match=msg):
+        host.HostProblem(p)

@@ -1,0 +1,94 @@
+"""Problems built by the Fortran host from .in files: HIP library vs CPU
+oracle, and the fdmap driver executable end to end."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fdmap_github_amd import capi, host, inputs
+from fdmap_github_amd.device import Context
+from oracle.binding import OracleContext
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    d = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (d if d > 0 else 1.0)
+
+
+def run_pair(path, nsteps, precision="fp64"):
+    hp = host.HostProblem(path, precision=precision)
+    F0 = hp.fields()
+    dev = Context(hp.ptr)
+    dev.set_fields(F0)
+    dev.init_edges(hp.t)
+    dev.step(hp.t, hp.dt, 1, nsteps)
+    hp2 = host.HostProblem(path, precision="fp64")
+    orc = OracleContext(hp2.ptr, threads=4)
+    orc.set_fields(F0)
+    orc.init_edges(hp2.t)
+    orc.step(hp2.t, hp2.dt, 1, nsteps)
+    return hp, dev, orc
+
+
+def test_c1_point_source_small_bit_exact(gpu, tmp_path):
+    p = inputs.write(str(tmp_path / "c1.in"), inputs.c1_pointsource(name=str(tmp_path / "c1"), n=81,
+                                                                     nsteps=150, outputs=False))
+    hp, dev, orc = run_pair(p, 150)
+    Fd, DFd = dev.get_fields()
+    Fo, DFo = orc.get_fields()
+    print("max |v|", np.abs(Fo[:2]).max(), "max |dF|", np.abs(Fd - Fo).max())
+    assert np.abs(Fo[:2]).max() > 0  # the source radiated
+    assert np.array_equal(Fd, Fo)
+    assert np.array_equal(DFd, DFo)
+
+
+def test_c2_planar_rupture_small(gpu, tmp_path):
+    p = inputs.write(str(tmp_path / "c2.in"), inputs.c2_planar_rs(name=str(tmp_path / "c2"), nx=201,
+                                                                   ny=101, nsteps=400, outputs=False,
+                                                                   R=0.5))
+    hp, dev, orc = run_pair(p, 400)
+    Fd, _ = dev.get_fields()
+    Fo, _ = orc.get_fields()
+    e = rel_l2(Fd, Fo)
+    print("fields", e)
+    assert e <= 1e-12
+    for f in ("D", "V", "S", "N", "Psi"):
+        a, b = dev.get_iface(0, f), orc.get_iface(0, f)
+        print(f, rel_l2(a, b), np.abs(b).max())
+        assert rel_l2(a, b) <= 1e-12
+    assert orc.get_iface(0, "V").max() > 1e-3  # rupture nucleated
+
+
+def test_fdmap_executable_outputs(gpu, tmp_path):
+    nsteps = 60
+    p = inputs.write(str(tmp_path / "c2x.in"),
+                     inputs.c2_planar_rs(name=str(tmp_path / "c2x"), nx=121, ny=61, nsteps=nsteps,
+                                         outputs=True, R=0.5))
+    r = subprocess.run([host.EXE_PATH, p], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    assert "timing: 1" in r.stdout and "Finished problem" in r.stdout
+    # stride nsteps//100 -> 1: V written at every step (stage-1 values)
+    V = np.fromfile(str(tmp_path / "c2x_I_V.dat"), dtype=np.float32).reshape(-1, 121)
+    t = np.fromfile(str(tmp_path / "c2x_I_V.t"), dtype=np.float32)
+    assert V.shape[0] == nsteps + 1 and t.shape[0] == nsteps + 1
+    # oracle: stage-1 V of step n is what the driver writes for step n
+    hp = host.HostProblem(p)
+    orc = OracleContext(hp.ptr, threads=4)
+    orc.set_fields(hp.fields())
+    orc.init_edges(hp.t)
+    for n in (1, 25, nsteps):
+        o2 = OracleContext(hp.ptr, threads=4)
+        o2.set_fields(hp.fields())
+        o2.init_edges(hp.t)
+        if n > 1:
+            o2.step(hp.t, hp.dt, 1, n - 1)
+        o2.stage1(hp.t, hp.dt, n)
+        Vo = o2.get_iface(0, "V", stage1=True).astype(np.float32)
+        np.testing.assert_allclose(V[n - 1], Vo, rtol=1e-6, atol=1e-30)
+        assert abs(t[n - 1] - np.float32(hp.t + (n - 1) * hp.dt)) <= 1e-6 * max(1.0, abs(t[n - 1]))
+    m = open(str(tmp_path / "c2x.m")).read()
+    assert "I_V.nx =" in m and "I_V.field = 'V';" in m
