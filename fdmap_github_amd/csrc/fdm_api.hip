@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -207,6 +208,15 @@ struct fdm_ctx {
   std::vector<int> pair_kind;
   std::string err;
   std::vector<void*> allocs;
+  // launch shape of the bulk kernel (tile 64 x tile_y points)
+  int tile_y = 4;
+  int xcd_swizzle = 0;
+  // row-marching kernel: strip width, rows per segment, segment table
+  int march = 1;
+  int march_tx = 128;
+  int march_h = 8;
+  int nsegs = 0;
+  int4* segs = nullptr;
 };
 
 namespace {
@@ -387,10 +397,21 @@ int launch_bulk(fdm_ctx* c, double A, double Bdt, int do_update, double t, doubl
   a.dt = dt;
   a.gammap = c->gammap;
   a.lambda = c->lambda;
-  dim3 block(64, 4);
-  dim3 grid((c->nx + 63) / 64, (c->ny + 3) / 4);
+  a.xcd_swizzle = c->xcd_swizzle;
   EventPair* ep = prof_begin(c, 0);
-  hipLaunchKernelGGL(bulk_kernel<T>, grid, block, 0, c->stream, a);
+  if (c->march) {
+    dim3 grid((c->nx + c->march_tx - 1) / c->march_tx, c->nsegs);
+    if (c->march_tx == 64)
+      hipLaunchKernelGGL((bulk_march<T, 64>), grid, dim3(64), 0, c->stream, a, c->segs);
+    else if (c->march_tx == 256)
+      hipLaunchKernelGGL((bulk_march<T, 256>), grid, dim3(256), 0, c->stream, a, c->segs);
+    else
+      hipLaunchKernelGGL((bulk_march<T, 128>), grid, dim3(128), 0, c->stream, a, c->segs);
+  } else {
+    dim3 block(64, c->tile_y);
+    dim3 grid((c->nx + 63) / 64, (c->ny + c->tile_y - 1) / c->tile_y);
+    hipLaunchKernelGGL(bulk_kernel<T>, grid, block, 0, c->stream, a);
+  }
   prof_end(c, ep, 0);
   HIPCHK(c, hipGetLastError());
   return FDM_OK;
@@ -483,6 +504,21 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   // row pitch: a multiple of 64 elements so every row starts 256/512-byte aligned
   c->ld = (P->nx + 63) / 64 * 64;
   c->ps = (long long)c->ld * P->ny;
+  // launch-shape overrides (tuning sweeps): FDM_TILE_Y in {1,2,4,8,16}, FDM_XCD_SWIZZLE in {0,1}
+  if (const char* e = getenv("FDM_TILE_Y")) {
+    int v = atoi(e);
+    if (v == 1 || v == 2 || v == 4) c->tile_y = v;
+  }
+  if (const char* e = getenv("FDM_BULK")) c->march = strcmp(e, "generic") != 0;
+  if (const char* e = getenv("FDM_MARCH_TX")) {
+    int v = atoi(e);
+    if (v == 64 || v == 128 || v == 256) c->march_tx = v;
+  }
+  if (const char* e = getenv("FDM_MARCH_H")) {
+    int v = atoi(e);
+    if (v >= 1 && v <= 4096) c->march_h = v;
+  }
+  if (const char* e = getenv("FDM_XCD_SWIZZLE")) c->xcd_swizzle = atoi(e) != 0;
   (void)hipGetDevice(&c->device);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
@@ -566,6 +602,21 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   }
   int rc;
   if ((rc = dupload(c, &c->blk, blk.data(), blk.size() * sizeof(int)))) return fail(rc);
+  {
+    // row segments of the marching kernel: per block row, the first and last
+    // four rows (closures in r) and chunks of interior rows
+    std::vector<int4> segs;
+    for (int cy = 0; cy < c->nby; cy++) {
+      const fdm_block& B0 = P->blocks[blk[(size_t)cy * c->nbx]];
+      int my = B0.my, py = B0.py;
+      segs.push_back(make_int4(my, my + 3, cy, 1));
+      for (int j = my + 4; j <= py - 4; j += c->march_h)
+        segs.push_back(make_int4(j, std::min(j + c->march_h - 1, py - 4), cy, 0));
+      segs.push_back(make_int4(py - 3, py, cy, 1));
+    }
+    c->nsegs = (int)segs.size();
+    if ((rc = dupload(c, &c->segs, segs.data(), segs.size() * sizeof(int4)))) return fail(rc);
+  }
   if ((rc = dupload(c, &c->colb, colb.data(), colb.size() * sizeof(int)))) return fail(rc);
   if ((rc = dupload(c, &c->rowb, rowb.data(), rowb.size() * sizeof(int)))) return fail(rc);
 

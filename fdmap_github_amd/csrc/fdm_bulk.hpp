@@ -39,7 +39,26 @@ struct BulkArgs {
   double dt;
   double* gammap;
   double* lambda;
+  int xcd_swizzle;  // remap workgroups so each XCD (and its L2) owns contiguous row bands
 };
+
+// Map the hardware workgroup id to a tile.  Workgroups are dispatched to the
+// 8 XCDs round-robin; with the swizzle, XCD x receives a contiguous range of
+// tiles (x-fastest), so the rows a tile shares with its neighbours above and
+// below are read through the same L2.
+__device__ __forceinline__ void tile_of(int swz, int& tx, int& ty) {
+  if (!swz) {
+    tx = blockIdx.x;
+    ty = blockIdx.y;
+    return;
+  }
+  const int nwg = gridDim.x * gridDim.y;
+  const int bid = blockIdx.y * gridDim.x + blockIdx.x;
+  const int q = nwg >> 3, r = nwg & 7, x = bid & 7, k = bid >> 3;
+  const int t = (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+  tx = t % gridDim.x;
+  ty = t / gridDim.x;
+}
 
 // field coupling of rates_*_mode2 (rates.f90:411-444): DF(l) is built from
 // F(UA) paired with y-metrics and F(UB) paired with x-metrics; gamma
@@ -244,11 +263,72 @@ __device__ __forceinline__ void plastic_point(double G, double K, double mu, dou
   Ds[3] = Ds[3] - lam * W[5];
 }
 
+// szz overwrite, sources, RK update, plasticity and stores of one point
+// (rates.f90:920-936, source.f90:196-296, fields.f90:1225-1242,
+// plastic.f90:8-59).  Fc: the point's fields at the start of the stage.
 template <typename T>
-__global__ void __launch_bounds__(256) bulk_kernel(BulkArgs<T> a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i >= a.nx || j >= a.ny) return;
+__device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDev& B, const int bi,
+                                             const int i, const int j, const long long cc,
+                                             T (&d)[6], const T (&Fc)[6]) {
+  const long long ps = a.ps;
+  // rates_szz (rates.f90:920-936): overwrite
+  d[5] = (T)B.nu * (d[2] + d[4]);
+  // singular source (source.f90:196-254)
+  for (int p = 0; p < a.nplace; p++) {
+    const SourcePlaceDev& P = a.src[p];
+    int di = i - P.i0 + 2, dj = j - P.j0 + 2;
+    if (P.block == bi && di >= 0 && di <= 4 && dj >= 0 && dj <= 4) {
+      double dx = P.dx[di], dy = P.dy[dj];
+      d[0] = d[0] + (T)(a.srcA * a.Fx * dx * dy / B.rho);
+      d[1] = d[1] + (T)(a.srcA * a.Fy * dx * dy / B.rho);
+      d[2] = d[2] - (T)(a.srcA * a.Mxx * dx * dy);
+      d[3] = d[3] - (T)(a.srcA * a.Mxy * dx * dy);
+      d[4] = d[4] - (T)(a.srcA * a.Myy * dx * dy);
+    }
+  }
+  // manufactured-solution source (source.f90:256-269)
+  if (B.mms) {
+    double x = a.X[(long long)j * a.nx + i], y = a.Y[(long long)j * a.nx + i];
+#pragma unroll 1
+    for (int l = 0; l < 6; l++) d[l] = d[l] + (T)mms_sin(x, y, a.t, B.id, 6 + l);
+  }
+  if (a.do_update) {
+    // update_fields_interior (fields.f90:1225-1242)
+    const T Bdt = (T)a.Bdt;
+    T f[6];
+#pragma unroll
+    for (int l = 0; l < 6; l++) f[l] = Fc[l] + Bdt * d[l];
+    if (a.do_plastic && B.plastic) {
+      // update_fields_plastic (plastic.f90:8-59), full dt, after the last stage
+      double s[4], Ds[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        s[k] = (double)f[2 + k];
+        Ds[k] = (double)d[2 + k];
+      }
+      long long pc = (long long)j * a.nx + i;
+      double lam = a.lambda[pc], gp = a.gammap[pc];
+      plastic_point(B.G, B.K, B.mu, B.beta, B.b, B.h, B.eta, B.s0, a.dt, s, Ds, lam, gp);
+      a.lambda[pc] = lam;
+      a.gammap[pc] = gp;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        f[2 + k] = (T)s[k];
+        d[2 + k] = (T)Ds[k];
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < 6; l++) a.Fout[l * ps + cc] = f[l];
+  }
+#pragma unroll
+  for (int l = 0; l < 6; l++) a.DF[l * ps + cc] = d[l];
+}
+
+// Everything that happens at one grid point in one stage, reading its
+// neighbours directly from global memory: used for points next to block
+// edges (one-sided closures in r) and as the reference path.
+template <typename T>
+__device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i, const int j) {
   const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
   const BlockDev& B = a.blocks[bi];
   const long long ps = a.ps;
@@ -262,7 +342,8 @@ __global__ void __launch_bounds__(256) bulk_kernel(BulkArgs<T> a) {
   } else {
     const T A = (T)a.A;
 #pragma unroll
-    for (int l = 0; l < 6; l++) d[l] = A * a.DF[l * ps + cc];
+    for (int l = 0; l < 5; l++) d[l] = A * a.DF[l * ps + cc];
+    d[5] = (T)0.0;  // DF6 is overwritten by the szz relation below
   }
   const int dl = i - B.mx, dr = B.px - i, db = j - B.my, dtp = B.py - j;
   // SAT penalties (time_step.f90:754-810), sides in the order L, R, B, T
@@ -360,57 +441,200 @@ __global__ void __launch_bounds__(256) bulk_kernel(BulkArgs<T> a) {
       d[4] = d[4] + bscale<T, 2>(qc<T, 4>(F, yr, xr, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
     }
   }
-  // rates_szz (rates.f90:920-936): overwrite
-  d[5] = (T)B.nu * (d[2] + d[4]);
-  // singular source (source.f90:196-254)
-  for (int p = 0; p < a.nplace; p++) {
-    const SourcePlaceDev& P = a.src[p];
-    int di = i - P.i0 + 2, dj = j - P.j0 + 2;
-    if (P.block == bi && di >= 0 && di <= 4 && dj >= 0 && dj <= 4) {
-      double dx = P.dx[di], dy = P.dy[dj];
-      d[0] = d[0] + (T)(a.srcA * a.Fx * dx * dy / B.rho);
-      d[1] = d[1] + (T)(a.srcA * a.Fy * dx * dy / B.rho);
-      d[2] = d[2] - (T)(a.srcA * a.Mxx * dx * dy);
-      d[3] = d[3] - (T)(a.srcA * a.Mxy * dx * dy);
-      d[4] = d[4] - (T)(a.srcA * a.Myy * dx * dy);
+  T Fc[6];
+#pragma unroll
+  for (int l = 0; l < 6; l++) Fc[l] = F[l * ps + cc];
+  finish_point<T>(a, B, bi, i, j, cc, d, Fc);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) bulk_kernel(BulkArgs<T> a) {
+  int tx, ty;
+  tile_of(a.xcd_swizzle, tx, ty);
+  const int i = tx * blockDim.x + threadIdx.x;
+  const int j = ty * blockDim.y + threadIdx.y;
+  if (i >= a.nx || j >= a.ny) return;
+  point_generic<T>(a, i, j);
+}
+
+
+// Row-marching bulk kernel.  A workgroup owns a strip of TX columns and
+// sweeps a segment of rows.  The arrays differentiated in r (F1..F5, xq,
+// yq) live in a five-row register window, so each is read once per row;
+// the arrays differentiated in q (F1..F5, xr, yr) of the current row are
+// exchanged through a double-buffered LDS row with a halo of 5 columns,
+// wide enough for the one-sided closures at block x-edges.  Segments that
+// touch the first or last four rows of a block (closures in r) use the
+// per-point path.  Each point's arithmetic is that of point_generic.
+// seg: x = first row, y = last row, z = block row, w = 1 for closure rows.
+template <typename T, int TX>
+__global__ void __launch_bounds__(TX) bulk_march(BulkArgs<T> a, const int4* __restrict__ segs) {
+  constexpr int HALO = 5;
+  constexpr int W = TX + 2 * HALO;
+  __shared__ T sh[2][7][W];
+  const int4 sg = segs[blockIdx.y];
+  const int t = threadIdx.x;
+  const int i = blockIdx.x * TX + t;
+  const bool valid = i < a.nx;
+  if (sg.w == 1) {
+    if (!valid) return;
+    for (int j = sg.x; j <= sg.y; j++) point_generic<T>(a, i, j);
+    return;
+  }
+  const long long ps = a.ps;
+  const int ld = a.ld;
+  const int ic = valid ? i : a.nx - 1;
+  const int bi = a.blk[sg.z * a.nbx + a.colb[ic]];
+  const BlockDev& B = a.blocks[bi];
+  const int dl = i - B.mx, dr = B.px - i;
+  const T* __restrict__ F = a.Fin;
+  const T* __restrict__ xq = a.xq;
+  const T* __restrict__ yq = a.yq;
+  const T* __restrict__ xr = a.xr;
+  const T* __restrict__ yr = a.yr;
+  const int i0 = blockIdx.x * TX - HALO;  // global column of LDS slot 0
+  int hslot = -1, hcol = 0;
+  if (t < 2 * HALO) {
+    hslot = (t < HALO) ? t : TX + t;
+    hcol = min(max(i0 + hslot, 0), a.nx - 1);
+  }
+  const int c = HALO + t;
+  const T gamma = (T)B.gamma, rho = (T)B.rho, G = (T)B.G, M = (T)B.M;
+  T cI[5][2];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    cI[l][0] = (T)B.cI[l][0];
+    cI[l][1] = (T)B.cI[l][1];
+  }
+  // window w[k][r]: array k (F1..F5, xq, yq) at row j-2+r
+  T w[7][5];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const long long idx = (long long)(sg.x - 2 + r) * ld + ic;
+#pragma unroll
+    for (int k = 0; k < 5; k++) w[k][r] = F[k * ps + idx];
+    w[5][r] = xq[idx];
+    w[6][r] = yq[idx];
+  }
+  for (int j = sg.x; j <= sg.y; j++) {
+    const long long rowj = (long long)j * ld;
+    const long long cc = rowj + ic;
+    {
+      const long long idx = (long long)(j + 2) * ld + ic;
+#pragma unroll
+      for (int k = 0; k < 5; k++) w[k][4] = F[k * ps + idx];
+      w[5][4] = xq[idx];
+      w[6][4] = yq[idx];
+    }
+    const int buf = j & 1;
+#pragma unroll
+    for (int k = 0; k < 5; k++) sh[buf][k][c] = w[k][2];
+    sh[buf][5][c] = xr[cc];
+    sh[buf][6][c] = yr[cc];
+    if (hslot >= 0) {
+      const long long h = rowj + hcol;
+#pragma unroll
+      for (int k = 0; k < 5; k++) sh[buf][k][hslot] = F[k * ps + h];
+      sh[buf][5][hslot] = xr[h];
+      sh[buf][6][hslot] = yr[h];
+    }
+    __syncthreads();
+    if (valid) {
+      const T(*S)[W] = sh[buf];
+      T Fc[6];
+#pragma unroll
+      for (int k = 0; k < 5; k++) Fc[k] = w[k][2];
+      Fc[5] = F[5 * ps + cc];
+      T d[6];
+      if (a.a_zero) {
+#pragma unroll
+        for (int l = 0; l < 6; l++) d[l] = (T)0.0;
+      } else {
+        const T A = (T)a.A;
+#pragma unroll
+        for (int l = 0; l < 5; l++) d[l] = A * a.DF[l * ps + cc];
+        d[5] = (T)0.0;
+      }
+      // SAT on the block's left/right sides (rows here are not edge rows)
+      if (dl == 0) sat_term<T>(d, Fc, a.sd, B.side_off[FDM_SIDE_L] + (j - B.my));
+      if (dr == 0) sat_term<T>(d, Fc, a.sd, B.side_off[FDM_SIDE_R] + (j - B.my));
+      const T Ji = a.Ji[cc];
+      if (dl >= 4 && dr >= 4) {
+        // rates_interior_mode2 (rates.f90:407-448)
+#define FDM_INTERIOR_FIELD(L)                                                                  \
+  {                                                                                            \
+    using P = Fld<L>;                                                                          \
+    _Pragma("unroll") for (int s = 1; s <= 2; s++) {                                           \
+      T ayp = (T)0.5 * (S[6][c + s] + S[6][c]), axp = (T)0.5 * (S[5][c + s] + S[5][c]);        \
+      T aym = (T)0.5 * (S[6][c - s] + S[6][c]), axm = (T)0.5 * (S[5][c - s] + S[5][c]);        \
+      T byp = (T)0.5 * (w[6][2 + s] + w[6][2]), bxp = (T)0.5 * (w[5][2 + s] + w[5][2]);        \
+      T bym = (T)0.5 * (w[6][2 - s] + w[6][2]), bxm = (T)0.5 * (w[5][2 - s] + w[5][2]);        \
+      T tt = prod<T, P::ga>(ayp, S[P::ua][c + s], gamma) -                                     \
+             prod<T, P::gb>(axp, S[P::ub][c + s], gamma) -                                     \
+             prod<T, P::ga>(aym, S[P::ua][c - s], gamma) +                                     \
+             prod<T, P::gb>(axm, S[P::ub][c - s], gamma) -                                     \
+             prod<T, P::ga>(byp, w[P::ua][2 + s], gamma) +                                     \
+             prod<T, P::gb>(bxp, w[P::ub][2 + s], gamma) +                                     \
+             prod<T, P::ga>(bym, w[P::ua][2 - s], gamma) -                                     \
+             prod<T, P::gb>(bxm, w[P::ub][2 - s], gamma);                                      \
+      d[L] = d[L] + tt * cI[L][s - 1] * Ji;                                                    \
+    }                                                                                          \
+  }
+        FDM_INTERIOR_FIELD(0)
+        FDM_INTERIOR_FIELD(1)
+        FDM_INTERIOR_FIELD(2)
+        FDM_INTERIOR_FIELD(3)
+        FDM_INTERIOR_FIELD(4)
+#undef FDM_INTERIOR_FIELD
+      } else {
+        // L or R strip (rates.f90:471-591): one-sided q, then central r
+#pragma unroll
+        for (int s = 0; s < 6; s++) {
+          const int k = (dl < 4) ? (B.mx + s - i0) : (B.px - s - i0);
+          const T Dq = (dl < 4) ? (T)kDL[dl][s] : (T)(-kDL[dr][s]);
+          const T ay = (T)0.5 * (S[6][k] + S[6][c]);
+          const T ax = (T)0.5 * (S[5][k] + S[5][c]);
+#define FDM_Q1(L)                                                                         \
+  {                                                                                       \
+    using P = Fld<L>;                                                                     \
+    T q = prod<T, P::ga>(ay, S[P::ua][k], gamma) - prod<T, P::gb>(ax, S[P::ub][k], gamma); \
+    d[L] = d[L] + bscale<T, P::mat>(q, Dq, rho, G, M) * Ji;                               \
+  }
+          FDM_Q1(0)
+          FDM_Q1(1)
+          FDM_Q1(2)
+          FDM_Q1(3)
+          FDM_Q1(4)
+#undef FDM_Q1
+        }
+#pragma unroll
+        for (int s = 1; s <= 2; s++) {
+          const T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
+          const T byp = (T)0.5 * (w[6][2 + s] + w[6][2]), bxp = (T)0.5 * (w[5][2 + s] + w[5][2]);
+          const T bym = (T)0.5 * (w[6][2 - s] + w[6][2]), bxm = (T)0.5 * (w[5][2 - s] + w[5][2]);
+#define FDM_RC(L)                                                                           \
+  {                                                                                         \
+    using P = Fld<L>;                                                                       \
+    T r = -prod<T, P::ga>(byp, w[P::ua][2 + s], gamma) + prod<T, P::gb>(bxp, w[P::ub][2 + s], gamma) + \
+          prod<T, P::ga>(bym, w[P::ua][2 - s], gamma) - prod<T, P::gb>(bxm, w[P::ub][2 - s], gamma);   \
+    d[L] = d[L] + bscale<T, P::mat>(r, D, rho, G, M) * Ji;                                  \
+  }
+          FDM_RC(0)
+          FDM_RC(1)
+          FDM_RC(2)
+          FDM_RC(3)
+          FDM_RC(4)
+#undef FDM_RC
+        }
+      }
+      finish_point<T>(a, B, bi, i, j, cc, d, Fc);
+    }
+#pragma unroll
+    for (int k = 0; k < 7; k++) {
+#pragma unroll
+      for (int r = 0; r < 4; r++) w[k][r] = w[k][r + 1];
     }
   }
-  // manufactured-solution source (source.f90:256-269)
-  if (B.mms) {
-    double x = a.X[(long long)j * a.nx + i], y = a.Y[(long long)j * a.nx + i];
-#pragma unroll 1
-    for (int l = 0; l < 6; l++) d[l] = d[l] + (T)mms_sin(x, y, a.t, B.id, 6 + l);
-  }
-  if (a.do_update) {
-    // update_fields_interior (fields.f90:1225-1242)
-    const T Bdt = (T)a.Bdt;
-    T f[6];
-#pragma unroll
-    for (int l = 0; l < 6; l++) f[l] = F[l * ps + cc] + Bdt * d[l];
-    if (a.do_plastic && B.plastic) {
-      // update_fields_plastic (plastic.f90:8-59), full dt, after the last stage
-      double s[4], Ds[4];
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        s[k] = (double)f[2 + k];
-        Ds[k] = (double)d[2 + k];
-      }
-      long long pc = (long long)j * a.nx + i;
-      double lam = a.lambda[pc], gp = a.gammap[pc];
-      plastic_point(B.G, B.K, B.mu, B.beta, B.b, B.h, B.eta, B.s0, a.dt, s, Ds, lam, gp);
-      a.lambda[pc] = lam;
-      a.gammap[pc] = gp;
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        f[2 + k] = (T)s[k];
-        d[2 + k] = (T)Ds[k];
-      }
-    }
-#pragma unroll
-    for (int l = 0; l < 6; l++) a.Fout[l * ps + cc] = f[l];
-  }
-#pragma unroll
-  for (int l = 0; l < 6; l++) a.DF[l * ps + cc] = d[l];
 }
 
 }  // namespace fdm
