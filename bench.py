@@ -81,6 +81,31 @@ def cpu_baseline(inpath, sample_steps):
                        f"({el:.1f} s wall on {threads} threads)")
 
 
+def measured_traffic():
+    """HBM bytes per bulk launch from the committed counter summary (two
+    rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, each scaled by what the
+    counter reads for a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "bulk_tile_c2_fp64_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("hbm_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_check(mib):
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copy check: 3 x {mib} MiB read + {mib} MiB written", file=sys.stderr)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +116,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
+                    help="after the run, copy MIB MiB device to device three times (a plain "
+                         "copy kernel of known size, to calibrate hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -134,7 +162,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "bulk_kernel<double> (SBP4 rates + SAT + RK update)",
+            "kernel": "bulk_tile<double,8,false> (SBP4 rates + SAT + RK update, 64x8 LDS tiles)",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -148,10 +176,7 @@ def main():
                                    / HBM_PEAK_GBS,
         },
     }
-    tr = os.path.join(ROOT, "profiles", "bulk_c2_fp64_traffic.json")
-    if os.path.exists(tr):
-        with open(tr) as f:
-            out["roofline"]["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+    out["roofline"]["traffic"] = measured_traffic()
     if not args.no_fp32:
         r32 = run_device(inpath, "fp32", args.steps, args.warmup)
         ups32 = npts * args.steps / r32["el"]
@@ -172,7 +197,9 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(inpath, args.cpu_steps)
         out["cpu_baseline"]["gpu_over_cpu"] = ups / out["cpu_baseline"]["value"]
-    print(json.dumps(out))
+    print(json.dumps(out), flush=True)
+    if args.copy_check:
+        copy_check(args.copy_check)
 
 
 if __name__ == "__main__":

@@ -196,6 +196,7 @@ struct fdm_ctx {
   SourcePlaceDev* dsrc = nullptr;
   // plasticity
   bool any_plastic = false;
+  bool any_mms = false;
   double* gammap = nullptr;
   double* lambda = nullptr;
   int* fail = nullptr;
@@ -211,9 +212,10 @@ struct fdm_ctx {
   // launch shape of the bulk kernel (tile 64 x tile_y points)
   int tile_y = 4;
   int xcd_swizzle = 0;
+  // bulk kernel: 0 per-point, 1 row-marching, 2 LDS tile (64 x tile_ty)
+  int bulk_mode = 2;
+  int tile_ty = 8;
   // row-marching kernel: strip width, rows per segment, segment table
-  int march = 1;
-  int march_tx = 128;
   int march_h = 8;
   int nsegs = 0;
   int4* segs = nullptr;
@@ -399,14 +401,25 @@ int launch_bulk(fdm_ctx* c, double A, double Bdt, int do_update, double t, doubl
   a.lambda = c->lambda;
   a.xcd_swizzle = c->xcd_swizzle;
   EventPair* ep = prof_begin(c, 0);
-  if (c->march) {
-    dim3 grid((c->nx + c->march_tx - 1) / c->march_tx, c->nsegs);
-    if (c->march_tx == 64)
-      hipLaunchKernelGGL((bulk_march<T, 64>), grid, dim3(64), 0, c->stream, a, c->segs);
-    else if (c->march_tx == 256)
-      hipLaunchKernelGGL((bulk_march<T, 256>), grid, dim3(256), 0, c->stream, a, c->segs);
-    else
-      hipLaunchKernelGGL((bulk_march<T, 128>), grid, dim3(128), 0, c->stream, a, c->segs);
+  if (c->bulk_mode == 2) {
+    // the lean variant serves launches without sources, manufactured
+    // solutions or a plastic update
+    const bool full = c->src.nplace > 0 || c->any_mms || a.do_plastic;
+    dim3 grid((c->nx + 63) / 64, (c->ny + c->tile_ty - 1) / c->tile_ty);
+    if (c->tile_ty == 4) {
+      if (full)
+        hipLaunchKernelGGL((bulk_tile<T, 4, true>), grid, dim3(64, 4), 0, c->stream, a);
+      else
+        hipLaunchKernelGGL((bulk_tile<T, 4, false>), grid, dim3(64, 4), 0, c->stream, a);
+    } else {
+      if (full)
+        hipLaunchKernelGGL((bulk_tile<T, 8, true>), grid, dim3(64, 8), 0, c->stream, a);
+      else
+        hipLaunchKernelGGL((bulk_tile<T, 8, false>), grid, dim3(64, 8), 0, c->stream, a);
+    }
+  } else if (c->bulk_mode == 1) {
+    dim3 grid((c->nx + 127) / 128, c->nsegs);
+    hipLaunchKernelGGL((bulk_march<T, 128>), grid, dim3(128), 0, c->stream, a, c->segs);
   } else {
     dim3 block(64, c->tile_y);
     dim3 grid((c->nx + 63) / 64, (c->ny + c->tile_y - 1) / c->tile_y);
@@ -504,15 +517,20 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   // row pitch: a multiple of 64 elements so every row starts 256/512-byte aligned
   c->ld = (P->nx + 63) / 64 * 64;
   c->ps = (long long)c->ld * P->ny;
-  // launch-shape overrides (tuning sweeps): FDM_TILE_Y in {1,2,4,8,16}, FDM_XCD_SWIZZLE in {0,1}
+  // launch-shape overrides (tuning sweeps): FDM_BULK in {tile, march, generic}, FDM_TILE_TY in
+  // {4, 8}, FDM_TILE_Y in {1, 2, 4} (per-point kernel), FDM_MARCH_H, FDM_XCD_SWIZZLE
   if (const char* e = getenv("FDM_TILE_Y")) {
     int v = atoi(e);
     if (v == 1 || v == 2 || v == 4) c->tile_y = v;
   }
-  if (const char* e = getenv("FDM_BULK")) c->march = strcmp(e, "generic") != 0;
-  if (const char* e = getenv("FDM_MARCH_TX")) {
+  if (const char* e = getenv("FDM_BULK")) {
+    if (!strcmp(e, "generic")) c->bulk_mode = 0;
+    else if (!strcmp(e, "march")) c->bulk_mode = 1;
+    else if (!strcmp(e, "tile")) c->bulk_mode = 2;
+  }
+  if (const char* e = getenv("FDM_TILE_TY")) {
     int v = atoi(e);
-    if (v == 64 || v == 128 || v == 256) c->march_tx = v;
+    if (v == 4 || v == 8) c->tile_ty = v;
   }
   if (const char* e = getenv("FDM_MARCH_H")) {
     int v = atoi(e);
@@ -872,6 +890,8 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
 
   // ---- bulk arrays ----
   size_t pbytes = (size_t)c->ps * c->esz;
+  // the tile kernel addresses a plane with 32-bit byte offsets
+  if (pbytes >= ((size_t)1 << 31) && c->bulk_mode == 2) c->bulk_mode = 0;
   for (int q = 0; q < 2; q++)
     if ((rc = dalloc(c, &c->F[q], 6 * pbytes))) return fail(rc);
   if ((rc = dalloc(c, &c->DF, 6 * pbytes))) return fail(rc);
@@ -890,9 +910,8 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
       if ((rc = dupload(c, &c->met[q], h.data(), h.size() * 4))) return fail(rc);
     }
   }
-  bool any_mms = false;
-  for (int b = 0; b < c->nb; b++) any_mms |= (P->blocks[b].mms != 0);
-  if (any_mms) {
+  for (int b = 0; b < c->nb; b++) c->any_mms |= (P->blocks[b].mms != 0);
+  if (c->any_mms) {
     size_t np = (size_t)c->nx * c->ny;
     if ((rc = dupload(c, &c->X, P->x, np * 8))) return fail(rc);
     if ((rc = dupload(c, &c->Y, P->y, np * 8))) return fail(rc);

@@ -149,29 +149,54 @@ __device__ __forceinline__ T matsel(int mat, T rho, T G, T M) {
 
 // interior point (rates_interior_mode2, rates.f90:407-448), s = 1, 2; the
 // s = 3 term of the reference has a zero coefficient and adds nothing.
-template <typename T, int l>
-__device__ __forceinline__ void interior_field(T* d, const T* __restrict__ F,
-                                               const T* __restrict__ xq, const T* __restrict__ xr,
-                                               const T* __restrict__ yq, const T* __restrict__ yr,
-                                               long long ps, long long cc, int ld, T gamma,
-                                               const T (&cI)[2], T Ji) {
+// Acc supplies the point's neighbourhood: f(k, di, dj) = F(k+1) at (i+di,
+// j+dj) with one of di, dj zero; xr(di), yr(di) along q; xq(dj), yq(dj)
+// along r.  Every kernel evaluates this one expression tree, whatever
+// memory the neighbourhood comes from.
+template <typename T, int l, typename Acc>
+__device__ __forceinline__ void interior_field(T* d, const Acc& A, T gamma, T c1, T c2, T Ji) {
   using P = Fld<l>;
 #pragma unroll
   for (int s = 1; s <= 2; s++) {
-    long long ip = cc + s, im = cc - s, jp = cc + (long long)s * ld, jm = cc - (long long)s * ld;
-    T ayp = (T)0.5 * (yr[ip] + yr[cc]), axp = (T)0.5 * (xr[ip] + xr[cc]);
-    T aym = (T)0.5 * (yr[im] + yr[cc]), axm = (T)0.5 * (xr[im] + xr[cc]);
-    T byp = (T)0.5 * (yq[jp] + yq[cc]), bxp = (T)0.5 * (xq[jp] + xq[cc]);
-    T bym = (T)0.5 * (yq[jm] + yq[cc]), bxm = (T)0.5 * (xq[jm] + xq[cc]);
-    const T* FA = F + P::ua * ps;
-    const T* FB = F + P::ub * ps;
-    T t = prod<T, P::ga>(ayp, FA[ip], gamma) - prod<T, P::gb>(axp, FB[ip], gamma) -
-          prod<T, P::ga>(aym, FA[im], gamma) + prod<T, P::gb>(axm, FB[im], gamma) -
-          prod<T, P::ga>(byp, FA[jp], gamma) + prod<T, P::gb>(bxp, FB[jp], gamma) +
-          prod<T, P::ga>(bym, FA[jm], gamma) - prod<T, P::gb>(bxm, FB[jm], gamma);
-    d[l] = d[l] + t * cI[s - 1] * Ji;
+    T ayp = (T)0.5 * (A.yr(s) + A.yr(0)), axp = (T)0.5 * (A.xr(s) + A.xr(0));
+    T aym = (T)0.5 * (A.yr(-s) + A.yr(0)), axm = (T)0.5 * (A.xr(-s) + A.xr(0));
+    T byp = (T)0.5 * (A.yq(s) + A.yq(0)), bxp = (T)0.5 * (A.xq(s) + A.xq(0));
+    T bym = (T)0.5 * (A.yq(-s) + A.yq(0)), bxm = (T)0.5 * (A.xq(-s) + A.xq(0));
+    T t = prod<T, P::ga>(ayp, A.f(P::ua, s, 0), gamma) - prod<T, P::gb>(axp, A.f(P::ub, s, 0), gamma) -
+          prod<T, P::ga>(aym, A.f(P::ua, -s, 0), gamma) + prod<T, P::gb>(axm, A.f(P::ub, -s, 0), gamma) -
+          prod<T, P::ga>(byp, A.f(P::ua, 0, s), gamma) + prod<T, P::gb>(bxp, A.f(P::ub, 0, s), gamma) +
+          prod<T, P::ga>(bym, A.f(P::ua, 0, -s), gamma) - prod<T, P::gb>(bxm, A.f(P::ub, 0, -s), gamma);
+    d[l] = d[l] + t * (s == 1 ? c1 : c2) * Ji;
   }
 }
+
+template <typename T, typename Acc>
+__device__ __forceinline__ void interior_point(T* d, const Acc& A, const BlockDev& B, T gamma, T Ji) {
+  interior_field<T, 0>(d, A, gamma, (T)B.cI[0][0], (T)B.cI[0][1], Ji);
+  interior_field<T, 1>(d, A, gamma, (T)B.cI[1][0], (T)B.cI[1][1], Ji);
+  interior_field<T, 2>(d, A, gamma, (T)B.cI[2][0], (T)B.cI[2][1], Ji);
+  interior_field<T, 3>(d, A, gamma, (T)B.cI[3][0], (T)B.cI[3][1], Ji);
+  interior_field<T, 4>(d, A, gamma, (T)B.cI[4][0], (T)B.cI[4][1], Ji);
+}
+
+// neighbourhood read straight from global memory
+template <typename T>
+struct GlobalAcc {
+  const T* __restrict__ pF;
+  const T* __restrict__ pxq;
+  const T* __restrict__ pxr;
+  const T* __restrict__ pyq;
+  const T* __restrict__ pyr;
+  long long ps, cc;
+  int ld;
+  __device__ __forceinline__ T f(int k, int di, int dj) const {
+    return pF[k * ps + cc + di + (long long)dj * ld];
+  }
+  __device__ __forceinline__ T xr(int di) const { return pxr[cc + di]; }
+  __device__ __forceinline__ T yr(int di) const { return pyr[cc + di]; }
+  __device__ __forceinline__ T xq(int dj) const { return pxq[cc + (long long)dj * ld]; }
+  __device__ __forceinline__ T yq(int dj) const { return pyq[cc + (long long)dj * ld]; }
+};
 
 template <typename T>
 __device__ __forceinline__ void sat_term(T* d, const T* Fe, const SidesDev& sd, int so) {
@@ -266,7 +291,10 @@ __device__ __forceinline__ void plastic_point(double G, double K, double mu, dou
 // szz overwrite, sources, RK update, plasticity and stores of one point
 // (rates.f90:920-936, source.f90:196-296, fields.f90:1225-1242,
 // plastic.f90:8-59).  Fc: the point's fields at the start of the stage.
-template <typename T>
+// FULL = false leaves out the singular source, the manufactured-solution
+// source and plasticity; the host selects it for launches that need none of
+// them.
+template <typename T, bool FULL>
 __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDev& B, const int bi,
                                              const int i, const int j, const long long cc,
                                              T (&d)[6], const T (&Fc)[6]) {
@@ -274,7 +302,7 @@ __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDe
   // rates_szz (rates.f90:920-936): overwrite
   d[5] = (T)B.nu * (d[2] + d[4]);
   // singular source (source.f90:196-254)
-  for (int p = 0; p < a.nplace; p++) {
+  if constexpr (FULL) for (int p = 0; p < a.nplace; p++) {
     const SourcePlaceDev& P = a.src[p];
     int di = i - P.i0 + 2, dj = j - P.j0 + 2;
     if (P.block == bi && di >= 0 && di <= 4 && dj >= 0 && dj <= 4) {
@@ -287,7 +315,7 @@ __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDe
     }
   }
   // manufactured-solution source (source.f90:256-269)
-  if (B.mms) {
+  if (FULL && B.mms) {
     double x = a.X[(long long)j * a.nx + i], y = a.Y[(long long)j * a.nx + i];
 #pragma unroll 1
     for (int l = 0; l < 6; l++) d[l] = d[l] + (T)mms_sin(x, y, a.t, B.id, 6 + l);
@@ -298,7 +326,7 @@ __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDe
     T f[6];
 #pragma unroll
     for (int l = 0; l < 6; l++) f[l] = Fc[l] + Bdt * d[l];
-    if (a.do_plastic && B.plastic) {
+    if (FULL && a.do_plastic && B.plastic) {
       // update_fields_plastic (plastic.f90:8-59), full dt, after the last stage
       double s[4], Ds[4];
 #pragma unroll
@@ -324,29 +352,20 @@ __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDe
   for (int l = 0; l < 6; l++) a.DF[l * ps + cc] = d[l];
 }
 
-// Everything that happens at one grid point in one stage, reading its
-// neighbours directly from global memory: used for points next to block
-// edges (one-sided closures in r) and as the reference path.
+// SAT penalties and one-sided stencils of a point within four points of a
+// block edge (time_step.f90:754-810, rates.f90:451-918): L/R strips
+// one-sided in q and central in r, B/T strips the converse, corners
+// one-sided in both, interleaved per s as in rates_boundary_mode2.  The
+// stencil loops are kept rolled: the closures are a thin band, and unrolled
+// they would set the register budget of every kernel that inlines them.
 template <typename T>
-__device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i, const int j) {
-  const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
-  const BlockDev& B = a.blocks[bi];
+__device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockDev& B, const int i,
+                                              const int j, const long long cc, T (&d)[6]) {
   const long long ps = a.ps;
   const int ld = a.ld;
-  const long long cc = (long long)j * ld + i;
   const T* __restrict__ F = a.Fin;
-  T d[6];
-  if (a.a_zero) {
-#pragma unroll
-    for (int l = 0; l < 6; l++) d[l] = (T)0.0;
-  } else {
-    const T A = (T)a.A;
-#pragma unroll
-    for (int l = 0; l < 5; l++) d[l] = A * a.DF[l * ps + cc];
-    d[5] = (T)0.0;  // DF6 is overwritten by the szz relation below
-  }
   const int dl = i - B.mx, dr = B.px - i, db = j - B.my, dtp = B.py - j;
-  // SAT penalties (time_step.f90:754-810), sides in the order L, R, B, T
+  // SAT penalties, sides in the order L, R, B, T
   if (dl == 0 || dr == 0 || db == 0 || dtp == 0) {
     T Fe[6];
 #pragma unroll
@@ -366,39 +385,9 @@ __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i,
   const bool rone = (db < 4) || (dtp < 4);
   const long long row = (long long)j * ld;
   const long long colm = (long long)B.my * ld + i, colp = (long long)B.py * ld + i;
-  if (!qone && !rone) {
-    T c0[2] = {(T)B.cI[0][0], (T)B.cI[0][1]};
-    T c1[2] = {(T)B.cI[1][0], (T)B.cI[1][1]};
-    T c2[2] = {(T)B.cI[2][0], (T)B.cI[2][1]};
-    T c3[2] = {(T)B.cI[3][0], (T)B.cI[3][1]};
-    T c4[2] = {(T)B.cI[4][0], (T)B.cI[4][1]};
-    interior_field<T, 0>(d, F, xq, xr, yq, yr, ps, cc, ld, gamma, c0, Ji);
-    interior_field<T, 1>(d, F, xq, xr, yq, yr, ps, cc, ld, gamma, c1, Ji);
-    interior_field<T, 2>(d, F, xq, xr, yq, yr, ps, cc, ld, gamma, c2, Ji);
-    interior_field<T, 3>(d, F, xq, xr, yq, yr, ps, cc, ld, gamma, c3, Ji);
-    interior_field<T, 4>(d, F, xq, xr, yq, yr, ps, cc, ld, gamma, c4, Ji);
-  } else if (qone && rone) {
-    // corners LB, RB, LT, RT: one-sided in q and r, interleaved per s
-#pragma unroll
-    for (int s = 0; s < 6; s++) {
-      long long kq = (dl < 4) ? row + B.mx + s : row + B.px - s;
-      T Dq = (dl < 4) ? (T)kDL[dl][s] : (T)(-kDL[dr][s]);
-      long long kr = (db < 4) ? colm + (long long)s * ld : colp - (long long)s * ld;
-      T Dr = (db < 4) ? (T)kDL[db][s] : (T)(-kDL[dtp][s]);
-      d[0] = d[0] + bscale<T, 0>(q1<T, 0>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
-      d[1] = d[1] + bscale<T, 0>(q1<T, 1>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
-      d[2] = d[2] + bscale<T, 2>(q1<T, 2>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
-      d[3] = d[3] + bscale<T, 1>(q1<T, 3>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
-      d[4] = d[4] + bscale<T, 2>(q1<T, 4>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
-      d[0] = d[0] + bscale<T, 0>(r1<T, 0>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
-      d[1] = d[1] + bscale<T, 0>(r1<T, 1>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
-      d[2] = d[2] + bscale<T, 2>(r1<T, 2>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
-      d[3] = d[3] + bscale<T, 1>(r1<T, 3>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
-      d[4] = d[4] + bscale<T, 2>(r1<T, 4>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
-    }
-  } else if (qone) {
-    // L or R strip: one-sided q, then central r
-#pragma unroll
+  if (qone) {
+    // one-sided q: L or R strip, or q part of a corner
+#pragma unroll 1
     for (int s = 0; s < 6; s++) {
       long long kq = (dl < 4) ? row + B.mx + s : row + B.px - s;
       T Dq = (dl < 4) ? (T)kDL[dl][s] : (T)(-kDL[dr][s]);
@@ -407,20 +396,33 @@ __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i,
       d[2] = d[2] + bscale<T, 2>(q1<T, 2>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
       d[3] = d[3] + bscale<T, 1>(q1<T, 3>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
       d[4] = d[4] + bscale<T, 2>(q1<T, 4>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
+      if (rone) {
+        // corner: the r term of the same s follows
+        long long kr = (db < 4) ? colm + (long long)s * ld : colp - (long long)s * ld;
+        T Dr = (db < 4) ? (T)kDL[db][s] : (T)(-kDL[dtp][s]);
+        d[0] = d[0] + bscale<T, 0>(r1<T, 0>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
+        d[1] = d[1] + bscale<T, 0>(r1<T, 1>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
+        d[2] = d[2] + bscale<T, 2>(r1<T, 2>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
+        d[3] = d[3] + bscale<T, 1>(r1<T, 3>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
+        d[4] = d[4] + bscale<T, 2>(r1<T, 4>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
+      }
     }
-#pragma unroll
-    for (int s = 1; s <= 2; s++) {
-      T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
-      long long kp = cc + (long long)s * ld, km = cc - (long long)s * ld;
-      d[0] = d[0] + bscale<T, 0>(rc<T, 0>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
-      d[1] = d[1] + bscale<T, 0>(rc<T, 1>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
-      d[2] = d[2] + bscale<T, 2>(rc<T, 2>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
-      d[3] = d[3] + bscale<T, 1>(rc<T, 3>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
-      d[4] = d[4] + bscale<T, 2>(rc<T, 4>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
+    if (!rone) {
+      // L or R strip: central r
+#pragma unroll 1
+      for (int s = 1; s <= 2; s++) {
+        T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
+        long long kp = cc + (long long)s * ld, km = cc - (long long)s * ld;
+        d[0] = d[0] + bscale<T, 0>(rc<T, 0>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
+        d[1] = d[1] + bscale<T, 0>(rc<T, 1>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
+        d[2] = d[2] + bscale<T, 2>(rc<T, 2>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
+        d[3] = d[3] + bscale<T, 1>(rc<T, 3>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
+        d[4] = d[4] + bscale<T, 2>(rc<T, 4>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
+      }
     }
   } else {
     // B or T strip: one-sided r, then central q
-#pragma unroll
+#pragma unroll 1
     for (int s = 0; s < 6; s++) {
       long long kr = (db < 4) ? colm + (long long)s * ld : colp - (long long)s * ld;
       T Dr = (db < 4) ? (T)kDL[db][s] : (T)(-kDL[dtp][s]);
@@ -430,7 +432,7 @@ __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i,
       d[3] = d[3] + bscale<T, 1>(r1<T, 3>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
       d[4] = d[4] + bscale<T, 2>(r1<T, 4>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
     }
-#pragma unroll
+#pragma unroll 1
     for (int s = 1; s <= 2; s++) {
       T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
       long long kp = cc + s, km = cc - s;
@@ -441,12 +443,62 @@ __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i,
       d[4] = d[4] + bscale<T, 2>(qc<T, 4>(F, yr, xr, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
     }
   }
-  T Fc[6];
-#pragma unroll
-  for (int l = 0; l < 6; l++) Fc[l] = F[l * ps + cc];
-  finish_point<T>(a, B, bi, i, j, cc, d, Fc);
 }
 
+// d = A * DF at the start of a stage (scale_rates, time_step.f90:138-141)
+template <typename T>
+__device__ __forceinline__ void scaled_rates(const BulkArgs<T>& a, const long long cc, T (&d)[6]) {
+  if (a.a_zero) {
+#pragma unroll
+    for (int l = 0; l < 6; l++) d[l] = (T)0.0;
+  } else {
+    const T A = (T)a.A;
+#pragma unroll
+    for (int l = 0; l < 5; l++) d[l] = A * a.DF[l * a.ps + cc];
+    d[5] = (T)0.0;  // DF6 is overwritten by the szz relation
+  }
+}
+
+// Everything that happens at one grid point in one stage, reading its
+// neighbours straight from global memory: the per-point kernel, and the
+// closure points of the tile and marching kernels.
+template <typename T, bool FULL>
+__device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i, const int j) {
+  const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
+  const BlockDev& B = a.blocks[bi];
+  const long long ps = a.ps;
+  const int ld = a.ld;
+  const long long cc = (long long)j * ld + i;
+  T d[6];
+  scaled_rates<T>(a, cc, d);
+  if (i - B.mx >= 4 && B.px - i >= 4 && j - B.my >= 4 && B.py - j >= 4) {
+    const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld};
+    interior_point<T>(d, acc, B, (T)B.gamma, a.Ji[cc]);
+  } else {
+    closure_rates<T>(a, B, i, j, cc, d);
+  }
+  T Fc[6];
+#pragma unroll
+  for (int l = 0; l < 6; l++) Fc[l] = a.Fin[l * ps + cc];
+  finish_point<T, FULL>(a, B, bi, i, j, cc, d, Fc);
+}
+
+// closure point (within four points of its block's edge)
+template <typename T, bool FULL>
+__device__ __forceinline__ void point_closure(const BulkArgs<T>& a, const BlockDev& B, const int bi,
+                                              const int i, const int j) {
+  const long long ps = a.ps;
+  const long long cc = (long long)j * a.ld + i;
+  T d[6];
+  scaled_rates<T>(a, cc, d);
+  closure_rates<T>(a, B, i, j, cc, d);
+  T Fc[6];
+#pragma unroll
+  for (int l = 0; l < 6; l++) Fc[l] = a.Fin[l * ps + cc];
+  finish_point<T, FULL>(a, B, bi, i, j, cc, d, Fc);
+}
+
+// per-point kernel: one lane per grid point, 64 x blockDim.y tiles
 template <typename T>
 __global__ void __launch_bounds__(256) bulk_kernel(BulkArgs<T> a) {
   int tx, ty;
@@ -454,9 +506,170 @@ __global__ void __launch_bounds__(256) bulk_kernel(BulkArgs<T> a) {
   const int i = tx * blockDim.x + threadIdx.x;
   const int j = ty * blockDim.y + threadIdx.y;
   if (i >= a.nx || j >= a.ny) return;
-  point_generic<T>(a, i, j);
+  point_generic<T, true>(a, i, j);
 }
 
+// neighbourhood of the marching kernel: q neighbours from the LDS row S
+// (F1..F5, xr, yr), r neighbours from the register window w (F1..F5, xq,
+// yq at rows j-2..j+2)
+template <typename T, int W>
+struct MarchAcc {
+  const T (*S)[W];
+  const T (&w)[7][5];
+  int c;
+  __device__ __forceinline__ T f(int k, int di, int dj) const {
+    return dj == 0 ? S[k][c + di] : w[k][2 + dj];
+  }
+  __device__ __forceinline__ T xr(int di) const { return S[5][c + di]; }
+  __device__ __forceinline__ T yr(int di) const { return S[6][c + di]; }
+  __device__ __forceinline__ T xq(int dj) const { return w[5][2 + dj]; }
+  __device__ __forceinline__ T yq(int dj) const { return w[6][2 + dj]; }
+};
+
+// global load at a 32-bit byte offset from a wave-uniform base, so that the
+// address is formed by the load itself (scalar base + vector offset)
+template <typename T>
+__device__ __forceinline__ T ld_off(const T* base, unsigned boff) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+}
+
+// neighbourhood of the tile kernel: everything from LDS.  sF: F1..F5 with a
+// halo of 2 in q and r; sR: xr, yr with a halo in q; sQ: xq, yq with a halo
+// in r.  (x, y): the lane's point in tile coordinates.
+template <typename T, int TX, int TY>
+struct TileAcc {
+  const T (*sF)[TY + 4][TX + 4];
+  const T (*sR)[TY][TX + 4];
+  const T (*sQ)[TY + 4][TX];
+  int x, y;
+  __device__ __forceinline__ T f(int k, int di, int dj) const { return sF[k][y + 2 + dj][x + 2 + di]; }
+  __device__ __forceinline__ T xr(int di) const { return sR[0][y][x + 2 + di]; }
+  __device__ __forceinline__ T yr(int di) const { return sR[1][y][x + 2 + di]; }
+  __device__ __forceinline__ T xq(int dj) const { return sQ[0][y + 2 + dj][x]; }
+  __device__ __forceinline__ T yq(int dj) const { return sQ[1][y + 2 + dj][x]; }
+};
+
+// Tile bulk kernel.  A workgroup of 64 x TY lanes computes a 64 x TY tile.
+// All of the tile's loads are issued before one barrier: F1..F5 with a
+// halo of two points in q and r, xr/yr with the q halo and xq/yq with the r
+// halo go to LDS; the lane's own DF, F6 and Ji stay in registers.  Lanes
+// within four points of a block edge (one-sided closures, SAT) take
+// point_closure.  Halo indices are clamped into the grid; a clamped value
+// only ever reaches closure lanes, which do not read LDS.
+template <typename T, int TY, bool FULL>
+__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4)))
+bulk_tile(BulkArgs<T> a) {
+  constexpr int TX = 64, HX = TX + 4, HY = TY + 4, NT = TX * TY;
+  constexpr int NF = (HY * HX + NT - 1) / NT;  // staging passes, F1..F5
+  constexpr int NR = (TY * HX + NT - 1) / NT;  // xr, yr
+  constexpr int NQ = (HY * TX + NT - 1) / NT;  // xq, yq
+  __shared__ T sF[5][HY][HX];
+  __shared__ T sR[2][TY][HX];
+  __shared__ T sQ[2][HY][TX];
+  int tbx, tby;
+  tile_of(a.xcd_swizzle, tbx, tby);
+  const int i0 = tbx * TX, j0 = tby * TY;
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  const int ld = a.ld, nx1 = a.nx - 1, ny1 = a.ny - 1;
+  const long long ps = a.ps;
+  const T* __restrict__ F = a.Fin;
+  T vF[NF][5], vR[NR][2], vQ[NQ][2];
+#pragma unroll
+  for (int p = 0; p < NF; p++) {
+    const int k = tid + p * NT;
+    if (k < HY * HX) {
+      const int r = k / HX, c = k - r * HX;
+      const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(max(i0 - 2 + c, 0), nx1);
+      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+#pragma unroll
+      for (int l = 0; l < 5; l++) vF[p][l] = ld_off(F + l * ps, o);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NR; p++) {
+    const int k = tid + p * NT;
+    if (k < TY * HX) {
+      const int r = k / HX, c = k - r * HX;
+      const int gj = min(j0 + r, ny1), gi = min(max(i0 - 2 + c, 0), nx1);
+      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+      vR[p][0] = ld_off(a.xr, o);
+      vR[p][1] = ld_off(a.yr, o);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NQ; p++) {
+    const int k = tid + p * NT;
+    if (k < HY * TX) {
+      const int r = k / TX, c = k - r * TX;
+      const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(i0 + c, nx1);
+      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+      vQ[p][0] = ld_off(a.xq, o);
+      vQ[p][1] = ld_off(a.yq, o);
+    }
+  }
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const bool valid = i <= nx1 && j <= ny1;
+  const unsigned oc = (unsigned)(min(j, ny1) * ld + min(i, nx1)) * (unsigned)sizeof(T);
+  T dfa[5];
+  if (!a.a_zero) {
+#pragma unroll
+    for (int l = 0; l < 5; l++) dfa[l] = ld_off(a.DF + l * ps, oc);
+  }
+  const T f6 = ld_off(F + 5 * ps, oc);
+  const T Ji = ld_off(a.Ji, oc);
+#pragma unroll
+  for (int p = 0; p < NF; p++) {
+    const int k = tid + p * NT;
+    if (k < HY * HX) {
+      const int r = k / HX, c = k - r * HX;
+#pragma unroll
+      for (int l = 0; l < 5; l++) sF[l][r][c] = vF[p][l];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NR; p++) {
+    const int k = tid + p * NT;
+    if (k < TY * HX) {
+      const int r = k / HX, c = k - r * HX;
+      sR[0][r][c] = vR[p][0];
+      sR[1][r][c] = vR[p][1];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NQ; p++) {
+    const int k = tid + p * NT;
+    if (k < HY * TX) {
+      const int r = k / TX, c = k - r * TX;
+      sQ[0][r][c] = vQ[p][0];
+      sQ[1][r][c] = vQ[p][1];
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
+  const BlockDev& B = a.blocks[bi];
+  if (i - B.mx < 4 || B.px - i < 4 || j - B.my < 4 || B.py - j < 4) {
+    point_closure<T, FULL>(a, B, bi, i, j);
+    return;
+  }
+  T d[6];
+  if (a.a_zero) {
+#pragma unroll
+    for (int l = 0; l < 6; l++) d[l] = (T)0.0;
+  } else {
+    const T A = (T)a.A;
+#pragma unroll
+    for (int l = 0; l < 5; l++) d[l] = A * dfa[l];
+    d[5] = (T)0.0;  // DF6 is overwritten by the szz relation
+  }
+  const TileAcc<T, TX, TY> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y};
+  interior_point<T>(d, acc, B, (T)B.gamma, Ji);
+  T Fc[6];
+#pragma unroll
+  for (int k = 0; k < 5; k++) Fc[k] = acc.f(k, 0, 0);
+  Fc[5] = f6;
+  finish_point<T, FULL>(a, B, bi, i, j, (long long)j * ld + i, d, Fc);
+}
 
 // Row-marching bulk kernel.  A workgroup owns a strip of TX columns and
 // sweeps a segment of rows.  The arrays differentiated in r (F1..F5, xq,
@@ -478,7 +691,7 @@ __global__ void __launch_bounds__(TX) bulk_march(BulkArgs<T> a, const int4* __re
   const bool valid = i < a.nx;
   if (sg.w == 1) {
     if (!valid) return;
-    for (int j = sg.x; j <= sg.y; j++) point_generic<T>(a, i, j);
+    for (int j = sg.x; j <= sg.y; j++) point_generic<T, true>(a, i, j);
     return;
   }
   const long long ps = a.ps;
@@ -500,12 +713,6 @@ __global__ void __launch_bounds__(TX) bulk_march(BulkArgs<T> a, const int4* __re
   }
   const int c = HALO + t;
   const T gamma = (T)B.gamma, rho = (T)B.rho, G = (T)B.G, M = (T)B.M;
-  T cI[5][2];
-#pragma unroll
-  for (int l = 0; l < 5; l++) {
-    cI[l][0] = (T)B.cI[l][0];
-    cI[l][1] = (T)B.cI[l][1];
-  }
   // window w[k][r]: array k (F1..F5, xq, yq) at row j-2+r
   T w[7][5];
 #pragma unroll
@@ -560,32 +767,8 @@ __global__ void __launch_bounds__(TX) bulk_march(BulkArgs<T> a, const int4* __re
       if (dr == 0) sat_term<T>(d, Fc, a.sd, B.side_off[FDM_SIDE_R] + (j - B.my));
       const T Ji = a.Ji[cc];
       if (dl >= 4 && dr >= 4) {
-        // rates_interior_mode2 (rates.f90:407-448)
-#define FDM_INTERIOR_FIELD(L)                                                                  \
-  {                                                                                            \
-    using P = Fld<L>;                                                                          \
-    _Pragma("unroll") for (int s = 1; s <= 2; s++) {                                           \
-      T ayp = (T)0.5 * (S[6][c + s] + S[6][c]), axp = (T)0.5 * (S[5][c + s] + S[5][c]);        \
-      T aym = (T)0.5 * (S[6][c - s] + S[6][c]), axm = (T)0.5 * (S[5][c - s] + S[5][c]);        \
-      T byp = (T)0.5 * (w[6][2 + s] + w[6][2]), bxp = (T)0.5 * (w[5][2 + s] + w[5][2]);        \
-      T bym = (T)0.5 * (w[6][2 - s] + w[6][2]), bxm = (T)0.5 * (w[5][2 - s] + w[5][2]);        \
-      T tt = prod<T, P::ga>(ayp, S[P::ua][c + s], gamma) -                                     \
-             prod<T, P::gb>(axp, S[P::ub][c + s], gamma) -                                     \
-             prod<T, P::ga>(aym, S[P::ua][c - s], gamma) +                                     \
-             prod<T, P::gb>(axm, S[P::ub][c - s], gamma) -                                     \
-             prod<T, P::ga>(byp, w[P::ua][2 + s], gamma) +                                     \
-             prod<T, P::gb>(bxp, w[P::ub][2 + s], gamma) +                                     \
-             prod<T, P::ga>(bym, w[P::ua][2 - s], gamma) -                                     \
-             prod<T, P::gb>(bxm, w[P::ub][2 - s], gamma);                                      \
-      d[L] = d[L] + tt * cI[L][s - 1] * Ji;                                                    \
-    }                                                                                          \
-  }
-        FDM_INTERIOR_FIELD(0)
-        FDM_INTERIOR_FIELD(1)
-        FDM_INTERIOR_FIELD(2)
-        FDM_INTERIOR_FIELD(3)
-        FDM_INTERIOR_FIELD(4)
-#undef FDM_INTERIOR_FIELD
+        const MarchAcc<T, W> acc{S, w, c};
+        interior_point<T>(d, acc, B, gamma, Ji);
       } else {
         // L or R strip (rates.f90:471-591): one-sided q, then central r
 #pragma unroll
@@ -627,7 +810,7 @@ __global__ void __launch_bounds__(TX) bulk_march(BulkArgs<T> a, const int4* __re
 #undef FDM_RC
         }
       }
-      finish_point<T>(a, B, bi, i, j, cc, d, Fc);
+      finish_point<T, true>(a, B, bi, i, j, cc, d, Fc);
     }
 #pragma unroll
     for (int k = 0; k < 7; k++) {

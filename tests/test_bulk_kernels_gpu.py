@@ -1,0 +1,73 @@
+"""The three bulk kernels (LDS tile, row marching, per point) evaluate the
+same expression tree per point, so on the same inputs they must give
+bit-identical fields, rates and fault state, in FP64 and in FP32-bulk.
+Grids here are large enough for interior tiles, ragged tile edges in both
+directions, block boundaries inside a tile and several block columns.
+"""
+import numpy as np
+import pytest
+
+import problems
+from fdmap_github_amd.device import Context
+
+pytestmark = pytest.mark.gpu
+
+CS, CP, H = 3.464, 6.0, 0.025
+DT = 0.5 * H / np.sqrt(CS ** 2 + CP ** 2)
+BG = [0.0, 0.0, -120.0, 70.0, -120.0, -120.0]
+MODES = [("generic", {}), ("tile", {"FDM_TILE_TY": "8"}), ("tile", {"FDM_TILE_TY": "4"}),
+         ("tile", {"FDM_TILE_TY": "8", "FDM_XCD_SWIZZLE": "1"}), ("march", {"FDM_MARCH_H": "5"})]
+
+
+def run_mode(monkeypatch, prob, F, nsteps, mode, env):
+    monkeypatch.setenv("FDM_BULK", mode)
+    for k in ("FDM_TILE_TY", "FDM_XCD_SWIZZLE", "FDM_MARCH_H"):
+        monkeypatch.delenv(k, raising=False)
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    dev = Context(prob)
+    dev.set_fields(F)
+    dev.init_edges(0.0)
+    dev.step(0.0, DT, 1, nsteps)
+    Fd, DFd = dev.get_fields()
+    faults = [{f: dev.get_iface(k, f) for f in ("D", "V", "S", "N", "Psi")}
+              for k in range(len(prob.ifaces))]
+    pl = dev.get_plastic() if prob.blocks[0].get("plastic") else None
+    dev.close()
+    return Fd, DFd, faults, pl
+
+
+CASES = {
+    # two block rows coupled by rate-and-state friction; ragged in x and y
+    "friction": dict(nx=203, nys=(61, 77), seed=23, warp=0.04, background=BG, perturb=1e-3,
+                     law="RFL"),
+    # 2 x 2 locked blocks: block columns inside a tile row
+    "locked2x2": dict(nx=90, nys=(45, 52), nbx=2, seed=29, warp=0.05, background=BG,
+                      perturb=1e-2, coupling="locked"),
+    # plasticity and a point source: the full finishing path
+    "plastic_source": dict(nx=130, nys=(40, 44), seed=31, warp=0.02,
+                           background=[0.0, 0.0, -60.0, 44.0, -100.0, -80.0], perturb=0.0,
+                           law="RFL", source=True, fault_overstress=20.0,
+                           plastic=dict(mu=0.6, beta=0.0, b=0.0, h=0.0, eta=0.0, s0=[0.0] * 6)),
+}
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_bulk_kernels_identical(gpu, monkeypatch, case, precision):
+    prob, F = problems.build(precision=precision, **CASES[case])
+    ref = None
+    for mode, env in MODES:
+        print("case", case, precision, "mode", mode, env, flush=True)
+        out = run_mode(monkeypatch, prob, F, 12, mode, env)
+        if ref is None:
+            ref = out
+            assert np.isfinite(out[0]).all()
+            continue
+        assert np.array_equal(out[0], ref[0]), (mode, env, np.abs(out[0] - ref[0]).max())
+        assert np.array_equal(out[1], ref[1]), (mode, env, np.abs(out[1] - ref[1]).max())
+        for a, b in zip(out[2], ref[2]):
+            for f in a:
+                assert np.array_equal(a[f], b[f]), (mode, env, f)
+        if ref[3] is not None:
+            assert np.array_equal(out[3][0], ref[3][0]) and np.array_equal(out[3][1], ref[3][1])
