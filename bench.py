@@ -95,17 +95,6 @@ def measured_traffic():
     return None
 
 
-def copy_check(mib):
-    import torch
-    n = mib * (1 << 20) // 8
-    a = torch.ones(n, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(f"copy check: 3 x {mib} MiB read + {mib} MiB written", file=sys.stderr)
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,9 +105,6 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--copy-check", type=int, default=0, metavar="MIB",
-                    help="after the run, copy MIB MiB device to device three times (a plain "
-                         "copy kernel of known size, to calibrate hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -198,8 +184,6 @@ def main():
         out["cpu_baseline"] = cpu_baseline(inpath, args.cpu_steps)
         out["cpu_baseline"]["gpu_over_cpu"] = ups / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
-    if args.copy_check:
-        copy_check(args.copy_check)
 
 
 if __name__ == "__main__":

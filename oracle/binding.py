@@ -47,7 +47,7 @@ def load():
     lib.orc_diff.restype = None
     lib.orc_rk_coeffs.argtypes = [C.c_int32, _dp, _dp, _dp]
     lib.orc_rk_coeffs.restype = C.c_int32
-    lib.orc_solve_friction_node.argtypes = [C.c_int32, _dp, _dp] + [C.c_double] * 10 + [_dp] * 4
+    lib.orc_solve_friction_node.argtypes = [C.c_int32, _dp, _dp] + [C.c_double] * 9 + [_dp] * 4
     lib.orc_solve_friction_node.restype = C.c_int
     lib.orc_state_rate.argtypes = [C.c_int32, _dp, C.c_double, C.c_double]
     lib.orc_state_rate.restype = C.c_double

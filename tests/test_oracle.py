@@ -1,0 +1,619 @@
+"""Pins of the CPU oracle against mathematics that fixes each part of the
+path, independently of the oracle's own code.
+
+The reference holds no fixtures or known-answer tests for this path, and
+its Fortran cannot be built here without stand-ins for the MPI module, so
+every check below comes from a closed form, an identity or a published
+formula, chosen so that a dropped term, a wrong sign or index, or a
+transposed table fails at least one of them:
+
+- SBP4 tables (fd_coeff.f90:97-126): the published diagonal-norm operator,
+  consistency, polynomial exactness, the SBP identity H D + (H D)^T = B,
+  and the 1-D application (fd.f90:42-80) against the dense matrix;
+- LSRK coefficients (time_step.f90:25-66): the published Carpenter-Kennedy
+  and Williamson rationals, the stability polynomial and the stage times;
+- friction (friction.f90:593-1264): residual and closed-form inverse of
+  every law, the tensile, locked, frictionless and slip-weakening branches,
+  and the state-evolution laws' steady states and slopes;
+- plasticity (plastic.f90:62-192): return onto the Drucker-Prager surface;
+- the manufactured solution (mms.f90:52-234): its source terms are
+  u_t - L(u) of its own fields;
+- the whole stage (rates.f90:384-936): stage-1 rates against a dense
+  split-form operator built here from the SBP matrices, energy decay under
+  absorbing boundaries, and the convergence rate of the MMS problem.
+"""
+import math
+from fractions import Fraction as Fr
+
+import numpy as np
+import pytest
+
+import problems
+from oracle import binding as ob
+from oracle.binding import OracleContext
+
+# ---------------------------------------------------------------------------
+# SBP4 tables
+# ---------------------------------------------------------------------------
+
+# diagonal-norm fourth-order SBP first derivative (Strand 1994; Mattsson and
+# Nordstrom 2004), closure rows 0..3 and norm weights
+DL_EXACT = [[Fr(-24, 17), Fr(59, 34), Fr(-4, 17), Fr(-3, 34), 0, 0],
+            [Fr(-1, 2), 0, Fr(1, 2), 0, 0, 0],
+            [Fr(4, 43), Fr(-59, 86), 0, Fr(59, 86), Fr(-4, 43), 0],
+            [Fr(3, 98), 0, Fr(-59, 98), 0, Fr(32, 49), Fr(-4, 49)]]
+HL_EXACT = [Fr(17, 48), Fr(59, 48), Fr(43, 48), Fr(49, 48)]
+DI_EXACT = [Fr(1, 12), Fr(-2, 3), 0, Fr(2, 3), Fr(-1, 12)]
+
+
+def dense_D(n):
+    """Dense SBP4 derivative matrix (unit spacing) from the exact tables;
+    the right closure is the antisymmetric mirror of the left one."""
+    D = np.zeros((n, n))
+    dl = np.array([[float(v) for v in row] for row in DL_EXACT])
+    for i in range(4):
+        D[i, :6] = dl[i]
+        D[n - 1 - i, n - 6:] = -dl[i][::-1]
+    for i in range(4, n - 4):
+        D[i, i - 2:i + 3] = [float(v) for v in DI_EXACT]
+    return D
+
+
+def test_sbp4_tables_match_published_operator():
+    DI, DL, HL, H00i = ob.sbp4_tables()
+    assert np.allclose(DI, [float(v) for v in DI_EXACT], rtol=0, atol=5e-15)
+    assert np.allclose(DL, [[float(v) for v in r] for r in DL_EXACT], rtol=0, atol=5e-15)
+    assert np.allclose(HL, [float(v) for v in HL_EXACT], rtol=0, atol=5e-15)
+    assert abs(H00i - 48.0 / 17.0) < 5e-15
+
+
+def test_sbp4_consistency_and_exactness():
+    DI, DL, HL, _ = ob.sbp4_tables()
+    k = np.arange(-2, 3)
+    for p in range(5):  # interior: exact through degree 4
+        assert abs(np.dot(DI, k.astype(float) ** p) - (1.0 if p == 1 else 0.0)) < 1e-14
+    for r in range(4):  # closures: exact through degree 2 at row r
+        kk = np.arange(6) - r
+        for p in range(3):
+            assert abs(np.dot(DL[r], kk.astype(float) ** p) - (1.0 if p == 1 else 0.0)) < 1e-14
+
+
+def test_sbp_identity_with_tables():
+    DI, DL, HL, _ = ob.sbp4_tables()
+    n = 17
+    D = np.zeros((n, n))
+    for i in range(4):
+        D[i, :6] = DL[i]
+        D[n - 1 - i, n - 6:] = -DL[i][::-1]
+    for i in range(4, n - 4):
+        D[i, i - 2:i + 3] = DI
+    h = np.ones(n)
+    h[:4] = HL
+    h[n - 4:] = HL[::-1]
+    Q = np.diag(h) @ D
+    B = np.zeros((n, n))
+    B[0, 0], B[-1, -1] = -1.0, 1.0
+    assert np.abs(Q + Q.T - B).max() < 2e-14
+
+
+@pytest.mark.parametrize("n", [9, 12, 31])
+def test_diff_matches_dense_operator(n):
+    rng = np.random.default_rng(n)
+    f = rng.standard_normal(n)
+    assert np.allclose(ob.diff(f), dense_D(n) @ f, rtol=0, atol=1e-13)
+    x = np.arange(n, dtype=float)
+    assert np.allclose(ob.diff(3.0 * x ** 2 - x + 2.0), 6.0 * x - 1.0, rtol=0, atol=1e-11)
+
+
+# ---------------------------------------------------------------------------
+# LSRK coefficients
+# ---------------------------------------------------------------------------
+
+LSRK = {
+    # Carpenter and Kennedy (1994), five-stage fourth order, solution 3
+    4: ([0, Fr(-567301805773, 1357537059087), Fr(-2404267990393, 2016746695238),
+         Fr(-3550918686646, 2091501179385), Fr(-1275806237668, 842570457699)],
+        [Fr(1432997174477, 9575080441755), Fr(5161836677717, 13612068292357),
+         Fr(1720146321549, 2090206949498), Fr(3134564353537, 4481467310338),
+         Fr(2277821191437, 14882151754819)]),
+    # Williamson (1980) three-stage third order
+    3: ([0, Fr(-5, 9), Fr(-153, 128)], [Fr(1, 3), Fr(15, 16), Fr(8, 15)]),
+}
+
+
+def lsrk_apply(A, B, Cc, f, y0, t0, h):
+    y, dy = y0, 0.0
+    for s in range(len(A)):
+        dy = A[s] * dy + h * f(t0 + Cc[s] * h, y)
+        y = y + B[s] * dy
+    return y
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4])
+def test_lsrk_stability_polynomial_and_stage_times(order):
+    A, B, Cc = ob.rk_coeffs(order)
+    if order in LSRK:
+        Ae, Be = LSRK[order]
+        assert np.allclose(A, [float(v) for v in Ae], rtol=1e-15, atol=0)
+        assert np.allclose(B, [float(v) for v in Be], rtol=1e-15, atol=0)
+    # stability polynomial R(z) = sum c_k z^k: c_k = 1/k! for k <= order
+    # (read off by applying the scheme to y' = z y at several z)
+    zs = np.linspace(-0.5, 0.5, len(A) + 1)
+    R = [lsrk_apply(A, B, Cc, lambda t, y, z=z: z * y, 1.0, 0.0, 1.0) for z in zs]
+    coef = np.polyfit(zs, R, len(A))[::-1]
+    for k in range(order + 1):
+        assert abs(coef[k] - 1.0 / math.factorial(k)) < 1e-9, (k, coef[k])
+    # stage times: y' = t^p integrated exactly for p < order
+    for p in range(order):
+        y1 = lsrk_apply(A, B, Cc, lambda t, y: t ** p, 0.0, 0.3, 0.7)
+        assert abs(y1 - (1.0 ** (p + 1) - 0.3 ** (p + 1)) / (p + 1)) < 1e-13, p
+    assert Cc[len(A)] == 1.0
+
+
+# ---------------------------------------------------------------------------
+# friction
+# ---------------------------------------------------------------------------
+
+RS = np.array([0.010, 0.014, 1e-6, 0.6, 0.4, 0.1, 0.1])  # a, b, V0, f0, L, fw, Vw
+ETA = 0.5 * 2.67 * 3.464  # half the shear impedance
+
+
+@pytest.mark.parametrize("law", ["RSL", "RFL", "RAL", "RSF"])
+@pytest.mark.parametrize("Slock,N,Psi", [(70.0, 120.0, 0.70), (95.0, 120.0, 0.45),
+                                          (-40.0, 60.0, 0.6), (72.0, 120.0, 0.8)])
+def test_regularized_laws_residual_and_inverse(law, Slock, N, Psi):
+    a, V0 = RS[0], RS[2]
+    rc, V, S, Nout, O = ob.solve_friction_node(law, RS, Slock, N, 0.0, 0.0, ETA, 0.0, Psi, 1e-9)
+    assert rc == 0 and Nout == N and O == 0.0
+    assert abs(S + ETA * V - Slock) <= 1e-12 * abs(Slock)
+    # the strength a N asinh(V e^(Psi/a) / (2 V0)) equals S
+    assert abs(a * N * np.arcsinh(V / (2 * V0) * np.exp(Psi / a)) - S) <= 1e4 * 2.3e-16 * 10
+    # closed-form inverse
+    Vinv = 2 * V0 * np.exp(-Psi / a) * np.sinh(S / (a * N))
+    assert abs(Vinv - V) <= 1e-8 * abs(V) + 1e-300
+
+
+@pytest.mark.parametrize("law", ["SL", "FL"])
+def test_unregularized_laws_residual(law):
+    a, V0 = RS[0], RS[2]
+    for Slock, N, Psi in [(70.0, 120.0, 0.62), (80.0, 110.0, 0.55)]:
+        rc, V, S, _, _ = ob.solve_friction_node(law, RS, Slock, N, 0.0, 0.0, ETA, 0.0, Psi, 1e-9)
+        assert rc == 0 and V > 0
+        assert abs(S + ETA * V - Slock) <= 1e-12 * Slock
+        assert abs(N * (a * np.log(V / V0) + Psi) - S) <= 1e-10
+
+
+def test_loading_tensile_and_simple_branches():
+    # Slock = S0 + phis, Nlock = N0 - (1 - skempton) phip
+    rc, V, S, N, _ = ob.solve_friction_node("RSL", RS, 60.0, 100.0, 5.0, 3.0, ETA, 0.0, 0.7,
+                                            1e-9, skempton=0.2)
+    assert N == 100.0 - 0.8 * 5.0
+    assert abs(S + ETA * V - 63.0) < 1e-12 * 63.0
+    # tensile: no strength, V = Slock / eta
+    rc, V, S, N, _ = ob.solve_friction_node("RFL", RS, 30.0, -5.0, 0.0, 0.0, ETA, 0.0, 0.7, 1e-9)
+    assert S == 0.0 and V == 30.0 / ETA and N == -5.0
+    rc, V, S, N, _ = ob.solve_friction_node("frictionless", RS, 30.0, 50.0, 0.0, 0.0, ETA, 0.0,
+                                            0.7, 1e-9)
+    assert S == 0.0 and V == 30.0 / ETA
+    rc, V, S, N, _ = ob.solve_friction_node("locked", RS, 30.0, 50.0, 0.0, 0.0, ETA, 0.0, 0.7,
+                                            1e-9)
+    assert V == 0.0 and S == 30.0
+
+
+def test_slip_weakening_branches():
+    sw = np.array([1.0, 0.677, 0.525, 0.4])  # cohesion, fs, fd, Dc
+    N = 120.0
+    for D, Slock in [(0.0, 70.0), (0.1, 70.0), (0.1, 90.0), (0.5, 70.0), (0.5, -70.0)]:
+        Sv = sw[0] + N * (sw[1] - (sw[1] - sw[2]) * min(D, sw[3]) / sw[3])
+        rc, V, S, _, _ = ob.solve_friction_node("SW", RS, Slock, N, 0.0, 0.0, ETA, D, 0.0, 0.0,
+                                                sw=sw)
+        if abs(Slock) >= Sv:
+            assert abs(S - np.sign(Slock) * Sv) < 1e-12 * Sv
+            assert abs(V - (Slock - S) / ETA) < 1e-14 * abs(Slock)
+        else:
+            assert V == 0.0 and S == Slock
+
+
+def _fss(law, V):
+    a, b, V0, f0, L, fw, Vw = RS
+    fLV = f0 - (b - a) * np.log(V / V0)
+    if law in ("SL", "RSL"):
+        return fLV
+    return fw + (fLV - fw) / (1.0 + (V / Vw) ** 8) ** 0.125  # flash heating
+
+
+@pytest.mark.parametrize("law", ["SL", "FL", "RSL", "RFL", "RSF", "RAL"])
+@pytest.mark.parametrize("V", [1e-9, 1e-3, 0.5, 3.0])
+def test_state_evolution_steady_state(law, V):
+    a, b, V0, f0, L, fw, Vw = RS
+    if law == "RAL":  # regularised aging law
+        Psi_ss = f0 - b * np.log(V / V0)
+        slope = -(b * V0 / L) * np.exp((f0 - Psi_ss) / b) / b
+    elif law in ("SL", "FL"):  # f = a ln(V/V0) + Psi
+        Psi_ss = _fss(law, V) - a * np.log(V / V0)
+        slope = -V / L
+    else:  # f = a asinh(V e^(Psi/a) / (2 V0))
+        Psi_ss = a * np.log(2 * V0 / V * np.sinh(_fss(law, V) / a))
+        slope = -V / L
+        if law != "RSF":
+            X = V / (2 * V0) * np.exp(Psi_ss / a)
+            slope *= X / np.sqrt(1 + X * X)  # d f / d Psi
+    scale = abs(slope) * max(1.0, abs(Psi_ss))
+    assert abs(ob.state_rate(law, RS, V, Psi_ss)) <= 1e-10 * scale
+    d = 1e-6
+    num = (ob.state_rate(law, RS, V, Psi_ss + d) - ob.state_rate(law, RS, V, Psi_ss - d)) / (2 * d)
+    assert abs(num - slope) <= 1e-5 * abs(slope)
+
+
+# ---------------------------------------------------------------------------
+# plasticity
+# ---------------------------------------------------------------------------
+
+
+def _dp_state(s, s0):
+    sxx, sxy, syy, szz = s0[0] + s[0], s0[1] + s[1], s0[3] + s[2], s0[5] + s[3]
+    sig = (sxx + syy + szz) / 3.0
+    J2 = ((sxx - syy) ** 2 + (syy - szz) ** 2 + (szz - sxx) ** 2) / 6.0 + sxy ** 2 + s0[2] ** 2 + s0[4] ** 2
+    return sig, np.sqrt(J2), np.array([sxx - sig, sxy, syy - sig, szz - sig])
+
+
+def test_plastic_return_onto_yield_surface():
+    G, K, mu, dt = 32.0, 60.0, 0.6, 1e-3
+    s0 = np.zeros(6)
+    s = np.array([-60.0, 44.0, -100.0, -80.0])
+    sig, tau, dev = _dp_state(s, s0)
+    assert tau - (-mu * sig) > 0  # above yield
+    s1, Ds1, lam, gp = ob.plastic_point(G, K, mu, 0.0, 0.0, 0.0, 0.0, s0, dt, s, np.zeros(4),
+                                        0.0, 0.0)
+    sig1, tau1, dev1 = _dp_state(s1, s0)
+    assert abs(tau1 - (-mu * sig1)) < 1e-12 * tau
+    assert abs(sig1 - sig) < 1e-12 * abs(sig)  # beta = 0: no dilatancy
+    assert np.allclose(dev1 / tau1, dev / tau, rtol=1e-12, atol=1e-14)  # radial return
+    assert abs(lam - (tau + mu * sig) / (dt * G)) < 1e-10 * lam
+    # plastic.f90 increments gammap by dt*lambda twice (lines 161 and 177)
+    assert abs(gp - 2.0 * dt * lam) < 1e-15 * gp
+    # rates pick up -lambda * G * dev / tau
+    assert np.allclose(Ds1, -lam * G * dev / tau, rtol=1e-12, atol=1e-12)
+    # below yield: nothing changes
+    s_el = np.array([-100.0, 10.0, -100.0, -100.0])
+    s2, Ds2, lam2, gp2 = ob.plastic_point(G, K, mu, 0.0, 0.0, 0.0, 0.0, s0, dt, s_el,
+                                          np.ones(4), 0.0, 0.0)
+    assert lam2 == 0.0 and gp2 == 0.0 and np.array_equal(s2, s_el) and np.array_equal(Ds2, np.ones(4))
+
+
+# ---------------------------------------------------------------------------
+# manufactured solution
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("side", [1, 2])
+def test_mms_sources_are_residuals_of_the_fields(side):
+    G, cs, cp = 32.0, 3.0, 5.0
+    rho = G / cs ** 2
+    lam = rho * cp ** 2 - 2 * G
+    M = lam + 2 * G
+    u = lambda x, y, t, l: ob.mms_sin(x, y, t, side, l)
+
+    def d(fn, i, h=1e-4):  # fourth-order central difference along argument i
+        def g(*args):
+            a = list(args)
+            vals = []
+            for k in (-2, -1, 1, 2):
+                b = list(a)
+                b[i] += k * h
+                vals.append(fn(*b))
+            return (vals[0] - 8 * vals[1] + 8 * vals[2] - vals[3]) / (12 * h)
+        return g
+
+    rng = np.random.default_rng(side)
+    for x, y, t in rng.uniform([-0.5, -0.5, 0.0], [0.5, 0.5, 0.3], (6, 3)):
+        D = {(l, i): d(lambda X, Y, T, l=l: u(X, Y, T, l), i)(x, y, t) for l in range(6) for i in range(3)}
+        L = [(D[2, 0] + D[3, 1]) / rho, (D[3, 0] + D[4, 1]) / rho,
+             M * D[0, 0] + lam * D[1, 1], G * (D[1, 0] + D[0, 1]),
+             lam * D[0, 0] + M * D[1, 1], lam * (D[0, 0] + D[1, 1])]
+        for l in range(6):
+            res = D[l, 2] - L[l]
+            src = u(x, y, t, 6 + l)
+            scale = max(1.0, abs(D[l, 2]), abs(L[l]))
+            assert abs(res - src) < 1e-6 * scale, (l, res, src)
+
+
+# ---------------------------------------------------------------------------
+# whole stage
+# ---------------------------------------------------------------------------
+
+
+def split_ops(prob):
+    Dq = dense_D(prob.nx)
+    Dr = dense_D(prob.ny)
+    dq = lambda f: f @ Dq.T
+    dr = lambda f: Dr @ f
+    sq = lambda a, f: 0.5 * (dq(a * f) + a * dq(f))
+    sr = lambda a, f: 0.5 * (dr(a * f) + a * dr(f))
+    dx = lambda f: prob.Ji * (sq(prob.yr, f) - sr(prob.yq, f))
+    dy = lambda f: prob.Ji * (-sq(prob.xr, f) + sr(prob.xq, f))
+    return dx, dy
+
+
+@pytest.mark.parametrize("seed,warp", [(1, 0.0), (2, 0.08)])
+def test_stage1_rates_match_dense_split_operator(seed, warp):
+    cs, cp, rho = 3.464, 6.0, 2.67
+    prob, F = problems.build(nx=13, nys=(15,), seed=seed, warp=warp, cs=cs, cp=cp, rho=rho,
+                             background=[0.1, -0.2, -120.0, 70.0, -110.0, -100.0], perturb=0.5,
+                             bcs=("absorbing", "free", "absorbing", "free"))
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.stage1(0.0, 1e-3, 1)
+    _, DF = o.get_fields()
+    G = rho * cs ** 2
+    M = rho * cp ** 2
+    lam = M - 2 * G
+    gamma = lam / M
+    nu = lam / (2 * (lam + G))
+    dx, dy = split_ops(prob)
+    vx, vy, sxx, sxy, syy = F[:5]
+    ref = np.empty_like(DF)
+    ref[0] = (dx(sxx) + dy(sxy)) / rho
+    ref[1] = (dx(sxy) + dy(syy)) / rho
+    ref[2] = M * (dx(vx) + gamma * dy(vy))
+    ref[3] = G * (dx(vy) + dy(vx))
+    ref[4] = M * (gamma * dx(vx) + dy(vy))
+    ref[5] = nu * (ref[2] + ref[4])
+    # boundary nodes carry SAT penalties on top; every other point, closure
+    # rows and corners included, is the operator alone
+    inner = (slice(None), slice(1, -1), slice(1, -1))
+    scale = np.abs(ref[inner]).max(axis=(1, 2), keepdims=True)
+    err = np.abs(DF[inner] - ref[inner]) / scale
+    assert err.max() < 1e-12, err.max()
+
+
+def energy(prob, F, rho, cs, cp):
+    _, _, HL, _ = ob.sbp4_tables()
+
+    def hw(n):
+        h = np.ones(n)
+        h[:4] = HL
+        h[n - 4:] = HL[::-1]
+        return h
+    W = np.outer(hw(prob.ny), hw(prob.nx)) / prob.Ji
+    G = rho * cs ** 2
+    lam = rho * cp ** 2 - 2 * G
+    nu = lam / (2 * (lam + G))
+    vx, vy, sxx, sxy, syy = F[:5]
+    e = 0.5 * rho * (vx ** 2 + vy ** 2) + ((1 - nu) * (sxx ** 2 + syy ** 2) - 2 * nu * sxx * syy
+                                         + 2 * sxy ** 2) / (4 * G)
+    return float((W * e).sum())
+
+
+@pytest.mark.parametrize("bcs", [("absorbing",) * 4, ("absorbing", "free", "absorbing", "free")])
+def test_energy_decays_with_absorbing_boundaries(bcs):
+    cs, cp, rho, h = 3.464, 6.0, 2.67, 0.025
+    prob, F = problems.build(nx=31, nys=(27,), seed=4, warp=0.06, h=h, cs=cs, cp=cp, rho=rho,
+                             background=[0.0] * 6, perturb=1.0, bcs=bcs)
+    dt = 0.5 * h / np.hypot(cs, cp)
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    E = [energy(prob, F, rho, cs, cp)]
+    for n in range(1, 61):
+        o.step(0.0, dt, n, 1)
+        E.append(energy(prob, o.get_fields()[0], rho, cs, cp))
+    E = np.array(E)
+    assert (np.diff(E) <= 1e-12 * E[0]).all(), np.diff(E).max() / E[0]
+    assert E[-1] < 0.95 * E[0]
+
+
+def mms_error(n, tend=0.04, warp=0.05):
+    G, cs, cp = 32.0, 3.0, 5.0
+    rho = G / cs ** 2
+    h = 1.0 / (n - 1)
+    prob, F = problems.build(nx=n, nys=(n,), seed=8, warp=warp, h=h, cs=cs, cp=cp, rho=rho,
+                             mms=True, bcs=("mms-sin-w",) * 4)
+    nsteps = int(np.ceil(tend / (0.5 * h / np.hypot(cs, cp))))
+    dt = tend / nsteps
+    o = OracleContext(prob, threads=4)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.step(0.0, dt, 1, nsteps)
+    Fo = o.get_fields()[0]
+    ex = np.empty_like(Fo)
+    for l in range(6):
+        ex[l] = np.vectorize(lambda x, y: ob.mms_sin(x, y, tend, 1, l))(prob.x, prob.y)
+    _, _, HL, _ = ob.sbp4_tables()
+    err = Fo[:5] - ex[:5]
+    return np.sqrt((err ** 2).sum(axis=(1, 2)) * h * h).sum()
+
+
+def test_mms_convergence_rate():
+    e = [mms_error(n) for n in (21, 41, 81)]
+    rates = np.log2(np.array(e[:-1]) / np.array(e[1:]))
+    print("MMS errors", e, "rates", rates)
+    assert rates[-1] > 2.7, rates
+
+
+def to_nt(F, n1, n2):
+    """velocity and in-plane stress in the (tangent, normal) frame, tangent
+    t = (n2, -n1): (v.t, v.n, t.s.t, n.s.t, n.s.n)"""
+    vx, vy, sxx, sxy, syy = F[:5]
+    t1, t2 = n2, -n1
+    return (vx * t1 + vy * t2, vx * n1 + vy * n2,
+            t1 * t1 * sxx + 2 * t1 * t2 * sxy + t2 * t2 * syy,
+            n1 * t1 * sxx + (n1 * t2 + n2 * t1) * sxy + n2 * t2 * syy,
+            n1 * n1 * sxx + 2 * n1 * n2 * sxy + n2 * n2 * syy)
+
+
+@pytest.mark.parametrize("bcs", [("absorbing", "free", "rigid", "absorbing"),
+                                 ("free", "rigid", "absorbing", "free")])
+def test_boundary_hat_states_satisfy_characteristic_conditions(bcs):
+    """At a boundary node the stage-1 rate is the operator plus the penalty
+    Ks Fs(Fhat - F) + Kp Fp(Fhat - F); rotated to (t, n) it yields the hat
+    differences.  Every hat state keeps the outgoing characteristics
+    snt - Zs vt and snn - Zp vn, meets its own condition (absorbing: the
+    incoming snt + Zs vt and snn + Zp vn vanish; free: snt = snn = 0; rigid:
+    vt = vn = 0), and moves stt by gamma times the change of snn, as a
+    P wave does."""
+    cs, cp, rho = 3.464, 6.0, 2.67
+    prob, F = problems.build(nx=13, nys=(14,), seed=12, warp=0.07, cs=cs, cp=cp, rho=rho,
+                             background=[0.0] * 6, perturb=3.0, bcs=bcs)
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.stage1(0.0, 1e-3, 1)
+    _, DF = o.get_fields()
+    G, M = rho * cs ** 2, rho * cp ** 2
+    gamma = (M - 2 * G) / M
+    dx, dy = split_ops(prob)
+    vx, vy, sxx, sxy, syy = F[:5]
+    ref = [(dx(sxx) + dy(sxy)) / rho, (dx(sxy) + dy(syy)) / rho, M * (dx(vx) + gamma * dy(vy)),
+           G * (dx(vy) + dy(vx)), M * (gamma * dx(vx) + dy(vy))]
+    SAT = np.array([DF[l] - ref[l] for l in range(5)])
+    blk = prob.blocks[0]
+    ny, nx = prob.ny, prob.nx
+    for s, side in enumerate(blk["sides"]):
+        idx = [(slice(None), 0), (slice(None), nx - 1), (0, slice(None)), (ny - 1, slice(None))][s]
+        n1, n2 = side["normal"]
+        Zs, Zp, gam, Ks, Kp = side["M"]
+        pen = to_nt(SAT[(slice(None),) + idx], n1, n2)
+        vt, vn, stt, snt, snn = to_nt(F[(slice(None),) + idx], n1, n2)
+        dvt, dvn, dstt, dsnt, dsnn = pen[0] / Ks, pen[1] / Kp, pen[2] / Kp, pen[3] / Ks, pen[4] / Kp
+        k = slice(1, -1)  # corners carry two penalties
+        h = dict(vt=(vt + dvt)[k], vn=(vn + dvn)[k], snt=(snt + dsnt)[k], snn=(snn + dsnn)[k])
+        sc = np.abs(np.concatenate([snn, snt, Zp[k].max() * vn, Zs[k].max() * vt])).max()
+        tol = 1e-9 * sc
+        assert np.abs((h["snt"] - Zs[k] * h["vt"]) - (snt - Zs * vt)[k]).max() < tol, (s, "S out")
+        assert np.abs((h["snn"] - Zp[k] * h["vn"]) - (snn - Zp * vn)[k]).max() < tol, (s, "P out")
+        assert np.abs(dstt - gam * dsnn)[k].max() < tol, (s, "stt")
+        bc = bcs[s]
+        if bc == "absorbing":
+            assert np.abs(h["snt"] + Zs[k] * h["vt"]).max() < tol, (s, bc)
+            assert np.abs(h["snn"] + Zp[k] * h["vn"]).max() < tol, (s, bc)
+        elif bc == "free":
+            assert np.abs(h["snt"]).max() < tol and np.abs(h["snn"]).max() < tol, (s, bc)
+        elif bc == "rigid":
+            assert np.abs(h["vt"]).max() < tol / Zs.max() and np.abs(h["vn"]).max() < tol / Zp.max()
+
+
+def test_locked_interface_hat_states():
+    """Locked interface (interfaces.f90:715-772): the hat states of the two
+    sides have equal velocity and traction, each side keeps its outgoing
+    characteristics (snt -/+ Zs vt, snn -/+ Zp vn along the minus side's
+    normal) and moves stt by gamma times its change of snn."""
+    cs, cp, rho = 3.464, 6.0, 2.67
+    prob, F = problems.build(nx=13, nys=(12, 13), seed=21, warp=0.06, cs=cs, cp=cp, rho=rho,
+                             background=[0.2, -0.1, -120.0, 70.0, -110.0, -100.0], perturb=3.0,
+                             coupling="locked", bcs=("absorbing", "free", "absorbing", "free"))
+    # a jump across the interface, so that the hat states differ from both sides
+    rng = np.random.default_rng(5)
+    F[:, 12:] += rng.uniform(0.5, 2.0, (6, 1, 1)) * np.cos(2.0 * prob.x[12:] + rng.uniform(0, 6, (6, 1, 1)))
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.stage1(0.0, 1e-3, 1)
+    _, DF = o.get_fields()
+    G, M = rho * cs ** 2, rho * cp ** 2
+    gamma = (M - 2 * G) / M
+    hat = {}
+    for b, (s, j) in enumerate([(3, 11), (2, 12)]):
+        blk = prob.blocks[b]
+        sj = slice(blk["my"], blk["py"] + 1)
+        sub = type("P", (), {})()
+        sub.nx, sub.ny = prob.nx, blk["py"] - blk["my"] + 1
+        for k in ("xq", "xr", "yq", "yr", "Ji"):
+            setattr(sub, k, getattr(prob, k)[sj])
+        dx, dy = split_ops(sub)
+        vx, vy, sxx, sxy, syy = F[:5, sj]
+        ref = [(dx(sxx) + dy(sxy)) / rho, (dx(sxy) + dy(syy)) / rho, M * (dx(vx) + gamma * dy(vy)),
+               G * (dx(vy) + dy(vx)), M * (gamma * dx(vx) + dy(vy))]
+        r = j - blk["my"]
+        SAT = np.array([DF[l, j] - ref[l][r] for l in range(5)])
+        n1, n2 = prob.blocks[0]["sides"][3]["normal"]  # minus side's outward normal
+        Zs, Zp, gam, Ks, Kp = blk["sides"][s]["M"]
+        pen = to_nt(SAT, n1, n2)
+        base = to_nt(F[:, j], n1, n2)
+        d = [pen[0] / Ks, pen[1] / Kp, pen[2] / Kp, pen[3] / Ks, pen[4] / Kp]
+        hat[b] = dict(vt=base[0] + d[0], vn=base[1] + d[1], snt=base[3] + d[3],
+                      snn=base[4] + d[4], base=base, Zs=Zs, Zp=Zp, dstt=d[2], dsnn=d[4], gam=gam)
+    k = slice(1, -1)
+    m, p = hat[0], hat[1]
+    sc = np.abs(np.concatenate([m["base"][4], m["base"][3], p["base"][4]])).max()
+    tol = 1e-9 * sc
+    for f in ("snt", "snn"):
+        assert np.abs(m[f] - p[f])[k].max() < tol, f
+    for f in ("vt", "vn"):
+        assert np.abs(m[f] - p[f])[k].max() * m["Zp"].max() < tol, f
+    for h, sgn in ((m, -1.0), (p, 1.0)):
+        vt, vn, _, snt, snn = h["base"]
+        assert np.abs((h["snt"] + sgn * h["Zs"] * h["vt"]) - (snt + sgn * h["Zs"] * vt))[k].max() < tol
+        assert np.abs((h["snn"] + sgn * h["Zp"] * h["vn"]) - (snn + sgn * h["Zp"] * vn))[k].max() < tol
+        assert np.abs(h["dstt"] - h["gam"] * h["dsnn"])[k].max() < tol
+
+
+@pytest.mark.parametrize("seed", [33, 34])
+def test_frictional_interface_hat_states_and_law(seed):
+    """Frictional interface (interfaces.f90:900-1012 with solve_friction):
+    with equal initial fields on both sides, the hat states have equal
+    traction and normal velocity, a tangential velocity jump equal to the
+    slip velocity V, keep each side's outgoing characteristics, and the
+    stored S, N are the hat tractions plus the load; (V, S, N, Psi) satisfy
+    the regularised slip law with flash heating (RFL)."""
+    cs, cp, rho = 3.464, 6.0, 2.67
+    prob, F = problems.build(nx=17, nys=(12, 13), seed=seed, warp=0.05, cs=cs, cp=cp, rho=rho,
+                             background=[0.0, 0.0, -120.0, 70.0, -120.0, -120.0], perturb=2.0,
+                             law="RFL", fault_overstress=25.0,
+                             bcs=("absorbing", "absorbing", "absorbing", "free"))
+    rng = np.random.default_rng(seed)
+    F[:, 12:] += rng.uniform(0.5, 2.0, (6, 1, 1)) * np.cos(2.0 * prob.x[12:] + rng.uniform(0, 6, (6, 1, 1)))
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.stage1(0.0, 1e-3, 1)
+    _, DF = o.get_fields()
+    st = {f: o.get_iface(0, f) for f in ("V", "S", "N", "Psi", "O")}
+    G, M = rho * cs ** 2, rho * cp ** 2
+    gamma = (M - 2 * G) / M
+    itf = prob.ifaces[0]
+    n1, n2 = itf["nhat"]
+    hat = {}
+    for b, (s, j) in enumerate([(3, 11), (2, 12)]):
+        blk = prob.blocks[b]
+        sj = slice(blk["my"], blk["py"] + 1)
+        sub = type("P", (), {})()
+        sub.nx, sub.ny = prob.nx, blk["py"] - blk["my"] + 1
+        for k in ("xq", "xr", "yq", "yr", "Ji"):
+            setattr(sub, k, getattr(prob, k)[sj])
+        dx, dy = split_ops(sub)
+        vx, vy, sxx, sxy, syy = F[:5, sj]
+        ref = [(dx(sxx) + dy(sxy)) / rho, (dx(sxy) + dy(syy)) / rho, M * (dx(vx) + gamma * dy(vy)),
+               G * (dx(vy) + dy(vx)), M * (gamma * dx(vx) + dy(vy))]
+        r = j - blk["my"]
+        SAT = np.array([DF[l, j] - ref[l][r] for l in range(5)])
+        Zs, Zp, gam, Ks, Kp = blk["sides"][s]["M"]
+        pen = to_nt(SAT, n1, n2)
+        base = to_nt(F[:, j], n1, n2)
+        d = [pen[0] / Ks, pen[1] / Kp, pen[2] / Kp, pen[3] / Ks, pen[4] / Kp]
+        hat[b] = dict(vt=base[0] + d[0], vn=base[1] + d[1], snt=base[3] + d[3],
+                      snn=base[4] + d[4], base=base, Zs=Zs, Zp=Zp, dstt=d[2], dsnn=d[4], gam=gam)
+    k = slice(1, -1)
+    m, p = hat[0], hat[1]
+    tol = 1e-9 * 120.0
+    vtol = tol / m["Zs"].max()
+    assert np.abs(m["snt"] - p["snt"])[k].max() < tol
+    assert np.abs(m["snn"] - p["snn"])[k].max() < tol
+    assert np.abs(m["vn"] - p["vn"])[k].max() < vtol
+    assert np.abs((p["vt"] - m["vt"]) - st["V"])[k].max() < vtol
+    assert np.abs(st["O"]).max() == 0.0
+    for h, sgn in ((m, -1.0), (p, 1.0)):
+        vt, vn, _, snt, snn = h["base"]
+        assert np.abs((h["snt"] + sgn * h["Zs"] * h["vt"]) - (snt + sgn * h["Zs"] * vt))[k].max() < tol
+        assert np.abs((h["snn"] + sgn * h["Zp"] * h["vn"]) - (snn + sgn * h["Zp"] * vn))[k].max() < tol
+        assert np.abs(h["dstt"] - h["gam"] * h["dsnn"])[k].max() < tol
+    # stored fault tractions are the hat tractions plus the load (ramp = 1 at t = 0 with ld_T = 0)
+    S_load = itf["S0base"] + itf["ldS"]
+    N_load = itf["N0base"] + itf["ldN"]
+    assert np.abs(st["S"] - (m["snt"] + S_load))[k].max() < tol
+    assert np.abs(st["N"] - (-m["snn"] - N_load))[k].max() < tol
+    a, V0 = itf["rs"][0], itf["rs"][2]
+    Sf = a * st["N"] * np.arcsinh(st["V"] / (2 * V0) * np.exp(st["Psi"] / a))
+    assert np.abs(Sf - st["S"]).max() < 1e4 * 2.3e-16 * 200
+    assert (st["V"][k] > 0).any() and (st["N"] > 0).all()
