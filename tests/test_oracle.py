@@ -617,3 +617,124 @@ def test_frictional_interface_hat_states_and_law(seed):
     Sf = a * st["N"] * np.arcsinh(st["V"] / (2 * V0) * np.exp(st["Psi"] / a))
     assert np.abs(Sf - st["S"]).max() < 1e4 * 2.3e-16 * 200
     assert (st["V"][k] > 0).any() and (st["N"] > 0).all()
+
+
+def _fault_problem(seed, **kw):
+    prob, F = problems.build(nx=17, nys=(12, 13), seed=seed, warp=0.05, cs=3.464, cp=6.0, rho=2.67,
+                             background=[0.0, 0.0, -120.0, 70.0, -120.0, -120.0], perturb=2.0,
+                             law="RFL", bcs=("absorbing", "absorbing", "absorbing", "free"), **kw)
+    rng = np.random.default_rng(seed)
+    F[:, 12:] += rng.uniform(0.5, 2.0, (6, 1, 1)) * np.cos(2.0 * prob.x[12:] + rng.uniform(0, 6, (6, 1, 1)))
+    return prob, F
+
+
+def test_lock_region_is_coulomb_friction():
+    """Outside the lock region (friction.f90:650-672) the fault obeys
+    Coulomb friction with coefficient flock: |S| <= flock N, slipping only at
+    |S| = flock N, in the direction of S."""
+    prob, F = _fault_problem(41, outside_frac=0.3, fault_overstress=60.0)
+    out = prob.ifaces[0]["outside"].astype(bool)
+    flock = prob.ifaces[0]["flock"]
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.stage1(0.0, 1e-3, 1)
+    V, S, N = (o.get_iface(0, f)[out] for f in ("V", "S", "N"))
+    assert (np.abs(S) <= flock * N * (1 + 1e-14)).all()
+    slip = V != 0.0
+    assert slip.any() and (~slip).any()
+    assert np.allclose(np.abs(S[slip]), flock * N[slip], rtol=1e-14, atol=0)
+    assert (np.sign(S[slip]) == np.sign(V[slip])).all()
+
+
+@pytest.mark.parametrize("T,t", [(0.01, 0.004), (0.05, 0.02)])
+def test_load_ramp(T, t):
+    """load_stress with ramp time T (friction.f90:1339-1374, 1455-1484): the
+    stored shear traction is the hat traction plus S0base + ldS (1 - e^(-t/T))."""
+    prob, F = _fault_problem(43, ld_T=T)
+    itf = prob.ifaces[0]
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    dt = 1e-3
+    n = int(round(t / dt)) + 1
+    o.stage1(0.0, dt, n)
+    S, N = o.get_iface(0, "S"), o.get_iface(0, "N")
+    S0, N0 = o.get_iface(0, "S0"), o.get_iface(0, "N0")
+    n1, n2 = itf["nhat"]
+    sxx, sxy, syy = -120.0, 70.0, -120.0
+    snt = n1 * n2 * (sxx - syy) + (n2 * n2 - n1 * n1) * sxy
+    ramp = 1.0 - np.exp(-t / T)
+    # S0, N0: the resolved initial traction of the plus side plus the ramped load
+    assert np.allclose(S0, snt + itf["S0base"] + itf["ldS"] * ramp, rtol=1e-13, atol=1e-12)
+    assert (N > 0).all() and np.isfinite(S).all()
+
+
+def test_fault_state_rates_over_one_small_step():
+    """set_rates_friction (friction.f90:1101-1124) and the fault update: over
+    one short step D grows by the time integral of |V|, W by that of S V and
+    Psi by that of its state rate (trapezoid rule, second order in dt)."""
+    prob, F = _fault_problem(47)
+    dt = 1e-6
+    rs = prob.ifaces[0]["rs"]
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    st = []
+    for n in (1, 2):
+        o.stage1(0.0, dt, n)
+        s = {f: o.get_iface(0, f).copy() for f in ("V", "S", "Psi", "D", "W")}
+        s["rate"] = np.array([ob.state_rate("RFL", rs[:, k], s["V"][k], s["Psi"][k])
+                              for k in range(len(s["V"]))])
+        st.append(s)
+        if n == 1:
+            o.step(0.0, dt, 1, 1)
+    a, b = st
+    k = slice(1, -1)
+    trap = lambda f: 0.5 * (f(a) + f(b))[k] * dt
+    assert np.allclose((b["D"] - a["D"])[k], trap(lambda s: np.abs(s["V"])), rtol=1e-5, atol=1e-20)
+    assert np.allclose((b["W"] - a["W"])[k], trap(lambda s: s["S"] * s["V"]), rtol=1e-5, atol=1e-20)
+    # increments below a few ulps of Psi (about 0.7) are lost to rounding
+    assert np.allclose((b["Psi"] - a["Psi"])[k], trap(lambda s: s["rate"]), rtol=1e-5, atol=5e-16)
+
+
+@pytest.mark.parametrize("stf", ["gaussian", "Ricker", "ramp", "sinusoid"])
+def test_point_source_moment_and_time_function(stf):
+    """The singular source (source.f90:196-254) adds A(t) M to the stress
+    rates through a discrete delta whose weights sum to 1/h in x and in y, so
+    sum(DF) h^2 = -A(t) M on a zero field, and A(t) has the defining
+    properties of each time function."""
+    h = 0.025
+    prob, F = problems.build(nx=21, nys=(19,), seed=2, warp=0.0, h=h, background=[0.0] * 6,
+                             perturb=0.0, source=True, bcs=("absorbing",) * 4)
+    tc, td = 0.02, 0.01
+    prob.source.update(stf=stf, t_center=tc, t_duration=td, Fx=0.3, Fy=-0.2, Mxx=1.0, Mxy=0.5,
+                       Myy=-0.7)
+    rho = prob.blocks[0]["rho"]
+
+    def amp(t):
+        o = OracleContext(prob)
+        o.set_fields(F)
+        o.init_edges(0.0)
+        o.stage1(t, 1.0, 1)
+        DF = o.get_fields()[1]
+        mom = DF.sum(axis=(1, 2)) * h * h
+        A = -mom[2] / 1.0
+        assert np.allclose(mom[:5], [A * 0.3 / rho, -A * 0.2 / rho, -A, -0.5 * A, 0.7 * A],
+                           rtol=1e-12, atol=1e-15)
+        return A
+
+    if stf == "gaussian":
+        assert abs(amp(tc) - 1.0) < 1e-14
+        assert abs(amp(tc + td) - np.exp(-0.5)) < 1e-14 and abs(amp(tc - td) - np.exp(-0.5)) < 1e-14
+    elif stf == "Ricker":
+        assert abs(amp(tc) - 1.0) < 1e-14
+        z = td / (np.pi * np.sqrt(2.0))
+        assert abs(amp(tc + z)) < 1e-14 and abs(amp(tc - z)) < 1e-14
+        assert amp(tc + 2 * z) < 0
+    elif stf == "ramp":
+        assert amp(tc - 0.1 * td) == 0.0 and amp(tc + 2 * td) == 1.0
+        assert abs(amp(tc + 0.25 * td) - 0.25) < 1e-14
+    else:
+        assert abs(amp(tc)) < 1e-14 and abs(amp(tc + 0.25 * td) - 1.0) < 1e-14
+        assert abs(amp(tc + 0.5 * td)) < 1e-14
