@@ -36,7 +36,10 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / (d if d > 0 else 1.0))
 
 
-def run_device(inpath, precision, steps, warmup):
+def run_device(inpath, precision, steps, warmup, prof_steps):
+    """Time `steps` steps after `warmup` untimed ones (the headline number),
+    then `prof_steps` more with HIP events around every launch (kernel
+    durations for the roofline)."""
     from fdmap_github_amd import host
     from fdmap_github_amd.device import Context
 
@@ -47,18 +50,23 @@ def run_device(inpath, precision, steps, warmup):
     ctx.init_edges(hp.t)
     ctx.step(hp.t, hp.dt, 1, warmup)
     ctx.sync()
-    ctx.profile_enable(True)
-    ctx.sync()
     t0 = time.perf_counter()
     ctx.step(hp.t, hp.dt, warmup + 1, steps)
     ctx.sync()
     el = time.perf_counter() - t0
-    prof = ctx.profile_read()
     F, _ = ctx.get_fields()
     fault = {f: ctx.get_iface(0, f) for f in ("D", "V", "S")}
+    ctx.profile_enable(True)
+    ctx.sync()
+    t0 = time.perf_counter()
+    ctx.step(hp.t, hp.dt, warmup + steps + 1, prof_steps)
+    ctx.sync()
+    el_prof = time.perf_counter() - t0
+    prof = ctx.profile_read()
     nstage = hp.ptr.contents.nstage
     ctx.close()
-    return dict(hp=hp, npts=npts, el=el, prof=prof, F=F, fault=fault, nstage=nstage)
+    return dict(hp=hp, npts=npts, el=el, el_prof=el_prof, prof=prof, F=F, fault=fault,
+                nstage=nstage)
 
 
 def cpu_baseline(inpath, sample_steps):
@@ -103,6 +111,8 @@ def main():
     ap.add_argument("--nx", type=int, default=1601)
     ap.add_argument("--ny", type=int, default=801)
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--prof-steps", type=int, default=200,
+                    help="steps timed after the headline ones with HIP events per launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     args = ap.parse_args()
@@ -116,14 +126,18 @@ def main():
     tmp = tempfile.mkdtemp(prefix="fdmap_bench_")
     inpath = inputs.write(os.path.join(tmp, "c2.in"),
                           inputs.c2_planar_rs(name=os.path.join(tmp, "c2"), nx=args.nx,
-                                              ny=args.ny, nsteps=args.steps + args.warmup,
+                                              ny=args.ny,
+                                              nsteps=args.steps + args.warmup + args.prof_steps,
                                               outputs=False))
-    r64 = run_device(inpath, "fp64", args.steps, args.warmup)
+    r64 = run_device(inpath, "fp64", args.steps, args.warmup, args.prof_steps)
     npts = r64["npts"]
     prof = r64["prof"]
     ns = r64["nstage"]
     b1, b2 = BYTES_STAGE["fp64"]
-    bytes_launch = npts * (b1 + (ns - 1) * b2) / ns
+    # the dominant kernel updates the block interiors; its launches cover
+    # prof["bulk_points"] points each
+    kpts = prof["bulk_points"]
+    bytes_launch = kpts * (b1 + (ns - 1) * b2) / ns
     avg_ms = prof["bulk_ms"] / max(1, prof["bulk_launches"])
     achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
     ups = npts * args.steps / r64["el"]
@@ -148,7 +162,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "bulk_tile<double,8,false> (SBP4 rates + SAT + RK update, 64x8 LDS tiles)",
+            "kernel": "bulk_interior<double,8,false> (SBP4 interior rates + RK update, 64x8 LDS "
+                      "tiles over the block interiors)",
+            "kernel_points": kpts,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -157,24 +173,27 @@ def main():
             "alg_bytes_per_launch": bytes_launch,
             "avg_launch_ms": avg_ms,
             "launches": prof["bulk_launches"],
+            "ms_per_step_with_events": 1e3 * r64["el_prof"] / args.prof_steps,
             "edge_kernel_avg_ms": prof["edge_ms"] / max(1, prof["edge_launches"]),
+            "band_kernel_avg_ms": prof["band_ms"] / max(1, prof["band_launches"]),
             "hbm_frac_end_to_end": npts * (b1 + (ns - 1) * b2) * args.steps / r64["el"] / 1e9
                                    / HBM_PEAK_GBS,
         },
     }
     out["roofline"]["traffic"] = measured_traffic()
     if not args.no_fp32:
-        r32 = run_device(inpath, "fp32", args.steps, args.warmup)
+        r32 = run_device(inpath, "fp32", args.steps, args.warmup, args.prof_steps)
         ups32 = npts * args.steps / r32["el"]
         p32 = r32["prof"]
         c1, c2 = BYTES_STAGE["fp32"]
         avg32 = p32["bulk_ms"] / max(1, p32["bulk_launches"])
+        k32 = p32["bulk_points"]
         out["fp32_bulk"] = {
             "updates_per_s": ups32,
             "speedup_vs_fp64": ups32 / ups,
             "ms_per_step": 1e3 * r32["el"] / args.steps,
             "bulk_avg_launch_ms": avg32,
-            "bulk_achieved_GBs": npts * (c1 + (ns - 1) * c2) / ns / (avg32 * 1e-3) / 1e9,
+            "bulk_achieved_GBs": k32 * (c1 + (ns - 1) * c2) / ns / (avg32 * 1e-3) / 1e9,
             "rel_l2_fields_vs_fp64": rel_l2(r32["F"], r64["F"]),
             "rel_l2_slip_vs_fp64": rel_l2(r32["fault"]["D"], r64["fault"]["D"]),
             "rel_l2_slip_velocity_vs_fp64": rel_l2(r32["fault"]["V"], r64["fault"]["V"]),

@@ -139,6 +139,7 @@ class fdm_profile_t(C.Structure):
         ("bulk_ms", C.c_double), ("bulk_launches", C.c_int64),
         ("edge_ms", C.c_double), ("edge_launches", C.c_int64),
         ("bulk_points", C.c_int64),
+        ("band_ms", C.c_double), ("band_launches", C.c_int64),
     ]
 
 

@@ -151,6 +151,11 @@ struct fdm_ctx {
   size_t esz = 8;
   double rkA[FDM_MAX_STAGES], rkB[FDM_MAX_STAGES], rkC[FDM_MAX_STAGES + 1];
   hipStream_t stream = nullptr;
+  // edge and band kernels run on a second stream (default priority) so that
+  // they overlap the interior launches; evI / evB mark the latest interior /
+  // band launch, evJ forks the side stream off the main one
+  hipStream_t side = nullptr;
+  hipEvent_t evI = nullptr, evB = nullptr, evJ = nullptr;
   int device = 0;
   // bulk storage (element type per precision)
   void* F[2] = {nullptr, nullptr};
@@ -164,6 +169,7 @@ struct fdm_ctx {
   int* blk = nullptr;
   BlockDev* blocks = nullptr;
   std::vector<BlockDev> hblocks;
+  std::vector<int> hcolb, hrowb, hblk;  // host copies of the block maps
   // sides
   int NS = 0;
   double* sM = nullptr;
@@ -204,22 +210,31 @@ struct fdm_ctx {
   bool prof = false;
   std::vector<EventPair> pool;
   size_t pool_used = 0;
-  double bulk_ms = 0.0, edge_ms = 0.0;
-  long long bulk_n = 0, edge_n = 0;
+  // kinds: 0 interior (or per-point) bulk, 1 edge, 2 band
+  double kind_ms[3] = {0.0, 0.0, 0.0};
+  long long kind_n[3] = {0, 0, 0};
   std::vector<int> pair_kind;
   std::string err;
   std::vector<void*> allocs;
   // launch shape of the bulk kernel (tile 64 x tile_y points)
   int tile_y = 4;
-  int xcd_swizzle = 0;
-  // bulk kernel: 0 per-point, 1 row-marching, 2 LDS tile (64 x tile_ty)
+  int xcd_swizzle = 1;
+  // bulk kernel: 0 per-point, 2 interior tiles (64 x tile_ty) + band points
   int bulk_mode = 2;
   int tile_ty = 8;
-  // row-marching kernel: strip width, rows per segment, segment table
-  int march_h = 8;
-  int nsegs = 0;
-  int4* segs = nullptr;
+  // interior tiles (block, i0, j0, 0) for tile heights 4 and 8; band points (i, j)
+  int4* tiles[2] = {nullptr, nullptr};
+  int ntiles[2] = {0, 0};
+  int2* band = nullptr;
+  int nband = 0;
+  long long ninterior = 0;
 };
+
+// block index of global point (i, j)
+static inline int block_of(const fdm_ctx* c, int i, int j) {
+  return c->hblk[(size_t)c->hrowb[j] * c->nbx + c->hcolb[i]];
+}
+
 
 namespace {
 
@@ -296,7 +311,7 @@ double load_ramp(double t, double T) {
   return (t < 0.0) ? 0.0 : 1.0 - std::exp(-t / T);
 }
 
-EventPair* prof_begin(fdm_ctx* c, int kind) {
+EventPair* prof_begin(fdm_ctx* c, int kind, hipStream_t s) {
   if (!c->prof) return nullptr;
   if (c->pool_used == c->pool.size()) {
     if (c->pool.size() < 65536) {
@@ -307,10 +322,11 @@ EventPair* prof_begin(fdm_ctx* c, int kind) {
     } else {
       // drain: accumulate and recycle
       (void)hipStreamSynchronize(c->stream);
+      if (c->side) (void)hipStreamSynchronize(c->side);
       for (size_t k = 0; k < c->pool_used; k++) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, c->pool[k].a, c->pool[k].b);
-        if (c->pair_kind[k] == 0) c->bulk_ms += ms; else c->edge_ms += ms;
+        c->kind_ms[c->pair_kind[k]] += ms;
       }
       c->pool_used = 0;
     }
@@ -318,19 +334,19 @@ EventPair* prof_begin(fdm_ctx* c, int kind) {
   EventPair* ep = &c->pool[c->pool_used];
   c->pair_kind[c->pool_used] = kind;
   c->pool_used++;
-  (void)hipEventRecord(ep->a, c->stream);
+  (void)hipEventRecord(ep->a, s);
   return ep;
 }
 
-void prof_end(fdm_ctx* c, EventPair* ep, int kind) {
+void prof_end(fdm_ctx* c, EventPair* ep, int kind, hipStream_t s) {
   if (!ep) return;
-  (void)hipEventRecord(ep->b, c->stream);
-  if (kind == 0) c->bulk_n++; else c->edge_n++;
+  (void)hipEventRecord(ep->b, s);
+  c->kind_n[kind]++;
 }
 
 template <typename T>
-int launch_edges(fdm_ctx* c, double t, double Bdt, double A, int do_scale, int do_update,
-                 int snapshot, double Bload) {
+int launch_edges(fdm_ctx* c, hipStream_t s, double t, double Bdt, double A, int do_scale,
+                 int do_update, int snapshot, double Bload) {
   if (c->ntasks == 0) return FDM_OK;
   EdgeArgs<T> a;
   a.F = (const T*)c->F[c->cur];
@@ -351,16 +367,16 @@ int launch_edges(fdm_ctx* c, double t, double Bdt, double A, int do_scale, int d
   a.snapshot = snapshot;
   a.fail = c->fail;
   dim3 grid((c->ntasks + 63) / 64), block(64);
-  EventPair* ep = prof_begin(c, 1);
-  hipLaunchKernelGGL(edge_kernel<T>, grid, block, 0, c->stream, a);
-  prof_end(c, ep, 1);
+  EventPair* ep = prof_begin(c, 1, s);
+  hipLaunchKernelGGL(edge_kernel<T>, grid, block, 0, s, a);
+  prof_end(c, ep, 1, s);
   HIPCHK(c, hipGetLastError());
   return FDM_OK;
 }
 
 template <typename T>
-int launch_bulk(fdm_ctx* c, double A, double Bdt, int do_update, double t, double srcA,
-                int do_plastic, double dt) {
+BulkArgs<T> bulk_args(fdm_ctx* c, double A, double Bdt, int do_update, double t, double srcA,
+                      int do_plastic, double dt) {
   BulkArgs<T> a;
   a.Fin = (const T*)c->F[c->cur];
   a.Fout = (T*)c->F[1 - c->cur];
@@ -400,33 +416,72 @@ int launch_bulk(fdm_ctx* c, double A, double Bdt, int do_update, double t, doubl
   a.gammap = c->gammap;
   a.lambda = c->lambda;
   a.xcd_swizzle = c->xcd_swizzle;
-  EventPair* ep = prof_begin(c, 0);
-  if (c->bulk_mode == 2) {
-    // the lean variant serves launches without sources, manufactured
-    // solutions or a plastic update
-    const bool full = c->src.nplace > 0 || c->any_mms || a.do_plastic;
-    dim3 grid((c->nx + 63) / 64, (c->ny + c->tile_ty - 1) / c->tile_ty);
-    if (c->tile_ty == 4) {
-      if (full)
-        hipLaunchKernelGGL((bulk_tile<T, 4, true>), grid, dim3(64, 4), 0, c->stream, a);
-      else
-        hipLaunchKernelGGL((bulk_tile<T, 4, false>), grid, dim3(64, 4), 0, c->stream, a);
-    } else {
-      if (full)
-        hipLaunchKernelGGL((bulk_tile<T, 8, true>), grid, dim3(64, 8), 0, c->stream, a);
-      else
-        hipLaunchKernelGGL((bulk_tile<T, 8, false>), grid, dim3(64, 8), 0, c->stream, a);
-    }
-  } else if (c->bulk_mode == 1) {
-    dim3 grid((c->nx + 127) / 128, c->nsegs);
-    hipLaunchKernelGGL((bulk_march<T, 128>), grid, dim3(128), 0, c->stream, a, c->segs);
+  return a;
+}
+
+// the lean variants serve launches without sources, manufactured solutions
+// or a plastic update
+template <typename T>
+bool full_variant(const fdm_ctx* c, const BulkArgs<T>& a) {
+  return c->src.nplace > 0 || c->any_mms || a.do_plastic;
+}
+
+template <typename T>
+void launch_band(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
+  if (!c->nband) return;
+  EventPair* ep = prof_begin(c, 2, s);
+  dim3 grid((c->nband + 255) / 256);
+  if (full_variant(c, a))
+    hipLaunchKernelGGL((bulk_band<T, true>), grid, dim3(256), 0, s, a, c->band, c->nband);
+  else
+    hipLaunchKernelGGL((bulk_band<T, false>), grid, dim3(256), 0, s, a, c->band, c->nband);
+  prof_end(c, ep, 2, s);
+}
+
+template <typename T>
+void launch_interior(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
+  const int q = c->tile_ty == 4 ? 0 : 1;
+  if (!c->ntiles[q]) return;
+  EventPair* ep = prof_begin(c, 0, s);
+  const bool full = full_variant(c, a);
+  dim3 grid(c->ntiles[q]);
+  if (q == 0) {
+    if (full)
+      hipLaunchKernelGGL((bulk_interior<T, 4, true>), grid, dim3(64, 4), 0, s, a, c->tiles[0]);
+    else
+      hipLaunchKernelGGL((bulk_interior<T, 4, false>), grid, dim3(64, 4), 0, s, a, c->tiles[0]);
   } else {
-    dim3 block(64, c->tile_y);
-    dim3 grid((c->nx + 63) / 64, (c->ny + c->tile_y - 1) / c->tile_y);
-    hipLaunchKernelGGL(bulk_kernel<T>, grid, block, 0, c->stream, a);
+    if (full)
+      hipLaunchKernelGGL((bulk_interior<T, 8, true>), grid, dim3(64, 8), 0, s, a, c->tiles[1]);
+    else
+      hipLaunchKernelGGL((bulk_interior<T, 8, false>), grid, dim3(64, 8), 0, s, a, c->tiles[1]);
   }
-  prof_end(c, ep, 0);
-  HIPCHK(c, hipGetLastError());
+  prof_end(c, ep, 0, s);
+}
+
+template <typename T>
+void launch_pointwise(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
+  EventPair* ep = prof_begin(c, 0, s);
+  dim3 block(64, c->tile_y);
+  dim3 grid((c->nx + 63) / 64, (c->ny + c->tile_y - 1) / c->tile_y);
+  hipLaunchKernelGGL(bulk_kernel<T>, grid, block, 0, s, a);
+  prof_end(c, ep, 0, s);
+}
+
+// Fork the side stream off the main one before a sequence of stages, and
+// join it back afterwards, so that work queued on the main stream later (and
+// every synchronisation of it) is ordered after both.
+int fork_side(fdm_ctx* c) {
+  HIPCHK(c, hipEventRecord(c->evJ, c->stream));
+  HIPCHK(c, hipStreamWaitEvent(c->side, c->evJ, 0));
+  HIPCHK(c, hipEventRecord(c->evB, c->side));
+  HIPCHK(c, hipEventRecord(c->evI, c->stream));
+  return FDM_OK;
+}
+
+int join_side(fdm_ctx* c) {
+  HIPCHK(c, hipEventRecord(c->evB, c->side));
+  HIPCHK(c, hipStreamWaitEvent(c->stream, c->evB, 0));
   return FDM_OK;
 }
 
@@ -436,27 +491,52 @@ int run_stage(fdm_ctx* c, int stage, double t0, double dt, int final_step, int s
   double Bdt = c->rkB[stage - 1] * dt;
   double t = t0 + c->rkC[stage - 1] * dt;  // D%t during this stage (time_step.f90:137)
   double Bload = c->ld_T.empty() ? 1.0 : load_ramp(t, c->ld_T[0]);
-  int rc = launch_edges<T>(c, t, Bdt, A, 1, final_step ? 0 : 1, snapshot, Bload);
-  if (rc) return rc;
   double srcA = c->src.nplace ? stf_amplitude(c->src, t) : 0.0;
   int last = (stage == c->nstage);
-  rc = launch_bulk<T>(c, A, Bdt, final_step ? 0 : 1, t, srcA, (!final_step && last) ? 1 : 0, dt);
-  if (rc) return rc;
+  const BulkArgs<T> a =
+      bulk_args<T>(c, A, Bdt, final_step ? 0 : 1, t, srcA, (!final_step && last) ? 1 : 0, dt);
+  int rc;
+  if (c->bulk_mode == 2) {
+    // Interior points need neither this stage's edge values nor this
+    // stage's band: only the band of the previous stage (halo values, and
+    // the buffer this launch overwrites is the one that band read).  The
+    // band needs this stage's edge values and the previous interior launch.
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->evB, 0));
+    launch_interior<T>(c, c->stream, a);
+    rc = launch_edges<T>(c, c->side, t, Bdt, A, 1, final_step ? 0 : 1, snapshot, Bload);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamWaitEvent(c->side, c->evI, 0));
+    launch_band<T>(c, c->side, a);
+    HIPCHK(c, hipEventRecord(c->evB, c->side));
+    HIPCHK(c, hipEventRecord(c->evI, c->stream));
+  } else {
+    rc = launch_edges<T>(c, c->stream, t, Bdt, A, 1, final_step ? 0 : 1, snapshot, Bload);
+    if (rc) return rc;
+    launch_pointwise<T>(c, c->stream, a);
+  }
+  HIPCHK(c, hipGetLastError());
   if (!final_step) c->cur = 1 - c->cur;
   return FDM_OK;
 }
 
 template <typename T>
 int step_impl(fdm_ctx* c, double t, double dt, long long n_first, int nsteps) {
-  for (int k = 0; k < nsteps; k++) {
+  int rc = fork_side(c);
+  for (int k = 0; k < nsteps && !rc; k++) {
     long long n = n_first + k;
     double t0 = t + (double)(n - 1) * dt;  // main.f90:176
-    for (int stage = 1; stage <= c->nstage; stage++) {
-      int rc = run_stage<T>(c, stage, t0, dt, 0, stage == 1);
-      if (rc) return rc;
-    }
+    for (int stage = 1; stage <= c->nstage && !rc; stage++) rc = run_stage<T>(c, stage, t0, dt, 0, stage == 1);
   }
-  return FDM_OK;
+  int rj = join_side(c);
+  return rc ? rc : rj;
+}
+
+template <typename T>
+int stage1_impl(fdm_ctx* c, double t0, double dt) {
+  int rc = fork_side(c);
+  if (!rc) rc = run_stage<T>(c, 1, t0, dt, 1, 1);
+  int rj = join_side(c);
+  return rc ? rc : rj;
 }
 
 int check_fail(fdm_ctx* c) {
@@ -477,11 +557,16 @@ const char* fdm_last_error(const fdm_ctx* c) { return c ? c->err.c_str() : g_cre
 void fdm_destroy(fdm_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->side) (void)hipStreamSynchronize(c->side);
   for (auto& ep : c->pool) {
     (void)hipEventDestroy(ep.a);
     (void)hipEventDestroy(ep.b);
   }
   for (void* p : c->allocs) (void)hipFree(p);
+  if (c->evI) (void)hipEventDestroy(c->evI);
+  if (c->evB) (void)hipEventDestroy(c->evB);
+  if (c->evJ) (void)hipEventDestroy(c->evJ);
+  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -517,30 +602,32 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   // row pitch: a multiple of 64 elements so every row starts 256/512-byte aligned
   c->ld = (P->nx + 63) / 64 * 64;
   c->ps = (long long)c->ld * P->ny;
-  // launch-shape overrides (tuning sweeps): FDM_BULK in {tile, march, generic}, FDM_TILE_TY in
-  // {4, 8}, FDM_TILE_Y in {1, 2, 4} (per-point kernel), FDM_MARCH_H, FDM_XCD_SWIZZLE
+  // launch-shape overrides (tuning sweeps): FDM_BULK in {tile, generic}, FDM_TILE_TY in {4, 8},
+  // FDM_TILE_Y in {1, 2, 4} (per-point kernel), FDM_XCD_SWIZZLE in {0, 1}
   if (const char* e = getenv("FDM_TILE_Y")) {
     int v = atoi(e);
     if (v == 1 || v == 2 || v == 4) c->tile_y = v;
   }
   if (const char* e = getenv("FDM_BULK")) {
     if (!strcmp(e, "generic")) c->bulk_mode = 0;
-    else if (!strcmp(e, "march")) c->bulk_mode = 1;
     else if (!strcmp(e, "tile")) c->bulk_mode = 2;
   }
   if (const char* e = getenv("FDM_TILE_TY")) {
     int v = atoi(e);
     if (v == 4 || v == 8) c->tile_ty = v;
   }
-  if (const char* e = getenv("FDM_MARCH_H")) {
-    int v = atoi(e);
-    if (v >= 1 && v <= 4096) c->march_h = v;
-  }
   if (const char* e = getenv("FDM_XCD_SWIZZLE")) c->xcd_swizzle = atoi(e) != 0;
   (void)hipGetDevice(&c->device);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return set_err(nullptr, FDM_EDEVICE, "fdm_create: hipStreamCreate failed");
+  }
+  if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evI, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evB, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evJ, hipEventDisableTiming) != hipSuccess) {
+    fdm_destroy(c);
+    return set_err(nullptr, FDM_EDEVICE, "fdm_create: stream/event creation failed");
   }
   auto fail = [&](int rc) {
     g_create_error = c->err.empty() ? g_create_error : c->err;
@@ -620,22 +707,39 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   }
   int rc;
   if ((rc = dupload(c, &c->blk, blk.data(), blk.size() * sizeof(int)))) return fail(rc);
-  {
-    // row segments of the marching kernel: per block row, the first and last
-    // four rows (closures in r) and chunks of interior rows
-    std::vector<int4> segs;
-    for (int cy = 0; cy < c->nby; cy++) {
-      const fdm_block& B0 = P->blocks[blk[(size_t)cy * c->nbx]];
-      int my = B0.my, py = B0.py;
-      segs.push_back(make_int4(my, my + 3, cy, 1));
-      for (int j = my + 4; j <= py - 4; j += c->march_h)
-        segs.push_back(make_int4(j, std::min(j + c->march_h - 1, py - 4), cy, 0));
-      segs.push_back(make_int4(py - 3, py, cy, 1));
-    }
-    c->nsegs = (int)segs.size();
-    if ((rc = dupload(c, &c->segs, segs.data(), segs.size() * sizeof(int4)))) return fail(rc);
-  }
   if ((rc = dupload(c, &c->colb, colb.data(), colb.size() * sizeof(int)))) return fail(rc);
+  c->hcolb = colb;
+  c->hrowb = rowb;
+  c->hblk = blk;
+  {
+    // interior tiles: each block's rectangle [mx+4, px-4] x [my+4, py-4]
+    // covered from its corner, x fastest; band: every other point, row-major
+    std::vector<int4> tl[2];
+    for (int b = 0; b < c->nb; b++) {
+      const fdm_block& B = P->blocks[b];
+      const int ia = B.mx + 4, ib = B.px - 4, ja = B.my + 4, jb = B.py - 4;
+      if (ia > ib || ja > jb) continue;
+      c->ninterior += (long long)(ib - ia + 1) * (jb - ja + 1);
+      for (int q = 0; q < 2; q++) {
+        const int ty = q == 0 ? 4 : 8;
+        for (int j0 = ja; j0 <= jb; j0 += ty)
+          for (int i0 = ia; i0 <= ib; i0 += 64) tl[q].push_back(make_int4(b, i0, j0, 0));
+      }
+    }
+    std::vector<int2> band;
+    for (int j = 0; j < P->ny; j++)
+      for (int i = 0; i < P->nx; i++) {
+        const fdm_block& B = P->blocks[block_of(c, i, j)];
+        if (i - B.mx < 4 || B.px - i < 4 || j - B.my < 4 || B.py - j < 4) band.push_back(make_int2(i, j));
+      }
+    for (int q = 0; q < 2; q++) {
+      c->ntiles[q] = (int)tl[q].size();
+      if (c->ntiles[q] && (rc = dupload(c, &c->tiles[q], tl[q].data(), tl[q].size() * sizeof(int4))))
+        return fail(rc);
+    }
+    c->nband = (int)band.size();
+    if (c->nband && (rc = dupload(c, &c->band, band.data(), band.size() * sizeof(int2)))) return fail(rc);
+  }
   if ((rc = dupload(c, &c->rowb, rowb.data(), rowb.size() * sizeof(int)))) return fail(rc);
 
   // ---- sides ----
@@ -945,11 +1049,27 @@ int fdm_set_fields(fdm_ctx* c, const double* F, const double* DF) {
           h[l * pe + (size_t)j * c->ld + i] = DF ? DF[((size_t)l * c->ny + j) * c->nx + i] : 1e40;
     HIPCHK(c, hipMemcpy(c->DF, h.data(), h.size() * 8, hipMemcpyHostToDevice));
   } else {
+    // FP32-bulk: each block's fields are stored as deviations from the
+    // block mean of the fields handed in here, so that FP32 resolves the
+    // waves riding on a large prestress
+    std::vector<double> sum((size_t)c->nb * 6, 0.0), cnt(c->nb, 0.0);
+    for (int j = 0; j < c->ny; j++)
+      for (int i = 0; i < c->nx; i++) {
+        int b = block_of(c, i, j);
+        cnt[b] += 1.0;
+        for (int l = 0; l < 6; l++) sum[(size_t)b * 6 + l] += F[((size_t)l * c->ny + j) * c->nx + i];
+      }
+    for (int b = 0; b < c->nb; b++)
+      for (int l = 0; l < 6; l++)
+        c->hblocks[b].Fb[l] = cnt[b] > 0 ? (double)(float)(sum[(size_t)b * 6 + l] / cnt[b]) : 0.0;
+    HIPCHK(c, hipMemcpy(c->blocks, c->hblocks.data(), c->hblocks.size() * sizeof(BlockDev),
+                        hipMemcpyHostToDevice));
     std::vector<float> h(6 * pe, 0.f);
     for (int l = 0; l < 6; l++)
       for (int j = 0; j < c->ny; j++)
         for (int i = 0; i < c->nx; i++)
-          h[l * pe + (size_t)j * c->ld + i] = (float)F[((size_t)l * c->ny + j) * c->nx + i];
+          h[l * pe + (size_t)j * c->ld + i] =
+              (float)(F[((size_t)l * c->ny + j) * c->nx + i] - c->hblocks[block_of(c, i, j)].Fb[l]);
     HIPCHK(c, hipMemcpy(c->F[c->cur], h.data(), h.size() * 4, hipMemcpyHostToDevice));
     for (int l = 0; l < 6; l++)
       for (int j = 0; j < c->ny; j++)
@@ -981,8 +1101,11 @@ int fdm_get_fields(fdm_ctx* c, double* F, double* DF) {
       HIPCHK(c, hipMemcpy(h.data(), src, h.size() * 4, hipMemcpyDeviceToHost));
       for (int l = 0; l < 6; l++)
         for (int j = 0; j < c->ny; j++)
-          for (int i = 0; i < c->nx; i++)
-            dst[((size_t)l * c->ny + j) * c->nx + i] = (double)h[l * pe + (size_t)j * c->ld + i];
+          for (int i = 0; i < c->nx; i++) {
+            double v = (double)h[l * pe + (size_t)j * c->ld + i];
+            if (which == 0) v = v + c->hblocks[block_of(c, i, j)].Fb[l];
+            dst[((size_t)l * c->ny + j) * c->nx + i] = v;
+          }
     }
   }
   return check_fail(c);
@@ -992,8 +1115,8 @@ int fdm_init_edges(fdm_ctx* c, double t) {
   if (!c) return FDM_EINVAL;
   // prepare_edges + enforce_edge_conditions(D,0d0) (domain.f90:346-356)
   double Bload = c->ld_T.empty() ? 1.0 : load_ramp(t, c->ld_T[0]);
-  int rc = c->precision == FDM_FP64 ? launch_edges<double>(c, t, 0.0, 1.0, 0, 0, 0, Bload)
-                                    : launch_edges<float>(c, t, 0.0, 1.0, 0, 0, 0, Bload);
+  int rc = c->precision == FDM_FP64 ? launch_edges<double>(c, c->stream, t, 0.0, 1.0, 0, 0, 0, Bload)
+                                    : launch_edges<float>(c, c->stream, t, 0.0, 1.0, 0, 0, 0, Bload);
   if (rc) return rc;
   return check_fail(c);
 }
@@ -1007,8 +1130,7 @@ int fdm_step(fdm_ctx* c, double t, double dt, int64_t n_first, int32_t nsteps) {
 int fdm_stage1(fdm_ctx* c, double t, double dt, int64_t n) {
   if (!c) return FDM_EINVAL;
   double t0 = t + (double)(n - 1) * dt;
-  return c->precision == FDM_FP64 ? run_stage<double>(c, 1, t0, dt, 1, 1)
-                                  : run_stage<float>(c, 1, t0, dt, 1, 1);
+  return c->precision == FDM_FP64 ? stage1_impl<double>(c, t0, dt) : stage1_impl<float>(c, t0, dt);
 }
 
 static int iface_copy(fdm_ctx* c, int32_t ii, int32_t field, double* out, const double* in,
@@ -1077,8 +1199,10 @@ int fdm_profile_enable(fdm_ctx* c, int32_t enable) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->prof = enable != 0;
   c->pool_used = 0;
-  c->bulk_ms = c->edge_ms = 0.0;
-  c->bulk_n = c->edge_n = 0;
+  for (int k = 0; k < 3; k++) {
+    c->kind_ms[k] = 0.0;
+    c->kind_n[k] = 0;
+  }
   return FDM_OK;
 }
 
@@ -1088,14 +1212,16 @@ int fdm_profile_read(fdm_ctx* c, fdm_profile_t* out) {
   for (size_t k = 0; k < c->pool_used; k++) {
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->pool[k].a, c->pool[k].b));
-    if (c->pair_kind[k] == 0) c->bulk_ms += ms; else c->edge_ms += ms;
+    c->kind_ms[c->pair_kind[k]] += ms;
   }
   c->pool_used = 0;
-  out->bulk_ms = c->bulk_ms;
-  out->bulk_launches = c->bulk_n;
-  out->edge_ms = c->edge_ms;
-  out->edge_launches = c->edge_n;
-  out->bulk_points = (int64_t)c->nx * c->ny;
+  out->bulk_ms = c->kind_ms[0];
+  out->bulk_launches = c->kind_n[0];
+  out->edge_ms = c->kind_ms[1];
+  out->edge_launches = c->kind_n[1];
+  out->band_ms = c->kind_ms[2];
+  out->band_launches = c->kind_n[2];
+  out->bulk_points = c->bulk_mode == 2 ? (int64_t)c->ninterior : (int64_t)c->nx * c->ny;
   return FDM_OK;
 }
 

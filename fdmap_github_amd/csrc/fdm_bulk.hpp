@@ -60,6 +60,17 @@ __device__ __forceinline__ void tile_of(int swz, int& tx, int& ty) {
   ty = t / gridDim.x;
 }
 
+// Linear tile index of this workgroup in a 1-D grid: with the swizzle, the
+// workgroups dispatched to XCD x (ids congruent to x mod 8) take a
+// contiguous range of the tile list.
+__device__ __forceinline__ int linear_tile(int swz) {
+  const int bid = blockIdx.x;
+  if (!swz) return bid;
+  const int nwg = gridDim.x;
+  const int q = nwg >> 3, r = nwg & 7, x = bid & 7, k = bid >> 3;
+  return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // field coupling of rates_*_mode2 (rates.f90:411-444): DF(l) is built from
 // F(UA) paired with y-metrics and F(UB) paired with x-metrics; gamma
 // multiplies the A or the B product.
@@ -334,11 +345,19 @@ __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDe
         s[k] = (double)f[2 + k];
         Ds[k] = (double)d[2 + k];
       }
+      if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[k] = s[k] + B.Fb[2 + k];
+      }
       long long pc = (long long)j * a.nx + i;
       double lam = a.lambda[pc], gp = a.gammap[pc];
       plastic_point(B.G, B.K, B.mu, B.beta, B.b, B.h, B.eta, B.s0, a.dt, s, Ds, lam, gp);
       a.lambda[pc] = lam;
       a.gammap[pc] = gp;
+      if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) s[k] = s[k] - B.Fb[2 + k];
+      }
 #pragma unroll
       for (int k = 0; k < 4; k++) {
         f[2 + k] = (T)s[k];
@@ -509,23 +528,6 @@ __global__ void __launch_bounds__(256) bulk_kernel(BulkArgs<T> a) {
   point_generic<T, true>(a, i, j);
 }
 
-// neighbourhood of the marching kernel: q neighbours from the LDS row S
-// (F1..F5, xr, yr), r neighbours from the register window w (F1..F5, xq,
-// yq at rows j-2..j+2)
-template <typename T, int W>
-struct MarchAcc {
-  const T (*S)[W];
-  const T (&w)[7][5];
-  int c;
-  __device__ __forceinline__ T f(int k, int di, int dj) const {
-    return dj == 0 ? S[k][c + di] : w[k][2 + dj];
-  }
-  __device__ __forceinline__ T xr(int di) const { return S[5][c + di]; }
-  __device__ __forceinline__ T yr(int di) const { return S[6][c + di]; }
-  __device__ __forceinline__ T xq(int dj) const { return w[5][2 + dj]; }
-  __device__ __forceinline__ T yq(int dj) const { return w[6][2 + dj]; }
-};
-
 // global load at a 32-bit byte offset from a wave-uniform base, so that the
 // address is formed by the load itself (scalar base + vector offset)
 template <typename T>
@@ -533,9 +535,14 @@ __device__ __forceinline__ T ld_off(const T* base, unsigned boff) {
   return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
 }
 
-// neighbourhood of the tile kernel: everything from LDS.  sF: F1..F5 with a
-// halo of 2 in q and r; sR: xr, yr with a halo in q; sQ: xq, yq with a halo
-// in r.  (x, y): the lane's point in tile coordinates.
+template <typename T>
+__device__ __forceinline__ void st_off(T* base, unsigned boff, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
+}
+
+// neighbourhood of the interior kernel: everything from LDS.  sF: F1..F5
+// with a halo of 2 in q and r; sR: xr, yr with a halo in q; sQ: xq, yq with
+// a halo in r.  (x, y): the lane's point in tile coordinates.
 template <typename T, int TX, int TY>
 struct TileAcc {
   const T (*sF)[TY + 4][TX + 4];
@@ -549,16 +556,21 @@ struct TileAcc {
   __device__ __forceinline__ T yq(int dj) const { return sQ[1][y + 2 + dj][x]; }
 };
 
-// Tile bulk kernel.  A workgroup of 64 x TY lanes computes a 64 x TY tile.
-// All of the tile's loads are issued before one barrier: F1..F5 with a
-// halo of two points in q and r, xr/yr with the q halo and xq/yq with the r
-// halo go to LDS; the lane's own DF, F6 and Ji stay in registers.  Lanes
-// within four points of a block edge (one-sided closures, SAT) take
-// point_closure.  Halo indices are clamped into the grid; a clamped value
-// only ever reaches closure lanes, which do not read LDS.
+typedef const __attribute__((address_space(4))) BlockDev ConstBlockDev;
+
+// Interior kernel.  The grid walks a list of 64 x TY tiles that cover each
+// block's interior rectangle [mx+4, px-4] x [my+4, py-4] from its corner, so
+// every lane of a tile that falls inside the rectangle is an interior point
+// of the tile's block: no one-sided closure, no SAT, and the block index is
+// wave-uniform (its parameters are read as scalars through the constant
+// address space).  All of the tile's loads are issued before one barrier:
+// F1..F5 with a halo of two points in q and r, xr/yr with the q halo and
+// xq/yq with the r halo go to LDS; the lane's own DF, F6 and Ji stay in
+// registers.  Halo indices are clamped into the grid; a clamped value never
+// reaches an active lane.  tiles[k] = (block, i0, j0, 0).
 template <typename T, int TY, bool FULL>
 __global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4)))
-bulk_tile(BulkArgs<T> a) {
+bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
   constexpr int TX = 64, HX = TX + 4, HY = TY + 4, NT = TX * TY;
   constexpr int NF = (HY * HX + NT - 1) / NT;  // staging passes, F1..F5
   constexpr int NR = (TY * HX + NT - 1) / NT;  // xr, yr
@@ -566,9 +578,9 @@ bulk_tile(BulkArgs<T> a) {
   __shared__ T sF[5][HY][HX];
   __shared__ T sR[2][TY][HX];
   __shared__ T sQ[2][HY][TX];
-  int tbx, tby;
-  tile_of(a.xcd_swizzle, tbx, tby);
-  const int i0 = tbx * TX, j0 = tby * TY;
+  const int4 td = tiles[linear_tile(a.xcd_swizzle)];
+  const int bu = td.x, i0 = td.y, j0 = td.z;
+  ConstBlockDev* B4 = (ConstBlockDev*)a.blocks + bu;
   const int tid = threadIdx.y * TX + threadIdx.x;
   const int ld = a.ld, nx1 = a.nx - 1, ny1 = a.ny - 1;
   const long long ps = a.ps;
@@ -608,7 +620,7 @@ bulk_tile(BulkArgs<T> a) {
     }
   }
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
-  const bool valid = i <= nx1 && j <= ny1;
+  const bool valid = i <= B4->px - 4 && j <= B4->py - 4;
   const unsigned oc = (unsigned)(min(j, ny1) * ld + min(i, nx1)) * (unsigned)sizeof(T);
   T dfa[5];
   if (!a.a_zero) {
@@ -646,12 +658,6 @@ bulk_tile(BulkArgs<T> a) {
   }
   __syncthreads();
   if (!valid) return;
-  const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
-  const BlockDev& B = a.blocks[bi];
-  if (i - B.mx < 4 || B.px - i < 4 || j - B.my < 4 || B.py - j < 4) {
-    point_closure<T, FULL>(a, B, bi, i, j);
-    return;
-  }
   T d[6];
   if (a.a_zero) {
 #pragma unroll
@@ -663,161 +669,40 @@ bulk_tile(BulkArgs<T> a) {
     d[5] = (T)0.0;  // DF6 is overwritten by the szz relation
   }
   const TileAcc<T, TX, TY> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y};
-  interior_point<T>(d, acc, B, (T)B.gamma, Ji);
+  const T gamma = (T)B4->gamma;
+  interior_field<T, 0>(d, acc, gamma, (T)B4->cI[0][0], (T)B4->cI[0][1], Ji);
+  interior_field<T, 1>(d, acc, gamma, (T)B4->cI[1][0], (T)B4->cI[1][1], Ji);
+  interior_field<T, 2>(d, acc, gamma, (T)B4->cI[2][0], (T)B4->cI[2][1], Ji);
+  interior_field<T, 3>(d, acc, gamma, (T)B4->cI[3][0], (T)B4->cI[3][1], Ji);
+  interior_field<T, 4>(d, acc, gamma, (T)B4->cI[4][0], (T)B4->cI[4][1], Ji);
   T Fc[6];
 #pragma unroll
   for (int k = 0; k < 5; k++) Fc[k] = acc.f(k, 0, 0);
   Fc[5] = f6;
-  finish_point<T, FULL>(a, B, bi, i, j, (long long)j * ld + i, d, Fc);
+  if constexpr (FULL) {
+    finish_point<T, true>(a, a.blocks[bu], bu, i, j, (long long)j * ld + i, d, Fc);
+  } else {
+    // rates_szz (rates.f90:920-936), update_fields_interior (fields.f90:1225-1242)
+    const T nu = (T)B4->nu;
+    d[5] = nu * (d[2] + d[4]);
+    if (a.do_update) {
+      const T Bdt = (T)a.Bdt;
+#pragma unroll
+      for (int l = 0; l < 6; l++) st_off(a.Fout + l * ps, oc, Fc[l] + Bdt * d[l]);
+    }
+#pragma unroll
+    for (int l = 0; l < 6; l++) st_off(a.DF + l * ps, oc, d[l]);
+  }
 }
 
-// Row-marching bulk kernel.  A workgroup owns a strip of TX columns and
-// sweeps a segment of rows.  The arrays differentiated in r (F1..F5, xq,
-// yq) live in a five-row register window, so each is read once per row;
-// the arrays differentiated in q (F1..F5, xr, yr) of the current row are
-// exchanged through a double-buffered LDS row with a halo of 5 columns,
-// wide enough for the one-sided closures at block x-edges.  Segments that
-// touch the first or last four rows of a block (closures in r) use the
-// per-point path.  Each point's arithmetic is that of point_generic.
-// seg: x = first row, y = last row, z = block row, w = 1 for closure rows.
-template <typename T, int TX>
-__global__ void __launch_bounds__(TX) bulk_march(BulkArgs<T> a, const int4* __restrict__ segs) {
-  constexpr int HALO = 5;
-  constexpr int W = TX + 2 * HALO;
-  __shared__ T sh[2][7][W];
-  const int4 sg = segs[blockIdx.y];
-  const int t = threadIdx.x;
-  const int i = blockIdx.x * TX + t;
-  const bool valid = i < a.nx;
-  if (sg.w == 1) {
-    if (!valid) return;
-    for (int j = sg.x; j <= sg.y; j++) point_generic<T, true>(a, i, j);
-    return;
-  }
-  const long long ps = a.ps;
-  const int ld = a.ld;
-  const int ic = valid ? i : a.nx - 1;
-  const int bi = a.blk[sg.z * a.nbx + a.colb[ic]];
-  const BlockDev& B = a.blocks[bi];
-  const int dl = i - B.mx, dr = B.px - i;
-  const T* __restrict__ F = a.Fin;
-  const T* __restrict__ xq = a.xq;
-  const T* __restrict__ yq = a.yq;
-  const T* __restrict__ xr = a.xr;
-  const T* __restrict__ yr = a.yr;
-  const int i0 = blockIdx.x * TX - HALO;  // global column of LDS slot 0
-  int hslot = -1, hcol = 0;
-  if (t < 2 * HALO) {
-    hslot = (t < HALO) ? t : TX + t;
-    hcol = min(max(i0 + hslot, 0), a.nx - 1);
-  }
-  const int c = HALO + t;
-  const T gamma = (T)B.gamma, rho = (T)B.rho, G = (T)B.G, M = (T)B.M;
-  // window w[k][r]: array k (F1..F5, xq, yq) at row j-2+r
-  T w[7][5];
-#pragma unroll
-  for (int r = 0; r < 4; r++) {
-    const long long idx = (long long)(sg.x - 2 + r) * ld + ic;
-#pragma unroll
-    for (int k = 0; k < 5; k++) w[k][r] = F[k * ps + idx];
-    w[5][r] = xq[idx];
-    w[6][r] = yq[idx];
-  }
-  for (int j = sg.x; j <= sg.y; j++) {
-    const long long rowj = (long long)j * ld;
-    const long long cc = rowj + ic;
-    {
-      const long long idx = (long long)(j + 2) * ld + ic;
-#pragma unroll
-      for (int k = 0; k < 5; k++) w[k][4] = F[k * ps + idx];
-      w[5][4] = xq[idx];
-      w[6][4] = yq[idx];
-    }
-    const int buf = j & 1;
-#pragma unroll
-    for (int k = 0; k < 5; k++) sh[buf][k][c] = w[k][2];
-    sh[buf][5][c] = xr[cc];
-    sh[buf][6][c] = yr[cc];
-    if (hslot >= 0) {
-      const long long h = rowj + hcol;
-#pragma unroll
-      for (int k = 0; k < 5; k++) sh[buf][k][hslot] = F[k * ps + h];
-      sh[buf][5][hslot] = xr[h];
-      sh[buf][6][hslot] = yr[h];
-    }
-    __syncthreads();
-    if (valid) {
-      const T(*S)[W] = sh[buf];
-      T Fc[6];
-#pragma unroll
-      for (int k = 0; k < 5; k++) Fc[k] = w[k][2];
-      Fc[5] = F[5 * ps + cc];
-      T d[6];
-      if (a.a_zero) {
-#pragma unroll
-        for (int l = 0; l < 6; l++) d[l] = (T)0.0;
-      } else {
-        const T A = (T)a.A;
-#pragma unroll
-        for (int l = 0; l < 5; l++) d[l] = A * a.DF[l * ps + cc];
-        d[5] = (T)0.0;
-      }
-      // SAT on the block's left/right sides (rows here are not edge rows)
-      if (dl == 0) sat_term<T>(d, Fc, a.sd, B.side_off[FDM_SIDE_L] + (j - B.my));
-      if (dr == 0) sat_term<T>(d, Fc, a.sd, B.side_off[FDM_SIDE_R] + (j - B.my));
-      const T Ji = a.Ji[cc];
-      if (dl >= 4 && dr >= 4) {
-        const MarchAcc<T, W> acc{S, w, c};
-        interior_point<T>(d, acc, B, gamma, Ji);
-      } else {
-        // L or R strip (rates.f90:471-591): one-sided q, then central r
-#pragma unroll
-        for (int s = 0; s < 6; s++) {
-          const int k = (dl < 4) ? (B.mx + s - i0) : (B.px - s - i0);
-          const T Dq = (dl < 4) ? (T)kDL[dl][s] : (T)(-kDL[dr][s]);
-          const T ay = (T)0.5 * (S[6][k] + S[6][c]);
-          const T ax = (T)0.5 * (S[5][k] + S[5][c]);
-#define FDM_Q1(L)                                                                         \
-  {                                                                                       \
-    using P = Fld<L>;                                                                     \
-    T q = prod<T, P::ga>(ay, S[P::ua][k], gamma) - prod<T, P::gb>(ax, S[P::ub][k], gamma); \
-    d[L] = d[L] + bscale<T, P::mat>(q, Dq, rho, G, M) * Ji;                               \
-  }
-          FDM_Q1(0)
-          FDM_Q1(1)
-          FDM_Q1(2)
-          FDM_Q1(3)
-          FDM_Q1(4)
-#undef FDM_Q1
-        }
-#pragma unroll
-        for (int s = 1; s <= 2; s++) {
-          const T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
-          const T byp = (T)0.5 * (w[6][2 + s] + w[6][2]), bxp = (T)0.5 * (w[5][2 + s] + w[5][2]);
-          const T bym = (T)0.5 * (w[6][2 - s] + w[6][2]), bxm = (T)0.5 * (w[5][2 - s] + w[5][2]);
-#define FDM_RC(L)                                                                           \
-  {                                                                                         \
-    using P = Fld<L>;                                                                       \
-    T r = -prod<T, P::ga>(byp, w[P::ua][2 + s], gamma) + prod<T, P::gb>(bxp, w[P::ub][2 + s], gamma) + \
-          prod<T, P::ga>(bym, w[P::ua][2 - s], gamma) - prod<T, P::gb>(bxm, w[P::ub][2 - s], gamma);   \
-    d[L] = d[L] + bscale<T, P::mat>(r, D, rho, G, M) * Ji;                                  \
-  }
-          FDM_RC(0)
-          FDM_RC(1)
-          FDM_RC(2)
-          FDM_RC(3)
-          FDM_RC(4)
-#undef FDM_RC
-        }
-      }
-      finish_point<T, true>(a, B, bi, i, j, cc, d, Fc);
-    }
-#pragma unroll
-    for (int k = 0; k < 7; k++) {
-#pragma unroll
-      for (int r = 0; r < 4; r++) w[k][r] = w[k][r + 1];
-    }
-  }
+// Band kernel: the points within four points of their block's edge (one-
+// sided closures, SAT), one lane each, from a list of (i, j).
+template <typename T, bool FULL>
+__global__ void __launch_bounds__(256) bulk_band(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npts) return;
+  const int2 p = pts[k];
+  point_generic<T, FULL>(a, p.x, p.y);
 }
 
 }  // namespace fdm

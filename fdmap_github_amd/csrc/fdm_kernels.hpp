@@ -25,6 +25,9 @@ struct BlockDev {
   double mu, beta, b, h, eta, K;
   double s0[6];
   int side_off[4], side_n[4], side_bc[4];
+  // FP32-bulk stores each field as its deviation from this per-block
+  // constant (zero in FP64); totals are formed in FP64 where needed
+  double Fb[6];
 };
 
 struct IfaceDev {
@@ -102,9 +105,17 @@ __global__ void __launch_bounds__(64) edge_kernel(EdgeArgs<T> a) {
       F[l] = (double)a.F[l * a.ps + cc];
       F0[l] = a.sd.F0[l * NS + so];
     }
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+      for (int l = 0; l < 6; l++) F[l] = F[l] + B.Fb[l];
+    }
     double x = a.sd.x ? a.sd.x[so] : 0.0, y = a.sd.y ? a.sd.y[so] : 0.0;
     set_bc_mode2(Fh, F, F0, a.sd.normal[so], a.sd.normal[NS + so], B.side_bc[s], a.sd.M[so],
                  a.sd.M[NS + so], a.sd.M[2 * NS + so], x, y, a.t, B.id);
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+      for (int l = 0; l < 6; l++) Fh[l] = Fh[l] - B.Fb[l];
+    }
 #pragma unroll
     for (int l = 0; l < 6; l++) a.sd.Fhat[l * NS + so] = Fh[l];
     return;
@@ -136,6 +147,13 @@ __global__ void __launch_bounds__(64) edge_kernel(EdgeArgs<T> a) {
     Fp[l] = (double)a.F[l * a.ps + ccp];
     F0m[l] = a.sd.F0[l * NS + som];
     F0p[l] = a.sd.F0[l * NS + sop];
+  }
+  if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      Fm[l] = Fm[l] + Bm.Fb[l];
+      Fp[l] = Fp[l] + Bp.Fb[l];
+    }
   }
   double n1 = a.fi.nhat[q], n2 = a.fi.nhat[NI + q];
   double Zsm = a.sd.M[som], Zsp = a.sd.M[sop];
@@ -255,6 +273,13 @@ __global__ void __launch_bounds__(64) edge_kernel(EdgeArgs<T> a) {
       ST(FDM_IF_NREG) = ST(FDM_IF_NREG) + Bdt * ST(FDM_IF_DNREG);
     }
 #undef ST
+  }
+  if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      Fhm[l] = Fhm[l] - Bm.Fb[l];
+      Fhp[l] = Fhp[l] - Bp.Fb[l];
+    }
   }
 #pragma unroll
   for (int l = 0; l < 6; l++) {

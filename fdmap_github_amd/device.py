@@ -120,4 +120,5 @@ class Context:
         p = capi.fdm_profile_t()
         self._check(self._fn("profile_read")(self.h, C.byref(p)))
         return dict(bulk_ms=p.bulk_ms, bulk_launches=p.bulk_launches, edge_ms=p.edge_ms,
-                    edge_launches=p.edge_launches, bulk_points=p.bulk_points)
+                    edge_launches=p.edge_launches, bulk_points=p.bulk_points,
+                    band_ms=p.band_ms, band_launches=p.band_launches)

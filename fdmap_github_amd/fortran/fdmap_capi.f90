@@ -77,6 +77,8 @@ module fdmap_capi
      integer(c_int64_t) :: bulk_launches
      real(c_double) :: edge_ms
      integer(c_int64_t) :: edge_launches, bulk_points
+     real(c_double) :: band_ms
+     integer(c_int64_t) :: band_launches
   end type fdm_profile_t
 
   interface

@@ -1,8 +1,8 @@
-"""The three bulk kernels (LDS tile, row marching, per point) evaluate the
-same expression tree per point, so on the same inputs they must give
-bit-identical fields, rates and fault state, in FP64 and in FP32-bulk.
-Grids here are large enough for interior tiles, ragged tile edges in both
-directions, block boundaries inside a tile and several block columns.
+"""The bulk kernels (interior LDS tiles plus band points, or one lane per
+point everywhere) evaluate the same expression tree per point, so on the
+same inputs they must give bit-identical fields, rates and fault state, in
+FP64 and in FP32-bulk.  Grids here are large enough for interior tiles,
+ragged tile edges in both directions and several block columns.
 """
 import numpy as np
 import pytest
@@ -16,12 +16,12 @@ CS, CP, H = 3.464, 6.0, 0.025
 DT = 0.5 * H / np.sqrt(CS ** 2 + CP ** 2)
 BG = [0.0, 0.0, -120.0, 70.0, -120.0, -120.0]
 MODES = [("generic", {}), ("tile", {"FDM_TILE_TY": "8"}), ("tile", {"FDM_TILE_TY": "4"}),
-         ("tile", {"FDM_TILE_TY": "8", "FDM_XCD_SWIZZLE": "1"}), ("march", {"FDM_MARCH_H": "5"})]
+         ("tile", {"FDM_TILE_TY": "8", "FDM_XCD_SWIZZLE": "1"})]
 
 
 def run_mode(monkeypatch, prob, F, nsteps, mode, env):
     monkeypatch.setenv("FDM_BULK", mode)
-    for k in ("FDM_TILE_TY", "FDM_XCD_SWIZZLE", "FDM_MARCH_H"):
+    for k in ("FDM_TILE_TY", "FDM_XCD_SWIZZLE"):
         monkeypatch.delenv(k, raising=False)
     for k, v in env.items():
         monkeypatch.setenv(k, v)

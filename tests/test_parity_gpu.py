@@ -123,3 +123,28 @@ def test_plasticity(gpu):
     go, lo = orc.get_plastic()
     assert rel_l2(gd, go) <= 1e-12 and rel_l2(ld, lo) <= 1e-12
     assert go.max() > 0.0  # plastic flow actually occurred
+
+
+def test_fp32_bulk_error_against_fp64(gpu):
+    """FP32-bulk (fields FP32 as deviations from a per-block background,
+    friction FP64) against the FP64 oracle on a prestressed rupture: the
+    stated tolerance is relative L2 <= 1e-5 on fields, slip and slip velocity."""
+    kw = dict(nx=96, nys=(40, 40), seed=11, warp=0.03, background=BG, perturb=0.0, law="RFL",
+              fault_overstress=25.0)
+    p32, F = problems.build(precision="fp32", **kw)
+    p64, _ = problems.build(precision="fp64", **kw)
+    dev = Context(p32)
+    orc = OracleContext(p64, threads=4)
+    for c in (dev, orc):
+        c.set_fields(F)
+        c.init_edges(0.0)
+        c.step(0.0, DT, 1, 150)
+    Fd, _ = dev.get_fields()
+    Fo, _ = orc.get_fields()
+    e = {"fields": rel_l2(Fd, Fo)}
+    for f in ("D", "V", "S"):
+        e[f] = rel_l2(dev.get_iface(0, f), orc.get_iface(0, f))
+    print("FP32-bulk vs FP64 relative L2", e)
+    assert orc.get_iface(0, "V").max() > 1e-3  # rupture under way
+    for k, v in e.items():
+        assert v <= 1e-5, (k, v)
