@@ -29,6 +29,9 @@ HBM_PEAK_GBS = 8000.0
 # DF(6) + F(6), read 5 metric arrays; the first stage does not read DF
 # (A1 = 0).  FP64: 232 B (184 B at stage 1); FP32-bulk: half.
 BYTES_STAGE = {"fp64": (184.0, 232.0), "fp32": (92.0, 116.0)}
+# what the interior kernel itself must move: DF6 is neither read nor stored
+# (no stage needs it), so 176 B at stage 1 and 216 B after (FP64)
+KERNEL_BYTES_STAGE = {"fp64": (176.0, 216.0), "fp32": (88.0, 108.0)}
 
 
 def rel_l2(a, b):
@@ -93,7 +96,7 @@ def measured_traffic():
     """HBM bytes per bulk launch from the committed counter summary (two
     rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, each scaled by what the
     counter reads for a plain copy of known size)."""
-    path = os.path.join(ROOT, "profiles", "bulk_tile_c2_fp64_traffic.md")
+    path = os.path.join(ROOT, "profiles", "bulk_interior_c2_fp64_traffic.md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -101,6 +104,131 @@ def measured_traffic():
             if line.startswith("hbm_bytes_per_launch:"):
                 return float(line.split(":", 1)[1])
     return None
+
+
+def run_c3(args, tmp):
+    """Rough fault: a seeded band-limited self-similar profile (rms amplitude
+    1e-2 of the fault length, wavelengths from 20 h to the fault length) as
+    the shared edge of two curvilinear 4001 x 2001 blocks, C2's friction and
+    prestress, Drucker-Prager plasticity (cohesion 0, friction 0.6); FP64 and
+    FP32-bulk, the FP32 error stated against FP64."""
+    from fdmap_github_amd import inputs
+
+    nx = 4001 if args.nx == 1601 else args.nx
+    ny = 2001 if args.ny == 801 else args.ny
+    steps = 3000 if args.steps == 4000 else args.steps
+    h = 0.025
+    x, y, n1, n2 = inputs.fractal_profile(nx, h, alpha=1e-2, seed=3)
+    curve = os.path.join(tmp, "fault.bin")
+    inputs.write_curve(curve, x, y, n1, n2)
+    inpath = inputs.write(os.path.join(tmp, "c3.in"),
+                          inputs.c2_planar_rs(name=os.path.join(tmp, "c3"), nx=nx, ny=ny, h=h,
+                                              nsteps=steps + args.warmup + args.prof_steps,
+                                              outputs=False, plastic=True, curve=curve))
+    r64 = run_device(inpath, "fp64", steps, args.warmup, args.prof_steps)
+    r32 = run_device(inpath, "fp32", steps, args.warmup, args.prof_steps)
+    npts = r64["npts"]
+    ups64 = npts * steps / r64["el"]
+    ups32 = npts * steps / r32["el"]
+    out = {
+        "metric": METRIC, "value": ups64, "unit": "updates/s", "n_gpus": 1, "steps": steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * r64["el"] / steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (rough-fault problem generated as an FDMAP .in file and curve file)",
+        "config": {"workload": f"rough curvilinear fault with plasticity, 2 blocks x ({nx}x{ny}), "
+                               "self-similar profile alpha 1e-2 seed 3, RFL slip law, "
+                               "Drucker-Prager mu 0.6 b 0, LSRK(5,4) CFL 0.5",
+                   "grid_points": npts, "fault_nodes": nx},
+        "fp32_bulk": {
+            "updates_per_s": ups32, "speedup_vs_fp64": ups32 / ups64,
+            "ms_per_step": 1e3 * r32["el"] / steps,
+            "rel_l2_fields_vs_fp64": rel_l2(r32["F"], r64["F"]),
+            "rel_l2_slip_vs_fp64": rel_l2(r32["fault"]["D"], r64["fault"]["D"]),
+            "rel_l2_slip_velocity_vs_fp64": rel_l2(r32["fault"]["V"], r64["fault"]["V"]),
+            "after_steps": steps + args.warmup,
+        },
+        "avg_launch_ms": {prec: {k: r["prof"][k + "_ms"] / max(1, r["prof"][k + "_launches"])
+                                 for k in ("bulk", "band", "edge")}
+                          for prec, r in (("fp64", r64), ("fp32", r32))},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def run_c4(args, tmp):
+    """Ensemble: scenarios with block sizes uniform in 401..2001 x 201..1001,
+    lognormal fault shear prestress (mean 70 MPa, log sd 0.1, correlation
+    length 1 km), uniform a in [0.008, 0.012]; FP32 fields, FP64 friction;
+    tile shapes from the autotuner; every RK step of all scenarios is one
+    hipGraph launch."""
+    import collections
+
+    from fdmap_github_amd import capi, ensemble, host
+    from fdmap_github_amd.device import Context
+
+    steps = 2000 if args.steps == 4000 else args.steps
+    scen = ensemble.c4_scenarios(count=args.scenarios, seed=0)
+    cache = ensemble.load_cache(args.tune_cache)
+    members, t, dt, npts, chosen = [], [], [], 0, collections.Counter()
+    t_setup = time.perf_counter()
+    for sc in scen:
+        path = ensemble.write_scenario(tmp, sc, steps + args.warmup + 200)
+        hp = host.HostProblem(path, precision="fp32")
+
+        def make(hp=hp):
+            c = Context(hp.ptr)
+            c.set_fields(hp.fields())
+            c.init_edges(hp.t)
+            return c
+
+        key = "fp32:%d:%d" % ensemble.size_class(hp.nx, hp.ny // 2)
+        tile = ensemble.tune(make, key, cache)
+        c = make()
+        ensemble.apply_tile(c, tile)
+        chosen[capi.TILE_SHAPES[tile]] += 1
+        members.append(c)
+        t.append(hp.t)
+        dt.append(hp.dt)
+        npts += hp.nx * hp.ny
+        print(f"scenario {sc['index']}: {hp.nx} x {hp.ny // 2} x 2, tile {capi.TILE_SHAPES[tile]}",
+              file=sys.stderr, flush=True)
+    ensemble.save_cache(args.tune_cache, cache)
+    t_setup = time.perf_counter() - t_setup
+    ens = ensemble.Ensemble(members)
+    ens.step(t, dt, 1, args.warmup)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.step(t, dt, args.warmup + 1, steps)
+    ens.sync()
+    el = time.perf_counter() - t0
+    # the same members stepped one call per member and step, no graph
+    eager_steps = min(100, steps)
+    n0 = args.warmup + steps + 1
+    t0 = time.perf_counter()
+    for k in range(eager_steps):
+        for m, c in enumerate(members):
+            c.step(t[m], dt[m], n0 + k, 1)
+    for c in members:
+        c.sync()
+    el_eager = time.perf_counter() - t0
+    vmax = max(float(np.abs(c.get_iface(0, "V")).max()) for c in members)
+    out = {
+        "metric": METRIC, "value": npts * steps / el, "unit": "updates/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": 1e3 * el / steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 fields, f64 friction",
+        "data": "synthetic (seeded scenarios generated as FDMAP .in, stress and state files)",
+        "config": {"workload": f"ensemble of {len(members)} two-block rate-and-state ruptures, "
+                               "blocks uniform in 401..2001 x 201..1001, lognormal prestress, "
+                               "uniform a, FP32-bulk, one hipGraph launch per RK step",
+                   "grid_points": npts, "scenarios": len(members)},
+        "tile_choices": dict(chosen),
+        "eager_updates_per_s": npts * eager_steps / el_eager,
+        "graph_speedup_vs_eager": (npts * steps / el) / (npts * eager_steps / el_eager),
+        "max_slip_velocity": vmax,
+        "setup_s": t_setup,
+    }
+    print(json.dumps(out), flush=True)
+    ens.close()
 
 
 def main():
@@ -115,6 +243,12 @@ def main():
                     help="steps timed after the headline ones with HIP events per launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--config", choices=("c2", "c3", "c4"), default="c2",
+                    help="c2: planar rupture (the headline); c3: rough curvilinear fault with "
+                         "off-fault plasticity, 2 x (4001x2001), FP64 and FP32-bulk; c4: 64 "
+                         "uneven rupture scenarios advanced as one graph launch per step")
+    ap.add_argument("--scenarios", type=int, default=64)
+    ap.add_argument("--tune-cache", default="fdmap_tuning.json")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -124,6 +258,12 @@ def main():
     from fdmap_github_amd import inputs
 
     tmp = tempfile.mkdtemp(prefix="fdmap_bench_")
+    if args.config == "c3":
+        run_c3(args, tmp)
+        return
+    if args.config == "c4":
+        run_c4(args, tmp)
+        return
     inpath = inputs.write(os.path.join(tmp, "c2.in"),
                           inputs.c2_planar_rs(name=os.path.join(tmp, "c2"), nx=args.nx,
                                               ny=args.ny,
@@ -171,6 +311,8 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "alg_bytes_per_launch": bytes_launch,
+            "kernel_min_bytes_per_launch": kpts * (KERNEL_BYTES_STAGE["fp64"][0]
+                                                   + (ns - 1) * KERNEL_BYTES_STAGE["fp64"][1]) / ns,
             "avg_launch_ms": avg_ms,
             "launches": prof["bulk_launches"],
             "ms_per_step_with_events": 1e3 * r64["el_prof"] / args.prof_steps,

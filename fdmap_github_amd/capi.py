@@ -165,13 +165,33 @@ ENTRY_POINTS = [
 ]
 
 
+# entry points declared in include/fdmap_ensemble.h (the MI355X library only)
+FDM_TUNE_TILE = 0
+FDM_TUNE_SWIZZLE = 1
+TILE_SHAPES = ["64x4", "64x8", "128x4"]
+ENSEMBLE_ENTRY_POINTS = [
+    ("fdm_tune", C.c_int, [_P, C.c_int32, C.c_int32]),
+    ("fdm_ensemble_create", C.c_int, [C.POINTER(_P), C.c_int32, C.POINTER(_P)]),
+    ("fdm_ensemble_step", C.c_int, [_P, _dp, _dp, C.c_int64, C.c_int32]),
+    ("fdm_ensemble_sync", C.c_int, [_P]),
+    ("fdm_ensemble_destroy", None, [_P]),
+    ("fdm_ensemble_last_error", C.c_char_p, [_P]),
+]
+
+
 def bind(lib, prefix_from="fdm_", prefix_to="fdm_"):
     """Attach restype/argtypes to every entry point of a library that
-    implements the fdmap.h interface (optionally under another prefix)."""
+    implements the fdmap.h interface (optionally under another prefix), and
+    to the ensemble entry points where the library has them."""
     for name, res, args in ENTRY_POINTS:
         fn = getattr(lib, prefix_to + name[len(prefix_from):])
         fn.restype = res
         fn.argtypes = args
+    for name, res, args in ENSEMBLE_ENTRY_POINTS:
+        if hasattr(lib, name):
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
     return lib
 
 

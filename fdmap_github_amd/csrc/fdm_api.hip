@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "fdm_bulk.hpp"
+#include "../../include/fdmap_ensemble.h"
 
 namespace fdm {
 
@@ -133,6 +134,9 @@ namespace {
 
 std::string g_create_error;
 
+// interior tile shapes (columns x rows) selectable with FDM_TILE
+const int kTileShape[3][2] = {{64, 4}, {64, 8}, {128, 4}};
+
 struct EventPair {
   hipEvent_t a, b;
 };
@@ -146,6 +150,7 @@ struct DevBuf {
 
 struct fdm_ctx {
   int nx = 0, ny = 0, ld = 0, nb = 0, ni = 0, nbx = 0, nby = 0, nstage = 0;
+  int coff = 0;      // column offset of the stored planes (see fdm_create)
   long long ps = 0;  // plane stride in elements
   int precision = FDM_FP64;
   size_t esz = 8;
@@ -219,14 +224,21 @@ struct fdm_ctx {
   // launch shape of the bulk kernel (tile 64 x tile_y points)
   int tile_y = 4;
   int xcd_swizzle = 1;
-  // bulk kernel: 0 per-point, 2 interior tiles (64 x tile_ty) + band points
+  // bulk kernel: 0 per-point, 2 interior tiles (kTileShape[tile_shape]) + band points
   int bulk_mode = 2;
-  int tile_ty = 8;
-  // interior tiles (block, i0, j0, 0) for tile heights 4 and 8; band points (i, j)
-  int4* tiles[2] = {nullptr, nullptr};
-  int ntiles[2] = {0, 0};
+  int tile_shape = 0;  // index into kTileShape
+  // interior tiles (block, i0, j0, 0) for the tile shapes 64x4, 64x8, 128x4; band points (i, j)
+  int4* tiles[3] = {nullptr, nullptr, nullptr};
+  int ntiles[3] = {0, 0, 0};
   int2* band = nullptr;
   int nband = 0;
+  // the latest interior launch was a lean one, which leaves DF6 unwritten at
+  // interior points
+  bool df6_stale = false;
+  // graph capture (ensembles): stage times come from the device step counter
+  long long* nstep_dev = nullptr;
+  bool graph_mode = false;
+  double g_tinit = 0.0, g_dt = 0.0, g_rkc = 0.0;
   long long ninterior = 0;
 };
 
@@ -359,6 +371,10 @@ int launch_edges(fdm_ctx* c, hipStream_t s, double t, double Bdt, double A, int 
   a.fi = IfacesDev{c->ifaces, c->inhat, c->irs, c->isw, c->iS0b, c->iN0b, c->ildS, c->ildN,
                    c->iout, c->ist, c->ist1, c->NI};
   a.t = t;
+  a.nstep = c->graph_mode ? c->nstep_dev : nullptr;
+  a.t_init = c->g_tinit;
+  a.dt = c->g_dt;
+  a.rkc = c->g_rkc;
   a.Bdt = Bdt;
   a.A = A;
   a.Bload = Bload;
@@ -430,37 +446,39 @@ template <typename T>
 void launch_band(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   if (!c->nband) return;
   EventPair* ep = prof_begin(c, 2, s);
-  dim3 grid((c->nband + 255) / 256);
+  dim3 grid((c->nband + 63) / 64);
   if (full_variant(c, a))
-    hipLaunchKernelGGL((bulk_band<T, true>), grid, dim3(256), 0, s, a, c->band, c->nband);
+    hipLaunchKernelGGL((bulk_band<T, true>), grid, dim3(64), 0, s, a, c->band, c->nband);
   else
-    hipLaunchKernelGGL((bulk_band<T, false>), grid, dim3(256), 0, s, a, c->band, c->nband);
+    hipLaunchKernelGGL((bulk_band<T, false>), grid, dim3(64), 0, s, a, c->band, c->nband);
   prof_end(c, ep, 2, s);
 }
 
 template <typename T>
 void launch_interior(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
-  const int q = c->tile_ty == 4 ? 0 : 1;
+  const int q = c->tile_shape;
   if (!c->ntiles[q]) return;
   EventPair* ep = prof_begin(c, 0, s);
   const bool full = full_variant(c, a);
+  c->df6_stale = !full;
   dim3 grid(c->ntiles[q]);
+  dim3 block(kTileShape[q][0], kTileShape[q][1]);
   if (q == 0) {
-    if (full)
-      hipLaunchKernelGGL((bulk_interior<T, 4, true>), grid, dim3(64, 4), 0, s, a, c->tiles[0]);
-    else
-      hipLaunchKernelGGL((bulk_interior<T, 4, false>), grid, dim3(64, 4), 0, s, a, c->tiles[0]);
+    if (full) hipLaunchKernelGGL((bulk_interior<T, 64, 4, true>), grid, block, 0, s, a, c->tiles[q]);
+    else hipLaunchKernelGGL((bulk_interior<T, 64, 4, false>), grid, block, 0, s, a, c->tiles[q]);
+  } else if (q == 1) {
+    if (full) hipLaunchKernelGGL((bulk_interior<T, 64, 8, true>), grid, block, 0, s, a, c->tiles[q]);
+    else hipLaunchKernelGGL((bulk_interior<T, 64, 8, false>), grid, block, 0, s, a, c->tiles[q]);
   } else {
-    if (full)
-      hipLaunchKernelGGL((bulk_interior<T, 8, true>), grid, dim3(64, 8), 0, s, a, c->tiles[1]);
-    else
-      hipLaunchKernelGGL((bulk_interior<T, 8, false>), grid, dim3(64, 8), 0, s, a, c->tiles[1]);
+    if (full) hipLaunchKernelGGL((bulk_interior<T, 128, 4, true>), grid, block, 0, s, a, c->tiles[q]);
+    else hipLaunchKernelGGL((bulk_interior<T, 128, 4, false>), grid, block, 0, s, a, c->tiles[q]);
   }
   prof_end(c, ep, 0, s);
 }
 
 template <typename T>
 void launch_pointwise(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
+  c->df6_stale = false;
   EventPair* ep = prof_begin(c, 0, s);
   dim3 block(64, c->tile_y);
   dim3 grid((c->nx + 63) / 64, (c->ny + c->tile_y - 1) / c->tile_y);
@@ -492,6 +510,7 @@ int run_stage(fdm_ctx* c, int stage, double t0, double dt, int final_step, int s
   double t = t0 + c->rkC[stage - 1] * dt;  // D%t during this stage (time_step.f90:137)
   double Bload = c->ld_T.empty() ? 1.0 : load_ramp(t, c->ld_T[0]);
   double srcA = c->src.nplace ? stf_amplitude(c->src, t) : 0.0;
+  c->g_rkc = c->rkC[stage - 1];
   int last = (stage == c->nstage);
   const BulkArgs<T> a =
       bulk_args<T>(c, A, Bdt, final_step ? 0 : 1, t, srcA, (!final_step && last) ? 1 : 0, dt);
@@ -537,6 +556,29 @@ int stage1_impl(fdm_ctx* c, double t0, double dt) {
   if (!rc) rc = run_stage<T>(c, 1, t0, dt, 1, 1);
   int rj = join_side(c);
   return rc ? rc : rj;
+}
+
+// DF6 = nu (DF3 + DF5) at the interior points of every block, in the
+// precision of the bulk fields: the value the lean interior kernel computes
+// and does not store (rates.f90:920-936)
+template <typename T>
+void restore_df6(const fdm_ctx* c, T* h) {
+  const size_t pe = (size_t)c->ps;
+  for (int b = 0; b < c->nb; b++) {
+    const BlockDev& B = c->hblocks[b];
+    const T nu = (T)B.nu;
+    for (int j = B.my + 4; j <= B.py - 4; j++)
+      for (int i = B.mx + 4; i <= B.px - 4; i++) {
+        const size_t k = (size_t)j * c->ld + i;
+        h[5 * pe + k] = nu * (h[2 * pe + k] + h[4 * pe + k]);
+      }
+  }
+}
+
+// one step of a member context inside a graph capture: advance the device
+// step counter after the stages
+__global__ void __launch_bounds__(64) step_counter_kernel(long long* n) {
+  if (threadIdx.x == 0) n[0] = n[0] + 1;
 }
 
 int check_fail(fdm_ctx* c) {
@@ -600,9 +642,24 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   }
   for (int s = 0; s <= P->nstage; s++) c->rkC[s] = P->rkC[s];
   // row pitch: a multiple of 64 elements so every row starts 256/512-byte aligned
-  c->ld = (P->nx + 63) / 64 * 64;
+  // Column i of a plane is stored at element coff + i, with coff chosen so
+  // that the interior tiles of the first block column (which start at column
+  // mx + 4) begin on a 512-byte boundary; every plane pointer below already
+  // includes the offset.
+  {
+    int mx0 = P->nblocks > 0 ? P->blocks[0].mx : 0;
+    for (int b = 1; b < P->nblocks; b++) mx0 = std::min(mx0, P->blocks[b].mx);
+    c->coff = (64 - (mx0 + 4) % 64) % 64;
+  }
+  c->ld = (P->nx + c->coff + 63) / 64 * 64;
   c->ps = (long long)c->ld * P->ny;
-  // launch-shape overrides (tuning sweeps): FDM_BULK in {tile, generic}, FDM_TILE_TY in {4, 8},
+  if (const char* e = getenv("FDM_PLANE_PAD")) {
+    // extra elements between planes (a multiple of 64 keeps rows aligned)
+    long long v = atoll(e);
+    if (v >= 0 && v <= (1 << 20)) c->ps += v / 64 * 64;
+  }
+  // launch-shape overrides (tuning sweeps): FDM_BULK in {tile, generic}, FDM_TILE in
+  // {64x4, 64x8, 128x4},
   // FDM_TILE_Y in {1, 2, 4} (per-point kernel), FDM_XCD_SWIZZLE in {0, 1}
   if (const char* e = getenv("FDM_TILE_Y")) {
     int v = atoi(e);
@@ -612,9 +669,12 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     if (!strcmp(e, "generic")) c->bulk_mode = 0;
     else if (!strcmp(e, "tile")) c->bulk_mode = 2;
   }
-  if (const char* e = getenv("FDM_TILE_TY")) {
-    int v = atoi(e);
-    if (v == 4 || v == 8) c->tile_ty = v;
+  if (const char* e = getenv("FDM_TILE")) {
+    for (int q = 0; q < 3; q++) {
+      char name[16];
+      snprintf(name, sizeof name, "%dx%d", kTileShape[q][0], kTileShape[q][1]);
+      if (!strcmp(e, name)) c->tile_shape = q;
+    }
   }
   if (const char* e = getenv("FDM_XCD_SWIZZLE")) c->xcd_swizzle = atoi(e) != 0;
   (void)hipGetDevice(&c->device);
@@ -714,16 +774,16 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   {
     // interior tiles: each block's rectangle [mx+4, px-4] x [my+4, py-4]
     // covered from its corner, x fastest; band: every other point, row-major
-    std::vector<int4> tl[2];
+    std::vector<int4> tl[3];
     for (int b = 0; b < c->nb; b++) {
       const fdm_block& B = P->blocks[b];
       const int ia = B.mx + 4, ib = B.px - 4, ja = B.my + 4, jb = B.py - 4;
       if (ia > ib || ja > jb) continue;
       c->ninterior += (long long)(ib - ia + 1) * (jb - ja + 1);
-      for (int q = 0; q < 2; q++) {
-        const int ty = q == 0 ? 4 : 8;
+      for (int q = 0; q < 3; q++) {
+        const int tx = kTileShape[q][0], ty = kTileShape[q][1];
         for (int j0 = ja; j0 <= jb; j0 += ty)
-          for (int i0 = ia; i0 <= ib; i0 += 64) tl[q].push_back(make_int4(b, i0, j0, 0));
+          for (int i0 = ia; i0 <= ib; i0 += tx) tl[q].push_back(make_int4(b, i0, j0, 0));
       }
     }
     std::vector<int2> band;
@@ -732,7 +792,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
         const fdm_block& B = P->blocks[block_of(c, i, j)];
         if (i - B.mx < 4 || B.px - i < 4 || j - B.my < 4 || B.py - j < 4) band.push_back(make_int2(i, j));
       }
-    for (int q = 0; q < 2; q++) {
+    for (int q = 0; q < 3; q++) {
       c->ntiles[q] = (int)tl[q].size();
       if (c->ntiles[q] && (rc = dupload(c, &c->tiles[q], tl[q].data(), tl[q].size() * sizeof(int4))))
         return fail(rc);
@@ -996,22 +1056,27 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   size_t pbytes = (size_t)c->ps * c->esz;
   // the tile kernel addresses a plane with 32-bit byte offsets
   if (pbytes >= ((size_t)1 << 31) && c->bulk_mode == 2) c->bulk_mode = 0;
-  for (int q = 0; q < 2; q++)
-    if ((rc = dalloc(c, &c->F[q], 6 * pbytes))) return fail(rc);
-  if ((rc = dalloc(c, &c->DF, 6 * pbytes))) return fail(rc);
-  HIPCHK(c, hipMemset(c->F[0], 0, 6 * pbytes));
-  HIPCHK(c, hipMemset(c->F[1], 0, 6 * pbytes));
-  HIPCHK(c, hipMemset(c->DF, 0, 6 * pbytes));
+  // one arena of 23 planes with a common stride: F (two buffers of 6), DF
+  // (6), metrics xq xr yq yr Ji (5)
+  void* arena = nullptr;
+  const size_t slack = 64 * c->esz;  // the last plane extends coff elements past 23 * ps
+  if ((rc = dalloc(c, &arena, 23 * pbytes + slack))) return fail(rc);
+  HIPCHK(c, hipMemset(arena, 0, 23 * pbytes + slack));
+  char* ab = (char*)arena + (size_t)c->coff * c->esz;
+  c->F[0] = ab;
+  c->F[1] = ab + 6 * pbytes;
+  c->DF = ab + 12 * pbytes;
+  for (int q = 0; q < 5; q++) c->met[q] = ab + (18 + q) * pbytes;
   const double* mets[5] = {P->xq, P->xr, P->yq, P->yr, P->Ji};
   for (int q = 0; q < 5; q++) {
     if (c->precision == FDM_FP64) {
       std::vector<double> h;
       pack_plane(h, mets[q], c->nx, c->ny, c->ld);
-      if ((rc = dupload(c, &c->met[q], h.data(), h.size() * 8))) return fail(rc);
+      HIPCHK(c, hipMemcpy(c->met[q], h.data(), h.size() * 8, hipMemcpyHostToDevice));
     } else {
       std::vector<float> h;
       pack_plane(h, mets[q], c->nx, c->ny, c->ld);
-      if ((rc = dupload(c, &c->met[q], h.data(), h.size() * 4))) return fail(rc);
+      HIPCHK(c, hipMemcpy(c->met[q], h.data(), h.size() * 4, hipMemcpyHostToDevice));
     }
   }
   for (int b = 0; b < c->nb; b++) c->any_mms |= (P->blocks[b].mms != 0);
@@ -1036,6 +1101,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
 
 int fdm_set_fields(fdm_ctx* c, const double* F, const double* DF) {
   if (!c || !F) return FDM_EINVAL;
+  c->df6_stale = false;
   size_t pe = (size_t)c->ps;
   if (c->precision == FDM_FP64) {
     std::vector<double> h(6 * pe, 0.0);
@@ -1093,12 +1159,14 @@ int fdm_get_fields(fdm_ctx* c, double* F, double* DF) {
     if (c->precision == FDM_FP64) {
       std::vector<double> h(6 * pe);
       HIPCHK(c, hipMemcpy(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost));
+      if (which == 1 && c->df6_stale) restore_df6(c, h.data());
       for (int l = 0; l < 6; l++)
         for (int j = 0; j < c->ny; j++)
           memcpy(dst + ((size_t)l * c->ny + j) * c->nx, &h[l * pe + (size_t)j * c->ld], c->nx * 8);
     } else {
       std::vector<float> h(6 * pe);
       HIPCHK(c, hipMemcpy(h.data(), src, h.size() * 4, hipMemcpyDeviceToHost));
+      if (which == 1 && c->df6_stale) restore_df6(c, h.data());
       for (int l = 0; l < 6; l++)
         for (int j = 0; j < c->ny; j++)
           for (int i = 0; i < c->nx; i++) {
@@ -1222,6 +1290,201 @@ int fdm_profile_read(fdm_ctx* c, fdm_profile_t* out) {
   out->band_ms = c->kind_ms[2];
   out->band_launches = c->kind_n[2];
   out->bulk_points = c->bulk_mode == 2 ? (int64_t)c->ninterior : (int64_t)c->nx * c->ny;
+  return FDM_OK;
+}
+
+int fdm_tune(fdm_ctx* c, int32_t knob, int32_t value) {
+  if (!c) return FDM_EINVAL;
+  if (knob == FDM_TUNE_TILE && value >= 0 && value < 3) {
+    c->tile_shape = value;
+    return FDM_OK;
+  }
+  if (knob == FDM_TUNE_SWIZZLE && (value == 0 || value == 1)) {
+    c->xcd_swizzle = value;
+    return FDM_OK;
+  }
+  return set_err(c, FDM_EINVAL, "fdm_tune: unknown knob or value");
+}
+
+}  // extern "C"
+
+struct fdm_ensemble {
+  std::vector<fdm_ctx*> m;
+  hipStream_t master = nullptr;
+  hipEvent_t fork = nullptr, done = nullptr;  // capture fork; end of eager launches
+  std::vector<hipEvent_t> join, pre;           // capture joins; eager ordering
+  hipGraphExec_t exec[2] = {nullptr, nullptr};  // one RK step from F buffer 0 / 1
+  std::vector<double> t, dt;                     // member times the graphs were captured for
+  int flip = 0;                                  // buffer parity change per step
+  std::string err;
+};
+
+namespace {
+
+int ens_err(fdm_ensemble* e, int rc, const std::string& msg) {
+  if (e) e->err = msg;
+  return rc;
+}
+
+#define ENSCHK(e, call)                                                              \
+  do {                                                                               \
+    hipError_t _st = (call);                                                         \
+    if (_st != hipSuccess)                                                           \
+      return ens_err(e, FDM_EDEVICE, std::string(#call ": ") + hipGetErrorString(_st)); \
+  } while (0)
+
+// Capture one RK step of every member (each from its current F buffer) into
+// one graph: the members' two streams fork off the master stream and join
+// back; the stage kernels are exactly those of fdm_step.
+int ens_capture(fdm_ensemble* e, int which) {
+  hipGraph_t g = nullptr;
+  ENSCHK(e, hipStreamBeginCapture(e->master, hipStreamCaptureModeRelaxed));
+  int rc = FDM_OK;
+  if (hipEventRecord(e->fork, e->master) != hipSuccess) rc = FDM_EDEVICE;
+  for (size_t k = 0; k < e->m.size() && !rc; k++) {
+    fdm_ctx* c = e->m[k];
+    c->graph_mode = true;
+    c->g_tinit = e->t[k];
+    c->g_dt = e->dt[k];
+    if (hipStreamWaitEvent(c->stream, e->fork, 0) != hipSuccess) rc = FDM_EDEVICE;
+    if (!rc) rc = fork_side(c);
+    for (int s = 1; s <= c->nstage && !rc; s++)
+      rc = c->precision == FDM_FP64 ? run_stage<double>(c, s, e->t[k], e->dt[k], 0, s == 1)
+                                    : run_stage<float>(c, s, e->t[k], e->dt[k], 0, s == 1);
+    if (!rc) rc = join_side(c);
+    if (!rc) hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(64), 0, c->stream, c->nstep_dev);
+    if (!rc && hipEventRecord(e->join[k], c->stream) != hipSuccess) rc = FDM_EDEVICE;
+    if (!rc && hipStreamWaitEvent(e->master, e->join[k], 0) != hipSuccess) rc = FDM_EDEVICE;
+    c->graph_mode = false;
+  }
+  hipError_t st = hipStreamEndCapture(e->master, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return ens_err(e, rc, "graph capture of a member step failed");
+  }
+  ENSCHK(e, st);
+  hipError_t si = hipGraphInstantiate(&e->exec[which], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  ENSCHK(e, si);
+  return FDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fdm_ensemble_last_error(const fdm_ensemble* e) { return e ? e->err.c_str() : g_create_error.c_str(); }
+
+void fdm_ensemble_destroy(fdm_ensemble* e) {
+  if (!e) return;
+  if (e->master) (void)hipStreamSynchronize(e->master);
+  for (int q = 0; q < 2; q++)
+    if (e->exec[q]) (void)hipGraphExecDestroy(e->exec[q]);
+  for (hipEvent_t ev : e->join)
+    if (ev) (void)hipEventDestroy(ev);
+  for (hipEvent_t ev : e->pre)
+    if (ev) (void)hipEventDestroy(ev);
+  if (e->fork) (void)hipEventDestroy(e->fork);
+  if (e->done) (void)hipEventDestroy(e->done);
+  if (e->master) (void)hipStreamDestroy(e->master);
+  delete e;
+}
+
+int fdm_ensemble_create(fdm_ctx* const* members, int32_t n, fdm_ensemble** out) {
+  if (!members || n <= 0 || !out) return FDM_EINVAL;
+  *out = nullptr;
+  fdm_ensemble* e = new fdm_ensemble();
+  for (int k = 0; k < n; k++) {
+    fdm_ctx* c = members[k];
+    if (!c) {
+      delete e;
+      return FDM_EINVAL;
+    }
+    if (c->src.nplace > 0 || c->any_mms || (!c->ld_T.empty() && c->ld_T[0] > 0.0) ||
+        c->bulk_mode != 2 || c->nstage != members[0]->nstage || c->cur != members[0]->cur) {
+      g_create_error =
+          "fdm_ensemble_create: members must have no point source, no manufactured solution, "
+          "no load ramp, the tile kernels, and the same RK order and buffer parity";
+      delete e;
+      return FDM_EINVAL;
+    }
+    if (!c->nstep_dev && dalloc(c, &c->nstep_dev, sizeof(long long)) != FDM_OK) {
+      delete e;
+      return FDM_ENOMEM;
+    }
+    e->m.push_back(c);
+  }
+  e->flip = members[0]->nstage & 1;
+  e->join.resize(n, nullptr);
+  e->pre.resize(n, nullptr);
+  bool ok = hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&e->done, hipEventDisableTiming) == hipSuccess;
+  for (int k = 0; k < n && ok; k++)
+    ok = hipEventCreateWithFlags(&e->join[k], hipEventDisableTiming) == hipSuccess &&
+         hipEventCreateWithFlags(&e->pre[k], hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    fdm_ensemble_destroy(e);
+    g_create_error = "fdm_ensemble_create: stream/event creation failed";
+    return FDM_EDEVICE;
+  }
+  *out = e;
+  return FDM_OK;
+}
+
+int fdm_ensemble_step(fdm_ensemble* e, const double* t, const double* dt, int64_t n_first,
+                      int32_t nsteps) {
+  if (!e || !t || !dt || nsteps < 0) return FDM_EINVAL;
+  const size_t n = e->m.size();
+  const int cur = e->m[0]->cur;
+  bool same = e->t.size() == n && e->exec[cur] && (!e->flip || e->exec[1 - cur]);
+  for (size_t k = 0; k < n && same; k++) same = e->t[k] == t[k] && e->dt[k] == dt[k];
+  for (size_t k = 0; k < n; k++) {
+    fdm_ctx* c = e->m[k];
+    if (c->prof) return ens_err(e, FDM_EINVAL, "fdm_ensemble_step: disable member profiling");
+    // the load ramp's switch at t = 0 is fixed in the captured graphs
+    if (t[k] < 0.0) return ens_err(e, FDM_EINVAL, "fdm_ensemble_step: times must be >= 0");
+    // the master stream follows each member's earlier work
+    ENSCHK(e, hipEventRecord(e->pre[k], c->stream));
+    ENSCHK(e, hipStreamWaitEvent(e->master, e->pre[k], 0));
+  }
+  if (!same) {
+    for (int q = 0; q < 2; q++)
+      if (e->exec[q]) {
+        (void)hipGraphExecDestroy(e->exec[q]);
+        e->exec[q] = nullptr;
+      }
+    e->t.assign(t, t + n);
+    e->dt.assign(dt, dt + n);
+    // graphs for both buffer parities; capturing a step advances each
+    // member's buffer index, so after both captures it is back
+    const int p0 = e->m[0]->cur;
+    int rc = ens_capture(e, p0);
+    if (!rc && e->flip) rc = ens_capture(e, 1 - p0);
+    if (rc) return rc;
+    if (!e->flip) e->exec[1 - p0] = nullptr;
+  }
+  for (size_t k = 0; k < n; k++) {
+    long long v = (long long)n_first;
+    ENSCHK(e, hipMemcpyAsync(e->m[k]->nstep_dev, &v, sizeof v, hipMemcpyHostToDevice, e->master));
+  }
+  // the copy source is on the stack: finish it before returning
+  ENSCHK(e, hipStreamSynchronize(e->master));
+  for (int s = 0; s < nsteps; s++) {
+    const int p = e->m[0]->cur;
+    ENSCHK(e, hipGraphLaunch(e->exec[p], e->master));
+    if (e->flip)
+      for (fdm_ctx* c : e->m) c->cur = 1 - c->cur;
+  }
+  // members' own streams follow the ensemble's work
+  ENSCHK(e, hipEventRecord(e->done, e->master));
+  for (fdm_ctx* c : e->m) ENSCHK(e, hipStreamWaitEvent(c->stream, e->done, 0));
+  return FDM_OK;
+}
+
+int fdm_ensemble_sync(fdm_ensemble* e) {
+  if (!e) return FDM_EINVAL;
+  ENSCHK(e, hipStreamSynchronize(e->master));
   return FDM_OK;
 }
 

@@ -163,32 +163,44 @@ __device__ __forceinline__ T matsel(int mat, T rho, T G, T M) {
 // Acc supplies the point's neighbourhood: f(k, di, dj) = F(k+1) at (i+di,
 // j+dj) with one of di, dj zero; xr(di), yr(di) along q; xq(dj), yq(dj)
 // along r.  Every kernel evaluates this one expression tree, whatever
-// memory the neighbourhood comes from.
+// memory the neighbourhood comes from.  The loop runs over s outside the
+// five fields: each field still adds its s = 1 term before its s = 2 term,
+// as in the reference, while only one set of metric averages is live.
 template <typename T, int l, typename Acc>
-__device__ __forceinline__ void interior_field(T* d, const Acc& A, T gamma, T c1, T c2, T Ji) {
+__device__ __forceinline__ void interior_term(T* d, const Acc& A, int s, T gamma, T c, T Ji,
+                                              T ayp, T axp, T aym, T axm, T byp, T bxp, T bym,
+                                              T bxm) {
   using P = Fld<l>;
+  T t = prod<T, P::ga>(ayp, A.f(P::ua, s, 0), gamma) - prod<T, P::gb>(axp, A.f(P::ub, s, 0), gamma) -
+        prod<T, P::ga>(aym, A.f(P::ua, -s, 0), gamma) + prod<T, P::gb>(axm, A.f(P::ub, -s, 0), gamma) -
+        prod<T, P::ga>(byp, A.f(P::ua, 0, s), gamma) + prod<T, P::gb>(bxp, A.f(P::ub, 0, s), gamma) +
+        prod<T, P::ga>(bym, A.f(P::ua, 0, -s), gamma) - prod<T, P::gb>(bxm, A.f(P::ub, 0, -s), gamma);
+  d[l] = d[l] + t * c * Ji;
+}
+
+// cI(l, s) returns the interior coefficient of field l and offset s
+template <typename T, typename Acc, typename Coef>
+__device__ __forceinline__ void interior_rates(T* d, const Acc& A, T gamma, const Coef& cI, T Ji) {
 #pragma unroll
   for (int s = 1; s <= 2; s++) {
-    T ayp = (T)0.5 * (A.yr(s) + A.yr(0)), axp = (T)0.5 * (A.xr(s) + A.xr(0));
-    T aym = (T)0.5 * (A.yr(-s) + A.yr(0)), axm = (T)0.5 * (A.xr(-s) + A.xr(0));
-    T byp = (T)0.5 * (A.yq(s) + A.yq(0)), bxp = (T)0.5 * (A.xq(s) + A.xq(0));
-    T bym = (T)0.5 * (A.yq(-s) + A.yq(0)), bxm = (T)0.5 * (A.xq(-s) + A.xq(0));
-    T t = prod<T, P::ga>(ayp, A.f(P::ua, s, 0), gamma) - prod<T, P::gb>(axp, A.f(P::ub, s, 0), gamma) -
-          prod<T, P::ga>(aym, A.f(P::ua, -s, 0), gamma) + prod<T, P::gb>(axm, A.f(P::ub, -s, 0), gamma) -
-          prod<T, P::ga>(byp, A.f(P::ua, 0, s), gamma) + prod<T, P::gb>(bxp, A.f(P::ub, 0, s), gamma) +
-          prod<T, P::ga>(bym, A.f(P::ua, 0, -s), gamma) - prod<T, P::gb>(bxm, A.f(P::ub, 0, -s), gamma);
-    d[l] = d[l] + t * (s == 1 ? c1 : c2) * Ji;
+    const T ayp = (T)0.5 * (A.yr(s) + A.yr(0)), axp = (T)0.5 * (A.xr(s) + A.xr(0));
+    const T aym = (T)0.5 * (A.yr(-s) + A.yr(0)), axm = (T)0.5 * (A.xr(-s) + A.xr(0));
+    const T byp = (T)0.5 * (A.yq(s) + A.yq(0)), bxp = (T)0.5 * (A.xq(s) + A.xq(0));
+    const T bym = (T)0.5 * (A.yq(-s) + A.yq(0)), bxm = (T)0.5 * (A.xq(-s) + A.xq(0));
+    interior_term<T, 0>(d, A, s, gamma, cI(0, s), Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term<T, 1>(d, A, s, gamma, cI(1, s), Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term<T, 2>(d, A, s, gamma, cI(2, s), Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term<T, 3>(d, A, s, gamma, cI(3, s), Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term<T, 4>(d, A, s, gamma, cI(4, s), Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
   }
 }
 
-template <typename T, typename Acc>
-__device__ __forceinline__ void interior_point(T* d, const Acc& A, const BlockDev& B, T gamma, T Ji) {
-  interior_field<T, 0>(d, A, gamma, (T)B.cI[0][0], (T)B.cI[0][1], Ji);
-  interior_field<T, 1>(d, A, gamma, (T)B.cI[1][0], (T)B.cI[1][1], Ji);
-  interior_field<T, 2>(d, A, gamma, (T)B.cI[2][0], (T)B.cI[2][1], Ji);
-  interior_field<T, 3>(d, A, gamma, (T)B.cI[3][0], (T)B.cI[3][1], Ji);
-  interior_field<T, 4>(d, A, gamma, (T)B.cI[4][0], (T)B.cI[4][1], Ji);
-}
+// interior coefficients of a block, from any address space
+template <typename T, typename BlockRef>
+struct BlockCoef {
+  BlockRef B;
+  __device__ __forceinline__ T operator()(int l, int s) const { return (T)B.cI[l][s - 1]; }
+};
 
 // neighbourhood read straight from global memory
 template <typename T>
@@ -375,8 +387,8 @@ __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDe
 // block edge (time_step.f90:754-810, rates.f90:451-918): L/R strips
 // one-sided in q and central in r, B/T strips the converse, corners
 // one-sided in both, interleaved per s as in rates_boundary_mode2.  The
-// stencil loops are kept rolled: the closures are a thin band, and unrolled
-// they would set the register budget of every kernel that inlines them.
+// stencil loops are unrolled so that all of a point's loads are in flight
+// together: the band is a thin set of points, latency-bound.
 template <typename T>
 __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockDev& B, const int i,
                                               const int j, const long long cc, T (&d)[6]) {
@@ -406,7 +418,7 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
   const long long colm = (long long)B.my * ld + i, colp = (long long)B.py * ld + i;
   if (qone) {
     // one-sided q: L or R strip, or q part of a corner
-#pragma unroll 1
+#pragma unroll
     for (int s = 0; s < 6; s++) {
       long long kq = (dl < 4) ? row + B.mx + s : row + B.px - s;
       T Dq = (dl < 4) ? (T)kDL[dl][s] : (T)(-kDL[dr][s]);
@@ -428,7 +440,7 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
     }
     if (!rone) {
       // L or R strip: central r
-#pragma unroll 1
+#pragma unroll
       for (int s = 1; s <= 2; s++) {
         T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
         long long kp = cc + (long long)s * ld, km = cc - (long long)s * ld;
@@ -441,7 +453,7 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
     }
   } else {
     // B or T strip: one-sided r, then central q
-#pragma unroll 1
+#pragma unroll
     for (int s = 0; s < 6; s++) {
       long long kr = (db < 4) ? colm + (long long)s * ld : colp - (long long)s * ld;
       T Dr = (db < 4) ? (T)kDL[db][s] : (T)(-kDL[dtp][s]);
@@ -451,7 +463,7 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
       d[3] = d[3] + bscale<T, 1>(r1<T, 3>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
       d[4] = d[4] + bscale<T, 2>(r1<T, 4>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
     }
-#pragma unroll 1
+#pragma unroll
     for (int s = 1; s <= 2; s++) {
       T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
       long long kp = cc + s, km = cc - s;
@@ -492,7 +504,7 @@ __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i,
   scaled_rates<T>(a, cc, d);
   if (i - B.mx >= 4 && B.px - i >= 4 && j - B.my >= 4 && B.py - j >= 4) {
     const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld};
-    interior_point<T>(d, acc, B, (T)B.gamma, a.Ji[cc]);
+    interior_rates<T>(d, acc, (T)B.gamma, BlockCoef<T, const BlockDev&>{B}, a.Ji[cc]);
   } else {
     closure_rates<T>(a, B, i, j, cc, d);
   }
@@ -558,7 +570,7 @@ struct TileAcc {
 
 typedef const __attribute__((address_space(4))) BlockDev ConstBlockDev;
 
-// Interior kernel.  The grid walks a list of 64 x TY tiles that cover each
+// Interior kernel.  The grid walks a list of TX x TY tiles that cover each
 // block's interior rectangle [mx+4, px-4] x [my+4, py-4] from its corner, so
 // every lane of a tile that falls inside the rectangle is an interior point
 // of the tile's block: no one-sided closure, no SAT, and the block index is
@@ -568,10 +580,10 @@ typedef const __attribute__((address_space(4))) BlockDev ConstBlockDev;
 // xq/yq with the r halo go to LDS; the lane's own DF, F6 and Ji stay in
 // registers.  Halo indices are clamped into the grid; a clamped value never
 // reaches an active lane.  tiles[k] = (block, i0, j0, 0).
-template <typename T, int TY, bool FULL>
-__global__ void __launch_bounds__(64 * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4)))
+template <typename T, int TX, int TY, bool FULL>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4)))
 bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
-  constexpr int TX = 64, HX = TX + 4, HY = TY + 4, NT = TX * TY;
+  constexpr int HX = TX + 4, HY = TY + 4, NT = TX * TY;
   constexpr int NF = (HY * HX + NT - 1) / NT;  // staging passes, F1..F5
   constexpr int NR = (TY * HX + NT - 1) / NT;  // xr, yr
   constexpr int NQ = (HY * TX + NT - 1) / NT;  // xq, yq
@@ -669,12 +681,7 @@ bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
     d[5] = (T)0.0;  // DF6 is overwritten by the szz relation
   }
   const TileAcc<T, TX, TY> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y};
-  const T gamma = (T)B4->gamma;
-  interior_field<T, 0>(d, acc, gamma, (T)B4->cI[0][0], (T)B4->cI[0][1], Ji);
-  interior_field<T, 1>(d, acc, gamma, (T)B4->cI[1][0], (T)B4->cI[1][1], Ji);
-  interior_field<T, 2>(d, acc, gamma, (T)B4->cI[2][0], (T)B4->cI[2][1], Ji);
-  interior_field<T, 3>(d, acc, gamma, (T)B4->cI[3][0], (T)B4->cI[3][1], Ji);
-  interior_field<T, 4>(d, acc, gamma, (T)B4->cI[4][0], (T)B4->cI[4][1], Ji);
+  interior_rates<T>(d, acc, (T)B4->gamma, BlockCoef<T, ConstBlockDev&>{*B4}, Ji);
   T Fc[6];
 #pragma unroll
   for (int k = 0; k < 5; k++) Fc[k] = acc.f(k, 0, 0);
@@ -682,7 +689,10 @@ bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
   if constexpr (FULL) {
     finish_point<T, true>(a, a.blocks[bu], bu, i, j, (long long)j * ld + i, d, Fc);
   } else {
-    // rates_szz (rates.f90:920-936), update_fields_interior (fields.f90:1225-1242)
+    // rates_szz (rates.f90:920-936), update_fields_interior (fields.f90:1225-1242).
+    // DF6 is not stored: no stage reads it (each overwrites it with this
+    // relation), and fdm_get_fields recomputes it from DF3 and DF5 with the
+    // same expression.
     const T nu = (T)B4->nu;
     d[5] = nu * (d[2] + d[4]);
     if (a.do_update) {
@@ -691,14 +701,14 @@ bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
       for (int l = 0; l < 6; l++) st_off(a.Fout + l * ps, oc, Fc[l] + Bdt * d[l]);
     }
 #pragma unroll
-    for (int l = 0; l < 6; l++) st_off(a.DF + l * ps, oc, d[l]);
+    for (int l = 0; l < 5; l++) st_off(a.DF + l * ps, oc, d[l]);
   }
 }
 
 // Band kernel: the points within four points of their block's edge (one-
 // sided closures, SAT), one lane each, from a list of (i, j).
 template <typename T, bool FULL>
-__global__ void __launch_bounds__(256) bulk_band(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
+__global__ void __launch_bounds__(64) bulk_band(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= npts) return;
   const int2 p = pts[k];

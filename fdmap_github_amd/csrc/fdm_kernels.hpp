@@ -80,12 +80,25 @@ struct EdgeArgs {
   double t, Bdt, A, Bload;
   int do_scale, do_update, snapshot;
   int* fail;
+  // graph replays: when set, the stage time is formed on the device from the
+  // step index *nstep as (t_init + (nstep - 1) dt) + rkc dt, the expression
+  // the host uses, instead of t
+  const long long* nstep;
+  double t_init, dt, rkc;
 };
+
+template <typename T>
+__device__ __forceinline__ double stage_time(const EdgeArgs<T>& a) {
+  if (!a.nstep) return a.t;
+  const double t0 = a.t_init + (double)(*a.nstep - 1) * a.dt;
+  return t0 + a.rkc * a.dt;
+}
 
 template <typename T>
 __global__ void __launch_bounds__(64) edge_kernel(EdgeArgs<T> a) {
   int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= a.ntasks) return;
+  const double tst = stage_time(a);
   int2 task = a.tasks[tid];
   const int NS = a.sd.NS;
   if (task.x >= 0) {
@@ -111,7 +124,7 @@ __global__ void __launch_bounds__(64) edge_kernel(EdgeArgs<T> a) {
     }
     double x = a.sd.x ? a.sd.x[so] : 0.0, y = a.sd.y ? a.sd.y[so] : 0.0;
     set_bc_mode2(Fh, F, F0, a.sd.normal[so], a.sd.normal[NS + so], B.side_bc[s], a.sd.M[so],
-                 a.sd.M[NS + so], a.sd.M[2 * NS + so], x, y, a.t, B.id);
+                 a.sd.M[NS + so], a.sd.M[2 * NS + so], x, y, tst, B.id);
     if constexpr (sizeof(T) == 4) {
 #pragma unroll
       for (int l = 0; l < 6; l++) Fh[l] = Fh[l] - B.Fb[l];
@@ -257,7 +270,7 @@ __global__ void __launch_bounds__(64) edge_kernel(EdgeArgs<T> a) {
     }
     ST(FDM_IF_DW) = ST(FDM_IF_DW) + S * V;
     bool ruptured = I.rup_field == 1 ? (D > I.rup_threshold) : (fabs(V) > I.rup_threshold);
-    if (ruptured) ST(FDM_IF_TRUP) = fmin(ST(FDM_IF_TRUP), a.t);
+    if (ruptured) ST(FDM_IF_TRUP) = fmin(ST(FDM_IF_TRUP), tst);
     if (a.snapshot) {
       for (int f = 0; f < FDM_IF_NFIELDS; f++)
         a.fi.st1[(long long)f * NI + q] = st[(long long)f * NI + q];

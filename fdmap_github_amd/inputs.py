@@ -68,11 +68,13 @@ vx vy
 
 def c2_planar_rs(name="c2", nx=1601, ny=801, h=0.025, nsteps=4000, outputs=True, ninfo=500,
                  law="RFL", plastic=False, curve=None, tau0=70.0, sn0=120.0, overstress=25.0,
-                 R=1.5, Vinit=1e-12, a=0.010, b=0.014):
+                 R=1.5, Vinit=1e-12, a=0.010, b=0.014, sxy0=None):
     """Two blocks nx x ny coupled across a horizontal fault by SAT; slip law
     with strong velocity weakening; Gaussian overstress at the fault centre.
     curve: optional path of a fault-profile curve file (FP64 x, y, n1, n2),
-    used for the top edge of block 1 and the bottom edge of block 2."""
+    used for the top edge of block 1 and the bottom edge of block 2.  The
+    background shear stress is sxy0 (default tau0); Psi0 gives slip velocity
+    Vinit under tau0."""
     Lx = 0.5 * (nx - 1) * h
     Ly = (ny - 1) * h
     psi0 = rs_state(a, 1e-6, tau0, sn0, Vinit)
@@ -115,7 +117,7 @@ D V S N Psi
 
 !---ALL BLOCKS---
 
-&fields_list sxx0 = {_f(-sn0)}, sxy0 = {_f(tau0)}, syy0 = {_f(-sn0)}, szz0 = {_f(-sn0)} /
+&fields_list sxx0 = {_f(-sn0)}, sxy0 = {_f(tau0 if sxy0 is None else sxy0)}, syy0 = {_f(-sn0)}, szz0 = {_f(-sn0)} /
 
 &material_list {mat} /
 

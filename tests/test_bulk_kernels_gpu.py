@@ -15,13 +15,13 @@ pytestmark = pytest.mark.gpu
 CS, CP, H = 3.464, 6.0, 0.025
 DT = 0.5 * H / np.sqrt(CS ** 2 + CP ** 2)
 BG = [0.0, 0.0, -120.0, 70.0, -120.0, -120.0]
-MODES = [("generic", {}), ("tile", {"FDM_TILE_TY": "8"}), ("tile", {"FDM_TILE_TY": "4"}),
-         ("tile", {"FDM_TILE_TY": "8", "FDM_XCD_SWIZZLE": "1"})]
+MODES = [("generic", {}), ("tile", {"FDM_TILE": "64x8"}), ("tile", {"FDM_TILE": "64x4"}),
+         ("tile", {"FDM_TILE": "128x4"}), ("tile", {"FDM_TILE": "64x8", "FDM_XCD_SWIZZLE": "0"})]
 
 
 def run_mode(monkeypatch, prob, F, nsteps, mode, env):
     monkeypatch.setenv("FDM_BULK", mode)
-    for k in ("FDM_TILE_TY", "FDM_XCD_SWIZZLE"):
+    for k in ("FDM_TILE", "FDM_XCD_SWIZZLE"):
         monkeypatch.delenv(k, raising=False)
     for k, v in env.items():
         monkeypatch.setenv(k, v)

@@ -12,10 +12,11 @@ from fdmap_github_amd import capi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "fdmap.h")
+ENSEMBLE_HEADER = os.path.join(ROOT, "include", "fdmap_ensemble.h")
 
 
-def declared_functions():
-    text = open(HEADER).read()
+def declared_functions(path=HEADER):
+    text = open(path).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"^[A-Za-z_][\w \*]*?\b(fdm_\w+)\s*\(", text, flags=re.M)))
 
@@ -28,6 +29,7 @@ def test_header_declares_the_entry_points():
         assert required in names
     # the Python binding knows every declared entry point, and nothing else
     assert sorted(n for n, _, _ in capi.ENTRY_POINTS) == names
+    assert sorted(n for n, _, _ in capi.ENSEMBLE_ENTRY_POINTS) == declared_functions(ENSEMBLE_HEADER)
 
 
 def test_library_exports_every_declared_symbol():
@@ -35,7 +37,8 @@ def test_library_exports_every_declared_symbol():
     if not os.path.exists(path):
         pytest.fail("libfdmap_hip.so is not built (run __graft_entry__.build())")
     lib = C.CDLL(path)
-    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    names = declared_functions() + declared_functions(ENSEMBLE_HEADER)
+    missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
 
 
