@@ -232,6 +232,9 @@ struct fdm_ctx {
   int ntiles[3] = {0, 0, 0};
   int2* band = nullptr;
   int nband = 0;
+  // host copies of the tile, band and edge-task lists (ensembles merge them)
+  std::vector<int4> htiles[3];
+  std::vector<int2> hband, htasks;
   // the latest interior launch was a lean one, which leaves DF6 unwritten at
   // interior points
   bool df6_stale = false;
@@ -357,9 +360,8 @@ void prof_end(fdm_ctx* c, EventPair* ep, int kind, hipStream_t s) {
 }
 
 template <typename T>
-int launch_edges(fdm_ctx* c, hipStream_t s, double t, double Bdt, double A, int do_scale,
-                 int do_update, int snapshot, double Bload) {
-  if (c->ntasks == 0) return FDM_OK;
+EdgeArgs<T> edge_args(fdm_ctx* c, double t, double Bdt, double A, int do_scale, int do_update,
+                      int snapshot, double Bload) {
   EdgeArgs<T> a;
   a.F = (const T*)c->F[c->cur];
   a.ld = c->ld;
@@ -382,6 +384,14 @@ int launch_edges(fdm_ctx* c, hipStream_t s, double t, double Bdt, double A, int 
   a.do_update = do_update;
   a.snapshot = snapshot;
   a.fail = c->fail;
+  return a;
+}
+
+template <typename T>
+int launch_edges(fdm_ctx* c, hipStream_t s, double t, double Bdt, double A, int do_scale,
+                 int do_update, int snapshot, double Bload) {
+  if (c->ntasks == 0) return FDM_OK;
+  const EdgeArgs<T> a = edge_args<T>(c, t, Bdt, A, do_scale, do_update, snapshot, Bload);
   dim3 grid((c->ntasks + 63) / 64), block(64);
   EventPair* ep = prof_begin(c, 1, s);
   hipLaunchKernelGGL(edge_kernel<T>, grid, block, 0, s, a);
@@ -799,6 +809,8 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     }
     c->nband = (int)band.size();
     if (c->nband && (rc = dupload(c, &c->band, band.data(), band.size() * sizeof(int2)))) return fail(rc);
+    for (int q = 0; q < 3; q++) c->htiles[q] = std::move(tl[q]);
+    c->hband = std::move(band);
   }
   if ((rc = dupload(c, &c->rowb, rowb.data(), rowb.size() * sizeof(int)))) return fail(rc);
 
@@ -1029,6 +1041,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   c->ntasks = (int)tasks.size();
   if ((rc = dupload(c, &c->tasks, tasks.data(), std::max<size_t>(1, tasks.size()) * sizeof(int2))))
     return fail(rc);
+  c->htasks = tasks;
 
   // ---- source ----
   c->src = P->source;
@@ -1308,14 +1321,35 @@ int fdm_tune(fdm_ctx* c, int32_t knob, int32_t value) {
 
 }  // extern "C"
 
+// An ensemble steps many contexts with batched launches: every RK stage is
+// one interior launch per tile shape in use, one edge launch and one band
+// launch over the merged work lists of all members, with the two-stream
+// ordering of a single context.  The per-member kernel arguments of each
+// (buffer parity, stage) sit in device tables; stage times come from one
+// device step counter.  A step is captured once per starting buffer parity
+// as a hipGraph (FDM_ENSEMBLE_GRAPH=0 launches the same kernels directly).
 struct fdm_ensemble {
   std::vector<fdm_ctx*> m;
-  hipStream_t master = nullptr;
-  hipEvent_t fork = nullptr, done = nullptr;  // capture fork; end of eager launches
-  std::vector<hipEvent_t> join, pre;           // capture joins; eager ordering
+  int precision = FDM_FP64, nstage = 0, flip = 0, swz = 1;
+  bool any_plastic = false, use_graph = true;
+  hipStream_t master = nullptr, side = nullptr;
+  hipEvent_t evJ = nullptr, evI = nullptr, evB = nullptr, done = nullptr;
+  std::vector<hipEvent_t> pre;  // each member's earlier work, before a step
+  // merged lists: tiles (block, i0, j0, member) per tile shape; band points
+  // (i, j, member, valid) and edge tasks (x, y, member, valid), each member's
+  // run padded to whole waves so that a wave serves one member
+  int4* tiles[3] = {nullptr, nullptr, nullptr};
+  int ntiles[3] = {0, 0, 0};
+  int4* band = nullptr;
+  int nband = 0;
+  int4* tasks = nullptr;
+  int ntasks = 0;
+  void* btab = nullptr;  // BulkArgs<T>[2][nstage][members]
+  void* etab = nullptr;  // EdgeArgs<T>[2][nstage][members]
+  long long* nstep = nullptr;
   hipGraphExec_t exec[2] = {nullptr, nullptr};  // one RK step from F buffer 0 / 1
-  std::vector<double> t, dt;                     // member times the graphs were captured for
-  int flip = 0;                                  // buffer parity change per step
+  std::vector<double> t, dt;                     // member times the tables hold
+  std::vector<void*> allocs;
   std::string err;
 };
 
@@ -1333,39 +1367,150 @@ int ens_err(fdm_ensemble* e, int rc, const std::string& msg) {
       return ens_err(e, FDM_EDEVICE, std::string(#call ": ") + hipGetErrorString(_st)); \
   } while (0)
 
-// Capture one RK step of every member (each from its current F buffer) into
-// one graph: the members' two streams fork off the master stream and join
-// back; the stage kernels are exactly those of fdm_step.
-int ens_capture(fdm_ensemble* e, int which) {
+template <typename P>
+int ens_upload(fdm_ensemble* e, P** p, const void* host, size_t bytes) {
+  ENSCHK(e, hipMalloc((void**)p, std::max<size_t>(bytes, 64)));
+  e->allocs.push_back((void*)*p);
+  if (host && bytes) ENSCHK(e, hipMemcpy(*p, host, bytes, hipMemcpyHostToDevice));
+  return FDM_OK;
+}
+
+__global__ void __launch_bounds__(64) step_set_kernel(long long* n, long long v) {
+  if (threadIdx.x == 0) n[0] = v;
+}
+
+// Per-member arguments of every (parity, stage): the values run_stage
+// passes for a member whose current buffer at that stage is the parity.
+template <typename T>
+int ens_tables(fdm_ensemble* e) {
+  const size_t M = e->m.size(), ns = e->nstage;
+  std::vector<BulkArgs<T>> bt(2 * ns * M);
+  std::vector<EdgeArgs<T>> et(2 * ns * M);
+  for (int p = 0; p < 2; p++)
+    for (size_t s = 1; s <= ns; s++)
+      for (size_t k = 0; k < M; k++) {
+        fdm_ctx* c = e->m[k];
+        const int cs = p ^ (int)((s - 1) & 1);
+        const double A = c->rkA[s - 1], Bdt = c->rkB[s - 1] * e->dt[k];
+        // stage time of step 1; the edge kernels form it from the step counter
+        const double ts = e->t[k] + c->rkC[s - 1] * e->dt[k];
+        const size_t o = ((size_t)p * ns + (s - 1)) * M + k;
+        const int saved = c->cur;
+        c->cur = cs;
+        bt[o] = bulk_args<T>(c, A, Bdt, 1, ts, 0.0, s == ns ? 1 : 0, e->dt[k]);
+        et[o] = edge_args<T>(c, ts, Bdt, A, 1, 1, s == 1 ? 1 : 0, 1.0);
+        c->cur = saved;
+        et[o].nstep = e->nstep;
+        et[o].t_init = e->t[k];
+        et[o].dt = e->dt[k];
+        et[o].rkc = c->rkC[s - 1];
+      }
+  if (!e->btab && ens_upload(e, (BulkArgs<T>**)&e->btab, nullptr, bt.size() * sizeof(bt[0])))
+    return FDM_EDEVICE;
+  if (!e->etab && ens_upload(e, (EdgeArgs<T>**)&e->etab, nullptr, et.size() * sizeof(et[0])))
+    return FDM_EDEVICE;
+  // no launch of an earlier step may still read the tables
+  ENSCHK(e, hipStreamSynchronize(e->master));
+  ENSCHK(e, hipMemcpy(e->btab, bt.data(), bt.size() * sizeof(bt[0]), hipMemcpyHostToDevice));
+  ENSCHK(e, hipMemcpy(e->etab, et.data(), et.size() * sizeof(et[0]), hipMemcpyHostToDevice));
+  return FDM_OK;
+}
+
+template <typename T, bool FULL>
+void ens_interior(fdm_ensemble* e, const BulkArgs<T>* tab) {
+  for (int q = 0; q < 3; q++) {
+    if (!e->ntiles[q]) continue;
+    const dim3 grid(e->ntiles[q]), block(kTileShape[q][0], kTileShape[q][1]);
+    if (q == 0)
+      hipLaunchKernelGGL((bulk_interior_batch<T, 64, 4, FULL>), grid, block, 0, e->master, tab, e->tiles[q], e->swz);
+    else if (q == 1)
+      hipLaunchKernelGGL((bulk_interior_batch<T, 64, 8, FULL>), grid, block, 0, e->master, tab, e->tiles[q], e->swz);
+    else
+      hipLaunchKernelGGL((bulk_interior_batch<T, 128, 4, FULL>), grid, block, 0, e->master, tab, e->tiles[q], e->swz);
+  }
+}
+
+// One RK step of every member from buffer parity p: the stream DAG of
+// run_stage (interior on the master stream, edge and band on the side
+// stream), then the step counter.
+template <typename T>
+int ens_step_launches(fdm_ensemble* e, int p) {
+  const size_t M = e->m.size(), ns = e->nstage;
+  ENSCHK(e, hipEventRecord(e->evJ, e->master));
+  ENSCHK(e, hipStreamWaitEvent(e->side, e->evJ, 0));
+  ENSCHK(e, hipEventRecord(e->evB, e->side));
+  ENSCHK(e, hipEventRecord(e->evI, e->master));
+  for (size_t s = 1; s <= ns; s++) {
+    const size_t o = ((size_t)p * ns + (s - 1)) * M;
+    const BulkArgs<T>* bt = (const BulkArgs<T>*)e->btab + o;
+    const EdgeArgs<T>* et = (const EdgeArgs<T>*)e->etab + o;
+    const bool full = e->any_plastic && s == ns;
+    ENSCHK(e, hipStreamWaitEvent(e->master, e->evB, 0));
+    if (full) ens_interior<T, true>(e, bt);
+    else ens_interior<T, false>(e, bt);
+    if (e->ntasks)
+      hipLaunchKernelGGL(edge_kernel_batch<T>, dim3(e->ntasks / 64), dim3(64), 0, e->side, et, e->tasks,
+                         e->ntasks);
+    ENSCHK(e, hipStreamWaitEvent(e->side, e->evI, 0));
+    if (e->nband) {
+      if (full)
+        hipLaunchKernelGGL((bulk_band_batch<T, true>), dim3(e->nband / 64), dim3(64), 0, e->side, bt, e->band,
+                           e->nband);
+      else
+        hipLaunchKernelGGL((bulk_band_batch<T, false>), dim3(e->nband / 64), dim3(64), 0, e->side, bt,
+                           e->band, e->nband);
+    }
+    ENSCHK(e, hipEventRecord(e->evB, e->side));
+    ENSCHK(e, hipEventRecord(e->evI, e->master));
+  }
+  hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(64), 0, e->side, e->nstep);
+  ENSCHK(e, hipEventRecord(e->evB, e->side));
+  ENSCHK(e, hipStreamWaitEvent(e->master, e->evB, 0));
+  ENSCHK(e, hipGetLastError());
+  return FDM_OK;
+}
+
+int ens_step_any(fdm_ensemble* e, int p) {
+  return e->precision == FDM_FP64 ? ens_step_launches<double>(e, p) : ens_step_launches<float>(e, p);
+}
+
+int ens_capture(fdm_ensemble* e, int p) {
   hipGraph_t g = nullptr;
   ENSCHK(e, hipStreamBeginCapture(e->master, hipStreamCaptureModeRelaxed));
-  int rc = FDM_OK;
-  if (hipEventRecord(e->fork, e->master) != hipSuccess) rc = FDM_EDEVICE;
-  for (size_t k = 0; k < e->m.size() && !rc; k++) {
-    fdm_ctx* c = e->m[k];
-    c->graph_mode = true;
-    c->g_tinit = e->t[k];
-    c->g_dt = e->dt[k];
-    if (hipStreamWaitEvent(c->stream, e->fork, 0) != hipSuccess) rc = FDM_EDEVICE;
-    if (!rc) rc = fork_side(c);
-    for (int s = 1; s <= c->nstage && !rc; s++)
-      rc = c->precision == FDM_FP64 ? run_stage<double>(c, s, e->t[k], e->dt[k], 0, s == 1)
-                                    : run_stage<float>(c, s, e->t[k], e->dt[k], 0, s == 1);
-    if (!rc) rc = join_side(c);
-    if (!rc) hipLaunchKernelGGL(step_counter_kernel, dim3(1), dim3(64), 0, c->stream, c->nstep_dev);
-    if (!rc && hipEventRecord(e->join[k], c->stream) != hipSuccess) rc = FDM_EDEVICE;
-    if (!rc && hipStreamWaitEvent(e->master, e->join[k], 0) != hipSuccess) rc = FDM_EDEVICE;
-    c->graph_mode = false;
-  }
+  const int rc = ens_step_any(e, p);
   hipError_t st = hipStreamEndCapture(e->master, &g);
   if (rc) {
     if (g) (void)hipGraphDestroy(g);
-    return ens_err(e, rc, "graph capture of a member step failed");
+    return rc;
   }
   ENSCHK(e, st);
-  hipError_t si = hipGraphInstantiate(&e->exec[which], g, nullptr, nullptr, 0);
+  hipError_t si = hipGraphInstantiate(&e->exec[p], g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   ENSCHK(e, si);
+  return FDM_OK;
+}
+
+// merged work lists of all members
+int ens_lists(fdm_ensemble* e) {
+  const int M = (int)e->m.size();
+  std::vector<int4> tl[3], band, tasks;
+  for (int k = 0; k < M; k++) {
+    const fdm_ctx* c = e->m[k];
+    for (const int4& t : c->htiles[c->tile_shape]) tl[c->tile_shape].push_back(make_int4(t.x, t.y, t.z, k));
+    for (const int2& b : c->hband) band.push_back(make_int4(b.x, b.y, k, 1));
+    while (band.size() % 64) band.push_back(make_int4(0, 0, k, 0));
+    for (const int2& t : c->htasks) tasks.push_back(make_int4(t.x, t.y, k, 1));
+    while (tasks.size() % 64) tasks.push_back(make_int4(0, 0, k, 0));
+  }
+  for (int q = 0; q < 3; q++) {
+    e->ntiles[q] = (int)tl[q].size();
+    if (e->ntiles[q] && ens_upload(e, &e->tiles[q], tl[q].data(), tl[q].size() * sizeof(int4)))
+      return FDM_EDEVICE;
+  }
+  e->nband = (int)band.size();
+  if (e->nband && ens_upload(e, &e->band, band.data(), band.size() * sizeof(int4))) return FDM_EDEVICE;
+  e->ntasks = (int)tasks.size();
+  if (e->ntasks && ens_upload(e, &e->tasks, tasks.data(), tasks.size() * sizeof(int4))) return FDM_EDEVICE;
   return FDM_OK;
 }
 
@@ -1378,55 +1523,64 @@ const char* fdm_ensemble_last_error(const fdm_ensemble* e) { return e ? e->err.c
 void fdm_ensemble_destroy(fdm_ensemble* e) {
   if (!e) return;
   if (e->master) (void)hipStreamSynchronize(e->master);
+  if (e->side) (void)hipStreamSynchronize(e->side);
   for (int q = 0; q < 2; q++)
     if (e->exec[q]) (void)hipGraphExecDestroy(e->exec[q]);
-  for (hipEvent_t ev : e->join)
-    if (ev) (void)hipEventDestroy(ev);
   for (hipEvent_t ev : e->pre)
     if (ev) (void)hipEventDestroy(ev);
-  if (e->fork) (void)hipEventDestroy(e->fork);
-  if (e->done) (void)hipEventDestroy(e->done);
+  for (hipEvent_t ev : {e->evJ, e->evI, e->evB, e->done})
+    if (ev) (void)hipEventDestroy(ev);
+  for (void* p : e->allocs) (void)hipFree(p);
   if (e->master) (void)hipStreamDestroy(e->master);
+  if (e->side) (void)hipStreamDestroy(e->side);
   delete e;
 }
 
 int fdm_ensemble_create(fdm_ctx* const* members, int32_t n, fdm_ensemble** out) {
   if (!members || n <= 0 || !out) return FDM_EINVAL;
   *out = nullptr;
-  fdm_ensemble* e = new fdm_ensemble();
   for (int k = 0; k < n; k++) {
-    fdm_ctx* c = members[k];
-    if (!c) {
-      delete e;
-      return FDM_EINVAL;
-    }
-    if (c->src.nplace > 0 || c->any_mms || (!c->ld_T.empty() && c->ld_T[0] > 0.0) ||
-        c->bulk_mode != 2 || c->nstage != members[0]->nstage || c->cur != members[0]->cur) {
+    const fdm_ctx* c = members[k];
+    if (!c) return FDM_EINVAL;
+    const fdm_ctx* c0 = members[0];
+    if (c->src.nplace > 0 || c->any_mms || (!c->ld_T.empty() && c->ld_T[0] > 0.0) || c->bulk_mode != 2 ||
+        c->nstage != c0->nstage || c->precision != c0->precision || c->device != c0->device) {
       g_create_error =
           "fdm_ensemble_create: members must have no point source, no manufactured solution, "
-          "no load ramp, the tile kernels, and the same RK order and buffer parity";
-      delete e;
+          "no load ramp, the tile kernels, and the same RK order, precision and device";
       return FDM_EINVAL;
     }
-    if (!c->nstep_dev && dalloc(c, &c->nstep_dev, sizeof(long long)) != FDM_OK) {
-      delete e;
-      return FDM_ENOMEM;
-    }
-    e->m.push_back(c);
+    for (int q = 0; q < k; q++)
+      if (members[q] == c) {
+        g_create_error = "fdm_ensemble_create: a context appears twice";
+        return FDM_EINVAL;
+      }
   }
-  e->flip = members[0]->nstage & 1;
-  e->join.resize(n, nullptr);
+  if (hipSetDevice(members[0]->device) != hipSuccess) {
+    g_create_error = "fdm_ensemble_create: hipSetDevice failed";
+    return FDM_EDEVICE;
+  }
+  fdm_ensemble* e = new fdm_ensemble();
+  e->m.assign(members, members + n);
+  e->precision = members[0]->precision;
+  e->nstage = members[0]->nstage;
+  e->flip = e->nstage & 1;
+  e->swz = members[0]->xcd_swizzle;
+  for (fdm_ctx* c : e->m) e->any_plastic = e->any_plastic || c->any_plastic;
+  if (const char* v = getenv("FDM_ENSEMBLE_GRAPH")) e->use_graph = atoi(v) != 0;
   e->pre.resize(n, nullptr);
   bool ok = hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking) == hipSuccess &&
-            hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) == hipSuccess &&
-            hipEventCreateWithFlags(&e->done, hipEventDisableTiming) == hipSuccess;
-  for (int k = 0; k < n && ok; k++)
-    ok = hipEventCreateWithFlags(&e->join[k], hipEventDisableTiming) == hipSuccess &&
-         hipEventCreateWithFlags(&e->pre[k], hipEventDisableTiming) == hipSuccess;
-  if (!ok) {
+            hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) == hipSuccess;
+  for (hipEvent_t* ev : {&e->evJ, &e->evI, &e->evB, &e->done})
+    ok = ok && hipEventCreateWithFlags(ev, hipEventDisableTiming) == hipSuccess;
+  for (int k = 0; k < n && ok; k++) ok = hipEventCreateWithFlags(&e->pre[k], hipEventDisableTiming) == hipSuccess;
+  int rc = ok ? FDM_OK : FDM_EDEVICE;
+  if (!rc) rc = ens_lists(e);
+  if (!rc) rc = ens_upload(e, &e->nstep, nullptr, sizeof(long long));
+  if (rc) {
+    g_create_error = "fdm_ensemble_create: " + (e->err.empty() ? std::string("stream/event creation failed") : e->err);
     fdm_ensemble_destroy(e);
-    g_create_error = "fdm_ensemble_create: stream/event creation failed";
-    return FDM_EDEVICE;
+    return rc;
   }
   *out = e;
   return FDM_OK;
@@ -1437,46 +1591,52 @@ int fdm_ensemble_step(fdm_ensemble* e, const double* t, const double* dt, int64_
   if (!e || !t || !dt || nsteps < 0) return FDM_EINVAL;
   const size_t n = e->m.size();
   const int cur = e->m[0]->cur;
-  bool same = e->t.size() == n && e->exec[cur] && (!e->flip || e->exec[1 - cur]);
-  for (size_t k = 0; k < n && same; k++) same = e->t[k] == t[k] && e->dt[k] == dt[k];
   for (size_t k = 0; k < n; k++) {
     fdm_ctx* c = e->m[k];
     if (c->prof) return ens_err(e, FDM_EINVAL, "fdm_ensemble_step: disable member profiling");
-    // the load ramp's switch at t = 0 is fixed in the captured graphs
-    if (t[k] < 0.0) return ens_err(e, FDM_EINVAL, "fdm_ensemble_step: times must be >= 0");
-    // the master stream follows each member's earlier work
-    ENSCHK(e, hipEventRecord(e->pre[k], c->stream));
+    if (c->cur != cur) return ens_err(e, FDM_EINVAL, "fdm_ensemble_step: members differ in buffer parity");
+    // the load factor of a member without a ramp is 1 from t = 0 on
+    if (t[k] < 0.0 || !(dt[k] > 0.0)) return ens_err(e, FDM_EINVAL, "fdm_ensemble_step: need t >= 0, dt > 0");
+  }
+  if (!nsteps) return FDM_OK;
+  ENSCHK(e, hipSetDevice(e->m[0]->device));
+  // the master stream follows each member's earlier work
+  for (size_t k = 0; k < n; k++) {
+    ENSCHK(e, hipEventRecord(e->pre[k], e->m[k]->stream));
     ENSCHK(e, hipStreamWaitEvent(e->master, e->pre[k], 0));
   }
+  bool same = e->t.size() == n;
+  for (size_t k = 0; k < n && same; k++) same = e->t[k] == t[k] && e->dt[k] == dt[k];
   if (!same) {
-    for (int q = 0; q < 2; q++)
-      if (e->exec[q]) {
-        (void)hipGraphExecDestroy(e->exec[q]);
-        e->exec[q] = nullptr;
-      }
     e->t.assign(t, t + n);
     e->dt.assign(dt, dt + n);
-    // graphs for both buffer parities; capturing a step advances each
-    // member's buffer index, so after both captures it is back
-    const int p0 = e->m[0]->cur;
-    int rc = ens_capture(e, p0);
-    if (!rc && e->flip) rc = ens_capture(e, 1 - p0);
-    if (rc) return rc;
-    if (!e->flip) e->exec[1 - p0] = nullptr;
+    const int rc = e->precision == FDM_FP64 ? ens_tables<double>(e) : ens_tables<float>(e);
+    if (rc) {
+      e->t.clear();
+      return rc;
+    }
   }
-  for (size_t k = 0; k < n; k++) {
-    long long v = (long long)n_first;
-    ENSCHK(e, hipMemcpyAsync(e->m[k]->nstep_dev, &v, sizeof v, hipMemcpyHostToDevice, e->master));
-  }
-  // the copy source is on the stack: finish it before returning
-  ENSCHK(e, hipStreamSynchronize(e->master));
+  hipLaunchKernelGGL(step_set_kernel, dim3(1), dim3(64), 0, e->master, e->nstep, (long long)n_first);
+  ENSCHK(e, hipGetLastError());
   for (int s = 0; s < nsteps; s++) {
     const int p = e->m[0]->cur;
-    ENSCHK(e, hipGraphLaunch(e->exec[p], e->master));
+    if (e->use_graph) {
+      if (!e->exec[p]) {
+        const int rc = ens_capture(e, p);
+        if (rc) return rc;
+      }
+      ENSCHK(e, hipGraphLaunch(e->exec[p], e->master));
+    } else {
+      const int rc = ens_step_any(e, p);
+      if (rc) return rc;
+    }
     if (e->flip)
       for (fdm_ctx* c : e->m) c->cur = 1 - c->cur;
   }
-  // members' own streams follow the ensemble's work
+  // interior points of a lean last-stage launch hold no DF6 (recomputed on
+  // read); the full one (any plastic member) stores it, plastic corrections
+  // included.  Members' own streams follow the ensemble's work.
+  for (fdm_ctx* c : e->m) c->df6_stale = !e->any_plastic;
   ENSCHK(e, hipEventRecord(e->done, e->master));
   for (fdm_ctx* c : e->m) ENSCHK(e, hipStreamWaitEvent(c->stream, e->done, 0));
   return FDM_OK;
@@ -1485,6 +1645,9 @@ int fdm_ensemble_step(fdm_ensemble* e, const double* t, const double* dt, int64_
 int fdm_ensemble_sync(fdm_ensemble* e) {
   if (!e) return FDM_EINVAL;
   ENSCHK(e, hipStreamSynchronize(e->master));
+  for (size_t k = 0; k < e->m.size(); k++)
+    if (check_fail(e->m[k]) != FDM_OK)
+      return ens_err(e, FDM_ENEWTON, "member " + std::to_string(k) + ": " + e->m[k]->err);
   return FDM_OK;
 }
 

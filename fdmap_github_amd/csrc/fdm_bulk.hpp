@@ -581,8 +581,7 @@ typedef const __attribute__((address_space(4))) BlockDev ConstBlockDev;
 // registers.  Halo indices are clamped into the grid; a clamped value never
 // reaches an active lane.  tiles[k] = (block, i0, j0, 0).
 template <typename T, int TX, int TY, bool FULL>
-__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4)))
-bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
+__device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 td) {
   constexpr int HX = TX + 4, HY = TY + 4, NT = TX * TY;
   constexpr int NF = (HY * HX + NT - 1) / NT;  // staging passes, F1..F5
   constexpr int NR = (TY * HX + NT - 1) / NT;  // xr, yr
@@ -590,7 +589,6 @@ bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
   __shared__ T sF[5][HY][HX];
   __shared__ T sR[2][TY][HX];
   __shared__ T sQ[2][HY][TX];
-  const int4 td = tiles[linear_tile(a.xcd_swizzle)];
   const int bu = td.x, i0 = td.y, j0 = td.z;
   ConstBlockDev* B4 = (ConstBlockDev*)a.blocks + bu;
   const int tid = threadIdx.y * TX + threadIdx.x;
@@ -705,6 +703,21 @@ bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
   }
 }
 
+template <typename T, int TX, int TY, bool FULL>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4)))
+bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
+  interior_tile<T, TX, TY, FULL>(a, tiles[linear_tile(a.xcd_swizzle)]);
+}
+
+// Batched interior kernel: the tiles of many problems in one launch; tile
+// .w selects the problem's arguments in tab.
+template <typename T, int TX, int TY, bool FULL>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4)))
+bulk_interior_batch(const BulkArgs<T>* __restrict__ tab, const int4* __restrict__ tiles, int swz) {
+  const int4 td = tiles[linear_tile(swz)];
+  interior_tile<T, TX, TY, FULL>(tab[td.w], td);
+}
+
 // Band kernel: the points within four points of their block's edge (one-
 // sided closures, SAT), one lane each, from a list of (i, j).
 template <typename T, bool FULL>
@@ -713,6 +726,19 @@ __global__ void __launch_bounds__(64) bulk_band(BulkArgs<T> a, const int2* __res
   if (k >= npts) return;
   const int2 p = pts[k];
   point_generic<T, FULL>(a, p.x, p.y);
+}
+
+// Batched band kernel: pts[k] = (i, j, problem, valid), every wave's 64
+// entries of one problem (padding entries have valid = 0).
+template <typename T, bool FULL>
+__global__ void __launch_bounds__(64) bulk_band_batch(const BulkArgs<T>* __restrict__ tab,
+                                                      const int4* __restrict__ pts, int npts) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npts) return;
+  const int4 p = pts[k];
+  const int m = __builtin_amdgcn_readfirstlane(p.z);
+  if (!p.w) return;
+  point_generic<T, FULL>(tab[m], p.x, p.y);
 }
 
 }  // namespace fdm

@@ -95,11 +95,8 @@ __device__ __forceinline__ double stage_time(const EdgeArgs<T>& a) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(64) edge_kernel(EdgeArgs<T> a) {
-  int tid = blockIdx.x * blockDim.x + threadIdx.x;
-  if (tid >= a.ntasks) return;
+__device__ __forceinline__ void edge_task(const EdgeArgs<T>& a, const int2 task) {
   const double tst = stage_time(a);
-  int2 task = a.tasks[tid];
   const int NS = a.sd.NS;
   if (task.x >= 0) {
     // boundary node (boundaries.f90:106-129)
@@ -299,6 +296,27 @@ __global__ void __launch_bounds__(64) edge_kernel(EdgeArgs<T> a) {
     a.sd.Fhat[l * NS + som] = Fhm[l];
     a.sd.Fhat[l * NS + sop] = Fhp[l];
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) edge_kernel(EdgeArgs<T> a) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= a.ntasks) return;
+  edge_task<T>(a, a.tasks[tid]);
+}
+
+// Batched edge kernel: tasks[k] = (task.x, task.y, problem, valid) over the
+// edge tasks of many problems, every wave's 64 entries of one problem
+// (padding entries have valid = 0).
+template <typename T>
+__global__ void __launch_bounds__(64) edge_kernel_batch(const EdgeArgs<T>* __restrict__ tab,
+                                                        const int4* __restrict__ tasks, int n) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= n) return;
+  const int4 t = tasks[tid];
+  const int m = __builtin_amdgcn_readfirstlane(t.z);
+  if (!t.w) return;
+  edge_task<T>(tab[m], make_int2(t.x, t.y));
 }
 
 }  // namespace fdm
