@@ -178,16 +178,30 @@ ENSEMBLE_ENTRY_POINTS = [
     ("fdm_ensemble_last_error", C.c_char_p, [_P]),
 ]
 
+# include/fdmap_adjoint.h
+FDM_GRAD_S0, FDM_GRAD_A, FDM_GRAD_B, FDM_GRAD_PSI0, FDM_GRAD_NFIELDS = range(5)
+ADJOINT_ENTRY_POINTS = [
+    ("fdm_adjoint_create", C.c_int,
+     [_P, C.c_int32, _ip, _ip, C.c_int32, C.c_int32, C.POINTER(_P)]),
+    ("fdm_adjoint_info", C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("fdm_adjoint_set_param", C.c_int, [_P, C.c_int32, _dp]),
+    ("fdm_adjoint_forward", C.c_int, [_P, C.c_double, C.c_double, _dp]),
+    ("fdm_adjoint_backward", C.c_int, [_P, _dp, _dp, _dp]),
+    ("fdm_adjoint_timing", C.c_int, [_P, _dp, _dp, _dp]),
+    ("fdm_adjoint_last_error", C.c_char_p, [_P]),
+    ("fdm_adjoint_destroy", None, [_P]),
+]
+
 
 def bind(lib, prefix_from="fdm_", prefix_to="fdm_"):
     """Attach restype/argtypes to every entry point of a library that
     implements the fdmap.h interface (optionally under another prefix), and
-    to the ensemble entry points where the library has them."""
+    to the ensemble and adjoint entry points where the library has them."""
     for name, res, args in ENTRY_POINTS:
         fn = getattr(lib, prefix_to + name[len(prefix_from):])
         fn.restype = res
         fn.argtypes = args
-    for name, res, args in ENSEMBLE_ENTRY_POINTS:
+    for name, res, args in ENSEMBLE_ENTRY_POINTS + ADJOINT_ENTRY_POINTS:
         if hasattr(lib, name):
             fn = getattr(lib, name)
             fn.restype = res

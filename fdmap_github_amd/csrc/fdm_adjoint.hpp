@@ -1,0 +1,848 @@
+// fdm_adjoint.hpp -- discrete adjoint of the time-stepping path.
+//
+// Forward stage s of LSRK (time_step.f90:71-167), per grid point and fault
+// node, with A = A_s and b = B_s dt:
+//   Fhat   = H(F, Psi)                        boundary / interface hats
+//   DPsi  <- A DPsi + G(V, Psi)               state rate at the solved V
+//   Psi   <- Psi + b DPsi
+//   DF    <- A DF + R(F, Fhat),  DF6 = nu (DF3 + DF5)
+//   F     <- F + b DF
+// R (rates + SAT) is affine, H is affine except at frictional nodes, where
+// V solves Slock - eta V = a N asinh(V exp(Psi/a) / (2 V0)).  The adjoint
+// keeps lambda_F and mu = lambda_DF + b lambda_F (the adjoint of the stage's
+// new DF including its share of the update); szz feeds nothing but itself,
+// so both carry the five fields vx, vy, sxx, sxy, syy.  A backward stage is
+//   nu_f   = lambda_DPsi + b lambda_Psi                (fault nodes)
+//   edge:  lambda_Fhat = SAT^T mu at edge points, through H^T (friction by
+//          the implicit function theorem at the traced V, Psi, N) into
+//          contributions gE to lambda_F at edge points, lambda_Psi, and the
+//          parameter gradients; lambda_DPsi <- A nu_f
+//   bulk:  lambda_F <- lambda_F + R^T mu + gE
+//          mu_prev  <- A mu + b_prev lambda_F
+// R^T is applied as a gather with the SBP4 matrix entries
+// D[row][col] (fd_coeff.f90:101-115, 242-246).
+#pragma once
+
+namespace fdm {
+
+// SBP4 first-derivative matrix entry (row r, column c) on n >= 12 points:
+// rows 0..3 the closure DL, the last four rows -DL mirrored, else DI.
+__device__ __forceinline__ double sbp_entry(int r, int c, int n) {
+  if (r < 4) return (c >= 0 && c < 6) ? kDL[r][c] : 0.0;
+  if (r > n - 5) {
+    const int rr = n - 1 - r, cr = n - 1 - c;
+    return (cr >= 0 && cr < 6) ? -kDL[rr][cr] : 0.0;
+  }
+  const int s = c - r;
+  if (s == 1) return FDM_DI1;
+  if (s == -1) return -FDM_DI1;
+  if (s == 2) return FDM_DI2;
+  if (s == -2) return -FDM_DI2;
+  return 0.0;
+}
+
+struct AdjBulkArgs {
+  const double* lin;  // lambda_F of the later state (5 planes)
+  double* lout;
+  const double* min;  // mu of this stage
+  double* mout;       // mu of the stage before
+  const double* xq;
+  const double* xr;
+  const double* yq;
+  const double* yr;
+  const double* Ji;
+  const int* colb;
+  const int* rowb;
+  const int* blk;
+  int nbx;
+  const BlockDev* blocks;
+  const double* gE;  // [5][NS] edge-point contributions of the edge adjoint
+  int NS;
+  int nx, ny, ld;
+  long long ps;
+  double A, Bdt_prev;
+};
+
+// R^T mu at (i, j), edge contributions, and the two recurrences
+__device__ __forceinline__ void adj_point(const AdjBulkArgs& a, const int i, const int j) {
+  const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
+  const BlockDev& B = a.blocks[bi];
+  const int nq = B.px - B.mx + 1, nr = B.py - B.my + 1;
+  const int il = i - B.mx, jl = j - B.my;
+  const long long ps = a.ps;
+  const int ld = a.ld;
+  const long long cc = (long long)j * ld + i;
+  const double gam = B.gamma;
+  const double cl[5] = {1.0 / B.rho, 1.0 / B.rho, B.M, B.G, B.M};
+  double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  // q: rows of the block's q-derivative whose stencil reaches column il
+  {
+    const double yr0 = a.yr[cc], xr0 = a.xr[cc];
+    const int r0 = max(0, il - 5), r1 = min(nq - 1, il + 5);
+    for (int r = r0; r <= r1; r++) {
+      const double D = sbp_entry(r, il, nq);
+      if (D == 0.0) continue;
+      const long long kk = (long long)j * ld + B.mx + r;
+      const double Jk = a.Ji[kk];
+      double w[5];
+#pragma unroll
+      for (int l = 0; l < 5; l++) w[l] = a.min[l * ps + kk] * Jk * cl[l];
+      const double ay = 0.5 * (a.yr[kk] + yr0) * D, ax = 0.5 * (a.xr[kk] + xr0) * D;
+      // field couplings of rates_*_mode2 (rates.f90:411-444), transposed
+      g[2] += ay * w[0];
+      g[3] -= ax * w[0];
+      g[3] += ay * w[1];
+      g[4] -= ax * w[1];
+      g[0] += ay * w[2];
+      g[1] -= ax * gam * w[2];
+      g[1] += ay * w[3];
+      g[0] -= ax * w[3];
+      g[0] += ay * gam * w[4];
+      g[1] -= ax * w[4];
+    }
+  }
+  // r: rows of the block's r-derivative whose stencil reaches row jl
+  {
+    const double yq0 = a.yq[cc], xq0 = a.xq[cc];
+    const int r0 = max(0, jl - 5), r1 = min(nr - 1, jl + 5);
+    for (int r = r0; r <= r1; r++) {
+      const double D = sbp_entry(r, jl, nr);
+      if (D == 0.0) continue;
+      const long long kk = (long long)(B.my + r) * ld + i;
+      const double Jk = a.Ji[kk];
+      double w[5];
+#pragma unroll
+      for (int l = 0; l < 5; l++) w[l] = a.min[l * ps + kk] * Jk * cl[l];
+      const double by = 0.5 * (a.yq[kk] + yq0) * D, bx = 0.5 * (a.xq[kk] + xq0) * D;
+      g[2] -= by * w[0];
+      g[3] += bx * w[0];
+      g[3] -= by * w[1];
+      g[4] += bx * w[1];
+      g[0] -= by * w[2];
+      g[1] += bx * gam * w[2];
+      g[1] -= by * w[3];
+      g[0] += bx * w[3];
+      g[0] -= by * gam * w[4];
+      g[1] += bx * w[4];
+    }
+  }
+  // SAT and hat contributions of the sides this point lies on
+  const int NS = a.NS;
+  int so[4], nso = 0;
+  if (il == 0) so[nso++] = B.side_off[FDM_SIDE_L] + jl;
+  if (il == nq - 1) so[nso++] = B.side_off[FDM_SIDE_R] + jl;
+  if (jl == 0) so[nso++] = B.side_off[FDM_SIDE_B] + il;
+  if (jl == nr - 1) so[nso++] = B.side_off[FDM_SIDE_T] + il;
+  for (int k = 0; k < nso; k++)
+#pragma unroll
+    for (int l = 0; l < 5; l++) g[l] += a.gE[l * NS + so[k]];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    const double lo = a.lin[l * ps + cc] + g[l];
+    a.lout[l * ps + cc] = lo;
+    a.mout[l * ps + cc] = a.A * a.min[l * ps + cc] + a.Bdt_prev * lo;
+  }
+}
+
+__global__ void __launch_bounds__(256) adj_bulk_kernel(AdjBulkArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= a.nx || j >= a.ny) return;
+  adj_point(a, i, j);
+}
+
+// ---- pointwise transposes ----
+
+// transpose of xy2nt: lambda of the xy fields from lambda of (vt, vn, stt,
+// snt, snn, szz)
+__device__ __forceinline__ void xy2nt_T(const NT& l, double n1, double n2, double* lF) {
+  lF[0] = n2 * l.vt + n1 * l.vn;
+  lF[1] = -n1 * l.vt + n2 * l.vn;
+  lF[2] = n2 * n2 * l.stt + n1 * n2 * l.snt + n1 * n1 * l.snn;
+  lF[3] = -2.0 * n1 * n2 * l.stt + (n2 * n2 - n1 * n1) * l.snt + 2.0 * n1 * n2 * l.snn;
+  lF[4] = n1 * n1 * l.stt - n1 * n2 * l.snt + n2 * n2 * l.snn;
+  lF[5] = l.szz;
+}
+
+// transpose of nt2xy
+__device__ __forceinline__ NT nt2xy_T(const double* lF, double n1, double n2) {
+  NT l;
+  l.vt = n2 * lF[0] - n1 * lF[1];
+  l.vn = n1 * lF[0] + n2 * lF[1];
+  l.stt = n2 * n2 * lF[2] - n1 * n2 * lF[3] + n1 * n1 * lF[4];
+  l.snt = 2.0 * n1 * n2 * lF[2] + (n2 * n2 - n1 * n1) * lF[3] - 2.0 * n1 * n2 * lF[4];
+  l.snn = n1 * n1 * lF[2] + n1 * n2 * lF[3] + n2 * n2 * lF[4];
+  l.szz = lF[5];
+  return l;
+}
+
+// SAT_term / split_sp (time_step.f90:813-851) transposed: mu (six fields,
+// szz zero) -> lambda of Fhat - F at the node
+__device__ __forceinline__ void sat_T(const double* mu, const SidesDev& sd, int so, double* ldd) {
+  const int NS = sd.NS;
+  const double n1 = sd.normal[so], n2 = sd.normal[NS + so];
+  const double Ks = sd.M[3 * NS + so], Kp = sd.M[4 * NS + so];
+  double ls[6], lp[6];
+#pragma unroll
+  for (int l = 0; l < 6; l++) {
+    ls[l] = Ks * mu[l];
+    lp[l] = Kp * mu[l];
+  }
+  const NT a = nt2xy_T(ls, n1, n2), b = nt2xy_T(lp, n1, n2);
+  NT l;
+  l.vt = a.vt;
+  l.snt = a.snt;
+  l.vn = b.vn;
+  l.stt = b.stt;
+  l.snn = b.snn;
+  l.szz = b.szz;
+  xy2nt_T(l, n1, n2, ldd);
+}
+
+// linear part of set_bc_mode2 (boundaries.f90:216-377) per wave type:
+// v = cvv fv + cvs Zi fs, s = csv Z fv + css fs
+__device__ __forceinline__ void bc_coef(int bc, double& cvv, double& cvs, double& csv, double& css) {
+  cvv = cvs = csv = css = 0.0;
+  switch (bc) {
+    case FDM_BC_MMS_SIN_W:
+    case FDM_BC_ABSORBING:
+    case FDM_BC_ABSORBING_0:
+      cvv = 0.5; cvs = -0.5; csv = -0.5; css = 0.5;
+      break;
+    case FDM_BC_FREE:
+    case FDM_BC_FREE_0:
+      cvv = 1.0; cvs = -1.0;
+      break;
+    case FDM_BC_RIGID:
+    case FDM_BC_RIGID_0:
+      csv = -1.0; css = 1.0;
+      break;
+    case FDM_BC_ABSORBING_VELOCITY:
+      cvs = -1.0; css = 1.0;
+      break;
+    case FDM_BC_FREE_VELOCITY:
+      cvv = 1.0; cvs = -1.0; css = 1.0;
+      break;
+    default:
+      break;
+  }
+}
+
+// reverse of the hat assembly shared by locked_interface_mode2 and
+// frictional_interface_mode2 (interfaces.f90:715-772, 960-1005): the hats
+// of both sides from the rotated fields p, m and the common normal stress X
+// and shear stress Y on the interface
+__device__ __forceinline__ void hats_T(const NT& lp_h, const NT& lm_h, double Zsip, double Zsim,
+                                       double Zpip, double Zpim, double gp, double gm, NT& lp,
+                                       NT& lm, double& lX, double& lY) {
+  lp.vt += lp_h.vt;
+  lp.snt += Zsip * lp_h.vt;
+  lY -= Zsip * lp_h.vt;
+  lm.vt += lm_h.vt;
+  lm.snt -= Zsim * lm_h.vt;
+  lY += Zsim * lm_h.vt;
+  lY += lp_h.snt + lm_h.snt;
+  lp.vn += lp_h.vn;
+  lp.snn += Zpip * lp_h.vn;
+  lX -= Zpip * lp_h.vn;
+  lm.vn += lm_h.vn;
+  lm.snn -= Zpim * lm_h.vn;
+  lX += Zpim * lm_h.vn;
+  lp.stt += lp_h.stt;
+  lp.snn -= gp * lp_h.stt;
+  lX += gp * lp_h.stt;
+  lm.stt += lm_h.stt;
+  lm.snn -= gm * lm_h.stt;
+  lX += gm * lm_h.stt;
+  lp.szz += lp_h.szz;
+  lp.snn -= gp * lp_h.szz;
+  lX += gp * lp_h.szz;
+  lm.szz += lm_h.szz;
+  lm.snn -= gm * lm_h.szz;
+  lX += gm * lm_h.szz;
+  lX += lp_h.snn + lm_h.snn;
+}
+
+// partial derivatives of the state rate G(V, Psi; a, b) (friction.f90:
+// 1202-1210, 1238-1243, 1254-1258) for the regularised laws
+__device__ __forceinline__ void state_rate_grad(int law, const double* rs, double V, double Psi,
+                                                double& GV, double& GPsi, double& Ga,
+                                                double& Gb) {
+  GV = GPsi = Ga = Gb = 0.0;
+  if (V == 0.0) return;
+  const double a = rs[0], b = rs[1], V0 = rs[2], f0 = rs[3], L = rs[4], fw = rs[5], Vw = rs[6];
+  const double u = fabs(V), sg = V > 0.0 ? 1.0 : -1.0;
+  if (law == FDM_LAW_RAL) {
+    const double e = exp((f0 - Psi) / b);
+    GV = -(b / L) * sg;
+    GPsi = -(V0 / L) * e;
+    Gb = (V0 / L) * (e - u / V0) - (V0 / L) * e * (f0 - Psi) / b;
+    return;
+  }
+  const double lnu = log(u / V0);
+  const double z = 0.5 * u / V0 * exp(Psi / a), sq = sqrt(1.0 + z * z);
+  const double frs = a * arcsinh_ref(z);
+  const double frs_u = a * (z / u) / sq, frs_Psi = z / sq, frs_a = frs / a - z * Psi / (a * sq);
+  double fss, fss_u, fss_a, fss_b;
+  const double fLV = f0 - (b - a) * lnu;
+  if (law == FDM_LAW_RFL) {
+    const double q = u / Vw, q8 = pow8(q), Dn = pow(1.0 + q8, 0.125);
+    const double Dn_u = Dn * (q8 / u) / (1.0 + q8);
+    fss = fw + (fLV - fw) / Dn;
+    fss_u = (-(b - a) / u) / Dn - (fLV - fw) * Dn_u / (Dn * Dn);
+    fss_a = lnu / Dn;
+    fss_b = -lnu / Dn;
+  } else {  // RSL
+    fss = fLV;
+    fss_u = -(b - a) / u;
+    fss_a = lnu;
+    fss_b = -lnu;
+  }
+  GV = ((fss - frs) / L + (u / L) * (fss_u - frs_u)) * sg;
+  GPsi = -(u / L) * frs_Psi;
+  Ga = (u / L) * (fss_a - frs_a);
+  Gb = (u / L) * fss_b;
+}
+
+struct AdjEdgeArgs {
+  const double* mu;  // mu of this stage (5 planes)
+  int ld;
+  long long ps;
+  const int2* tasks;
+  int ntasks;
+  const BlockDev* blocks;
+  SidesDev sd;
+  IfacesDev fi;
+  const double* trace;  // [3][NI] V, Psi, N of this stage
+  double* lpsi;         // [NI]
+  double* ldpsi;        // [NI]
+  double* grad;         // [FDM_GRAD_NFIELDS][NI]
+  double* gE;           // [5][NS]
+  double A, Bdt;
+};
+
+__device__ __forceinline__ void load_mu(const AdjEdgeArgs& a, long long cc, double* mu) {
+#pragma unroll
+  for (int l = 0; l < 5; l++) mu[l] = a.mu[l * a.ps + cc];
+  mu[5] = 0.0;
+}
+
+__device__ __forceinline__ void store_gE(const AdjEdgeArgs& a, int so, const double* lF,
+                                         const double* ldd) {
+#pragma unroll
+  for (int l = 0; l < 5; l++) a.gE[l * a.sd.NS + so] = lF[l] - ldd[l];
+}
+
+__global__ void __launch_bounds__(64) adj_edge_kernel(AdjEdgeArgs a) {
+  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (tid >= a.ntasks) return;
+  const int2 task = a.tasks[tid];
+  const int NS = a.sd.NS;
+  if (task.x >= 0) {
+    // boundary node: SAT and set_bc_mode2 transposed
+    const int b = task.x >> 2, s = task.x & 3, k = task.y;
+    const BlockDev& B = a.blocks[b];
+    int i, j;
+    if (s == FDM_SIDE_L) { i = B.mx; j = B.my + k; }
+    else if (s == FDM_SIDE_R) { i = B.px; j = B.my + k; }
+    else if (s == FDM_SIDE_B) { i = B.mx + k; j = B.my; }
+    else { i = B.mx + k; j = B.py; }
+    const int so = B.side_off[s] + k;
+    double mu[6], ldd[6], lF[6];
+    load_mu(a, (long long)j * a.ld + i, mu);
+    sat_T(mu, a.sd, so, ldd);
+    const double n1 = a.sd.normal[so], n2 = a.sd.normal[NS + so];
+    const double Zs = a.sd.M[so], Zp = a.sd.M[NS + so], gamma = a.sd.M[2 * NS + so];
+    const double Zsi = (Zs == 0.0) ? 0.0 : 1.0 / Zs, Zpi = 1.0 / Zp;
+    const NT lh = nt2xy_T(ldd, n1, n2);
+    NT lf;
+    lf.stt = lh.stt;
+    lf.szz = lh.szz;
+    const double lsnn = lh.snn + gamma * (lh.stt + lh.szz);
+    double cvv, cvs, csv, css;
+    bc_coef(B.side_bc[s], cvv, cvs, csv, css);
+    lf.vt = cvv * lh.vt + csv * Zs * lh.snt;
+    lf.snt = cvs * Zsi * lh.vt + css * lh.snt;
+    lf.vn = cvv * lh.vn + csv * Zp * lsnn;
+    lf.snn = cvs * Zpi * lh.vn + css * lsnn - gamma * (lh.stt + lh.szz);
+    xy2nt_T(lf, n1, n2, lF);
+    store_gE(a, so, lF, ldd);
+    return;
+  }
+  // interface node
+  const IfaceDev& I = a.fi.tab[-task.x - 1];
+  const int k = task.y, g = I.m + k, q = I.node_off + k;
+  const int NI = a.fi.NI;
+  const BlockDev& Bm = a.blocks[I.bm];
+  const BlockDev& Bp = a.blocks[I.bp];
+  int im, jm, ip, jp, km, kp, sm, sp;
+  if (I.direction == 0) {
+    sm = FDM_SIDE_R; sp = FDM_SIDE_L;
+    im = Bm.px; jm = g; ip = Bp.mx; jp = g;
+    km = g - Bm.my; kp = g - Bp.my;
+  } else {
+    sm = FDM_SIDE_T; sp = FDM_SIDE_B;
+    im = g; jm = Bm.py; ip = g; jp = Bp.my;
+    km = g - Bm.mx; kp = g - Bp.mx;
+  }
+  const int som = Bm.side_off[sm] + km, sop = Bp.side_off[sp] + kp;
+  double mum[6], mup[6], lddm[6], lddp[6];
+  load_mu(a, (long long)jm * a.ld + im, mum);
+  load_mu(a, (long long)jp * a.ld + ip, mup);
+  sat_T(mum, a.sd, som, lddm);
+  sat_T(mup, a.sd, sop, lddp);
+  const double n1 = a.fi.nhat[q], n2 = a.fi.nhat[NI + q];
+  const double Zsm = a.sd.M[som], Zsp = a.sd.M[sop];
+  const double Zpm = a.sd.M[NS + som], Zpp = a.sd.M[NS + sop];
+  const double gm = a.sd.M[2 * NS + som], gp = a.sd.M[2 * NS + sop];
+  const double Zsip = 1.0 / Zsp, Zsim = 1.0 / Zsm, Zpip = 1.0 / Zpp, Zpim = 1.0 / Zpm;
+  const double etas = 1.0 / (Zsip + Zsim), etap = 1.0 / (Zpip + Zpim);
+  const NT lhp = nt2xy_T(lddp, n1, n2), lhm = nt2xy_T(lddm, n1, n2);
+  NT lp{0, 0, 0, 0, 0, 0}, lm{0, 0, 0, 0, 0, 0};
+  double lX = 0.0, lY = 0.0, lphis = 0.0, lphip = 0.0;
+  hats_T(lhp, lhm, Zsip, Zsim, Zpip, Zpim, gp, gm, lp, lm, lX, lY);
+  if (I.coupling == FDM_COUPLING_LOCKED) {
+    lphip = lX;
+    lphis = lY;
+  } else {
+    // frictional node: S = Y + fS0, N from X; V from the friction solve
+    double rs[7];
+#pragma unroll
+    for (int c = 0; c < 7; c++) rs[c] = a.fi.rs[(long long)c * NI + q];
+    const double V = a.trace[q], Psi = a.trace[NI + q], N = a.trace[2 * NI + q];
+    const double skm = 1.0 - I.skempton;
+    double* gr = a.grad;
+    // state variable: nu = lambda_DPsi + b lambda_Psi, DPsi <- A DPsi + G
+    const double lpsi_next = a.lpsi[q];
+    const double nu = a.ldpsi[q] + a.Bdt * lpsi_next;
+    double GV, GPsi, Ga, Gb;
+    state_rate_grad(I.law, rs, V, Psi, GV, GPsi, Ga, Gb);
+    double lPsi = lpsi_next + GPsi * nu;
+    double la = Ga * nu, lb = Gb * nu;
+    double lV = GV * nu;
+    // snn = -(N - fN0)/skm, snt = S - fS0
+    double lN = -lX / skm;
+    double lS = lY, lfS0 = -lY;
+    double lSlock = 0.0;
+    if (N <= 0.0) {
+      // tensile branch: S = 0, V = Slock / eta (friction.f90:675-683)
+      lSlock += lV / etas;
+    } else {
+      // S = Slock - eta V; R(V) = -(Slock - eta V) + a N asinh(O V) = 0
+      lSlock += lS;
+      lV -= etas * lS;
+      const double aa = rs[0], V0 = rs[2];
+      const double O = 0.5 / V0 * exp(Psi / aa), z = O * V, sq = sqrt(1.0 + z * z);
+      const double RV = etas + aa * N * O / sq;
+      const double ash = arcsinh_ref(z);
+      const double RN = aa * ash, RPsi = N * z / sq, Ra = N * ash - N * z * Psi / (aa * sq);
+      const double t = lV / RV;
+      lSlock += t;
+      lN -= t * RN;
+      lPsi -= t * RPsi;
+      la -= t * Ra;
+    }
+    // N = fN0 - skm phip, Slock = fS0 + phis
+    lphip = -skm * lN;
+    lfS0 += lSlock;
+    lphis = lSlock;
+    a.lpsi[q] = lPsi;
+    a.ldpsi[q] = a.A * nu;
+    gr[FDM_GRAD_S0 * NI + q] += lfS0;
+    gr[FDM_GRAD_A * NI + q] += la;
+    gr[FDM_GRAD_B * NI + q] += lb;
+  }
+  // phis = etas (p.snt Zsip + m.snt Zsim + p.vt - m.vt), phip likewise
+  lp.snt += etas * Zsip * lphis;
+  lm.snt += etas * Zsim * lphis;
+  lp.vt += etas * lphis;
+  lm.vt -= etas * lphis;
+  lp.snn += etap * Zpip * lphip;
+  lm.snn += etap * Zpim * lphip;
+  lp.vn += etap * lphip;
+  lm.vn -= etap * lphip;
+  double lFp[6], lFm[6];
+  xy2nt_T(lp, n1, n2, lFp);
+  xy2nt_T(lm, n1, n2, lFm);
+  store_gE(a, sop, lFp, lddp);
+  store_gE(a, som, lFm, lddm);
+}
+
+// vx, vy at the receivers after a step
+__global__ void __launch_bounds__(64) adj_record_kernel(const double* F, long long ps,
+                                                        const long long* cc, int nrec, double* out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  out[2 * r] = F[cc[r]];
+  out[2 * r + 1] = F[ps + cc[r]];
+}
+
+// lambda_F += g and mu += b g at the receivers (one lane: receivers may
+// share a grid point)
+__global__ void __launch_bounds__(64) adj_inject_kernel(double* lF, double* mu, long long ps,
+                                                        const long long* cc, int nrec,
+                                                        const double* g, double b) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (int r = 0; r < nrec; r++) {
+    const long long k = cc[r];
+    lF[k] += g[2 * r];
+    lF[ps + k] += g[2 * r + 1];
+    mu[k] += b * g[2 * r];
+    mu[ps + k] += b * g[2 * r + 1];
+  }
+}
+
+}  // namespace fdm
+
+struct fdm_adjoint {
+  fdm_ctx* c = nullptr;
+  int nrec = 0, nsteps = 0, interval = 0, nseg = 0;
+  long long* rec_cc = nullptr;   // [nrec] plane offsets of the receivers
+  double* rec = nullptr;         // [nsteps][nrec][2]
+  double* rec_bar = nullptr;     // [nsteps][nrec][2]
+  double* ckpt = nullptr;        // [nseg][6 ps + 18 NI]
+  double* trace = nullptr;       // [interval][nstage][3][NI]
+  bool trace_valid = false;      // trace holds the last interval of the latest forward pass
+  bool have_forward = false;
+  double t = 0.0, dt = 0.0;
+  double* planes = nullptr;      // lambda_F x 2, mu x 2 (5 planes each, stride ps)
+  double* lF[2] = {nullptr, nullptr};
+  double* mu[2] = {nullptr, nullptr};
+  double* gE = nullptr;          // [5][NS]
+  double* lpsi = nullptr;        // [NI]
+  double* ldpsi = nullptr;       // [NI]
+  double* grad = nullptr;        // [FDM_GRAD_NFIELDS][NI]
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  double ms_fwd = 0.0, ms_rec = 0.0, ms_sweep = 0.0;
+  std::vector<void*> allocs;
+  std::string err;
+};
+
+namespace {
+
+int adj_err(fdm_adjoint* a, int rc, const std::string& msg) {
+  if (a) a->err = msg;
+  else g_create_error = msg;
+  return rc;
+}
+
+#define ADJCHK(adj, call)                                                              \
+  do {                                                                                 \
+    hipError_t _st = (call);                                                           \
+    if (_st != hipSuccess)                                                             \
+      return adj_err(adj, FDM_EDEVICE, std::string(#call ": ") + hipGetErrorString(_st)); \
+  } while (0)
+
+template <typename P>
+int adj_alloc(fdm_adjoint* a, P** p, size_t bytes) {
+  void* q = nullptr;
+  if (hipMalloc(&q, std::max<size_t>(bytes, 64)) != hipSuccess)
+    return adj_err(a, FDM_ENOMEM, "fdm_adjoint: device allocation of " + std::to_string(bytes) + " bytes failed");
+  a->allocs.push_back(q);
+  *p = (P*)q;
+  return FDM_OK;
+}
+
+size_t ckpt_elems(const fdm_ctx* c) { return (size_t)6 * c->ps + (size_t)FDM_IF_NFIELDS * c->NI; }
+
+int adj_checkpoint(fdm_adjoint* a, int seg, bool save) {
+  fdm_ctx* c = a->c;
+  double* base = a->ckpt + (size_t)seg * ckpt_elems(c);
+  double* F = (double*)c->F[c->cur];
+  const size_t fb = (size_t)6 * c->ps * 8, sb = (size_t)FDM_IF_NFIELDS * c->NI * 8;
+  if (save) {
+    ADJCHK(a, hipMemcpyAsync(base, F, fb, hipMemcpyDeviceToDevice, c->stream));
+    if (sb) ADJCHK(a, hipMemcpyAsync(base + 6 * c->ps, c->ist, sb, hipMemcpyDeviceToDevice, c->stream));
+  } else {
+    ADJCHK(a, hipMemcpyAsync(F, base, fb, hipMemcpyDeviceToDevice, c->stream));
+    if (sb) ADJCHK(a, hipMemcpyAsync(c->ist, base + 6 * c->ps, sb, hipMemcpyDeviceToDevice, c->stream));
+    c->df6_stale = false;
+  }
+  return FDM_OK;
+}
+
+// forward steps n0..n1 (1-based); traces of the steps go to the trace
+// buffer when `record` is set
+int adj_run_steps(fdm_adjoint* a, int n0, int n1, bool record, bool save_ckpt, bool receivers) {
+  fdm_ctx* c = a->c;
+  const int K = a->interval;
+  for (int n = n0; n <= n1; n++) {
+    if (save_ckpt && (n - 1) % K == 0) {
+      int rc = adj_checkpoint(a, (n - 1) / K, true);
+      if (rc) return rc;
+    }
+    const int slot = (n - 1) % K;
+    c->trace = record ? a->trace + (size_t)slot * c->nstage * 3 * c->NI : nullptr;
+    int rc = step_impl<double>(c, a->t, a->dt, n, 1);
+    c->trace = nullptr;
+    if (rc) return adj_err(a, rc, c->err);
+    if (receivers && a->nrec) {
+      hipLaunchKernelGGL(adj_record_kernel, dim3((a->nrec + 63) / 64), dim3(64), 0, c->stream,
+                         (const double*)c->F[c->cur], c->ps, a->rec_cc, a->nrec,
+                         a->rec + (size_t)(n - 1) * a->nrec * 2);
+      ADJCHK(a, hipGetLastError());
+    }
+  }
+  return FDM_OK;
+}
+
+// backward stage s (1-based) of step n
+int adj_stage(fdm_adjoint* a, int n, int s, int& cur) {
+  fdm_ctx* c = a->c;
+  const int K = a->interval, ns = c->nstage;
+  const double A = c->rkA[s - 1], Bdt = c->rkB[s - 1] * a->dt;
+  const double Bprev = (s > 1 ? c->rkB[s - 2] : c->rkB[ns - 1]) * a->dt;
+  if (c->ntasks) {
+    AdjEdgeArgs e;
+    e.mu = a->mu[cur];
+    e.ld = c->ld;
+    e.ps = c->ps;
+    e.tasks = c->tasks;
+    e.ntasks = c->ntasks;
+    e.blocks = c->blocks;
+    e.sd = SidesDev{c->sM, c->snormal, c->sF0, c->sx, c->sy, c->sFhat, c->NS};
+    e.fi = IfacesDev{c->ifaces, c->inhat, c->irs, c->isw, c->iS0b, c->iN0b, c->ildS, c->ildN,
+                     c->iout, c->ist, c->ist1, c->NI};
+    e.trace = a->trace + ((size_t)((n - 1) % K) * ns + (s - 1)) * 3 * c->NI;
+    e.lpsi = a->lpsi;
+    e.ldpsi = a->ldpsi;
+    e.grad = a->grad;
+    e.gE = a->gE;
+    e.A = A;
+    e.Bdt = Bdt;
+    hipLaunchKernelGGL(adj_edge_kernel, dim3((c->ntasks + 63) / 64), dim3(64), 0, c->stream, e);
+  }
+  AdjBulkArgs b;
+  b.lin = a->lF[cur];
+  b.lout = a->lF[1 - cur];
+  b.min = a->mu[cur];
+  b.mout = a->mu[1 - cur];
+  b.xq = (const double*)c->met[0];
+  b.xr = (const double*)c->met[1];
+  b.yq = (const double*)c->met[2];
+  b.yr = (const double*)c->met[3];
+  b.Ji = (const double*)c->met[4];
+  b.colb = c->colb;
+  b.rowb = c->rowb;
+  b.blk = c->blk;
+  b.nbx = c->nbx;
+  b.blocks = c->blocks;
+  b.gE = a->gE;
+  b.NS = c->NS;
+  b.nx = c->nx;
+  b.ny = c->ny;
+  b.ld = c->ld;
+  b.ps = c->ps;
+  b.A = A;
+  b.Bdt_prev = Bprev;
+  hipLaunchKernelGGL(adj_bulk_kernel, dim3((c->nx + 63) / 64, (c->ny + 3) / 4), dim3(64, 4), 0,
+                     c->stream, b);
+  ADJCHK(a, hipGetLastError());
+  cur = 1 - cur;
+  return FDM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* fdm_adjoint_last_error(const fdm_adjoint* a) { return a ? a->err.c_str() : g_create_error.c_str(); }
+
+void fdm_adjoint_destroy(fdm_adjoint* a) {
+  if (!a) return;
+  if (a->c && a->c->stream) (void)hipStreamSynchronize(a->c->stream);
+  for (hipEvent_t e : a->ev)
+    if (e) (void)hipEventDestroy(e);
+  for (void* p : a->allocs) (void)hipFree(p);
+  delete a;
+}
+
+int fdm_adjoint_create(fdm_ctx* c, int32_t nrec, const int32_t* rec_i, const int32_t* rec_j,
+                       int32_t nsteps, int32_t interval, fdm_adjoint** out) {
+  if (!c || !out || nsteps < 1 || nrec < 0 || (nrec && (!rec_i || !rec_j)))
+    return adj_err(nullptr, FDM_EINVAL, "fdm_adjoint_create: invalid arguments");
+  *out = nullptr;
+  if (c->precision != FDM_FP64 || c->any_plastic)
+    return adj_err(nullptr, FDM_EINVAL, "fdm_adjoint_create: needs an FP64 context without plasticity");
+  for (const BlockDev& B : c->hblocks)
+    if (B.px - B.mx + 1 < 12 || B.py - B.my + 1 < 12)
+      return adj_err(nullptr, FDM_EINVAL, "fdm_adjoint_create: blocks must be at least 12 x 12 points");
+  if (c->NI) {
+    std::vector<int> outside(c->NI);
+    if (hipMemcpy(outside.data(), c->iout, outside.size() * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+      return adj_err(nullptr, FDM_EDEVICE, "fdm_adjoint_create: reading the interface maps failed");
+    for (const IfaceDev& I : c->hifaces) {
+      if (I.coupling == FDM_COUPLING_LOCKED) continue;
+      const bool law_ok = I.law == FDM_LAW_RSL || I.law == FDM_LAW_RFL || I.law == FDM_LAW_RAL;
+      bool out_any = false;
+      for (int k = 0; k < I.n; k++) out_any = out_any || outside[I.node_off + k];
+      if (I.coupling != FDM_COUPLING_FRICTION || !law_ok || I.reg || out_any)
+        return adj_err(nullptr, FDM_EINVAL,
+                       "fdm_adjoint_create: interfaces must be locked or frictional with law RSL, RFL "
+                       "or RAL, without normal-stress regularisation or locked region");
+    }
+  }
+  for (int r = 0; r < nrec; r++)
+    if (rec_i[r] < 0 || rec_i[r] >= c->nx || rec_j[r] < 0 || rec_j[r] >= c->ny)
+      return adj_err(nullptr, FDM_EINVAL, "fdm_adjoint_create: receiver outside the grid");
+  fdm_adjoint* a = new fdm_adjoint();
+  a->c = c;
+  a->nrec = nrec;
+  a->nsteps = nsteps;
+  a->interval = interval > 0 ? std::min(interval, nsteps) : (int)std::ceil(std::sqrt((double)nsteps));
+  a->nseg = (nsteps + a->interval - 1) / a->interval;
+  std::vector<long long> cc(std::max(1, nrec));
+  for (int r = 0; r < nrec; r++) cc[r] = (long long)rec_j[r] * c->ld + rec_i[r];
+  const size_t NI = std::max(1, c->NI), NS = std::max(1, c->NS);
+  int rc = adj_alloc(a, &a->rec_cc, cc.size() * sizeof(long long));
+  if (!rc && hipMemcpy(a->rec_cc, cc.data(), cc.size() * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess)
+    rc = adj_err(a, FDM_EDEVICE, "fdm_adjoint_create: upload failed");
+  const size_t nrv = (size_t)nsteps * std::max(1, nrec) * 2;
+  if (!rc) rc = adj_alloc(a, &a->rec, nrv * 8);
+  if (!rc) rc = adj_alloc(a, &a->rec_bar, nrv * 8);
+  if (!rc) rc = adj_alloc(a, &a->ckpt, (size_t)a->nseg * ckpt_elems(c) * 8);
+  if (!rc) rc = adj_alloc(a, &a->trace, (size_t)a->interval * c->nstage * 3 * NI * 8);
+  if (!rc) rc = adj_alloc(a, &a->planes, ((size_t)20 * c->ps + c->coff + 64) * 8);
+  if (!rc) rc = adj_alloc(a, &a->gE, (size_t)5 * NS * 8);
+  if (!rc) rc = adj_alloc(a, &a->lpsi, NI * 8);
+  if (!rc) rc = adj_alloc(a, &a->ldpsi, NI * 8);
+  if (!rc) rc = adj_alloc(a, &a->grad, (size_t)FDM_GRAD_NFIELDS * NI * 8);
+  for (int k = 0; k < 4 && !rc; k++)
+    if (hipEventCreate(&a->ev[k]) != hipSuccess) rc = adj_err(a, FDM_EDEVICE, "fdm_adjoint_create: event creation failed");
+  if (rc) {
+    g_create_error = a->err;
+    fdm_adjoint_destroy(a);
+    return rc;
+  }
+  double* pb = a->planes + c->coff;
+  a->lF[0] = pb;
+  a->lF[1] = pb + 5 * c->ps;
+  a->mu[0] = pb + 10 * c->ps;
+  a->mu[1] = pb + 15 * c->ps;
+  *out = a;
+  return FDM_OK;
+}
+
+int fdm_adjoint_info(const fdm_adjoint* a, int32_t* nodes, int32_t* interval) {
+  if (!a) return FDM_EINVAL;
+  if (nodes) *nodes = a->c->NI;
+  if (interval) *interval = a->interval;
+  return FDM_OK;
+}
+
+int fdm_adjoint_set_param(fdm_adjoint* a, int32_t field, const double* values) {
+  if (!a || !values) return FDM_EINVAL;
+  fdm_ctx* c = a->c;
+  double* dst = nullptr;
+  if (field == FDM_GRAD_S0) dst = c->iS0b;
+  else if (field == FDM_GRAD_A) dst = c->irs;
+  else if (field == FDM_GRAD_B) dst = c->irs + c->NI;
+  else return adj_err(a, FDM_EINVAL, "fdm_adjoint_set_param: field must be S0, a or b");
+  if (!c->NI) return FDM_OK;
+  ADJCHK(a, hipStreamSynchronize(c->stream));
+  ADJCHK(a, hipMemcpy(dst, values, (size_t)c->NI * 8, hipMemcpyHostToDevice));
+  a->have_forward = false;
+  return FDM_OK;
+}
+
+int fdm_adjoint_forward(fdm_adjoint* a, double t, double dt, double* rec) {
+  if (!a) return FDM_EINVAL;
+  fdm_ctx* c = a->c;
+  if (c->prof) return adj_err(a, FDM_EINVAL, "fdm_adjoint_forward: disable profiling");
+  a->t = t;
+  a->dt = dt;
+  const int last0 = (a->nseg - 1) * a->interval + 1;
+  ADJCHK(a, hipEventRecord(a->ev[0], c->stream));
+  int rc = adj_run_steps(a, 1, last0 - 1, false, true, true);
+  if (!rc) rc = adj_run_steps(a, last0, a->nsteps, true, true, true);
+  if (rc) return rc;
+  ADJCHK(a, hipEventRecord(a->ev[1], c->stream));
+  ADJCHK(a, hipEventSynchronize(a->ev[1]));
+  float ms = 0.0f;
+  ADJCHK(a, hipEventElapsedTime(&ms, a->ev[0], a->ev[1]));
+  a->ms_fwd = ms;
+  rc = check_fail(c);
+  if (rc) return adj_err(a, rc, c->err);
+  if (rec && a->nrec)
+    ADJCHK(a, hipMemcpy(rec, a->rec, (size_t)a->nsteps * a->nrec * 2 * 8, hipMemcpyDeviceToHost));
+  a->trace_valid = true;
+  a->have_forward = true;
+  return FDM_OK;
+}
+
+int fdm_adjoint_backward(fdm_adjoint* a, const double* rec_bar, double* grad, double* gF) {
+  if (!a || (a->nrec && !rec_bar)) return FDM_EINVAL;
+  if (!a->have_forward) return adj_err(a, FDM_EINVAL, "fdm_adjoint_backward: no forward pass");
+  fdm_ctx* c = a->c;
+  const int K = a->interval, ns = c->nstage;
+  if (a->nrec)
+    ADJCHK(a, hipMemcpy(a->rec_bar, rec_bar, (size_t)a->nsteps * a->nrec * 2 * 8, hipMemcpyHostToDevice));
+  ADJCHK(a, hipMemsetAsync(a->planes, 0, ((size_t)20 * c->ps + c->coff + 64) * 8, c->stream));
+  ADJCHK(a, hipMemsetAsync(a->gE, 0, (size_t)5 * std::max(1, c->NS) * 8, c->stream));
+  const size_t NI = std::max(1, c->NI);
+  ADJCHK(a, hipMemsetAsync(a->lpsi, 0, NI * 8, c->stream));
+  ADJCHK(a, hipMemsetAsync(a->ldpsi, 0, NI * 8, c->stream));
+  ADJCHK(a, hipMemsetAsync(a->grad, 0, (size_t)FDM_GRAD_NFIELDS * NI * 8, c->stream));
+  int cur = 0;
+  const double b_last = c->rkB[ns - 1] * a->dt;
+  float ms_rec = 0.0f, ms_sweep = 0.0f;
+  for (int seg = a->nseg - 1; seg >= 0; seg--) {
+    const int n0 = seg * K + 1, n1 = std::min(a->nsteps, n0 + K - 1);
+    ADJCHK(a, hipEventRecord(a->ev[0], c->stream));
+    if (!(seg == a->nseg - 1 && a->trace_valid)) {
+      int rc = adj_checkpoint(a, seg, false);
+      if (!rc) rc = adj_run_steps(a, n0, n1, true, false, false);
+      if (rc) return rc;
+    }
+    ADJCHK(a, hipEventRecord(a->ev[1], c->stream));
+    for (int n = n1; n >= n0; n--) {
+      if (a->nrec) {
+        hipLaunchKernelGGL(adj_inject_kernel, dim3(1), dim3(64), 0, c->stream, a->lF[cur], a->mu[cur],
+                           c->ps, a->rec_cc, a->nrec, a->rec_bar + (size_t)(n - 1) * a->nrec * 2, b_last);
+        ADJCHK(a, hipGetLastError());
+      }
+      for (int s = ns; s >= 1; s--) {
+        int rc = adj_stage(a, n, s, cur);
+        if (rc) return rc;
+      }
+    }
+    ADJCHK(a, hipEventRecord(a->ev[2], c->stream));
+    ADJCHK(a, hipEventSynchronize(a->ev[2]));
+    float m1 = 0.0f, m2 = 0.0f;
+    ADJCHK(a, hipEventElapsedTime(&m1, a->ev[0], a->ev[1]));
+    ADJCHK(a, hipEventElapsedTime(&m2, a->ev[1], a->ev[2]));
+    ms_rec += m1;
+    ms_sweep += m2;
+  }
+  a->trace_valid = (a->nseg == 1);
+  a->ms_rec = ms_rec;
+  a->ms_sweep = ms_sweep;
+  int rc = check_fail(c);
+  if (rc) return adj_err(a, rc, c->err);
+  if (c->NI) {
+    ADJCHK(a, hipMemcpy(a->grad + (size_t)FDM_GRAD_PSI0 * c->NI, a->lpsi, (size_t)c->NI * 8,
+                        hipMemcpyDeviceToDevice));
+    if (grad)
+      ADJCHK(a, hipMemcpy(grad, a->grad, (size_t)FDM_GRAD_NFIELDS * c->NI * 8, hipMemcpyDeviceToHost));
+  }
+  if (gF) {
+    std::vector<double> h((size_t)5 * c->ps);
+    ADJCHK(a, hipMemcpy(h.data(), a->lF[cur], h.size() * 8, hipMemcpyDeviceToHost));
+    const size_t np = (size_t)c->nx * c->ny;
+    for (int l = 0; l < 6; l++)
+      for (int j = 0; j < c->ny; j++)
+        for (int i = 0; i < c->nx; i++)
+          gF[l * np + (size_t)j * c->nx + i] = l < 5 ? h[(size_t)l * c->ps + (size_t)j * c->ld + i] : 0.0;
+  }
+  return FDM_OK;
+}
+
+int fdm_adjoint_timing(const fdm_adjoint* a, double* fwd_ms, double* recompute_ms, double* sweep_ms) {
+  if (!a) return FDM_EINVAL;
+  if (fwd_ms) *fwd_ms = a->ms_fwd;
+  if (recompute_ms) *recompute_ms = a->ms_rec;
+  if (sweep_ms) *sweep_ms = a->ms_sweep;
+  return FDM_OK;
+}
+
+}  // extern "C"

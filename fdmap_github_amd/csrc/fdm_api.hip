@@ -243,6 +243,10 @@ struct fdm_ctx {
   bool graph_mode = false;
   double g_tinit = 0.0, g_dt = 0.0, g_rkc = 0.0;
   long long ninterior = 0;
+  // adjoint forward passes: fault trajectories of the current step
+  // ([nstage][3][NI]), and the slot of the stage being launched
+  double* trace = nullptr;
+  double* trace_cur = nullptr;
 };
 
 // block index of global point (i, j)
@@ -384,6 +388,7 @@ EdgeArgs<T> edge_args(fdm_ctx* c, double t, double Bdt, double A, int do_scale, 
   a.do_update = do_update;
   a.snapshot = snapshot;
   a.fail = c->fail;
+  a.trace = c->trace_cur;
   return a;
 }
 
@@ -532,14 +537,18 @@ int run_stage(fdm_ctx* c, int stage, double t0, double dt, int final_step, int s
     // band needs this stage's edge values and the previous interior launch.
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->evB, 0));
     launch_interior<T>(c, c->stream, a);
+    c->trace_cur = (c->trace && !final_step) ? c->trace + (size_t)(stage - 1) * 3 * c->NI : nullptr;
     rc = launch_edges<T>(c, c->side, t, Bdt, A, 1, final_step ? 0 : 1, snapshot, Bload);
+    c->trace_cur = nullptr;
     if (rc) return rc;
     HIPCHK(c, hipStreamWaitEvent(c->side, c->evI, 0));
     launch_band<T>(c, c->side, a);
     HIPCHK(c, hipEventRecord(c->evB, c->side));
     HIPCHK(c, hipEventRecord(c->evI, c->stream));
   } else {
+    c->trace_cur = (c->trace && !final_step) ? c->trace + (size_t)(stage - 1) * 3 * c->NI : nullptr;
     rc = launch_edges<T>(c, c->stream, t, Bdt, A, 1, final_step ? 0 : 1, snapshot, Bload);
+    c->trace_cur = nullptr;
     if (rc) return rc;
     launch_pointwise<T>(c, c->stream, a);
   }
@@ -1652,3 +1661,7 @@ int fdm_ensemble_sync(fdm_ensemble* e) {
 }
 
 }  // extern "C"
+
+// discrete adjoint (include/fdmap_adjoint.h)
+#include "../../include/fdmap_adjoint.h"
+#include "fdm_adjoint.hpp"

@@ -85,6 +85,9 @@ struct EdgeArgs {
   // the host uses, instead of t
   const long long* nstep;
   double t_init, dt, rkc;
+  // when set, [3][NI]: slip velocity, state variable at the start of the
+  // stage and normal stress of every frictional node (adjoint trajectories)
+  double* trace;
 };
 
 template <typename T>
@@ -231,6 +234,11 @@ __device__ __forceinline__ void edge_task(const EdgeArgs<T>& a, const int2 task)
     bool fail = solve_friction_core(I.law, rs, sw, I.reg, a.fi.outside[q], I.flock, fS0, fN0,
                                     phip, phis, etas, D, Psi, I.skempton, a.Bdt, V, S, O, N, Nreg);
     if (fail) *a.fail = 1;
+    if (a.trace) {
+      a.trace[q] = V;
+      a.trace[NI + q] = Psi;
+      a.trace[2 * NI + q] = N;
+    }
     ST(FDM_IF_V) = V;
     ST(FDM_IF_S) = S;
     ST(FDM_IF_O) = O;
