@@ -244,6 +244,121 @@ def run_c4(args, tmp):
     print(json.dumps(out), flush=True)
 
 
+def run_c5(args, tmp):
+    """Adjoint gradient: the planar rate-and-state rupture (C2 parameters)
+    on two blocks of 801 x 401 points with a free surface on top, 2000
+    steps; misfit 0.5 dt sum (v - v_target)^2 of vx, vy at 32 free-surface
+    receivers against the traces of a run whose fault shear prestress
+    carries a seeded Gaussian random perturbation (1 MPa rms, correlation
+    length 1 km).  Gradient with respect to the 801 fault shear-prestress
+    values, a and b (and the initial state), by the hand-written adjoint
+    (the fault trace kept in HBM, and with square-root checkpointing and
+    recomputation); its time against one forward run, and
+    central differences (eps = 1e-6) along one seeded direction per
+    parameter."""
+    from fdmap_github_amd import ensemble, host
+    from fdmap_github_amd.adjoint import FaultInversion
+    from fdmap_github_amd.device import Context
+
+    steps = 2000 if args.steps == 4000 else args.steps
+    nx, ny, h = 801, 401, 0.025
+    rng = np.random.default_rng(5)
+    sc = dict(index=0, nx=nx, ny=ny, a=0.010, S0=np.full(nx, 70.0), h=h)
+    path = ensemble.write_scenario(tmp, sc, steps + 10, bcT="free")
+    hp = host.HostProblem(path, precision="fp64")
+    ctx = Context(hp.ptr)
+    ctx.set_fields(hp.fields())
+    ctx.init_edges(hp.t)
+    ri = np.linspace(0.05 * (nx - 1), 0.95 * (nx - 1), 32).round().astype(np.int32)
+    rj = np.full(32, hp.ny - 1, dtype=np.int32)
+    n = nx
+    p0 = {"S0": np.full(n, 70.0), "a": np.full(n, 0.010), "b": np.full(n, 0.014)}
+    pert = ensemble.correlated_lognormal(n, h, 1.0, 1.0, 1.0, rng)
+    pert = (np.log(pert) + 0.5) / np.std(np.log(pert))  # unit-variance Gaussian field
+    inv = FaultInversion(ctx, hp.t, hp.dt, ri, rj, steps, interval=0)  # automatic policy
+    target = inv.run(S0=p0["S0"] + pert, a=p0["a"], b=p0["b"])
+    dt = hp.dt
+
+    def misfit(rec):
+        return 0.5 * dt * float(np.sum((rec - target) ** 2))
+
+    # one plain forward run (fdm_step), after a short warm-up
+    inv.reset()
+    ctx.step(hp.t, dt, 1, 20)
+    ctx.sync()
+    inv.adj.set_param("S0", p0["S0"])
+    inv.reset()
+    t0 = time.perf_counter()
+    ctx.step(hp.t, dt, 1, steps)
+    ctx.sync()
+    t_fwd = time.perf_counter() - t0
+    vmax = ctx.max_slip_velocity()
+    out_modes = {}
+    grads = None
+    for label, itv in (("auto", 0), ("sqrtN", -1)):
+        fi = inv if itv == 0 else FaultInversion(ctx, hp.t, dt, ri, rj, steps, interval=itv)
+        if fi is not inv:
+            fi.F0, fi.DF0, fi.iface0 = inv.F0, inv.DF0, inv.iface0
+        fi.run(**p0)  # warm-up of this workspace
+        fi.adj.backward(dt * np.zeros((steps, 32, 2)))
+        t0 = time.perf_counter()
+        rec = fi.run(**p0)
+        t1 = time.perf_counter()
+        g = fi.adj.backward(dt * (rec - target))
+        t2 = time.perf_counter()
+        tm = fi.adj.timing()
+        out_modes[label] = {"interval": fi.adj.interval, "forward_pass_s": t1 - t0,
+                            "backward_pass_s": t2 - t1, "gradient_s": t2 - t0,
+                            "ratio_to_forward": (t2 - t0) / t_fwd,
+                            "device_ms": {k: round(v, 3) for k, v in tm.items()}}
+        if grads is None:
+            grads, J0 = g, misfit(rec)
+        else:
+            out_modes[label]["same_gradient_as_auto"] = all(
+                np.array_equal(g[k], grads[k]) for k in ("S0", "a", "b", "Psi0"))
+            fi.adj.close()
+    # central differences at eps = 1e-6 along seeded directions (N(0, 1)
+    # entries times 10 MPa for S0, a for a, 10 for b, 1 for the initial state)
+    eps = 1e-6
+    fd = {}
+    for name, scale in (("S0", 10.0), ("a", p0["a"]), ("b", 10.0), ("Psi0", 1.0)):
+        d = rng.standard_normal(n) * scale
+        vals = []
+        for sgn in (1.0, -1.0):
+            if name == "Psi0":
+                psi0 = inv.iface0[0]["Psi"].copy()
+                inv.iface0[0]["Psi"] = psi0 + sgn * eps * d
+                vals.append(misfit(inv.run(**p0)))
+                inv.iface0[0]["Psi"] = psi0
+            else:
+                p = dict(p0)
+                p[name] = p0[name] + sgn * eps * d
+                vals.append(misfit(inv.run(**p)))
+        fdv = (vals[0] - vals[1]) / (2 * eps)
+        ad = float(grads[name] @ d)
+        fd[name] = {"fd": fdv, "adjoint": ad,
+                    "rel_err": abs(fdv - ad) / abs(ad) if ad != 0.0 else float(fdv != 0.0)}
+        print(f"fd {name}: {fd[name]}", file=sys.stderr, flush=True)
+    ratio = out_modes["auto"]["ratio_to_forward"]
+    out = {
+        "metric": METRIC, "value": ratio, "unit": "adjoint gradient time / forward time",
+        "n_gpus": 1, "steps": steps, "warmup": 20, "ms_per_step": 1e3 * t_fwd / steps,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded .in, stress and state files; targets from a seeded "
+                "prestress perturbation)",
+        "config": {"workload": "adjoint gradient of a receiver misfit, 2 x (801 x 401) planar "
+                               "rate-and-state rupture (RFL), free surface, 32 receivers, "
+                               f"{steps} steps; fault trace kept in HBM (automatic policy; "
+                               "square-root checkpointing with recomputation in modes.sqrtN)",
+                   "grid_points": hp.nx * hp.ny, "fault_nodes": n, "receivers": 32},
+        "forward_s": t_fwd, "forward_updates_per_s": hp.nx * hp.ny * steps / t_fwd,
+        "modes": out_modes, "misfit": J0, "max_slip_velocity": vmax,
+        "fd_check": {"eps": eps, **fd},
+        "fd_max_rel_err": max(v["rel_err"] for v in fd.values()),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,10 +371,11 @@ def main():
                     help="steps timed after the headline ones with HIP events per launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--config", choices=("c2", "c3", "c4"), default="c2",
+    ap.add_argument("--config", choices=("c2", "c3", "c4", "c5"), default="c2",
                     help="c2: planar rupture (the headline); c3: rough curvilinear fault with "
                          "off-fault plasticity, 2 x (4001x2001), FP64 and FP32-bulk; c4: 64 "
-                         "uneven rupture scenarios advanced as one graph launch per step")
+                         "uneven rupture scenarios advanced as one graph launch per step; c5: "
+                         "adjoint gradient of a receiver misfit, 2 x (801x401), 2000 steps")
     ap.add_argument("--scenarios", type=int, default=64)
     ap.add_argument("--tune-cache", default="fdmap_tuning.json")
     args = ap.parse_args()
@@ -276,6 +392,9 @@ def main():
         return
     if args.config == "c4":
         run_c4(args, tmp)
+        return
+    if args.config == "c5":
+        run_c5(args, tmp)
         return
     inpath = inputs.write(os.path.join(tmp, "c2.in"),
                           inputs.c2_planar_rs(name=os.path.join(tmp, "c2"), nx=args.nx,

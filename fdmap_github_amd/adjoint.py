@@ -21,6 +21,10 @@ PARAMS = {"S0": capi.FDM_GRAD_S0, "a": capi.FDM_GRAD_A, "b": capi.FDM_GRAD_B}
 
 
 class Adjoint:
+    """interval: checkpoint interval in steps; 0 keeps the whole fault trace
+    when it fits the trace budget (else square-root checkpointing), -1
+    forces square-root checkpointing."""
+
     def __init__(self, ctx, rec_i, rec_j, nsteps, interval=0):
         self.lib = capi.load_library()
         self.ctx = ctx

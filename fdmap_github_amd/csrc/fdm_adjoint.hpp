@@ -78,7 +78,9 @@ __device__ __forceinline__ void adj_point(const AdjBulkArgs& a, const int i, con
   // q: rows of the block's q-derivative whose stencil reaches column il
   {
     const double yr0 = a.yr[cc], xr0 = a.xr[cc];
-    const int r0 = max(0, il - 5), r1 = min(nq - 1, il + 5);
+    // columns 6..nq-7 are reached by interior rows il-2..il+2 only
+    const bool cen = il >= 6 && il <= nq - 7;
+    const int r0 = cen ? il - 2 : max(0, il - 5), r1 = cen ? il + 2 : min(nq - 1, il + 5);
     for (int r = r0; r <= r1; r++) {
       const double D = sbp_entry(r, il, nq);
       if (D == 0.0) continue;
@@ -104,7 +106,8 @@ __device__ __forceinline__ void adj_point(const AdjBulkArgs& a, const int i, con
   // r: rows of the block's r-derivative whose stencil reaches row jl
   {
     const double yq0 = a.yq[cc], xq0 = a.xq[cc];
-    const int r0 = max(0, jl - 5), r1 = min(nr - 1, jl + 5);
+    const bool cen = jl >= 6 && jl <= nr - 7;
+    const int r0 = cen ? jl - 2 : max(0, jl - 5), r1 = cen ? jl + 2 : min(nr - 1, jl + 5);
     for (int r = r0; r <= r1; r++) {
       const double D = sbp_entry(r, jl, nr);
       if (D == 0.0) continue;
@@ -149,6 +152,153 @@ __global__ void __launch_bounds__(256) adj_bulk_kernel(AdjBulkArgs a) {
   const int j = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= a.nx || j >= a.ny) return;
   adj_point(a, i, j);
+}
+
+// points within six of their block's edge (closure rows reach them, or SAT)
+__global__ void __launch_bounds__(64) adj_band_kernel(AdjBulkArgs a, const int2* __restrict__ pts,
+                                                      int npts) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npts) return;
+  const int2 p = pts[k];
+  adj_point(a, p.x, p.y);
+}
+
+// Interior adjoint kernel over TX x TY tiles of each block's rectangle
+// [mx+6, px-6] x [my+6, py-6], where only interior SBP rows reach a point
+// and no SAT applies: R^T mu is the central stencil of the forward kernel
+// with the roles of rows and columns exchanged,
+//   sum_s DI(s) [a(P-s) w(P-s) - a(P+s) w(P+s)],  w_l = mu_l Ji c_l,
+// a the metric average of the pair.  w over the tile with a halo of two in
+// q and r, xr/yr with the q halo and xq/yq with the r halo are staged in
+// LDS.  Halo indices are clamped into the grid; a clamped value never
+// reaches an active lane.  tiles[k] = (block, i0, j0, 0).
+template <int TX, int TY>
+__global__ void __launch_bounds__(TX * TY) adj_interior_kernel(AdjBulkArgs a,
+                                                               const int4* __restrict__ tiles) {
+  constexpr int HX = TX + 4, HY = TY + 4, NT = TX * TY;
+  constexpr int NF = (HY * HX + NT - 1) / NT;
+  constexpr int NR = (TY * HX + NT - 1) / NT;
+  constexpr int NQ = (HY * TX + NT - 1) / NT;
+  __shared__ double sW[5][HY][HX];
+  __shared__ double sR[2][TY][HX];
+  __shared__ double sQ[2][HY][TX];
+  const int4 td = tiles[linear_tile(1)];
+  const int bu = td.x, i0 = td.y, j0 = td.z;
+  ConstBlockDev* B4 = (ConstBlockDev*)a.blocks + bu;
+  const double rho = B4->rho, Gm = B4->G, Mm = B4->M, gam = B4->gamma;
+  const double cl[5] = {1.0 / rho, 1.0 / rho, Mm, Gm, Mm};
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  const int ld = a.ld, nx1 = a.nx - 1, ny1 = a.ny - 1;
+  const long long ps = a.ps;
+  // every load of the tile is issued before the LDS stores and the barrier
+  double vW[NF][6], vR[NR][2], vQ[NQ][2];
+#pragma unroll
+  for (int p = 0; p < NF; p++) {
+    const int k = tid + p * NT;
+    if (k < HY * HX) {
+      const int r = k / HX, c = k - r * HX;
+      const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(max(i0 - 2 + c, 0), nx1);
+      const unsigned o = (unsigned)(gj * ld + gi) * 8u;
+      vW[p][5] = ld_off(a.Ji, o);
+#pragma unroll
+      for (int l = 0; l < 5; l++) vW[p][l] = ld_off(a.min + l * ps, o);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NR; p++) {
+    const int k = tid + p * NT;
+    if (k < TY * HX) {
+      const int r = k / HX, c = k - r * HX;
+      const int gj = min(j0 + r, ny1), gi = min(max(i0 - 2 + c, 0), nx1);
+      const unsigned o = (unsigned)(gj * ld + gi) * 8u;
+      vR[p][0] = ld_off(a.xr, o);
+      vR[p][1] = ld_off(a.yr, o);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NQ; p++) {
+    const int k = tid + p * NT;
+    if (k < HY * TX) {
+      const int r = k / TX, c = k - r * TX;
+      const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(i0 + c, nx1);
+      const unsigned o = (unsigned)(gj * ld + gi) * 8u;
+      vQ[p][0] = ld_off(a.xq, o);
+      vQ[p][1] = ld_off(a.yq, o);
+    }
+  }
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const bool valid = i <= B4->px - 6 && j <= B4->py - 6;
+  const unsigned oc = (unsigned)(min(j, ny1) * ld + min(i, nx1)) * 8u;
+  double lin[5], mown[5];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    lin[l] = ld_off(a.lin + l * ps, oc);
+    mown[l] = ld_off(a.min + l * ps, oc);
+  }
+#pragma unroll
+  for (int p = 0; p < NF; p++) {
+    const int k = tid + p * NT;
+    if (k < HY * HX) {
+      const int r = k / HX, c = k - r * HX;
+#pragma unroll
+      for (int l = 0; l < 5; l++) sW[l][r][c] = vW[p][l] * vW[p][5] * cl[l];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NR; p++) {
+    const int k = tid + p * NT;
+    if (k < TY * HX) {
+      const int r = k / HX, c = k - r * HX;
+      sR[0][r][c] = vR[p][0];
+      sR[1][r][c] = vR[p][1];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NQ; p++) {
+    const int k = tid + p * NT;
+    if (k < HY * TX) {
+      const int r = k / TX, c = k - r * TX;
+      sQ[0][r][c] = vQ[p][0];
+      sQ[1][r][c] = vQ[p][1];
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  const int x = threadIdx.x, y = threadIdx.y;
+  double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int s = 1; s <= 2; s++) {
+    const double D = (s == 1) ? FDM_DI1 : FDM_DI2;
+    // along q: rows P-s (entry +DI(s)) and P+s (entry -DI(s))
+    const double yr0 = sR[1][y][x + 2], xr0 = sR[0][y][x + 2];
+    const double aym = 0.5 * (sR[1][y][x + 2 - s] + yr0) * D, ayp = 0.5 * (sR[1][y][x + 2 + s] + yr0) * D;
+    const double axm = 0.5 * (sR[0][y][x + 2 - s] + xr0) * D, axp = 0.5 * (sR[0][y][x + 2 + s] + xr0) * D;
+    const double yq0 = sQ[1][y + 2][x], xq0 = sQ[0][y + 2][x];
+    const double bym = 0.5 * (sQ[1][y + 2 - s][x] + yq0) * D, byp = 0.5 * (sQ[1][y + 2 + s][x] + yq0) * D;
+    const double bxm = 0.5 * (sQ[0][y + 2 - s][x] + xq0) * D, bxp = 0.5 * (sQ[0][y + 2 + s][x] + xq0) * D;
+    double Qa[5], Qb[5], Ra[5], Rb[5];
+#pragma unroll
+    for (int l = 0; l < 5; l++) {
+      const double wqm = sW[l][y + 2][x + 2 - s], wqp = sW[l][y + 2][x + 2 + s];
+      const double wrm = sW[l][y + 2 - s][x + 2], wrp = sW[l][y + 2 + s][x + 2];
+      Qa[l] = aym * wqm - ayp * wqp;
+      Qb[l] = axm * wqm - axp * wqp;
+      Ra[l] = bym * wrm - byp * wrp;
+      Rb[l] = bxm * wrm - bxp * wrp;
+    }
+    // field couplings of rates_*_mode2 (rates.f90:411-444), transposed
+    g[0] += Qa[2] - Qb[3] + gam * Qa[4] - Ra[2] + Rb[3] - gam * Ra[4];
+    g[1] += -gam * Qb[2] + Qa[3] - Qb[4] + gam * Rb[2] - Ra[3] + Rb[4];
+    g[2] += Qa[0] - Ra[0];
+    g[3] += -Qb[0] + Qa[1] + Rb[0] - Ra[1];
+    g[4] += -Qb[1] + Rb[1];
+  }
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    const double lo = lin[l] + g[l];
+    st_off(a.lout + l * ps, oc, lo);
+    st_off(a.mout + l * ps, oc, a.A * mown[l] + a.Bdt_prev * lo);
+  }
 }
 
 // ---- pointwise transposes ----
@@ -468,28 +618,26 @@ __global__ void __launch_bounds__(64) adj_edge_kernel(AdjEdgeArgs a) {
   store_gE(a, som, lFm, lddm);
 }
 
-// vx, vy at the receivers after a step
-__global__ void __launch_bounds__(64) adj_record_kernel(const double* F, long long ps,
-                                                        const long long* cc, int nrec, double* out) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrec) return;
-  out[2 * r] = F[cc[r]];
-  out[2 * r + 1] = F[ps + cc[r]];
-}
-
-// lambda_F += g and mu += b g at the receivers (one lane: receivers may
-// share a grid point)
+// lambda_F += g and mu += b g at the receivers: one lane per distinct grid
+// point, adding the receivers of that point (first[u] .. first[u+1]-1 of
+// the receivers ordered by point) in order
 __global__ void __launch_bounds__(64) adj_inject_kernel(double* lF, double* mu, long long ps,
-                                                        const long long* cc, int nrec,
-                                                        const double* g, double b) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  for (int r = 0; r < nrec; r++) {
-    const long long k = cc[r];
-    lF[k] += g[2 * r];
-    lF[ps + k] += g[2 * r + 1];
-    mu[k] += b * g[2 * r];
-    mu[ps + k] += b * g[2 * r + 1];
+                                                        const long long* cc, const int* first,
+                                                        const int* order, int nu, const double* g,
+                                                        double b) {
+  const int u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nu) return;
+  const long long k = cc[order[first[u]]];
+  double gx = 0.0, gy = 0.0;
+  for (int q = first[u]; q < first[u + 1]; q++) {
+    const int r = order[q];
+    gx += g[2 * r];
+    gy += g[2 * r + 1];
   }
+  lF[k] += gx;
+  lF[ps + k] += gy;
+  mu[k] += b * gx;
+  mu[ps + k] += b * gy;
 }
 
 }  // namespace fdm
@@ -498,6 +646,9 @@ struct fdm_adjoint {
   fdm_ctx* c = nullptr;
   int nrec = 0, nsteps = 0, interval = 0, nseg = 0;
   long long* rec_cc = nullptr;   // [nrec] plane offsets of the receivers
+  int* rec_first = nullptr;      // [nuniq + 1] receivers grouped by grid point
+  int* rec_order = nullptr;      // [nrec]
+  int nuniq = 0;
   double* rec = nullptr;         // [nsteps][nrec][2]
   double* rec_bar = nullptr;     // [nsteps][nrec][2]
   double* ckpt = nullptr;        // [nseg][6 ps + 18 NI]
@@ -512,6 +663,14 @@ struct fdm_adjoint {
   double* lpsi = nullptr;        // [NI]
   double* ldpsi = nullptr;       // [NI]
   double* grad = nullptr;        // [FDM_GRAD_NFIELDS][NI]
+  // bulk adjoint: interior tiles (64 x 4) of [mx+6, px-6] x [my+6, py-6]
+  // on the main stream, the other points on the side stream after the edge
+  // adjoint; or one per-point kernel (FDM_ADJ_BULK=generic)
+  bool tiled = true;
+  int4* tiles = nullptr;
+  int ntiles = 0;
+  int2* band = nullptr;
+  int nband = 0;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   double ms_fwd = 0.0, ms_rec = 0.0, ms_sweep = 0.0;
   std::vector<void*> allocs;
@@ -561,29 +720,34 @@ int adj_checkpoint(fdm_adjoint* a, int seg, bool save) {
   return FDM_OK;
 }
 
-// forward steps n0..n1 (1-based); traces of the steps go to the trace
-// buffer when `record` is set
+// forward steps n0..n1 (1-based), one fdm_step call per checkpoint
+// interval: checkpoints before the first step of every interval when
+// save_ckpt, traces of the steps in the trace buffer (slot (n-1) mod
+// interval) when record, receivers after every step when receivers
 int adj_run_steps(fdm_adjoint* a, int n0, int n1, bool record, bool save_ckpt, bool receivers) {
   fdm_ctx* c = a->c;
   const int K = a->interval;
-  for (int n = n0; n <= n1; n++) {
-    if (save_ckpt && (n - 1) % K == 0) {
-      int rc = adj_checkpoint(a, (n - 1) / K, true);
-      if (rc) return rc;
-    }
-    const int slot = (n - 1) % K;
-    c->trace = record ? a->trace + (size_t)slot * c->nstage * 3 * c->NI : nullptr;
-    int rc = step_impl<double>(c, a->t, a->dt, n, 1);
-    c->trace = nullptr;
-    if (rc) return adj_err(a, rc, c->err);
+  int rc = FDM_OK;
+  for (int s0 = n0; s0 <= n1 && !rc; ) {
+    const int seg = (s0 - 1) / K, s1 = std::min(n1, (seg + 1) * K);
+    if (save_ckpt && (s0 - 1) % K == 0) rc = adj_checkpoint(a, seg, true);
+    if (rc) break;
+    c->trace = record ? a->trace : nullptr;
+    c->trace_n0 = (long long)seg * K + 1;
     if (receivers && a->nrec) {
-      hipLaunchKernelGGL(adj_record_kernel, dim3((a->nrec + 63) / 64), dim3(64), 0, c->stream,
-                         (const double*)c->F[c->cur], c->ps, a->rec_cc, a->nrec,
-                         a->rec + (size_t)(n - 1) * a->nrec * 2);
-      ADJCHK(a, hipGetLastError());
+      c->rec_cc = a->rec_cc;
+      c->nrec = a->nrec;
+      c->rec_out = a->rec;
+      c->rec_n0 = 1;
     }
+    rc = step_impl<double>(c, a->t, a->dt, s0, s1 - s0 + 1);
+    c->trace = nullptr;
+    c->rec_out = nullptr;
+    c->nrec = 0;
+    if (rc) return adj_err(a, rc, c->err);
+    s0 = s1 + 1;
   }
-  return FDM_OK;
+  return rc;
 }
 
 // backward stage s (1-based) of step n
@@ -592,8 +756,8 @@ int adj_stage(fdm_adjoint* a, int n, int s, int& cur) {
   const int K = a->interval, ns = c->nstage;
   const double A = c->rkA[s - 1], Bdt = c->rkB[s - 1] * a->dt;
   const double Bprev = (s > 1 ? c->rkB[s - 2] : c->rkB[ns - 1]) * a->dt;
+  AdjEdgeArgs e;
   if (c->ntasks) {
-    AdjEdgeArgs e;
     e.mu = a->mu[cur];
     e.ld = c->ld;
     e.ps = c->ps;
@@ -610,7 +774,6 @@ int adj_stage(fdm_adjoint* a, int n, int s, int& cur) {
     e.gE = a->gE;
     e.A = A;
     e.Bdt = Bdt;
-    hipLaunchKernelGGL(adj_edge_kernel, dim3((c->ntasks + 63) / 64), dim3(64), 0, c->stream, e);
   }
   AdjBulkArgs b;
   b.lin = a->lF[cur];
@@ -635,8 +798,29 @@ int adj_stage(fdm_adjoint* a, int n, int s, int& cur) {
   b.ps = c->ps;
   b.A = A;
   b.Bdt_prev = Bprev;
-  hipLaunchKernelGGL(adj_bulk_kernel, dim3((c->nx + 63) / 64, (c->ny + 3) / 4), dim3(64, 4), 0,
-                     c->stream, b);
+  if (a->tiled) {
+    // The interior points read mu of this stage within two points (the band
+    // of the later stage wrote it there) and overwrite the buffer the later
+    // stage's band read; the band reads mu within five points (the later
+    // interior launch) and needs this stage's edge adjoint.
+    ADJCHK(a, hipStreamWaitEvent(c->stream, c->evB, 0));
+    if (a->ntiles)
+      hipLaunchKernelGGL((adj_interior_kernel<64, 4>), dim3(a->ntiles), dim3(64, 4), 0, c->stream, b,
+                         a->tiles);
+    if (c->ntasks)
+      hipLaunchKernelGGL(adj_edge_kernel, dim3((c->ntasks + 63) / 64), dim3(64), 0, c->side, e);
+    ADJCHK(a, hipStreamWaitEvent(c->side, c->evI, 0));
+    if (a->nband)
+      hipLaunchKernelGGL(adj_band_kernel, dim3((a->nband + 63) / 64), dim3(64), 0, c->side, b, a->band,
+                         a->nband);
+    ADJCHK(a, hipEventRecord(c->evB, c->side));
+    ADJCHK(a, hipEventRecord(c->evI, c->stream));
+  } else {
+    if (c->ntasks)
+      hipLaunchKernelGGL(adj_edge_kernel, dim3((c->ntasks + 63) / 64), dim3(64), 0, c->stream, e);
+    hipLaunchKernelGGL(adj_bulk_kernel, dim3((c->nx + 63) / 64, (c->ny + 3) / 4), dim3(64, 4), 0,
+                       c->stream, b);
+  }
   ADJCHK(a, hipGetLastError());
   cur = 1 - cur;
   return FDM_OK;
@@ -689,13 +873,36 @@ int fdm_adjoint_create(fdm_ctx* c, int32_t nrec, const int32_t* rec_i, const int
   a->c = c;
   a->nrec = nrec;
   a->nsteps = nsteps;
-  a->interval = interval > 0 ? std::min(interval, nsteps) : (int)std::ceil(std::sqrt((double)nsteps));
+  // interval 0: keep the whole fault trace when it fits the trace budget
+  // (FDM_ADJ_TRACE_MB, default 2048 MiB), else square-root checkpointing
+  const int isqrt = (int)std::ceil(std::sqrt((double)nsteps));
+  if (interval > 0) {
+    a->interval = std::min(interval, nsteps);
+  } else if (interval < 0) {
+    a->interval = isqrt;
+  } else {
+    double budget = 2048.0;
+    if (const char* v = getenv("FDM_ADJ_TRACE_MB")) budget = atof(v);
+    const double mb = (double)nsteps * c->nstage * 3.0 * std::max(1, c->NI) * 8.0 / (1024.0 * 1024.0);
+    a->interval = mb <= budget ? nsteps : isqrt;
+  }
   a->nseg = (nsteps + a->interval - 1) / a->interval;
   std::vector<long long> cc(std::max(1, nrec));
   for (int r = 0; r < nrec; r++) cc[r] = (long long)rec_j[r] * c->ld + rec_i[r];
+  std::vector<int> order(std::max(1, nrec)), first;
+  for (int r = 0; r < nrec; r++) order[r] = r;
+  std::stable_sort(order.begin(), order.begin() + nrec, [&](int x, int y) { return cc[x] < cc[y]; });
+  for (int q = 0; q < nrec; q++)
+    if (q == 0 || cc[order[q]] != cc[order[q - 1]]) first.push_back(q);
+  a->nuniq = (int)first.size();
+  first.push_back(nrec);
   const size_t NI = std::max(1, c->NI), NS = std::max(1, c->NS);
   int rc = adj_alloc(a, &a->rec_cc, cc.size() * sizeof(long long));
-  if (!rc && hipMemcpy(a->rec_cc, cc.data(), cc.size() * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess)
+  if (!rc) rc = adj_alloc(a, &a->rec_first, first.size() * sizeof(int));
+  if (!rc) rc = adj_alloc(a, &a->rec_order, order.size() * sizeof(int));
+  if (!rc && (hipMemcpy(a->rec_cc, cc.data(), cc.size() * sizeof(long long), hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(a->rec_first, first.data(), first.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(a->rec_order, order.data(), order.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess))
     rc = adj_err(a, FDM_EDEVICE, "fdm_adjoint_create: upload failed");
   const size_t nrv = (size_t)nsteps * std::max(1, nrec) * 2;
   if (!rc) rc = adj_alloc(a, &a->rec, nrv * 8);
@@ -713,6 +920,37 @@ int fdm_adjoint_create(fdm_ctx* c, int32_t nrec, const int32_t* rec_i, const int
     g_create_error = a->err;
     fdm_adjoint_destroy(a);
     return rc;
+  }
+  // bulk adjoint work lists; per-plane byte offsets of the tile kernel are
+  // 32-bit
+  if (const char* v = getenv("FDM_ADJ_BULK")) a->tiled = strcmp(v, "generic") != 0;
+  if ((size_t)c->ps * 8 >= ((size_t)1 << 31)) a->tiled = false;
+  if (a->tiled) {
+    std::vector<int4> tl;
+    std::vector<int2> band;
+    for (int b = 0; b < c->nb; b++) {
+      const BlockDev& B = c->hblocks[b];
+      for (int j0 = B.my + 6; j0 <= B.py - 6; j0 += 4)
+        for (int i0 = B.mx + 6; i0 <= B.px - 6; i0 += 64) tl.push_back(make_int4(b, i0, j0, 0));
+      for (int j = B.my; j <= B.py; j++)
+        for (int i = B.mx; i <= B.px; i++)
+          if (i - B.mx < 6 || B.px - i < 6 || j - B.my < 6 || B.py - j < 6) band.push_back(make_int2(i, j));
+    }
+    a->ntiles = (int)tl.size();
+    a->nband = (int)band.size();
+    rc = adj_alloc(a, &a->tiles, tl.size() * sizeof(int4));
+    if (!rc) rc = adj_alloc(a, &a->band, band.size() * sizeof(int2));
+    if (!rc && a->ntiles &&
+        hipMemcpy(a->tiles, tl.data(), tl.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess)
+      rc = adj_err(a, FDM_EDEVICE, "fdm_adjoint_create: upload failed");
+    if (!rc && a->nband &&
+        hipMemcpy(a->band, band.data(), band.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess)
+      rc = adj_err(a, FDM_EDEVICE, "fdm_adjoint_create: upload failed");
+    if (rc) {
+      g_create_error = a->err;
+      fdm_adjoint_destroy(a);
+      return rc;
+    }
   }
   double* pb = a->planes + c->coff;
   a->lF[0] = pb;
@@ -795,16 +1033,33 @@ int fdm_adjoint_backward(fdm_adjoint* a, const double* rec_bar, double* grad, do
       if (rc) return rc;
     }
     ADJCHK(a, hipEventRecord(a->ev[1], c->stream));
+    if (a->tiled) {
+      const int rf = fork_side(c);
+      if (rf) return adj_err(a, rf, c->err);
+    }
     for (int n = n1; n >= n0; n--) {
       if (a->nrec) {
-        hipLaunchKernelGGL(adj_inject_kernel, dim3(1), dim3(64), 0, c->stream, a->lF[cur], a->mu[cur],
-                           c->ps, a->rec_cc, a->nrec, a->rec_bar + (size_t)(n - 1) * a->nrec * 2, b_last);
+        // after both streams' last writes of the later step, before either
+        // reads this step's last stage
+        hipStream_t sr = c->stream;
+        if (a->tiled) {
+          ADJCHK(a, hipStreamWaitEvent(c->side, c->evI, 0));
+          sr = c->side;
+        }
+        hipLaunchKernelGGL(adj_inject_kernel, dim3((a->nuniq + 63) / 64), dim3(64), 0, sr, a->lF[cur],
+                           a->mu[cur], c->ps, a->rec_cc, a->rec_first, a->rec_order, a->nuniq,
+                           a->rec_bar + (size_t)(n - 1) * a->nrec * 2, b_last);
         ADJCHK(a, hipGetLastError());
+        if (a->tiled) ADJCHK(a, hipEventRecord(c->evB, c->side));
       }
       for (int s = ns; s >= 1; s--) {
         int rc = adj_stage(a, n, s, cur);
         if (rc) return rc;
       }
+    }
+    if (a->tiled) {
+      const int rj = join_side(c);
+      if (rj) return adj_err(a, rj, c->err);
     }
     ADJCHK(a, hipEventRecord(a->ev[2], c->stream));
     ADJCHK(a, hipEventSynchronize(a->ev[2]));

@@ -243,10 +243,19 @@ struct fdm_ctx {
   bool graph_mode = false;
   double g_tinit = 0.0, g_dt = 0.0, g_rkc = 0.0;
   long long ninterior = 0;
-  // adjoint forward passes: fault trajectories of the current step
-  // ([nstage][3][NI]), and the slot of the stage being launched
+  // adjoint forward passes: fault trajectories ([nstage][3][NI] per step,
+  // step n at trace + (n - trace_n0) nstage 3 NI), the slot of the stage
+  // being launched, and vx, vy at receivers after every step (step n at
+  // rec_out + (n - rec_n0) nrec 2)
   double* trace = nullptr;
+  long long trace_n0 = 0;
   double* trace_cur = nullptr;
+  const long long* rec_cc = nullptr;
+  int nrec = 0;
+  double* rec_out = nullptr;
+  long long rec_n0 = 0;
+  long long cur_n = 0;       // step being launched
+  long long first_n = 0;     // first step of the current fdm_step call
 };
 
 // block index of global point (i, j)
@@ -501,6 +510,23 @@ void launch_pointwise(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   prof_end(c, ep, 0, s);
 }
 
+// vx, vy of the fields F at the receivers
+__global__ void __launch_bounds__(64) record_receivers_kernel(const double* F, long long ps,
+                                                              const long long* cc, int nrec,
+                                                              double* out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= nrec) return;
+  out[2 * r] = F[cc[r]];
+  out[2 * r + 1] = F[ps + cc[r]];
+}
+
+// receivers of the fields in F[c->cur] after step n, on stream s
+void record_receivers(fdm_ctx* c, hipStream_t s, long long n) {
+  hipLaunchKernelGGL(record_receivers_kernel, dim3((c->nrec + 63) / 64), dim3(64), 0, s,
+                     (const double*)c->F[c->cur], c->ps, c->rec_cc, c->nrec,
+                     c->rec_out + (size_t)(n - c->rec_n0) * c->nrec * 2);
+}
+
 // Fork the side stream off the main one before a sequence of stages, and
 // join it back afterwards, so that work queued on the main stream later (and
 // every synchronisation of it) is ordered after both.
@@ -537,20 +563,28 @@ int run_stage(fdm_ctx* c, int stage, double t0, double dt, int final_step, int s
     // band needs this stage's edge values and the previous interior launch.
     HIPCHK(c, hipStreamWaitEvent(c->stream, c->evB, 0));
     launch_interior<T>(c, c->stream, a);
-    c->trace_cur = (c->trace && !final_step) ? c->trace + (size_t)(stage - 1) * 3 * c->NI : nullptr;
+    c->trace_cur = (c->trace && !final_step)
+                       ? c->trace + ((size_t)(c->cur_n - c->trace_n0) * c->nstage + (stage - 1)) * 3 * c->NI
+                       : nullptr;
     rc = launch_edges<T>(c, c->side, t, Bdt, A, 1, final_step ? 0 : 1, snapshot, Bload);
     c->trace_cur = nullptr;
     if (rc) return rc;
     HIPCHK(c, hipStreamWaitEvent(c->side, c->evI, 0));
     launch_band<T>(c, c->side, a);
+    // the previous step's fields are this stage's input until stage 2
+    // writes them; both streams have finished them here
+    if (c->rec_out && stage == 1 && !final_step && c->cur_n > c->first_n) record_receivers(c, c->side, c->cur_n - 1);
     HIPCHK(c, hipEventRecord(c->evB, c->side));
     HIPCHK(c, hipEventRecord(c->evI, c->stream));
   } else {
-    c->trace_cur = (c->trace && !final_step) ? c->trace + (size_t)(stage - 1) * 3 * c->NI : nullptr;
+    c->trace_cur = (c->trace && !final_step)
+                       ? c->trace + ((size_t)(c->cur_n - c->trace_n0) * c->nstage + (stage - 1)) * 3 * c->NI
+                       : nullptr;
     rc = launch_edges<T>(c, c->stream, t, Bdt, A, 1, final_step ? 0 : 1, snapshot, Bload);
     c->trace_cur = nullptr;
     if (rc) return rc;
     launch_pointwise<T>(c, c->stream, a);
+    if (c->rec_out && stage == 1 && !final_step && c->cur_n > c->first_n) record_receivers(c, c->stream, c->cur_n - 1);
   }
   HIPCHK(c, hipGetLastError());
   if (!final_step) c->cur = 1 - c->cur;
@@ -560,12 +594,18 @@ int run_stage(fdm_ctx* c, int stage, double t0, double dt, int final_step, int s
 template <typename T>
 int step_impl(fdm_ctx* c, double t, double dt, long long n_first, int nsteps) {
   int rc = fork_side(c);
+  c->first_n = n_first;
   for (int k = 0; k < nsteps && !rc; k++) {
     long long n = n_first + k;
+    c->cur_n = n;
     double t0 = t + (double)(n - 1) * dt;  // main.f90:176
     for (int stage = 1; stage <= c->nstage && !rc; stage++) rc = run_stage<T>(c, stage, t0, dt, 0, stage == 1);
   }
   int rj = join_side(c);
+  if (!rc && !rj && c->rec_out && nsteps > 0) {
+    record_receivers(c, c->stream, n_first + nsteps - 1);
+    HIPCHK(c, hipGetLastError());
+  }
   return rc ? rc : rj;
 }
 

@@ -52,7 +52,8 @@ def c4_scenarios(count=64, seed=0, nx_range=(401, 2001), ny_range=(201, 1001), h
     return out
 
 
-def write_scenario(directory, sc, nsteps, sn0=120.0, Vinit=1e-12, b=0.014, V0=1e-6):
+def write_scenario(directory, sc, nsteps, sn0=120.0, Vinit=1e-12, b=0.014, V0=1e-6,
+                   bcT="absorbing"):
     """Write scenario sc as an FDMAP .in file plus its fault stress and state
     files (friction.f90:422-443, 469-489 formats); returns the .in path.  The
     medium carries the normal prestress; the fault's shear prestress is the
@@ -70,7 +71,7 @@ def write_scenario(directory, sc, nsteps, sn0=120.0, Vinit=1e-12, b=0.014, V0=1e
         psi.tofile(f)
     text = inputs.c2_planar_rs(name=os.path.join(directory, f"s{k}"), nx=nx, ny=ny, h=sc["h"],
                                nsteps=nsteps, outputs=False, tau0=70.0, sxy0=0.0, sn0=sn0, a=a,
-                               b=b, Vinit=Vinit)
+                               b=b, Vinit=Vinit, bcT=bcT)
     text = text.replace(
         "&friction_list\n",
         f"&friction_list\n stress_file = T, stress_filename = '{stress}',\n"

@@ -68,13 +68,14 @@ vx vy
 
 def c2_planar_rs(name="c2", nx=1601, ny=801, h=0.025, nsteps=4000, outputs=True, ninfo=500,
                  law="RFL", plastic=False, curve=None, tau0=70.0, sn0=120.0, overstress=25.0,
-                 R=1.5, Vinit=1e-12, a=0.010, b=0.014, sxy0=None):
+                 R=1.5, Vinit=1e-12, a=0.010, b=0.014, sxy0=None, bcT="absorbing"):
     """Two blocks nx x ny coupled across a horizontal fault by SAT; slip law
     with strong velocity weakening; Gaussian overstress at the fault centre.
     curve: optional path of a fault-profile curve file (FP64 x, y, n1, n2),
     used for the top edge of block 1 and the bottom edge of block 2.  The
     background shear stress is sxy0 (default tau0); Psi0 gives slip velocity
-    Vinit under tau0."""
+    Vinit under tau0.  bcT: boundary condition of the top of the upper block
+    ('free' for a free surface)."""
     Lx = 0.5 * (nx - 1) * h
     Ly = (ny - 1) * h
     psi0 = rs_state(a, 1e-6, tau0, sn0, Vinit)
@@ -113,7 +114,7 @@ D V S N Psi
  LB = {_f(-Lx)},0d0, LT = {_f(-Lx)},{_f(Ly)},
  RB = {_f(Lx)},0d0, RT = {_f(Lx)},{_f(Ly)} /
 
-&boundaries_list bcL = 'absorbing', bcR = 'absorbing', bcB = 'none', bcT = 'absorbing' /
+&boundaries_list bcL = 'absorbing', bcR = 'absorbing', bcB = 'none', bcT = '{bcT}' /
 
 !---ALL BLOCKS---
 

@@ -16,12 +16,16 @@
  *
  * Trajectory storage: the backward pass needs, per stage, the slip velocity,
  * state variable and normal stress of every fault node (the bulk operator is
- * affine, so no bulk state is kept).  The forward pass stores checkpoints of
- * the full state every `interval` steps; the backward pass recomputes each
- * interval's steps from its checkpoint to regenerate these traces, then
- * sweeps the interval backwards.  interval = 0 picks ceil(sqrt(nsteps))
- * (square-root checkpointing); interval >= nsteps keeps the whole trace from
- * the forward pass and recomputes nothing.
+ * affine, so no bulk state is kept): 3 x 8 bytes per fault node and stage.
+ * The forward pass stores checkpoints of the full state every `interval`
+ * steps and keeps the traces of the last interval; the backward pass
+ * recomputes every other interval's steps from its checkpoint to regenerate
+ * its traces, then sweeps the interval backwards.
+ *   interval > 0   : that interval (clamped to nsteps; nsteps keeps the whole
+ *                    trace and recomputes nothing)
+ *   interval < 0   : ceil(sqrt(nsteps)) (square-root checkpointing)
+ *   interval = 0   : the whole trace when it needs at most FDM_ADJ_TRACE_MB
+ *                    MiB (environment, default 2048), else ceil(sqrt(nsteps))
  *
  * Supported problems: FP64 contexts without plasticity, every block at least
  * 12 points wide in both directions, interfaces locked or frictional with a
@@ -75,7 +79,8 @@ int fdm_adjoint_forward(fdm_adjoint* adj, double t, double dt, double* rec);
  * [FDM_GRAD_NFIELDS][nodes] accumulated gradients; gF (may be NULL):
  * [6][ny][nx] gradient with respect to the fields at the start of the
  * forward pass (the szz plane is zero: nothing depends on szz but szz).
- * Leaves the context at the state after the first checkpoint interval. */
+ * Leaves the context at the end of the first checkpoint interval when it
+ * recomputes, at the end of the forward pass otherwise. */
 int fdm_adjoint_backward(fdm_adjoint* adj, const double* rec_bar, double* grad, double* gF);
 
 /* Device time of the most recent passes in ms: forward, backward

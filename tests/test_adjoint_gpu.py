@@ -13,7 +13,8 @@ differences of the forward path itself.
   steps 1e-7..1e-4 within 2e-7 (the forward path's rounding noise, which
   the unstable rupture amplifies, dominates the difference quotient at
   small steps).
-- Checkpoint intervals 1, sqrt(N) and N give identical gradients; the
+- Checkpoint intervals 1, 7 (a partial last interval), sqrt(N), N and the
+  automatic choice give identical gradients; the
   torch.autograd.Function gives the gradient of the direct call.
 """
 import numpy as np
@@ -93,9 +94,9 @@ def check_direction(label, J, ad, rng_eps=(1e-7, 1e-6, 1e-5, 1e-4)):
 @pytest.mark.parametrize("law,seed", [("RFL", 11), ("RSL", 12), ("RAL", 13)])
 def test_friction_gradients_match_central_differences(gpu, law, seed):
     """Directions with N(0, 1) entries times a scale per parameter: 10 MPa
-    for the shear prestress (70 MPa on the fault), 1 for b and the initial
-    state, a itself for a (the misfit is strongly nonlinear in a: an
-    absolute step of 1e-6 is already in the eps^2 truncation regime
+    for the shear prestress (70 MPa on the fault), 10 for b, 1 for the
+    initial state, a itself for a (the misfit is strongly nonlinear in a:
+    an absolute step of 1e-6 is already in the eps^2 truncation regime
     there)."""
     prob, ctx, inv, p0, target, rng = friction_case(law, seed)
     rec = inv.run(**p0)
@@ -105,7 +106,7 @@ def test_friction_gradients_match_central_differences(gpu, law, seed):
     assert vmax > 1e-3  # the rupture is under way
     n = len(p0["S0"])
     for name in ("S0", "a", "b"):
-        d = rng.standard_normal(n) * {"S0": 10.0, "a": p0["a"], "b": 1.0}[name]
+        d = rng.standard_normal(n) * {"S0": 10.0, "a": p0["a"], "b": 10.0}[name]
 
         def J(e, name=name, d=d):
             p = dict(p0)
@@ -133,10 +134,11 @@ def test_checkpoint_intervals_and_autograd_agree(gpu):
     prob, ctx, inv, p0, target, rng = friction_case("RFL", 21)
     grads = []
     ri, rj = top_receivers(prob, 8)
-    for interval in (1, 0, 150):
+    for interval, expect in ((1, 1), (-1, 13), (150, 150), (0, 150), (7, 7)):
         other = FaultInversion(ctx, 0.0, DT, ri, rj, 150, interval)
         other.F0, other.DF0, other.iface0 = inv.F0, inv.DF0, inv.iface0
         print("interval", interval, "->", other.adj.interval, flush=True)
+        assert other.adj.interval == expect
         rec = other.run(**p0)
         grads.append(other.adj.backward(DT * (rec - target)))
         t = other.adj.timing()
@@ -168,3 +170,24 @@ def test_unsupported_problems_are_refused(gpu):
     prob, F = problems.build(nx=40, nys=(20, 20), seed=1)
     with pytest.raises(FdmError, match="receiver"):
         FaultInversion(Context(prob), 0.0, DT, [3], [40], 10)
+
+
+def test_tiled_and_pointwise_bulk_adjoints_agree(gpu, monkeypatch):
+    """The interior tile kernel (central stencil, LDS) and the per-point
+    kernel (SBP matrix entries) sum the same terms in different orders."""
+    out = []
+    for mode in ("tile", "generic"):
+        monkeypatch.setenv("FDM_ADJ_BULK", mode)
+        prob, F, ctx, inv = setup(nx=90, nys=(40, 46), nbx=1, seed=7, warp=0.04, background=BG,
+                                  perturb=1e-3, law="RFL", nsteps=60)
+        rng = np.random.default_rng(8)
+        target = inv.run() + 1e-4 * rng.standard_normal((60, 8, 2))
+        rec = inv.run()
+        out.append(inv.adj.backward(DT * (rec - target), fields=True))
+        inv.adj.close()
+        ctx.close()
+    for k in ("S0", "a", "b", "Psi0", "F0"):
+        a, b = out[0][k], out[1][k]
+        err = np.abs(a - b).max() / np.abs(b).max()
+        print(k, "max rel diff", err, flush=True)
+        assert err <= 1e-12, k
