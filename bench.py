@@ -434,8 +434,9 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "bulk_interior<double,8,false> (SBP4 interior rates + RK update, 64x8 LDS "
-                      "tiles over the block interiors)",
+            "kernel": "bulk_interior<double,64,4,false,true> (SBP4 interior rates + RK update, "
+                      "64x4 LDS tiles over the block interiors, non-temporal stores since the "
+                      "23 field, rate and metric planes exceed 256 MB)",
             "kernel_points": kpts,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,

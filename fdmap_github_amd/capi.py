@@ -168,6 +168,7 @@ ENTRY_POINTS = [
 # entry points declared in include/fdmap_ensemble.h (the MI355X library only)
 FDM_TUNE_TILE = 0
 FDM_TUNE_SWIZZLE = 1
+FDM_TUNE_NT_STORE = 2
 TILE_SHAPES = ["64x4", "64x8", "128x4"]
 ENSEMBLE_ENTRY_POINTS = [
     ("fdm_tune", C.c_int, [_P, C.c_int32, C.c_int32]),

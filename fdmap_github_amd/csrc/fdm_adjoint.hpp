@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(64) adj_band_kernel(AdjBulkArgs a, const int2*
 // q and r, xr/yr with the q halo and xq/yq with the r halo are staged in
 // LDS.  Halo indices are clamped into the grid; a clamped value never
 // reaches an active lane.  tiles[k] = (block, i0, j0, 0).
-template <int TX, int TY>
+template <int TX, int TY, bool NTS>
 __global__ void __launch_bounds__(TX * TY) adj_interior_kernel(AdjBulkArgs a,
                                                                const int4* __restrict__ tiles) {
   constexpr int HX = TX + 4, HY = TY + 4, NT = TX * TY;
@@ -296,8 +296,8 @@ __global__ void __launch_bounds__(TX * TY) adj_interior_kernel(AdjBulkArgs a,
 #pragma unroll
   for (int l = 0; l < 5; l++) {
     const double lo = lin[l] + g[l];
-    st_off(a.lout + l * ps, oc, lo);
-    st_off(a.mout + l * ps, oc, a.A * mown[l] + a.Bdt_prev * lo);
+    st_sel<NTS>(a.lout + l * ps, oc, lo);
+    st_sel<NTS>(a.mout + l * ps, oc, a.A * mown[l] + a.Bdt_prev * lo);
   }
 }
 
@@ -667,6 +667,7 @@ struct fdm_adjoint {
   // on the main stream, the other points on the side stream after the edge
   // adjoint; or one per-point kernel (FDM_ADJ_BULK=generic)
   bool tiled = true;
+  bool nt_store = false;  // adjoint planes and metrics beyond the last-level cache
   int4* tiles = nullptr;
   int ntiles = 0;
   int2* band = nullptr;
@@ -804,9 +805,14 @@ int adj_stage(fdm_adjoint* a, int n, int s, int& cur) {
     // stage's band read; the band reads mu within five points (the later
     // interior launch) and needs this stage's edge adjoint.
     ADJCHK(a, hipStreamWaitEvent(c->stream, c->evB, 0));
-    if (a->ntiles)
-      hipLaunchKernelGGL((adj_interior_kernel<64, 4>), dim3(a->ntiles), dim3(64, 4), 0, c->stream, b,
-                         a->tiles);
+    if (a->ntiles) {
+      if (a->nt_store)
+        hipLaunchKernelGGL((adj_interior_kernel<64, 4, true>), dim3(a->ntiles), dim3(64, 4), 0, c->stream,
+                           b, a->tiles);
+      else
+        hipLaunchKernelGGL((adj_interior_kernel<64, 4, false>), dim3(a->ntiles), dim3(64, 4), 0, c->stream,
+                           b, a->tiles);
+    }
     if (c->ntasks)
       hipLaunchKernelGGL(adj_edge_kernel, dim3((c->ntasks + 63) / 64), dim3(64), 0, c->side, e);
     ADJCHK(a, hipStreamWaitEvent(c->side, c->evI, 0));
@@ -924,6 +930,7 @@ int fdm_adjoint_create(fdm_ctx* c, int32_t nrec, const int32_t* rec_i, const int
   // bulk adjoint work lists; per-plane byte offsets of the tile kernel are
   // 32-bit
   if (const char* v = getenv("FDM_ADJ_BULK")) a->tiled = strcmp(v, "generic") != 0;
+  a->nt_store = nt_store_choice(25.0 * (double)c->ps * 8.0);
   if ((size_t)c->ps * 8 >= ((size_t)1 << 31)) a->tiled = false;
   if (a->tiled) {
     std::vector<int4> tl;

@@ -224,6 +224,9 @@ struct fdm_ctx {
   // launch shape of the bulk kernel (tile 64 x tile_y points)
   int tile_y = 4;
   int xcd_swizzle = 1;
+  // lean interior launches store F and DF non-temporally (working set beyond
+  // the last-level cache)
+  bool nt_store = false;
   // bulk kernel: 0 per-point, 2 interior tiles (kTileShape[tile_shape]) + band points
   int bulk_mode = 2;
   int tile_shape = 0;  // index into kTileShape
@@ -257,6 +260,16 @@ struct fdm_ctx {
   long long cur_n = 0;       // step being launched
   long long first_n = 0;     // first step of the current fdm_step call
 };
+
+// Non-temporal stores pay when a launch's working set (bytes of the planes
+// it streams) exceeds MI355X's 256 MB Infinity Cache, and cost when it fits:
+// FDM_NT_STORE = 0/1 forces the choice, FDM_NT_MB moves the threshold.
+static inline bool nt_store_choice(double working_set_bytes) {
+  if (const char* v = getenv("FDM_NT_STORE")) return atoi(v) != 0;
+  double mb = 256.0;
+  if (const char* v = getenv("FDM_NT_MB")) mb = atof(v);
+  return working_set_bytes > mb * 1024.0 * 1024.0;
+}
 
 // block index of global point (i, j)
 static inline int block_of(const fdm_ctx* c, int i, int j) {
@@ -487,16 +500,17 @@ void launch_interior(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   c->df6_stale = !full;
   dim3 grid(c->ntiles[q]);
   dim3 block(kTileShape[q][0], kTileShape[q][1]);
-  if (q == 0) {
-    if (full) hipLaunchKernelGGL((bulk_interior<T, 64, 4, true>), grid, block, 0, s, a, c->tiles[q]);
-    else hipLaunchKernelGGL((bulk_interior<T, 64, 4, false>), grid, block, 0, s, a, c->tiles[q]);
-  } else if (q == 1) {
-    if (full) hipLaunchKernelGGL((bulk_interior<T, 64, 8, true>), grid, block, 0, s, a, c->tiles[q]);
-    else hipLaunchKernelGGL((bulk_interior<T, 64, 8, false>), grid, block, 0, s, a, c->tiles[q]);
-  } else {
-    if (full) hipLaunchKernelGGL((bulk_interior<T, 128, 4, true>), grid, block, 0, s, a, c->tiles[q]);
-    else hipLaunchKernelGGL((bulk_interior<T, 128, 4, false>), grid, block, 0, s, a, c->tiles[q]);
-  }
+#define FDM_INTERIOR(TX_, TY_)                                                                       \
+  do {                                                                                               \
+    if (full) hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, true, false>), grid, block, 0, s, a, c->tiles[q]); \
+    else if (c->nt_store)                                                                            \
+      hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, false, true>), grid, block, 0, s, a, c->tiles[q]); \
+    else hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, false, false>), grid, block, 0, s, a, c->tiles[q]); \
+  } while (0)
+  if (q == 0) FDM_INTERIOR(64, 4);
+  else if (q == 1) FDM_INTERIOR(64, 8);
+  else FDM_INTERIOR(128, 4);
+#undef FDM_INTERIOR
   prof_end(c, ep, 0, s);
 }
 
@@ -1125,6 +1139,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   if ((rc = dalloc(c, &arena, 23 * pbytes + slack))) return fail(rc);
   HIPCHK(c, hipMemset(arena, 0, 23 * pbytes + slack));
   char* ab = (char*)arena + (size_t)c->coff * c->esz;
+  c->nt_store = nt_store_choice(23.0 * (double)pbytes);
   c->F[0] = ab;
   c->F[1] = ab + 6 * pbytes;
   c->DF = ab + 12 * pbytes;
@@ -1365,6 +1380,10 @@ int fdm_tune(fdm_ctx* c, int32_t knob, int32_t value) {
     c->xcd_swizzle = value;
     return FDM_OK;
   }
+  if (knob == FDM_TUNE_NT_STORE && (value == 0 || value == 1)) {
+    c->nt_store = value != 0;
+    return FDM_OK;
+  }
   return set_err(c, FDM_EINVAL, "fdm_tune: unknown knob or value");
 }
 
@@ -1380,7 +1399,7 @@ int fdm_tune(fdm_ctx* c, int32_t knob, int32_t value) {
 struct fdm_ensemble {
   std::vector<fdm_ctx*> m;
   int precision = FDM_FP64, nstage = 0, flip = 0, swz = 1;
-  bool any_plastic = false, use_graph = true;
+  bool any_plastic = false, use_graph = true, nt_store = false;
   hipStream_t master = nullptr, side = nullptr;
   hipEvent_t evJ = nullptr, evI = nullptr, evB = nullptr, done = nullptr;
   std::vector<hipEvent_t> pre;  // each member's earlier work, before a step
@@ -1465,17 +1484,17 @@ int ens_tables(fdm_ensemble* e) {
   return FDM_OK;
 }
 
-template <typename T, bool FULL>
+template <typename T, bool FULL, bool NT>
 void ens_interior(fdm_ensemble* e, const BulkArgs<T>* tab) {
   for (int q = 0; q < 3; q++) {
     if (!e->ntiles[q]) continue;
     const dim3 grid(e->ntiles[q]), block(kTileShape[q][0], kTileShape[q][1]);
     if (q == 0)
-      hipLaunchKernelGGL((bulk_interior_batch<T, 64, 4, FULL>), grid, block, 0, e->master, tab, e->tiles[q], e->swz);
+      hipLaunchKernelGGL((bulk_interior_batch<T, 64, 4, FULL, NT>), grid, block, 0, e->master, tab, e->tiles[q], e->swz);
     else if (q == 1)
-      hipLaunchKernelGGL((bulk_interior_batch<T, 64, 8, FULL>), grid, block, 0, e->master, tab, e->tiles[q], e->swz);
+      hipLaunchKernelGGL((bulk_interior_batch<T, 64, 8, FULL, NT>), grid, block, 0, e->master, tab, e->tiles[q], e->swz);
     else
-      hipLaunchKernelGGL((bulk_interior_batch<T, 128, 4, FULL>), grid, block, 0, e->master, tab, e->tiles[q], e->swz);
+      hipLaunchKernelGGL((bulk_interior_batch<T, 128, 4, FULL, NT>), grid, block, 0, e->master, tab, e->tiles[q], e->swz);
   }
 }
 
@@ -1495,8 +1514,9 @@ int ens_step_launches(fdm_ensemble* e, int p) {
     const EdgeArgs<T>* et = (const EdgeArgs<T>*)e->etab + o;
     const bool full = e->any_plastic && s == ns;
     ENSCHK(e, hipStreamWaitEvent(e->master, e->evB, 0));
-    if (full) ens_interior<T, true>(e, bt);
-    else ens_interior<T, false>(e, bt);
+    if (full) ens_interior<T, true, false>(e, bt);
+    else if (e->nt_store) ens_interior<T, false, true>(e, bt);
+    else ens_interior<T, false, false>(e, bt);
     if (e->ntasks)
       hipLaunchKernelGGL(edge_kernel_batch<T>, dim3(e->ntasks / 64), dim3(64), 0, e->side, et, e->tasks,
                          e->ntasks);
@@ -1615,7 +1635,12 @@ int fdm_ensemble_create(fdm_ctx* const* members, int32_t n, fdm_ensemble** out) 
   e->nstage = members[0]->nstage;
   e->flip = e->nstage & 1;
   e->swz = members[0]->xcd_swizzle;
-  for (fdm_ctx* c : e->m) e->any_plastic = e->any_plastic || c->any_plastic;
+  double ws = 0.0;
+  for (fdm_ctx* c : e->m) {
+    e->any_plastic = e->any_plastic || c->any_plastic;
+    ws += 23.0 * (double)c->ps * (double)c->esz;
+  }
+  e->nt_store = nt_store_choice(ws);
   if (const char* v = getenv("FDM_ENSEMBLE_GRAPH")) e->use_graph = atoi(v) != 0;
   e->pre.resize(n, nullptr);
   bool ok = hipStreamCreateWithFlags(&e->master, hipStreamNonBlocking) == hipSuccess &&

@@ -552,6 +552,15 @@ __device__ __forceinline__ void st_off(T* base, unsigned boff, T v) {
   *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff) = v;
 }
 
+// NT: a non-temporal store, for planes that are streamed once per launch
+// when the working set exceeds the last-level cache (no allocation there)
+template <bool NT, typename T>
+__device__ __forceinline__ void st_sel(T* base, unsigned boff, T v) {
+  T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff);
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // neighbourhood of the interior kernel: everything from LDS.  sF: F1..F5
 // with a halo of 2 in q and r; sR: xr, yr with a halo in q; sQ: xq, yq with
 // a halo in r.  (x, y): the lane's point in tile coordinates.
@@ -580,7 +589,7 @@ typedef const __attribute__((address_space(4))) BlockDev ConstBlockDev;
 // xq/yq with the r halo go to LDS; the lane's own DF, F6 and Ji stay in
 // registers.  Halo indices are clamped into the grid; a clamped value never
 // reaches an active lane.  tiles[k] = (block, i0, j0, 0).
-template <typename T, int TX, int TY, bool FULL>
+template <typename T, int TX, int TY, bool FULL, bool NTS>
 __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 td) {
   constexpr int HX = TX + 4, HY = TY + 4, NT = TX * TY;
   constexpr int NF = (HY * HX + NT - 1) / NT;  // staging passes, F1..F5
@@ -696,26 +705,26 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     if (a.do_update) {
       const T Bdt = (T)a.Bdt;
 #pragma unroll
-      for (int l = 0; l < 6; l++) st_off(a.Fout + l * ps, oc, Fc[l] + Bdt * d[l]);
+      for (int l = 0; l < 6; l++) st_sel<NTS>(a.Fout + l * ps, oc, Fc[l] + Bdt * d[l]);
     }
 #pragma unroll
-    for (int l = 0; l < 5; l++) st_off(a.DF + l * ps, oc, d[l]);
+    for (int l = 0; l < 5; l++) st_sel<NTS>(a.DF + l * ps, oc, d[l]);
   }
 }
 
-template <typename T, int TX, int TY, bool FULL>
+template <typename T, int TX, int TY, bool FULL, bool NTS>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4)))
 bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
-  interior_tile<T, TX, TY, FULL>(a, tiles[linear_tile(a.xcd_swizzle)]);
+  interior_tile<T, TX, TY, FULL, NTS>(a, tiles[linear_tile(a.xcd_swizzle)]);
 }
 
 // Batched interior kernel: the tiles of many problems in one launch; tile
 // .w selects the problem's arguments in tab.
-template <typename T, int TX, int TY, bool FULL>
+template <typename T, int TX, int TY, bool FULL, bool NTS>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4)))
 bulk_interior_batch(const BulkArgs<T>* __restrict__ tab, const int4* __restrict__ tiles, int swz) {
   const int4 td = tiles[linear_tile(swz)];
-  interior_tile<T, TX, TY, FULL>(tab[td.w], td);
+  interior_tile<T, TX, TY, FULL, NTS>(tab[td.w], td);
 }
 
 // Band kernel: the points within four points of their block's edge (one-

@@ -19,6 +19,9 @@ extern "C" {
 /* launch knobs of one context */
 #define FDM_TUNE_TILE 0    /* interior tile shape: 0 = 64x4, 1 = 64x8, 2 = 128x4 */
 #define FDM_TUNE_SWIZZLE 1 /* XCD-contiguous tile order: 0 or 1 */
+#define FDM_TUNE_NT_STORE 2 /* non-temporal stores of the interior kernel: 0 or 1
+                               (default: 1 when the 23 field, rate and metric
+                               planes exceed 256 MB) */
 
 /* Set a launch knob of ctx.  Results do not depend on it.  Set knobs
    before putting ctx into an ensemble (its graphs keep the shapes they were
@@ -28,11 +31,14 @@ int fdm_tune(fdm_ctx* ctx, int32_t knob, int32_t value);
 typedef struct fdm_ensemble fdm_ensemble;
 
 /* Group n contexts (each an independent problem with its own storage) so
-   that they advance together: every RK step of all members is one hipGraph
-   launch, captured on first use.  Members must have no point source, no
+   that they advance together: every RK stage of all members is one interior
+   launch per tile shape in use, one edge launch and one band launch over
+   the members' merged work lists, and every step is one hipGraph launch,
+   captured on first use.  Members must have no point source, no
    manufactured solution and no load ramp, use the tile kernels, and share
-   the RK order and current buffer parity.  The contexts stay owned by the
-   caller and must outlive the ensemble. */
+   the precision, RK order and device; at each step they must share the
+   current buffer parity.  The contexts stay owned by the caller and must
+   outlive the ensemble. */
 int fdm_ensemble_create(fdm_ctx* const* members, int32_t n, fdm_ensemble** out);
 
 /* nsteps steps of every member k, step n starting at t[k] + (n-1)*dt[k] for

@@ -176,8 +176,9 @@ def test_tiled_and_pointwise_bulk_adjoints_agree(gpu, monkeypatch):
     """The interior tile kernel (central stencil, LDS) and the per-point
     kernel (SBP matrix entries) sum the same terms in different orders."""
     out = []
-    for mode in ("tile", "generic"):
+    for mode, nt in (("tile", "0"), ("generic", "0"), ("tile", "1")):
         monkeypatch.setenv("FDM_ADJ_BULK", mode)
+        monkeypatch.setenv("FDM_NT_STORE", nt)
         prob, F, ctx, inv = setup(nx=90, nys=(40, 46), nbx=1, seed=7, warp=0.04, background=BG,
                                   perturb=1e-3, law="RFL", nsteps=60)
         rng = np.random.default_rng(8)
@@ -191,3 +192,4 @@ def test_tiled_and_pointwise_bulk_adjoints_agree(gpu, monkeypatch):
         err = np.abs(a - b).max() / np.abs(b).max()
         print(k, "max rel diff", err, flush=True)
         assert err <= 1e-12, k
+        assert np.array_equal(out[2][k], a), k  # non-temporal stores: same bits

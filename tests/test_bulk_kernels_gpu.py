@@ -1,7 +1,7 @@
 """The bulk kernels (interior LDS tiles plus band points, or one lane per
 point everywhere) evaluate the same expression tree per point, so on the
 same inputs they must give bit-identical fields, rates and fault state, in
-FP64 and in FP32-bulk.  Grids here are large enough for interior tiles,
+FP64 and in FP32-bulk, with either kind of store (plain or non-temporal).  Grids here are large enough for interior tiles,
 ragged tile edges in both directions and several block columns.
 """
 import numpy as np
@@ -16,12 +16,14 @@ CS, CP, H = 3.464, 6.0, 0.025
 DT = 0.5 * H / np.sqrt(CS ** 2 + CP ** 2)
 BG = [0.0, 0.0, -120.0, 70.0, -120.0, -120.0]
 MODES = [("generic", {}), ("tile", {"FDM_TILE": "64x8"}), ("tile", {"FDM_TILE": "64x4"}),
-         ("tile", {"FDM_TILE": "128x4"}), ("tile", {"FDM_TILE": "64x8", "FDM_XCD_SWIZZLE": "0"})]
+         ("tile", {"FDM_TILE": "128x4"}), ("tile", {"FDM_TILE": "64x8", "FDM_XCD_SWIZZLE": "0"}),
+         ("tile", {"FDM_TILE": "64x4", "FDM_NT_STORE": "1"}),
+         ("tile", {"FDM_TILE": "128x4", "FDM_NT_STORE": "1"})]
 
 
 def run_mode(monkeypatch, prob, F, nsteps, mode, env):
     monkeypatch.setenv("FDM_BULK", mode)
-    for k in ("FDM_TILE", "FDM_XCD_SWIZZLE"):
+    for k in ("FDM_TILE", "FDM_XCD_SWIZZLE", "FDM_NT_STORE"):
         monkeypatch.delenv(k, raising=False)
     for k, v in env.items():
         monkeypatch.setenv(k, v)

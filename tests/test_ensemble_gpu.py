@@ -38,6 +38,9 @@ def snapshot(c):
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_graph_replays_match_eager_steps(gpu, tmp_path, monkeypatch, precision, graph):
     monkeypatch.setenv("FDM_ENSEMBLE_GRAPH", graph)
+    # the graph cases also run the non-temporal interior stores (members and
+    # their eager twins alike)
+    monkeypatch.setenv("FDM_NT_STORE", graph)
     pairs, t, dt = build_members(tmp_path, precision)
     for k, (_, member) in enumerate(pairs):
         ensemble.apply_tile(member, k % 3)
