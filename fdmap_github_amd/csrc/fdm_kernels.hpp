@@ -231,8 +231,11 @@ __device__ __forceinline__ void edge_task(const EdgeArgs<T>& a, const int2 task)
     double V = ST(FDM_IF_V), S = ST(FDM_IF_S), O = ST(FDM_IF_O), N = ST(FDM_IF_N),
            Nreg = ST(FDM_IF_NREG);
     double D = ST(FDM_IF_D), Psi = ST(FDM_IF_PSI);
+    // exp(Psi/a) of the regularised laws, shared by every strength
+    // evaluation of the Newton solve and by the state rate
+    const double E = I.law >= FDM_LAW_RSL ? exp(Psi / rs[0]) : 0.0;
     bool fail = solve_friction_core(I.law, rs, sw, I.reg, a.fi.outside[q], I.flock, fS0, fN0,
-                                    phip, phis, etas, D, Psi, I.skempton, a.Bdt, V, S, O, N, Nreg);
+                                    phip, phis, etas, D, Psi, E, I.skempton, a.Bdt, V, S, O, N, Nreg);
     if (fail) *a.fail = 1;
     if (a.trace) {
       a.trace[q] = V;
@@ -267,7 +270,7 @@ __device__ __forceinline__ void edge_task(const EdgeArgs<T>& a, const int2 task)
     // set_rates_friction (friction.f90:1101-1124)
     ST(FDM_IF_DDS) = ST(FDM_IF_DDS) + V;
     ST(FDM_IF_DDN) = ST(FDM_IF_DDN) + O;
-    ST(FDM_IF_DPSI) = ST(FDM_IF_DPSI) + state_rate(I.law, rs, V, Psi);
+    ST(FDM_IF_DPSI) = ST(FDM_IF_DPSI) + state_rate(I.law, rs, V, Psi, E);
     if (I.reg) {
       double absV = fabs(V);
       double Nr = -(absV / rs[4]) * (Nreg - fmax(N, 0.0));

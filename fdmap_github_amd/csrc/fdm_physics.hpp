@@ -169,9 +169,11 @@ __device__ __forceinline__ void locked_interface_mode2(double* Fhatm, double* Fh
   nt2xy(Fhatm, n1, n2, vtm, vnm, sttm, sntm, snnm, szzm);
 }
 
-// strength (friction.f90:1041-1074)
+// strength (friction.f90:1041-1074).  E = exp(Psi/a), evaluated once per
+// node and stage by the caller (the same value the reference recomputes
+// on every call).
 __device__ __forceinline__ void strength(int law, double a, double V0, double V, double Psi,
-                                         double N, double& S, double& dSdV) {
+                                         double E, double N, double& S, double& dSdV) {
   if (law == FDM_LAW_SL || law == FDM_LAW_FL) {
     if (V == 0.0) {
       S = 0.0;
@@ -183,7 +185,7 @@ __device__ __forceinline__ void strength(int law, double a, double V0, double V,
       dSdV = N * a / absV;
     }
   } else {
-    double O = 0.5 / V0 * exp(Psi / a);
+    double O = 0.5 / V0 * E;
     S = a * N * arcsinh_ref(O * V);
     dSdV = a * N * O / sqrt(1.0 + (V * O) * (V * O));
   }
@@ -204,7 +206,7 @@ __device__ __forceinline__ void estimate_V(int law, double a, double V0, double&
 // newton_solver (friction.f90:909-1038): returns true on failure
 __device__ __forceinline__ bool newton_solver(int law, double a, double V0, double& Vio,
                                              double& Sio, double Slock, double N, double eta,
-                                             double Psi) {
+                                             double Psi, double E) {
   const int imax = 99;
   const double atolR = 2.220446049250313e-16 * 1e4;
   double V = Vio, S = Sio;
@@ -217,7 +219,7 @@ __device__ __forceinline__ bool newton_solver(int law, double a, double V0, doub
     S = Slock - eta * V;
     double dSdV = -eta;
     double Sf, dSfdV;
-    strength(law, a, V0, V, Psi, N, Sf, dSfdV);
+    strength(law, a, V0, V, Psi, E, N, Sf, dSfdV);
     double R = -S + Sf;
     if (R > 0.0) Vmax = V;
     if (R < 0.0) Vmin = V;
@@ -242,8 +244,9 @@ __device__ __forceinline__ bool newton_solver(int law, double a, double V0, doub
   return i == imax;
 }
 
-// state_rate (friction.f90:1147-1264)
-__device__ __forceinline__ double state_rate(int law, const double* rs, double V, double Psi) {
+// state_rate (friction.f90:1147-1264); E = exp(Psi/a) as for strength
+__device__ __forceinline__ double state_rate(int law, const double* rs, double V, double Psi,
+                                             double E) {
   if (law == FDM_LAW_FRICTIONLESS || law == FDM_LAW_SW || law == FDM_LAW_LOCKED) return 0.0;
   if (V == 0.0) return 0.0;
   double a = rs[0], b = rs[1], V0 = rs[2], f0 = rs[3], L = rs[4], fw = rs[5], Vw = rs[6];
@@ -261,12 +264,12 @@ __device__ __forceinline__ double state_rate(int law, const double* rs, double V
       return -(absV / L) * (frs - fss);
     case FDM_LAW_RSL:
       fss = f0 - (b - a) * log(absV / V0);
-      frs = a * arcsinh_ref(0.5 * absV / V0 * exp(Psi / a));
+      frs = a * arcsinh_ref(0.5 * absV / V0 * E);
       return -(absV / L) * (frs - fss);
     case FDM_LAW_RFL:
       fLV = f0 - (b - a) * log(absV / V0);
       fss = fw + (fLV - fw) / pow(1.0 + pow8(V / Vw), 0.125);
-      frs = a * arcsinh_ref(0.5 * absV / V0 * exp(Psi / a));
+      frs = a * arcsinh_ref(0.5 * absV / V0 * E);
       return (absV / L) * (fss - frs);
     case FDM_LAW_RSF:
       fLV = f0 - (b - a) * log(absV / V0);
@@ -286,9 +289,9 @@ __device__ __forceinline__ bool solve_friction_core(int law, const double* rs, c
                                                     int reg, int outside, double flock,
                                                     double S0i, double N0i, double phip,
                                                     double phis, double eta, double D,
-                                                    double Psi, double skempton, double dt,
-                                                    double& V, double& S, double& O, double& N,
-                                                    double& Nreg) {
+                                                    double Psi, double E, double skempton,
+                                                    double dt, double& V, double& S, double& O,
+                                                    double& N, double& Nreg) {
   double Slock = S0i + phis;
   double Nlock = N0i - (1.0 - skempton) * phip;
   O = 0.0;
@@ -335,8 +338,8 @@ __device__ __forceinline__ bool solve_friction_core(int law, const double* rs, c
       return false;
     }
     default: {
-      bool fail = newton_solver(law, rs[0], rs[2], V, S, Slock, Nreg, eta, Psi);
-      if (fail) fail = newton_solver(law, rs[0], rs[2], V, S, Slock, Nreg, eta, Psi);
+      bool fail = newton_solver(law, rs[0], rs[2], V, S, Slock, Nreg, eta, Psi, E);
+      if (fail) fail = newton_solver(law, rs[0], rs[2], V, S, Slock, Nreg, eta, Psi, E);
       return fail;
     }
   }
