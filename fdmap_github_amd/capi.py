@@ -131,6 +131,7 @@ class fdm_problem(C.Structure):
         ("blocks", C.POINTER(fdm_block)),
         ("ifaces", C.POINTER(fdm_iface)),
         ("source", fdm_source),
+        ("mat_rho", _dp), ("mat_cs", _dp), ("mat_cp", _dp),
     ]
 
 

@@ -227,6 +227,10 @@ struct fdm_ctx {
   // lean interior launches store F and DF non-temporally (working set beyond
   // the last-level cache)
   bool nt_store = false;
+  // heterogeneous medium: rho, G, M, gamma, nu planes (field precision)
+  bool het = false;
+  void* mat[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  std::vector<double> hnu;  // nu of the medium ([ny][nx]), for restore_df6
   // bulk kernel: 0 per-point, 2 interior tiles (kTileShape[tile_shape]) + band points
   int bulk_mode = 2;
   int tile_shape = 0;  // index into kTileShape
@@ -469,6 +473,11 @@ BulkArgs<T> bulk_args(fdm_ctx* c, double A, double Bdt, int do_update, double t,
   a.gammap = c->gammap;
   a.lambda = c->lambda;
   a.xcd_swizzle = c->xcd_swizzle;
+  a.mrho = (const T*)c->mat[0];
+  a.mG = (const T*)c->mat[1];
+  a.mM = (const T*)c->mat[2];
+  a.mgam = (const T*)c->mat[3];
+  a.mnu = (const T*)c->mat[4];
   return a;
 }
 
@@ -500,16 +509,19 @@ void launch_interior(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   c->df6_stale = !full;
   dim3 grid(c->ntiles[q]);
   dim3 block(kTileShape[q][0], kTileShape[q][1]);
-#define FDM_INTERIOR(TX_, TY_)                                                                       \
+#define FDM_INTERIOR(TX_, TY_, HET_)                                                                 \
   do {                                                                                               \
-    if (full) hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, true, false>), grid, block, 0, s, a, c->tiles[q]); \
+    if (full) hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, true, false, HET_>), grid, block, 0, s, a, c->tiles[q]); \
     else if (c->nt_store)                                                                            \
-      hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, false, true>), grid, block, 0, s, a, c->tiles[q]); \
-    else hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, false, false>), grid, block, 0, s, a, c->tiles[q]); \
+      hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, false, true, HET_>), grid, block, 0, s, a, c->tiles[q]); \
+    else hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, false, false, HET_>), grid, block, 0, s, a, c->tiles[q]); \
   } while (0)
-  if (q == 0) FDM_INTERIOR(64, 4);
-  else if (q == 1) FDM_INTERIOR(64, 8);
-  else FDM_INTERIOR(128, 4);
+  // a heterogeneous medium runs the 64 x 4 tiles only (fdm_create and
+  // fdm_tune keep tile_shape 0)
+  if (c->het) FDM_INTERIOR(64, 4, true);
+  else if (q == 0) FDM_INTERIOR(64, 4, false);
+  else if (q == 1) FDM_INTERIOR(64, 8, false);
+  else FDM_INTERIOR(128, 4, false);
 #undef FDM_INTERIOR
   prof_end(c, ep, 0, s);
 }
@@ -639,10 +651,10 @@ void restore_df6(const fdm_ctx* c, T* h) {
   const size_t pe = (size_t)c->ps;
   for (int b = 0; b < c->nb; b++) {
     const BlockDev& B = c->hblocks[b];
-    const T nu = (T)B.nu;
     for (int j = B.my + 4; j <= B.py - 4; j++)
       for (int i = B.mx + 4; i <= B.px - 4; i++) {
         const size_t k = (size_t)j * c->ld + i;
+        const T nu = c->het ? (T)c->hnu[(size_t)j * c->nx + i] : (T)B.nu;
         h[5 * pe + k] = nu * (h[2 * pe + k] + h[4 * pe + k]);
       }
   }
@@ -1133,13 +1145,44 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   // the tile kernel addresses a plane with 32-bit byte offsets
   if (pbytes >= ((size_t)1 << 31) && c->bulk_mode == 2) c->bulk_mode = 0;
   // one arena of 23 planes with a common stride: F (two buffers of 6), DF
-  // (6), metrics xq xr yq yr Ji (5)
+  // (6), metrics xq xr yq yr Ji (5); a heterogeneous medium adds rho, G, M,
+  // gamma, nu (5)
+  c->het = P->mat_rho && P->mat_cs && P->mat_cp;
+  if (c->het) c->tile_shape = 0;
+  const int nplanes = c->het ? 28 : 23;
   void* arena = nullptr;
-  const size_t slack = 64 * c->esz;  // the last plane extends coff elements past 23 * ps
-  if ((rc = dalloc(c, &arena, 23 * pbytes + slack))) return fail(rc);
-  HIPCHK(c, hipMemset(arena, 0, 23 * pbytes + slack));
+  const size_t slack = 64 * c->esz;  // the last plane extends coff elements past nplanes * ps
+  if ((rc = dalloc(c, &arena, nplanes * pbytes + slack))) return fail(rc);
+  HIPCHK(c, hipMemset(arena, 0, nplanes * pbytes + slack));
   char* ab = (char*)arena + (size_t)c->coff * c->esz;
-  c->nt_store = nt_store_choice(23.0 * (double)pbytes);
+  c->nt_store = nt_store_choice((double)nplanes * (double)pbytes);
+  if (c->het) {
+    // init_medium (material.f90:254-262), in FP64, then the field precision
+    const size_t np = (size_t)c->nx * c->ny;
+    std::vector<double> m[5];
+    for (auto& v : m) v.resize(np);
+    for (size_t q = 0; q < np; q++) {
+      const double rho = P->mat_rho[q], cs = P->mat_cs[q], cp = P->mat_cp[q];
+      m[0][q] = rho;
+      m[1][q] = rho * (cs * cs);
+      m[2][q] = rho * (cp * cp);
+      m[3][q] = 1.0 - 2.0 * ((cs / cp) * (cs / cp));
+      m[4][q] = 0.5 * (m[2][q] - 2.0 * m[1][q]) / (m[2][q] - m[1][q]);
+    }
+    c->hnu = m[4];
+    for (int k = 0; k < 5; k++) {
+      c->mat[k] = ab + (23 + k) * pbytes;
+      if (c->precision == FDM_FP64) {
+        std::vector<double> h;
+        pack_plane(h, m[k].data(), c->nx, c->ny, c->ld);
+        HIPCHK(c, hipMemcpy(c->mat[k], h.data(), h.size() * 8, hipMemcpyHostToDevice));
+      } else {
+        std::vector<float> h;
+        pack_plane(h, m[k].data(), c->nx, c->ny, c->ld);
+        HIPCHK(c, hipMemcpy(c->mat[k], h.data(), h.size() * 4, hipMemcpyHostToDevice));
+      }
+    }
+  }
   c->F[0] = ab;
   c->F[1] = ab + 6 * pbytes;
   c->DF = ab + 12 * pbytes;
@@ -1373,6 +1416,7 @@ int fdm_profile_read(fdm_ctx* c, fdm_profile_t* out) {
 int fdm_tune(fdm_ctx* c, int32_t knob, int32_t value) {
   if (!c) return FDM_EINVAL;
   if (knob == FDM_TUNE_TILE && value >= 0 && value < 3) {
+    if (c->het && value != 0) return set_err(c, FDM_EINVAL, "fdm_tune: a heterogeneous medium runs 64x4 tiles only");
     c->tile_shape = value;
     return FDM_OK;
   }
@@ -1613,10 +1657,11 @@ int fdm_ensemble_create(fdm_ctx* const* members, int32_t n, fdm_ensemble** out) 
     if (!c) return FDM_EINVAL;
     const fdm_ctx* c0 = members[0];
     if (c->src.nplace > 0 || c->any_mms || (!c->ld_T.empty() && c->ld_T[0] > 0.0) || c->bulk_mode != 2 ||
-        c->nstage != c0->nstage || c->precision != c0->precision || c->device != c0->device) {
+        c->het || c->nstage != c0->nstage || c->precision != c0->precision || c->device != c0->device) {
       g_create_error =
           "fdm_ensemble_create: members must have no point source, no manufactured solution, "
-          "no load ramp, the tile kernels, and the same RK order, precision and device";
+          "no load ramp, no heterogeneous medium, the tile kernels, and the same RK order, "
+          "precision and device";
       return FDM_EINVAL;
     }
     for (int q = 0; q < k; q++)

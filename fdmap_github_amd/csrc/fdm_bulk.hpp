@@ -40,6 +40,12 @@ struct BulkArgs {
   double* gammap;
   double* lambda;
   int xcd_swizzle;  // remap workgroups so each XCD (and its L2) owns contiguous row bands
+  // heterogeneous medium: rho, G, M, gamma, nu per point (null: the block's)
+  const T* mrho;
+  const T* mG;
+  const T* mM;
+  const T* mgam;
+  const T* mnu;
 };
 
 // Map the hardware workgroup id to a tile.  Workgroups are dispatched to the
@@ -166,16 +172,16 @@ __device__ __forceinline__ T matsel(int mat, T rho, T G, T M) {
 // memory the neighbourhood comes from.  The loop runs over s outside the
 // five fields: each field still adds its s = 1 term before its s = 2 term,
 // as in the reference, while only one set of metric averages is live.
-template <typename T, int l, typename Acc>
-__device__ __forceinline__ void interior_term(T* d, const Acc& A, int s, T gamma, T c, T Ji,
-                                              T ayp, T axp, T aym, T axm, T byp, T bxp, T bym,
-                                              T bxm) {
+template <typename T, int l, typename Acc, typename Coef>
+__device__ __forceinline__ void interior_term(T* d, const Acc& A, int s, T gamma, const Coef& cI,
+                                              T Ji, T ayp, T axp, T aym, T axm, T byp, T bxp,
+                                              T bym, T bxm) {
   using P = Fld<l>;
   T t = prod<T, P::ga>(ayp, A.f(P::ua, s, 0), gamma) - prod<T, P::gb>(axp, A.f(P::ub, s, 0), gamma) -
         prod<T, P::ga>(aym, A.f(P::ua, -s, 0), gamma) + prod<T, P::gb>(axm, A.f(P::ub, -s, 0), gamma) -
         prod<T, P::ga>(byp, A.f(P::ua, 0, s), gamma) + prod<T, P::gb>(bxp, A.f(P::ub, 0, s), gamma) +
         prod<T, P::ga>(bym, A.f(P::ua, 0, -s), gamma) - prod<T, P::gb>(bxm, A.f(P::ub, 0, -s), gamma);
-  d[l] = d[l] + t * c * Ji;
+  d[l] = d[l] + cI.template scale<l>(t, s, Ji);
 }
 
 // cI(l, s) returns the interior coefficient of field l and offset s
@@ -187,19 +193,38 @@ __device__ __forceinline__ void interior_rates(T* d, const Acc& A, T gamma, cons
     const T aym = (T)0.5 * (A.yr(-s) + A.yr(0)), axm = (T)0.5 * (A.xr(-s) + A.xr(0));
     const T byp = (T)0.5 * (A.yq(s) + A.yq(0)), bxp = (T)0.5 * (A.xq(s) + A.xq(0));
     const T bym = (T)0.5 * (A.yq(-s) + A.yq(0)), bxm = (T)0.5 * (A.xq(-s) + A.xq(0));
-    interior_term<T, 0>(d, A, s, gamma, cI(0, s), Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
-    interior_term<T, 1>(d, A, s, gamma, cI(1, s), Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
-    interior_term<T, 2>(d, A, s, gamma, cI(2, s), Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
-    interior_term<T, 3>(d, A, s, gamma, cI(3, s), Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
-    interior_term<T, 4>(d, A, s, gamma, cI(4, s), Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term<T, 0>(d, A, s, gamma, cI, Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term<T, 1>(d, A, s, gamma, cI, Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term<T, 2>(d, A, s, gamma, cI, Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term<T, 3>(d, A, s, gamma, cI, Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term<T, 4>(d, A, s, gamma, cI, Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
   }
 }
 
-// interior coefficients of a block, from any address space
+// interior coefficients of a block, from any address space: t DI(s)/rho Ji
+// etc. with DI(s)/rho precomputed (rates.f90:400-405, 411-444)
 template <typename T, typename BlockRef>
 struct BlockCoef {
   BlockRef B;
-  __device__ __forceinline__ T operator()(int l, int s) const { return (T)B.cI[l][s - 1]; }
+  template <int l>
+  __device__ __forceinline__ T scale(T t, int s, T Ji) const {
+    return t * (T)B.cI[l][s - 1] * Ji;
+  }
+};
+
+// the material of one point of a heterogeneous medium:
+// ")*DI(s)*JacInv/rho", ")*DI(s)*JacInv*M", ")*DI(s)*JacInv*G"
+// (rates_heterogeneous.f90:392-428)
+template <typename T>
+struct PointCoef {
+  T rho, G, M;
+  template <int l>
+  __device__ __forceinline__ T scale(T t, int s, T Ji) const {
+    const T u = t * (s == 1 ? (T)FDM_DI1 : (T)FDM_DI2) * Ji;
+    if constexpr (l < 2) return u / rho;
+    else if constexpr (l == 3) return u * G;
+    else return u * M;
+  }
 };
 
 // neighbourhood read straight from global memory
@@ -323,7 +348,8 @@ __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDe
                                              T (&d)[6], const T (&Fc)[6]) {
   const long long ps = a.ps;
   // rates_szz (rates.f90:920-936): overwrite
-  d[5] = (T)B.nu * (d[2] + d[4]);
+  // (rates_h_szz, rates_heterogeneous.f90:901-917: the point's nu)
+  d[5] = (a.mnu ? a.mnu[cc] : (T)B.nu) * (d[2] + d[4]);
   // singular source (source.f90:196-254)
   if constexpr (FULL) for (int p = 0; p < a.nplace; p++) {
     const SourcePlaceDev& P = a.src[p];
@@ -363,7 +389,10 @@ __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDe
       }
       long long pc = (long long)j * a.nx + i;
       double lam = a.lambda[pc], gp = a.gammap[pc];
-      plastic_point(B.G, B.K, B.mu, B.beta, B.b, B.h, B.eta, B.s0, a.dt, s, Ds, lam, gp);
+      // a heterogeneous medium gives each point its G and K (plastic.f90:37-43)
+      const double Gel = a.mG ? (double)a.mG[cc] : B.G;
+      const double Kel = a.mG ? (double)a.mM[cc] - 1.333333333333333 * (double)a.mG[cc] : B.K;
+      plastic_point(Gel, Kel, B.mu, B.beta, B.b, B.h, B.eta, B.s0, a.dt, s, Ds, lam, gp);
       a.lambda[pc] = lam;
       a.gammap[pc] = gp;
       if constexpr (sizeof(T) == 4) {
@@ -411,7 +440,11 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
   const T* __restrict__ yq = a.yq;
   const T* __restrict__ yr = a.yr;
   const T Ji = a.Ji[cc];
-  const T gamma = (T)B.gamma, rho = (T)B.rho, G = (T)B.G, M = (T)B.M;
+  // the point's material in a heterogeneous medium (rates_h_boundary_mode2,
+  // rates_heterogeneous.f90:435-898, the same expressions)
+  const bool het = a.mrho != nullptr;
+  const T gamma = het ? a.mgam[cc] : (T)B.gamma, rho = het ? a.mrho[cc] : (T)B.rho;
+  const T G = het ? a.mG[cc] : (T)B.G, M = het ? a.mM[cc] : (T)B.M;
   const bool qone = (dl < 4) || (dr < 4);
   const bool rone = (db < 4) || (dtp < 4);
   const long long row = (long long)j * ld;
@@ -504,7 +537,10 @@ __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i,
   scaled_rates<T>(a, cc, d);
   if (i - B.mx >= 4 && B.px - i >= 4 && j - B.my >= 4 && B.py - j >= 4) {
     const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld};
-    interior_rates<T>(d, acc, (T)B.gamma, BlockCoef<T, const BlockDev&>{B}, a.Ji[cc]);
+    if (a.mrho)
+      interior_rates<T>(d, acc, a.mgam[cc], PointCoef<T>{a.mrho[cc], a.mG[cc], a.mM[cc]}, a.Ji[cc]);
+    else
+      interior_rates<T>(d, acc, (T)B.gamma, BlockCoef<T, const BlockDev&>{B}, a.Ji[cc]);
   } else {
     closure_rates<T>(a, B, i, j, cc, d);
   }
@@ -589,7 +625,7 @@ typedef const __attribute__((address_space(4))) BlockDev ConstBlockDev;
 // xq/yq with the r halo go to LDS; the lane's own DF, F6 and Ji stay in
 // registers.  Halo indices are clamped into the grid; a clamped value never
 // reaches an active lane.  tiles[k] = (block, i0, j0, 0).
-template <typename T, int TX, int TY, bool FULL, bool NTS>
+template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET>
 __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 td) {
   constexpr int HX = TX + 4, HY = TY + 4, NT = TX * TY;
   constexpr int NF = (HY * HX + NT - 1) / NT;  // staging passes, F1..F5
@@ -648,6 +684,15 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
   }
   const T f6 = ld_off(F + 5 * ps, oc);
   const T Ji = ld_off(a.Ji, oc);
+  // a heterogeneous medium: the lane's own material (rho, G, M, gamma, nu)
+  T mat[5];
+  if constexpr (HET) {
+    mat[0] = ld_off(a.mrho, oc);
+    mat[1] = ld_off(a.mG, oc);
+    mat[2] = ld_off(a.mM, oc);
+    mat[3] = ld_off(a.mgam, oc);
+    mat[4] = ld_off(a.mnu, oc);
+  }
 #pragma unroll
   for (int p = 0; p < NF; p++) {
     const int k = tid + p * NT;
@@ -688,7 +733,10 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     d[5] = (T)0.0;  // DF6 is overwritten by the szz relation
   }
   const TileAcc<T, TX, TY> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y};
-  interior_rates<T>(d, acc, (T)B4->gamma, BlockCoef<T, ConstBlockDev&>{*B4}, Ji);
+  if constexpr (HET)
+    interior_rates<T>(d, acc, mat[3], PointCoef<T>{mat[0], mat[1], mat[2]}, Ji);
+  else
+    interior_rates<T>(d, acc, (T)B4->gamma, BlockCoef<T, ConstBlockDev&>{*B4}, Ji);
   T Fc[6];
 #pragma unroll
   for (int k = 0; k < 5; k++) Fc[k] = acc.f(k, 0, 0);
@@ -700,7 +748,9 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     // DF6 is not stored: no stage reads it (each overwrites it with this
     // relation), and fdm_get_fields recomputes it from DF3 and DF5 with the
     // same expression.
-    const T nu = (T)B4->nu;
+    T nu;
+    if constexpr (HET) nu = mat[4];
+    else nu = (T)B4->nu;
     d[5] = nu * (d[2] + d[4]);
     if (a.do_update) {
       const T Bdt = (T)a.Bdt;
@@ -712,10 +762,12 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
   }
 }
 
-template <typename T, int TX, int TY, bool FULL, bool NTS>
-__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4)))
+// a heterogeneous medium holds five more values per lane: three waves per
+// SIMD keep them in registers
+template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : (HET ? 3 : 4))))
 bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
-  interior_tile<T, TX, TY, FULL, NTS>(a, tiles[linear_tile(a.xcd_swizzle)]);
+  interior_tile<T, TX, TY, FULL, NTS, HET>(a, tiles[linear_tile(a.xcd_swizzle)]);
 }
 
 // Batched interior kernel: the tiles of many problems in one launch; tile
@@ -724,7 +776,7 @@ template <typename T, int TX, int TY, bool FULL, bool NTS>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4)))
 bulk_interior_batch(const BulkArgs<T>* __restrict__ tab, const int4* __restrict__ tiles, int swz) {
   const int4 td = tiles[linear_tile(swz)];
-  interior_tile<T, TX, TY, FULL, NTS>(tab[td.w], td);
+  interior_tile<T, TX, TY, FULL, NTS, false>(tab[td.w], td);
 }
 
 // Band kernel: the points within four points of their block's edge (one-

@@ -70,6 +70,7 @@ module fdmap_capi
      type(c_ptr) :: xq, xr, yq, yr, Ji, x, y
      type(c_ptr) :: blocks, ifaces
      type(fdm_source) :: source
+     type(c_ptr) :: mat_rho = c_null_ptr, mat_cs = c_null_ptr, mat_cp = c_null_ptr
   end type fdm_problem
 
   type,bind(C) :: fdm_profile_t

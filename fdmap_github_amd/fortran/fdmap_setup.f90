@@ -47,6 +47,7 @@ module fdmap_setup
      type(block_grid) :: G
      character(256) :: response
      real(dp) :: rho,G_,cs,cp,M_,lambda,nu,K,gamma,Zs,Zp,mu,beta,b,h,eta
+     logical :: het = .false.
      real(dp) :: F0blk(9)
      character(256) :: problem
      type(side_data) :: side(4)
@@ -76,6 +77,8 @@ module fdmap_setup
   type(host_iface),allocatable,target :: HI(:)
   real(dp),allocatable,target :: gcx(:,:),gcy(:,:),gxq(:,:),gxr(:,:),gyq(:,:),gyr(:,:),gJ(:,:),gJi(:,:)
   real(dp),allocatable,target :: gF(:,:,:)
+  ! heterogeneous medium (material.f90:226-321): rho, cs, cp at every point
+  real(dp),allocatable,target :: gmrho(:,:),gmcs(:,:),gmcp(:,:)
   real(dp) :: hmin,hmax
 
   ! C description
@@ -432,6 +435,7 @@ contains
     if (allocated(HI)) deallocate(HI)
     if (allocated(gcx)) deallocate(gcx,gcy,gxq,gxr,gyq,gyr,gJ,gJi)
     if (allocated(gF)) deallocate(gF)
+    if (allocated(gmrho)) deallocate(gmrho,gmcs,gmcp)
     if (allocated(cblk)) deallocate(cblk)
     if (allocated(cif)) deallocate(cif)
   end subroutine release_problem
@@ -680,7 +684,7 @@ contains
     type(fields_perturb) :: P1,P2,P3,P4,P5
     logical :: prestress_from_file,initial_condition_from_file
     character(256) :: prestress_filename,initial_condition_filename
-    real(dp) :: F0(6),FP(6,5),A(5),xx,yy,Fv(6),Ks,Kp,nrm
+    real(dp) :: F0(6),FP(6,5),A(5),xx,yy,Fv(6),Ks,Kp,nrm,erho,ecs,ecp
     namelist /material_list/ response,heterogeneous,filename,rho,G,cs,cp,beta,mu,b,h,eta, &
          plastic_strain_tensor,plastic_work
     namelist /boundaries_list/ bcL,bcR,bcB,bcT,rhog,problem
@@ -706,13 +710,16 @@ contains
          if (stat>0) call fail('Error in material_list')
          if (trim(response)/='elastic'.and.trim(response)/='plastic') &
               call fail('Invalid material response: '//trim(response))
-         if (heterogeneous) call fail('heterogeneous material is not supported')
-         if (cs==0.0_dp) call fail('Must specify S-wave speed')
+         ! the first heterogeneous block reads the medium of the whole grid
+         ! (material.f90:80-81)
+         if (heterogeneous.and.(.not.allocated(gmrho))) call read_medium(filename)
+         if (cs==0.0_dp.and.(.not.heterogeneous)) call fail('Must specify S-wave speed')
          if (setup_error/='') return
          if (cp==0.0_dp) cp = sqrt(3.0_dp)*cs
          if (rho==0.0_dp) rho = G/cs**2
          if (G==0.0_dp) G = rho*cs**2
          HBk%response = response
+         HBk%het = heterogeneous
          HBk%rho = rho
          HBk%G_ = G
          HBk%cs = cs
@@ -819,11 +826,18 @@ contains
                Fv = F0+A(1)*FP(:,1)+A(2)*FP(:,2)+A(3)*FP(:,3)+A(4)*FP(:,4)
                call initial_fields(Fv,xx,yy,t_init,problem,ib)
                HBk%side(s)%F0(k,:) = Fv
-               Ks = H00i*cs*nrm/gJ(i,j)
-               Kp = H00i*cp*nrm/gJ(i,j)
-               HBk%side(s)%M(k,1) = rho*cs
-               HBk%side(s)%M(k,2) = rho*cp
-               HBk%side(s)%M(k,3) = 1.0_dp-2.0_dp*(cs/cp)**2
+               ! init_edges: the medium at the node for a heterogeneous block
+               ! (domain.f90:1379-1456)
+               if (HBk%het) then
+                  erho = gmrho(i,j); ecs = gmcs(i,j); ecp = gmcp(i,j)
+               else
+                  erho = rho; ecs = cs; ecp = cp
+               end if
+               Ks = H00i*ecs*nrm/gJ(i,j)
+               Kp = H00i*ecp*nrm/gJ(i,j)
+               HBk%side(s)%M(k,1) = erho*ecs
+               HBk%side(s)%M(k,2) = erho*ecp
+               HBk%side(s)%M(k,3) = 1.0_dp-2.0_dp*(ecs/ecp)**2
                HBk%side(s)%M(k,4) = Ks
                HBk%side(s)%M(k,5) = Kp
             end do
@@ -1092,6 +1106,26 @@ contains
   end subroutine read_arrays
 
 
+  subroutine read_medium(fname)
+    ! mediumIO('read') (material.f90:278-321): rho, cs, cp of the whole grid,
+    ! each an FP64 array [gnx, gny], x fastest
+    character(*),intent(in) :: fname
+    integer :: u,stat
+    allocate(gmrho(gnx,gny),gmcs(gnx,gny),gmcp(gnx,gny))
+    open(newunit=u,file=fname,access='stream',form='unformatted',status='old', &
+         action='read',iostat=stat)
+    if (stat/=0) then
+       call fail('cannot open medium file '//trim(fname))
+       return
+    end if
+    read(u,iostat=stat) gmrho
+    if (stat==0) read(u,iostat=stat) gmcs
+    if (stat==0) read(u,iostat=stat) gmcp
+    if (stat/=0) call fail('cannot read medium file '//trim(fname))
+    close(u)
+  end subroutine read_medium
+
+
   subroutine read_pair(fname,n,a,b)
     ! S0 then N0 (friction.f90:422-443)
     character(*),intent(in) :: fname
@@ -1119,12 +1153,17 @@ contains
        call fail('Minimum grid spacing must be positive')
        return
     end if
-    cs = 0.0_dp
-    cp = 0.0_dp
-    do i = 1,nblocks
-       cs = max(cs,HB(i)%cs)
-       cp = max(cp,HB(i)%cp)
-    end do
+    if (allocated(gmrho)) then ! heterogeneous properties
+       cs = maxval(gmcs)
+       cp = maxval(gmcp)
+    else
+       cs = 0.0_dp
+       cp = 0.0_dp
+       do i = 1,nblocks
+          cs = max(cs,HB(i)%cs)
+          cp = max(cp,HB(i)%cp)
+       end do
+    end if
     c = sqrt(cs**2+cp**2)
     dt = dt/refine
     if (CFL==0.0_dp) then
@@ -1251,6 +1290,15 @@ contains
     cprob%Ji = c_loc(gJi(1,1))
     cprob%x = c_loc(gcx(1,1))
     cprob%y = c_loc(gcy(1,1))
+    if (allocated(gmrho)) then
+       cprob%mat_rho = c_loc(gmrho(1,1))
+       cprob%mat_cs = c_loc(gmcs(1,1))
+       cprob%mat_cp = c_loc(gmcp(1,1))
+    else
+       cprob%mat_rho = c_null_ptr
+       cprob%mat_cs = c_null_ptr
+       cprob%mat_cp = c_null_ptr
+    end if
     ! the fields problem of the last block selects the body source
     ! (fields_type%problem is shared, fields.f90:132; source.f90:256-296)
     last_problem = HB(nblocks)%problem

@@ -68,14 +68,16 @@ vx vy
 
 def c2_planar_rs(name="c2", nx=1601, ny=801, h=0.025, nsteps=4000, outputs=True, ninfo=500,
                  law="RFL", plastic=False, curve=None, tau0=70.0, sn0=120.0, overstress=25.0,
-                 R=1.5, Vinit=1e-12, a=0.010, b=0.014, sxy0=None, bcT="absorbing"):
+                 R=1.5, Vinit=1e-12, a=0.010, b=0.014, sxy0=None, bcT="absorbing",
+                 medium=None):
     """Two blocks nx x ny coupled across a horizontal fault by SAT; slip law
     with strong velocity weakening; Gaussian overstress at the fault centre.
     curve: optional path of a fault-profile curve file (FP64 x, y, n1, n2),
     used for the top edge of block 1 and the bottom edge of block 2.  The
     background shear stress is sxy0 (default tau0); Psi0 gives slip velocity
     Vinit under tau0.  bcT: boundary condition of the top of the upper block
-    ('free' for a free surface)."""
+    ('free' for a free surface).  medium: path of a heterogeneous medium file
+    (write_medium) that both blocks read."""
     Lx = 0.5 * (nx - 1) * h
     Ly = (ny - 1) * h
     psi0 = rs_state(a, 1e-6, tau0, sn0, Vinit)
@@ -84,6 +86,8 @@ def c2_planar_rs(name="c2", nx=1601, ny=801, h=0.025, nsteps=4000, outputs=True,
     mat = f"rho = {_f(RHO)}, cs = {_f(CS)}, cp = {_f(CP)}"
     if plastic:
         mat = f"response = 'plastic', {mat}, mu = 0.6d0, b = 0d0, beta = 0d0"
+    if medium:
+        mat = f"{mat}, heterogeneous = T, filename = '{medium}'"
     out = ""
     if outputs:
         st = max(1, nsteps // 100)
@@ -167,6 +171,16 @@ def write_curve(path, x, y, n1, n2):
     with open(path, "wb") as f:
         for a in (x, y, n1, n2):
             np.ascontiguousarray(a, dtype="<f8").tofile(f)
+
+
+def write_medium(path, rho, cs, cp):
+    """A heterogeneous medium file (material.f90:278-321): density, S and P
+    wave speed of every grid point, each an FP64 array [ny, nx] of the whole
+    grid, x fastest."""
+    with open(path, "wb") as f:
+        for a in (rho, cs, cp):
+            np.ascontiguousarray(a, dtype="<f8").tofile(f)
+    return path
 
 
 def write(path, text):

@@ -57,6 +57,8 @@ class Problem:
         self.blocks: list[dict] = []
         self.ifaces: list[dict] = []
         self.source: dict | None = None
+        # heterogeneous medium: (rho, cs, cp) arrays [ny, nx], or None
+        self.medium: tuple | None = None
         self._keep = []
 
     # ---- construction helpers ----
@@ -107,6 +109,10 @@ class Problem:
             P(self.xq), P(self.xr), P(self.yq), P(self.yr), P(self.Ji))
         prob.x = P(self.x) if self.x is not None else capi.dptr(None)
         prob.y = P(self.y) if self.y is not None else capi.dptr(None)
+        if self.medium is not None:
+            prob.mat_rho, prob.mat_cs, prob.mat_cp = (P(a) for a in self.medium)
+        else:
+            prob.mat_rho = prob.mat_cs = prob.mat_cp = capi.dptr(None)
 
         barr = (capi.fdm_block * max(1, len(self.blocks)))()
         for b, blk in enumerate(self.blocks):

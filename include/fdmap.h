@@ -175,6 +175,16 @@ typedef struct fdm_problem {
   const fdm_block* blocks;
   const fdm_iface* ifaces;
   fdm_source source;
+  /* Heterogeneous medium (material.f90:226-275): density and wave speeds
+     at every point ([ny][nx]), or all three NULL.  When given, the rates of
+     every block use them (rates_heterogeneous.f90:375-917), as the reference
+     does once any block has read a medium file; so do plasticity and the
+     szz relation.  Edge coefficients stay those of fdm_side.M, which the
+     host fills per block (domain.f90:1379-1456), and sources keep the
+     block's density. */
+  const double* mat_rho;
+  const double* mat_cs;
+  const double* mat_cp;
 } fdm_problem;
 
 /* Kernel timing gathered with HIP events on the context's stream. */

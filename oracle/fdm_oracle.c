@@ -344,6 +344,9 @@ struct orc_ctx {
   double *xq, *xr, *yq, *yr, *Ji, *x, *y;
   double *F, *DF;
   double *gammap, *lambda;
+  /* heterogeneous medium (material.f90:226-275): rho, G, M, gamma, nu per
+     point, or NULL */
+  double *mrho, *mG, *mM, *mgam, *mnu;
   oblock_t* B;
   oiface_t* I;
   fdm_source src;
@@ -408,6 +411,22 @@ int orc_create(const fdm_problem* P, orc_ctx** out) {
   c->Ji = dup_arr(P->Ji, c->np);
   c->x = dup_arr(P->x, c->np);
   c->y = dup_arr(P->y, c->np);
+  if (P->mat_rho && P->mat_cs && P->mat_cp) {
+    /* init_medium (material.f90:254-262) */
+    c->mrho = dup_arr(P->mat_rho, c->np);
+    c->mG = alloc_fill(c->np, 0.0);
+    c->mM = alloc_fill(c->np, 0.0);
+    c->mgam = alloc_fill(c->np, 0.0);
+    c->mnu = alloc_fill(c->np, 0.0);
+    if (c->mrho && c->mG && c->mM && c->mgam && c->mnu)
+      for (size_t q = 0; q < c->np; q++) {
+        double rho = P->mat_rho[q], cs = P->mat_cs[q], cp = P->mat_cp[q];
+        c->mG[q] = rho * (cs * cs);
+        c->mM[q] = rho * (cp * cp);
+        c->mgam[q] = 1.0 - 2.0 * ((cs / cp) * (cs / cp));
+        c->mnu[q] = 0.5 * (c->mM[q] - 2.0 * c->mG[q]) / (c->mM[q] - c->mG[q]);
+      }
+  }
   c->F = alloc_fill(6 * c->np, 0.0);
   c->DF = alloc_fill(6 * c->np, 1e40);
   c->gammap = alloc_fill(c->np, 0.0);
@@ -533,6 +552,11 @@ void orc_destroy(orc_ctx* c) {
   free(c->DF);
   free(c->gammap);
   free(c->lambda);
+  free(c->mrho);
+  free(c->mG);
+  free(c->mM);
+  free(c->mgam);
+  free(c->mnu);
   if (c->B)
     for (int ib = 0; ib < c->nb; ib++)
       for (int s = 0; s < 4; s++) {
@@ -1214,34 +1238,50 @@ typedef struct {
   const double *F[5], *xq, *xr, *yq, *yr, *Ji;
   double *DF[5];
   double rho, G, M, gamma;
+  /* heterogeneous medium: material at each point (NULL: the block's) */
+  const double *prho, *pG, *pM, *pgam;
   int nx;
 } sten_t;
 
+/* the material at centre cc: the block's, or the medium's */
+static inline void mat_at(const sten_t* S, size_t cc, double* rho, double* G, double* M,
+                          double* gamma) {
+  if (S->prho) {
+    *rho = S->prho[cc];
+    *G = S->pG[cc];
+    *M = S->pM[cc];
+    *gamma = S->pgam[cc];
+  } else {
+    *rho = S->rho;
+    *G = S->G;
+    *M = S->M;
+    *gamma = S->gamma;
+  }
+}
+
 /* one-sided q term at neighbour kk (same row) of centre cc */
-static inline double q1(const sten_t* S, int l, size_t kk, size_t cc) {
+static inline double q1(const sten_t* S, int l, size_t kk, size_t cc, double g) {
   double ay = 0.5 * (S->yr[kk] + S->yr[cc]);
   double ax = 0.5 * (S->xr[kk] + S->xr[cc]);
-  return prodg(ay, S->F[UA[l]][kk], GA[l], S->gamma) - prodg(ax, S->F[UB[l]][kk], GB[l], S->gamma);
+  return prodg(ay, S->F[UA[l]][kk], GA[l], g) - prodg(ax, S->F[UB[l]][kk], GB[l], g);
 }
 /* one-sided r term at neighbour kk (same column) of centre cc */
-static inline double r1(const sten_t* S, int l, size_t kk, size_t cc) {
+static inline double r1(const sten_t* S, int l, size_t kk, size_t cc, double g) {
   double by = 0.5 * (S->yq[kk] + S->yq[cc]);
   double bx = 0.5 * (S->xq[kk] + S->xq[cc]);
-  return -prodg(by, S->F[UA[l]][kk], GA[l], S->gamma) + prodg(bx, S->F[UB[l]][kk], GB[l], S->gamma);
+  return -prodg(by, S->F[UA[l]][kk], GA[l], g) + prodg(bx, S->F[UB[l]][kk], GB[l], g);
 }
 /* central q (4 terms) */
-static inline double qc(const sten_t* S, int l, size_t kp, size_t km, size_t cc) {
+static inline double qc(const sten_t* S, int l, size_t kp, size_t km, size_t cc, double g) {
   double ayp = 0.5 * (S->yr[kp] + S->yr[cc]), axp = 0.5 * (S->xr[kp] + S->xr[cc]);
   double aym = 0.5 * (S->yr[km] + S->yr[cc]), axm = 0.5 * (S->xr[km] + S->xr[cc]);
-  double g = S->gamma;
   return prodg(ayp, S->F[UA[l]][kp], GA[l], g) - prodg(axp, S->F[UB[l]][kp], GB[l], g) -
          prodg(aym, S->F[UA[l]][km], GA[l], g) + prodg(axm, S->F[UB[l]][km], GB[l], g);
 }
 /* central r (4 terms) */
-static inline double rc(const sten_t* S, int l, size_t kp, size_t km, size_t cc) {
+static inline double rc(const sten_t* S, int l, size_t kp, size_t km, size_t cc, double g) {
   double byp = 0.5 * (S->yq[kp] + S->yq[cc]), bxp = 0.5 * (S->xq[kp] + S->xq[cc]);
   double bym = 0.5 * (S->yq[km] + S->yq[cc]), bxm = 0.5 * (S->xq[km] + S->xq[cc]);
-  double g = S->gamma;
   return -prodg(byp, S->F[UA[l]][kp], GA[l], g) + prodg(bxp, S->F[UB[l]][kp], GB[l], g) +
          prodg(bym, S->F[UA[l]][km], GA[l], g) - prodg(bxm, S->F[UB[l]][km], GB[l], g);
 }
@@ -1262,10 +1302,11 @@ static void rates_interior(const sten_t* S, int mx, int px, int my, int py) {
   for (int j = my; j <= py; j++)
     for (int i = mx; i <= px; i++) {
       size_t cc = (size_t)j * nx + i;
+      double rho, G, M, g;
+      mat_at(S, cc, &rho, &G, &M, &g);
       for (int s = 1; s <= 3; s++)
         for (int l = 0; l < 5; l++) {
           size_t ip = cc + s, im = cc - s, jp = cc + (size_t)s * nx, jm = cc - (size_t)s * nx;
-          double g = S->gamma;
           double ayp = 0.5 * (S->yr[ip] + S->yr[cc]), axp = 0.5 * (S->xr[ip] + S->xr[cc]);
           double aym = 0.5 * (S->yr[im] + S->yr[cc]), axm = 0.5 * (S->xr[im] + S->xr[cc]);
           double byp = 0.5 * (S->yq[jp] + S->yq[cc]), bxp = 0.5 * (S->xq[jp] + S->xq[cc]);
@@ -1276,7 +1317,18 @@ static void rates_interior(const sten_t* S, int mx, int px, int my, int py) {
                      prodg(aym, FA[im], GA[l], g) + prodg(axm, FB[im], GB[l], g) -
                      prodg(byp, FA[jp], GA[l], g) + prodg(bxp, FB[jp], GB[l], g) +
                      prodg(bym, FA[jm], GA[l], g) - prodg(bxm, FB[jm], GB[l], g);
-          S->DF[l][cc] = S->DF[l][cc] + t * coef[l][s] * S->Ji[cc];
+          if (!S->prho) {
+            S->DF[l][cc] = S->DF[l][cc] + t * coef[l][s] * S->Ji[cc];
+          } else {
+            /* rates_h_interior_mode2 (rates_heterogeneous.f90:375-432):
+               ")*DI(s)*JacInv/rho", ")*DI(s)*JacInv*M", ")*DI(s)*JacInv*G" */
+            double D = s <= PI_I ? DIc(s) : 0.0;
+            double u = t * D * S->Ji[cc];
+            if (l < 2) u = u / rho;
+            else if (l == 3) u = u * G;
+            else u = u * M;
+            S->DF[l][cc] = S->DF[l][cc] + u;
+          }
         }
     }
 }
@@ -1291,6 +1343,10 @@ static void rates_boundary(const sten_t* S, int mx, int px, int my, int py, int 
     for (int i = mx; i <= px; i++) {
       size_t cc = (size_t)j * nx + i;
       double Ji = S->Ji[cc];
+      /* rates_h_boundary_mode2 (rates_heterogeneous.f90:435-898) has the
+         same expressions with the point's material */
+      double rho, G, M, g;
+      mat_at(S, cc, &rho, &G, &M, &g);
       if (qside && rside) {
         for (int s = 0; s < NBND; s++) {
           size_t kq, kr;
@@ -1310,9 +1366,9 @@ static void rates_boundary(const sten_t* S, int mx, int px, int my, int py, int 
             Dr = DRc(s, by1 - j);
           }
           for (int l = 0; l < 5; l++)
-            S->DF[l][cc] = S->DF[l][cc] + bscale(l, q1(S, l, kq, cc), Dq, S->rho, S->G, S->M) * Ji;
+            S->DF[l][cc] = S->DF[l][cc] + bscale(l, q1(S, l, kq, cc, g), Dq, rho, G, M) * Ji;
           for (int l = 0; l < 5; l++)
-            S->DF[l][cc] = S->DF[l][cc] + bscale(l, r1(S, l, kr, cc), Dr, S->rho, S->G, S->M) * Ji;
+            S->DF[l][cc] = S->DF[l][cc] + bscale(l, r1(S, l, kr, cc, g), Dr, rho, G, M) * Ji;
         }
       } else if (qside) {
         for (int s = 0; s < NBND; s++) {
@@ -1326,13 +1382,13 @@ static void rates_boundary(const sten_t* S, int mx, int px, int my, int py, int 
             Dq = DRc(s, bx1 - i);
           }
           for (int l = 0; l < 5; l++)
-            S->DF[l][cc] = S->DF[l][cc] + bscale(l, q1(S, l, kq, cc), Dq, S->rho, S->G, S->M) * Ji;
+            S->DF[l][cc] = S->DF[l][cc] + bscale(l, q1(S, l, kq, cc, g), Dq, rho, G, M) * Ji;
         }
         for (int s = 1; s <= PI_I; s++) {
           size_t jp = cc + (size_t)s * nx, jm = cc - (size_t)s * nx;
           for (int l = 0; l < 5; l++)
             S->DF[l][cc] =
-                S->DF[l][cc] + bscale(l, rc(S, l, jp, jm, cc), DIc(s), S->rho, S->G, S->M) * Ji;
+                S->DF[l][cc] + bscale(l, rc(S, l, jp, jm, cc, g), DIc(s), rho, G, M) * Ji;
         }
       } else {
         for (int s = 0; s < NBND; s++) {
@@ -1346,13 +1402,13 @@ static void rates_boundary(const sten_t* S, int mx, int px, int my, int py, int 
             Dr = DRc(s, by1 - j);
           }
           for (int l = 0; l < 5; l++)
-            S->DF[l][cc] = S->DF[l][cc] + bscale(l, r1(S, l, kr, cc), Dr, S->rho, S->G, S->M) * Ji;
+            S->DF[l][cc] = S->DF[l][cc] + bscale(l, r1(S, l, kr, cc, g), Dr, rho, G, M) * Ji;
         }
         for (int s = 1; s <= PI_I; s++) {
           size_t ip = cc + s, im = cc - s;
           for (int l = 0; l < 5; l++)
             S->DF[l][cc] =
-                S->DF[l][cc] + bscale(l, qc(S, l, ip, im, cc), DIc(s), S->rho, S->G, S->M) * Ji;
+                S->DF[l][cc] + bscale(l, qc(S, l, ip, im, cc, g), DIc(s), rho, G, M) * Ji;
         }
       }
     }
@@ -1375,6 +1431,10 @@ static void set_rates(orc_ctx* c, int ib) {
   S.G = b->G;
   S.M = b->M;
   S.gamma = b->gamma;
+  S.prho = c->mrho;
+  S.pG = c->mG;
+  S.pM = c->mM;
+  S.pgam = c->mgam;
   S.nx = c->nx;
   int mgx = b->mx, pgx = b->px, mgy = b->my, pgy = b->py;
   int mix = mgx + NBST, pix = pgx - NBST, miy = mgy + NBST, piy = pgy - NBST;
@@ -1387,14 +1447,14 @@ static void set_rates(orc_ctx* c, int ib) {
   rates_boundary(&S, pix + 1, pgx, mgy, miy - 1, 2, 1, mgx, pgx, mgy, pgy);     /* RB */
   rates_boundary(&S, mgx, mix - 1, piy + 1, pgy, 1, 2, mgx, pgx, mgy, pgy);     /* LT */
   rates_boundary(&S, pix + 1, pgx, piy + 1, pgy, 2, 2, mgx, pgx, mgy, pgy);     /* RT */
-  /* rates_szz, rates.f90:920-936 */
+  /* rates_szz, rates.f90:920-936 (rates_h_szz, rates_heterogeneous.f90:901-917) */
   double* DF3 = DFL(2);
   double* DF5 = DFL(4);
   double* DF6 = DFL(5);
   for (int j = mgy; j <= pgy; j++)
     for (int i = mgx; i <= pgx; i++) {
       size_t q = IDX(i, j);
-      DF6[q] = b->nu * (DF3[q] + DF5[q]);
+      DF6[q] = (c->mnu ? c->mnu[q] : b->nu) * (DF3[q] + DF5[q]);
     }
 }
 
@@ -1525,7 +1585,10 @@ static void update_fields_plastic(orc_ctx* c, double dt) {
           s[k] = FL(2 + k)[q];
           Ds[k] = DFL(2 + k)[q];
         }
-        orc_plastic_point(b->G, b->K, b->mu, b->beta, b->b, b->h, b->eta, b->s0, dt, s, Ds,
+        /* heterogeneous medium: G and K of the point (plastic.f90:37-43) */
+        double Gel = c->mG ? c->mG[q] : b->G;
+        double Kel = c->mG ? c->mM[q] - 1.333333333333333 * c->mG[q] : b->K;
+        orc_plastic_point(Gel, Kel, b->mu, b->beta, b->b, b->h, b->eta, b->s0, dt, s, Ds,
                           &c->lambda[q], &c->gammap[q]);
         for (int k = 0; k < 4; k++) {
           FL(2 + k)[q] = s[k];

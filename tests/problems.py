@@ -57,9 +57,12 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
           coupling="friction", law="RFL", bcs=None, precision="fp64", background=None,
           perturb=1e-3, source=False, plastic=None, nbx=1, rk_order=4, outside_frac=0.0,
           fault_overstress=25.0, ld_T=0.0, reg=False, rup_field="V", Vinit=1e-12,
-          skempton=0.0, mms=False):
+          skempton=0.0, mms=False, medium=None):
     """Blocks stacked in y (len(nys) rows) and nbx columns; interfaces
-    between them.  Returns (Problem, F0 fields [6,ny,nx])."""
+    between them.  medium: amplitude of a smooth seeded heterogeneous medium
+    (rho, cs, cp varying by that relative amount around the given values;
+    the edge coefficients then use the medium's values at each node).
+    Returns (Problem, F0 fields [6,ny,nx])."""
     rng = np.random.default_rng(seed)
     nyb = list(nys)
     ny = sum(nyb)
@@ -108,6 +111,11 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
     prob.Ji = 1.0 / J
     prob.x, prob.y = X, Y
 
+    if medium:
+        def smooth(k):
+            ph = rng.uniform(0, 2 * np.pi, 2)
+            return 1.0 + medium * np.sin((2 + k) * np.pi * X / Lx + ph[0]) * np.cos((1 + k) * np.pi * Y / Lx + ph[1])
+        prob.medium = (rho * smooth(0), cs * smooth(1), cp * smooth(2))
     bg = np.zeros(6) if background is None else np.asarray(background, dtype=np.float64)
     G = rho * cs ** 2
     gamma = 1.0 - 2.0 * (cs / cp) ** 2
@@ -139,11 +147,16 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
                 n = len(n1)
                 Jv = J[idx]
                 M = np.empty((5, n))
-                M[0] = rho * cs
-                M[1] = rho * cp
-                M[2] = 1.0 - 2.0 * (cs / cp) ** 2
-                M[3] = H00I * cs * nrm / Jv
-                M[4] = H00I * cp * nrm / Jv
+                if prob.medium is not None:
+                    # init_edges with the medium at each node (domain.f90:1379-1456)
+                    r_, s_, p_ = (a[idx] for a in prob.medium)
+                else:
+                    r_, s_, p_ = rho, cs, cp
+                M[0] = r_ * s_
+                M[1] = r_ * p_
+                M[2] = 1.0 - 2.0 * (s_ / p_) ** 2
+                M[3] = H00I * s_ * nrm / Jv
+                M[4] = H00I * p_ * nrm / Jv
                 F0 = np.tile(bg[:, None], (1, n))
                 outer = ((s == 0 and bx == 0) or (s == 1 and bx == nbx - 1) or
                          (s == 2 and by == 0) or (s == 3 and by == len(nyb) - 1))

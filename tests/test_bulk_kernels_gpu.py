@@ -46,6 +46,9 @@ CASES = {
     # 2 x 2 locked blocks: block columns inside a tile row
     "locked2x2": dict(nx=90, nys=(45, 52), nbx=2, seed=29, warp=0.05, background=BG,
                       perturb=1e-2, coupling="locked"),
+    # a heterogeneous medium (64 x 4 tiles whatever FDM_TILE asks)
+    "medium": dict(nx=203, nys=(61, 77), seed=37, warp=0.04, background=BG, perturb=1e-3,
+                   law="RFL", medium=0.3),
     # plasticity and a point source: the full finishing path
     "plastic_source": dict(nx=130, nys=(40, 44), seed=31, warp=0.02,
                            background=[0.0, 0.0, -60.0, 44.0, -100.0, -80.0], perturb=0.0,

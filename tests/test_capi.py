@@ -59,7 +59,8 @@ def test_descriptor_layout_matches_header():
         "fdm_source_place": ["block", "i0", "j0", "dx", "dy"],
         "fdm_source": ["nplace", "stf", "t_center", "Fx", "Myy", "place"],
         "fdm_problem": ["nx", "ny", "nblocks", "nifaces", "precision", "nstage", "rkA", "rkB",
-                        "rkC", "xq", "Ji", "x", "y", "blocks", "ifaces", "source"],
+                        "rkC", "xq", "Ji", "x", "y", "blocks", "ifaces", "source", "mat_rho",
+                        "mat_cs", "mat_cp"],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fl in fields.items():

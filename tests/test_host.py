@@ -76,3 +76,38 @@ def test_unsupported_options_fail_loudly(tmp_path, text, msg):
     p = inputs.write(str(tmp_path / "bad.in"), text)
     with pytest.raises(RuntimeError, match=msg):
         host.HostProblem(p)
+
+
+def test_heterogeneous_medium_file(tmp_path):
+    """heterogeneous = T with a medium file (material.f90:80-81, 226-321):
+    the whole grid's rho, cs, cp reach the problem description, the edge
+    coefficients take the medium's values at each node (domain.f90:1379-
+    1456), and dt follows the largest wave speeds (domain.f90:1498-1519)"""
+    nx, ny, h = 41, 21, 0.025
+    rng = np.random.default_rng(2)
+    gny = 2 * ny
+    rho = 2.67 * (1 + 0.1 * rng.random((gny, nx)))
+    cs = 3.464 * (1 + 0.1 * rng.random((gny, nx)))
+    cp = 6.0 * (1 + 0.1 * rng.random((gny, nx)))
+    med = inputs.write_medium(str(tmp_path / "medium.bin"), rho, cs, cp)
+    p = inputs.write(str(tmp_path / "h.in"),
+                     inputs.c2_planar_rs(name="h", nx=nx, ny=ny, nsteps=5, medium=med))
+    hp = host.HostProblem(p)
+    P = hp.ptr.contents
+    for name, a in (("mat_rho", rho), ("mat_cs", cs), ("mat_cp", cp)):
+        got = np.ctypeslib.as_array(getattr(P, name), shape=(gny, nx))
+        np.testing.assert_array_equal(got, a)
+    assert math.isclose(hp.dt, 0.5 * h / math.sqrt(cs.max() ** 2 + cp.max() ** 2), rel_tol=1e-13)
+    # block 1's top side (j = ny - 1): the medium's impedances and penalties
+    b1 = P.blocks[0]
+    n = b1.side[3].n
+    M = np.ctypeslib.as_array(b1.side[3].M, shape=(5, n))
+    j = ny - 1
+    np.testing.assert_allclose(M[0], rho[j] * cs[j], rtol=1e-15)
+    np.testing.assert_allclose(M[1], rho[j] * cp[j], rtol=1e-15)
+    np.testing.assert_allclose(M[2], 1 - 2 * (cs[j] / cp[j]) ** 2, rtol=1e-15)
+    np.testing.assert_allclose(M[3], H00I * cs[j] / h, rtol=1e-12)
+    # a homogeneous problem hands over no medium
+    q = inputs.write(str(tmp_path / "c.in"), inputs.c2_planar_rs(name="c", nx=nx, ny=ny, nsteps=5))
+    Q = host.HostProblem(q).ptr.contents
+    assert not Q.mat_rho and not Q.mat_cs and not Q.mat_cp

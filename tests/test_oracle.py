@@ -368,6 +368,66 @@ def test_stage1_rates_match_dense_split_operator(seed, warp):
     assert err.max() < 1e-12, err.max()
 
 
+@pytest.mark.parametrize("seed,warp", [(5, 0.0), (6, 0.08)])
+def test_stage1_rates_heterogeneous_match_dense_split_operator(seed, warp):
+    """rates_h_*_mode2 (rates_heterogeneous.f90:375-917): the material of
+    each point scales the same split-form operator, and gamma multiplies the
+    terms of the coupled velocity inside it"""
+    cs, cp, rho = 3.464, 6.0, 2.67
+    prob, F = problems.build(nx=13, nys=(15,), seed=seed, warp=warp, cs=cs, cp=cp, rho=rho,
+                             background=[0.1, -0.2, -120.0, 70.0, -110.0, -100.0], perturb=0.5,
+                             bcs=("absorbing", "free", "absorbing", "free"), medium=0.3)
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.stage1(0.0, 1e-3, 1)
+    _, DF = o.get_fields()
+    r, s_, p_ = prob.medium
+    assert np.ptp(r) > 0.1 * rho and np.ptp(s_) > 0.1 * cs
+    G = r * s_ ** 2
+    M = r * p_ ** 2
+    gamma = 1 - 2 * (s_ / p_) ** 2
+    nu = 0.5 * (M - 2 * G) / (M - G)
+    dx, dy = split_ops(prob)
+    vx, vy, sxx, sxy, syy = F[:5]
+    ref = np.empty_like(DF)
+    ref[0] = (dx(sxx) + dy(sxy)) / r
+    ref[1] = (dx(sxy) + dy(syy)) / r
+    ref[2] = M * (dx(vx) + gamma * dy(vy))
+    ref[3] = G * (dx(vy) + dy(vx))
+    ref[4] = M * (gamma * dx(vx) + dy(vy))
+    ref[5] = nu * (ref[2] + ref[4])
+    inner = (slice(None), slice(1, -1), slice(1, -1))
+    scale = np.abs(ref[inner]).max(axis=(1, 2), keepdims=True)
+    err = np.abs(DF[inner] - ref[inner]) / scale
+    assert err.max() < 1e-12, err.max()
+
+
+def test_uniform_medium_matches_homogeneous_material():
+    """a medium file holding the block's own constants: the heterogeneous
+    path differs from the homogeneous one only in rounding order"""
+    cs, cp, rho = 3.464, 6.0, 2.67
+    kw = dict(nx=40, nys=(20, 24), seed=9, warp=0.03, cs=cs, cp=cp, rho=rho,
+              background=[0.0, 0.0, -120.0, 70.0, -120.0, -120.0], perturb=1e-2, law="RFL",
+              plastic=dict(mu=0.6, beta=0.0, b=0.0, h=0.0, eta=0.0, s0=[0.0] * 6),
+              fault_overstress=15.0)
+    out = []
+    for uniform in (False, True):
+        prob, F = problems.build(**kw)
+        if uniform:
+            prob.medium = (np.full((prob.ny, prob.nx), rho), np.full((prob.ny, prob.nx), cs),
+                           np.full((prob.ny, prob.nx), cp))
+        o = OracleContext(prob)
+        o.set_fields(F)
+        o.init_edges(0.0)
+        o.step(0.0, 1e-3, 1, 20)
+        out.append((o.get_fields()[0], o.get_iface(0, "V"), o.get_plastic()[0]))
+    a, b = out
+    assert np.abs(a[0] - b[0]).max() / np.abs(a[0]).max() < 1e-13
+    assert np.abs(a[1] - b[1]).max() / np.abs(a[1]).max() < 1e-12
+    assert np.abs(a[2] - b[2]).max() <= 1e-12 * max(1.0, np.abs(a[2]).max())
+
+
 def energy(prob, F, rho, cs, cp):
     _, _, HL, _ = ob.sbp4_tables()
 

@@ -125,6 +125,39 @@ def test_plasticity(gpu):
     assert go.max() > 0.0  # plastic flow actually occurred
 
 
+def test_heterogeneous_medium_bulk_bit_exact(gpu):
+    """rates_h_*_mode2 (rates_heterogeneous.f90:375-917): a smooth medium
+    varying by 30 %, curvilinear, with a point source; interior tiles, band
+    and closures, bit for bit"""
+    prob, F = problems.build(nx=60, nys=(44,), seed=23, warp=0.05, background=BG, perturb=1e-2,
+                             source=True, bcs=("absorbing", "free", "rigid", "free"), medium=0.3)
+    dev, orc = run_both(prob, F, 25)
+    compare(dev, orc, 0, bit_exact=True)
+
+
+def test_heterogeneous_medium_locked_2x2_bit_exact(gpu):
+    prob, F = problems.build(nx=36, nys=(30, 34), nbx=2, seed=29, warp=0.04, background=BG,
+                             perturb=1e-2, coupling="locked", medium=0.25)
+    dev, orc = run_both(prob, F, 20)
+    compare(dev, orc, 0, bit_exact=True)
+
+
+def test_heterogeneous_medium_friction_and_plasticity(gpu):
+    """a fault across a material contrast, with plastic blocks: plasticity
+    with the point's G and K (plastic.f90:37-43)"""
+    pl = dict(mu=0.6, beta=0.0, b=0.0, h=0.0, eta=0.0, s0=[0.0] * 6)
+    bg = [0.0, 0.0, -60.0, 44.0, -100.0, -80.0]
+    prob, F = problems.build(nx=48, nys=(24, 28), seed=31, warp=0.02, background=bg,
+                             perturb=0.0, law="RFL", plastic=pl, fault_overstress=20.0,
+                             medium=0.2)
+    dev, orc = run_both(prob, F, 30)
+    compare(dev, orc, 1, bit_exact=False)
+    gd, ld = dev.get_plastic()
+    go, lo = orc.get_plastic()
+    assert rel_l2(gd, go) <= 1e-12 and rel_l2(ld, lo) <= 1e-12
+    assert go.max() > 0.0
+
+
 def test_fp32_bulk_error_against_fp64(gpu):
     """FP32-bulk (fields FP32 as deviations from a per-block background,
     friction FP64) against the FP64 oracle on a prestressed rupture: the
