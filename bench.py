@@ -154,6 +154,67 @@ def run_c3(args, tmp):
     print(json.dumps(out), flush=True)
 
 
+def run_c2het(args, tmp):
+    """C2 in a heterogeneous medium: the same two blocks and fault, with rho,
+    cs and cp varying smoothly by 10 % over the grid (seeded, read from a
+    medium file).  The interior kernel reads five more planes per point
+    (rho, G, M, gamma, nu): 40 B more per point in FP64, 20 B in FP32."""
+    from fdmap_github_amd import inputs
+
+    nx, ny = args.nx, args.ny
+    rho, cs, cp = inputs.smooth_medium(nx, 2 * ny, 0.025, amp=0.1, seed=5)
+    med = inputs.write_medium(os.path.join(tmp, "medium.bin"), rho, cs, cp)
+    inpath = inputs.write(os.path.join(tmp, "c2het.in"),
+                          inputs.c2_planar_rs(name=os.path.join(tmp, "c2het"), nx=nx, ny=ny,
+                                              nsteps=args.steps + args.warmup + args.prof_steps,
+                                              outputs=False, medium=med))
+    res = {}
+    for prec in ("fp64",) if args.no_fp32 else ("fp64", "fp32"):
+        r = run_device(inpath, prec, args.steps, args.warmup, args.prof_steps)
+        ns, prof = r["nstage"], r["prof"]
+        extra = 40.0 if prec == "fp64" else 20.0
+        b1, b2 = (b + extra for b in BYTES_STAGE[prec])
+        kpts = prof["bulk_points"]
+        bytes_launch = kpts * (b1 + (ns - 1) * b2) / ns
+        avg_ms = prof["bulk_ms"] / max(1, prof["bulk_launches"])
+        achieved = bytes_launch / (avg_ms * 1e-3) / 1e9
+        res[prec] = dict(r=r, ups=r["npts"] * args.steps / r["el"], avg_ms=avg_ms,
+                         achieved=achieved, bytes_launch=bytes_launch, kpts=kpts)
+    r64 = res["fp64"]
+    out = {
+        "metric": METRIC, "value": r64["ups"], "unit": "updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * r64["r"]["el"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (planar fault problem and seeded medium file)",
+        "config": {"workload": f"planar rate-and-state rupture in a heterogeneous medium, 2 blocks "
+                               f"x ({nx}x{ny}), rho/cs/cp varying by 10 % (seed 5), RFL slip law, "
+                               "LSRK(5,4) CFL 0.5",
+                   "grid_points": r64["r"]["npts"], "fault_nodes": nx,
+                   "dt": r64["r"]["hp"].dt},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "bulk_interior<double,64,4,false,NT,true> (heterogeneous medium)",
+            "kernel_points": r64["kpts"], "achieved": r64["achieved"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS, "traffic": None,
+            "alg_bytes_per_launch": r64["bytes_launch"], "avg_launch_ms": r64["avg_ms"],
+        },
+    }
+    if "fp32" in res:
+        r32 = res["fp32"]
+        out["fp32_bulk"] = {
+            "updates_per_s": r32["ups"], "speedup_vs_fp64": r32["ups"] / r64["ups"],
+            "ms_per_step": 1e3 * r32["r"]["el"] / args.steps,
+            "bulk_avg_launch_ms": r32["avg_ms"], "bulk_achieved_GBs": r32["achieved"],
+            "rel_l2_fields_vs_fp64": rel_l2(r32["r"]["F"], r64["r"]["F"]),
+            "rel_l2_slip_vs_fp64": rel_l2(r32["r"]["fault"]["D"], r64["r"]["fault"]["D"]),
+            "rel_l2_slip_velocity_vs_fp64": rel_l2(r32["r"]["fault"]["V"],
+                                                   r64["r"]["fault"]["V"]),
+            "after_steps": args.steps + args.warmup,
+        }
+    print(json.dumps(out), flush=True)
+
+
 def run_c4(args, tmp):
     """Ensemble: scenarios with block sizes uniform in 401..2001 x 201..1001,
     lognormal fault shear prestress (mean 70 MPa, log sd 0.1, correlation
@@ -371,8 +432,9 @@ def main():
                     help="steps timed after the headline ones with HIP events per launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--config", choices=("c2", "c3", "c4", "c5"), default="c2",
-                    help="c2: planar rupture (the headline); c3: rough curvilinear fault with "
+    ap.add_argument("--config", choices=("c2", "c2het", "c3", "c4", "c5"), default="c2",
+                    help="c2: planar rupture (the headline); c2het: the same in a "
+                         "heterogeneous medium; c3: rough curvilinear fault with "
                          "off-fault plasticity, 2 x (4001x2001), FP64 and FP32-bulk; c4: 64 "
                          "uneven rupture scenarios advanced as one graph launch per step; c5: "
                          "adjoint gradient of a receiver misfit, 2 x (801x401), 2000 steps")
@@ -387,6 +449,9 @@ def main():
     from fdmap_github_amd import inputs
 
     tmp = tempfile.mkdtemp(prefix="fdmap_bench_")
+    if args.config == "c2het":
+        run_c2het(args, tmp)
+        return
     if args.config == "c3":
         run_c3(args, tmp)
         return

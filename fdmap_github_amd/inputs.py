@@ -183,6 +183,30 @@ def write_medium(path, rho, cs, cp):
     return path
 
 
+def smooth_medium(nx, ny, h, amp=0.1, seed=0, nmodes=12):
+    """Seeded smooth medium on the whole grid [ny, nx] of spacing h: rho, cs
+    and cp times (1 + amp * g), g a sum of nmodes random plane waves with
+    wavelengths between 1/8 and the grid's height, scaled to max |g| = 1,
+    plus an independent 1/4-amplitude field for cp so the speed ratio varies
+    too.  Returns rho, cs, cp."""
+    rng = np.random.default_rng(seed)
+    Ly = (ny - 1) * h
+    y, x = np.meshgrid(np.arange(ny) * h, np.arange(nx) * h, indexing="ij")
+
+    def field():
+        g = np.zeros((ny, nx))
+        for _ in range(nmodes):
+            lam = Ly * rng.uniform(0.125, 1.0)
+            th = rng.uniform(0, 2 * np.pi)
+            k = 2 * np.pi / lam
+            g += np.cos(k * (np.cos(th) * x + np.sin(th) * y) + rng.uniform(0, 2 * np.pi))
+        return g / np.abs(g).max()
+
+    g, g2 = field(), field()
+    return (RHO * (1 + 0.5 * amp * g), CS * (1 + amp * g),
+            CP * (1 + amp * g + 0.25 * amp * g2))
+
+
 def write(path, text):
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     with open(path, "w") as f:

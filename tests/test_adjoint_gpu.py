@@ -167,6 +167,9 @@ def test_unsupported_problems_are_refused(gpu):
     prob, F = problems.build(nx=40, nys=(20, 20), seed=1, law="SW")
     with pytest.raises(FdmError, match="RSL, RFL"):
         FaultInversion(Context(prob), 0.0, DT, [3], [39], 10)
+    prob, F = problems.build(nx=40, nys=(20, 20), seed=1, medium=0.2)
+    with pytest.raises(FdmError, match="heterogeneous"):
+        FaultInversion(Context(prob), 0.0, DT, [3], [39], 10)
     prob, F = problems.build(nx=40, nys=(20, 20), seed=1)
     with pytest.raises(FdmError, match="receiver"):
         FaultInversion(Context(prob), 0.0, DT, [3], [40], 10)

@@ -118,3 +118,6 @@ def test_members_of_mixed_precision_are_refused(gpu):
     b = Context(problems.build(nx=40, nys=(20, 20), seed=1, precision="fp32")[0])
     with pytest.raises(FdmError, match="precision"):
         ensemble.Ensemble([a, b])
+    c = Context(problems.build(nx=40, nys=(20, 20), seed=1, medium=0.2)[0])
+    with pytest.raises(FdmError, match="heterogeneous"):
+        ensemble.Ensemble([a, c])
