@@ -159,6 +159,7 @@ ENTRY_POINTS = [
     ("fdm_set_iface", C.c_int, [_P, C.c_int32, C.c_int32, _dp]),
     ("fdm_get_iface_stage1", C.c_int, [_P, C.c_int32, C.c_int32, _dp]),
     ("fdm_get_plastic", C.c_int, [_P, _dp, _dp]),
+    ("fdm_set_plastic", C.c_int, [_P, _dp, _dp]),
     ("fdm_max_slip_velocity", C.c_int, [_P, _dp]),
     ("fdm_sync", C.c_int, [_P]),
     ("fdm_profile_enable", C.c_int, [_P, C.c_int32]),

@@ -1362,6 +1362,16 @@ int fdm_get_plastic(fdm_ctx* c, double* gammap, double* lambda) {
   return FDM_OK;
 }
 
+int fdm_set_plastic(fdm_ctx* c, const double* gammap, const double* lambda) {
+  if (!c) return FDM_EINVAL;
+  if (!c->any_plastic) return set_err(c, FDM_EINVAL, "fdm_set_plastic: no block is plastic");
+  size_t np = (size_t)c->nx * c->ny;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (gammap) HIPCHK(c, hipMemcpy(c->gammap, gammap, np * 8, hipMemcpyHostToDevice));
+  if (lambda) HIPCHK(c, hipMemcpy(c->lambda, lambda, np * 8, hipMemcpyHostToDevice));
+  return FDM_OK;
+}
+
 int fdm_max_slip_velocity(fdm_ctx* c, double* vmax) {
   if (!c || !vmax) return FDM_EINVAL;
   double v = 0.0;

@@ -104,6 +104,12 @@ class Context:
         self._check(self._fn("get_plastic")(self.h, capi.dptr(g), capi.dptr(lam)))
         return g, lam
 
+    def set_plastic(self, gammap, lam):
+        g = np.ascontiguousarray(gammap, dtype=np.float64)
+        l = np.ascontiguousarray(lam, dtype=np.float64)
+        assert g.shape == l.shape == (self.ny, self.nx)
+        self._check(self._fn("set_plastic")(self.h, capi.dptr(g), capi.dptr(l)))
+
     # ---- device-only helpers ----
     def sync(self):
         self._check(self._fn("sync")(self.h))

@@ -158,6 +158,19 @@ module fdmap_capi
        integer(c_int) :: rc
      end function fdm_get_plastic
 
+     function fdm_set_plastic(ctx,gammap,lambda) bind(C,name='fdm_set_plastic') result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr),value :: ctx, gammap, lambda
+       integer(c_int) :: rc
+     end function fdm_set_plastic
+
+     function fdm_set_iface(ctx,iface,field,in) bind(C,name='fdm_set_iface') result(rc)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr),value :: ctx, in
+       integer(c_int32_t),value :: iface, field
+       integer(c_int) :: rc
+     end function fdm_set_iface
+
      function fdm_max_slip_velocity(ctx,vmax) bind(C,name='fdm_max_slip_velocity') result(rc)
        import :: c_ptr, c_int, c_double
        type(c_ptr),value :: ctx

@@ -14,6 +14,7 @@ module fdmap_driver
   use fdmap_capi
   use fdmap_grid, only : dp
   use fdmap_setup
+  use fdmap_checkpoint
   implicit none
 
   type :: out_item
@@ -157,15 +158,34 @@ contains
   end subroutine read_output
 
 
-  subroutine open_outputs()
+  subroutine open_outputs(begin)
     ! <name>_<item>.dat (FP32 data) and <name>_<item>.t (FP32 times)
-    ! (output.f90:916-987)
-    integer :: k
+    ! (output.f90:916-987).  A run resumed after step begin keeps the files
+    ! and writes from the record of that step on (or the next recorded one)
+    integer,intent(in) :: begin
+    integer :: k,noffset
+    integer(8) :: off
+    character(16) :: st
     do k = 1,nitems
+       st = 'replace'
+       noffset = 0
+       if (begin>0) then
+          st = 'unknown'
+          if (begin>=items(k)%nmin) then
+             noffset = (begin-items(k)%nmin)/items(k)%st
+             if (mod(begin-items(k)%nmin,items(k)%st)/=0) noffset = noffset+1
+          end if
+          if (noffset>items(k)%nmax) noffset = 0
+       end if
        open(newunit=items(k)%ufile,file=trim(pb_name)//'_'//trim(items(k)%name)//'.dat', &
-            access='stream',form='unformatted',status='replace',action='write')
+            access='stream',form='unformatted',status=trim(st),action='write')
        open(newunit=items(k)%utime,file=trim(pb_name)//'_'//trim(items(k)%name)//'.t', &
-            access='stream',form='unformatted',status='replace',action='write')
+            access='stream',form='unformatted',status=trim(st),action='write')
+       if (begin>0) then
+          off = int(noffset,8)*int(items(k)%nx,8)*int(items(k)%ny,8)*4_8
+          write(items(k)%ufile,pos=off+1_8)
+          write(items(k)%utime,pos=int(noffset,8)*4_8+1_8)
+       end if
     end do
   end subroutine open_outputs
 
@@ -353,14 +373,15 @@ contains
     character(*),intent(in) :: infile
     integer(c_int32_t),intent(in) :: precision
     integer :: rc
-    logical :: ok,final_step,due,slipping,need_body,need_plastic
-    integer :: n,k,nnext,nrun,stat
+    logical :: ok,final_step,due,slipping,need_body,need_plastic,abort
+    integer :: n,k,nnext,nrun,stat,nstart
     integer(c_int64_t) :: n64
-    real(dp) :: t,wstart,w1,w2,wave,vmax
+    real(dp) :: t,wstart,w1,w2,wave,vmax,wprog
     real(dp),allocatable,target :: F(:,:,:),gp(:,:),lam(:,:)
     character(512) :: str
 
     rc = 0
+    wprog = walltime()
     call message('Reading input file '//trim(adjustl(infile)))
     call message('Number of processors = 1')
     call setup_problem(infile,precision,ok)
@@ -371,6 +392,7 @@ contains
     end if
     call message('Starting problem: '//trim(adjustl(pb_name)))
     call read_output(infile)
+    if (setup_error=='') call init_checkpoint(infile)
     if (setup_error/='') then
        call message('Error: '//trim(setup_error))
        rc = 1
@@ -389,8 +411,21 @@ contains
        rc = 1
        return
     end if
+    ! resume: fields, rates, plastic strain and fault state of step begin
+    ! (main.f90:150-154)
+    call ckpt_read(dev_ctx,ok)
+    if (.not.ok) then
+       if (setup_error/='') then
+          call message('Error: '//trim(setup_error))
+       else
+          call message('Error: '//fdm_error_message(dev_ctx))
+       end if
+       rc = 1
+       return
+    end if
+    nstart = ck%begin+1
     call write_echo()
-    call open_outputs()
+    call open_outputs(ck%begin)
     allocate(F(gnx,gny,6),gp(gnx,gny),lam(gnx,gny))
     gp = 0.0_dp
     lam = 0.0_dp
@@ -398,7 +433,8 @@ contains
 
     wstart = walltime()
     wave = 0.0_dp
-    n = 1
+    abort = .false.
+    n = nstart
     do while (n<=nt)
        w1 = walltime()
        t = t_init+dble(n-1)*dt
@@ -434,6 +470,10 @@ contains
                 if (item_due(items(k),nnext-1)) ok = .true.
              end do
              if (ok) exit
+             ! a checkpoint is due after step nnext-1
+             if (ck%ckpt) then
+                if (mod(nnext-1,ck%interval)==0) exit
+             end if
              nnext = nnext+1
           end do
           nrun = nnext-n
@@ -451,6 +491,20 @@ contains
           end do
        end if
        n = n+nrun
+       ! abort request and checkpoint after step n-1 (main.f90:191-198); an
+       ! abort is seen at the end of a run of steps
+       abort = abort_now(walltime()-wprog)
+       if (abort) call message('Aborting now...')
+       call ckpt_write_delete(dev_ctx,n-1,abort,ok)
+       if (.not.ok) then
+          if (setup_error/='') then
+             call message('Error: '//trim(setup_error))
+          else
+             call message('Error: '//fdm_error_message(dev_ctx))
+          end if
+          rc = 1
+          exit
+       end if
        ! status update (main.f90:202-211), for the last step of the run
        if (mod(n-2,ninfo)==0) then
           stat = fdm_sync(dev_ctx)
@@ -470,12 +524,15 @@ contains
              end if
           end if
        end if
+       if (abort) exit
        if (n-1==nt-1) then
           stat = fdm_sync(dev_ctx)
-          wave = (walltime()-wstart)/dble(max(1,nt-1))
+          wave = (walltime()-wstart)/dble(max(1,nt-nstart))
        end if
     end do
     stat = fdm_sync(dev_ctx)
+    ! the last checkpoint goes once the run has finished (main.f90:233-235)
+    if (rc==0.and.(.not.abort).and.ck%del_final) call ckpt_delete(.true.)
     call close_outputs()
     write(str,'(a,f0.8,a,i0,a,f0.4,a)') 'total: ',walltime()-wstart,' s for ',nt, &
          ' time steps (',wave,' s/step)'

@@ -45,6 +45,10 @@ def load():
     lib.fdmap_host_coords.restype = C.POINTER(C.c_double)
     lib.fdmap_host_run.argtypes = [C.c_char_p, C.c_int32]
     lib.fdmap_host_run.restype = C.c_int
+    lib.fdmap_host_ckpt_init.argtypes = [C.c_char_p]
+    lib.fdmap_host_ckpt_init.restype = C.c_int
+    lib.fdmap_host_ckpt_files.argtypes = [C.c_int32, C.c_int32] + [C.c_void_p] * 5
+    lib.fdmap_host_ckpt_files.restype = C.c_int
     _lib = lib
     return lib
 
@@ -79,6 +83,43 @@ class HostProblem:
         k = names.index(which) if isinstance(which, str) else int(which)
         p = load().fdmap_host_coords(k)
         return np.ctypeslib.as_array(p, shape=(self.ny, self.nx)).copy()
+
+
+    def checkpoint_init(self):
+        """Parse checkpoint_list and the number file <name>.ckpt as the
+        driver does at start; returns the step a run resumes after (0 for a
+        fresh start)."""
+        lib = load()
+        begin = lib.fdmap_host_ckpt_init(self.path.encode())
+        if begin < 0:
+            raise RuntimeError(f"fdmap host: {lib.fdmap_host_error().decode()}")
+        return begin
+
+    def fric_nodes(self):
+        P = self.ptr.contents
+        return sum(P.ifaces[k].p - P.ifaces[k].m + 1 for k in range(P.nifaces)
+                   if P.ifaces[k].coupling == capi.COUPLING["friction"])
+
+    def checkpoint_files(self, op, number, F=None, DF=None, gammap=None, lam=None, IF=None):
+        """op 'write' | 'read' | 'delete' on checkpoint number with host
+        arrays F, DF [6, ny, nx], gammap, lam [ny, nx], IF [18, nodes of the
+        frictional interfaces]; 'read' returns them."""
+        lib = load()
+        code = {"write": 0, "read": 1, "delete": 2}[op]
+        nn = max(1, self.fric_nodes())
+        shp = {"F": (6, self.ny, self.nx), "DF": (6, self.ny, self.nx),
+               "gammap": (self.ny, self.nx), "lam": (self.ny, self.nx),
+               "IF": (len(capi.IFACE_FIELDS), nn)}
+        arrs = {}
+        for name, v in (("F", F), ("DF", DF), ("gammap", gammap), ("lam", lam), ("IF", IF)):
+            if op == "read" or v is None:
+                arrs[name] = np.zeros(shp[name])
+            else:
+                arrs[name] = np.ascontiguousarray(v, dtype=np.float64).reshape(shp[name])
+        ptrs = [arrs[k].ctypes.data for k in ("F", "DF", "gammap", "lam", "IF")]
+        if lib.fdmap_host_ckpt_files(code, int(number), *ptrs) != 0:
+            raise RuntimeError(f"fdmap host: {lib.fdmap_host_error().decode()}")
+        return arrs
 
 
 def run(path: str, precision: str = "fp64") -> int:

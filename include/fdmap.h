@@ -247,6 +247,11 @@ int fdm_get_iface_stage1(fdm_ctx* ctx, int32_t iface, int32_t field, double* out
    (fields.f90:194-196). */
 int fdm_get_plastic(fdm_ctx* ctx, double* gammap, double* lambda);
 
+/* Restore gammap and lambda, [ny][nx], from a checkpoint
+   (checkpoint_fields 'read', fields.f90:380-381); either may be NULL.
+   FDM_EINVAL when no block is plastic. */
+int fdm_set_plastic(fdm_ctx* ctx, const double* gammap, const double* lambda);
+
 /* max over all frictional interfaces of V (check_nucleation,
    domain.f90:474-499, interfaces.f90:506-517). */
 int fdm_max_slip_velocity(fdm_ctx* ctx, double* vmax);
