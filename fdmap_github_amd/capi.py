@@ -88,6 +88,7 @@ class fdm_block(C.Structure):
         ("h", C.c_double), ("eta", C.c_double), ("K", C.c_double),
         ("s0", C.c_double * 6),
         ("side", fdm_side * 4),
+        ("Cdiss", C.c_double),
     ]
 
 
@@ -132,6 +133,7 @@ class fdm_problem(C.Structure):
         ("ifaces", C.POINTER(fdm_iface)),
         ("source", fdm_source),
         ("mat_rho", _dp), ("mat_cs", _dp), ("mat_cp", _dp),
+        ("J", _dp),
     ]
 
 

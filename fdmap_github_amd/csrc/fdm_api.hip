@@ -230,6 +230,9 @@ struct fdm_ctx {
   // heterogeneous medium: rho, G, M, gamma, nu planes (field precision)
   bool het = false;
   void* mat[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // artificial dissipation in some block: the Jacobian plane
+  bool diss = false;
+  void* Jp = nullptr;
   std::vector<double> hnu;  // nu of the medium ([ny][nx]), for restore_df6
   // bulk kernel: 0 per-point, 2 interior tiles (kTileShape[tile_shape]) + band points
   int bulk_mode = 2;
@@ -478,6 +481,7 @@ BulkArgs<T> bulk_args(fdm_ctx* c, double A, double Bdt, int do_update, double t,
   a.mM = (const T*)c->mat[2];
   a.mgam = (const T*)c->mat[3];
   a.mnu = (const T*)c->mat[4];
+  a.J = (const T*)c->Jp;
   return a;
 }
 
@@ -509,19 +513,26 @@ void launch_interior(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   c->df6_stale = !full;
   dim3 grid(c->ntiles[q]);
   dim3 block(kTileShape[q][0], kTileShape[q][1]);
-#define FDM_INTERIOR(TX_, TY_, HET_)                                                                 \
+#define FDM_INTERIOR(TX_, TY_, HET_, DISS_)                                                          \
   do {                                                                                               \
-    if (full) hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, true, false, HET_>), grid, block, 0, s, a, c->tiles[q]); \
+    if (full)                                                                                        \
+      hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, true, false, HET_, DISS_>), grid, block, 0, s, a, \
+                         c->tiles[q]);                                                               \
     else if (c->nt_store)                                                                            \
-      hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, false, true, HET_>), grid, block, 0, s, a, c->tiles[q]); \
-    else hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, false, false, HET_>), grid, block, 0, s, a, c->tiles[q]); \
+      hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, false, true, HET_, DISS_>), grid, block, 0, s, a, \
+                         c->tiles[q]);                                                               \
+    else                                                                                             \
+      hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, false, false, HET_, DISS_>), grid, block, 0, s, a, \
+                         c->tiles[q]);                                                               \
   } while (0)
-  // a heterogeneous medium runs the 64 x 4 tiles only (fdm_create and
-  // fdm_tune keep tile_shape 0)
-  if (c->het) FDM_INTERIOR(64, 4, true);
-  else if (q == 0) FDM_INTERIOR(64, 4, false);
-  else if (q == 1) FDM_INTERIOR(64, 8, false);
-  else FDM_INTERIOR(128, 4, false);
+  // a heterogeneous medium and artificial dissipation run the 64 x 4 tiles
+  // only (fdm_create and fdm_tune keep tile_shape 0)
+  if (c->het && c->diss) FDM_INTERIOR(64, 4, true, true);
+  else if (c->het) FDM_INTERIOR(64, 4, true, false);
+  else if (c->diss) FDM_INTERIOR(64, 4, false, true);
+  else if (q == 0) FDM_INTERIOR(64, 4, false, false);
+  else if (q == 1) FDM_INTERIOR(64, 8, false, false);
+  else FDM_INTERIOR(128, 4, false, false);
 #undef FDM_INTERIOR
   prof_end(c, ep, 0, s);
 }
@@ -934,6 +945,13 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     D.eta = B.eta;
     D.K = B.K;
     for (int k = 0; k < 6; k++) D.s0[k] = B.s0[k];
+    // dissipation = (Cdiss > 0) (domain.f90:180); d(0) = 2 AI(0) Cdiss,
+    // d(s) = AI(s) Cdiss (rates.f90:1416-1421)
+    D.Cdiss = B.Cdiss > 0.0 ? B.Cdiss : 0.0;
+    D.dA[0] = 2.0 * FDM_AI0 * D.Cdiss;
+    D.dA[1] = FDM_AI1 * D.Cdiss;
+    D.dA[2] = FDM_AI2 * D.Cdiss;
+    if (D.Cdiss > 0.0) c->diss = true;
     if (B.plastic) c->any_plastic = true;
     if (B.mms && (!P->x || !P->y)) {
       set_err(c, FDM_EINVAL, "fdm_create: MMS source needs coordinates x, y");
@@ -1148,8 +1166,13 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   // (6), metrics xq xr yq yr Ji (5); a heterogeneous medium adds rho, G, M,
   // gamma, nu (5)
   c->het = P->mat_rho && P->mat_cs && P->mat_cp;
-  if (c->het) c->tile_shape = 0;
-  const int nplanes = c->het ? 28 : 23;
+  if (c->het || c->diss) c->tile_shape = 0;
+  if (c->diss && !P->J) {
+    set_err(c, FDM_EINVAL, "fdm_create: artificial dissipation needs the Jacobian J");
+    return fail(FDM_EINVAL);
+  }
+  // artificial dissipation adds the Jacobian J (1)
+  const int nplanes = (c->het ? 28 : 23) + (c->diss ? 1 : 0);
   void* arena = nullptr;
   const size_t slack = 64 * c->esz;  // the last plane extends coff elements past nplanes * ps
   if ((rc = dalloc(c, &arena, nplanes * pbytes + slack))) return fail(rc);
@@ -1181,6 +1204,18 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
         pack_plane(h, m[k].data(), c->nx, c->ny, c->ld);
         HIPCHK(c, hipMemcpy(c->mat[k], h.data(), h.size() * 4, hipMemcpyHostToDevice));
       }
+    }
+  }
+  if (c->diss) {
+    c->Jp = ab + (size_t)(c->het ? 28 : 23) * pbytes;
+    if (c->precision == FDM_FP64) {
+      std::vector<double> h;
+      pack_plane(h, P->J, c->nx, c->ny, c->ld);
+      HIPCHK(c, hipMemcpy(c->Jp, h.data(), h.size() * 8, hipMemcpyHostToDevice));
+    } else {
+      std::vector<float> h;
+      pack_plane(h, P->J, c->nx, c->ny, c->ld);
+      HIPCHK(c, hipMemcpy(c->Jp, h.data(), h.size() * 4, hipMemcpyHostToDevice));
     }
   }
   c->F[0] = ab;
@@ -1426,7 +1461,9 @@ int fdm_profile_read(fdm_ctx* c, fdm_profile_t* out) {
 int fdm_tune(fdm_ctx* c, int32_t knob, int32_t value) {
   if (!c) return FDM_EINVAL;
   if (knob == FDM_TUNE_TILE && value >= 0 && value < 3) {
-    if (c->het && value != 0) return set_err(c, FDM_EINVAL, "fdm_tune: a heterogeneous medium runs 64x4 tiles only");
+    if ((c->het || c->diss) && value != 0)
+      return set_err(c, FDM_EINVAL,
+                     "fdm_tune: a heterogeneous medium or artificial dissipation runs 64x4 tiles only");
     c->tile_shape = value;
     return FDM_OK;
   }
@@ -1667,11 +1704,12 @@ int fdm_ensemble_create(fdm_ctx* const* members, int32_t n, fdm_ensemble** out) 
     if (!c) return FDM_EINVAL;
     const fdm_ctx* c0 = members[0];
     if (c->src.nplace > 0 || c->any_mms || (!c->ld_T.empty() && c->ld_T[0] > 0.0) || c->bulk_mode != 2 ||
-        c->het || c->nstage != c0->nstage || c->precision != c0->precision || c->device != c0->device) {
+        c->het || c->diss || c->nstage != c0->nstage || c->precision != c0->precision ||
+        c->device != c0->device) {
       g_create_error =
           "fdm_ensemble_create: members must have no point source, no manufactured solution, "
-          "no load ramp, no heterogeneous medium, the tile kernels, and the same RK order, "
-          "precision and device";
+          "no load ramp, no heterogeneous medium, no artificial dissipation, the tile kernels, "
+          "and the same RK order, precision and device";
       return FDM_EINVAL;
     }
     for (int q = 0; q < k; q++)

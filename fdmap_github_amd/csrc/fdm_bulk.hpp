@@ -46,6 +46,8 @@ struct BulkArgs {
   const T* mM;
   const T* mgam;
   const T* mnu;
+  // Jacobian J, when a block has artificial dissipation (else null)
+  const T* J;
 };
 
 // Map the hardware workgroup id to a tile.  Workgroups are dispatched to the
@@ -201,14 +203,59 @@ __device__ __forceinline__ void interior_rates(T* d, const Acc& A, T gamma, cons
   }
 }
 
+// interior point with artificial dissipation (rates_dissipation_interior_
+// mode2, rates.f90:1397-1482; rates_h_dissipation_interior_mode2,
+// rates_heterogeneous.f90:1376-1458): d(0) F first, then per s
+// (a(s) X + d(s) Y) Ji, where X is the split-form sum and Y the J-weighted
+// neighbours of the field itself.  The s = 3 term is zero for SBP4 (the
+// heterogeneous loop reads DI(3) outside DI(-2:2); it is taken as zero).
+template <typename T, int l, typename Acc, typename Coef>
+__device__ __forceinline__ void interior_term_diss(T* d, const Acc& A, int s, T gamma, const Coef& cI,
+                                                   T ds, T Ji, T ayp, T axp, T aym, T axm, T byp,
+                                                   T bxp, T bym, T bxm) {
+  using P = Fld<l>;
+  T x = prod<T, P::ga>(ayp, A.f(P::ua, s, 0), gamma) - prod<T, P::gb>(axp, A.f(P::ub, s, 0), gamma) -
+        prod<T, P::ga>(aym, A.f(P::ua, -s, 0), gamma) + prod<T, P::gb>(axm, A.f(P::ub, -s, 0), gamma) -
+        prod<T, P::ga>(byp, A.f(P::ua, 0, s), gamma) + prod<T, P::gb>(bxp, A.f(P::ub, 0, s), gamma) +
+        prod<T, P::ga>(bym, A.f(P::ua, 0, -s), gamma) - prod<T, P::gb>(bxm, A.f(P::ub, 0, -s), gamma);
+  T y = A.j(s, 0) * A.f(l, s, 0) + A.j(-s, 0) * A.f(l, -s, 0) + A.j(0, s) * A.f(l, 0, s) +
+        A.j(0, -s) * A.f(l, 0, -s);
+  d[l] = d[l] + (cI.template coef<l>(s) * x + ds * y) * Ji;
+}
+
+template <typename T, typename Acc, typename Coef>
+__device__ __forceinline__ void interior_rates_diss(T* d, const Acc& A, T gamma, const Coef& cI,
+                                                    const T (&dA)[3], T Ji) {
+#pragma unroll
+  for (int l = 0; l < 5; l++) d[l] = d[l] + dA[0] * A.f(l, 0, 0);
+#pragma unroll
+  for (int s = 1; s <= 2; s++) {
+    const T ayp = (T)0.5 * (A.yr(s) + A.yr(0)), axp = (T)0.5 * (A.xr(s) + A.xr(0));
+    const T aym = (T)0.5 * (A.yr(-s) + A.yr(0)), axm = (T)0.5 * (A.xr(-s) + A.xr(0));
+    const T byp = (T)0.5 * (A.yq(s) + A.yq(0)), bxp = (T)0.5 * (A.xq(s) + A.xq(0));
+    const T bym = (T)0.5 * (A.yq(-s) + A.yq(0)), bxm = (T)0.5 * (A.xq(-s) + A.xq(0));
+    const T ds = dA[s];
+    interior_term_diss<T, 0>(d, A, s, gamma, cI, ds, Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term_diss<T, 1>(d, A, s, gamma, cI, ds, Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term_diss<T, 2>(d, A, s, gamma, cI, ds, Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term_diss<T, 3>(d, A, s, gamma, cI, ds, Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+    interior_term_diss<T, 4>(d, A, s, gamma, cI, ds, Ji, ayp, axp, aym, axm, byp, bxp, bym, bxm);
+  }
+}
+
 // interior coefficients of a block, from any address space: t DI(s)/rho Ji
-// etc. with DI(s)/rho precomputed (rates.f90:400-405, 411-444)
+// etc. with DI(s)/rho precomputed (rates.f90:400-405, 411-444); coef() is
+// a(s), b(s), c(s) itself, for the dissipative form
 template <typename T, typename BlockRef>
 struct BlockCoef {
   BlockRef B;
   template <int l>
   __device__ __forceinline__ T scale(T t, int s, T Ji) const {
     return t * (T)B.cI[l][s - 1] * Ji;
+  }
+  template <int l>
+  __device__ __forceinline__ T coef(int s) const {
+    return (T)B.cI[l][s - 1];
   }
 };
 
@@ -225,6 +272,15 @@ struct PointCoef {
     else if constexpr (l == 3) return u * G;
     else return u * M;
   }
+  // "DI(s)/rho(i,j)*(", "DI(s)*M(i,j)*(", "DI(s)*G(i,j)*(" of the
+  // dissipative form (rates_heterogeneous.f90:1409-1450)
+  template <int l>
+  __device__ __forceinline__ T coef(int s) const {
+    const T D = s == 1 ? (T)FDM_DI1 : (T)FDM_DI2;
+    if constexpr (l < 2) return D / rho;
+    else if constexpr (l == 3) return D * G;
+    else return D * M;
+  }
 };
 
 // neighbourhood read straight from global memory
@@ -237,9 +293,11 @@ struct GlobalAcc {
   const T* __restrict__ pyr;
   long long ps, cc;
   int ld;
+  const T* __restrict__ pJ;
   __device__ __forceinline__ T f(int k, int di, int dj) const {
     return pF[k * ps + cc + di + (long long)dj * ld];
   }
+  __device__ __forceinline__ T j(int di, int dj) const { return pJ[cc + di + (long long)dj * ld]; }
   __device__ __forceinline__ T xr(int di) const { return pxr[cc + di]; }
   __device__ __forceinline__ T yr(int di) const { return pyr[cc + di]; }
   __device__ __forceinline__ T xq(int dj) const { return pxq[cc + (long long)dj * ld]; }
@@ -412,12 +470,43 @@ __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDe
   for (int l = 0; l < 6; l++) a.DF[l * ps + cc] = d[l];
 }
 
+// dissipation terms of rates_dissipation_boundary_mode2 (rates.f90:1485-
+// 2021) for fields 1..5 (the szz relation overwrites the sixth):
+// one-sided "(Jac*F)*A*JacInv*Cdiss", the strips' diagonal "F*AI(0)*Cdiss",
+// central "(Jac(+)*F(+)+Jac(-)*F(-))*AI(s)*JacInv*Cdiss"
+template <typename T>
+__device__ __forceinline__ void diss_one(T (&d)[6], const T* __restrict__ F, const T* __restrict__ J,
+                                         long long ps, long long k, T A, T Ji, T Cd) {
+  const T Jk = J[k];
+#pragma unroll
+  for (int l = 0; l < 5; l++) d[l] = d[l] + Jk * F[l * ps + k] * A * Ji * Cd;
+}
+
+template <typename T>
+__device__ __forceinline__ void diss_diag(T (&d)[6], const T* __restrict__ F, long long ps,
+                                          long long cc, T Cd) {
+#pragma unroll
+  for (int l = 0; l < 5; l++) d[l] = d[l] + F[l * ps + cc] * (T)FDM_AI0 * Cd;
+}
+
+template <typename T>
+__device__ __forceinline__ void diss_central(T (&d)[6], const T* __restrict__ F, const T* __restrict__ J,
+                                             long long ps, long long kp, long long km, T A, T Ji,
+                                             T Cd) {
+  const T Jp = J[kp], Jm = J[km];
+#pragma unroll
+  for (int l = 0; l < 5; l++) d[l] = d[l] + (Jp * F[l * ps + kp] + Jm * F[l * ps + km]) * A * Ji * Cd;
+}
+
 // SAT penalties and one-sided stencils of a point within four points of a
 // block edge (time_step.f90:754-810, rates.f90:451-918): L/R strips
 // one-sided in q and central in r, B/T strips the converse, corners
-// one-sided in both, interleaved per s as in rates_boundary_mode2.  The
-// stencil loops are unrolled so that all of a point's loads are in flight
-// together: the band is a thin set of points, latency-bound.
+// one-sided in both, interleaved per s as in rates_boundary_mode2.  With
+// artificial dissipation each one-sided or central term is followed by its
+// dissipation term, and strips add the diagonal between their two loops
+// (rates_dissipation_boundary_mode2).  The stencil loops are unrolled so
+// that all of a point's loads are in flight together: the band is a thin
+// set of points, latency-bound.
 template <typename T>
 __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockDev& B, const int i,
                                               const int j, const long long cc, T (&d)[6]) {
@@ -445,6 +534,9 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
   const bool het = a.mrho != nullptr;
   const T gamma = het ? a.mgam[cc] : (T)B.gamma, rho = het ? a.mrho[cc] : (T)B.rho;
   const T G = het ? a.mG[cc] : (T)B.G, M = het ? a.mM[cc] : (T)B.M;
+  const bool diss = B.Cdiss > 0.0;
+  const T Cd = (T)B.Cdiss;
+  const T* __restrict__ J = a.J;
   const bool qone = (dl < 4) || (dr < 4);
   const bool rone = (db < 4) || (dtp < 4);
   const long long row = (long long)j * ld;
@@ -460,6 +552,7 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
       d[2] = d[2] + bscale<T, 2>(q1<T, 2>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
       d[3] = d[3] + bscale<T, 1>(q1<T, 3>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
       d[4] = d[4] + bscale<T, 2>(q1<T, 4>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
+      if (diss) diss_one<T>(d, F, J, ps, kq, (T)((dl < 4) ? kAL[dl][s] : kAL[dr][s]), Ji, Cd);
       if (rone) {
         // corner: the r term of the same s follows
         long long kr = (db < 4) ? colm + (long long)s * ld : colp - (long long)s * ld;
@@ -469,10 +562,12 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
         d[2] = d[2] + bscale<T, 2>(r1<T, 2>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
         d[3] = d[3] + bscale<T, 1>(r1<T, 3>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
         d[4] = d[4] + bscale<T, 2>(r1<T, 4>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
+        if (diss) diss_one<T>(d, F, J, ps, kr, (T)((db < 4) ? kAL[db][s] : kAL[dtp][s]), Ji, Cd);
       }
     }
     if (!rone) {
       // L or R strip: central r
+      if (diss) diss_diag<T>(d, F, ps, cc, Cd);
 #pragma unroll
       for (int s = 1; s <= 2; s++) {
         T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
@@ -482,6 +577,7 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
         d[2] = d[2] + bscale<T, 2>(rc<T, 2>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
         d[3] = d[3] + bscale<T, 1>(rc<T, 3>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
         d[4] = d[4] + bscale<T, 2>(rc<T, 4>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
+        if (diss) diss_central<T>(d, F, J, ps, kp, km, (s == 1) ? (T)FDM_AI1 : (T)FDM_AI2, Ji, Cd);
       }
     }
   } else {
@@ -495,7 +591,9 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
       d[2] = d[2] + bscale<T, 2>(r1<T, 2>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
       d[3] = d[3] + bscale<T, 1>(r1<T, 3>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
       d[4] = d[4] + bscale<T, 2>(r1<T, 4>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
+      if (diss) diss_one<T>(d, F, J, ps, kr, (T)((db < 4) ? kAL[db][s] : kAL[dtp][s]), Ji, Cd);
     }
+    if (diss) diss_diag<T>(d, F, ps, cc, Cd);
 #pragma unroll
     for (int s = 1; s <= 2; s++) {
       T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
@@ -505,6 +603,7 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
       d[2] = d[2] + bscale<T, 2>(qc<T, 2>(F, yr, xr, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
       d[3] = d[3] + bscale<T, 1>(qc<T, 3>(F, yr, xr, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
       d[4] = d[4] + bscale<T, 2>(qc<T, 4>(F, yr, xr, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
+      if (diss) diss_central<T>(d, F, J, ps, kp, km, (s == 1) ? (T)FDM_AI1 : (T)FDM_AI2, Ji, Cd);
     }
   }
 }
@@ -536,11 +635,19 @@ __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i,
   T d[6];
   scaled_rates<T>(a, cc, d);
   if (i - B.mx >= 4 && B.px - i >= 4 && j - B.my >= 4 && B.py - j >= 4) {
-    const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld};
-    if (a.mrho)
+    const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld, a.J};
+    if (B.Cdiss > 0.0) {
+      const T dA[3] = {(T)B.dA[0], (T)B.dA[1], (T)B.dA[2]};
+      if (a.mrho)
+        interior_rates_diss<T>(d, acc, a.mgam[cc], PointCoef<T>{a.mrho[cc], a.mG[cc], a.mM[cc]}, dA,
+                               a.Ji[cc]);
+      else
+        interior_rates_diss<T>(d, acc, (T)B.gamma, BlockCoef<T, const BlockDev&>{B}, dA, a.Ji[cc]);
+    } else if (a.mrho) {
       interior_rates<T>(d, acc, a.mgam[cc], PointCoef<T>{a.mrho[cc], a.mG[cc], a.mM[cc]}, a.Ji[cc]);
-    else
+    } else {
       interior_rates<T>(d, acc, (T)B.gamma, BlockCoef<T, const BlockDev&>{B}, a.Ji[cc]);
+    }
   } else {
     closure_rates<T>(a, B, i, j, cc, d);
   }
@@ -606,7 +713,9 @@ struct TileAcc {
   const T (*sR)[TY][TX + 4];
   const T (*sQ)[TY + 4][TX];
   int x, y;
+  const T* sJ;  // [TY + 4][TX + 4], with artificial dissipation only
   __device__ __forceinline__ T f(int k, int di, int dj) const { return sF[k][y + 2 + dj][x + 2 + di]; }
+  __device__ __forceinline__ T j(int di, int dj) const { return sJ[(y + 2 + dj) * (TX + 4) + x + 2 + di]; }
   __device__ __forceinline__ T xr(int di) const { return sR[0][y][x + 2 + di]; }
   __device__ __forceinline__ T yr(int di) const { return sR[1][y][x + 2 + di]; }
   __device__ __forceinline__ T xq(int dj) const { return sQ[0][y + 2 + dj][x]; }
@@ -625,7 +734,7 @@ typedef const __attribute__((address_space(4))) BlockDev ConstBlockDev;
 // xq/yq with the r halo go to LDS; the lane's own DF, F6 and Ji stay in
 // registers.  Halo indices are clamped into the grid; a clamped value never
 // reaches an active lane.  tiles[k] = (block, i0, j0, 0).
-template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET>
+template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET, bool DISS>
 __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 td) {
   constexpr int HX = TX + 4, HY = TY + 4, NT = TX * TY;
   constexpr int NF = (HY * HX + NT - 1) / NT;  // staging passes, F1..F5
@@ -634,13 +743,15 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
   __shared__ T sF[5][HY][HX];
   __shared__ T sR[2][TY][HX];
   __shared__ T sQ[2][HY][TX];
+  // the Jacobian with the same halo as F, for artificial dissipation
+  __shared__ T sJ[DISS ? HY : 1][DISS ? HX : 1];
   const int bu = td.x, i0 = td.y, j0 = td.z;
   ConstBlockDev* B4 = (ConstBlockDev*)a.blocks + bu;
   const int tid = threadIdx.y * TX + threadIdx.x;
   const int ld = a.ld, nx1 = a.nx - 1, ny1 = a.ny - 1;
   const long long ps = a.ps;
   const T* __restrict__ F = a.Fin;
-  T vF[NF][5], vR[NR][2], vQ[NQ][2];
+  T vF[NF][5], vR[NR][2], vQ[NQ][2], vJ[DISS ? NF : 1];
 #pragma unroll
   for (int p = 0; p < NF; p++) {
     const int k = tid + p * NT;
@@ -650,6 +761,7 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
       const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
 #pragma unroll
       for (int l = 0; l < 5; l++) vF[p][l] = ld_off(F + l * ps, o);
+      if constexpr (DISS) vJ[p] = ld_off(a.J, o);
     }
   }
 #pragma unroll
@@ -700,6 +812,7 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
       const int r = k / HX, c = k - r * HX;
 #pragma unroll
       for (int l = 0; l < 5; l++) sF[l][r][c] = vF[p][l];
+      if constexpr (DISS) sJ[r][c] = vJ[p];
     }
   }
 #pragma unroll
@@ -732,11 +845,25 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     for (int l = 0; l < 5; l++) d[l] = A * dfa[l];
     d[5] = (T)0.0;  // DF6 is overwritten by the szz relation
   }
-  const TileAcc<T, TX, TY> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y};
-  if constexpr (HET)
-    interior_rates<T>(d, acc, mat[3], PointCoef<T>{mat[0], mat[1], mat[2]}, Ji);
-  else
-    interior_rates<T>(d, acc, (T)B4->gamma, BlockCoef<T, ConstBlockDev&>{*B4}, Ji);
+  const TileAcc<T, TX, TY> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y, &sJ[0][0]};
+  bool diss_done = false;
+  if constexpr (DISS) {
+    // a block without dissipation keeps the plain expression
+    if (B4->Cdiss > 0.0) {
+      const T dA[3] = {(T)B4->dA[0], (T)B4->dA[1], (T)B4->dA[2]};
+      if constexpr (HET)
+        interior_rates_diss<T>(d, acc, mat[3], PointCoef<T>{mat[0], mat[1], mat[2]}, dA, Ji);
+      else
+        interior_rates_diss<T>(d, acc, (T)B4->gamma, BlockCoef<T, ConstBlockDev&>{*B4}, dA, Ji);
+      diss_done = true;
+    }
+  }
+  if (!diss_done) {
+    if constexpr (HET)
+      interior_rates<T>(d, acc, mat[3], PointCoef<T>{mat[0], mat[1], mat[2]}, Ji);
+    else
+      interior_rates<T>(d, acc, (T)B4->gamma, BlockCoef<T, ConstBlockDev&>{*B4}, Ji);
+  }
   T Fc[6];
 #pragma unroll
   for (int k = 0; k < 5; k++) Fc[k] = acc.f(k, 0, 0);
@@ -762,12 +889,13 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
   }
 }
 
-// a heterogeneous medium holds five more values per lane: three waves per
-// SIMD keep them in registers
-template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET>
-__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : (HET ? 3 : 4))))
+// a heterogeneous medium holds five more values per lane, dissipation the
+// Jacobian's staging: three waves per SIMD keep them in registers
+template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET, bool DISS>
+__global__ void __launch_bounds__(TX * TY)
+    __attribute__((amdgpu_waves_per_eu(FULL ? 2 : ((HET || DISS) ? 3 : 4))))
 bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
-  interior_tile<T, TX, TY, FULL, NTS, HET>(a, tiles[linear_tile(a.xcd_swizzle)]);
+  interior_tile<T, TX, TY, FULL, NTS, HET, DISS>(a, tiles[linear_tile(a.xcd_swizzle)]);
 }
 
 // Batched interior kernel: the tiles of many problems in one launch; tile
@@ -776,7 +904,7 @@ template <typename T, int TX, int TY, bool FULL, bool NTS>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(FULL ? 2 : 4)))
 bulk_interior_batch(const BulkArgs<T>* __restrict__ tab, const int4* __restrict__ tiles, int swz) {
   const int4 td = tiles[linear_tile(swz)];
-  interior_tile<T, TX, TY, FULL, NTS, false>(tab[td.w], td);
+  interior_tile<T, TX, TY, FULL, NTS, false, false>(tab[td.w], td);
 }
 
 // Band kernel: the points within four points of their block's edge (one-

@@ -28,6 +28,11 @@ struct BlockDev {
   // FP32-bulk stores each field as its deviation from this per-block
   // constant (zero in FP64); totals are formed in FP64 where needed
   double Fb[6];
+  // artificial dissipation (rates.f90:1397-2021): Cdiss as the rates use it
+  // (0: none) and the interior weights d(0) = 2 AI(0) Cdiss, d(s) =
+  // AI(s) Cdiss (rates.f90:1416-1421)
+  double Cdiss;
+  double dA[3];
 };
 
 struct IfaceDev {

@@ -26,6 +26,21 @@ __device__ constexpr double kDL[4][6] = {
     {0.030612244897959, 0.000000000000000, -0.602040816326531, 0.000000000000000,
      0.653061224489796, -0.081632653061224}};
 
+// artificial dissipation AI(0..2) (fd_coeff.f90:103-104) and AL(s,r)
+// (fd_coeff.f90:117-124); AR(-s,-r) = AL(s,r) (fd_coeff.f90:242-246)
+#define FDM_AI0 (-0.500000000000000)
+#define FDM_AI1 0.333333333333333
+#define FDM_AI2 (-0.083333333333333)
+__device__ constexpr double kAL[4][6] = {
+    {-0.470588235294118, 0.941176470588235, -0.470588235294118, 0.000000000000000,
+     0.000000000000000, 0.000000000000000},
+    {0.271186440677966, -0.610169491525424, 0.406779661016949, -0.067796610169492,
+     0.000000000000000, 0.000000000000000},
+    {-0.186046511627907, 0.558139534883721, -0.651162790697674, 0.372093023255814,
+     -0.093023255813953, 0.000000000000000},
+    {0.000000000000000, -0.081632653061224, 0.326530612244898, -0.489795918367347,
+     0.326530612244898, -0.081632653061224}};
+
 struct NT {
   double vt, vn, stt, snt, snn, szz;
 };

@@ -45,6 +45,7 @@ module fdmap_capi
      real(c_double) :: mu, beta, b, h, eta, K
      real(c_double) :: s0(6)
      type(fdm_side) :: side(4)
+     real(c_double) :: Cdiss = 0.0_c_double
   end type fdm_block
 
   type,bind(C) :: fdm_iface
@@ -71,6 +72,7 @@ module fdmap_capi
      type(c_ptr) :: blocks, ifaces
      type(fdm_source) :: source
      type(c_ptr) :: mat_rho = c_null_ptr, mat_cs = c_null_ptr, mat_cp = c_null_ptr
+     type(c_ptr) :: J = c_null_ptr
   end type fdm_problem
 
   type,bind(C) :: fdm_profile_t

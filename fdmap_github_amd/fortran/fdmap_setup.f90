@@ -48,6 +48,7 @@ module fdmap_setup
      character(256) :: response
      real(dp) :: rho,G_,cs,cp,M_,lambda,nu,K,gamma,Zs,Zp,mu,beta,b,h,eta
      logical :: het = .false.
+     real(dp) :: Cdiss = 0.0_dp
      real(dp) :: F0blk(9)
      character(256) :: problem
      type(side_data) :: side(4)
@@ -554,7 +555,7 @@ contains
        end if
        read(input,nml=operator_list,iostat=stat)
        if (stat>0) call fail('Error in operator_list')
-       if (Cdiss>0.0_dp) call fail('artificial dissipation (Cdiss > 0) is not supported')
+       HB(i)%Cdiss = Cdiss
     end do
 
     do i = 1,nblocks
@@ -1290,6 +1291,7 @@ contains
     cprob%Ji = c_loc(gJi(1,1))
     cprob%x = c_loc(gcx(1,1))
     cprob%y = c_loc(gcy(1,1))
+    cprob%J = c_loc(gJ(1,1))
     if (allocated(gmrho)) then
        cprob%mat_rho = c_loc(gmrho(1,1))
        cprob%mat_cs = c_loc(gmcs(1,1))
@@ -1332,6 +1334,9 @@ contains
          cb%h = HBk%h
          cb%eta = HBk%eta
          cb%K = HBk%K
+         ! artificial dissipation scaled as (wave speed)/(grid spacing)
+         ! (domain.f90:316-320)
+         cb%Cdiss = HBk%Cdiss*CFL/dt
          ! initial_stress (fields.f90:1007-1058), mode 2, default problem
          cb%s0 = (/ 0.0_dp,0.0_dp,HBk%F0blk(6),0.0_dp,HBk%F0blk(8),0.0_dp /)
          do s = 1,4

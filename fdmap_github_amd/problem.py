@@ -59,6 +59,8 @@ class Problem:
         self.source: dict | None = None
         # heterogeneous medium: (rho, cs, cp) arrays [ny, nx], or None
         self.medium: tuple | None = None
+        # Jacobian [ny, nx], needed by blocks with artificial dissipation
+        self.J = None
         self._keep = []
 
     # ---- construction helpers ----
@@ -113,6 +115,7 @@ class Problem:
             prob.mat_rho, prob.mat_cs, prob.mat_cp = (P(a) for a in self.medium)
         else:
             prob.mat_rho = prob.mat_cs = prob.mat_cp = capi.dptr(None)
+        prob.J = P(self.J) if self.J is not None else capi.dptr(None)
 
         barr = (capi.fdm_block * max(1, len(self.blocks)))()
         for b, blk in enumerate(self.blocks):
@@ -124,6 +127,7 @@ class Problem:
             cb.plastic = 1 if pl else 0
             cb.mms = 1 if blk.get("mms") else 0
             cb.id = int(blk["id"])
+            cb.Cdiss = float(blk.get("Cdiss", 0.0))
             if pl:
                 for k in ("mu", "beta", "b", "h", "eta"):
                     setattr(cb, k, float(pl.get(k, 0.0)))

@@ -115,6 +115,9 @@ typedef struct fdm_block {
   double s0[6];                  /* initial stress sxx,sxy,sxz,syy,syz,szz used
                                     by plasticity (fields.f90:1007-1058)      */
   fdm_side side[4];              /* L, R, B, T                                */
+  double Cdiss;                  /* artificial dissipation as the rates use it,
+                                    Cdiss*CFL/dt (domain.f90:170-180, 316-320);
+                                    0: none                                   */
 } fdm_block;
 
 /* One block interface (iface_type + fr_type, interfaces.f90:9-20,
@@ -185,6 +188,10 @@ typedef struct fdm_problem {
   const double* mat_rho;
   const double* mat_cs;
   const double* mat_cp;
+  /* Jacobian J (grid.f90:465-534), [ny][nx]; needed when a block has
+     Cdiss > 0 (rates_dissipation_*_mode2, rates.f90:1397-2021), else may be
+     NULL. */
+  const double* J;
 } fdm_problem;
 
 /* Kernel timing gathered with HIP events on the context's stream. */

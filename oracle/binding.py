@@ -45,6 +45,8 @@ def load():
     lib.orc_sbp4_tables.restype = None
     lib.orc_diff.argtypes = [C.c_int32, _dp, _dp]
     lib.orc_diff.restype = None
+    lib.orc_diss1d.argtypes = [C.c_int32, _dp, _dp]
+    lib.orc_diss1d.restype = None
     lib.orc_rk_coeffs.argtypes = [C.c_int32, _dp, _dp, _dp]
     lib.orc_rk_coeffs.restype = C.c_int32
     lib.orc_solve_friction_node.argtypes = [C.c_int32, _dp, _dp] + [C.c_double] * 9 + [_dp] * 4
@@ -81,6 +83,15 @@ def diff(f):
     f = np.ascontiguousarray(f, dtype=np.float64)
     out = np.empty_like(f)
     lib.orc_diff(len(f), capi.dptr(f), capi.dptr(out))
+    return out
+
+
+def diss1d(f):
+    """the one-dimensional dissipation operator (unit J and Cdiss) applied to f"""
+    lib = load()
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.empty_like(f)
+    lib.orc_diss1d(len(f), capi.dptr(f), capi.dptr(out))
     return out
 
 

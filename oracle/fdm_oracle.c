@@ -48,7 +48,24 @@ static const double H00I4 = 2.823529411764706;
 #define NBST 4 /* fd_coeff.f90:64 */
 #define PI_I 2 /* (nI-1)/2 for SBP4 */
 
+/* artificial dissipation AI(-2..2) and AL4[r][s] = AL(s,r),
+   fd_coeff.f90:103-124 */
+static const double AI4[5] = {-0.083333333333333, 0.333333333333333, -0.500000000000000,
+                              0.333333333333333, -0.083333333333333};
+static const double AL4[4][6] = {
+    {-0.470588235294118, 0.941176470588235, -0.470588235294118, 0.000000000000000,
+     0.000000000000000, 0.000000000000000},
+    {0.271186440677966, -0.610169491525424, 0.406779661016949, -0.067796610169492,
+     0.000000000000000, 0.000000000000000},
+    {-0.186046511627907, 0.558139534883721, -0.651162790697674, 0.372093023255814,
+     -0.093023255813953, 0.000000000000000},
+    {0.000000000000000, -0.081632653061224, 0.326530612244898, -0.489795918367347,
+     0.326530612244898, -0.081632653061224}};
+
 static inline double DIc(int s) { return DI4[2 + s]; }
+static inline double AIc(int s) { return AI4[2 + s]; }
+/* AL(s,r); AR(-s,-r) = AL(s,r) (fd_coeff.f90:242-246, no sign change) */
+static inline double ALc(int s, int r) { return AL4[r][s]; }
 /* DL(s,r) */
 static inline double DLc(int s, int r) { return DL4[r][s]; }
 /* DR(-s,-r) = -DL(s,r)  (fd_coeff.f90:242-246) */
@@ -86,6 +103,27 @@ void orc_diff(int32_t n, const double* f, double* Df) {
     double s = 0.0;
     for (int k = -2; k <= 2; k++) s = s + DIc(k) * f[i + k];
     Df[i] = s;
+  }
+}
+
+/* the one-dimensional dissipation operator of the rates with unit J and
+   Cdiss: AL rows near the ends, AI in between (the q part of
+   rates_dissipation_boundary_mode2 'L'/'R' and of the interior) */
+void orc_diss1d(int32_t n, const double* f, double* Af) {
+  for (int i = 0; i < n; i++) Af[i] = 0.0;
+  if (n < 2 * NBND) return;
+  for (int i = 0; i < NBST; i++) {
+    double s = 0.0;
+    for (int k = 0; k < NBND; k++) s = s + ALc(k, i) * f[k];
+    Af[i] = s;
+    double t = 0.0;
+    for (int k = 0; k < NBND; k++) t = t + ALc(k, i) * f[n - 1 - k];
+    Af[n - 1 - i] = t;
+  }
+  for (int i = NBST; i <= n - 1 - NBST; i++) {
+    double s = 0.0;
+    for (int k = -2; k <= 2; k++) s = s + AIc(k) * f[i + k];
+    Af[i] = s;
   }
 }
 
@@ -347,6 +385,8 @@ struct orc_ctx {
   /* heterogeneous medium (material.f90:226-275): rho, G, M, gamma, nu per
      point, or NULL */
   double *mrho, *mG, *mM, *mgam, *mnu;
+  /* Jacobian J, when a block has artificial dissipation, or NULL */
+  double* J;
   oblock_t* B;
   oiface_t* I;
   fdm_source src;
@@ -436,6 +476,14 @@ int orc_create(const fdm_problem* P, orc_ctx** out) {
   for (int ib = 0; ib < c->nb; ib++) {
     const fdm_block* b = &P->blocks[ib];
     c->B[ib].b = *b;
+    if (b->Cdiss > 0.0 && !c->J) {
+      if (!P->J) {
+        snprintf(g_err, sizeof g_err, "orc_create: block %d has Cdiss > 0 and no Jacobian", ib);
+        orc_destroy(c);
+        return FDM_EINVAL;
+      }
+      c->J = dup_arr(P->J, c->np);
+    }
     if (b->px - b->mx + 1 < 8 || b->py - b->my + 1 < 8) {
       snprintf(g_err, sizeof g_err, "orc_create: block %d has fewer than 8 points", ib);
       orc_destroy(c);
@@ -557,6 +605,7 @@ void orc_destroy(orc_ctx* c) {
   free(c->mM);
   free(c->mgam);
   free(c->mnu);
+  free(c->J);
   if (c->B)
     for (int ib = 0; ib < c->nb; ib++)
       for (int s = 0; s < 4; s++) {
@@ -1240,6 +1289,9 @@ typedef struct {
   double rho, G, M, gamma;
   /* heterogeneous medium: material at each point (NULL: the block's) */
   const double *prho, *pG, *pM, *pgam;
+  /* artificial dissipation: Jacobian and the scaled Cdiss (0: none) */
+  const double* J;
+  double Cdiss;
   int nx;
 } sten_t;
 
@@ -1286,8 +1338,60 @@ static inline double rc(const sten_t* S, int l, size_t kp, size_t km, size_t cc,
          prodg(bym, S->F[UA[l]][km], GA[l], g) - prodg(bxm, S->F[UB[l]][km], GB[l], g);
 }
 
+/* coefficient of the elastic sum in the dissipative interior rates:
+   a(s) = DI(s)/rho, c(s) = DI(s)*M, b(s) = DI(s)*G (rates.f90:1416-1421),
+   per point in a heterogeneous medium (rates_heterogeneous.f90:1409-1450) */
+static inline double diss_coef(int l, double D, double rho, double G, double M) {
+  if (l < 2) return D / rho;
+  if (l == 3) return D * G;
+  return D * M;
+}
+
+/* rates_dissipation_interior_mode2 (rates.f90:1397-1482) and its
+   heterogeneous form (rates_heterogeneous.f90:1376-1458): d(0) F first, then
+   per s (a(s) X + d(s) Y) Ji with Y the J-weighted neighbours.  DI(3) of
+   the heterogeneous loop lies outside DI(-2:2) for SBP4 (the homogeneous
+   routine zeroes its s = 3 coefficient); the s = 3 term is taken as zero. */
+static void rates_dissipation_interior(const sten_t* S, int mx, int px, int my, int py) {
+  double d[4];
+  d[0] = 2.0 * AIc(0) * S->Cdiss;
+  for (int s = 1; s <= 3; s++) d[s] = s <= PI_I ? AIc(s) * S->Cdiss : 0.0;
+  int nx = S->nx;
+  const double* J = S->J;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int j = my; j <= py; j++)
+    for (int i = mx; i <= px; i++) {
+      size_t cc = (size_t)j * nx + i;
+      double rho, G, M, g;
+      mat_at(S, cc, &rho, &G, &M, &g);
+      for (int l = 0; l < 5; l++) S->DF[l][cc] = S->DF[l][cc] + d[0] * S->F[l][cc];
+      for (int s = 1; s <= 3; s++)
+        for (int l = 0; l < 5; l++) {
+          size_t ip = cc + s, im = cc - s, jp = cc + (size_t)s * nx, jm = cc - (size_t)s * nx;
+          double ayp = 0.5 * (S->yr[ip] + S->yr[cc]), axp = 0.5 * (S->xr[ip] + S->xr[cc]);
+          double aym = 0.5 * (S->yr[im] + S->yr[cc]), axm = 0.5 * (S->xr[im] + S->xr[cc]);
+          double byp = 0.5 * (S->yq[jp] + S->yq[cc]), bxp = 0.5 * (S->xq[jp] + S->xq[cc]);
+          double bym = 0.5 * (S->yq[jm] + S->yq[cc]), bxm = 0.5 * (S->xq[jm] + S->xq[cc]);
+          const double* FA = S->F[UA[l]];
+          const double* FB = S->F[UB[l]];
+          const double* Fl = S->F[l];
+          double x = prodg(ayp, FA[ip], GA[l], g) - prodg(axp, FB[ip], GB[l], g) -
+                     prodg(aym, FA[im], GA[l], g) + prodg(axm, FB[im], GB[l], g) -
+                     prodg(byp, FA[jp], GA[l], g) + prodg(bxp, FB[jp], GB[l], g) +
+                     prodg(bym, FA[jm], GA[l], g) - prodg(bxm, FB[jm], GB[l], g);
+          double y = J[ip] * Fl[ip] + J[im] * Fl[im] + J[jp] * Fl[jp] + J[jm] * Fl[jm];
+          double a = diss_coef(l, s <= PI_I ? DIc(s) : 0.0, rho, G, M);
+          S->DF[l][cc] = S->DF[l][cc] + (a * x + d[s] * y) * S->Ji[cc];
+        }
+    }
+}
+
 /* rates_interior_mode2, rates.f90:384-450 */
 static void rates_interior(const sten_t* S, int mx, int px, int my, int py) {
+  if (S->Cdiss > 0.0) {
+    rates_dissipation_interior(S, mx, px, my, py);
+    return;
+  }
   double coef[5][4];
   for (int s = 1; s <= 3; s++) {
     double D = s <= PI_I ? DIc(s) : 0.0;
@@ -1333,9 +1437,30 @@ static void rates_interior(const sten_t* S, int mx, int px, int my, int py) {
     }
 }
 
+/* dissipation terms of rates_dissipation_boundary_mode2 (rates.f90:1485-
+   2021), fields 1..5 (the sixth is overwritten by rates_szz):
+   one-sided   DF += (J(k) F(k)) A(s) Ji Cdiss
+   diagonal    DF += F AI(0) Cdiss
+   central     DF += (J(k+) F(k+) + J(k-) F(k-)) AI(s) Ji Cdiss */
+static inline void diss_one(const sten_t* S, size_t k, size_t cc, double A) {
+  for (int l = 0; l < 5; l++)
+    S->DF[l][cc] = S->DF[l][cc] + S->J[k] * S->F[l][k] * A * S->Ji[cc] * S->Cdiss;
+}
+static inline void diss_diag(const sten_t* S, size_t cc) {
+  for (int l = 0; l < 5; l++) S->DF[l][cc] = S->DF[l][cc] + S->F[l][cc] * AIc(0) * S->Cdiss;
+}
+static inline void diss_central(const sten_t* S, size_t kp, size_t km, size_t cc, int s) {
+  for (int l = 0; l < 5; l++)
+    S->DF[l][cc] = S->DF[l][cc] + (S->J[kp] * S->F[l][kp] + S->J[km] * S->F[l][km]) * AIc(s) *
+                                      S->Ji[cc] * S->Cdiss;
+}
+
 /* rates_boundary_mode2, rates.f90:453-917.  qside: 0 none (central q),
    1 left one-sided, 2 right one-sided; rside likewise for r.  (mx,px,my,py)
-   is the loop range; (bx0,bx1,by0,by1) the block limits mgx,pgx,mgy,pgy. */
+   is the loop range; (bx0,bx1,by0,by1) the block limits mgx,pgx,mgy,pgy.
+   With Cdiss > 0 the dissipation terms follow each one-sided or central
+   term, and strips add the diagonal between their two loops; corners have
+   none (rates.f90:1485-2021). */
 static void rates_boundary(const sten_t* S, int mx, int px, int my, int py, int qside, int rside,
                            int bx0, int bx1, int by0, int by1) {
   int nx = S->nx;
@@ -1347,68 +1472,85 @@ static void rates_boundary(const sten_t* S, int mx, int px, int my, int py, int 
          same expressions with the point's material */
       double rho, G, M, g;
       mat_at(S, cc, &rho, &G, &M, &g);
+      const int diss = S->Cdiss > 0.0;
       if (qside && rside) {
         for (int s = 0; s < NBND; s++) {
           size_t kq, kr;
-          double Dq, Dr;
+          double Dq, Dr, Aq, Ar;
           if (qside == 1) {
             kq = (size_t)j * nx + (bx0 + s);
             Dq = DLc(s, i - bx0);
+            Aq = ALc(s, i - bx0);
           } else {
             kq = (size_t)j * nx + (bx1 - s);
             Dq = DRc(s, bx1 - i);
+            Aq = ALc(s, bx1 - i);
           }
           if (rside == 1) {
             kr = (size_t)(by0 + s) * nx + i;
             Dr = DLc(s, j - by0);
+            Ar = ALc(s, j - by0);
           } else {
             kr = (size_t)(by1 - s) * nx + i;
             Dr = DRc(s, by1 - j);
+            Ar = ALc(s, by1 - j);
           }
           for (int l = 0; l < 5; l++)
             S->DF[l][cc] = S->DF[l][cc] + bscale(l, q1(S, l, kq, cc, g), Dq, rho, G, M) * Ji;
+          if (diss) diss_one(S, kq, cc, Aq);
           for (int l = 0; l < 5; l++)
             S->DF[l][cc] = S->DF[l][cc] + bscale(l, r1(S, l, kr, cc, g), Dr, rho, G, M) * Ji;
+          if (diss) diss_one(S, kr, cc, Ar);
         }
       } else if (qside) {
         for (int s = 0; s < NBND; s++) {
           size_t kq;
-          double Dq;
+          double Dq, Aq;
           if (qside == 1) {
             kq = (size_t)j * nx + (bx0 + s);
             Dq = DLc(s, i - bx0);
+            Aq = ALc(s, i - bx0);
           } else {
             kq = (size_t)j * nx + (bx1 - s);
             Dq = DRc(s, bx1 - i);
+            Aq = ALc(s, bx1 - i);
           }
           for (int l = 0; l < 5; l++)
             S->DF[l][cc] = S->DF[l][cc] + bscale(l, q1(S, l, kq, cc, g), Dq, rho, G, M) * Ji;
+          if (diss) diss_one(S, kq, cc, Aq);
         }
+        if (diss) diss_diag(S, cc);
         for (int s = 1; s <= PI_I; s++) {
           size_t jp = cc + (size_t)s * nx, jm = cc - (size_t)s * nx;
           for (int l = 0; l < 5; l++)
             S->DF[l][cc] =
                 S->DF[l][cc] + bscale(l, rc(S, l, jp, jm, cc, g), DIc(s), rho, G, M) * Ji;
+          if (diss) diss_central(S, jp, jm, cc, s);
         }
       } else {
         for (int s = 0; s < NBND; s++) {
           size_t kr;
-          double Dr;
+          double Dr, Ar;
           if (rside == 1) {
             kr = (size_t)(by0 + s) * nx + i;
             Dr = DLc(s, j - by0);
+            Ar = ALc(s, j - by0);
           } else {
             kr = (size_t)(by1 - s) * nx + i;
             Dr = DRc(s, by1 - j);
+            Ar = ALc(s, by1 - j);
           }
           for (int l = 0; l < 5; l++)
             S->DF[l][cc] = S->DF[l][cc] + bscale(l, r1(S, l, kr, cc, g), Dr, rho, G, M) * Ji;
+          if (diss) diss_one(S, kr, cc, Ar);
         }
+        if (diss) diss_diag(S, cc);
         for (int s = 1; s <= PI_I; s++) {
           size_t ip = cc + s, im = cc - s;
           for (int l = 0; l < 5; l++)
             S->DF[l][cc] =
                 S->DF[l][cc] + bscale(l, qc(S, l, ip, im, cc, g), DIc(s), rho, G, M) * Ji;
+          if (diss) diss_central(S, ip, im, cc, s);
         }
       }
     }
@@ -1435,6 +1577,8 @@ static void set_rates(orc_ctx* c, int ib) {
   S.pG = c->mG;
   S.pM = c->mM;
   S.pgam = c->mgam;
+  S.J = c->J;
+  S.Cdiss = b->Cdiss;
   S.nx = c->nx;
   int mgx = b->mx, pgx = b->px, mgy = b->my, pgy = b->py;
   int mix = mgx + NBST, pix = pgx - NBST, miy = mgy + NBST, piy = pgy - NBST;

@@ -43,6 +43,7 @@ void orc_sbp4_tables(double* DI, double* DL, double* HL, double* H00i);
 /* 1-D SBP4 first derivative with unit spacing over n points (fd.f90:42-80
    with both ends inside the range). */
 void orc_diff(int32_t n, const double* f, double* Df);
+void orc_diss1d(int32_t n, const double* f, double* Af);
 
 /* LSRK coefficients (time_step.f90:25-66): returns nstage, fills A, B,
    C[0..nstage]. */

@@ -57,11 +57,13 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
           coupling="friction", law="RFL", bcs=None, precision="fp64", background=None,
           perturb=1e-3, source=False, plastic=None, nbx=1, rk_order=4, outside_frac=0.0,
           fault_overstress=25.0, ld_T=0.0, reg=False, rup_field="V", Vinit=1e-12,
-          skempton=0.0, mms=False, medium=None):
+          skempton=0.0, mms=False, medium=None, Cdiss=None):
     """Blocks stacked in y (len(nys) rows) and nbx columns; interfaces
     between them.  medium: amplitude of a smooth seeded heterogeneous medium
     (rho, cs, cp varying by that relative amount around the given values;
     the edge coefficients then use the medium's values at each node).
+    Cdiss: artificial dissipation of every block as the rates use it (a
+    number, or one per block).
     Returns (Problem, F0 fields [6,ny,nx])."""
     rng = np.random.default_rng(seed)
     nyb = list(nys)
@@ -109,6 +111,7 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
     J = xq * yr - xr * yq
     prob.xq, prob.xr, prob.yq, prob.yr = xq, xr, yq, yr
     prob.Ji = 1.0 / J
+    prob.J = J
     prob.x, prob.y = X, Y
 
     if medium:
@@ -164,6 +167,8 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
                 sides.append(dict(bc=bc, normal=np.vstack([n1, n2]), M=M, F0=F0,
                                   x=X[idx].copy(), y=Y[idx].copy()))
             blk = prob.add_block(mx, px, my, py, rho, cs, cp, sides, plastic=plastic, mms=mms)
+            if Cdiss is not None:
+                blk["Cdiss"] = Cdiss[len(prob.blocks) - 1] if np.ndim(Cdiss) else float(Cdiss)
     # interfaces between vertically adjacent blocks (direction 'y') and
     # horizontally adjacent ones ('x'), domain.f90:1648-1682
     for by in range(len(nyb)):

@@ -49,6 +49,12 @@ CASES = {
     # a heterogeneous medium (64 x 4 tiles whatever FDM_TILE asks)
     "medium": dict(nx=203, nys=(61, 77), seed=37, warp=0.04, background=BG, perturb=1e-3,
                    law="RFL", medium=0.3),
+    # artificial dissipation in one of the two block rows (the other keeps
+    # the plain expression in the same launch)
+    "dissipation": dict(nx=203, nys=(61, 77), seed=41, warp=0.04, background=BG, perturb=1e-3,
+                        law="RFL", Cdiss=[300.0, 0.0]),
+    "dissipation_medium": dict(nx=203, nys=(61, 77), seed=43, warp=0.04, background=BG,
+                               perturb=1e-3, law="RFL", Cdiss=300.0, medium=0.3),
     # plasticity and a point source: the full finishing path
     "plastic_source": dict(nx=130, nys=(40, 44), seed=31, warp=0.02,
                            background=[0.0, 0.0, -60.0, 44.0, -100.0, -80.0], perturb=0.0,

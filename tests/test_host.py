@@ -111,3 +111,19 @@ def test_heterogeneous_medium_file(tmp_path):
     q = inputs.write(str(tmp_path / "c.in"), inputs.c2_planar_rs(name="c", nx=nx, ny=ny, nsteps=5))
     Q = host.HostProblem(q).ptr.contents
     assert not Q.mat_rho and not Q.mat_cs and not Q.mat_cp
+
+
+def test_operator_list_dissipation(tmp_path):
+    """operator_list Cdiss per block (domain.f90:167-180), scaled by CFL/dt
+    once dt is known (domain.f90:316-320); the Jacobian goes with it"""
+    txt = inputs.c2_planar_rs(name=str(tmp_path / "d"), nx=41, ny=21, nsteps=5, outputs=False)
+    marker = "!---BLOCK2---"
+    assert marker in txt
+    txt = txt.replace(marker, "&operator_list Cdiss = 0.3d0 /\n\n" + marker)
+    hp = host.HostProblem(inputs.write(str(tmp_path / "d.in"), txt))
+    P = hp.ptr.contents
+    assert math.isclose(P.blocks[0].Cdiss, 0.3 * hp.CFL / hp.dt, rel_tol=1e-15)
+    assert P.blocks[1].Cdiss == 0.0
+    J = np.ctypeslib.as_array(P.J, shape=(hp.ny, hp.nx))
+    Ji = np.ctypeslib.as_array(P.Ji, shape=(hp.ny, hp.nx))
+    np.testing.assert_allclose(J * Ji, 1.0, rtol=1e-14)

@@ -428,6 +428,64 @@ def test_uniform_medium_matches_homogeneous_material():
     assert np.abs(a[2] - b[2]).max() <= 1e-12 * max(1.0, np.abs(a[2]).max())
 
 
+@pytest.mark.parametrize("n", [12, 17, 40])
+def test_dissipation_operator_is_negative_semidefinite_in_the_norm(n):
+    """AL/AI/AR of fd_coeff.f90:103-124, 240-246: H A is symmetric and
+    negative semidefinite, with null space exactly the linear functions
+    (Mattsson, Svard and Nordstrom 2004); the interior rows are the fourth
+    difference -(1, -4, 6, -4, 1)/12"""
+    A = np.array([ob.diss1d(np.eye(n)[k]) for k in range(n)]).T
+    _, _, HL, _ = ob.sbp4_tables()
+    h = np.ones(n)
+    h[:4] = HL
+    h[-4:] = HL[::-1]
+    HA = h[:, None] * A
+    assert np.abs(HA - HA.T).max() < 1e-14
+    ev = np.linalg.eigvalsh(0.5 * (HA + HA.T))
+    assert ev.max() < 1e-14
+    assert np.sum(np.abs(ev) < 1e-12) == 2
+    x = np.arange(n, dtype=float)
+    assert np.abs(A @ np.ones(n)).max() < 1e-14 and np.abs(A @ x).max() < 1e-12
+    for i in range(4, n - 4):
+        np.testing.assert_allclose(A[i, i - 2:i + 3], np.array([-1, 4, -6, 4, -1]) / 12.0,
+                                   rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("medium", [None, 0.3])
+def test_stage1_dissipation_matches_dense_operator(medium):
+    """rates_dissipation_*_mode2 (rates.f90:1397-2021, rates_heterogeneous.f90:
+    1376-1997): the rates with Cdiss exceed those without by
+    Cdiss Ji (A_q + A_r)(J F) in each block, fields 1..5, with A the
+    one-dimensional operator; DF6 follows through szz"""
+    kw = dict(nx=26, nys=(17, 19), seed=4, warp=0.06, background=[0.1, -0.2, -120.0, 70.0, -110.0, -100.0],
+              perturb=0.5, coupling="locked", medium=medium)
+    out = []
+    for Cd in (0.0, 2000.0):
+        prob, F = problems.build(Cdiss=Cd, **kw)
+        o = OracleContext(prob)
+        o.set_fields(F)
+        o.init_edges(0.0)
+        o.stage1(0.0, 1e-3, 1)
+        out.append(o.get_fields()[1])
+    diff = out[1] - out[0]
+    JF = prob.J[None] * F
+    ref = np.zeros_like(diff)
+    for b in prob.blocks:
+        sj, si = slice(b["my"], b["py"] + 1), slice(b["mx"], b["px"] + 1)
+        for l in range(5):
+            blk = JF[l, sj, si]
+            aq = np.array([ob.diss1d(row) for row in blk])
+            ar = np.array([ob.diss1d(col) for col in blk.T]).T
+            ref[l, sj, si] = 2000.0 * prob.Ji[sj, si] * (aq + ar)
+    nu = prob.blocks[0]["nu"]
+    if medium is None:
+        ref[5] = nu * (ref[2] + ref[4])
+    scale = np.abs(out[0]).max(axis=(1, 2), keepdims=True)
+    err = np.abs(diff - ref)[:5 if medium else 6] / scale[:5 if medium else 6]
+    assert np.all(np.abs(ref[:5]).max(axis=(1, 2)) > 0.05 * scale[:5, 0, 0])
+    assert err.max() < 1e-12, err.max()
+
+
 def energy(prob, F, rho, cs, cp):
     _, _, HL, _ = ob.sbp4_tables()
 

@@ -158,6 +158,38 @@ def test_heterogeneous_medium_friction_and_plasticity(gpu):
     assert go.max() > 0.0
 
 
+def test_dissipation_bit_exact(gpu):
+    """rates_dissipation_*_mode2 (rates.f90:1397-2021) with a point source,
+    curvilinear: interior tiles, band and closures, bit for bit"""
+    prob, F = problems.build(nx=60, nys=(44,), seed=47, warp=0.05, background=BG, perturb=1e-2,
+                             source=True, bcs=("absorbing", "free", "rigid", "free"), Cdiss=250.0)
+    dev, orc = run_both(prob, F, 25)
+    compare(dev, orc, 0, bit_exact=True)
+
+
+def test_dissipation_locked_2x2_mixed_blocks_bit_exact(gpu):
+    """two of four blocks dissipative: the others keep the plain rates"""
+    prob, F = problems.build(nx=36, nys=(30, 34), nbx=2, seed=53, warp=0.04, background=BG,
+                             perturb=1e-2, coupling="locked", Cdiss=[250.0, 0.0, 0.0, 120.0])
+    dev, orc = run_both(prob, F, 20)
+    compare(dev, orc, 0, bit_exact=True)
+
+
+def test_dissipation_heterogeneous_friction_plasticity(gpu):
+    """rates_h_dissipation_*_mode2 (rates_heterogeneous.f90:1376-1997) with
+    a fault and plastic blocks"""
+    pl = dict(mu=0.6, beta=0.0, b=0.0, h=0.0, eta=0.0, s0=[0.0] * 6)
+    bg = [0.0, 0.0, -60.0, 44.0, -100.0, -80.0]
+    prob, F = problems.build(nx=48, nys=(24, 28), seed=59, warp=0.02, background=bg,
+                             perturb=0.0, law="RFL", plastic=pl, fault_overstress=20.0,
+                             medium=0.2, Cdiss=200.0)
+    dev, orc = run_both(prob, F, 30)
+    compare(dev, orc, 1, bit_exact=False)
+    gd, _ = dev.get_plastic()
+    go, _ = orc.get_plastic()
+    assert rel_l2(gd, go) <= 1e-12 and go.max() > 0.0
+
+
 def test_fp32_bulk_error_against_fp64(gpu):
     """FP32-bulk (fields FP32 as deviations from a per-block background,
     friction FP64) against the FP64 oracle on a prestressed rupture: the
