@@ -154,25 +154,33 @@ def run_c3(args, tmp):
     print(json.dumps(out), flush=True)
 
 
-def run_c2het(args, tmp):
-    """C2 in a heterogeneous medium: the same two blocks and fault, with rho,
-    cs and cp varying smoothly by 10 % over the grid (seeded, read from a
-    medium file).  The interior kernel reads five more planes per point
-    (rho, G, M, gamma, nu): 40 B more per point in FP64, 20 B in FP32."""
+def run_c2_variant(args, tmp, het, cdiss):
+    """C2 in a heterogeneous medium and/or with artificial dissipation: the
+    same two blocks and fault.  het: rho, cs and cp vary smoothly by 10 %
+    over the grid (seeded, read from a medium file); the interior kernel
+    reads five more planes per point (rho, G, M, gamma, nu), 40 B more per
+    point in FP64.  cdiss: Cdiss of both blocks (scaled by CFL/dt); the
+    interior kernel reads the Jacobian, 8 B more per point.  FP32 halves
+    both."""
     from fdmap_github_amd import inputs
 
     nx, ny = args.nx, args.ny
-    rho, cs, cp = inputs.smooth_medium(nx, 2 * ny, 0.025, amp=0.1, seed=5)
-    med = inputs.write_medium(os.path.join(tmp, "medium.bin"), rho, cs, cp)
-    inpath = inputs.write(os.path.join(tmp, "c2het.in"),
-                          inputs.c2_planar_rs(name=os.path.join(tmp, "c2het"), nx=nx, ny=ny,
+    med = None
+    if het:
+        rho, cs, cp = inputs.smooth_medium(nx, 2 * ny, 0.025, amp=0.1, seed=5)
+        med = inputs.write_medium(os.path.join(tmp, "medium.bin"), rho, cs, cp)
+    name = args.config
+    inpath = inputs.write(os.path.join(tmp, name + ".in"),
+                          inputs.c2_planar_rs(name=os.path.join(tmp, name), nx=nx, ny=ny,
                                               nsteps=args.steps + args.warmup + args.prof_steps,
-                                              outputs=False, medium=med))
+                                              outputs=False, medium=med, Cdiss=cdiss))
     res = {}
     for prec in ("fp64",) if args.no_fp32 else ("fp64", "fp32"):
         r = run_device(inpath, prec, args.steps, args.warmup, args.prof_steps)
         ns, prof = r["nstage"], r["prof"]
-        extra = 40.0 if prec == "fp64" else 20.0
+        extra = (40.0 if het else 0.0) + (8.0 if cdiss else 0.0)
+        if prec == "fp32":
+            extra *= 0.5
         b1, b2 = (b + extra for b in BYTES_STAGE[prec])
         kpts = prof["bulk_points"]
         bytes_launch = kpts * (b1 + (ns - 1) * b2) / ns
@@ -186,15 +194,18 @@ def run_c2het(args, tmp):
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * r64["r"]["el"] / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-        "data": "synthetic (planar fault problem and seeded medium file)",
-        "config": {"workload": f"planar rate-and-state rupture in a heterogeneous medium, 2 blocks "
-                               f"x ({nx}x{ny}), rho/cs/cp varying by 10 % (seed 5), RFL slip law, "
-                               "LSRK(5,4) CFL 0.5",
+        "data": "synthetic (planar fault problem" + (" and seeded medium file)" if het else ")"),
+        "config": {"workload": "planar rate-and-state rupture"
+                               + (" in a heterogeneous medium (rho/cs/cp varying by 10 %, seed 5)"
+                                  if het else "")
+                               + (f" with artificial dissipation Cdiss {cdiss}" if cdiss else "")
+                               + f", 2 blocks x ({nx}x{ny}), RFL slip law, LSRK(5,4) CFL 0.5",
                    "grid_points": r64["r"]["npts"], "fault_nodes": nx,
                    "dt": r64["r"]["hp"].dt},
         "roofline": {
             "bound": "hbm",
-            "kernel": "bulk_interior<double,64,4,false,NT,true> (heterogeneous medium)",
+            "kernel": f"bulk_interior<double,64,4,false,NT,{str(bool(het)).lower()},"
+                      f"{str(bool(cdiss)).lower()}>",
             "kernel_points": r64["kpts"], "achieved": r64["achieved"], "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS, "traffic": None,
             "alg_bytes_per_launch": r64["bytes_launch"], "avg_launch_ms": r64["avg_ms"],
@@ -432,9 +443,10 @@ def main():
                     help="steps timed after the headline ones with HIP events per launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--config", choices=("c2", "c2het", "c3", "c4", "c5"), default="c2",
+    ap.add_argument("--config", choices=("c2", "c2het", "c2diss", "c3", "c4", "c5"), default="c2",
                     help="c2: planar rupture (the headline); c2het: the same in a "
-                         "heterogeneous medium; c3: rough curvilinear fault with "
+                         "heterogeneous medium; c2diss: the same with artificial dissipation "
+                         "(Cdiss 0.1); c3: rough curvilinear fault with "
                          "off-fault plasticity, 2 x (4001x2001), FP64 and FP32-bulk; c4: 64 "
                          "uneven rupture scenarios advanced as one graph launch per step; c5: "
                          "adjoint gradient of a receiver misfit, 2 x (801x401), 2000 steps")
@@ -449,8 +461,9 @@ def main():
     from fdmap_github_amd import inputs
 
     tmp = tempfile.mkdtemp(prefix="fdmap_bench_")
-    if args.config == "c2het":
-        run_c2het(args, tmp)
+    if args.config in ("c2het", "c2diss"):
+        run_c2_variant(args, tmp, het=args.config == "c2het",
+                       cdiss=0.1 if args.config == "c2diss" else 0.0)
         return
     if args.config == "c3":
         run_c3(args, tmp)

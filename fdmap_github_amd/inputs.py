@@ -69,7 +69,7 @@ vx vy
 def c2_planar_rs(name="c2", nx=1601, ny=801, h=0.025, nsteps=4000, outputs=True, ninfo=500,
                  law="RFL", plastic=False, curve=None, tau0=70.0, sn0=120.0, overstress=25.0,
                  R=1.5, Vinit=1e-12, a=0.010, b=0.014, sxy0=None, bcT="absorbing",
-                 medium=None):
+                 medium=None, Cdiss=0.0):
     """Two blocks nx x ny coupled across a horizontal fault by SAT; slip law
     with strong velocity weakening; Gaussian overstress at the fault centre.
     curve: optional path of a fault-profile curve file (FP64 x, y, n1, n2),
@@ -77,13 +77,15 @@ def c2_planar_rs(name="c2", nx=1601, ny=801, h=0.025, nsteps=4000, outputs=True,
     background shear stress is sxy0 (default tau0); Psi0 gives slip velocity
     Vinit under tau0.  bcT: boundary condition of the top of the upper block
     ('free' for a free surface).  medium: path of a heterogeneous medium file
-    (write_medium) that both blocks read."""
+    (write_medium) that both blocks read.  Cdiss: artificial dissipation of
+    both blocks (operator_list, domain.f90:167-180)."""
     Lx = 0.5 * (nx - 1) * h
     Ly = (ny - 1) * h
     psi0 = rs_state(a, 1e-6, tau0, sn0, Vinit)
     c1 = f"bndTfile = '{curve}'," if curve else ""
     c2 = f"bndBfile = '{curve}'," if curve else ""
     mat = f"rho = {_f(RHO)}, cs = {_f(CS)}, cp = {_f(CP)}"
+    op = f"&operator_list Cdiss = {_f(Cdiss)} /\n\n" if Cdiss else ""
     if plastic:
         mat = f"response = 'plastic', {mat}, mu = 0.6d0, b = 0d0, beta = 0d0"
     if medium:
@@ -104,7 +106,7 @@ D V S N Psi
 
 !---BLOCK1---
 
-&grid_list
+{op}&grid_list
  nx = {nx}, ny = {ny}, {c1}
  LB = {_f(-Lx)},{_f(-Ly)}, LT = {_f(-Lx)},0d0,
  RB = {_f(Lx)},{_f(-Ly)}, RT = {_f(Lx)},0d0 /
@@ -113,7 +115,7 @@ D V S N Psi
 
 !---BLOCK2---
 
-&grid_list
+{op}&grid_list
  nx = {nx}, ny = {ny}, {c2}
  LB = {_f(-Lx)},0d0, LT = {_f(-Lx)},{_f(Ly)},
  RB = {_f(Lx)},0d0, RT = {_f(Lx)},{_f(Ly)} /
