@@ -134,6 +134,7 @@ class fdm_problem(C.Structure):
         ("source", fdm_source),
         ("mat_rho", _dp), ("mat_cs", _dp), ("mat_cp", _dp),
         ("J", _dp),
+        ("mode", C.c_int32), ("pad_mode_", C.c_int32),
     ]
 
 

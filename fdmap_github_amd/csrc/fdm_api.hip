@@ -233,6 +233,8 @@ struct fdm_ctx {
   // artificial dissipation in some block: the Jacobian plane
   bool diss = false;
   void* Jp = nullptr;
+  // 2: plane strain, 6 fields; 3: antiplane shear, 3 fields
+  int mode = 2, nF = 6;
   std::vector<double> hnu;  // nu of the medium ([ny][nx]), for restore_df6
   // bulk kernel: 0 per-point, 2 interior tiles (kTileShape[tile_shape]) + band points
   int bulk_mode = 2;
@@ -418,6 +420,7 @@ EdgeArgs<T> edge_args(fdm_ctx* c, double t, double Bdt, double A, int do_scale, 
   a.snapshot = snapshot;
   a.fail = c->fail;
   a.trace = c->trace_cur;
+  a.mode = c->mode;
   return a;
 }
 
@@ -497,7 +500,9 @@ void launch_band(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   if (!c->nband) return;
   EventPair* ep = prof_begin(c, 2, s);
   dim3 grid((c->nband + 63) / 64);
-  if (full_variant(c, a))
+  if (c->mode == 3)
+    hipLaunchKernelGGL(bulk_band3<T>, grid, dim3(64), 0, s, a, c->band, c->nband);
+  else if (full_variant(c, a))
     hipLaunchKernelGGL((bulk_band<T, true>), grid, dim3(64), 0, s, a, c->band, c->nband);
   else
     hipLaunchKernelGGL((bulk_band<T, false>), grid, dim3(64), 0, s, a, c->band, c->nband);
@@ -513,6 +518,16 @@ void launch_interior(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   c->df6_stale = !full;
   dim3 grid(c->ntiles[q]);
   dim3 block(kTileShape[q][0], kTileShape[q][1]);
+  if (c->mode == 3) {
+    // antiplane shear runs the 64 x 4 tiles (fdm_create keeps tile_shape 0)
+    c->df6_stale = false;
+    if (c->nt_store)
+      hipLaunchKernelGGL((bulk_interior3<T, 64, 4, true>), grid, block, 0, s, a, c->tiles[q]);
+    else
+      hipLaunchKernelGGL((bulk_interior3<T, 64, 4, false>), grid, block, 0, s, a, c->tiles[q]);
+    prof_end(c, ep, 0, s);
+    return;
+  }
 #define FDM_INTERIOR(TX_, TY_, HET_, DISS_)                                                          \
   do {                                                                                               \
     if (full)                                                                                        \
@@ -543,7 +558,10 @@ void launch_pointwise(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   EventPair* ep = prof_begin(c, 0, s);
   dim3 block(64, c->tile_y);
   dim3 grid((c->nx + 63) / 64, (c->ny + c->tile_y - 1) / c->tile_y);
-  hipLaunchKernelGGL(bulk_kernel<T>, grid, block, 0, s, a);
+  if (c->mode == 3)
+    hipLaunchKernelGGL(bulk_kernel3<T>, grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL(bulk_kernel<T>, grid, block, 0, s, a);
   prof_end(c, ep, 0, s);
 }
 
@@ -723,8 +741,21 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     return set_err(nullptr, FDM_EDEVICE, "fdm_create: no HIP device available");
   if (P->source.nplace < 0 || P->source.nplace > FDM_MAX_SOURCE_PLACES)
     return set_err(nullptr, FDM_EINVAL, "fdm_create: bad source placement count");
+  if (P->mode != 0 && P->mode != 2 && P->mode != 3)
+    return set_err(nullptr, FDM_EINVAL, "fdm_create: mode must be 2 or 3");
+  if (P->mode == 3) {
+    bool unsupported = P->source.nplace > 0 || P->mat_rho;
+    for (int b = 0; b < P->nblocks; b++)
+      unsupported |= P->blocks[b].plastic || P->blocks[b].mms || P->blocks[b].Cdiss > 0.0;
+    if (unsupported)
+      return set_err(nullptr, FDM_EINVAL,
+                     "fdm_create: mode 3 covers elastic homogeneous blocks without sources, "
+                     "plasticity, manufactured solutions or dissipation");
+  }
 
   fdm_ctx* c = new fdm_ctx();
+  c->mode = P->mode == 3 ? 3 : 2;
+  c->nF = c->mode == 3 ? 3 : 6;
   c->nx = P->nx;
   c->ny = P->ny;
   c->nb = P->nblocks;
@@ -974,7 +1005,8 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
       for (int k = 0; k < n; k++) {
         for (int q = 0; q < 5; q++) hM[(size_t)q * NS + o + k] = S.M[(size_t)q * n + k];
         for (int q = 0; q < 2; q++) hN[(size_t)q * NS + o + k] = S.normal[(size_t)q * n + k];
-        for (int q = 0; q < 6; q++) hF0[(size_t)q * NS + o + k] = S.F0[(size_t)q * n + k];
+        for (int q = 0; q < 6; q++)
+          hF0[(size_t)q * NS + o + k] = q < c->nF ? S.F0[(size_t)q * n + k] : 0.0;
         if (S.x && S.y) {
           hx[o + k] = S.x[k];
           hy[o + k] = S.y[k];
@@ -1166,13 +1198,15 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   // (6), metrics xq xr yq yr Ji (5); a heterogeneous medium adds rho, G, M,
   // gamma, nu (5)
   c->het = P->mat_rho && P->mat_cs && P->mat_cp;
-  if (c->het || c->diss) c->tile_shape = 0;
+  if (c->het || c->diss || c->mode == 3) c->tile_shape = 0;
   if (c->diss && !P->J) {
     set_err(c, FDM_EINVAL, "fdm_create: artificial dissipation needs the Jacobian J");
     return fail(FDM_EINVAL);
   }
-  // artificial dissipation adds the Jacobian J (1)
-  const int nplanes = (c->het ? 28 : 23) + (c->diss ? 1 : 0);
+  // artificial dissipation adds the Jacobian J (1); antiplane shear has 3
+  // fields instead of 6 (14 planes without the extras)
+  const int nF = c->nF, pmet = 3 * nF, pmat = pmet + 5, pJ = pmat + (c->het ? 5 : 0);
+  const int nplanes = pJ + (c->diss ? 1 : 0);
   void* arena = nullptr;
   const size_t slack = 64 * c->esz;  // the last plane extends coff elements past nplanes * ps
   if ((rc = dalloc(c, &arena, nplanes * pbytes + slack))) return fail(rc);
@@ -1194,7 +1228,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     }
     c->hnu = m[4];
     for (int k = 0; k < 5; k++) {
-      c->mat[k] = ab + (23 + k) * pbytes;
+      c->mat[k] = ab + (size_t)(pmat + k) * pbytes;
       if (c->precision == FDM_FP64) {
         std::vector<double> h;
         pack_plane(h, m[k].data(), c->nx, c->ny, c->ld);
@@ -1207,7 +1241,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     }
   }
   if (c->diss) {
-    c->Jp = ab + (size_t)(c->het ? 28 : 23) * pbytes;
+    c->Jp = ab + (size_t)pJ * pbytes;
     if (c->precision == FDM_FP64) {
       std::vector<double> h;
       pack_plane(h, P->J, c->nx, c->ny, c->ld);
@@ -1219,9 +1253,9 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     }
   }
   c->F[0] = ab;
-  c->F[1] = ab + 6 * pbytes;
-  c->DF = ab + 12 * pbytes;
-  for (int q = 0; q < 5; q++) c->met[q] = ab + (18 + q) * pbytes;
+  c->F[1] = ab + (size_t)nF * pbytes;
+  c->DF = ab + (size_t)(2 * nF) * pbytes;
+  for (int q = 0; q < 5; q++) c->met[q] = ab + (size_t)(pmet + q) * pbytes;
   const double* mets[5] = {P->xq, P->xr, P->yq, P->yr, P->Ji};
   for (int q = 0; q < 5; q++) {
     if (c->precision == FDM_FP64) {
@@ -1259,12 +1293,12 @@ int fdm_set_fields(fdm_ctx* c, const double* F, const double* DF) {
   c->df6_stale = false;
   size_t pe = (size_t)c->ps;
   if (c->precision == FDM_FP64) {
-    std::vector<double> h(6 * pe, 0.0);
-    for (int l = 0; l < 6; l++)
+    std::vector<double> h((size_t)c->nF * pe, 0.0);
+    for (int l = 0; l < c->nF; l++)
       for (int j = 0; j < c->ny; j++)
         memcpy(&h[l * pe + (size_t)j * c->ld], F + ((size_t)l * c->ny + j) * c->nx, c->nx * 8);
     HIPCHK(c, hipMemcpy(c->F[c->cur], h.data(), h.size() * 8, hipMemcpyHostToDevice));
-    for (int l = 0; l < 6; l++)
+    for (int l = 0; l < c->nF; l++)
       for (int j = 0; j < c->ny; j++)
         for (int i = 0; i < c->nx; i++)
           h[l * pe + (size_t)j * c->ld + i] = DF ? DF[((size_t)l * c->ny + j) * c->nx + i] : 1e40;
@@ -1278,21 +1312,21 @@ int fdm_set_fields(fdm_ctx* c, const double* F, const double* DF) {
       for (int i = 0; i < c->nx; i++) {
         int b = block_of(c, i, j);
         cnt[b] += 1.0;
-        for (int l = 0; l < 6; l++) sum[(size_t)b * 6 + l] += F[((size_t)l * c->ny + j) * c->nx + i];
+        for (int l = 0; l < c->nF; l++) sum[(size_t)b * 6 + l] += F[((size_t)l * c->ny + j) * c->nx + i];
       }
     for (int b = 0; b < c->nb; b++)
-      for (int l = 0; l < 6; l++)
+      for (int l = 0; l < c->nF; l++)
         c->hblocks[b].Fb[l] = cnt[b] > 0 ? (double)(float)(sum[(size_t)b * 6 + l] / cnt[b]) : 0.0;
     HIPCHK(c, hipMemcpy(c->blocks, c->hblocks.data(), c->hblocks.size() * sizeof(BlockDev),
                         hipMemcpyHostToDevice));
-    std::vector<float> h(6 * pe, 0.f);
-    for (int l = 0; l < 6; l++)
+    std::vector<float> h((size_t)c->nF * pe, 0.f);
+    for (int l = 0; l < c->nF; l++)
       for (int j = 0; j < c->ny; j++)
         for (int i = 0; i < c->nx; i++)
           h[l * pe + (size_t)j * c->ld + i] =
               (float)(F[((size_t)l * c->ny + j) * c->nx + i] - c->hblocks[block_of(c, i, j)].Fb[l]);
     HIPCHK(c, hipMemcpy(c->F[c->cur], h.data(), h.size() * 4, hipMemcpyHostToDevice));
-    for (int l = 0; l < 6; l++)
+    for (int l = 0; l < c->nF; l++)
       for (int j = 0; j < c->ny; j++)
         for (int i = 0; i < c->nx; i++) {
           double v = DF ? DF[((size_t)l * c->ny + j) * c->nx + i] : 3.0e38;
@@ -1312,17 +1346,17 @@ int fdm_get_fields(fdm_ctx* c, double* F, double* DF) {
     if (!dst) continue;
     const void* src = which == 0 ? c->F[c->cur] : c->DF;
     if (c->precision == FDM_FP64) {
-      std::vector<double> h(6 * pe);
+      std::vector<double> h((size_t)c->nF * pe);
       HIPCHK(c, hipMemcpy(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost));
-      if (which == 1 && c->df6_stale) restore_df6(c, h.data());
-      for (int l = 0; l < 6; l++)
+      if (which == 1 && c->df6_stale && c->mode == 2) restore_df6(c, h.data());
+      for (int l = 0; l < c->nF; l++)
         for (int j = 0; j < c->ny; j++)
           memcpy(dst + ((size_t)l * c->ny + j) * c->nx, &h[l * pe + (size_t)j * c->ld], c->nx * 8);
     } else {
-      std::vector<float> h(6 * pe);
+      std::vector<float> h((size_t)c->nF * pe);
       HIPCHK(c, hipMemcpy(h.data(), src, h.size() * 4, hipMemcpyDeviceToHost));
-      if (which == 1 && c->df6_stale) restore_df6(c, h.data());
-      for (int l = 0; l < 6; l++)
+      if (which == 1 && c->df6_stale && c->mode == 2) restore_df6(c, h.data());
+      for (int l = 0; l < c->nF; l++)
         for (int j = 0; j < c->ny; j++)
           for (int i = 0; i < c->nx; i++) {
             double v = (double)h[l * pe + (size_t)j * c->ld + i];
@@ -1704,11 +1738,11 @@ int fdm_ensemble_create(fdm_ctx* const* members, int32_t n, fdm_ensemble** out) 
     if (!c) return FDM_EINVAL;
     const fdm_ctx* c0 = members[0];
     if (c->src.nplace > 0 || c->any_mms || (!c->ld_T.empty() && c->ld_T[0] > 0.0) || c->bulk_mode != 2 ||
-        c->het || c->diss || c->nstage != c0->nstage || c->precision != c0->precision ||
+        c->het || c->diss || c->mode != 2 || c->nstage != c0->nstage || c->precision != c0->precision ||
         c->device != c0->device) {
       g_create_error =
           "fdm_ensemble_create: members must have no point source, no manufactured solution, "
-          "no load ramp, no heterogeneous medium, no artificial dissipation, the tile kernels, "
+          "no load ramp, no heterogeneous medium, no artificial dissipation, plane strain, the tile kernels, "
           "and the same RK order, precision and device";
       return FDM_EINVAL;
     }

@@ -930,4 +930,330 @@ __global__ void __launch_bounds__(64) bulk_band_batch(const BulkArgs<T>* __restr
   point_generic<T, FULL>(tab[m], p.x, p.y);
 }
 
+// ---------------------------------------------------------------------
+// Antiplane shear (mode 3): fields vz, sxz, syz (rates_interior_mode3,
+// rates_boundary_mode3, rates.f90:8-381).  DF1 pairs sxz with the
+// y metrics and syz with the x metrics, scaled by DI/rho; DF2 takes vz with
+// the y metrics and DF3 vz with the x metrics, scaled by DI*G.  The terms a
+// field lacks are absent from its sum; the rest keep the order of the
+// plane-strain expressions.
+// ---------------------------------------------------------------------
+
+// interior point: a(s) = DI(s)/rho, b(s) = DI(s)*G (rates.f90:24-28)
+template <typename T, typename Acc>
+__device__ __forceinline__ void interior_rates3(T* d, const Acc& A, const T (&as)[2], const T (&bs)[2],
+                                                T Ji) {
+#pragma unroll
+  for (int s = 1; s <= 2; s++) {
+    const T ayp = (T)0.5 * (A.yr(s) + A.yr(0)), axp = (T)0.5 * (A.xr(s) + A.xr(0));
+    const T aym = (T)0.5 * (A.yr(-s) + A.yr(0)), axm = (T)0.5 * (A.xr(-s) + A.xr(0));
+    const T byp = (T)0.5 * (A.yq(s) + A.yq(0)), bxp = (T)0.5 * (A.xq(s) + A.xq(0));
+    const T bym = (T)0.5 * (A.yq(-s) + A.yq(0)), bxm = (T)0.5 * (A.xq(-s) + A.xq(0));
+    const T t1 = ayp * A.f(1, s, 0) - axp * A.f(2, s, 0) - aym * A.f(1, -s, 0) + axm * A.f(2, -s, 0) -
+                 byp * A.f(1, 0, s) + bxp * A.f(2, 0, s) + bym * A.f(1, 0, -s) - bxm * A.f(2, 0, -s);
+    d[0] = d[0] + t1 * as[s - 1] * Ji;
+    const T t2 = ayp * A.f(0, s, 0) - aym * A.f(0, -s, 0) - byp * A.f(0, 0, s) + bym * A.f(0, 0, -s);
+    d[1] = d[1] + t2 * bs[s - 1] * Ji;
+    const T t3 = -(axp * A.f(0, s, 0)) + axm * A.f(0, -s, 0) + bxp * A.f(0, 0, s) - bxm * A.f(0, 0, -s);
+    d[2] = d[2] + t3 * bs[s - 1] * Ji;
+  }
+}
+
+// one-sided q, one-sided r, central q, central r terms of field l
+template <typename T, int l>
+__device__ __forceinline__ T q1_3(const T* __restrict__ F, const T* __restrict__ yr, const T* __restrict__ xr,
+                                  long long ps, long long kk, long long cc) {
+  const T ay = (T)0.5 * (yr[kk] + yr[cc]), ax = (T)0.5 * (xr[kk] + xr[cc]);
+  if constexpr (l == 0) return ay * F[ps + kk] - ax * F[2 * ps + kk];
+  else if constexpr (l == 1) return ay * F[kk];
+  else return -(ax * F[kk]);
+}
+
+template <typename T, int l>
+__device__ __forceinline__ T r1_3(const T* __restrict__ F, const T* __restrict__ yq, const T* __restrict__ xq,
+                                  long long ps, long long kk, long long cc) {
+  const T by = (T)0.5 * (yq[kk] + yq[cc]), bx = (T)0.5 * (xq[kk] + xq[cc]);
+  if constexpr (l == 0) return -(by * F[ps + kk]) + bx * F[2 * ps + kk];
+  else if constexpr (l == 1) return -(by * F[kk]);
+  else return bx * F[kk];
+}
+
+template <typename T, int l>
+__device__ __forceinline__ T qc_3(const T* __restrict__ F, const T* __restrict__ yr, const T* __restrict__ xr,
+                                  long long ps, long long kp, long long km, long long cc) {
+  const T ayp = (T)0.5 * (yr[kp] + yr[cc]), axp = (T)0.5 * (xr[kp] + xr[cc]);
+  const T aym = (T)0.5 * (yr[km] + yr[cc]), axm = (T)0.5 * (xr[km] + xr[cc]);
+  if constexpr (l == 0)
+    return ayp * F[ps + kp] - axp * F[2 * ps + kp] - aym * F[ps + km] + axm * F[2 * ps + km];
+  else if constexpr (l == 1) return ayp * F[kp] - aym * F[km];
+  else return -(axp * F[kp]) + axm * F[km];
+}
+
+template <typename T, int l>
+__device__ __forceinline__ T rc_3(const T* __restrict__ F, const T* __restrict__ yq, const T* __restrict__ xq,
+                                  long long ps, long long kp, long long km, long long cc) {
+  const T byp = (T)0.5 * (yq[kp] + yq[cc]), bxp = (T)0.5 * (xq[kp] + xq[cc]);
+  const T bym = (T)0.5 * (yq[km] + yq[cc]), bxm = (T)0.5 * (xq[km] + xq[cc]);
+  if constexpr (l == 0)
+    return -(byp * F[ps + kp]) + bxp * F[2 * ps + kp] + bym * F[ps + km] - bxm * F[2 * ps + km];
+  else if constexpr (l == 1) return -(byp * F[kp]) + bym * F[km];
+  else return bxp * F[kp] - bxm * F[km];
+}
+
+// ")*D/rho*JacInv" for DF1, ")*D*G*JacInv" for DF2 and DF3
+template <typename T, int l>
+__device__ __forceinline__ T bscale3(T t, T D, T rho, T G) {
+  if constexpr (l == 0) return t * D / rho;
+  else return t * D * G;
+}
+
+// SAT (DF = DF+Ks*(Fhat-F), time_step.f90:829-830) and the closures of a
+// point within four points of its block's edge
+template <typename T>
+__device__ __forceinline__ void closure_rates3(const BulkArgs<T>& a, const BlockDev& B, const int i,
+                                               const int j, const long long cc, T (&d)[6]) {
+  const long long ps = a.ps;
+  const int ld = a.ld;
+  const T* __restrict__ F = a.Fin;
+  const int dl = i - B.mx, dr = B.px - i, db = j - B.my, dtp = B.py - j;
+  if (dl == 0 || dr == 0 || db == 0 || dtp == 0) {
+    T Fe[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) Fe[l] = F[l * ps + cc];
+    const int NS = a.sd.NS;
+    const int sides[4] = {dl == 0 ? B.side_off[FDM_SIDE_L] + db : -1,
+                          dr == 0 ? B.side_off[FDM_SIDE_R] + db : -1,
+                          db == 0 ? B.side_off[FDM_SIDE_B] + dl : -1,
+                          dtp == 0 ? B.side_off[FDM_SIDE_T] + dl : -1};
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int so = sides[e];
+      if (so < 0) continue;
+      const T Ks = (T)a.sd.M[3 * NS + so];
+#pragma unroll
+      for (int l = 0; l < 3; l++) d[l] = d[l] + Ks * ((T)a.sd.Fhat[l * NS + so] - Fe[l]);
+    }
+  }
+  const T* __restrict__ xq = a.xq;
+  const T* __restrict__ xr = a.xr;
+  const T* __restrict__ yq = a.yq;
+  const T* __restrict__ yr = a.yr;
+  const T Ji = a.Ji[cc];
+  const T rho = (T)B.rho, G = (T)B.G;
+  const bool qone = (dl < 4) || (dr < 4);
+  const bool rone = (db < 4) || (dtp < 4);
+  const long long row = (long long)j * ld;
+  const long long colm = (long long)B.my * ld + i, colp = (long long)B.py * ld + i;
+  if (qone) {
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      const long long kq = (dl < 4) ? row + B.mx + s : row + B.px - s;
+      const T Dq = (dl < 4) ? (T)kDL[dl][s] : (T)(-kDL[dr][s]);
+      d[0] = d[0] + bscale3<T, 0>(q1_3<T, 0>(F, yr, xr, ps, kq, cc), Dq, rho, G) * Ji;
+      d[1] = d[1] + bscale3<T, 1>(q1_3<T, 1>(F, yr, xr, ps, kq, cc), Dq, rho, G) * Ji;
+      d[2] = d[2] + bscale3<T, 2>(q1_3<T, 2>(F, yr, xr, ps, kq, cc), Dq, rho, G) * Ji;
+      if (rone) {
+        const long long kr = (db < 4) ? colm + (long long)s * ld : colp - (long long)s * ld;
+        const T Dr = (db < 4) ? (T)kDL[db][s] : (T)(-kDL[dtp][s]);
+        d[0] = d[0] + bscale3<T, 0>(r1_3<T, 0>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
+        d[1] = d[1] + bscale3<T, 1>(r1_3<T, 1>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
+        d[2] = d[2] + bscale3<T, 2>(r1_3<T, 2>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
+      }
+    }
+    if (!rone) {
+#pragma unroll
+      for (int s = 1; s <= 2; s++) {
+        const T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
+        const long long kp = cc + (long long)s * ld, km = cc - (long long)s * ld;
+        d[0] = d[0] + bscale3<T, 0>(rc_3<T, 0>(F, yq, xq, ps, kp, km, cc), D, rho, G) * Ji;
+        d[1] = d[1] + bscale3<T, 1>(rc_3<T, 1>(F, yq, xq, ps, kp, km, cc), D, rho, G) * Ji;
+        d[2] = d[2] + bscale3<T, 2>(rc_3<T, 2>(F, yq, xq, ps, kp, km, cc), D, rho, G) * Ji;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      const long long kr = (db < 4) ? colm + (long long)s * ld : colp - (long long)s * ld;
+      const T Dr = (db < 4) ? (T)kDL[db][s] : (T)(-kDL[dtp][s]);
+      d[0] = d[0] + bscale3<T, 0>(r1_3<T, 0>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
+      d[1] = d[1] + bscale3<T, 1>(r1_3<T, 1>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
+      d[2] = d[2] + bscale3<T, 2>(r1_3<T, 2>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
+    }
+#pragma unroll
+    for (int s = 1; s <= 2; s++) {
+      const T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
+      const long long kp = cc + s, km = cc - s;
+      d[0] = d[0] + bscale3<T, 0>(qc_3<T, 0>(F, yr, xr, ps, kp, km, cc), D, rho, G) * Ji;
+      d[1] = d[1] + bscale3<T, 1>(qc_3<T, 1>(F, yr, xr, ps, kp, km, cc), D, rho, G) * Ji;
+      d[2] = d[2] + bscale3<T, 2>(qc_3<T, 2>(F, yr, xr, ps, kp, km, cc), D, rho, G) * Ji;
+    }
+  }
+}
+
+// one grid point of one stage in antiplane shear, neighbours from global
+// memory: A*DF, SAT, rates, update (fields.f90:1117-1133, 1225-1242)
+template <typename T>
+__device__ __forceinline__ void point_generic3(const BulkArgs<T>& a, const int i, const int j) {
+  const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
+  const BlockDev& B = a.blocks[bi];
+  const long long ps = a.ps;
+  const int ld = a.ld;
+  const long long cc = (long long)j * ld + i;
+  T d[6];
+  if (a.a_zero) {
+#pragma unroll
+    for (int l = 0; l < 3; l++) d[l] = (T)0.0;
+  } else {
+    const T A = (T)a.A;
+#pragma unroll
+    for (int l = 0; l < 3; l++) d[l] = A * a.DF[l * ps + cc];
+  }
+  if (i - B.mx >= 4 && B.px - i >= 4 && j - B.my >= 4 && B.py - j >= 4) {
+    const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld, nullptr};
+    const T as[2] = {(T)B.cI[0][0], (T)B.cI[0][1]};
+    const T bs[2] = {(T)B.cI[3][0], (T)B.cI[3][1]};
+    interior_rates3<T>(d, acc, as, bs, a.Ji[cc]);
+  } else {
+    closure_rates3<T>(a, B, i, j, cc, d);
+  }
+  if (a.do_update) {
+    const T Bdt = (T)a.Bdt;
+#pragma unroll
+    for (int l = 0; l < 3; l++) a.Fout[l * ps + cc] = a.Fin[l * ps + cc] + Bdt * d[l];
+  }
+#pragma unroll
+  for (int l = 0; l < 3; l++) a.DF[l * ps + cc] = d[l];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) bulk_kernel3(BulkArgs<T> a) {
+  int tx, ty;
+  tile_of(a.xcd_swizzle, tx, ty);
+  const int i = tx * blockDim.x + threadIdx.x;
+  const int j = ty * blockDim.y + threadIdx.y;
+  if (i >= a.nx || j >= a.ny) return;
+  point_generic3<T>(a, i, j);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) bulk_band3(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npts) return;
+  const int2 p = pts[k];
+  point_generic3<T>(a, p.x, p.y);
+}
+
+// interior tiles in antiplane shear: vz, sxz, syz with the two-point halo,
+// the metrics as in the plane-strain kernel, one barrier
+template <typename T, int TX, int TY, bool NTS>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4)))
+bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
+  constexpr int HX = TX + 4, HY = TY + 4, NT = TX * TY;
+  constexpr int NF = (HY * HX + NT - 1) / NT;
+  constexpr int NR = (TY * HX + NT - 1) / NT;
+  constexpr int NQ = (HY * TX + NT - 1) / NT;
+  __shared__ T sF[3][HY][HX];
+  __shared__ T sR[2][TY][HX];
+  __shared__ T sQ[2][HY][TX];
+  const int4 td = tiles[linear_tile(a.xcd_swizzle)];
+  const int bu = td.x, i0 = td.y, j0 = td.z;
+  ConstBlockDev* B4 = (ConstBlockDev*)a.blocks + bu;
+  const int tid = threadIdx.y * TX + threadIdx.x;
+  const int ld = a.ld, nx1 = a.nx - 1, ny1 = a.ny - 1;
+  const long long ps = a.ps;
+  const T* __restrict__ F = a.Fin;
+  T vF[NF][3], vR[NR][2], vQ[NQ][2];
+#pragma unroll
+  for (int p = 0; p < NF; p++) {
+    const int k = tid + p * NT;
+    if (k < HY * HX) {
+      const int r = k / HX, c = k - r * HX;
+      const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(max(i0 - 2 + c, 0), nx1);
+      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+#pragma unroll
+      for (int l = 0; l < 3; l++) vF[p][l] = ld_off(F + l * ps, o);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NR; p++) {
+    const int k = tid + p * NT;
+    if (k < TY * HX) {
+      const int r = k / HX, c = k - r * HX;
+      const int gj = min(j0 + r, ny1), gi = min(max(i0 - 2 + c, 0), nx1);
+      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+      vR[p][0] = ld_off(a.xr, o);
+      vR[p][1] = ld_off(a.yr, o);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NQ; p++) {
+    const int k = tid + p * NT;
+    if (k < HY * TX) {
+      const int r = k / TX, c = k - r * TX;
+      const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(i0 + c, nx1);
+      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+      vQ[p][0] = ld_off(a.xq, o);
+      vQ[p][1] = ld_off(a.yq, o);
+    }
+  }
+  const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
+  const bool valid = i <= B4->px - 4 && j <= B4->py - 4;
+  const unsigned oc = (unsigned)(min(j, ny1) * ld + min(i, nx1)) * (unsigned)sizeof(T);
+  T dfa[3];
+  if (!a.a_zero) {
+#pragma unroll
+    for (int l = 0; l < 3; l++) dfa[l] = ld_off(a.DF + l * ps, oc);
+  }
+  const T Ji = ld_off(a.Ji, oc);
+#pragma unroll
+  for (int p = 0; p < NF; p++) {
+    const int k = tid + p * NT;
+    if (k < HY * HX) {
+      const int r = k / HX, c = k - r * HX;
+#pragma unroll
+      for (int l = 0; l < 3; l++) sF[l][r][c] = vF[p][l];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NR; p++) {
+    const int k = tid + p * NT;
+    if (k < TY * HX) {
+      const int r = k / HX, c = k - r * HX;
+      sR[0][r][c] = vR[p][0];
+      sR[1][r][c] = vR[p][1];
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NQ; p++) {
+    const int k = tid + p * NT;
+    if (k < HY * TX) {
+      const int r = k / TX, c = k - r * TX;
+      sQ[0][r][c] = vQ[p][0];
+      sQ[1][r][c] = vQ[p][1];
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  T d[6];
+  if (a.a_zero) {
+#pragma unroll
+    for (int l = 0; l < 3; l++) d[l] = (T)0.0;
+  } else {
+    const T A = (T)a.A;
+#pragma unroll
+    for (int l = 0; l < 3; l++) d[l] = A * dfa[l];
+  }
+  const TileAcc<T, TX, TY> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y, nullptr};
+  const T as[2] = {(T)B4->cI[0][0], (T)B4->cI[0][1]};
+  const T bs[2] = {(T)B4->cI[3][0], (T)B4->cI[3][1]};
+  interior_rates3<T>(d, acc, as, bs, Ji);
+  if (a.do_update) {
+    const T Bdt = (T)a.Bdt;
+#pragma unroll
+    for (int l = 0; l < 3; l++) st_sel<NTS>(a.Fout + l * ps, oc, acc.f(l, 0, 0) + Bdt * d[l]);
+  }
+#pragma unroll
+  for (int l = 0; l < 3; l++) st_sel<NTS>(a.DF + l * ps, oc, d[l]);
+}
+
 }  // namespace fdm

@@ -93,6 +93,8 @@ struct EdgeArgs {
   // when set, [3][NI]: slip velocity, state variable at the start of the
   // stage and normal stress of every frictional node (adjoint trajectories)
   double* trace;
+  // 2: plane strain; 3: antiplane shear (fields vz, sxz, syz)
+  int mode;
 };
 
 template <typename T>
@@ -100,6 +102,140 @@ __device__ __forceinline__ double stage_time(const EdgeArgs<T>& a) {
   if (!a.nstep) return a.t;
   const double t0 = a.t_init + (double)(*a.nstep - 1) * a.dt;
   return t0 + a.rkc * a.dt;
+}
+
+// interface node in antiplane shear: locked_interface_mode3
+// (interfaces.f90:676-712), or frictional_interface_mode3 (interfaces.f90:
+// 806-897) with the fault bookkeeping of the plane-strain path.  The normal
+// stress is the plus block's in-plane prestress resolved on the fault
+// (rotate_xy2nt_mat, geometry.f90:311-322), the stress transfer phip is zero,
+// and the initial fields arrive in their own order (interfaces.f90:798).
+template <typename T>
+__device__ __forceinline__ void edge_iface3(const EdgeArgs<T>& a, const IfaceDev& I, const int q,
+                                            const BlockDev& Bm, const BlockDev& Bp, const int som,
+                                            const int sop, const long long ccm, const long long ccp,
+                                            const double tst) {
+  const int NS = a.sd.NS, NI = a.fi.NI;
+  double Fm[3], Fp[3], F0m[3], F0p[3], Fhm[3], Fhp[3];
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    Fm[l] = (double)a.F[l * a.ps + ccm];
+    Fp[l] = (double)a.F[l * a.ps + ccp];
+    F0m[l] = a.sd.F0[l * NS + som];
+    F0p[l] = a.sd.F0[l * NS + sop];
+  }
+  if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      Fm[l] = Fm[l] + Bm.Fb[l];
+      Fp[l] = Fp[l] + Bp.Fb[l];
+    }
+  }
+  const double n1 = a.fi.nhat[q], n2 = a.fi.nhat[NI + q];
+  const double Zsm = a.sd.M[som], Zsp = a.sd.M[sop];
+  if (I.coupling == FDM_COUPLING_LOCKED) {
+    locked_interface_mode3(Fhm, Fhp, Fm, Fp, n1, n2, Zsm, Zsp);
+  } else {
+    double* st = a.fi.st;
+#define ST(f) st[(long long)(f) * NI + q]
+    if (a.do_scale) {
+      ST(FDM_IF_DDS) = a.A * ST(FDM_IF_DDS);
+      ST(FDM_IF_DDN) = a.A * ST(FDM_IF_DDN);
+      ST(FDM_IF_DPSI) = a.A * ST(FDM_IF_DPSI);
+      ST(FDM_IF_DW) = a.A * ST(FDM_IF_DW);
+      ST(FDM_IF_DNREG) = a.A * ST(FDM_IF_DNREG);
+    }
+    double vz0p, stz0p, snz0p, vz0m, stz0m, snz0m;
+    xy2nt3(F0p, n1, n2, vz0p, stz0p, snz0p);
+    xy2nt3(F0m, n1, n2, vz0m, stz0m, snz0m);
+    double dp[3], dm[3];
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      dp[l] = Fp[l] - F0p[l];
+      dm[l] = Fm[l] - F0m[l];
+    }
+    double vzFDp, stzFDp, snzFDp, vzFDm, stzFDm, snzFDm;
+    xy2nt3(dp, n1, n2, vzFDp, stzFDp, snzFDp);
+    xy2nt3(dm, n1, n2, vzFDm, stzFDm, snzFDm);
+    const double Zsip = 1.0 / Zsp, Zsim = 1.0 / Zsm;
+    const double etas = 1.0 / (Zsip + Zsim);
+    const double phis = etas * (snzFDp * Zsip + snzFDm * Zsim + vzFDp - vzFDm);
+    const double S0 = snz0p;
+    const double snn = 2.0 * n1 * n2 * Bp.s0[1] + n1 * n1 * Bp.s0[0] + n2 * n2 * Bp.s0[3];
+    const double N0 = -snn - 0.0;
+    const double phip = 0.0;
+    const double S0l = a.fi.S0b[q] + a.fi.ldS[q] * a.Bload;
+    const double N0l = a.fi.N0b[q] + a.fi.ldN[q] * a.Bload;
+    const double fS0 = S0l + S0;
+    const double fN0 = N0l + N0;
+    ST(FDM_IF_S0) = fS0;
+    ST(FDM_IF_N0) = fN0;
+    double rs[7], sw[4];
+#pragma unroll
+    for (int c = 0; c < 7; c++) rs[c] = I.law >= FDM_LAW_SL ? a.fi.rs[(long long)c * NI + q] : 1.0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) sw[c] = I.law == FDM_LAW_SW ? a.fi.sw[(long long)c * NI + q] : 1.0;
+    double V = ST(FDM_IF_V), S = ST(FDM_IF_S), O = ST(FDM_IF_O), N = ST(FDM_IF_N),
+           Nreg = ST(FDM_IF_NREG);
+    const double D = ST(FDM_IF_D), Psi = ST(FDM_IF_PSI);
+    const double E = I.law >= FDM_LAW_RSL ? exp(Psi / rs[0]) : 0.0;
+    const bool fail = solve_friction_core(I.law, rs, sw, I.reg, a.fi.outside[q], I.flock, fS0, fN0,
+                                          phip, phis, etas, D, Psi, E, I.skempton, a.Bdt, V, S, O, N,
+                                          Nreg);
+    if (fail) *a.fail = 1;
+    ST(FDM_IF_V) = V;
+    ST(FDM_IF_S) = S;
+    ST(FDM_IF_O) = O;
+    ST(FDM_IF_N) = N;
+    ST(FDM_IF_NREG) = Nreg;
+    const double snzp = S - fS0, snzm = S - fS0;
+    const double vzp = vzFDp + Zsip * (snzFDp - snzp);
+    const double vzm = vzFDm - Zsim * (snzFDm - snzm);
+    nt2xy3(Fhp, n1, n2, vzp, stzFDp, snzp);
+    nt2xy3(Fhm, n1, n2, vzm, stzFDm, snzm);
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      Fhp[l] = Fhp[l] + F0p[l];
+      Fhm[l] = Fhm[l] + F0m[l];
+    }
+    // set_rates_friction (friction.f90:1101-1124)
+    ST(FDM_IF_DDS) = ST(FDM_IF_DDS) + V;
+    ST(FDM_IF_DDN) = ST(FDM_IF_DDN) + O;
+    ST(FDM_IF_DPSI) = ST(FDM_IF_DPSI) + state_rate(I.law, rs, V, Psi, E);
+    if (I.reg) {
+      const double absV = fabs(V);
+      const double Nr = -(absV / rs[4]) * (Nreg - fmax(N, 0.0));
+      ST(FDM_IF_DNREG) = ST(FDM_IF_DNREG) + Nr;
+    }
+    ST(FDM_IF_DW) = ST(FDM_IF_DW) + S * V;
+    const bool ruptured = I.rup_field == 1 ? (D > I.rup_threshold) : (fabs(V) > I.rup_threshold);
+    if (ruptured) ST(FDM_IF_TRUP) = fmin(ST(FDM_IF_TRUP), tst);
+    if (a.snapshot) {
+      for (int f = 0; f < FDM_IF_NFIELDS; f++) a.fi.st1[(long long)f * NI + q] = st[(long long)f * NI + q];
+    }
+    if (a.do_update) {
+      const double Bdt = a.Bdt;
+      ST(FDM_IF_DS) = ST(FDM_IF_DS) + Bdt * ST(FDM_IF_DDS);
+      ST(FDM_IF_DN) = ST(FDM_IF_DN) + Bdt * ST(FDM_IF_DDN);
+      ST(FDM_IF_D) = ST(FDM_IF_D) + Bdt * fabs(ST(FDM_IF_DDS));
+      ST(FDM_IF_PSI) = ST(FDM_IF_PSI) + Bdt * ST(FDM_IF_DPSI);
+      ST(FDM_IF_W) = ST(FDM_IF_W) + Bdt * ST(FDM_IF_DW);
+      ST(FDM_IF_NREG) = ST(FDM_IF_NREG) + Bdt * ST(FDM_IF_DNREG);
+    }
+#undef ST
+  }
+  if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      Fhm[l] = Fhm[l] - Bm.Fb[l];
+      Fhp[l] = Fhp[l] - Bp.Fb[l];
+    }
+  }
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    a.sd.Fhat[l * NS + som] = Fhm[l];
+    a.sd.Fhat[l * NS + sop] = Fhp[l];
+  }
 }
 
 template <typename T>
@@ -117,6 +253,27 @@ __device__ __forceinline__ void edge_task(const EdgeArgs<T>& a, const int2 task)
     else { i = B.mx + k; j = B.py; }
     int so = B.side_off[s] + k;
     long long cc = (long long)j * a.ld + i;
+    if (a.mode == 3) {
+      // set_bc_mode3 (boundaries.f90:148-149, 155-213)
+      double F3[3], F03[3], Fh3[3];
+#pragma unroll
+      for (int l = 0; l < 3; l++) {
+        F3[l] = (double)a.F[l * a.ps + cc];
+        F03[l] = a.sd.F0[l * NS + so];
+      }
+      if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int l = 0; l < 3; l++) F3[l] = F3[l] + B.Fb[l];
+      }
+      set_bc_mode3(Fh3, F3, F03, a.sd.normal[so], a.sd.normal[NS + so], B.side_bc[s], a.sd.M[so]);
+      if constexpr (sizeof(T) == 4) {
+#pragma unroll
+        for (int l = 0; l < 3; l++) Fh3[l] = Fh3[l] - B.Fb[l];
+      }
+#pragma unroll
+      for (int l = 0; l < 3; l++) a.sd.Fhat[l * NS + so] = Fh3[l];
+      return;
+    }
     double F[6], F0[6], Fh[6];
 #pragma unroll
     for (int l = 0; l < 6; l++) {
@@ -158,6 +315,10 @@ __device__ __forceinline__ void edge_task(const EdgeArgs<T>& a, const int2 task)
   }
   int som = Bm.side_off[sm] + km, sop = Bp.side_off[sp] + kp;
   long long ccm = (long long)jm * a.ld + im, ccp = (long long)jp * a.ld + ip;
+  if (a.mode == 3) {
+    edge_iface3<T>(a, I, q, Bm, Bp, som, sop, ccm, ccp, tst);
+    return;
+  }
   double Fm[6], Fp[6], F0m[6], F0p[6], Fhm[6], Fhp[6];
 #pragma unroll
   for (int l = 0; l < 6; l++) {

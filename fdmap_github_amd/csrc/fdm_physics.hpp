@@ -157,6 +157,71 @@ __device__ __forceinline__ void set_bc_mode2(double* Fhat, const double* F, cons
   }
 }
 
+// ---- antiplane shear (mode 3): vz, sxz, syz ----
+
+// rotate_fields_xy2nt_mode3 (fields.f90:1297-1309): vz, and (sxz, syz) as a
+// vector (rotate_xy2nt_vec, geometry.f90:285-295)
+__device__ __forceinline__ void xy2nt3(const double* F, double n1, double n2, double& vz,
+                                       double& stz, double& snz) {
+  vz = F[0];
+  stz = n2 * F[1] - n1 * F[2];
+  snz = n1 * F[1] + n2 * F[2];
+}
+
+// rotate_fields_nt2xy_mode3 (fields.f90:1328-1340, geometry.f90:298-308)
+__device__ __forceinline__ void nt2xy3(double* F, double n1, double n2, double vz, double stz,
+                                       double snz) {
+  F[0] = vz;
+  F[1] = n2 * stz + n1 * snz;
+  F[2] = -n1 * stz + n2 * snz;
+}
+
+// set_bc_mode3 (boundaries.f90:155-213): absorbing, free or rigid
+__device__ __forceinline__ void set_bc_mode3(double* Fhat, const double* F, const double* F0,
+                                             double n1, double n2, int bc, double Zs) {
+  if (Zs == 0.0) {
+#pragma unroll
+    for (int l = 0; l < 3; l++) Fhat[l] = F[l];
+    return;
+  }
+  const double Zsi = 1.0 / Zs;
+  double d[3], vzFD, stzFD, snzFD;
+#pragma unroll
+  for (int l = 0; l < 3; l++) d[l] = F[l] - F0[l];
+  xy2nt3(d, n1, n2, vzFD, stzFD, snzFD);
+  double vz = 0.0, snz = 0.0;
+  if (bc == FDM_BC_ABSORBING) {
+    vz = 0.5 * (vzFD - Zsi * snzFD);
+    snz = 0.5 * (snzFD - Zs * vzFD);
+  } else if (bc == FDM_BC_FREE) {
+    vz = vzFD - Zsi * snzFD;
+    snz = 0.0;
+  } else if (bc == FDM_BC_RIGID) {
+    vz = 0.0;
+    snz = snzFD - Zs * vzFD;
+  }
+  nt2xy3(Fhat, n1, n2, vz, stzFD, snz);
+#pragma unroll
+  for (int l = 0; l < 3; l++) Fhat[l] = Fhat[l] + F0[l];
+}
+
+// locked_interface_mode3 (interfaces.f90:676-712)
+__device__ __forceinline__ void locked_interface_mode3(double* Fhatm, double* Fhatp, const double* Fm,
+                                                       const double* Fp, double n1, double n2,
+                                                       double Zsm, double Zsp) {
+  double vzFDp, stzFDp, snzFDp, vzFDm, stzFDm, snzFDm;
+  xy2nt3(Fp, n1, n2, vzFDp, stzFDp, snzFDp);
+  xy2nt3(Fm, n1, n2, vzFDm, stzFDm, snzFDm);
+  const double Zsip = 1.0 / Zsp, Zsim = 1.0 / Zsm;
+  const double etas = 1.0 / (Zsip + Zsim);
+  const double phis = etas * (snzFDp * Zsip + snzFDm * Zsim + vzFDp - vzFDm);
+  const double snzp = phis, snzm = phis;
+  const double vzp = vzFDp + Zsip * (snzFDp - snzp);
+  const double vzm = vzFDm - Zsim * (snzFDm - snzm);
+  nt2xy3(Fhatp, n1, n2, vzp, stzFDp, snzp);
+  nt2xy3(Fhatm, n1, n2, vzm, stzFDm, snzm);
+}
+
 // locked_interface_mode2 (interfaces.f90:715-772)
 __device__ __forceinline__ void locked_interface_mode2(double* Fhatm, double* Fhatp,
                                                        const double* Fm, const double* Fp,

@@ -30,10 +30,12 @@ class Context:
             self._cprob = problem.cstruct()
             pp = C.pointer(self._cprob)
             self.nx, self.ny = problem.nx, problem.ny
+            self.nF = 3 if getattr(problem, "mode", 2) == 3 else 6
             self.ifaces_n = [itf["p"] - itf["m"] + 1 for itf in problem.ifaces]
         else:
             pp = problem
             self.nx, self.ny = pp.contents.nx, pp.contents.ny
+            self.nF = 3 if pp.contents.mode == 3 else 6
             self.ifaces_n = [pp.contents.ifaces[k].p - pp.contents.ifaces[k].m + 1
                              for k in range(pp.contents.nifaces)]
         h = C.c_void_p()
@@ -64,13 +66,14 @@ class Context:
 
     # ---- fields ----
     def set_fields(self, F, DF=None):
-        F = np.ascontiguousarray(F, dtype=np.float64).reshape(6, self.ny, self.nx)
-        DFa = None if DF is None else np.ascontiguousarray(DF, dtype=np.float64).reshape(6, self.ny, self.nx)
+        F = np.ascontiguousarray(F, dtype=np.float64).reshape(self.nF, self.ny, self.nx)
+        DFa = None if DF is None else np.ascontiguousarray(DF, dtype=np.float64).reshape(
+            self.nF, self.ny, self.nx)
         self._check(self._fn("set_fields")(self.h, capi.dptr(F), capi.dptr(DFa)))
 
     def get_fields(self):
-        F = np.empty((6, self.ny, self.nx))
-        DF = np.empty((6, self.ny, self.nx))
+        F = np.empty((self.nF, self.ny, self.nx))
+        DF = np.empty((self.nF, self.ny, self.nx))
         self._check(self._fn("get_fields")(self.h, capi.dptr(F), capi.dptr(DF)))
         return F, DF
 

@@ -6,12 +6,14 @@
 ! writes them through MPI-IO on one process:
 !   <name>.ckpt                    number of the latest checkpoint (list-
 !                                  directed integer; -1 once a run finished)
-!   <name>F.ckpt<n>                F(6 planes), DF(6 planes), then gammap and
-!                                  lambda when block 1 is plastic, each plane
-!                                  nx*ny, x fastest (fields.f90:335-429)
-!   <name>block<i><L|R|B|T>.ckpt<n>  per side: F, Fhat, F0 of each of the 6
-!                                  components, U and DU of 2 components, E,
-!                                  DE (fields.f90:432-644, 848-860)
+!   <name>F.ckpt<n>                F(nF planes), DF(nF planes), then gammap
+!                                  and lambda when block 1 is plastic, each
+!                                  plane nx*ny, x fastest (fields.f90:335-429);
+!                                  nF = 6 in plane strain, 3 in antiplane shear
+!   <name>block<i><L|R|B|T>.ckpt<n>  per side: F, Fhat, F0 of each of the nF
+!                                  components, U and DU of nU components (2,
+!                                  or 1 in antiplane shear), E, DE
+!                                  (fields.f90:158-175, 432-644, 848-860)
 !   <name>iface<k>side<1|2>.ckpt<n>  the 18 fault arrays of a frictional
 !                                  interface in checkpoint order (friction.f90:
 !                                  491-552; interfaces.f90:329-398); empty for
@@ -204,17 +206,23 @@ contains
     ! interfaces in order, 18]
     integer,intent(in) :: number
     real(dp),intent(in) :: F(:,:,:),DF(:,:,:),gp(:,:),lam(:,:),IF(:,:)
-    integer :: u,l,i,s,k,n,off,side
+    integer :: u,l,i,s,k,n,off,side,nF,nU
     logical :: ok
     character(1),parameter :: sn(4) = (/ 'L','R','B','T' /)
     real(dp),allocatable :: fill(:),zero(:)
 
+    nF = 6
+    nU = 2
+    if (pb_mode==3) then
+       nF = 3
+       nU = 1
+    end if
     call open_ckpt(u,ckname('F',number),ok)
     if (.not.ok) return
-    do l = 1,6
+    do l = 1,nF
        write(u) F(:,:,l)
     end do
-    do l = 1,6
+    do l = 1,nF
        write(u) DF(:,:,l)
     end do
     if (block1_plastic()) then
@@ -231,12 +239,12 @@ contains
           zero = 0.0_dp
           call open_ckpt(u,ckname('block'//itoa(i)//sn(s),number),ok)
           if (.not.ok) return
-          do l = 1,6
+          do l = 1,nF
              write(u) fill
              write(u) fill
              write(u) HB(i)%side(s)%F0(:,l)
           end do
-          do l = 1,2
+          do l = 1,nU
              write(u) zero
              write(u) fill
           end do
@@ -270,9 +278,11 @@ contains
     ! interface
     integer,intent(in) :: number
     real(dp),intent(out) :: F(:,:,:),DF(:,:,:),gp(:,:),lam(:,:),IF(:,:)
-    integer :: u,l,k,n,off,stat
+    integer :: u,l,k,n,off,stat,nF
     character(len=:),allocatable :: name
 
+    nF = 6
+    if (pb_mode==3) nF = 3
     gp = 0.0_dp
     lam = 0.0_dp
     name = ckname('F',number)
@@ -282,11 +292,11 @@ contains
        call fail('Error opening '//name)
        return
     end if
-    do l = 1,6
+    do l = 1,nF
        read(u,iostat=stat) F(:,:,l)
        if (stat/=0) exit
     end do
-    do l = 1,6
+    do l = 1,nF
        if (stat/=0) exit
        read(u,iostat=stat) DF(:,:,l)
     end do

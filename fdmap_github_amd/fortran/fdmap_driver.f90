@@ -219,7 +219,7 @@ contains
     call wm_r(u,'refine',refine)
     call wm_c(u,'method','LS','RK')
     call wm_i(u,'order',RKorder,'RK')
-    call wm_i(u,'mode',2)
+    call wm_i(u,'mode',pb_mode)
     call wm_c(u,'FDmethod','SBP4')
     call wm_i(u,'nblocks',nblocks)
     call wm_i(u,'nifaces',nifaces)
@@ -312,12 +312,18 @@ contains
        case('yq');  rec = real(gyq(mx:px,my:py),c_float)
        case('yr');  rec = real(gyr(mx:px,my:py),c_float)
        case('J');   rec = real(gJ(mx:px,my:py),c_float)
-       case('vx');  rec = real(F(mx:px,my:py,1),c_float)
-       case('vy');  rec = real(F(mx:px,my:py,2),c_float)
-       case('sxx'); rec = real(F(mx:px,my:py,3),c_float)
-       case('sxy'); rec = real(F(mx:px,my:py,4),c_float)
-       case('syy'); rec = real(F(mx:px,my:py,5),c_float)
-       case('szz'); rec = real(F(mx:px,my:py,6),c_float)
+       case('vx','vy','sxx','sxy','syy','szz','vz','sxz','syz')
+          ! the fields of this mode, in their order (fields.f90:158-175)
+          if (pb_mode==3) then
+             field = index('vz  sxz syz ',trim(it%field)//' ')
+          else
+             field = index('vx  vy  sxx sxy syy szz ',trim(it%field)//' ')
+          end if
+          if (field>0) then
+             rec = real(F(mx:px,my:py,(field-1)/4+1),c_float)
+          else
+             rec = 0.0
+          end if
        case('gammap'); rec = real(gp(mx:px,my:py),c_float)
        case('lambda'); rec = real(lam(mx:px,my:py),c_float)
        case default

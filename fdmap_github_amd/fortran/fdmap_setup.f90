@@ -72,8 +72,10 @@ module fdmap_setup
   integer :: nt,ninfo,RKorder
   logical :: constant_dt,solid
 
-  ! domain
+  ! domain; pb_mode 2: plane strain (vx vy sxx sxy syy szz), 3: antiplane
+  ! shear (vz sxz syz, stored in the first three field slots)
   integer :: nblocks_x,nblocks_y,nblocks,nifaces,gnx,gny
+  integer :: pb_mode = 2
   type(host_block),allocatable,target :: HB(:)
   type(host_iface),allocatable,target :: HI(:)
   real(dp),allocatable,target :: gcx(:,:),gcy(:,:),gxq(:,:),gxr(:,:),gyq(:,:),gyr(:,:),gJ(:,:),gJi(:,:)
@@ -111,6 +113,17 @@ contains
     end do
 100 found = .false.
   end subroutine seek_to_string
+
+
+  logical function mode3_bc(bc)
+    character(*),intent(in) :: bc
+    select case(trim(bc))
+    case('none','absorbing','free','rigid')
+       mode3_bc = .true.
+    case default
+       mode3_bc = .false.
+    end select
+  end function mode3_bc
 
 
   function bc_code(bc) result(code)
@@ -527,7 +540,8 @@ contains
        call fail('Error in domain_list')
        return
     end if
-    if (mode/=2) call fail('only mode = 2 (plane strain) is supported')
+    if (mode/=2.and.mode/=3) call fail('mode must be 2 (plane strain) or 3 (antiplane shear)')
+    pb_mode = mode
     if (trim(FDmethod)/='SBP4') call fail("only FDmethod = 'SBP4' is supported")
     if (exact_metric) call fail('exact_metric is not supported')
     if (energy_balance.or.displacement.or.peak.or.acoustic_gravity_waves.or. &
@@ -745,6 +759,11 @@ contains
          call seek_to_string(input,str,found)
          read(input,nml=boundaries_list,iostat=stat)
          if (stat>0) call fail(trim(str)//' :: Error in boundaries_list')
+         ! set_bc_mode3 knows absorbing, free and rigid (boundaries.f90:186-200)
+         if (pb_mode==3) then
+            if (.not.(mode3_bc(bcL).and.mode3_bc(bcR).and.mode3_bc(bcB).and.mode3_bc(bcT))) &
+                 call fail("mode 3 boundary conditions are 'absorbing', 'free' or 'rigid'")
+         end if
          HBk%side(1)%bc = bc_code(bcL)
          HBk%side(2)%bc = bc_code(bcR)
          HBk%side(3)%bc = bc_code(bcB)
@@ -779,12 +798,27 @@ contains
             szz0 = 0.5_dp*(sxx0+syy0)
          end if
          HBk%F0blk = (/ vx0,vy0,vz0,sxx0,sxy0,sxz0,syy0,syz0,szz0 /)
-         F0 = (/ vx0,vy0,sxx0,sxy0,syy0,szz0 /)
-         FP(:,1) = (/ P1%vx,P1%vy,P1%sxx,P1%sxy,P1%syy,P1%szz /)
-         FP(:,2) = (/ P2%vx,P2%vy,P2%sxx,P2%sxy,P2%syy,P2%szz /)
-         FP(:,3) = (/ P3%vx,P3%vy,P3%sxx,P3%sxy,P3%syy,P3%szz /)
-         FP(:,4) = (/ P4%vx,P4%vy,P4%sxx,P4%sxy,P4%syy,P4%szz /)
-         FP(:,5) = (/ P5%vx,P5%vy,P5%sxx,P5%sxy,P5%syy,P5%szz /)
+         if (pb_mode==3) then
+            ! fields.f90:167-174, 803-809
+            if (index(problem,'mms')>0.or.index(problem,'inplane')>0) &
+                 call fail("fields problem '"//trim(problem)//"' is not available in mode 3")
+            if (trim(response)=='plastic'.or.heterogeneous.or.HBk%Cdiss>0.0_dp) &
+                 call fail('mode 3 covers elastic homogeneous blocks without dissipation')
+            if (setup_error/='') return
+            F0 = (/ vz0,sxz0,syz0,0.0_dp,0.0_dp,0.0_dp /)
+            FP(:,1) = (/ P1%vz,P1%sxz,P1%syz,0.0_dp,0.0_dp,0.0_dp /)
+            FP(:,2) = (/ P2%vz,P2%sxz,P2%syz,0.0_dp,0.0_dp,0.0_dp /)
+            FP(:,3) = (/ P3%vz,P3%sxz,P3%syz,0.0_dp,0.0_dp,0.0_dp /)
+            FP(:,4) = (/ P4%vz,P4%sxz,P4%syz,0.0_dp,0.0_dp,0.0_dp /)
+            FP(:,5) = (/ P5%vz,P5%sxz,P5%syz,0.0_dp,0.0_dp,0.0_dp /)
+         else
+            F0 = (/ vx0,vy0,sxx0,sxy0,syy0,szz0 /)
+            FP(:,1) = (/ P1%vx,P1%vy,P1%sxx,P1%sxy,P1%syy,P1%szz /)
+            FP(:,2) = (/ P2%vx,P2%vy,P2%sxx,P2%sxy,P2%syy,P2%szz /)
+            FP(:,3) = (/ P3%vx,P3%vy,P3%sxx,P3%sxy,P3%syy,P3%szz /)
+            FP(:,4) = (/ P4%vx,P4%vy,P4%sxx,P4%sxy,P4%syy,P4%szz /)
+            FP(:,5) = (/ P5%vx,P5%vy,P5%sxx,P5%sxy,P5%syy,P5%szz /)
+         end if
 
          ! sides: curve coordinates, normals, F0, M (init_fields_side, init_edges)
          do s = 1,4
@@ -1238,6 +1272,10 @@ contains
     end select
     if (.not.((Fx/=0.0_dp.or.Fy/=0.0_dp.or.Fz/=0.0_dp).or. &
          (Mxx/=0.0_dp.or.Mxy/=0.0_dp.or.Myy/=0.0_dp.or.Mzx/=0.0_dp.or.Mzy/=0.0_dp))) return
+    if (pb_mode==3) then
+       call fail('point sources are not available in mode 3')
+       return
+    end if
     np = 0
     do ib = 1,nblocks
        associate(G => HB(ib)%G)
@@ -1292,6 +1330,7 @@ contains
     cprob%x = c_loc(gcx(1,1))
     cprob%y = c_loc(gcy(1,1))
     cprob%J = c_loc(gJ(1,1))
+    cprob%mode = pb_mode
     if (allocated(gmrho)) then
        cprob%mat_rho = c_loc(gmrho(1,1))
        cprob%mat_cs = c_loc(gmcs(1,1))
@@ -1337,8 +1376,14 @@ contains
          ! artificial dissipation scaled as (wave speed)/(grid spacing)
          ! (domain.f90:316-320)
          cb%Cdiss = HBk%Cdiss*CFL/dt
-         ! initial_stress (fields.f90:1007-1058), mode 2, default problem
-         cb%s0 = (/ 0.0_dp,0.0_dp,HBk%F0blk(6),0.0_dp,HBk%F0blk(8),0.0_dp /)
+         ! initial_stress (fields.f90:1007-1058), default problem: the stress
+         ! components that are not fields; in mode 3 the in-plane prestress
+         ! that frictional interfaces resolve onto the fault
+         if (pb_mode==3) then
+            cb%s0 = (/ HBk%F0blk(4),HBk%F0blk(5),0.0_dp,HBk%F0blk(7),0.0_dp,HBk%F0blk(9) /)
+         else
+            cb%s0 = (/ 0.0_dp,0.0_dp,HBk%F0blk(6),0.0_dp,HBk%F0blk(8),0.0_dp /)
+         end if
          do s = 1,4
             cb%side(s)%bc = HBk%side(s)%bc
             cb%side(s)%n = size(HBk%side(s)%x)

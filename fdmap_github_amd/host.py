@@ -75,8 +75,11 @@ class HostProblem:
         self.nifaces = self.ptr.contents.nifaces
 
     def fields(self):
+        """initial fields [nF, ny, nx]: 6 in plane strain, 3 (vz, sxz, syz) in
+        antiplane shear"""
         p = load().fdmap_host_fields()
-        return np.ctypeslib.as_array(p, shape=(6, self.ny, self.nx)).copy()
+        nF = 3 if self.ptr.contents.mode == 3 else 6
+        return np.ctypeslib.as_array(p, shape=(6, self.ny, self.nx))[:nF].copy()
 
     def coords(self, which):
         names = ["x", "y", "xq", "xr", "yq", "yr", "J", "Ji"]

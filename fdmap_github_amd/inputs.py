@@ -144,6 +144,72 @@ D V S N Psi
 """
 
 
+def antiplane_rs(name="m3", nx=301, ny=301, h=0.1, nsteps=1000, outputs=True, law="RAL",
+                 sxx0=-50.0, sxz0=30.0, a=0.010, b=0.016, L=0.05, Psi0=0.6, ld_S=5.0,
+                 ld_T=0.0, ld_width=4.0, ninfo=100):
+    """Antiplane shear (mode 3) rupture after the reference's mode-3 example
+    (problems/kali-heterogeneous-input.in): two blocks side by side with a
+    vertical rate-and-state fault between them, a free surface on top,
+    absorbing elsewhere; the fault normal stress is -sxx0 and the shear
+    prestress sxz0, with a Gaussian shear load ld_S at mid-depth."""
+    Lx = (nx - 1) * h
+    Ly = (ny - 1) * h
+    out = ""
+    if outputs:
+        st = max(1, nsteps // 100)
+        out = f"""I
+iface1
+D V S N Psi
+0d0 1d10 {st}
+B
+body
+vz sxz syz
+0d0 1d10 {max(1, nsteps // 10)}
+"""
+    return f"""&problem_list
+ name = '{name}', nt = {nsteps + 1}, CFL = 0.5d0, RKorder = 4, ninfo = {ninfo} /
+
+&domain_list
+ mode = 3, FDmethod = 'SBP4', nblocks_x = 2, nblocks_y = 1, nx = {2 * nx}, ny = {ny} /
+
+!---BLOCK1---
+
+&grid_list
+ nx = {nx}, ny = {ny}, mgx = 1, mgy = 1, iblock_x = 1, iblock_y = 1,
+ LT = {_f(-Lx)},0d0, RT = 0d0,0d0,
+ LB = {_f(-Lx)},{_f(-Ly)}, RB = 0d0,{_f(-Ly)} /
+
+&boundaries_list bcL = 'absorbing', bcR = 'none', bcB = 'absorbing', bcT = 'free' /
+
+!---BLOCK2---
+
+&grid_list
+ nx = {nx}, ny = {ny}, mgx = {nx + 1}, mgy = 1, iblock_x = 2, iblock_y = 1,
+ LT = 0d0,0d0, RT = {_f(Lx)},0d0,
+ LB = 0d0,{_f(-Ly)}, RB = {_f(Lx)},{_f(-Ly)} /
+
+&boundaries_list bcL = 'none', bcR = 'absorbing', bcB = 'absorbing', bcT = 'free' /
+
+!---ALL BLOCKS---
+
+&material_list rho = {_f(RHO)}, cs = {_f(CS)}, cp = {_f(CP)} /
+
+&fields_list sxx0 = {_f(sxx0)}, sxz0 = {_f(sxz0)} /
+
+!---IFACE1---
+
+&interface_list coupling = 'friction', iblockm = 1, iblockp = 2, direction = 'x', mg = 1, pg = {ny} /
+
+&friction_list
+ angle = 90d0, friction_law = '{law}', Psi0 = {_f(Psi0)},
+ rs = {_f(a)},{_f(b)},1d-6,0.6d0,{_f(L)},
+ ld = 'gaussian',{_f(ld_S)},0d0,{_f(ld_T)},{_f(ld_width)},{_f(-0.5 * Ly)},0d0 / 
+
+!---begin:output_list---
+{out}!---end:output_list---
+"""
+
+
 def fractal_profile(nx, h, alpha=1e-2, seed=0, lam_min_h=20):
     """Seeded band-limited self-similar profile y(x) over nx points of
     spacing h: power spectrum ~ k^-3 between wavelengths lam_min_h*h and the

@@ -42,8 +42,10 @@ def _f64(a):
 class Problem:
     """One problem in the fdm_problem layout (0-based global indices)."""
 
-    def __init__(self, nx: int, ny: int, rk_order: int = 4, precision: str = "fp64"):
+    def __init__(self, nx: int, ny: int, rk_order: int = 4, precision: str = "fp64", mode: int = 2):
         self.nx, self.ny = int(nx), int(ny)
+        # 2: plane strain (6 fields); 3: antiplane shear (3 fields)
+        self.mode = int(mode)
         self.rk_order = rk_order
         self.precision = precision
         shape = (self.ny, self.nx)
@@ -116,6 +118,7 @@ class Problem:
         else:
             prob.mat_rho = prob.mat_cs = prob.mat_cp = capi.dptr(None)
         prob.J = P(self.J) if self.J is not None else capi.dptr(None)
+        prob.mode = self.mode
 
         barr = (capi.fdm_block * max(1, len(self.blocks)))()
         for b, blk in enumerate(self.blocks):
@@ -131,8 +134,8 @@ class Problem:
             if pl:
                 for k in ("mu", "beta", "b", "h", "eta"):
                     setattr(cb, k, float(pl.get(k, 0.0)))
-                for k, v in enumerate(pl.get("s0", [0.0] * 6)):
-                    cb.s0[k] = float(v)
+            for k, v in enumerate(pl.get("s0", [0.0] * 6) if pl else blk.get("s0", [0.0] * 6)):
+                cb.s0[k] = float(v)
             for s in range(4):
                 sd = blk["sides"][s]
                 n = (blk["py"] - blk["my"] + 1) if s < 2 else (blk["px"] - blk["mx"] + 1)

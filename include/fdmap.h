@@ -192,6 +192,16 @@ typedef struct fdm_problem {
      Cdiss > 0 (rates_dissipation_*_mode2, rates.f90:1397-2021), else may be
      NULL. */
   const double* J;
+  /* 2: plane strain, fields vx vy sxx sxy syy szz; 3: antiplane shear,
+     fields vz sxz syz (domain.f90:220-227, fields.f90:158-175); 0 means 2.
+     In mode 3 the field arrays of fdm_set_fields and fdm_get_fields are
+     [3][ny][nx], fdm_side.F0 holds 3 components, fdm_side.M uses Zs (M[0])
+     and the SAT coefficient Ks (M[3]), and fdm_block.s0 holds the in-plane
+     prestress sxx, sxy, -, syy, -, szz that frictional interfaces resolve
+     onto the fault (interfaces.f90:859-876).  Mode 3 covers elastic
+     homogeneous blocks without sources or dissipation. */
+  int32_t mode;
+  int32_t pad_mode_;
 } fdm_problem;
 
 /* Kernel timing gathered with HIP events on the context's stream. */

@@ -377,6 +377,7 @@ typedef struct {
 
 struct orc_ctx {
   int32_t nx, ny, nb, ni, nstage;
+  int32_t mode, nF; /* 2: plane strain, 6 fields; 3: antiplane shear, 3 fields */
   size_t np;
   double rkA[FDM_MAX_STAGES], rkB[FDM_MAX_STAGES], rkC[FDM_MAX_STAGES + 1];
   double *xq, *xr, *yq, *yr, *Ji, *x, *y;
@@ -439,6 +440,8 @@ int orc_create(const fdm_problem* P, orc_ctx** out) {
   c->nb = P->nblocks;
   c->ni = P->nifaces;
   c->nstage = P->nstage;
+  c->mode = P->mode == 3 ? 3 : 2;
+  c->nF = c->mode == 3 ? 3 : 6;
   for (int s = 0; s < P->nstage; s++) {
     c->rkA[s] = P->rkA[s];
     c->rkB[s] = P->rkB[s];
@@ -476,6 +479,14 @@ int orc_create(const fdm_problem* P, orc_ctx** out) {
   for (int ib = 0; ib < c->nb; ib++) {
     const fdm_block* b = &P->blocks[ib];
     c->B[ib].b = *b;
+    if (c->mode == 3 && (b->plastic || b->mms || b->Cdiss > 0.0 || P->mat_rho ||
+                         P->source.nplace > 0)) {
+      snprintf(g_err, sizeof g_err,
+               "orc_create: mode 3 covers elastic homogeneous blocks without sources or "
+               "dissipation");
+      orc_destroy(c);
+      return FDM_EINVAL;
+    }
     if (b->Cdiss > 0.0 && !c->J) {
       if (!P->J) {
         snprintf(g_err, sizeof g_err, "orc_create: block %d has Cdiss > 0 and no Jacobian", ib);
@@ -643,17 +654,19 @@ void orc_destroy(orc_ctx* c) {
 const char* orc_last_error(const orc_ctx* c) { return c ? c->err : g_err; }
 
 int orc_set_fields(orc_ctx* c, const double* F, const double* DF) {
-  memcpy(c->F, F, 6 * c->np * sizeof(double));
+  size_t nf = (size_t)c->nF * c->np;
+  memcpy(c->F, F, nf * sizeof(double));
   if (DF)
-    memcpy(c->DF, DF, 6 * c->np * sizeof(double));
+    memcpy(c->DF, DF, nf * sizeof(double));
   else
-    for (size_t k = 0; k < 6 * c->np; k++) c->DF[k] = 1e40;
+    for (size_t k = 0; k < nf; k++) c->DF[k] = 1e40;
   return FDM_OK;
 }
 
 int orc_get_fields(orc_ctx* c, double* F, double* DF) {
-  if (F) memcpy(F, c->F, 6 * c->np * sizeof(double));
-  if (DF) memcpy(DF, c->DF, 6 * c->np * sizeof(double));
+  size_t nf = (size_t)c->nF * c->np;
+  if (F) memcpy(F, c->F, nf * sizeof(double));
+  if (DF) memcpy(DF, c->DF, nf * sizeof(double));
   return FDM_OK;
 }
 
@@ -686,7 +699,7 @@ int orc_get_plastic(orc_ctx* c, double* gammap, double* lambda) {
 /* ---------------------------------------------------------------------- */
 
 static void scale_rates_all(orc_ctx* c, double A) {
-  size_t n6 = 6 * c->np;
+  size_t n6 = (size_t)c->nF * c->np;
   double* DF = c->DF;
 #pragma omp parallel for schedule(static) num_threads(g_threads)
   for (size_t k = 0; k < n6; k++) DF[k] = A * DF[k];
@@ -704,7 +717,7 @@ static void scale_rates_all(orc_ctx* c, double A) {
 }
 
 static void update_fields_all(orc_ctx* c, double Bdt) {
-  size_t n6 = 6 * c->np;
+  size_t n6 = (size_t)c->nF * c->np;
   double* F = c->F;
   const double* DF = c->DF;
 #pragma omp parallel for schedule(static) num_threads(g_threads)
@@ -747,7 +760,7 @@ static void prepare_edges(orc_ctx* c) {
           i = b->mx + k;
           j = b->py;
         }
-        for (int l = 0; l < 6; l++) sd->F[(size_t)l * sd->n + k] = FL(l)[IDX(i, j)];
+        for (int l = 0; l < c->nF; l++) sd->F[(size_t)l * sd->n + k] = FL(l)[IDX(i, j)];
       }
     }
   }
@@ -823,6 +836,53 @@ static void set_bc_mode2(double* Fhat, const double* F, const double* F0, double
     for (int l = 0; l < 6; l++) Fhat[l] = Fhat[l] + F0[l];
 }
 
+/* antiplane shear: rotate_fields_xy2nt_mode3 (fields.f90:1297-1309), vz
+   and the vector (sxz, syz) (rotate_xy2nt_vec, geometry.f90:285-295) */
+static inline void xy2nt3(const double* F, double n1, double n2, double* vz, double* stz,
+                          double* snz) {
+  *vz = F[0];
+  *stz = n2 * F[1] - n1 * F[2];
+  *snz = n1 * F[1] + n2 * F[2];
+}
+
+/* rotate_fields_nt2xy_mode3 (fields.f90:1328-1340, geometry.f90:298-308) */
+static inline void nt2xy3(double* F, double n1, double n2, double vz, double stz, double snz) {
+  F[0] = vz;
+  F[1] = n2 * stz + n1 * snz;
+  F[2] = -n1 * stz + n2 * snz;
+}
+
+/* set_bc_mode3 (boundaries.f90:155-213) */
+static void set_bc_mode3(double* Fhat, const double* F, const double* F0, double n1, double n2,
+                         int bc, double Zs) {
+  if (Zs == 0.0) {
+    for (int l = 0; l < 3; l++) Fhat[l] = F[l];
+    return;
+  }
+  double Zsi = 1.0 / Zs;
+  double d[3], vzFD, stzFD, snzFD, vz = 0.0, snz = 0.0;
+  for (int l = 0; l < 3; l++) d[l] = F[l] - F0[l];
+  xy2nt3(d, n1, n2, &vzFD, &stzFD, &snzFD);
+  switch (bc) {
+    case FDM_BC_ABSORBING:
+      vz = 0.5 * (vzFD - Zsi * snzFD);
+      snz = 0.5 * (snzFD - Zs * vzFD);
+      break;
+    case FDM_BC_FREE:
+      vz = vzFD - Zsi * snzFD;
+      snz = 0.0;
+      break;
+    case FDM_BC_RIGID:
+      vz = 0.0;
+      snz = snzFD - Zs * vzFD;
+      break;
+    default:
+      break;
+  }
+  nt2xy3(Fhat, n1, n2, vz, stzFD, snz);
+  for (int l = 0; l < 3; l++) Fhat[l] = Fhat[l] + F0[l];
+}
+
 /* boundaries.f90:74-129 */
 static void enforce_boundary_conditions(orc_ctx* c, double t) {
   for (int ib = 0; ib < c->nb; ib++) {
@@ -832,9 +892,14 @@ static void enforce_boundary_conditions(orc_ctx* c, double t) {
       int n = sd->n;
       for (int k = 0; k < n; k++) {
         double F[6], F0[6], Fh[6];
-        for (int l = 0; l < 6; l++) {
+        for (int l = 0; l < c->nF; l++) {
           F[l] = sd->F[(size_t)l * n + k];
           F0[l] = sd->F0[(size_t)l * n + k];
+        }
+        if (c->mode == 3) {
+          set_bc_mode3(Fh, F, F0, sd->normal[k], sd->normal[n + k], sd->bc, sd->M[k]);
+          for (int l = 0; l < 3; l++) sd->Fhat[(size_t)l * n + k] = Fh[l];
+          continue;
         }
         set_bc_mode2(Fh, F, F0, sd->normal[k], sd->normal[n + k], sd->bc, sd->M[k],
                      sd->M[n + k], sd->M[2 * (size_t)n + k], sd->x ? sd->x[k] : 0.0,
@@ -1140,6 +1205,87 @@ static int frictional_interface_mode2(orc_ctx* c, oiface_t* I, int k, double* Fh
   return fail;
 }
 
+/* locked_interface_mode3 (interfaces.f90:676-712) */
+static void locked_interface_mode3(double* Fhatm, double* Fhatp, const double* Fm,
+                                   const double* Fp, double n1, double n2, double Zsm,
+                                   double Zsp) {
+  double vzFDp, stzFDp, snzFDp, vzFDm, stzFDm, snzFDm;
+  xy2nt3(Fp, n1, n2, &vzFDp, &stzFDp, &snzFDp);
+  xy2nt3(Fm, n1, n2, &vzFDm, &stzFDm, &snzFDm);
+  double Zsip = 1.0 / Zsp, Zsim = 1.0 / Zsm;
+  double etas = 1.0 / (Zsip + Zsim);
+  double phis = etas * (snzFDp * Zsip + snzFDm * Zsim + vzFDp - vzFDm);
+  double snzp = phis, snzm = phis;
+  double vzp = vzFDp + Zsip * (snzFDp - snzp);
+  double vzm = vzFDm - Zsim * (snzFDm - snzm);
+  nt2xy3(Fhatp, n1, n2, vzp, stzFDp, snzp);
+  nt2xy3(Fhatm, n1, n2, vzm, stzFDm, snzm);
+}
+
+/* frictional_interface_mode3 (interfaces.f90:806-897) with solve_friction.
+   The arguments arrive in their own order here (interfaces.f90:798).  The
+   normal stress comes from the plus block's in-plane prestress F0(4), F0(5),
+   F0(7) resolved on the fault (rotate_xy2nt_mat, geometry.f90:311-322); the
+   stress transfer phip is zero. */
+static int frictional_interface_mode3(orc_ctx* c, oiface_t* I, int k, double* Fhatm,
+                                      double* Fhatp, const double* Fm, const double* Fp,
+                                      const double* F0m, const double* F0p, double n1, double n2,
+                                      double Zsm, double Zsp, const double* s0p, double t,
+                                      double dt) {
+  (void)c;
+  int n = I->n;
+  double vz0p, stz0p, snz0p, vz0m, stz0m, snz0m;
+  xy2nt3(F0p, n1, n2, &vz0p, &stz0p, &snz0p);
+  xy2nt3(F0m, n1, n2, &vz0m, &stz0m, &snz0m);
+  double dp[3], dm[3];
+  for (int l = 0; l < 3; l++) {
+    dp[l] = Fp[l] - F0p[l];
+    dm[l] = Fm[l] - F0m[l];
+  }
+  double vzFDp, stzFDp, snzFDp, vzFDm, stzFDm, snzFDm;
+  xy2nt3(dp, n1, n2, &vzFDp, &stzFDp, &snzFDp);
+  xy2nt3(dm, n1, n2, &vzFDm, &stzFDm, &snzFDm);
+  double Zsip = 1.0 / Zsp, Zsim = 1.0 / Zsm;
+  double etas = 1.0 / (Zsip + Zsim);
+  double phis = etas * (snzFDp * Zsip + snzFDm * Zsim + vzFDp - vzFDm);
+  double S0 = snz0p;
+  double snn = 2.0 * n1 * n2 * s0p[1] + n1 * n1 * s0p[0] + n2 * n2 * s0p[3];
+  double pp = 0.0;
+  double N0 = -snn - pp;
+  double phip = 0.0;
+  double Bl = load_ramp(t, I->I.ld_T);
+  double S0l = I->S0base[k] + I->ldS[k] * Bl;
+  double N0l = I->N0base[k] + I->ldN[k] * Bl;
+  double* fS0 = &I->f[FDM_IF_S0][k];
+  double* fN0 = &I->f[FDM_IF_N0][k];
+  *fS0 = S0l + S0;
+  *fN0 = N0l + N0;
+  const double* rs = NULL;
+  double rsk[7], swk[4];
+  if (I->rs) {
+    for (int q = 0; q < 7; q++) rsk[q] = I->rs[(size_t)q * n + k];
+    rs = rsk;
+  }
+  if (I->sw)
+    for (int q = 0; q < 4; q++) swk[q] = I->sw[(size_t)q * n + k];
+  int fail = solve_friction_core(I->I.law, rs, swk, I->I.reg, I->outside[k], I->I.flock, *fS0,
+                                 *fN0, phip, phis, etas, I->f[FDM_IF_D][k],
+                                 I->f[FDM_IF_PSI][k], I->I.skempton, dt, &I->f[FDM_IF_V][k],
+                                 &I->f[FDM_IF_S][k], &I->f[FDM_IF_O][k], &I->f[FDM_IF_N][k],
+                                 &I->f[FDM_IF_NREG][k]);
+  double snzp = I->f[FDM_IF_S][k] - *fS0;
+  double snzm = I->f[FDM_IF_S][k] - *fS0;
+  double vzp = vzFDp + Zsip * (snzFDp - snzp);
+  double vzm = vzFDm - Zsim * (snzFDm - snzm);
+  nt2xy3(Fhatp, n1, n2, vzp, stzFDp, snzp);
+  nt2xy3(Fhatm, n1, n2, vzm, stzFDm, snzm);
+  for (int l = 0; l < 3; l++) {
+    Fhatp[l] = Fhatp[l] + F0p[l];
+    Fhatm[l] = Fhatm[l] + F0m[l];
+  }
+  return fail;
+}
+
 /* set_rates_friction (friction.f90:1101-1124) for one node */
 static void set_rates_friction_node(oiface_t* I, int k, double t) {
   int n = I->n;
@@ -1183,13 +1329,26 @@ static int enforce_interface_conditions(orc_ctx* c, double t, double dt) {
     for (int k = 0; k < n; k++) {
       int km = I->I.m + k - offm, kp = I->I.m + k - offp;
       double Fm[6], Fp[6], F0m[6], F0p[6], Fhm[6], Fhp[6];
-      for (int l = 0; l < 6; l++) {
+      for (int l = 0; l < c->nF; l++) {
         Fm[l] = SM->F[(size_t)l * SM->n + km];
         Fp[l] = SP->F[(size_t)l * SP->n + kp];
         F0m[l] = SM->F0[(size_t)l * SM->n + km];
         F0p[l] = SP->F0[(size_t)l * SP->n + kp];
       }
       double n1 = I->nhat[k], n2 = I->nhat[n + k];
+      if (c->mode == 3) {
+        double Zsm3 = SM->M[km], Zsp3 = SP->M[kp];
+        if (I->I.coupling == FDM_COUPLING_LOCKED)
+          locked_interface_mode3(Fhm, Fhp, Fm, Fp, n1, n2, Zsm3, Zsp3);
+        else
+          fail |= frictional_interface_mode3(c, I, k, Fhm, Fhp, Fm, Fp, F0m, F0p, n1, n2, Zsm3,
+                                             Zsp3, c->B[bp].b.s0, t, dt);
+        for (int l = 0; l < 3; l++) {
+          SM->Fhat[(size_t)l * SM->n + km] = Fhm[l];
+          SP->Fhat[(size_t)l * SP->n + kp] = Fhp[l];
+        }
+        continue;
+      }
       double Zsm = SM->M[km], Zsp = SP->M[kp];
       double Zpm = SM->M[SM->n + km], Zpp = SP->M[SP->n + kp];
       double gm = SM->M[2 * (size_t)SM->n + km], gp = SP->M[2 * (size_t)SP->n + kp];
@@ -1248,6 +1407,13 @@ static void set_rates_SAT(orc_ctx* c, int ib) {
       }
       double DF6[6], Fh[6], Fe[6];
       size_t q = IDX(i, j);
+      if (c->mode == 3) {
+        /* SAT_term, mode 3: DF = DF+Ks*(Fhat-F) (time_step.f90:829-830) */
+        double Ks = sd->M[3 * (size_t)n + k];
+        for (int l = 0; l < 3; l++)
+          DFL(l)[q] = DFL(l)[q] + Ks * (sd->Fhat[(size_t)l * n + k] - sd->F[(size_t)l * n + k]);
+        continue;
+      }
       for (int l = 0; l < 6; l++) {
         DF6[l] = DFL(l)[q];
         Fh[l] = sd->Fhat[(size_t)l * n + k];
@@ -1556,6 +1722,124 @@ static void rates_boundary(const sten_t* S, int mx, int px, int my, int py, int 
     }
 }
 
+/* antiplane shear (rates_interior_mode3, rates_boundary_mode3,
+   rates.f90:8-381): DF1 from sxz (with y metrics) and syz (x metrics),
+   scaled by DI/rho; DF2 from vz with y metrics, DF3 from vz with x metrics,
+   scaled by DI*G.  Terms absent from a field are absent from its sum. */
+static inline double q1m3(const sten_t* S, int l, size_t kk, size_t cc) {
+  double ay = 0.5 * (S->yr[kk] + S->yr[cc]);
+  double ax = 0.5 * (S->xr[kk] + S->xr[cc]);
+  if (l == 0) return ay * S->F[1][kk] - ax * S->F[2][kk];
+  if (l == 1) return ay * S->F[0][kk];
+  return -(ax * S->F[0][kk]);
+}
+static inline double r1m3(const sten_t* S, int l, size_t kk, size_t cc) {
+  double by = 0.5 * (S->yq[kk] + S->yq[cc]);
+  double bx = 0.5 * (S->xq[kk] + S->xq[cc]);
+  if (l == 0) return -(by * S->F[1][kk]) + bx * S->F[2][kk];
+  if (l == 1) return -(by * S->F[0][kk]);
+  return bx * S->F[0][kk];
+}
+static inline double qcm3(const sten_t* S, int l, size_t kp, size_t km, size_t cc) {
+  double ayp = 0.5 * (S->yr[kp] + S->yr[cc]), axp = 0.5 * (S->xr[kp] + S->xr[cc]);
+  double aym = 0.5 * (S->yr[km] + S->yr[cc]), axm = 0.5 * (S->xr[km] + S->xr[cc]);
+  if (l == 0)
+    return ayp * S->F[1][kp] - axp * S->F[2][kp] - aym * S->F[1][km] + axm * S->F[2][km];
+  if (l == 1) return ayp * S->F[0][kp] - aym * S->F[0][km];
+  return -(axp * S->F[0][kp]) + axm * S->F[0][km];
+}
+static inline double rcm3(const sten_t* S, int l, size_t kp, size_t km, size_t cc) {
+  double byp = 0.5 * (S->yq[kp] + S->yq[cc]), bxp = 0.5 * (S->xq[kp] + S->xq[cc]);
+  double bym = 0.5 * (S->yq[km] + S->yq[cc]), bxm = 0.5 * (S->xq[km] + S->xq[cc]);
+  if (l == 0)
+    return -(byp * S->F[1][kp]) + bxp * S->F[2][kp] + bym * S->F[1][km] - bxm * S->F[2][km];
+  if (l == 1) return -(byp * S->F[0][kp]) + bym * S->F[0][km];
+  return bxp * S->F[0][kp] - bxm * S->F[0][km];
+}
+/* ")*D/rho*JacInv", ")*D*G*JacInv" */
+static inline double bscale3(int l, double t, double D, double rho, double G) {
+  return l == 0 ? t * D / rho : t * D * G;
+}
+
+/* rates_interior_mode3 (rates.f90:8-56): a(s) = DI(s)/rho, b(s) = DI(s)*G;
+   the s = 3 coefficient is zero */
+static void rates_interior3(const sten_t* S, int mx, int px, int my, int py) {
+  double coef[3][3];
+  for (int s = 1; s <= 2; s++) {
+    coef[0][s] = DIc(s) / S->rho;
+    coef[1][s] = DIc(s) * S->G;
+    coef[2][s] = DIc(s) * S->G;
+  }
+  int nx = S->nx;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int j = my; j <= py; j++)
+    for (int i = mx; i <= px; i++) {
+      size_t cc = (size_t)j * nx + i;
+      for (int s = 1; s <= 2; s++) {
+        size_t ip = cc + s, im = cc - s, jp = cc + (size_t)s * nx, jm = cc - (size_t)s * nx;
+        double ayp = 0.5 * (S->yr[ip] + S->yr[cc]), axp = 0.5 * (S->xr[ip] + S->xr[cc]);
+        double aym = 0.5 * (S->yr[im] + S->yr[cc]), axm = 0.5 * (S->xr[im] + S->xr[cc]);
+        double byp = 0.5 * (S->yq[jp] + S->yq[cc]), bxp = 0.5 * (S->xq[jp] + S->xq[cc]);
+        double bym = 0.5 * (S->yq[jm] + S->yq[cc]), bxm = 0.5 * (S->xq[jm] + S->xq[cc]);
+        const double *F1 = S->F[0], *F2 = S->F[1], *F3 = S->F[2];
+        double t1 = ayp * F2[ip] - axp * F3[ip] - aym * F2[im] + axm * F3[im] - byp * F2[jp] +
+                    bxp * F3[jp] + bym * F2[jm] - bxm * F3[jm];
+        S->DF[0][cc] = S->DF[0][cc] + t1 * coef[0][s] * S->Ji[cc];
+        double t2 = ayp * F1[ip] - aym * F1[im] - byp * F1[jp] + bym * F1[jm];
+        S->DF[1][cc] = S->DF[1][cc] + t2 * coef[1][s] * S->Ji[cc];
+        double t3 = -(axp * F1[ip]) + axm * F1[im] + bxp * F1[jp] - bxm * F1[jm];
+        S->DF[2][cc] = S->DF[2][cc] + t3 * coef[2][s] * S->Ji[cc];
+      }
+    }
+}
+
+/* rates_boundary_mode3 (rates.f90:59-381), the loop structure of the mode-2
+   routine */
+static void rates_boundary3(const sten_t* S, int mx, int px, int my, int py, int qside,
+                            int rside, int bx0, int bx1, int by0, int by1) {
+  int nx = S->nx;
+  double rho = S->rho, G = S->G;
+  for (int j = my; j <= py; j++)
+    for (int i = mx; i <= px; i++) {
+      size_t cc = (size_t)j * nx + i;
+      double Ji = S->Ji[cc];
+      if (qside) {
+        for (int s = 0; s < NBND; s++) {
+          size_t kq = qside == 1 ? (size_t)j * nx + (bx0 + s) : (size_t)j * nx + (bx1 - s);
+          double Dq = qside == 1 ? DLc(s, i - bx0) : DRc(s, bx1 - i);
+          for (int l = 0; l < 3; l++)
+            S->DF[l][cc] = S->DF[l][cc] + bscale3(l, q1m3(S, l, kq, cc), Dq, rho, G) * Ji;
+          if (rside) {
+            size_t kr = rside == 1 ? (size_t)(by0 + s) * nx + i : (size_t)(by1 - s) * nx + i;
+            double Dr = rside == 1 ? DLc(s, j - by0) : DRc(s, by1 - j);
+            for (int l = 0; l < 3; l++)
+              S->DF[l][cc] = S->DF[l][cc] + bscale3(l, r1m3(S, l, kr, cc), Dr, rho, G) * Ji;
+          }
+        }
+        if (!rside)
+          for (int s = 1; s <= PI_I; s++) {
+            size_t jp = cc + (size_t)s * nx, jm = cc - (size_t)s * nx;
+            for (int l = 0; l < 3; l++)
+              S->DF[l][cc] =
+                  S->DF[l][cc] + bscale3(l, rcm3(S, l, jp, jm, cc), DIc(s), rho, G) * Ji;
+          }
+      } else {
+        for (int s = 0; s < NBND; s++) {
+          size_t kr = rside == 1 ? (size_t)(by0 + s) * nx + i : (size_t)(by1 - s) * nx + i;
+          double Dr = rside == 1 ? DLc(s, j - by0) : DRc(s, by1 - j);
+          for (int l = 0; l < 3; l++)
+            S->DF[l][cc] = S->DF[l][cc] + bscale3(l, r1m3(S, l, kr, cc), Dr, rho, G) * Ji;
+        }
+        for (int s = 1; s <= PI_I; s++) {
+          size_t ip = cc + s, im = cc - s;
+          for (int l = 0; l < 3; l++)
+            S->DF[l][cc] =
+                S->DF[l][cc] + bscale3(l, qcm3(S, l, ip, im, cc), DIc(s), rho, G) * Ji;
+        }
+      }
+    }
+}
+
 /* set_rates, mode-2 homogeneous branch (time_step.f90:457-490) */
 static void set_rates(orc_ctx* c, int ib) {
   fdm_block* b = &c->B[ib].b;
@@ -1582,6 +1866,19 @@ static void set_rates(orc_ctx* c, int ib) {
   S.nx = c->nx;
   int mgx = b->mx, pgx = b->px, mgy = b->my, pgy = b->py;
   int mix = mgx + NBST, pix = pgx - NBST, miy = mgy + NBST, piy = pgy - NBST;
+  if (c->mode == 3) {
+    /* set_rates, mode 3 (time_step.f90:522-547): no szz relation */
+    rates_interior3(&S, mix, pix, miy, piy);
+    rates_boundary3(&S, mgx, mix - 1, miy, piy, 1, 0, mgx, pgx, mgy, pgy);
+    rates_boundary3(&S, pix + 1, pgx, miy, piy, 2, 0, mgx, pgx, mgy, pgy);
+    rates_boundary3(&S, mix, pix, mgy, miy - 1, 0, 1, mgx, pgx, mgy, pgy);
+    rates_boundary3(&S, mix, pix, piy + 1, pgy, 0, 2, mgx, pgx, mgy, pgy);
+    rates_boundary3(&S, mgx, mix - 1, mgy, miy - 1, 1, 1, mgx, pgx, mgy, pgy);
+    rates_boundary3(&S, pix + 1, pgx, mgy, miy - 1, 2, 1, mgx, pgx, mgy, pgy);
+    rates_boundary3(&S, mgx, mix - 1, piy + 1, pgy, 1, 2, mgx, pgx, mgy, pgy);
+    rates_boundary3(&S, pix + 1, pgx, piy + 1, pgy, 2, 2, mgx, pgx, mgy, pgy);
+    return;
+  }
   rates_interior(&S, mix, pix, miy, piy);
   rates_boundary(&S, mgx, mix - 1, miy, piy, 1, 0, mgx, pgx, mgy, pgy);         /* L  */
   rates_boundary(&S, pix + 1, pgx, miy, piy, 2, 0, mgx, pgx, mgy, pgy);         /* R  */

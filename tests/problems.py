@@ -57,13 +57,16 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
           coupling="friction", law="RFL", bcs=None, precision="fp64", background=None,
           perturb=1e-3, source=False, plastic=None, nbx=1, rk_order=4, outside_frac=0.0,
           fault_overstress=25.0, ld_T=0.0, reg=False, rup_field="V", Vinit=1e-12,
-          skempton=0.0, mms=False, medium=None, Cdiss=None):
+          skempton=0.0, mms=False, medium=None, Cdiss=None, mode=2,
+          inplane=(-120.0, 0.0, -120.0, -120.0)):
     """Blocks stacked in y (len(nys) rows) and nbx columns; interfaces
     between them.  medium: amplitude of a smooth seeded heterogeneous medium
     (rho, cs, cp varying by that relative amount around the given values;
     the edge coefficients then use the medium's values at each node).
     Cdiss: artificial dissipation of every block as the rates use it (a
-    number, or one per block).
+    number, or one per block).  mode 3: antiplane shear, background
+    (vz, sxz, syz); inplane (sxx, sxy, syy, szz) is the in-plane prestress
+    that frictional interfaces resolve onto the fault.
     Returns (Problem, F0 fields [6,ny,nx])."""
     rng = np.random.default_rng(seed)
     nyb = list(nys)
@@ -95,7 +98,7 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
         X2 = X + warp * Lx * 0.05 * np.sin(2 * np.pi * Y / Lx + ph[0]) * np.cos(np.pi * X / Lx + ph[1])
         Y2 = Y + warp * Lx * 0.05 * np.sin(2 * np.pi * X / Lx + ph[2]) * np.cos(np.pi * Y / Lx + ph[3])
         X, Y = X2, Y2
-    prob = Problem(NX, ny, rk_order=rk_order, precision=precision)
+    prob = Problem(NX, ny, rk_order=rk_order, precision=precision, mode=mode)
     xq = np.empty_like(X)
     xr = np.empty_like(X)
     yq = np.empty_like(X)
@@ -119,7 +122,9 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
             ph = rng.uniform(0, 2 * np.pi, 2)
             return 1.0 + medium * np.sin((2 + k) * np.pi * X / Lx + ph[0]) * np.cos((1 + k) * np.pi * Y / Lx + ph[1])
         prob.medium = (rho * smooth(0), cs * smooth(1), cp * smooth(2))
-    bg = np.zeros(6) if background is None else np.asarray(background, dtype=np.float64)
+    nF = 3 if mode == 3 else 6
+    bg = np.zeros(nF) if background is None else np.asarray(background, dtype=np.float64)
+    assert bg.shape == (nF,)
     G = rho * cs ** 2
     gamma = 1.0 - 2.0 * (cs / cp) ** 2
     nbl = nbx * len(nyb)
@@ -167,6 +172,9 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
                 sides.append(dict(bc=bc, normal=np.vstack([n1, n2]), M=M, F0=F0,
                                   x=X[idx].copy(), y=Y[idx].copy()))
             blk = prob.add_block(mx, px, my, py, rho, cs, cp, sides, plastic=plastic, mms=mms)
+            if mode == 3:
+                sxx_, sxy_, syy_, szz_ = inplane
+                blk["s0"] = [sxx_, sxy_, 0.0, syy_, 0.0, szz_]
             if Cdiss is not None:
                 blk["Cdiss"] = Cdiss[len(prob.blocks) - 1] if np.ndim(Cdiss) else float(Cdiss)
     # interfaces between vertically adjacent blocks (direction 'y') and
@@ -180,7 +188,7 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
             n = B["py"] - B["my"] + 1
             prob.ifaces.append(_iface(rng, "x", bm, bp, B["my"], B["py"], nrm_side, n, X[B["my"]:B["py"] + 1, B["px"]],
                                       coupling, law, bg, fault_overstress, outside_frac, ld_T,
-                                      reg, rup_field, Vinit, skempton))
+                                      reg, rup_field, Vinit, skempton, mode, inplane))
     for bx in range(nbx):
         for by in range(len(nyb) - 1):
             bm = bx + by * nbx
@@ -190,11 +198,12 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
             n = B["px"] - B["mx"] + 1
             prob.ifaces.append(_iface(rng, "y", bm, bp, B["mx"], B["px"], nrm_side, n, X[B["py"], B["mx"]:B["px"] + 1],
                                       coupling, law, bg, fault_overstress, outside_frac, ld_T,
-                                      reg, rup_field, Vinit, skempton))
+                                      reg, rup_field, Vinit, skempton, mode, inplane))
     # initial fields: background + smooth random perturbation
-    F = np.empty((6, ny, NX))
-    for l in range(6):
-        scale = perturb * (1.0 if l < 2 else 10.0)
+    F = np.empty((nF, ny, NX))
+    nvel = 1 if mode == 3 else 2
+    for l in range(nF):
+        scale = perturb * (1.0 if l < nvel else 10.0)
         k1, k2 = rng.uniform(1, 3, 2)
         ph = rng.uniform(0, 2 * np.pi, 2)
         F[l] = bg[l] + scale * np.sin(k1 * np.pi * X / Lx + ph[0]) * np.cos(k2 * np.pi * Y / Lx + ph[1])
@@ -226,7 +235,7 @@ def _delta(a, h):
 
 
 def _iface(rng, direction, bm, bp, m, p, nrm, n, xs, coupling, law, bg, overstress, outside_frac,
-           ld_T, reg, rup_field, Vinit, skempton):
+           ld_T, reg, rup_field, Vinit, skempton, mode=2, inplane=None):
     itf = dict(direction=direction, blockm=bm, blockp=bp, m=int(m), p=int(p), coupling=coupling,
                law=law, nhat=nrm.copy(), reg=reg, rup_field=rup_field, rup_threshold=1e-3,
                skempton=skempton, flock=0.6, ld_T=ld_T)
@@ -243,10 +252,14 @@ def _iface(rng, direction, bm, bp, m, p, nrm, n, xs, coupling, law, bg, overstre
     itf["sw"] = np.vstack([np.zeros(n), np.full(n, 0.677), np.full(n, 0.525), np.full(n, 0.4)])
     # shear/normal prestress resolved on the fault from the background
     n1, n2 = nrm[0], nrm[1]
-    sxx, sxy, syy = bg[2], bg[3], bg[4]
-    snt = n1 * n2 * (sxx - syy) + (n2 * n2 - n1 * n1) * sxy
+    if mode == 3:
+        # snz of (sxz, syz); snn of the in-plane prestress (interfaces.f90:859-872)
+        sxx, sxy, syy = inplane[0], inplane[1], inplane[2]
+        S0 = n1 * bg[1] + n2 * bg[2]
+    else:
+        sxx, sxy, syy = bg[2], bg[3], bg[4]
+        S0 = n1 * n2 * (sxx - syy) + (n2 * n2 - n1 * n1) * sxy
     snn = 2.0 * n1 * n2 * sxy + n1 * n1 * sxx + n2 * n2 * syy
-    S0 = snt
     N0 = -snn
     xmid = 0.5 * (xs[0] + xs[-1])
     R = 0.15 * (xs[-1] - xs[0])

@@ -130,3 +130,31 @@ def test_checkpoint_list_errors(tmp_path):
     hp, _ = make(tmp_path, ckpt="ckpt = T, intervall = 4")
     with pytest.raises(RuntimeError, match="checkpoint_list"):
         hp.checkpoint_init()
+
+
+def test_checkpoint_layout_antiplane(tmp_path):
+    """mode 3: three field planes and rates, side files with the three
+    components and one displacement component (fields.f90:167-172)"""
+    name = os.path.join(str(tmp_path), "m")
+    hp = host.HostProblem(inputs.write(name + ".in", inputs.antiplane_rs(
+        name=name, nx=21, ny=17, nsteps=5, outputs=False)))
+    rng = np.random.default_rng(5)
+    st = dict(F=rng.standard_normal((3, hp.ny, hp.nx)), DF=rng.standard_normal((3, hp.ny, hp.nx)),
+              IF=rng.standard_normal((18, hp.fric_nodes())))
+    F6 = np.zeros((6, hp.ny, hp.nx))
+    D6 = np.zeros((6, hp.ny, hp.nx))
+    F6[:3], D6[:3] = st["F"], st["DF"]
+    hp.checkpoint_files("write", 3, F=F6, DF=D6, IF=st["IF"])
+    planes = np.fromfile(name + "F.ckpt3", dtype="<f8").reshape(-1, hp.ny, hp.nx)
+    assert planes.shape[0] == 6
+    np.testing.assert_array_equal(planes[:3], st["F"])
+    np.testing.assert_array_equal(planes[3:], st["DF"])
+    side = hp.ptr.contents.blocks[0].side[0]
+    raw = np.fromfile(name + "block1L.ckpt3", dtype="<f8").reshape(-1, side.n)
+    assert raw.shape[0] == 3 * 3 + 2 + 2
+    F0 = np.ctypeslib.as_array(side.F0, shape=(3, side.n))
+    for l in range(3):
+        np.testing.assert_array_equal(raw[3 * l + 2], F0[l])
+    r = hp.checkpoint_files("read", 3)
+    np.testing.assert_array_equal(r["F"][:3], st["F"])
+    np.testing.assert_array_equal(r["IF"], st["IF"])

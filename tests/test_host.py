@@ -69,7 +69,7 @@ def test_c1_source_placement(tmp_path):
 
 
 @pytest.mark.parametrize("text,msg", [
-    ("&problem_list nt = 2 /\n&domain_list mode = 3, FDmethod = 'SBP4' /\n", "mode = 2"),
+    ("&problem_list nt = 2 /\n&domain_list mode = 4, FDmethod = 'SBP4' /\n", "mode must be 2"),
     ("&problem_list nt = 2 /\n&domain_list FDmethod = 'SBP6' /\n", "SBP4"),
 ])
 def test_unsupported_options_fail_loudly(tmp_path, text, msg):
@@ -127,3 +127,34 @@ def test_operator_list_dissipation(tmp_path):
     J = np.ctypeslib.as_array(P.J, shape=(hp.ny, hp.nx))
     Ji = np.ctypeslib.as_array(P.Ji, shape=(hp.ny, hp.nx))
     np.testing.assert_allclose(J * Ji, 1.0, rtol=1e-14)
+
+
+def test_antiplane_problem(tmp_path):
+    """mode = 3 (domain.f90:220-227, fields.f90:167-174): three fields vz,
+    sxz, syz; a vertical fault resolves the in-plane prestress sxx0 of the
+    plus block as its normal stress (interfaces.f90:859-876)"""
+    p = inputs.write(str(tmp_path / "m3.in"),
+                     inputs.antiplane_rs(name=str(tmp_path / "m3"), nx=41, ny=31, nsteps=5,
+                                         sxx0=-50.0, sxz0=30.0))
+    hp = host.HostProblem(p)
+    P = hp.ptr.contents
+    assert P.mode == 3
+    F = hp.fields()
+    assert F.shape == (3, 31, 82)
+    assert np.all(F[0] == 0.0) and np.all(F[1] == 30.0) and np.all(F[2] == 0.0)
+    assert list(P.blocks[1].s0)[:4] == [-50.0, 0.0, 0.0, 0.0]
+    side = P.blocks[0].side[1]
+    F0 = np.ctypeslib.as_array(side.F0, shape=(3, side.n))
+    np.testing.assert_array_equal(F0, np.tile([[0.0], [30.0], [0.0]], (1, side.n)))
+    I = P.ifaces[0]
+    assert I.direction == 0
+    # a plastic block, or a boundary condition set_bc_mode3 does not know
+    txt = open(p).read()
+    bad = txt.replace("rho = 2.67d0", "response = 'plastic', rho = 2.67d0")
+    assert bad != txt
+    with pytest.raises(RuntimeError, match="mode 3 covers"):
+        host.HostProblem(inputs.write(str(tmp_path / "b1.in"), bad))
+    bad = txt.replace("bcB = 'absorbing', bcT = 'free' /", "bcB = 'absorbing-velocity', bcT = 'free' /", 1)
+    assert bad != txt
+    with pytest.raises(RuntimeError, match="mode 3 boundary"):
+        host.HostProblem(inputs.write(str(tmp_path / "b2.in"), bad))

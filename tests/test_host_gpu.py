@@ -1,6 +1,7 @@
 """Problems built by the Fortran host from .in files: HIP library vs CPU
 oracle, and the fdmap driver executable end to end."""
 import os
+import re
 import subprocess
 
 import numpy as np
@@ -92,3 +93,48 @@ def test_fdmap_executable_outputs(gpu, tmp_path):
         assert abs(t[n - 1] - np.float32(hp.t + (n - 1) * hp.dt)) <= 1e-6 * max(1.0, abs(t[n - 1]))
     m = open(str(tmp_path / "c2x.m")).read()
     assert "I_V.nx =" in m and "I_V.field = 'V';" in m
+
+
+def test_antiplane_rupture_from_input_file(gpu, tmp_path):
+    """an antiplane (mode 3) rate-and-state rupture built from its .in file
+    after the reference's mode-3 example: vertical fault between two blocks,
+    free surface, Gaussian load"""
+    p = inputs.write(str(tmp_path / "m3.in"),
+                     inputs.antiplane_rs(name=str(tmp_path / "m3"), nx=61, ny=61, nsteps=250,
+                                         outputs=False))
+    hp, dev, orc = run_pair(p, 250)
+    Fd, _ = dev.get_fields()
+    Fo, _ = orc.get_fields()
+    assert Fd.shape == (3, 61, 122)
+    e = rel_l2(Fd, Fo)
+    print("fields", e)
+    assert e <= 1e-12
+    for f in ("D", "V", "S", "N", "Psi"):
+        a, b = dev.get_iface(0, f), orc.get_iface(0, f)
+        print(f, rel_l2(a, b))
+        assert rel_l2(a, b) <= 1e-12
+    assert orc.get_iface(0, "V").max() > 1e-3  # rupture nucleated
+
+
+def test_antiplane_executable_outputs(gpu, tmp_path):
+    nsteps = 60
+    p = inputs.write(str(tmp_path / "m3x.in"),
+                     inputs.antiplane_rs(name=str(tmp_path / "m3x"), nx=41, ny=41, nsteps=nsteps))
+    r = subprocess.run([host.EXE_PATH, p], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-1500:], r.stderr[-800:])
+    assert r.returncode == 0 and "Finished problem" in r.stdout
+    vz = np.fromfile(str(tmp_path / "m3x_B_vz.dat"), dtype=np.float32).reshape(-1, 41, 82)
+    sxz = np.fromfile(str(tmp_path / "m3x_B_sxz.dat"), dtype=np.float32).reshape(-1, 41, 82)
+    V = np.fromfile(str(tmp_path / "m3x_I_V.dat"), dtype=np.float32).reshape(-1, 41)
+    assert vz.shape[0] == nsteps // 6 + 1 and V.shape[0] == nsteps + 1
+    assert np.isfinite(vz).all() and np.abs(sxz[0] - 30.0).max() == 0.0
+    # the body record of step 6: stage-1 fields of step 7, i.e. after 6 steps
+    hp = host.HostProblem(p)
+    orc = OracleContext(hp.ptr, threads=4)
+    orc.set_fields(hp.fields())
+    orc.init_edges(hp.t)
+    orc.step(hp.t, hp.dt, 1, 6)
+    np.testing.assert_allclose(vz[1], orc.get_fields()[0][0].astype(np.float32), rtol=1e-6,
+                               atol=1e-12)
+    m = open(str(tmp_path / "m3x.m")).read()
+    assert re.search(r"^mode = +3;$", m, re.M)

@@ -522,6 +522,100 @@ def test_energy_decays_with_absorbing_boundaries(bcs):
     assert E[-1] < 0.95 * E[0]
 
 
+# ---- antiplane shear (mode 3) ----
+
+@pytest.mark.parametrize("seed,warp", [(11, 0.0), (12, 0.08)])
+def test_stage1_rates_mode3_match_dense_split_operator(seed, warp):
+    """rates_interior_mode3 / rates_boundary_mode3 (rates.f90:8-381):
+    DF1 = (dx sxz + dy syz)/rho, DF2 = G dx vz, DF3 = G dy vz with the
+    split-form curvilinear operator, closures and corners included"""
+    cs, cp, rho = 3.464, 6.0, 2.67
+    prob, F = problems.build(nx=13, nys=(15,), seed=seed, warp=warp, cs=cs, cp=cp, rho=rho,
+                             background=[0.1, 30.0, -20.0], perturb=0.5, mode=3,
+                             bcs=("absorbing", "free", "rigid", "free"))
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.stage1(0.0, 1e-3, 1)
+    _, DF = o.get_fields()
+    assert DF.shape == (3, prob.ny, prob.nx)
+    G = rho * cs ** 2
+    dx, dy = split_ops(prob)
+    vz, sxz, syz = F
+    ref = np.array([(dx(sxz) + dy(syz)) / rho, G * dx(vz), G * dy(vz)])
+    inner = (slice(None), slice(1, -1), slice(1, -1))
+    scale = np.abs(ref[inner]).max(axis=(1, 2), keepdims=True)
+    err = np.abs(DF[inner] - ref[inner]) / scale
+    assert err.max() < 1e-12, err.max()
+
+
+def energy3(prob, F, rho, cs):
+    _, _, HL, _ = ob.sbp4_tables()
+
+    def hw(n):
+        h = np.ones(n)
+        h[:4] = HL
+        h[n - 4:] = HL[::-1]
+        return h
+    W = np.zeros((prob.ny, prob.nx))
+    for b in prob.blocks:
+        sj, si = slice(b["my"], b["py"] + 1), slice(b["mx"], b["px"] + 1)
+        W[sj, si] += np.outer(hw(b["py"] - b["my"] + 1), hw(b["px"] - b["mx"] + 1)) / prob.Ji[sj, si]
+    G = rho * cs ** 2
+    vz, sxz, syz = F
+    return float((W * (0.5 * rho * vz ** 2 + (sxz ** 2 + syz ** 2) / (2 * G))).sum())
+
+
+@pytest.mark.parametrize("bcs,coupling", [(("absorbing",) * 4, None),
+                                          (("free", "rigid", "free", "rigid"), None),
+                                          (("absorbing", "free", "rigid", "free"), "locked"),
+                                          (("free", "free", "rigid", "free"), "frictionless")])
+def test_mode3_energy_does_not_grow(bcs, coupling):
+    """set_bc_mode3 (boundaries.f90:155-213), the mode-3 SAT (time_step.f90:
+    829-830) and the mode-3 interfaces (interfaces.f90:676-712, 806-897) are
+    energy stable: the discrete energy never grows, and absorbing sides
+    drain it"""
+    cs, cp, rho, h = 3.464, 6.0, 2.67, 0.025
+    nys = (27,) if coupling is None else (15, 17)
+    kw = dict(coupling="locked") if coupling == "locked" else dict(coupling="friction", law=coupling)
+    if coupling is None:
+        kw = {}
+    prob, F = problems.build(nx=31, nys=nys, seed=4, warp=0.06, h=h, cs=cs, cp=cp, rho=rho,
+                             background=[0.0] * 3, perturb=1.0, bcs=bcs, mode=3,
+                             inplane=(-50.0, 0.0, -50.0, -50.0), fault_overstress=0.0, **kw)
+    dt = 0.5 * h / np.hypot(cs, cp)
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    E = [energy3(prob, F, rho, cs)]
+    for n in range(1, 61):
+        o.step(0.0, dt, n, 1)
+        E.append(energy3(prob, o.get_fields()[0], rho, cs))
+    E = np.array(E)
+    assert (np.diff(E) <= 1e-12 * E[0]).all(), np.diff(E).max() / E[0]
+    if "absorbing" in bcs:
+        assert E[-1] < 0.95 * E[0]
+
+
+def test_mode3_steady_sliding_under_resolved_prestress():
+    """frictional_interface_mode3: S0 is the background (sxz, syz) resolved
+    on the fault and N0 the plus block's in-plane prestress (interfaces.f90:
+    859-876).  With the state for steady sliding at V = 1e-12 under those,
+    the unperturbed fault keeps sliding at that rate"""
+    prob, F = problems.build(nx=40, nys=(20, 22), seed=3, warp=0.0, background=[0.0, 0.0, 40.0],
+                             perturb=0.0, mode=3, law="RFL", fault_overstress=0.0,
+                             inplane=(-90.0, 0.0, -80.0, -85.0), bcs=("free",) * 4)
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.step(0.0, 1e-3, 1, 5)
+    V = o.get_iface(0, "V")
+    N = o.get_iface(0, "N")
+    np.testing.assert_allclose(N, 80.0, rtol=1e-13)
+    np.testing.assert_allclose(o.get_iface(0, "S0"), 40.0, rtol=1e-13)
+    np.testing.assert_allclose(V, 1e-12, rtol=1e-6)
+
+
 def mms_error(n, tend=0.04, warp=0.05):
     G, cs, cp = 32.0, 3.0, 5.0
     rho = G / cs ** 2

@@ -190,6 +190,60 @@ def test_dissipation_heterogeneous_friction_plasticity(gpu):
     assert rel_l2(gd, go) <= 1e-12 and go.max() > 0.0
 
 
+# ---- antiplane shear (mode 3) ----
+BG3 = [0.0, 30.0, 40.0]
+INPLANE = (-120.0, 0.0, -100.0, -110.0)
+
+
+def test_antiplane_boundaries_bit_exact(gpu):
+    """rates_*_mode3 (rates.f90:8-381), set_bc_mode3 (boundaries.f90:155-213)
+    and the mode-3 SAT: interior tiles, band and closures, bit for bit"""
+    prob, F = problems.build(nx=60, nys=(44,), seed=61, warp=0.05, background=BG3, perturb=1e-2,
+                             mode=3, bcs=("absorbing", "free", "rigid", "free"))
+    dev, orc = run_both(prob, F, 25)
+    compare(dev, orc, 0, bit_exact=True)
+
+
+def test_antiplane_locked_2x2_bit_exact(gpu):
+    """locked_interface_mode3 (interfaces.f90:676-712) on x and y interfaces"""
+    prob, F = problems.build(nx=36, nys=(30, 34), nbx=2, seed=67, warp=0.04, background=BG3,
+                             perturb=1e-2, coupling="locked", mode=3, inplane=INPLANE)
+    dev, orc = run_both(prob, F, 20)
+    compare(dev, orc, 0, bit_exact=True)
+
+
+@pytest.mark.parametrize("law", ["RFL", "RAL", "SL", "SW", "frictionless"])
+def test_antiplane_friction_laws(gpu, law):
+    """frictional_interface_mode3 (interfaces.f90:806-897)"""
+    prob, F = problems.build(nx=48, nys=(20, 22), seed=71, warp=0.03, background=[0.0, 0.0, 70.0],
+                             perturb=0.0, law=law, fault_overstress=25.0, mode=3,
+                             inplane=INPLANE)
+    dev, orc = run_both(prob, F, 60)
+    compare(dev, orc, 1, bit_exact=False)
+    if law in ("RFL", "RAL"):
+        assert orc.get_iface(0, "V").max() > 1e-3
+
+
+def test_antiplane_fp32_bulk_error_against_fp64(gpu):
+    kw = dict(nx=96, nys=(40, 40), seed=73, warp=0.03, background=[0.0, 0.0, 70.0], perturb=0.0,
+              law="RFL", fault_overstress=25.0, mode=3, inplane=INPLANE)
+    p32, F = problems.build(precision="fp32", **kw)
+    p64, _ = problems.build(precision="fp64", **kw)
+    dev = Context(p32)
+    orc = OracleContext(p64, threads=4)
+    for c in (dev, orc):
+        c.set_fields(F)
+        c.init_edges(0.0)
+        c.step(0.0, DT, 1, 150)
+    e = {"fields": rel_l2(dev.get_fields()[0], orc.get_fields()[0])}
+    for f in ("D", "V", "S"):
+        e[f] = rel_l2(dev.get_iface(0, f), orc.get_iface(0, f))
+    print("mode 3 FP32-bulk vs FP64 relative L2", e)
+    assert orc.get_iface(0, "V").max() > 1e-3
+    for k, v in e.items():
+        assert v <= 1e-5, (k, v)
+
+
 def test_fp32_bulk_error_against_fp64(gpu):
     """FP32-bulk (fields FP32 as deviations from a per-block background,
     friction FP64) against the FP64 oracle on a prestressed rupture: the
