@@ -226,6 +226,65 @@ def run_c2_variant(args, tmp, het, cdiss):
     print(json.dumps(out), flush=True)
 
 
+def run_m3(args, tmp):
+    """Antiplane shear (mode 3): two blocks of 1601 x 801 points (h = 0.025
+    km) side by side with a vertical aging-law fault between them, a free
+    surface on top, built from its .in file.  Per point per stage the
+    interior kernel reads vz, sxz, syz, their rates and the 5 metric planes
+    and writes fields and rates: 136 B in FP64 (112 B at stage 1, where the
+    rates are not read), half in FP32."""
+    from fdmap_github_amd import inputs
+
+    # each block is --ny wide and --nx high: C2's point count, turned upright
+    inpath = inputs.write(os.path.join(tmp, "m3.in"),
+                          inputs.antiplane_rs(name=os.path.join(tmp, "m3"), nx=args.ny, ny=args.nx,
+                                              h=0.025,
+                                              nsteps=args.steps + args.warmup + args.prof_steps,
+                                              outputs=False, ld_width=4.0, ninfo=100000))
+    res = {}
+    for prec in ("fp64",) if args.no_fp32 else ("fp64", "fp32"):
+        r = run_device(inpath, prec, args.steps, args.warmup, args.prof_steps)
+        ns, prof = r["nstage"], r["prof"]
+        b1, b2 = (112.0, 136.0) if prec == "fp64" else (56.0, 68.0)
+        kpts = prof["bulk_points"]
+        bytes_launch = kpts * (b1 + (ns - 1) * b2) / ns
+        avg_ms = prof["bulk_ms"] / max(1, prof["bulk_launches"])
+        res[prec] = dict(r=r, ups=r["npts"] * args.steps / r["el"], avg_ms=avg_ms,
+                         achieved=bytes_launch / (avg_ms * 1e-3) / 1e9, bytes_launch=bytes_launch,
+                         kpts=kpts)
+    r64 = res["fp64"]
+    out = {
+        "metric": METRIC, "value": r64["ups"], "unit": "updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * r64["r"]["el"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (antiplane fault problem generated as an FDMAP .in file)",
+        "config": {"workload": "antiplane shear (mode 3) rate-and-state rupture, 2 blocks "
+                               f"x ({r64['r']['hp'].nx // 2}x{r64['r']['hp'].ny}) side by side, "
+                               "vertical RAL fault, free surface, LSRK(5,4) CFL 0.5",
+                   "grid_points": r64["r"]["npts"], "fault_nodes": r64["r"]["hp"].ny},
+        "roofline": {
+            "bound": "hbm", "kernel": "bulk_interior3<double,64,4,NT>",
+            "kernel_points": r64["kpts"], "achieved": r64["achieved"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS, "traffic": None,
+            "alg_bytes_per_launch": r64["bytes_launch"], "avg_launch_ms": r64["avg_ms"],
+        },
+    }
+    if "fp32" in res:
+        r32 = res["fp32"]
+        out["fp32_bulk"] = {
+            "updates_per_s": r32["ups"], "speedup_vs_fp64": r32["ups"] / r64["ups"],
+            "ms_per_step": 1e3 * r32["r"]["el"] / args.steps,
+            "bulk_avg_launch_ms": r32["avg_ms"], "bulk_achieved_GBs": r32["achieved"],
+            "rel_l2_fields_vs_fp64": rel_l2(r32["r"]["F"], r64["r"]["F"]),
+            "rel_l2_slip_vs_fp64": rel_l2(r32["r"]["fault"]["D"], r64["r"]["fault"]["D"]),
+            "rel_l2_slip_velocity_vs_fp64": rel_l2(r32["r"]["fault"]["V"],
+                                                   r64["r"]["fault"]["V"]),
+            "after_steps": args.steps + args.warmup,
+        }
+    print(json.dumps(out), flush=True)
+
+
 def run_c4(args, tmp):
     """Ensemble: scenarios with block sizes uniform in 401..2001 x 201..1001,
     lognormal fault shear prestress (mean 70 MPa, log sd 0.1, correlation
@@ -443,10 +502,11 @@ def main():
                     help="steps timed after the headline ones with HIP events per launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--config", choices=("c2", "c2het", "c2diss", "c3", "c4", "c5"), default="c2",
+    ap.add_argument("--config", choices=("c2", "c2het", "c2diss", "m3", "c3", "c4", "c5"), default="c2",
                     help="c2: planar rupture (the headline); c2het: the same in a "
                          "heterogeneous medium; c2diss: the same with artificial dissipation "
-                         "(Cdiss 0.1); c3: rough curvilinear fault with "
+                         "(Cdiss 0.1); m3: antiplane shear rupture, 2 x (801x1601); "
+                         "c3: rough curvilinear fault with "
                          "off-fault plasticity, 2 x (4001x2001), FP64 and FP32-bulk; c4: 64 "
                          "uneven rupture scenarios advanced as one graph launch per step; c5: "
                          "adjoint gradient of a receiver misfit, 2 x (801x401), 2000 steps")
@@ -461,6 +521,9 @@ def main():
     from fdmap_github_amd import inputs
 
     tmp = tempfile.mkdtemp(prefix="fdmap_bench_")
+    if args.config == "m3":
+        run_m3(args, tmp)
+        return
     if args.config in ("c2het", "c2diss"):
         run_c2_variant(args, tmp, het=args.config == "c2het",
                        cdiss=0.1 if args.config == "c2diss" else 0.0)
