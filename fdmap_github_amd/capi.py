@@ -101,6 +101,7 @@ class fdm_iface(C.Structure):
         ("ld_T", C.c_double),
         ("nhat", _dp), ("rs", _dp), ("sw", _dp), ("S0base", _dp), ("N0base", _dp),
         ("ldS", _dp), ("ldN", _dp), ("outside", _ip), ("Psi0", _dp),
+        ("force", C.c_int32), ("pad2_", C.c_int32), ("kn", C.c_double * 8), ("kn_x", _dp),
     ]
 
 

@@ -767,7 +767,7 @@ int adj_stage(fdm_adjoint* a, int n, int s, int& cur) {
     e.blocks = c->blocks;
     e.sd = SidesDev{c->sM, c->snormal, c->sF0, c->sx, c->sy, c->sFhat, c->NS};
     e.fi = IfacesDev{c->ifaces, c->inhat, c->irs, c->isw, c->iS0b, c->iN0b, c->ildS, c->ildN,
-                     c->iout, c->ist, c->ist1, c->NI};
+                     c->iout, c->iknx, c->ist, c->ist1, c->NI};
     e.trace = a->trace + ((size_t)((n - 1) % K) * ns + (s - 1)) * 3 * c->NI;
     e.lpsi = a->lpsi;
     e.ldpsi = a->ldpsi;
@@ -868,10 +868,11 @@ int fdm_adjoint_create(fdm_ctx* c, int32_t nrec, const int32_t* rec_i, const int
       const bool law_ok = I.law == FDM_LAW_RSL || I.law == FDM_LAW_RFL || I.law == FDM_LAW_RAL;
       bool out_any = false;
       for (int k = 0; k < I.n; k++) out_any = out_any || outside[I.node_off + k];
-      if (I.coupling != FDM_COUPLING_FRICTION || !law_ok || I.reg || out_any)
+      if (I.coupling != FDM_COUPLING_FRICTION || !law_ok || I.reg || out_any || I.force)
         return adj_err(nullptr, FDM_EINVAL,
                        "fdm_adjoint_create: interfaces must be locked or frictional with law RSL, RFL "
-                       "or RAL, without normal-stress regularisation or locked region");
+                       "or RAL, without normal-stress regularisation, locked region or kinematic "
+                       "forcing");
     }
   }
   for (int r = 0; r < nrec; r++)

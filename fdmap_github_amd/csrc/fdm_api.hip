@@ -195,6 +195,7 @@ struct fdm_ctx {
   double* ildS = nullptr;
   double* ildN = nullptr;
   int* iout = nullptr;
+  double* iknx = nullptr;  // [NI] distance along the fault (kinematic forcing)
   double* ist = nullptr;
   double* ist1 = nullptr;
   bool any_friction = false;
@@ -406,7 +407,7 @@ EdgeArgs<T> edge_args(fdm_ctx* c, double t, double Bdt, double A, int do_scale, 
   a.blocks = c->blocks;
   a.sd = SidesDev{c->sM, c->snormal, c->sF0, c->sx, c->sy, c->sFhat, c->NS};
   a.fi = IfacesDev{c->ifaces, c->inhat, c->irs, c->isw, c->iS0b, c->iN0b, c->ildS, c->ildN,
-                   c->iout, c->ist, c->ist1, c->NI};
+                   c->iout, c->iknx, c->ist, c->ist1, c->NI};
   a.t = t;
   a.nstep = c->graph_mode ? c->nstep_dev : nullptr;
   a.t_init = c->g_tinit;
@@ -1085,6 +1086,14 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     D.skempton = I.skempton;
     D.flock = I.flock;
     D.ld_T = I.ld_T;
+    D.force = I.coupling == FDM_COUPLING_FRICTION && I.force ? 1 : 0;
+    D.pad2 = 0;
+    for (int q = 0; q < 8; q++) D.kn[q] = I.kn[q];
+    if (D.force && !I.kn_x) {
+      set_err(c, FDM_EINVAL, "fdm_create: interface " + std::to_string(ii) +
+                                 " has kinematic forcing and no kn_x");
+      return fail(FDM_EINVAL);
+    }
     c->ld_T.push_back(I.ld_T);
     for (int k = 0; k < n; k++) tasks.push_back(make_int2(-(ii + 1), k));
     NI += n;
@@ -1109,6 +1118,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     std::vector<double> nh(2 * NIs, 0.0), rs(7 * NIs, 1.0), sw(4 * NIs, 1.0), S0b(NIs, 0.0),
         N0b(NIs, 0.0), ldS(NIs, 0.0), ldN(NIs, 0.0), st((size_t)FDM_IF_NFIELDS * NIs, 0.0);
     std::vector<int> outv(NIs, 0);
+    std::vector<double> knx(NIs, 0.0);
     for (int ii = 0; ii < c->ni; ii++) {
       const fdm_iface& I = P->ifaces[ii];
       int n = c->hifaces[ii].n, o = c->hifaces[ii].node_off;
@@ -1124,6 +1134,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
         if (I.ldS) ldS[o + k] = I.ldS[k];
         if (I.ldN) ldN[o + k] = I.ldN[k];
         if (I.outside) outv[o + k] = I.outside[k];
+        if (c->hifaces[ii].force) knx[o + k] = I.kn_x[k];
         // initial values (friction.f90:217-249)
         auto S = [&](int f) -> double& { return st[(size_t)f * NIs + o + k]; };
         S(FDM_IF_D) = 0.0;
@@ -1160,6 +1171,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     if ((rc = dupload(c, &c->ildS, ldS.data(), ldS.size() * 8))) return fail(rc);
     if ((rc = dupload(c, &c->ildN, ldN.data(), ldN.size() * 8))) return fail(rc);
     if ((rc = dupload(c, &c->iout, outv.data(), outv.size() * sizeof(int)))) return fail(rc);
+    if ((rc = dupload(c, &c->iknx, knx.data(), knx.size() * 8))) return fail(rc);
     if ((rc = dupload(c, &c->ist, st.data(), st.size() * 8))) return fail(rc);
     if ((rc = dupload(c, &c->ist1, st.data(), st.size() * 8))) return fail(rc);
   }

@@ -38,6 +38,9 @@ struct BlockDev {
 struct IfaceDev {
   int direction, bm, bp, m, p, coupling, law, reg, rup_field, node_off, n, pad;
   double rup_threshold, skempton, flock, ld_T;
+  // kinematic forcing: c0, fd, dfdx, vL, vR, xL, xR, tmax
+  int force, pad2;
+  double kn[8];
 };
 
 struct SourcePlaceDev {
@@ -67,6 +70,7 @@ struct IfacesDev {
   const double* ldS;
   const double* ldN;
   const int* outside;
+  const double* knx;     // [NI] distance along the fault (kinematic forcing)
   double* st;            // [18][NI] fault state (FDM_IF_* order)
   double* st1;           // [18][NI] stage-1 snapshot
   int NI;
@@ -179,9 +183,10 @@ __device__ __forceinline__ void edge_iface3(const EdgeArgs<T>& a, const IfaceDev
            Nreg = ST(FDM_IF_NREG);
     const double D = ST(FDM_IF_D), Psi = ST(FDM_IF_PSI);
     const double E = I.law >= FDM_LAW_RSL ? exp(Psi / rs[0]) : 0.0;
+    const double kf = I.force ? kinematic_friction(I.kn, a.fi.knx[q], tst) : 0.0;
     const bool fail = solve_friction_core(I.law, rs, sw, I.reg, a.fi.outside[q], I.flock, fS0, fN0,
                                           phip, phis, etas, D, Psi, E, I.skempton, a.Bdt, V, S, O, N,
-                                          Nreg);
+                                          Nreg, I.force, I.kn[0], kf);
     if (fail) *a.fail = 1;
     ST(FDM_IF_V) = V;
     ST(FDM_IF_S) = S;
@@ -400,8 +405,10 @@ __device__ __forceinline__ void edge_task(const EdgeArgs<T>& a, const int2 task)
     // exp(Psi/a) of the regularised laws, shared by every strength
     // evaluation of the Newton solve and by the state rate
     const double E = I.law >= FDM_LAW_RSL ? exp(Psi / rs[0]) : 0.0;
+    const double kf = I.force ? kinematic_friction(I.kn, a.fi.knx[q], tst) : 0.0;
     bool fail = solve_friction_core(I.law, rs, sw, I.reg, a.fi.outside[q], I.flock, fS0, fN0,
-                                    phip, phis, etas, D, Psi, E, I.skempton, a.Bdt, V, S, O, N, Nreg);
+                                    phip, phis, etas, D, Psi, E, I.skempton, a.Bdt, V, S, O, N, Nreg,
+                                    I.force, I.kn[0], kf);
     if (fail) *a.fail = 1;
     if (a.trace) {
       a.trace[q] = V;

@@ -365,13 +365,28 @@ __device__ __forceinline__ double state_rate(int law, const double* rs, double V
 
 // solve_friction after load_stress (friction.f90:655-826).  Returns true on
 // a second Newton failure (the reference stops the run there).
+// kinematic forcing (friction.f90:717-745): max(fkm, fkp) of a node at
+// distance xx along the fault at time t; kn = c0, fd, dfdx, vL, vR, xL, xR,
+// tmax
+__device__ __forceinline__ double kinematic_friction(const double* kn, double xx, double t) {
+  const double xm = kn[5] + kn[3] * fmin(t, kn[7]);
+  const double fkm = xx < xm ? kn[1] + kn[2] * (xm - xx) : kn[1];
+  const double xp = kn[6] + kn[4] * fmin(t, kn[7]);
+  const double fkp = xx > xp ? kn[1] + kn[2] * (xx - xp) : kn[1];
+  return fmax(fkm, fkp);
+}
+
+// force: the forced strength Sk = kc0 + Nreg kf caps the slip-weakening
+// strength (friction.f90:765-767) and replaces a larger rate-and-state
+// strength (friction.f90:808-815)
 __device__ __forceinline__ bool solve_friction_core(int law, const double* rs, const double* sw,
                                                     int reg, int outside, double flock,
                                                     double S0i, double N0i, double phip,
                                                     double phis, double eta, double D,
                                                     double Psi, double E, double skempton,
                                                     double dt, double& V, double& S, double& O,
-                                                    double& N, double& Nreg) {
+                                                    double& N, double& Nreg, int force = 0,
+                                                    double kc0 = 0.0, double kf = 0.0) {
   double Slock = S0i + phis;
   double Nlock = N0i - (1.0 - skempton) * phip;
   O = 0.0;
@@ -407,6 +422,7 @@ __device__ __forceinline__ bool solve_friction_core(int law, const double* rs, c
     case FDM_LAW_SW: {
       double c0 = sw[0], fs = sw[1], fd = sw[2], Dc = sw[3];
       double Sv = c0 + fmax(Nreg, 0.0) * ((fs - (fs - fd) * fmin(D, Dc) / Dc));
+      if (force) Sv = fmin(Sv, kc0 + Nreg * kf);
       if (fabs(Slock) >= Sv) {
         Sv = fsign(Sv, Slock);
         V = (Slock - Sv) / eta;
@@ -420,6 +436,13 @@ __device__ __forceinline__ bool solve_friction_core(int law, const double* rs, c
     default: {
       bool fail = newton_solver(law, rs[0], rs[2], V, S, Slock, Nreg, eta, Psi, E);
       if (fail) fail = newton_solver(law, rs[0], rs[2], V, S, Slock, Nreg, eta, Psi, E);
+      if (force) {
+        const double Sk = kc0 + Nreg * kf;
+        if (Sk < fabs(S)) {
+          S = fsign(Sk, Slock);
+          V = (Slock - S) / eta;
+        }
+      }
       return fail;
     }
   }

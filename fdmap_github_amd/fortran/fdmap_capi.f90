@@ -52,6 +52,9 @@ module fdmap_capi
      integer(c_int32_t) :: direction, blockm, blockp, m, p, coupling, law, reg, rup_field, pad_
      real(c_double) :: rup_threshold, skempton, flock, ld_T
      type(c_ptr) :: nhat, rs, sw, S0base, N0base, ldS, ldN, outside, Psi0
+     integer(c_int32_t) :: force = 0, pad2_ = 0
+     real(c_double) :: kn(8) = 0.0_c_double
+     type(c_ptr) :: kn_x = c_null_ptr
   end type fdm_iface
 
   type,bind(C) :: fdm_source_place

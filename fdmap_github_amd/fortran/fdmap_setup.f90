@@ -64,6 +64,11 @@ module fdmap_setup
      logical :: reg
      real(dp),allocatable :: nhat(:,:),x(:),y(:),rs(:,:),sw(:,:),S0b(:),N0b(:),ldS(:),ldN(:),Psi0(:)
      integer(c_int32_t),allocatable :: outside(:)
+     ! kinematic forcing: c0, fd, dfdx, vL, vR, xL, xR, tmax, and the distance
+     ! along the fault of every node
+     logical :: force = .false.
+     real(dp) :: kn(8) = 0.0_dp
+     real(dp),allocatable :: knx(:)
   end type host_iface
 
   ! problem_list (main.f90:29)
@@ -1050,7 +1055,6 @@ contains
          end if
          read(input,nml=friction_list,iostat=stat)
          if (stat>0) call fail('Error in friction_list')
-         if (force) call fail('kinematic forcing (force = T) is not supported')
          if (trim(problem)/='') call fail("friction problem '"//trim(problem)//"' is not supported")
          if (strength_file) call fail('strength_file is not supported')
          if (trim(rup_field)/='V'.and.trim(rup_field)/='D') &
@@ -1064,7 +1068,10 @@ contains
          I%rup_field = rup_field
          I%reg = reg
          I%ld_T = ld%T
-         allocate(I%rs(n,7),I%sw(n,4),I%S0b(n),I%N0b(n),I%ldS(n),I%ldN(n),I%Psi0(n),I%outside(n))
+         I%force = force
+         I%kn = (/ kn%c0,kn%fd,kn%dfdx,kn%vL,kn%vR,kn%xL,kn%xR,kn%tmax /)
+         allocate(I%rs(n,7),I%sw(n,4),I%S0b(n),I%N0b(n),I%ldS(n),I%ldN(n),I%Psi0(n),I%outside(n), &
+              I%knx(n))
          I%rs(:,1) = rs%a; I%rs(:,2) = rs%b; I%rs(:,3) = rs%V0; I%rs(:,4) = rs%f0
          I%rs(:,5) = rs%L; I%rs(:,6) = rs%fw; I%rs(:,7) = rs%Vw
          I%sw(:,1) = sw%c0; I%sw(:,2) = sw%fs; I%sw(:,3) = sw%fd; I%sw(:,4) = sw%Dc
@@ -1086,6 +1093,7 @@ contains
          ang = 0.017453292519943_dp*angle
          do k = 1,n
             xx = I%x(k)*cos(ang)+I%y(k)*sin(ang)
+            I%knx(k) = xx
             r = abs((xx-ld%x0)/ld%R)
             select case(trim(ld%shape))
             case('')
@@ -1450,6 +1458,10 @@ contains
             ci%ldN = c_loc(I%ldN(1))
             ci%outside = c_loc(I%outside(1))
             ci%Psi0 = c_loc(I%Psi0(1))
+            ci%force = 0
+            if (I%force) ci%force = 1
+            ci%kn = I%kn
+            ci%kn_x = c_loc(I%knx(1))
          end if
        end associate
     end do

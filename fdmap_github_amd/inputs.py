@@ -210,6 +210,73 @@ vz sxz syz
 """
 
 
+def gp229_sw3(name="gp229", nx=801, nsteps=1000, CFL=1.0, outputs=True, ninfo=10):
+    """The reference's antiplane slip-weakening example (gp229/gp229sw3ex1.in):
+    two 80 km x 40 km blocks stacked in y with a horizontal fault, prestress
+    syy0 = -120, syz0 = 70 MPa, slip-weakening friction with forced rupture
+    expansion, and a smooth boxcar lowering the shear stress by 3 MPa for
+    x > 10 km.  nx points along the fault (801 in the example), nx/2 + 1
+    across each block.  The example leaves FDmethod at the reference's
+    default, SBP6 (domain.f90:94); this one asks for SBP4, the operator
+    built here."""
+    ny = nx // 2 + 1
+    out = ""
+    if outputs:
+        out = f"""I
+iface1
+D V S N
+0d0 1d10 1
+B
+body
+vz sxz syz
+0d0 1d10 10
+"""
+    return f"""&problem_list
+ name = '{name}', ninfo = {ninfo}, nt = {nsteps + 1}, CFL = {_f(CFL)}, refine = 1d0 /
+
+&domain_list
+ mode = 3, FDmethod = 'SBP4', nx = {nx}, ny = {2 * ny}, nblocks = 2, nblocks_x = 1, nblocks_y = 2,
+ nifaces = 1 /
+
+!---BLOCK1---
+
+&grid_list
+ nx = {nx}, ny = {ny}, mgx = 1, mgy = 1, iblock_x = 1, iblock_y = 1,
+ LT = -40d0,0d0, RT = 40d0,0d0, LB = -40d0,-40d0, RB = 40d0,-40d0 /
+
+&boundaries_list bcL = 'absorbing', bcR = 'absorbing', bcB = 'absorbing', bcT = 'none' /
+
+!---BLOCK2---
+
+&grid_list
+ nx = {nx}, ny = {ny}, mgx = 1, mgy = {ny + 1}, iblock_x = 1, iblock_y = 2,
+ LT = -40d0,40d0, RT = 40d0,40d0, LB = -40d0,0d0, RB = 40d0,0d0 /
+
+&boundaries_list bcL = 'absorbing', bcR = 'absorbing', bcB = 'none', bcT = 'absorbing' /
+
+!---ALL BLOCKS---
+
+&fields_list syy0 = -120d0, syz0 = 70d0 /
+
+&material_list rho = 2.67d0, cs = 3.464d0, cp = 6d0 /
+
+&operator_list Cdiss = 0d0 /
+
+!---IFACE1---
+
+&interface_list coupling = 'friction', iblockm = 1, iblockp = 2, direction = 'y', mg = 1, pg = {nx} /
+
+&friction_list
+ friction_law = 'SW', sw%fs = 0.677d0, sw%fd = 0.525d0, sw%Dc = 0.4d0,
+ force = T, kn%fd = 0.525d0, kn%dfdx = 0.2d0, kn%vL = -1d0, kn%vR = 1d0,
+ kn%xL = 0.76d0, kn%xR = -0.76d0, kn%tmax = 4d0,
+ ld%shape = 'smooth_boxcar', ld%S = -3d0, ld%x0 = 40d0, ld%R = 30d0 /
+
+!---begin:output_list---
+{out}!---end:output_list---
+"""
+
+
 def fractal_profile(nx, h, alpha=1e-2, seed=0, lam_min_h=20):
     """Seeded band-limited self-similar profile y(x) over nx points of
     spacing h: power spectrum ~ k^-3 between wavelengths lam_min_h*h and the

@@ -173,6 +173,11 @@ class Problem:
             ci.ldN = P(itf.get("ldN"))
             ci.outside = IP(itf.get("outside"))
             ci.Psi0 = P(itf.get("Psi0"))
+            if itf.get("force") is not None:
+                ci.force = 1
+                for q, v in enumerate(itf["force"]):
+                    ci.kn[q] = float(v)
+                ci.kn_x = P(itf["kn_x"])
         keep.append(iarr)
         prob.ifaces = iarr
 

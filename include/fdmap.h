@@ -141,6 +141,13 @@ typedef struct fdm_iface {
   const double* ldN;      /* [n] ld%N * shape(x) (friction.f90:1374)           */
   const int32_t* outside; /* [n] 1: node outside [xlockm,xlockp] (:686-698)    */
   const double* Psi0;     /* [n] initial state variable                        */
+  /* kinematic forcing (force = T, friction.f90:717-745, 765-767, 808-815):
+     kn = c0, fd, dfdx, vL, vR, xL, xR, tmax; kn_x [n] the distance along the
+     fault, x cos(angle) + y sin(angle), of every node (NULL without forcing) */
+  int32_t force;
+  int32_t pad2_;
+  double kn[8];
+  const double* kn_x;
 } fdm_iface;
 
 /* A placement of the singular point source in one block

@@ -370,6 +370,7 @@ typedef struct {
   fdm_iface I;
   int32_t n;
   double *nhat, *rs, *sw, *S0base, *N0base, *ldS, *ldN, *Psi0;
+  double* knx; /* distance along the fault, kinematic forcing only */
   int32_t* outside;
   double* f[FDM_IF_NFIELDS];
   double* st1[FDM_IF_NFIELDS];
@@ -534,6 +535,14 @@ int orc_create(const fdm_problem* P, orc_ctx** out) {
     I->ldS = I->I.ldS ? dup_arr(I->I.ldS, n) : alloc_fill(n, 0.0);
     I->ldN = I->I.ldN ? dup_arr(I->I.ldN, n) : alloc_fill(n, 0.0);
     I->Psi0 = I->I.Psi0 ? dup_arr(I->I.Psi0, n) : alloc_fill(n, 0.0);
+    I->knx = (I->I.coupling == FDM_COUPLING_FRICTION && I->I.force && I->I.kn_x)
+                 ? dup_arr(I->I.kn_x, n)
+                 : NULL;
+    if (I->I.coupling == FDM_COUPLING_FRICTION && I->I.force && !I->knx) {
+      snprintf(g_err, sizeof g_err, "orc_create: interface %d: kinematic forcing without kn_x", ii);
+      orc_destroy(c);
+      return FDM_EINVAL;
+    }
     I->outside = (int32_t*)calloc(n, sizeof(int32_t));
     if (I->I.outside) memcpy(I->outside, I->I.outside, n * sizeof(int32_t));
     /* friction.f90:217-249 initial values */
@@ -641,6 +650,7 @@ void orc_destroy(orc_ctx* c) {
       free(I->ldS);
       free(I->ldN);
       free(I->Psi0);
+      free(I->knx);
       free(I->outside);
       for (int k = 0; k < FDM_IF_NFIELDS; k++) {
         free(I->f[k]);
@@ -991,11 +1001,25 @@ static int newton_solver(int law, const double* rs, double* Vio, double* Sio, do
 
 /* Slock/Nlock part and law branches of solve_friction (friction.f90:655-826)
    once FR%S0(i), FR%N0(i) are known.  Returns 1 on Newton failure. */
-static int solve_friction_core(int law, const double* rs, const double* sw, int reg,
-                               int outside, double flock, double S0i, double N0i,
-                               double phip, double phis, double eta, double D, double Psi,
-                               double skempton, double dt, double* V, double* S, double* O,
-                               double* N, double* Nreg) {
+/* kinematic forcing (friction.f90:717-745): the forced friction
+   max(fkm, fkp) at distance xx along the fault and time t; kn = c0, fd,
+   dfdx, vL, vR, xL, xR, tmax */
+static double kinematic_friction(const double* kn, double xx, double t) {
+  double xm = kn[5] + kn[3] * fmin(t, kn[7]);
+  double fkm = xx < xm ? kn[1] + kn[2] * (xm - xx) : kn[1];
+  double xp = kn[6] + kn[4] * fmin(t, kn[7]);
+  double fkp = xx > xp ? kn[1] + kn[2] * (xx - xp) : kn[1];
+  return fmax(fkm, fkp);
+}
+
+/* solve_friction (friction.f90:593-828).  force: the forced strength
+   Sk = kc0 + Nreg kf caps the slip-weakening strength (:765-767) and
+   replaces a larger rate-and-state strength (:808-815) */
+static int solve_friction_core_f(int law, const double* rs, const double* sw, int reg,
+                                 int outside, double flock, double S0i, double N0i,
+                                 double phip, double phis, double eta, double D, double Psi,
+                                 double skempton, double dt, double* V, double* S, double* O,
+                                 double* N, double* Nreg, int force, double kc0, double kf) {
   double Slock = S0i + phis;
   double Nlock = N0i - (1.0 - skempton) * phip;
   *O = 0.0;
@@ -1031,6 +1055,7 @@ static int solve_friction_core(int law, const double* rs, const double* sw, int 
     case FDM_LAW_SW: {
       double c0 = sw[0], fs = sw[1], fd = sw[2], Dc = sw[3];
       double Sv = c0 + fmax(*Nreg, 0.0) * ((fs - (fs - fd) * fmin(D, Dc) / Dc));
+      if (force) Sv = fmin(Sv, kc0 + *Nreg * kf);
       if (fabs(Slock) >= Sv) {
         Sv = fsign(Sv, Slock);
         *V = (Slock - Sv) / eta;
@@ -1044,9 +1069,25 @@ static int solve_friction_core(int law, const double* rs, const double* sw, int 
     default: {
       int fail = newton_solver(law, rs, V, S, Slock, *Nreg, eta, Psi);
       if (fail) fail = newton_solver(law, rs, V, S, Slock, *Nreg, eta, Psi);
+      if (force) {
+        double Sk = kc0 + *Nreg * kf;
+        if (Sk < fabs(*S)) {
+          *S = fsign(Sk, Slock);
+          *V = (Slock - *S) / eta;
+        }
+      }
       return fail;
     }
   }
+}
+
+static int solve_friction_core(int law, const double* rs, const double* sw, int reg,
+                               int outside, double flock, double S0i, double N0i,
+                               double phip, double phis, double eta, double D, double Psi,
+                               double skempton, double dt, double* V, double* S, double* O,
+                               double* N, double* Nreg) {
+  return solve_friction_core_f(law, rs, sw, reg, outside, flock, S0i, N0i, phip, phis, eta, D,
+                               Psi, skempton, dt, V, S, O, N, Nreg, 0, 0.0, 0.0);
 }
 
 int orc_solve_friction_node(int32_t law, const double* rs, const double* sw, double S0,
@@ -1178,11 +1219,13 @@ static int frictional_interface_mode2(orc_ctx* c, oiface_t* I, int k, double* Fh
   }
   if (I->sw)
     for (int q = 0; q < 4; q++) swk[q] = I->sw[(size_t)q * n + k];
-  int fail = solve_friction_core(I->I.law, rs, swk, I->I.reg, I->outside[k], I->I.flock, *fS0,
-                                 *fN0, phip, phis, etas, I->f[FDM_IF_D][k],
-                                 I->f[FDM_IF_PSI][k], I->I.skempton, dt, &I->f[FDM_IF_V][k],
-                                 &I->f[FDM_IF_S][k], &I->f[FDM_IF_O][k], &I->f[FDM_IF_N][k],
-                                 &I->f[FDM_IF_NREG][k]);
+  int force = I->knx != NULL;
+  double kf = force ? kinematic_friction(I->I.kn, I->knx[k], t) : 0.0;
+  int fail = solve_friction_core_f(I->I.law, rs, swk, I->I.reg, I->outside[k], I->I.flock, *fS0,
+                                   *fN0, phip, phis, etas, I->f[FDM_IF_D][k],
+                                   I->f[FDM_IF_PSI][k], I->I.skempton, dt, &I->f[FDM_IF_V][k],
+                                   &I->f[FDM_IF_S][k], &I->f[FDM_IF_O][k], &I->f[FDM_IF_N][k],
+                                   &I->f[FDM_IF_NREG][k], force, I->I.kn[0], kf);
   double skm = 1.0 - I->I.skempton;
   double snnp = -(I->f[FDM_IF_N][k] - *fN0) / skm;
   double snnm = -(I->f[FDM_IF_N][k] - *fN0) / skm;
@@ -1268,11 +1311,13 @@ static int frictional_interface_mode3(orc_ctx* c, oiface_t* I, int k, double* Fh
   }
   if (I->sw)
     for (int q = 0; q < 4; q++) swk[q] = I->sw[(size_t)q * n + k];
-  int fail = solve_friction_core(I->I.law, rs, swk, I->I.reg, I->outside[k], I->I.flock, *fS0,
-                                 *fN0, phip, phis, etas, I->f[FDM_IF_D][k],
-                                 I->f[FDM_IF_PSI][k], I->I.skempton, dt, &I->f[FDM_IF_V][k],
-                                 &I->f[FDM_IF_S][k], &I->f[FDM_IF_O][k], &I->f[FDM_IF_N][k],
-                                 &I->f[FDM_IF_NREG][k]);
+  int force = I->knx != NULL;
+  double kf = force ? kinematic_friction(I->I.kn, I->knx[k], t) : 0.0;
+  int fail = solve_friction_core_f(I->I.law, rs, swk, I->I.reg, I->outside[k], I->I.flock, *fS0,
+                                   *fN0, phip, phis, etas, I->f[FDM_IF_D][k],
+                                   I->f[FDM_IF_PSI][k], I->I.skempton, dt, &I->f[FDM_IF_V][k],
+                                   &I->f[FDM_IF_S][k], &I->f[FDM_IF_O][k], &I->f[FDM_IF_N][k],
+                                   &I->f[FDM_IF_NREG][k], force, I->I.kn[0], kf);
   double snzp = I->f[FDM_IF_S][k] - *fS0;
   double snzm = I->f[FDM_IF_S][k] - *fS0;
   double vzp = vzFDp + Zsip * (snzFDp - snzp);

@@ -58,7 +58,7 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
           perturb=1e-3, source=False, plastic=None, nbx=1, rk_order=4, outside_frac=0.0,
           fault_overstress=25.0, ld_T=0.0, reg=False, rup_field="V", Vinit=1e-12,
           skempton=0.0, mms=False, medium=None, Cdiss=None, mode=2,
-          inplane=(-120.0, 0.0, -120.0, -120.0)):
+          inplane=(-120.0, 0.0, -120.0, -120.0), force=None):
     """Blocks stacked in y (len(nys) rows) and nbx columns; interfaces
     between them.  medium: amplitude of a smooth seeded heterogeneous medium
     (rho, cs, cp varying by that relative amount around the given values;
@@ -66,7 +66,10 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
     Cdiss: artificial dissipation of every block as the rates use it (a
     number, or one per block).  mode 3: antiplane shear, background
     (vz, sxz, syz); inplane (sxx, sxy, syy, szz) is the in-plane prestress
-    that frictional interfaces resolve onto the fault.
+    that frictional interfaces resolve onto the fault.  force: kinematic
+    forcing of every frictional interface, a dict of c0, fd, dfdx, vL, vR, xL,
+    xR, tmax (distance along the fault: x for horizontal faults, y for
+    vertical ones).
     Returns (Problem, F0 fields [6,ny,nx])."""
     rng = np.random.default_rng(seed)
     nyb = list(nys)
@@ -199,6 +202,22 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
             prob.ifaces.append(_iface(rng, "y", bm, bp, B["mx"], B["px"], nrm_side, n, X[B["py"], B["mx"]:B["px"] + 1],
                                       coupling, law, bg, fault_overstress, outside_frac, ld_T,
                                       reg, rup_field, Vinit, skempton, mode, inplane))
+    if force is not None:
+        kn = [float(force.get(k, d)) for k, d in
+              (("c0", 0.0), ("fd", 0.0), ("dfdx", 1.0), ("vL", 0.0), ("vR", 0.0), ("xL", 0.0),
+               ("xR", 0.0), ("tmax", 1e300))]
+        for itf in prob.ifaces:
+            if itf["coupling"] != "friction":
+                continue
+            m, p_ = itf["m"], itf["p"]
+            if itf["direction"] == "y":
+                bm = prob.blocks[itf["blockm"]]
+                xs = X[bm["py"], m:p_ + 1]
+            else:
+                bm = prob.blocks[itf["blockm"]]
+                xs = Y[m:p_ + 1, bm["px"]]
+            itf["force"] = kn
+            itf["kn_x"] = np.ascontiguousarray(xs, dtype=np.float64)
     # initial fields: background + smooth random perturbation
     F = np.empty((nF, ny, NX))
     nvel = 1 if mode == 3 else 2

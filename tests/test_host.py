@@ -158,3 +158,23 @@ def test_antiplane_problem(tmp_path):
     assert bad != txt
     with pytest.raises(RuntimeError, match="mode 3 boundary"):
         host.HostProblem(inputs.write(str(tmp_path / "b2.in"), bad))
+
+
+def test_kinematic_forcing_is_read(tmp_path):
+    """friction_list force = T with kn%... (friction.f90:88-126): the
+    parameters in order c0, fd, dfdx, vL, vR, xL, xR, tmax, and the distance
+    along the fault x cos(angle) + y sin(angle) of every node"""
+    txt = inputs.c2_planar_rs(name=str(tmp_path / "k"), nx=41, ny=21, nsteps=5, outputs=False,
+                              law="SW")
+    marker = "&friction_list"
+    assert marker in txt
+    txt = txt.replace(marker, marker + " force = T, kn%fd = 0.525d0, kn%dfdx = 0.2d0, kn%vL = -1d0,"
+                      " kn%vR = 1d0, kn%xL = 0.76d0, kn%xR = -0.76d0, kn%tmax = 4d0,", 1)
+    hp = host.HostProblem(inputs.write(str(tmp_path / "k.in"), txt))
+    I = hp.ptr.contents.ifaces[0]
+    n = I.p - I.m + 1
+    assert I.force == 1
+    assert list(I.kn) == [0.0, 0.525, 0.2, -1.0, 1.0, 0.76, -0.76, 4.0]
+    x = np.ctypeslib.as_array(I.kn_x, shape=(n,))
+    h = 0.025
+    np.testing.assert_allclose(x, (np.arange(n) - (n - 1) / 2) * h, atol=1e-12)

@@ -138,3 +138,20 @@ def test_antiplane_executable_outputs(gpu, tmp_path):
                                atol=1e-12)
     m = open(str(tmp_path / "m3x.m")).read()
     assert re.search(r"^mode = +3;$", m, re.M)
+
+
+def test_gp229_forced_slip_weakening_rupture(gpu, tmp_path):
+    """the reference's antiplane slip-weakening example with forced rupture
+    expansion (gp229/gp229sw3ex1.in), at a quarter of its resolution"""
+    p = inputs.write(str(tmp_path / "gp.in"),
+                     inputs.gp229_sw3(name=str(tmp_path / "gp"), nx=201, nsteps=150, outputs=False))
+    hp, dev, orc = run_pair(p, 150)
+    e = rel_l2(dev.get_fields()[0], orc.get_fields()[0])
+    print("fields", e)
+    assert e <= 1e-12
+    for f in ("D", "V", "S", "N"):
+        a, b = dev.get_iface(0, f), orc.get_iface(0, f)
+        print(f, rel_l2(a, b))
+        assert rel_l2(a, b) <= 1e-12
+    V = orc.get_iface(0, "V")
+    assert V.max() > 0.1 and np.sum(V > 0) > 10  # the forced rupture has grown

@@ -616,6 +616,43 @@ def test_mode3_steady_sliding_under_resolved_prestress():
     np.testing.assert_allclose(V, 1e-12, rtol=1e-6)
 
 
+@pytest.mark.parametrize("mode", [2, 3])
+def test_kinematic_forcing_moves_the_rupture_tips(mode):
+    """force = T (friction.f90:717-745, 765-767): with a static friction no
+    prestress overcomes, only the forced strength c0 + N max(fkm, fkp) lets
+    the fault slip; its tips xL + vL t and xR + vR t bound the slipping
+    nodes, and inside them the fault slides at the forced strength"""
+    v, dt = 2.0, 1e-3
+    kn = dict(c0=0.0, fd=0.525, dfdx=20.0, vL=-v, vR=v, xL=0.05, xR=-0.05, tmax=1e9)
+    if mode == 2:
+        bg, kw = [0.0, 0.0, -120.0, 70.0, -120.0, -120.0], {}
+    else:
+        bg, kw = [0.0, 0.0, 70.0], dict(inplane=(-120.0, 0.0, -120.0, -120.0))
+    prob, F = problems.build(nx=81, nys=(20, 22), seed=5, warp=0.0, background=bg, perturb=0.0,
+                             law="SW", fault_overstress=0.0, mode=mode, force=kn, **kw)
+    for itf in prob.ifaces:
+        itf["sw"][1] = 50.0  # static friction out of reach
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    nsteps = 200
+    o.step(0.0, dt, 1, nsteps)
+    o.stage1(0.0, dt, nsteps + 1)
+    t = nsteps * dt
+    x = prob.ifaces[0]["kn_x"]
+    V = o.get_iface(0, "V", stage1=True)
+    S = o.get_iface(0, "S", stage1=True)
+    N = o.get_iface(0, "N", stage1=True)
+    tip = -0.05 + v * t
+    # beyond the tips the forced strength exceeds the prestress by the
+    # margin dfdx (|x| - tip) N: locked, exactly
+    far = np.abs(x) > tip + 0.02
+    assert np.all(V[far] == 0.0)
+    inside = np.abs(x) < tip - 0.01
+    assert inside.sum() > 5 and np.all(V[inside] > 0.0)
+    np.testing.assert_allclose(S[inside], 0.525 * N[inside], rtol=1e-14)
+
+
 def mms_error(n, tend=0.04, warp=0.05):
     G, cs, cp = 32.0, 3.0, 5.0
     rho = G / cs ** 2

@@ -244,6 +244,22 @@ def test_antiplane_fp32_bulk_error_against_fp64(gpu):
         assert v <= 1e-5, (k, v)
 
 
+@pytest.mark.parametrize("mode,law", [(2, "SW"), (3, "SW"), (2, "RFL")])
+def test_kinematic_forcing(gpu, mode, law):
+    """force = T (friction.f90:717-745, 765-767, 808-815): tips expanding at
+    2 km/s from the fault centre"""
+    kn = dict(c0=0.0, fd=0.525, dfdx=0.5, vL=-2.0, vR=2.0, xL=0.05, xR=-0.05, tmax=0.08)
+    if mode == 2:
+        bg, kw = BG, {}
+    else:
+        bg, kw = [0.0, 0.0, 70.0], dict(inplane=INPLANE)
+    prob, F = problems.build(nx=60, nys=(22, 24), seed=79, warp=0.02, background=bg, perturb=0.0,
+                             law=law, fault_overstress=0.0, mode=mode, force=kn, **kw)
+    dev, orc = run_both(prob, F, 80)
+    compare(dev, orc, 1, bit_exact=False)
+    assert orc.get_iface(0, "V").max() > 1e-3
+
+
 def test_fp32_bulk_error_against_fp64(gpu):
     """FP32-bulk (fields FP32 as deviations from a per-block background,
     friction FP64) against the FP64 oracle on a prestressed rupture: the
