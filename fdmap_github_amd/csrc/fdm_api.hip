@@ -234,6 +234,8 @@ struct fdm_ctx {
   // artificial dissipation in some block: the Jacobian plane
   bool diss = false;
   void* Jp = nullptr;
+  // kinematic forcing on some interface: the edge kernels carrying it
+  bool any_force = false;
   // 2: plane strain, 6 fields; 3: antiplane shear, 3 fields
   int mode = 2, nF = 6;
   std::vector<double> hnu;  // nu of the medium ([ny][nx]), for restore_df6
@@ -432,7 +434,14 @@ int launch_edges(fdm_ctx* c, hipStream_t s, double t, double Bdt, double A, int 
   const EdgeArgs<T> a = edge_args<T>(c, t, Bdt, A, do_scale, do_update, snapshot, Bload);
   dim3 grid((c->ntasks + 63) / 64), block(64);
   EventPair* ep = prof_begin(c, 1, s);
-  hipLaunchKernelGGL(edge_kernel<T>, grid, block, 0, s, a);
+  if (c->mode == 3 && c->any_force)
+    hipLaunchKernelGGL((edge_kernel<T, 3, true>), grid, block, 0, s, a);
+  else if (c->mode == 3)
+    hipLaunchKernelGGL((edge_kernel<T, 3, false>), grid, block, 0, s, a);
+  else if (c->any_force)
+    hipLaunchKernelGGL((edge_kernel<T, 2, true>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((edge_kernel<T, 2, false>), grid, block, 0, s, a);
   prof_end(c, ep, 1, s);
   HIPCHK(c, hipGetLastError());
   return FDM_OK;
@@ -503,10 +512,14 @@ void launch_band(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   dim3 grid((c->nband + 63) / 64);
   if (c->mode == 3)
     hipLaunchKernelGGL(bulk_band3<T>, grid, dim3(64), 0, s, a, c->band, c->nband);
+  else if (full_variant(c, a) && (c->het || c->diss))
+    hipLaunchKernelGGL((bulk_band<T, true, true>), grid, dim3(64), 0, s, a, c->band, c->nband);
   else if (full_variant(c, a))
-    hipLaunchKernelGGL((bulk_band<T, true>), grid, dim3(64), 0, s, a, c->band, c->nband);
+    hipLaunchKernelGGL((bulk_band<T, true, false>), grid, dim3(64), 0, s, a, c->band, c->nband);
+  else if (c->het || c->diss)
+    hipLaunchKernelGGL((bulk_band<T, false, true>), grid, dim3(64), 0, s, a, c->band, c->nband);
   else
-    hipLaunchKernelGGL((bulk_band<T, false>), grid, dim3(64), 0, s, a, c->band, c->nband);
+    hipLaunchKernelGGL((bulk_band<T, false, false>), grid, dim3(64), 0, s, a, c->band, c->nband);
   prof_end(c, ep, 2, s);
 }
 
@@ -1089,6 +1102,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     D.force = I.coupling == FDM_COUPLING_FRICTION && I.force ? 1 : 0;
     D.pad2 = 0;
     for (int q = 0; q < 8; q++) D.kn[q] = I.kn[q];
+    if (D.force) c->any_force = true;
     if (D.force && !I.kn_x) {
       set_err(c, FDM_EINVAL, "fdm_create: interface " + std::to_string(ii) +
                                  " has kinematic forcing and no kn_x");
@@ -1536,7 +1550,7 @@ int fdm_tune(fdm_ctx* c, int32_t knob, int32_t value) {
 struct fdm_ensemble {
   std::vector<fdm_ctx*> m;
   int precision = FDM_FP64, nstage = 0, flip = 0, swz = 1;
-  bool any_plastic = false, use_graph = true, nt_store = false;
+  bool any_plastic = false, use_graph = true, nt_store = false, any_force = false;
   hipStream_t master = nullptr, side = nullptr;
   hipEvent_t evJ = nullptr, evI = nullptr, evB = nullptr, done = nullptr;
   std::vector<hipEvent_t> pre;  // each member's earlier work, before a step
@@ -1654,9 +1668,12 @@ int ens_step_launches(fdm_ensemble* e, int p) {
     if (full) ens_interior<T, true, false>(e, bt);
     else if (e->nt_store) ens_interior<T, false, true>(e, bt);
     else ens_interior<T, false, false>(e, bt);
-    if (e->ntasks)
-      hipLaunchKernelGGL(edge_kernel_batch<T>, dim3(e->ntasks / 64), dim3(64), 0, e->side, et, e->tasks,
-                         e->ntasks);
+    if (e->ntasks && e->any_force)
+      hipLaunchKernelGGL((edge_kernel_batch<T, true>), dim3(e->ntasks / 64), dim3(64), 0, e->side, et,
+                         e->tasks, e->ntasks);
+    else if (e->ntasks)
+      hipLaunchKernelGGL((edge_kernel_batch<T, false>), dim3(e->ntasks / 64), dim3(64), 0, e->side, et,
+                         e->tasks, e->ntasks);
     ENSCHK(e, hipStreamWaitEvent(e->side, e->evI, 0));
     if (e->nband) {
       if (full)
@@ -1777,6 +1794,7 @@ int fdm_ensemble_create(fdm_ctx* const* members, int32_t n, fdm_ensemble** out) 
   double ws = 0.0;
   for (fdm_ctx* c : e->m) {
     e->any_plastic = e->any_plastic || c->any_plastic;
+    e->any_force = e->any_force || c->any_force;
     ws += 23.0 * (double)c->ps * (double)c->esz;
   }
   e->nt_store = nt_store_choice(ws);

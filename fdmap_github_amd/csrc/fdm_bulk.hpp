@@ -506,8 +506,9 @@ __device__ __forceinline__ void diss_central(T (&d)[6], const T* __restrict__ F,
 // dissipation term, and strips add the diagonal between their two loops
 // (rates_dissipation_boundary_mode2).  The stencil loops are unrolled so
 // that all of a point's loads are in flight together: the band is a thin
-// set of points, latency-bound.
-template <typename T>
+// set of points, latency-bound.  EXT = false leaves out the heterogeneous
+// medium and dissipation code (launches of problems with neither).
+template <typename T, bool EXT>
 __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockDev& B, const int i,
                                               const int j, const long long cc, T (&d)[6]) {
   const long long ps = a.ps;
@@ -531,10 +532,10 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
   const T Ji = a.Ji[cc];
   // the point's material in a heterogeneous medium (rates_h_boundary_mode2,
   // rates_heterogeneous.f90:435-898, the same expressions)
-  const bool het = a.mrho != nullptr;
+  const bool het = EXT && a.mrho != nullptr;
   const T gamma = het ? a.mgam[cc] : (T)B.gamma, rho = het ? a.mrho[cc] : (T)B.rho;
   const T G = het ? a.mG[cc] : (T)B.G, M = het ? a.mM[cc] : (T)B.M;
-  const bool diss = B.Cdiss > 0.0;
+  const bool diss = EXT && B.Cdiss > 0.0;
   const T Cd = (T)B.Cdiss;
   const T* __restrict__ J = a.J;
   const bool qone = (dl < 4) || (dr < 4);
@@ -624,8 +625,8 @@ __device__ __forceinline__ void scaled_rates(const BulkArgs<T>& a, const long lo
 
 // Everything that happens at one grid point in one stage, reading its
 // neighbours straight from global memory: the per-point kernel, and the
-// closure points of the tile and marching kernels.
-template <typename T, bool FULL>
+// closure points of the tile and marching kernels.  EXT as in closure_rates.
+template <typename T, bool FULL, bool EXT>
 __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i, const int j) {
   const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
   const BlockDev& B = a.blocks[bi];
@@ -636,36 +637,21 @@ __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i,
   scaled_rates<T>(a, cc, d);
   if (i - B.mx >= 4 && B.px - i >= 4 && j - B.my >= 4 && B.py - j >= 4) {
     const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld, a.J};
-    if (B.Cdiss > 0.0) {
+    if (EXT && B.Cdiss > 0.0) {
       const T dA[3] = {(T)B.dA[0], (T)B.dA[1], (T)B.dA[2]};
       if (a.mrho)
         interior_rates_diss<T>(d, acc, a.mgam[cc], PointCoef<T>{a.mrho[cc], a.mG[cc], a.mM[cc]}, dA,
                                a.Ji[cc]);
       else
         interior_rates_diss<T>(d, acc, (T)B.gamma, BlockCoef<T, const BlockDev&>{B}, dA, a.Ji[cc]);
-    } else if (a.mrho) {
+    } else if (EXT && a.mrho) {
       interior_rates<T>(d, acc, a.mgam[cc], PointCoef<T>{a.mrho[cc], a.mG[cc], a.mM[cc]}, a.Ji[cc]);
     } else {
       interior_rates<T>(d, acc, (T)B.gamma, BlockCoef<T, const BlockDev&>{B}, a.Ji[cc]);
     }
   } else {
-    closure_rates<T>(a, B, i, j, cc, d);
+    closure_rates<T, EXT>(a, B, i, j, cc, d);
   }
-  T Fc[6];
-#pragma unroll
-  for (int l = 0; l < 6; l++) Fc[l] = a.Fin[l * ps + cc];
-  finish_point<T, FULL>(a, B, bi, i, j, cc, d, Fc);
-}
-
-// closure point (within four points of its block's edge)
-template <typename T, bool FULL>
-__device__ __forceinline__ void point_closure(const BulkArgs<T>& a, const BlockDev& B, const int bi,
-                                              const int i, const int j) {
-  const long long ps = a.ps;
-  const long long cc = (long long)j * a.ld + i;
-  T d[6];
-  scaled_rates<T>(a, cc, d);
-  closure_rates<T>(a, B, i, j, cc, d);
   T Fc[6];
 #pragma unroll
   for (int l = 0; l < 6; l++) Fc[l] = a.Fin[l * ps + cc];
@@ -680,7 +666,7 @@ __global__ void __launch_bounds__(256) bulk_kernel(BulkArgs<T> a) {
   const int i = tx * blockDim.x + threadIdx.x;
   const int j = ty * blockDim.y + threadIdx.y;
   if (i >= a.nx || j >= a.ny) return;
-  point_generic<T, true>(a, i, j);
+  point_generic<T, true, true>(a, i, j);
 }
 
 // global load at a 32-bit byte offset from a wave-uniform base, so that the
@@ -909,12 +895,12 @@ bulk_interior_batch(const BulkArgs<T>* __restrict__ tab, const int4* __restrict_
 
 // Band kernel: the points within four points of their block's edge (one-
 // sided closures, SAT), one lane each, from a list of (i, j).
-template <typename T, bool FULL>
+template <typename T, bool FULL, bool EXT>
 __global__ void __launch_bounds__(64) bulk_band(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= npts) return;
   const int2 p = pts[k];
-  point_generic<T, FULL>(a, p.x, p.y);
+  point_generic<T, FULL, EXT>(a, p.x, p.y);
 }
 
 // Batched band kernel: pts[k] = (i, j, problem, valid), every wave's 64
@@ -927,7 +913,8 @@ __global__ void __launch_bounds__(64) bulk_band_batch(const BulkArgs<T>* __restr
   const int4 p = pts[k];
   const int m = __builtin_amdgcn_readfirstlane(p.z);
   if (!p.w) return;
-  point_generic<T, FULL>(tab[m], p.x, p.y);
+  // ensemble members have neither a heterogeneous medium nor dissipation
+  point_generic<T, FULL, false>(tab[m], p.x, p.y);
 }
 
 // ---------------------------------------------------------------------

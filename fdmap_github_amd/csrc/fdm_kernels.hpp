@@ -114,7 +114,7 @@ __device__ __forceinline__ double stage_time(const EdgeArgs<T>& a) {
 // stress is the plus block's in-plane prestress resolved on the fault
 // (rotate_xy2nt_mat, geometry.f90:311-322), the stress transfer phip is zero,
 // and the initial fields arrive in their own order (interfaces.f90:798).
-template <typename T>
+template <typename T, bool FORCE>
 __device__ __forceinline__ void edge_iface3(const EdgeArgs<T>& a, const IfaceDev& I, const int q,
                                             const BlockDev& Bm, const BlockDev& Bp, const int som,
                                             const int sop, const long long ccm, const long long ccp,
@@ -183,10 +183,11 @@ __device__ __forceinline__ void edge_iface3(const EdgeArgs<T>& a, const IfaceDev
            Nreg = ST(FDM_IF_NREG);
     const double D = ST(FDM_IF_D), Psi = ST(FDM_IF_PSI);
     const double E = I.law >= FDM_LAW_RSL ? exp(Psi / rs[0]) : 0.0;
-    const double kf = I.force ? kinematic_friction(I.kn, a.fi.knx[q], tst) : 0.0;
+    const int force = FORCE ? I.force : 0;
+    const double kf = force ? kinematic_friction(I.kn, a.fi.knx[q], tst) : 0.0;
     const bool fail = solve_friction_core(I.law, rs, sw, I.reg, a.fi.outside[q], I.flock, fS0, fN0,
                                           phip, phis, etas, D, Psi, E, I.skempton, a.Bdt, V, S, O, N,
-                                          Nreg, I.force, I.kn[0], kf);
+                                          Nreg, force, I.kn[0], kf);
     if (fail) *a.fail = 1;
     ST(FDM_IF_V) = V;
     ST(FDM_IF_S) = S;
@@ -243,7 +244,9 @@ __device__ __forceinline__ void edge_iface3(const EdgeArgs<T>& a, const IfaceDev
   }
 }
 
-template <typename T>
+// MODE 2 or 3: each kernel carries the code of one mode only; FORCE = false
+// leaves out kinematic forcing (the host selects it when no interface forces)
+template <typename T, int MODE, bool FORCE>
 __device__ __forceinline__ void edge_task(const EdgeArgs<T>& a, const int2 task) {
   const double tst = stage_time(a);
   const int NS = a.sd.NS;
@@ -258,7 +261,7 @@ __device__ __forceinline__ void edge_task(const EdgeArgs<T>& a, const int2 task)
     else { i = B.mx + k; j = B.py; }
     int so = B.side_off[s] + k;
     long long cc = (long long)j * a.ld + i;
-    if (a.mode == 3) {
+    if constexpr (MODE == 3) {
       // set_bc_mode3 (boundaries.f90:148-149, 155-213)
       double F3[3], F03[3], Fh3[3];
 #pragma unroll
@@ -320,8 +323,8 @@ __device__ __forceinline__ void edge_task(const EdgeArgs<T>& a, const int2 task)
   }
   int som = Bm.side_off[sm] + km, sop = Bp.side_off[sp] + kp;
   long long ccm = (long long)jm * a.ld + im, ccp = (long long)jp * a.ld + ip;
-  if (a.mode == 3) {
-    edge_iface3<T>(a, I, q, Bm, Bp, som, sop, ccm, ccp, tst);
+  if constexpr (MODE == 3) {
+    edge_iface3<T, FORCE>(a, I, q, Bm, Bp, som, sop, ccm, ccp, tst);
     return;
   }
   double Fm[6], Fp[6], F0m[6], F0p[6], Fhm[6], Fhp[6];
@@ -405,10 +408,11 @@ __device__ __forceinline__ void edge_task(const EdgeArgs<T>& a, const int2 task)
     // exp(Psi/a) of the regularised laws, shared by every strength
     // evaluation of the Newton solve and by the state rate
     const double E = I.law >= FDM_LAW_RSL ? exp(Psi / rs[0]) : 0.0;
-    const double kf = I.force ? kinematic_friction(I.kn, a.fi.knx[q], tst) : 0.0;
+    const int force = FORCE ? I.force : 0;
+    const double kf = force ? kinematic_friction(I.kn, a.fi.knx[q], tst) : 0.0;
     bool fail = solve_friction_core(I.law, rs, sw, I.reg, a.fi.outside[q], I.flock, fS0, fN0,
                                     phip, phis, etas, D, Psi, E, I.skempton, a.Bdt, V, S, O, N, Nreg,
-                                    I.force, I.kn[0], kf);
+                                    force, I.kn[0], kf);
     if (fail) *a.fail = 1;
     if (a.trace) {
       a.trace[q] = V;
@@ -482,17 +486,17 @@ __device__ __forceinline__ void edge_task(const EdgeArgs<T>& a, const int2 task)
   }
 }
 
-template <typename T>
+template <typename T, int MODE, bool FORCE>
 __global__ void __launch_bounds__(64) edge_kernel(EdgeArgs<T> a) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
   if (tid >= a.ntasks) return;
-  edge_task<T>(a, a.tasks[tid]);
+  edge_task<T, MODE, FORCE>(a, a.tasks[tid]);
 }
 
 // Batched edge kernel: tasks[k] = (task.x, task.y, problem, valid) over the
 // edge tasks of many problems, every wave's 64 entries of one problem
 // (padding entries have valid = 0).
-template <typename T>
+template <typename T, bool FORCE>
 __global__ void __launch_bounds__(64) edge_kernel_batch(const EdgeArgs<T>* __restrict__ tab,
                                                         const int4* __restrict__ tasks, int n) {
   const int tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -500,7 +504,7 @@ __global__ void __launch_bounds__(64) edge_kernel_batch(const EdgeArgs<T>* __res
   const int4 t = tasks[tid];
   const int m = __builtin_amdgcn_readfirstlane(t.z);
   if (!t.w) return;
-  edge_task<T>(tab[m], make_int2(t.x, t.y));
+  edge_task<T, 2, FORCE>(tab[m], make_int2(t.x, t.y));
 }
 
 }  // namespace fdm
