@@ -89,6 +89,9 @@ class fdm_block(C.Structure):
         ("s0", C.c_double * 6),
         ("side", fdm_side * 4),
         ("Cdiss", C.c_double),
+        ("pmlx", C.c_int32),
+        ("pmly", C.c_int32),
+        ("pml", C.c_double * 7),
     ]
 
 
@@ -164,6 +167,8 @@ ENTRY_POINTS = [
     ("fdm_get_iface_stage1", C.c_int, [_P, C.c_int32, C.c_int32, _dp]),
     ("fdm_get_plastic", C.c_int, [_P, _dp, _dp]),
     ("fdm_set_plastic", C.c_int, [_P, _dp, _dp]),
+    ("fdm_get_pml", C.c_int, [_P, _dp, _dp]),
+    ("fdm_set_pml", C.c_int, [_P, _dp, _dp]),
     ("fdm_max_slip_velocity", C.c_int, [_P, _dp]),
     ("fdm_sync", C.c_int, [_P]),
     ("fdm_profile_enable", C.c_int, [_P, C.c_int32]),

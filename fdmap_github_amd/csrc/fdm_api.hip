@@ -236,6 +236,11 @@ struct fdm_ctx {
   void* Jp = nullptr;
   // kinematic forcing on some interface: the edge kernels carrying it
   bool any_force = false;
+  // perfectly matched layer: some block has it in effect (pml); Wx / Wy
+  // exist when some block sets pmlx / pmly (pml_wx, pml_wy); W holds Wx,
+  // Wy, DWx, DWy, 6 planes each (field precision, plane stride ps)
+  bool pml = false, pml_wx = false, pml_wy = false;
+  void* W = nullptr;
   // 2: plane strain, 6 fields; 3: antiplane shear, 3 fields
   int mode = 2, nF = 6;
   std::vector<double> hnu;  // nu of the medium ([ny][nx]), for restore_df6
@@ -495,6 +500,7 @@ BulkArgs<T> bulk_args(fdm_ctx* c, double A, double Bdt, int do_update, double t,
   a.mgam = (const T*)c->mat[3];
   a.mnu = (const T*)c->mat[4];
   a.J = (const T*)c->Jp;
+  a.W = (T*)c->W;
   return a;
 }
 
@@ -512,11 +518,11 @@ void launch_band(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   dim3 grid((c->nband + 63) / 64);
   if (c->mode == 3)
     hipLaunchKernelGGL(bulk_band3<T>, grid, dim3(64), 0, s, a, c->band, c->nband);
-  else if (full_variant(c, a) && (c->het || c->diss))
+  else if (full_variant(c, a) && (c->het || c->diss || c->pml))
     hipLaunchKernelGGL((bulk_band<T, true, true>), grid, dim3(64), 0, s, a, c->band, c->nband);
   else if (full_variant(c, a))
     hipLaunchKernelGGL((bulk_band<T, true, false>), grid, dim3(64), 0, s, a, c->band, c->nband);
-  else if (c->het || c->diss)
+  else if (c->het || c->diss || c->pml)
     hipLaunchKernelGGL((bulk_band<T, false, true>), grid, dim3(64), 0, s, a, c->band, c->nband);
   else
     hipLaunchKernelGGL((bulk_band<T, false, false>), grid, dim3(64), 0, s, a, c->band, c->nband);
@@ -760,11 +766,12 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   if (P->mode == 3) {
     bool unsupported = P->source.nplace > 0 || P->mat_rho;
     for (int b = 0; b < P->nblocks; b++)
-      unsupported |= P->blocks[b].plastic || P->blocks[b].mms || P->blocks[b].Cdiss > 0.0;
+      unsupported |= P->blocks[b].plastic || P->blocks[b].mms || P->blocks[b].Cdiss > 0.0 ||
+                     P->blocks[b].pmlx || P->blocks[b].pmly;
     if (unsupported)
       return set_err(nullptr, FDM_EINVAL,
                      "fdm_create: mode 3 covers elastic homogeneous blocks without sources, "
-                     "plasticity, manufactured solutions or dissipation");
+                     "plasticity, manufactured solutions, dissipation or PML");
   }
 
   fdm_ctx* c = new fdm_ctx();
@@ -914,10 +921,14 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   c->hblk = blk;
   {
     // interior tiles: each block's rectangle [mx+4, px-4] x [my+4, py-4]
-    // covered from its corner, x fastest; band: every other point, row-major
+    // covered from its corner, x fastest; band: every other point, row-major.
+    // A block with a perfectly matched layer in effect has no tiles: all of
+    // its points go to the band, whose per-point path carries the layer.
+    auto layer = [&](const fdm_block& B) { return (B.pmlx || B.pmly) && !(B.Cdiss > 0.0); };
     std::vector<int4> tl[3];
     for (int b = 0; b < c->nb; b++) {
       const fdm_block& B = P->blocks[b];
+      if (layer(B)) continue;
       const int ia = B.mx + 4, ib = B.px - 4, ja = B.my + 4, jb = B.py - 4;
       if (ia > ib || ja > jb) continue;
       c->ninterior += (long long)(ib - ia + 1) * (jb - ja + 1);
@@ -931,7 +942,8 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     for (int j = 0; j < P->ny; j++)
       for (int i = 0; i < P->nx; i++) {
         const fdm_block& B = P->blocks[block_of(c, i, j)];
-        if (i - B.mx < 4 || B.px - i < 4 || j - B.my < 4 || B.py - j < 4) band.push_back(make_int2(i, j));
+        if (layer(B) || i - B.mx < 4 || B.px - i < 4 || j - B.my < 4 || B.py - j < 4)
+          band.push_back(make_int2(i, j));
       }
     for (int q = 0; q < 3; q++) {
       c->ntiles[q] = (int)tl[q].size();
@@ -997,6 +1009,29 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     D.dA[1] = FDM_AI1 * D.Cdiss;
     D.dA[2] = FDM_AI2 * D.Cdiss;
     if (D.Cdiss > 0.0) c->diss = true;
+    // perfectly matched layer (material.f90:152-223), in effect unless the
+    // block has dissipation (set_rates_all, time_step.f90:241-246); hxi =
+    // Ji*yr and hyi = Ji*xq at the first point of each region (rates.f90:
+    // 2060-2061, 2158-2159; regions as time_step.f90:357-420 bound them)
+    D.pmlx = (B.pmlx && D.Cdiss == 0.0) ? 1 : 0;
+    D.pmly = (B.pmly && D.Cdiss == 0.0) ? 1 : 0;
+    for (int k = 0; k < 7; k++) D.pml[k] = B.pml[k];
+    {
+      const int i0[3] = {B.mx, B.mx + 4, B.px - 3}, j0[3] = {B.my, B.my + 4, B.py - 3};
+      for (int ry = 0; ry < 3; ry++)
+        for (int rx = 0; rx < 3; rx++) {
+          const size_t q = (size_t)j0[ry] * P->nx + i0[rx];
+          D.phx[3 * ry + rx] = P->Ji[q] * P->yr[q];
+          D.phy[3 * ry + rx] = P->Ji[q] * P->xq[q];
+        }
+    }
+    if (D.pmlx || D.pmly) c->pml = true;
+    if (B.pmlx) c->pml_wx = true;
+    if (B.pmly) c->pml_wy = true;
+    if ((D.pmlx || D.pmly) && B.pml[0] > 0.0 && (!P->x || !P->y)) {
+      set_err(c, FDM_EINVAL, "fdm_create: a graded PML needs coordinates x, y");
+      return fail(FDM_EINVAL);
+    }
     if (B.plastic) c->any_plastic = true;
     if (B.mms && (!P->x || !P->y)) {
       set_err(c, FDM_EINVAL, "fdm_create: MMS source needs coordinates x, y");
@@ -1295,10 +1330,18 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     }
   }
   for (int b = 0; b < c->nb; b++) c->any_mms |= (P->blocks[b].mms != 0);
-  if (c->any_mms) {
+  bool graded = false;
+  for (int b = 0; b < c->nb; b++) graded |= (c->hblocks[b].pmlx || c->hblocks[b].pmly) && c->hblocks[b].pml[0] > 0.0;
+  if (c->any_mms || graded) {
     size_t np = (size_t)c->nx * c->ny;
     if ((rc = dupload(c, &c->X, P->x, np * 8))) return fail(rc);
     if ((rc = dupload(c, &c->Y, P->y, np * 8))) return fail(rc);
+  }
+  if (c->pml_wx || c->pml_wy) {
+    // Wx, Wy start at zero (fields.f90:203-209), and so do their rates
+    const size_t bytes = 24 * (size_t)c->ps * c->esz;
+    if ((rc = dalloc(c, &c->W, bytes))) return fail(rc);
+    HIPCHK(c, hipMemset(c->W, 0, bytes));
   }
   if (c->any_plastic) {
     size_t np = (size_t)c->nx * c->ny;
@@ -1464,6 +1507,61 @@ int fdm_set_plastic(fdm_ctx* c, const double* gammap, const double* lambda) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   if (gammap) HIPCHK(c, hipMemcpy(c->gammap, gammap, np * 8, hipMemcpyHostToDevice));
   if (lambda) HIPCHK(c, hipMemcpy(c->lambda, lambda, np * 8, hipMemcpyHostToDevice));
+  return FDM_OK;
+}
+
+namespace {
+
+// copy Wx (first = 0) or Wy (first = 6) between [6][ny][nx] doubles and the
+// device planes
+int pml_copy(fdm_ctx* c, int first, double* out, const double* in) {
+  const size_t pe = (size_t)c->ps;
+  const size_t n = 6 * pe;
+  char* base = (char*)c->W + (size_t)first * pe * c->esz;
+  std::vector<double> hd;
+  std::vector<float> hf;
+  if (c->precision == FDM_FP64) hd.assign(n, 0.0);
+  else hf.assign(n, 0.f);
+  if (out) {
+    if (c->precision == FDM_FP64) HIPCHK(c, hipMemcpy(hd.data(), base, n * 8, hipMemcpyDeviceToHost));
+    else HIPCHK(c, hipMemcpy(hf.data(), base, n * 4, hipMemcpyDeviceToHost));
+  }
+  for (int l = 0; l < 6; l++)
+    for (int j = 0; j < c->ny; j++)
+      for (int i = 0; i < c->nx; i++) {
+        const size_t k = l * pe + (size_t)j * c->ld + i, o = ((size_t)l * c->ny + j) * c->nx + i;
+        if (out) out[o] = c->precision == FDM_FP64 ? hd[k] : (double)hf[k];
+        else if (c->precision == FDM_FP64) hd[k] = in[o];
+        else hf[k] = (float)in[o];
+      }
+  if (!out) {
+    if (c->precision == FDM_FP64) HIPCHK(c, hipMemcpy(base, hd.data(), n * 8, hipMemcpyHostToDevice));
+    else HIPCHK(c, hipMemcpy(base, hf.data(), n * 4, hipMemcpyHostToDevice));
+  }
+  return FDM_OK;
+}
+
+}  // namespace
+
+int fdm_get_pml(fdm_ctx* c, double* Wx, double* Wy) {
+  if (!c) return FDM_EINVAL;
+  if ((Wx && !c->pml_wx) || (Wy && !c->pml_wy))
+    return set_err(c, FDM_EINVAL, "fdm_get_pml: no block has that PML direction");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc;
+  if (Wx && (rc = pml_copy(c, 0, Wx, nullptr))) return rc;
+  if (Wy && (rc = pml_copy(c, 6, Wy, nullptr))) return rc;
+  return FDM_OK;
+}
+
+int fdm_set_pml(fdm_ctx* c, const double* Wx, const double* Wy) {
+  if (!c) return FDM_EINVAL;
+  if ((Wx && !c->pml_wx) || (Wy && !c->pml_wy))
+    return set_err(c, FDM_EINVAL, "fdm_set_pml: no block has that PML direction");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int rc;
+  if (Wx && (rc = pml_copy(c, 0, nullptr, Wx))) return rc;
+  if (Wy && (rc = pml_copy(c, 6, nullptr, Wy))) return rc;
   return FDM_OK;
 }
 
@@ -1767,11 +1865,11 @@ int fdm_ensemble_create(fdm_ctx* const* members, int32_t n, fdm_ensemble** out) 
     if (!c) return FDM_EINVAL;
     const fdm_ctx* c0 = members[0];
     if (c->src.nplace > 0 || c->any_mms || (!c->ld_T.empty() && c->ld_T[0] > 0.0) || c->bulk_mode != 2 ||
-        c->het || c->diss || c->mode != 2 || c->nstage != c0->nstage || c->precision != c0->precision ||
+        c->het || c->diss || c->pml || c->mode != 2 || c->nstage != c0->nstage || c->precision != c0->precision ||
         c->device != c0->device) {
       g_create_error =
           "fdm_ensemble_create: members must have no point source, no manufactured solution, "
-          "no load ramp, no heterogeneous medium, no artificial dissipation, plane strain, the tile kernels, "
+          "no load ramp, no heterogeneous medium, no artificial dissipation, no PML, plane strain, the tile kernels, "
           "and the same RK order, precision and device";
       return FDM_EINVAL;
     }

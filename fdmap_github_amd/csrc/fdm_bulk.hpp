@@ -48,6 +48,9 @@ struct BulkArgs {
   const T* mnu;
   // Jacobian J, when a block has artificial dissipation (else null)
   const T* J;
+  // PML auxiliary fields: Wx (planes 0-5), Wy (6-11), DWx (12-17), DWy
+  // (18-23), plane stride ps (null without a layer)
+  T* W;
 };
 
 // Map the hardware workgroup id to a tile.  Workgroups are dispatched to the
@@ -609,6 +612,204 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
   }
 }
 
+// Derivative terms of a PML block's boundary forms (rates.f90:2194-2218
+// and the like): v the fields at a one-sided neighbour or a central
+// difference, D its coefficient; the auxiliary rate dw takes sigma times
+// the same derivative when the layer acts in that direction.
+template <typename T>
+__device__ __forceinline__ void pml_xterm(T (&d)[6], T (&dw)[6], const T (&v)[6], T D, T hxi, T sx,
+                                          bool on, T rho, T G, T M, T gamma) {
+  if (on) {
+#pragma unroll
+    for (int l = 0; l < 6; l++) dw[l] = dw[l] + sx * hxi * v[l] * D;
+  }
+  d[0] = d[0] + hxi * v[2] * D / rho;
+  d[1] = d[1] + hxi * v[3] * D / rho;
+  d[2] = d[2] + hxi * v[0] * D * M;
+  d[3] = d[3] + hxi * v[1] * D * G;
+  d[4] = d[4] + hxi * v[0] * D * M * gamma;
+}
+
+template <typename T>
+__device__ __forceinline__ void pml_yterm(T (&d)[6], T (&dw)[6], const T (&v)[6], T D, T hyi, T sy,
+                                          bool on, T rho, T G, T M, T gamma) {
+  if (on) {
+#pragma unroll
+    for (int l = 0; l < 6; l++) dw[l] = dw[l] + sy * hyi * v[l] * D;
+  }
+  d[0] = d[0] + hyi * v[3] * D / rho;
+  d[1] = d[1] + hyi * v[4] * D / rho;
+  d[2] = d[2] + hyi * v[1] * D * M * gamma;
+  d[3] = d[3] + hyi * v[0] * D * G;
+  d[4] = d[4] + hyi * v[1] * D * M;
+}
+
+// A point of a perfectly matched layer block: the SAT penalties, then
+// rates_interior_mode2_pml / rates_boundary_mode2_pml (rates.f90:2024-2619):
+// the auxiliary fields' coupling and damping (sigma + kappa), and Cartesian
+// derivatives with hxi, hyi of the point's region.  Interior points sum the
+// x and y differences per s; strips take the one-sided direction's loop and
+// the central one's in the reference's order; corners interleave the two
+// one-sided loops.  The block's scalar material is used.  The auxiliary
+// rates are scaled by A and the fields updated in place (fields.f90:
+// 1074-1077, 1181-1184): each lane touches only its own point's W.
+template <typename T>
+__device__ __forceinline__ void pml_rates(const BulkArgs<T>& a, const BlockDev& B, const int i, const int j,
+                                          const long long cc, T (&d)[6]) {
+  const long long ps = a.ps;
+  const int ld = a.ld;
+  const T* __restrict__ F = a.Fin;
+  const int dl = i - B.mx, dr = B.px - i, db = j - B.my, dtp = B.py - j;
+  if (dl == 0 || dr == 0 || db == 0 || dtp == 0) {
+    T Fe[6];
+#pragma unroll
+    for (int l = 0; l < 6; l++) Fe[l] = F[l * ps + cc];
+    if (dl == 0) sat_term<T>(d, Fe, a.sd, B.side_off[FDM_SIDE_L] + db);
+    if (dr == 0) sat_term<T>(d, Fe, a.sd, B.side_off[FDM_SIDE_R] + db);
+    if (db == 0) sat_term<T>(d, Fe, a.sd, B.side_off[FDM_SIDE_B] + dl);
+    if (dtp == 0) sat_term<T>(d, Fe, a.sd, B.side_off[FDM_SIDE_T] + dl);
+  }
+  const int rx = dl < 4 ? 0 : (dr < 4 ? 2 : 1);
+  const int ry = db < 4 ? 0 : (dtp < 4 ? 2 : 1);
+  const T hxi = (T)B.phx[3 * ry + rx], hyi = (T)B.phy[3 * ry + rx];
+  // damping profile (rates.f90:2048-2049, 2063-2066)
+  double sxd = B.pml[1], syd = B.pml[1];
+  if (B.pml[0] > 0.0) {
+    const long long pc = (long long)j * a.nx + i;
+    const double x = a.X[pc], y = a.Y[pc];
+    sxd = B.pml[1] * pow(fabs(fmax(0.0, fmax(B.pml[3] - x, x - B.pml[4]))), B.pml[0]);
+    syd = B.pml[1] * pow(fabs(fmax(0.0, fmax(B.pml[5] - y, y - B.pml[6]))), B.pml[0]);
+  }
+  const T sx = (T)sxd, sy = (T)syd, kap = (T)B.pml[2];
+  const T rho = (T)B.rho, G = (T)B.G, M = (T)B.M, gamma = (T)B.gamma;
+  const bool onx = B.pmlx != 0, ony = B.pmly != 0;
+  T* __restrict__ W = a.W;
+  const T A = (T)a.A;
+  T dwx[6], dwy[6];
+#pragma unroll
+  for (int l = 0; l < 6; l++) {
+    dwx[l] = (!onx || a.a_zero) ? (T)0.0 : A * W[(12 + l) * ps + cc];
+    dwy[l] = (!ony || a.a_zero) ? (T)0.0 : A * W[(18 + l) * ps + cc];
+  }
+  if (onx) {
+    T w[6];
+#pragma unroll
+    for (int l = 0; l < 6; l++) w[l] = W[l * ps + cc];
+    d[0] = d[0] - w[2] / rho;
+    d[1] = d[1] - w[3] / rho;
+    d[2] = d[2] - w[0] * M;
+    d[3] = d[3] - w[1] * G;
+    d[4] = d[4] - w[0] * gamma * M;
+#pragma unroll
+    for (int l = 0; l < 6; l++) dwx[l] = dwx[l] - (sx + kap) * w[l];
+  }
+  if (ony) {
+    T w[6];
+#pragma unroll
+    for (int l = 0; l < 6; l++) w[l] = W[(6 + l) * ps + cc];
+    d[0] = d[0] - w[3] / rho;
+    d[1] = d[1] - w[4] / rho;
+    d[2] = d[2] - w[1] * gamma * M;
+    d[3] = d[3] - w[0] * G;
+    d[4] = d[4] - w[1] * M;
+#pragma unroll
+    for (int l = 0; l < 6; l++) dwy[l] = dwy[l] - (sy + kap) * w[l];
+  }
+  const long long row = (long long)j * ld;
+  const long long colm = (long long)B.my * ld + i, colp = (long long)B.py * ld + i;
+  T v[6];
+  if (rx == 1 && ry == 1) {
+    // interior (rates.f90:2088-2124); the s = 3 terms carry DI(3) = 0
+#pragma unroll
+    for (int s = 1; s <= 2; s++) {
+      const T Dc = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
+      const T ca = Dc / rho, cb = Dc * G, cm = Dc * M;
+      T dx[6], dy[6];
+#pragma unroll
+      for (int l = 0; l < 6; l++) {
+        dx[l] = F[l * ps + cc + s] - F[l * ps + cc - s];
+        dy[l] = F[l * ps + cc + (long long)s * ld] - F[l * ps + cc - (long long)s * ld];
+      }
+      if (onx) {
+#pragma unroll
+        for (int l = 0; l < 6; l++) dwx[l] = dwx[l] + sx * hxi * dx[l] * Dc;
+      }
+      if (ony) {
+#pragma unroll
+        for (int l = 0; l < 6; l++) dwy[l] = dwy[l] + sy * hyi * dy[l] * Dc;
+      }
+      d[0] = d[0] + (hxi * dx[2] + hyi * dy[3]) * ca;
+      d[1] = d[1] + (hxi * dx[3] + hyi * dy[4]) * ca;
+      d[2] = d[2] + (hxi * dx[0] + hyi * dy[1] * gamma) * cm;
+      d[3] = d[3] + (hxi * dx[1] + hyi * dy[0]) * cb;
+      d[4] = d[4] + (hxi * dx[0] * gamma + hyi * dy[1]) * cm;
+    }
+  } else if (ry == 1) {
+    // L or R strip: one-sided x, then central y
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      const long long k = (rx == 0) ? row + B.mx + s : row + B.px - s;
+      const T D = (rx == 0) ? (T)kDL[dl][s] : (T)(-kDL[dr][s]);
+#pragma unroll
+      for (int l = 0; l < 6; l++) v[l] = F[l * ps + k];
+      pml_xterm<T>(d, dwx, v, D, hxi, sx, onx, rho, G, M, gamma);
+    }
+#pragma unroll
+    for (int s = 1; s <= 2; s++) {
+#pragma unroll
+      for (int l = 0; l < 6; l++)
+        v[l] = F[l * ps + cc + (long long)s * ld] - F[l * ps + cc - (long long)s * ld];
+      pml_yterm<T>(d, dwy, v, (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2, hyi, sy, ony, rho, G, M, gamma);
+    }
+  } else if (rx == 1) {
+    // B or T strip: central x, then one-sided y
+#pragma unroll
+    for (int s = 1; s <= 2; s++) {
+#pragma unroll
+      for (int l = 0; l < 6; l++) v[l] = F[l * ps + cc + s] - F[l * ps + cc - s];
+      pml_xterm<T>(d, dwx, v, (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2, hxi, sx, onx, rho, G, M, gamma);
+    }
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      const long long k = (ry == 0) ? colm + (long long)s * ld : colp - (long long)s * ld;
+      const T D = (ry == 0) ? (T)kDL[db][s] : (T)(-kDL[dtp][s]);
+#pragma unroll
+      for (int l = 0; l < 6; l++) v[l] = F[l * ps + k];
+      pml_yterm<T>(d, dwy, v, D, hyi, sy, ony, rho, G, M, gamma);
+    }
+  } else {
+    // corner: the two one-sided loops interleaved per s
+#pragma unroll
+    for (int s = 0; s < 6; s++) {
+      const long long kx = (rx == 0) ? row + B.mx + s : row + B.px - s;
+      const T Dx = (rx == 0) ? (T)kDL[dl][s] : (T)(-kDL[dr][s]);
+#pragma unroll
+      for (int l = 0; l < 6; l++) v[l] = F[l * ps + kx];
+      pml_xterm<T>(d, dwx, v, Dx, hxi, sx, onx, rho, G, M, gamma);
+      const long long ky = (ry == 0) ? colm + (long long)s * ld : colp - (long long)s * ld;
+      const T Dy = (ry == 0) ? (T)kDL[db][s] : (T)(-kDL[dtp][s]);
+#pragma unroll
+      for (int l = 0; l < 6; l++) v[l] = F[l * ps + ky];
+      pml_yterm<T>(d, dwy, v, Dy, hyi, sy, ony, rho, G, M, gamma);
+    }
+  }
+  const T Bdt = (T)a.Bdt;
+  if (onx) {
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      if (a.do_update) W[l * ps + cc] = W[l * ps + cc] + Bdt * dwx[l];
+      W[(12 + l) * ps + cc] = dwx[l];
+    }
+  }
+  if (ony) {
+#pragma unroll
+    for (int l = 0; l < 6; l++) {
+      if (a.do_update) W[(6 + l) * ps + cc] = W[(6 + l) * ps + cc] + Bdt * dwy[l];
+      W[(18 + l) * ps + cc] = dwy[l];
+    }
+  }
+}
+
 // d = A * DF at the start of a stage (scale_rates, time_step.f90:138-141)
 template <typename T>
 __device__ __forceinline__ void scaled_rates(const BulkArgs<T>& a, const long long cc, T (&d)[6]) {
@@ -625,7 +826,8 @@ __device__ __forceinline__ void scaled_rates(const BulkArgs<T>& a, const long lo
 
 // Everything that happens at one grid point in one stage, reading its
 // neighbours straight from global memory: the per-point kernel, and the
-// closure points of the tile and marching kernels.  EXT as in closure_rates.
+// closure points of the tile and marching kernels.  EXT as in closure_rates,
+// and also carries the perfectly matched layer.
 template <typename T, bool FULL, bool EXT>
 __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i, const int j) {
   const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
@@ -635,7 +837,9 @@ __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i,
   const long long cc = (long long)j * ld + i;
   T d[6];
   scaled_rates<T>(a, cc, d);
-  if (i - B.mx >= 4 && B.px - i >= 4 && j - B.my >= 4 && B.py - j >= 4) {
+  if (EXT && (B.pmlx | B.pmly)) {
+    pml_rates<T>(a, B, i, j, cc, d);
+  } else if (i - B.mx >= 4 && B.px - i >= 4 && j - B.my >= 4 && B.py - j >= 4) {
     const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld, a.J};
     if (EXT && B.Cdiss > 0.0) {
       const T dA[3] = {(T)B.dA[0], (T)B.dA[1], (T)B.dA[2]};

@@ -33,6 +33,14 @@ struct BlockDev {
   // AI(s) Cdiss (rates.f90:1416-1421)
   double Cdiss;
   double dA[3];
+  // perfectly matched layer (rates.f90:2024-2619): pmlx/pmly are zero unless
+  // the layer is in effect (mode 2, no dissipation in the block); pml = n,
+  // sigma, kappa, mx, px, my, py; phx/phy: Ji*yr and Ji*xq at the first
+  // point of each of the block's nine regions, index 3*ry + rx with rx, ry
+  // 0 (minus-side closure), 1 (interior), 2 (plus-side closure)
+  int pmlx, pmly;
+  double pml[7];
+  double phx[9], phy[9];
 };
 
 struct IfaceDev {

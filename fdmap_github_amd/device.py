@@ -113,6 +113,18 @@ class Context:
         assert g.shape == l.shape == (self.ny, self.nx)
         self._check(self._fn("set_plastic")(self.h, capi.dptr(g), capi.dptr(l)))
 
+    def get_pml(self, x=True, y=True):
+        """PML auxiliary fields (Wx, Wy), each [6, ny, nx] or None"""
+        Wx = np.empty((6, self.ny, self.nx)) if x else None
+        Wy = np.empty((6, self.ny, self.nx)) if y else None
+        self._check(self._fn("get_pml")(self.h, capi.dptr(Wx), capi.dptr(Wy)))
+        return Wx, Wy
+
+    def set_pml(self, Wx=None, Wy=None):
+        a = None if Wx is None else np.ascontiguousarray(Wx, dtype=np.float64).reshape(6, self.ny, self.nx)
+        b = None if Wy is None else np.ascontiguousarray(Wy, dtype=np.float64).reshape(6, self.ny, self.nx)
+        self._check(self._fn("set_pml")(self.h, capi.dptr(a), capi.dptr(b)))
+
     # ---- device-only helpers ----
     def sync(self):
         self._check(self._fn("sync")(self.h))

@@ -46,6 +46,8 @@ module fdmap_capi
      real(c_double) :: s0(6)
      type(fdm_side) :: side(4)
      real(c_double) :: Cdiss = 0.0_c_double
+     integer(c_int32_t) :: pmlx = 0, pmly = 0
+     real(c_double) :: pml(7) = 0.0_c_double
   end type fdm_block
 
   type,bind(C) :: fdm_iface
@@ -169,6 +171,18 @@ module fdmap_capi
        type(c_ptr),value :: ctx, gammap, lambda
        integer(c_int) :: rc
      end function fdm_set_plastic
+
+     function fdm_get_pml(ctx,Wx,Wy) bind(C,name='fdm_get_pml') result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr),value :: ctx, Wx, Wy
+       integer(c_int) :: rc
+     end function fdm_get_pml
+
+     function fdm_set_pml(ctx,Wx,Wy) bind(C,name='fdm_set_pml') result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr),value :: ctx, Wx, Wy
+       integer(c_int) :: rc
+     end function fdm_set_pml
 
      function fdm_set_iface(ctx,iface,field,in) bind(C,name='fdm_set_iface') result(rc)
        import :: c_ptr, c_int, c_int32_t

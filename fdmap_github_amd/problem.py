@@ -131,6 +131,16 @@ class Problem:
             cb.mms = 1 if blk.get("mms") else 0
             cb.id = int(blk["id"])
             cb.Cdiss = float(blk.get("Cdiss", 0.0))
+            # perfectly matched layer: {"x": bool, "y": bool, "n", "sigma",
+            # "kappa", "mx", "px", "my", "py"} (material.f90:152-223)
+            pml = blk.get("pml")
+            if pml:
+                cb.pmlx = 1 if pml.get("x") else 0
+                cb.pmly = 1 if pml.get("y") else 0
+                big = 1.7976931348623157e308
+                for k, (name, dflt) in enumerate((("n", 0.0), ("sigma", 0.0), ("kappa", 0.0), ("mx", -big),
+                                                  ("px", big), ("my", -big), ("py", big))):
+                    cb.pml[k] = float(pml.get(name, dflt))
             if pl:
                 for k in ("mu", "beta", "b", "h", "eta"):
                     setattr(cb, k, float(pl.get(k, 0.0)))

@@ -118,6 +118,15 @@ typedef struct fdm_block {
   double Cdiss;                  /* artificial dissipation as the rates use it,
                                     Cdiss*CFL/dt (domain.f90:170-180, 316-320);
                                     0: none                                   */
+  /* Perfectly matched layer (pml_type, material.f90:9-15, 152-223): with
+     pmlx or pmly set (mode 2, no dissipation in the block) every point of
+     the block takes the Cartesian rates of rates_interior_mode2_pml and
+     rates_boundary_mode2_pml (rates.f90:2024-2619) and the auxiliary fields
+     Wx, Wy (6 components each, fields.f90:35, 203-209) of fdm_get_pml.
+     pml = n, sigma, kappa, mx, px, my, py: with n > 0 the damping is
+     sigma*max(0, mx-x, x-px)^n in x (likewise in y), else sigma. */
+  int32_t pmlx, pmly;
+  double pml[7];
 } fdm_block;
 
 /* One block interface (iface_type + fr_type, interfaces.f90:9-20,
@@ -275,6 +284,13 @@ int fdm_get_plastic(fdm_ctx* ctx, double* gammap, double* lambda);
    (checkpoint_fields 'read', fields.f90:380-381); either may be NULL.
    FDM_EINVAL when no block is plastic. */
 int fdm_set_plastic(fdm_ctx* ctx, const double* gammap, const double* lambda);
+
+/* PML auxiliary fields Wx and Wy, [6][ny][nx] each (fields.f90:35,
+   203-209; zero outside the layer blocks), for checkpoints
+   (checkpoint_fields, fields.f90:383-384, 406-407).  Either may be NULL;
+   FDM_EINVAL when one is asked for and no block has that direction. */
+int fdm_get_pml(fdm_ctx* ctx, double* Wx, double* Wy);
+int fdm_set_pml(fdm_ctx* ctx, const double* Wx, const double* Wy);
 
 /* max over all frictional interfaces of V (check_nucleation,
    domain.f90:474-499, interfaces.f90:506-517). */

@@ -57,6 +57,8 @@ def load():
     lib.orc_mms_sin.restype = C.c_double
     lib.orc_plastic_point.argtypes = [C.c_double] * 7 + [_dp, C.c_double, _dp, _dp, _dp, _dp]
     lib.orc_plastic_point.restype = None
+    lib.orc_get_pml_rates.argtypes = [C.c_void_p, _dp, _dp]
+    lib.orc_get_pml_rates.restype = C.c_int
     _lib = lib
     return lib
 
@@ -66,6 +68,13 @@ class OracleContext(Context):
         lib = load()
         lib.orc_set_threads(int(threads))
         super().__init__(problem, lib=lib, prefix="orc_")
+
+    def get_pml_rates(self, x=True, y=True):
+        """PML rates (DWx, DWy), each [6, ny, nx] or None"""
+        DWx = np.empty((6, self.ny, self.nx)) if x else None
+        DWy = np.empty((6, self.ny, self.nx)) if y else None
+        self._check(self.lib.orc_get_pml_rates(self.h, capi.dptr(DWx), capi.dptr(DWy)))
+        return DWx, DWy
 
 
 def sbp4_tables():

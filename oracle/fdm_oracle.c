@@ -389,6 +389,9 @@ struct orc_ctx {
   double *mrho, *mG, *mM, *mgam, *mnu;
   /* Jacobian J, when a block has artificial dissipation, or NULL */
   double* J;
+  /* PML auxiliary fields and rates, [6][np] each (fields.f90:35, 203-209):
+     allocated when some block sets pmlx (Wx) or pmly (Wy), else NULL */
+  double *Wx, *DWx, *Wy, *DWy;
   oblock_t* B;
   oiface_t* I;
   fdm_source src;
@@ -475,16 +478,31 @@ int orc_create(const fdm_problem* P, orc_ctx** out) {
   c->DF = alloc_fill(6 * c->np, 1e40);
   c->gammap = alloc_fill(c->np, 0.0);
   c->lambda = alloc_fill(c->np, 0.0);
+  for (int ib = 0; ib < P->nblocks; ib++) {
+    if (P->blocks[ib].pmlx && !c->Wx) {
+      c->Wx = alloc_fill(6 * c->np, 0.0);
+      c->DWx = alloc_fill(6 * c->np, 0.0);
+    }
+    if (P->blocks[ib].pmly && !c->Wy) {
+      c->Wy = alloc_fill(6 * c->np, 0.0);
+      c->DWy = alloc_fill(6 * c->np, 0.0);
+    }
+  }
   c->src = P->source;
   c->B = (oblock_t*)calloc(c->nb, sizeof(oblock_t));
   for (int ib = 0; ib < c->nb; ib++) {
     const fdm_block* b = &P->blocks[ib];
     c->B[ib].b = *b;
     if (c->mode == 3 && (b->plastic || b->mms || b->Cdiss > 0.0 || P->mat_rho ||
-                         P->source.nplace > 0)) {
+                         P->source.nplace > 0 || b->pmlx || b->pmly)) {
       snprintf(g_err, sizeof g_err,
-               "orc_create: mode 3 covers elastic homogeneous blocks without sources or "
-               "dissipation");
+               "orc_create: mode 3 covers elastic homogeneous blocks without sources, "
+               "dissipation or PML");
+      orc_destroy(c);
+      return FDM_EINVAL;
+    }
+    if ((b->pmlx || b->pmly) && b->pml[0] > 0.0 && (!c->x || !c->y)) {
+      snprintf(g_err, sizeof g_err, "orc_create: block %d has a graded PML and no coordinates", ib);
       orc_destroy(c);
       return FDM_EINVAL;
     }
@@ -620,6 +638,10 @@ void orc_destroy(orc_ctx* c) {
   free(c->DF);
   free(c->gammap);
   free(c->lambda);
+  free(c->Wx);
+  free(c->DWx);
+  free(c->Wy);
+  free(c->DWy);
   free(c->mrho);
   free(c->mG);
   free(c->mM);
@@ -704,6 +726,36 @@ int orc_get_plastic(orc_ctx* c, double* gammap, double* lambda) {
   return FDM_OK;
 }
 
+int orc_get_pml(orc_ctx* c, double* Wx, double* Wy) {
+  if ((Wx && !c->Wx) || (Wy && !c->Wy)) {
+    snprintf(c->err, sizeof c->err, "orc_get_pml: no block has that PML direction");
+    return FDM_EINVAL;
+  }
+  if (Wx) memcpy(Wx, c->Wx, 6 * c->np * sizeof(double));
+  if (Wy) memcpy(Wy, c->Wy, 6 * c->np * sizeof(double));
+  return FDM_OK;
+}
+
+int orc_get_pml_rates(orc_ctx* c, double* DWx, double* DWy) {
+  if ((DWx && !c->DWx) || (DWy && !c->DWy)) {
+    snprintf(c->err, sizeof c->err, "orc_get_pml_rates: no block has that PML direction");
+    return FDM_EINVAL;
+  }
+  if (DWx) memcpy(DWx, c->DWx, 6 * c->np * sizeof(double));
+  if (DWy) memcpy(DWy, c->DWy, 6 * c->np * sizeof(double));
+  return FDM_OK;
+}
+
+int orc_set_pml(orc_ctx* c, const double* Wx, const double* Wy) {
+  if ((Wx && !c->Wx) || (Wy && !c->Wy)) {
+    snprintf(c->err, sizeof c->err, "orc_set_pml: no block has that PML direction");
+    return FDM_EINVAL;
+  }
+  if (Wx) memcpy(c->Wx, Wx, 6 * c->np * sizeof(double));
+  if (Wy) memcpy(c->Wy, Wy, 6 * c->np * sizeof(double));
+  return FDM_OK;
+}
+
 /* ---------------------------------------------------------------------- */
 /* scale / update, fields.f90:1061-1133, 1167-1242; friction.f90:555-590  */
 /* ---------------------------------------------------------------------- */
@@ -713,6 +765,11 @@ static void scale_rates_all(orc_ctx* c, double A) {
   double* DF = c->DF;
 #pragma omp parallel for schedule(static) num_threads(g_threads)
   for (size_t k = 0; k < n6; k++) DF[k] = A * DF[k];
+  /* scale_rates_interior (fields.f90:1074-1077) */
+  if (c->DWx)
+    for (size_t k = 0; k < 6 * c->np; k++) c->DWx[k] = A * c->DWx[k];
+  if (c->DWy)
+    for (size_t k = 0; k < 6 * c->np; k++) c->DWy[k] = A * c->DWy[k];
   for (int ii = 0; ii < c->ni; ii++) {
     oiface_t* I = &c->I[ii];
     if (I->I.coupling != FDM_COUPLING_FRICTION) continue;
@@ -732,6 +789,11 @@ static void update_fields_all(orc_ctx* c, double Bdt) {
   const double* DF = c->DF;
 #pragma omp parallel for schedule(static) num_threads(g_threads)
   for (size_t k = 0; k < n6; k++) F[k] = F[k] + Bdt * DF[k];
+  /* update_fields_interior (fields.f90:1181-1184) */
+  if (c->Wx)
+    for (size_t k = 0; k < 6 * c->np; k++) c->Wx[k] = c->Wx[k] + Bdt * c->DWx[k];
+  if (c->Wy)
+    for (size_t k = 0; k < 6 * c->np; k++) c->Wy[k] = c->Wy[k] + Bdt * c->DWy[k];
   for (int ii = 0; ii < c->ni; ii++) {
     oiface_t* I = &c->I[ii];
     if (I->I.coupling != FDM_COUPLING_FRICTION) continue;
@@ -1886,6 +1948,174 @@ static void rates_boundary3(const sten_t* S, int mx, int px, int my, int py, int
 }
 
 /* set_rates, mode-2 homogeneous branch (time_step.f90:457-490) */
+/* ---------------------------------------------------------------------- */
+/* perfectly matched layer: rates_interior_mode2_pml and                    */
+/* rates_boundary_mode2_pml (rates.f90:2024-2619)                           */
+/* ---------------------------------------------------------------------- */
+
+/* damping at (x, y), rates.f90:2048-2049, 2063-2066: sigma*|max(0, mx-x,
+   x-px)|^n (likewise in y) for n > 0, else sigma */
+static void pml_sigma(const fdm_block* b, double x, double y, double* sx, double* sy) {
+  const double n = b->pml[0], sigma = b->pml[1];
+  if (n > 0.0) {
+    *sx = sigma * pow(fabs(fmax(0.0, fmax(b->pml[3] - x, x - b->pml[4]))), n);
+    *sy = sigma * pow(fabs(fmax(0.0, fmax(b->pml[5] - y, y - b->pml[6]))), n);
+  } else {
+    *sx = sigma;
+    *sy = sigma;
+  }
+}
+
+typedef struct {
+  double hxi, hyi, sx, sy, rho, G, M, gamma;
+  int pmlx, pmly;
+} pml_pt;
+
+/* x-derivative term of the boundary forms, v the fields at the one-sided
+   neighbour or the central difference, D its coefficient (rates.f90:
+   2194-2206 and the like) */
+static inline void pml_xterm(const pml_pt* p, const double* v, double D, double* d, double* dwx) {
+  if (p->pmlx)
+    for (int l = 0; l < 6; l++) dwx[l] = dwx[l] + p->sx * p->hxi * v[l] * D;
+  d[0] = d[0] + p->hxi * v[2] * D / p->rho;
+  d[1] = d[1] + p->hxi * v[3] * D / p->rho;
+  d[2] = d[2] + p->hxi * v[0] * D * p->M;
+  d[3] = d[3] + p->hxi * v[1] * D * p->G;
+  d[4] = d[4] + p->hxi * v[0] * D * p->M * p->gamma;
+}
+
+/* y-derivative term of the boundary forms (rates.f90:2208-2218) */
+static inline void pml_yterm(const pml_pt* p, const double* v, double D, double* d, double* dwy) {
+  if (p->pmly)
+    for (int l = 0; l < 6; l++) dwy[l] = dwy[l] + p->sy * p->hyi * v[l] * D;
+  d[0] = d[0] + p->hyi * v[3] * D / p->rho;
+  d[1] = d[1] + p->hyi * v[4] * D / p->rho;
+  d[2] = d[2] + p->hyi * v[1] * D * p->M * p->gamma;
+  d[3] = d[3] + p->hyi * v[0] * D * p->G;
+  d[4] = d[4] + p->hyi * v[1] * D * p->M;
+}
+
+/* One region of a PML block: the interior (qside = rside = 0), a strip or
+   a corner (qside/rside 1: minus side one-sided, 2: plus side).  hxi, hyi
+   are Ji*yr and Ji*xq at the region's first point (rates.f90:2060-2061,
+   2158-2159): the layer's stencils are Cartesian. */
+static void rates_pml(orc_ctx* c, const fdm_block* b, int mx, int px, int my, int py, int qside,
+                      int rside) {
+  if (mx > px || my > py) return;
+  const size_t np = c->np;
+  const int mgx = b->mx, pgx = b->px, mgy = b->my, pgy = b->py;
+  const size_t q0 = IDX(mx, my);
+  pml_pt p;
+  p.hxi = c->Ji[q0] * c->yr[q0];
+  p.hyi = c->Ji[q0] * c->xq[q0];
+  p.rho = b->rho;
+  p.G = b->G;
+  p.M = b->M;
+  p.gamma = b->gamma;
+  p.pmlx = b->pmlx;
+  p.pmly = b->pmly;
+  const double kappa = b->pml[2];
+  /* interior coefficients a, b, c (rates.f90:2053-2058) */
+  double ca[3], cb[3], cc[3];
+  for (int s = 1; s <= 2; s++) {
+    ca[s] = DIc(s) / p.rho;
+    cb[s] = DIc(s) * p.G;
+    cc[s] = DIc(s) * p.M;
+  }
+#define PF(l, k) c->F[(size_t)(l) * np + (k)]
+  for (int j = my; j <= py; j++)
+    for (int i = mx; i <= px; i++) {
+      const size_t q = IDX(i, j);
+      pml_sigma(b, c->x[q], c->y[q], &p.sx, &p.sy);
+      double d[5], dwx[6], dwy[6], v[6];
+      for (int l = 0; l < 5; l++) d[l] = c->DF[(size_t)l * np + q];
+      for (int l = 0; l < 6; l++) {
+        dwx[l] = p.pmlx ? c->DWx[(size_t)l * np + q] : 0.0;
+        dwy[l] = p.pmly ? c->DWy[(size_t)l * np + q] : 0.0;
+      }
+      if (p.pmlx) {
+        const double* W = c->Wx;
+        d[0] = d[0] - W[2 * np + q] / p.rho;
+        d[1] = d[1] - W[3 * np + q] / p.rho;
+        d[2] = d[2] - W[0 * np + q] * p.M;
+        d[3] = d[3] - W[1 * np + q] * p.G;
+        d[4] = d[4] - W[0 * np + q] * p.gamma * p.M;
+        for (int l = 0; l < 6; l++) dwx[l] = dwx[l] - (p.sx + kappa) * W[(size_t)l * np + q];
+      }
+      if (p.pmly) {
+        const double* W = c->Wy;
+        d[0] = d[0] - W[3 * np + q] / p.rho;
+        d[1] = d[1] - W[4 * np + q] / p.rho;
+        d[2] = d[2] - W[1 * np + q] * p.gamma * p.M;
+        d[3] = d[3] - W[0 * np + q] * p.G;
+        d[4] = d[4] - W[1 * np + q] * p.M;
+        for (int l = 0; l < 6; l++) dwy[l] = dwy[l] - (p.sy + kappa) * W[(size_t)l * np + q];
+      }
+      if (qside == 0 && rside == 0) {
+        /* interior: x and y differences in one sum (rates.f90:2088-2124);
+           the s = 3 terms carry DI(3) = 0 for SBP4 */
+        for (int s = 1; s <= 2; s++) {
+          double dx[6], dy[6];
+          for (int l = 0; l < 6; l++) {
+            dx[l] = PF(l, IDX(i + s, j)) - PF(l, IDX(i - s, j));
+            dy[l] = PF(l, IDX(i, j + s)) - PF(l, IDX(i, j - s));
+          }
+          if (p.pmlx)
+            for (int l = 0; l < 6; l++) dwx[l] = dwx[l] + p.sx * p.hxi * dx[l] * DIc(s);
+          if (p.pmly)
+            for (int l = 0; l < 6; l++) dwy[l] = dwy[l] + p.sy * p.hyi * dy[l] * DIc(s);
+          d[0] = d[0] + (p.hxi * dx[2] + p.hyi * dy[3]) * ca[s];
+          d[1] = d[1] + (p.hxi * dx[3] + p.hyi * dy[4]) * ca[s];
+          d[2] = d[2] + (p.hxi * dx[0] + p.hyi * dy[1] * p.gamma) * cc[s];
+          d[3] = d[3] + (p.hxi * dx[1] + p.hyi * dy[0]) * cb[s];
+          d[4] = d[4] + (p.hxi * dx[0] * p.gamma + p.hyi * dy[1]) * cc[s];
+        }
+      } else if (rside == 0) {
+        /* L or R strip: one-sided x, then central y (rates.f90:2165-2218) */
+        for (int s = 0; s < 6; s++) {
+          const size_t k = qside == 1 ? IDX(mgx + s, j) : IDX(pgx - s, j);
+          const double D = qside == 1 ? DLc(s, i - mgx) : DRc(s, pgx - i);
+          for (int l = 0; l < 6; l++) v[l] = PF(l, k);
+          pml_xterm(&p, v, D, d, dwx);
+        }
+        for (int s = 1; s <= 2; s++) {
+          for (int l = 0; l < 6; l++) v[l] = PF(l, IDX(i, j + s)) - PF(l, IDX(i, j - s));
+          pml_yterm(&p, v, DIc(s), d, dwy);
+        }
+      } else if (qside == 0) {
+        /* B or T strip: central x, then one-sided y (rates.f90:2287-2335) */
+        for (int s = 1; s <= 2; s++) {
+          for (int l = 0; l < 6; l++) v[l] = PF(l, IDX(i + s, j)) - PF(l, IDX(i - s, j));
+          pml_xterm(&p, v, DIc(s), d, dwx);
+        }
+        for (int s = 0; s < 6; s++) {
+          const size_t k = rside == 1 ? IDX(i, mgy + s) : IDX(i, pgy - s);
+          const double D = rside == 1 ? DLc(s, j - mgy) : DRc(s, pgy - j);
+          for (int l = 0; l < 6; l++) v[l] = PF(l, k);
+          pml_yterm(&p, v, D, d, dwy);
+        }
+      } else {
+        /* corner: one-sided x and y, interleaved per s (rates.f90:2399-2441) */
+        for (int s = 0; s < 6; s++) {
+          const size_t kx = qside == 1 ? IDX(mgx + s, j) : IDX(pgx - s, j);
+          const double Dx = qside == 1 ? DLc(s, i - mgx) : DRc(s, pgx - i);
+          for (int l = 0; l < 6; l++) v[l] = PF(l, kx);
+          pml_xterm(&p, v, Dx, d, dwx);
+          const size_t ky = rside == 1 ? IDX(i, mgy + s) : IDX(i, pgy - s);
+          const double Dy = rside == 1 ? DLc(s, j - mgy) : DRc(s, pgy - j);
+          for (int l = 0; l < 6; l++) v[l] = PF(l, ky);
+          pml_yterm(&p, v, Dy, d, dwy);
+        }
+      }
+      for (int l = 0; l < 5; l++) c->DF[(size_t)l * np + q] = d[l];
+      for (int l = 0; l < 6; l++) {
+        if (p.pmlx) c->DWx[(size_t)l * np + q] = dwx[l];
+        if (p.pmly) c->DWy[(size_t)l * np + q] = dwy[l];
+      }
+    }
+#undef PF
+}
+
 static void set_rates(orc_ctx* c, int ib) {
   fdm_block* b = &c->B[ib].b;
   sten_t S;
@@ -1924,6 +2154,20 @@ static void set_rates(orc_ctx* c, int ib) {
     rates_boundary3(&S, pix + 1, pgx, piy + 1, pgy, 2, 2, mgx, pgx, mgy, pgy);
     return;
   }
+  if ((b->pmlx || b->pmly) && !(b->Cdiss > 0.0)) {
+    /* set_rates with a PML (time_step.f90:354-420); a block with
+       dissipation takes set_rates_dissipation instead (time_step.f90:
+       241-246) */
+    rates_pml(c, b, mix, pix, miy, piy, 0, 0);
+    rates_pml(c, b, mgx, mix - 1, miy, piy, 1, 0);
+    rates_pml(c, b, pix + 1, pgx, miy, piy, 2, 0);
+    rates_pml(c, b, mix, pix, mgy, miy - 1, 0, 1);
+    rates_pml(c, b, mix, pix, piy + 1, pgy, 0, 2);
+    rates_pml(c, b, mgx, mix - 1, mgy, miy - 1, 1, 1);
+    rates_pml(c, b, pix + 1, pgx, mgy, miy - 1, 2, 1);
+    rates_pml(c, b, mgx, mix - 1, piy + 1, pgy, 1, 2);
+    rates_pml(c, b, pix + 1, pgx, piy + 1, pgy, 2, 2);
+  } else {
   rates_interior(&S, mix, pix, miy, piy);
   rates_boundary(&S, mgx, mix - 1, miy, piy, 1, 0, mgx, pgx, mgy, pgy);         /* L  */
   rates_boundary(&S, pix + 1, pgx, miy, piy, 2, 0, mgx, pgx, mgy, pgy);         /* R  */
@@ -1933,6 +2177,7 @@ static void set_rates(orc_ctx* c, int ib) {
   rates_boundary(&S, pix + 1, pgx, mgy, miy - 1, 2, 1, mgx, pgx, mgy, pgy);     /* RB */
   rates_boundary(&S, mgx, mix - 1, piy + 1, pgy, 1, 2, mgx, pgx, mgy, pgy);     /* LT */
   rates_boundary(&S, pix + 1, pgx, piy + 1, pgy, 2, 2, mgx, pgx, mgy, pgy);     /* RT */
+  }
   /* rates_szz, rates.f90:920-936 (rates_h_szz, rates_heterogeneous.f90:901-917) */
   double* DF3 = DFL(2);
   double* DF5 = DFL(4);

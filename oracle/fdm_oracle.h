@@ -31,6 +31,11 @@ int orc_get_iface(orc_ctx* ctx, int32_t iface, int32_t field, double* out);
 int orc_set_iface(orc_ctx* ctx, int32_t iface, int32_t field, const double* in);
 int orc_get_iface_stage1(orc_ctx* ctx, int32_t iface, int32_t field, double* out);
 int orc_get_plastic(orc_ctx* ctx, double* gammap, double* lambda);
+int orc_get_pml(orc_ctx* ctx, double* Wx, double* Wy);
+int orc_set_pml(orc_ctx* ctx, const double* Wx, const double* Wy);
+/* the PML rates DWx, DWy, [6][ny][nx] each, as the last stage left them
+   (for tests; the device keeps them internal) */
+int orc_get_pml_rates(orc_ctx* ctx, double* DWx, double* DWy);
 /* OpenMP threads used by orc_step (1 = scalar restatement). */
 int orc_set_threads(int32_t nthreads);
 

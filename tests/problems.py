@@ -58,7 +58,7 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
           perturb=1e-3, source=False, plastic=None, nbx=1, rk_order=4, outside_frac=0.0,
           fault_overstress=25.0, ld_T=0.0, reg=False, rup_field="V", Vinit=1e-12,
           skempton=0.0, mms=False, medium=None, Cdiss=None, mode=2,
-          inplane=(-120.0, 0.0, -120.0, -120.0), force=None):
+          inplane=(-120.0, 0.0, -120.0, -120.0), force=None, pml=None):
     """Blocks stacked in y (len(nys) rows) and nbx columns; interfaces
     between them.  medium: amplitude of a smooth seeded heterogeneous medium
     (rho, cs, cp varying by that relative amount around the given values;
@@ -69,7 +69,8 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
     that frictional interfaces resolve onto the fault.  force: kinematic
     forcing of every frictional interface, a dict of c0, fd, dfdx, vL, vR, xL,
     xR, tmax (distance along the fault: x for horizontal faults, y for
-    vertical ones).
+    vertical ones).  pml: a perfectly matched layer for every block (a dict
+    of x, y, n, sigma, kappa, mx, px, my, py), or one dict or None per block.
     Returns (Problem, F0 fields [6,ny,nx])."""
     rng = np.random.default_rng(seed)
     nyb = list(nys)
@@ -180,6 +181,8 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
                 blk["s0"] = [sxx_, sxy_, 0.0, syy_, 0.0, szz_]
             if Cdiss is not None:
                 blk["Cdiss"] = Cdiss[len(prob.blocks) - 1] if np.ndim(Cdiss) else float(Cdiss)
+            if pml is not None:
+                blk["pml"] = pml[len(prob.blocks) - 1] if isinstance(pml, (list, tuple)) else pml
     # interfaces between vertically adjacent blocks (direction 'y') and
     # horizontally adjacent ones ('x'), domain.f90:1648-1682
     for by in range(len(nyb)):

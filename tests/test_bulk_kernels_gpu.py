@@ -55,6 +55,10 @@ CASES = {
                         law="RFL", Cdiss=[300.0, 0.0]),
     "dissipation_medium": dict(nx=203, nys=(61, 77), seed=43, warp=0.04, background=BG,
                                perturb=1e-3, law="RFL", Cdiss=300.0, medium=0.3),
+    # a perfectly matched layer in one block row (all of its points in the
+    # band kernel) next to a tiled one
+    "pml": dict(nx=203, nys=(61, 77), seed=53, warp=0.04, background=BG, perturb=1e-3,
+                law="RFL", pml=[dict(x=True, y=True, sigma=15.0, kappa=0.2), None]),
     # antiplane shear with a fault
     "antiplane": dict(nx=203, nys=(61, 77), seed=47, warp=0.04, background=[0.0, 0.0, 70.0],
                       perturb=1e-3, law="RFL", mode=3, inplane=(-120.0, 0.0, -100.0, -110.0)),
