@@ -7,7 +7,9 @@
 !   <name>.ckpt                    number of the latest checkpoint (list-
 !                                  directed integer; -1 once a run finished)
 !   <name>F.ckpt<n>                F(nF planes), DF(nF planes), then gammap
-!                                  and lambda when block 1 is plastic, each
+!                                  and lambda when block 1 is plastic, then
+!                                  the PML fields Wx (6 planes) when a block
+!                                  sets pmlx and Wy when one sets pmly, each
 !                                  plane nx*ny, x fastest (fields.f90:335-429);
 !                                  nF = 6 in plane strain, 3 in antiplane shear
 !   <name>block<i><L|R|B|T>.ckpt<n>  per side: F, Fhat, F0 of each of the nF
@@ -45,6 +47,9 @@ module fdmap_checkpoint
   type(ckpt_type) :: ck
 
   real(dp),parameter :: unset = 1d40
+
+  ! PML auxiliary fields [nx,ny,6] between the device and the files
+  real(dp),allocatable,target :: ckWx(:,:,:),ckWy(:,:,:)
 
 contains
 
@@ -173,6 +178,33 @@ contains
   end function abort_now
 
 
+  logical function any_pmlx()
+    ! Wx exists when some block sets pmlx (fields.f90:203-206)
+    integer :: i
+    any_pmlx = .false.
+    do i = 1,nblocks
+       any_pmlx = any_pmlx.or.HB(i)%pmlx
+    end do
+  end function any_pmlx
+
+
+  logical function any_pmly()
+    integer :: i
+    any_pmly = .false.
+    do i = 1,nblocks
+       any_pmly = any_pmly.or.HB(i)%pmly
+    end do
+  end function any_pmly
+
+
+  subroutine alloc_w()
+    if (any_pmlx().and..not.allocated(ckWx)) allocate(ckWx(gnx,gny,6))
+    if (any_pmly().and..not.allocated(ckWy)) allocate(ckWy(gnx,gny,6))
+    if (allocated(ckWx)) ckWx = 0.0_dp
+    if (allocated(ckWy)) ckWy = 0.0_dp
+  end subroutine alloc_w
+
+
   logical function block1_plastic()
     ! gammap and lambda exist when the first block is plastic (fields.f90:194)
     block1_plastic = (trim(HB(1)%response)=='plastic')
@@ -229,6 +261,9 @@ contains
        write(u) gp
        write(u) lam
     end if
+    ! fields.f90:406-407
+    if (any_pmlx()) write(u) ckWx
+    if (any_pmly()) write(u) ckWy
     close(u)
 
     do i = 1,nblocks
@@ -304,6 +339,10 @@ contains
        read(u,iostat=stat) gp
        if (stat==0) read(u,iostat=stat) lam
     end if
+    ! fields.f90:383-384
+    call alloc_w()
+    if (stat==0.and.any_pmlx()) read(u,iostat=stat) ckWx
+    if (stat==0.and.any_pmly()) read(u,iostat=stat) ckWy
     close(u)
     if (stat/=0) then
        call fail('Error reading '//name)
@@ -371,6 +410,9 @@ contains
     real(dp),allocatable,target :: buf(:)
     stat = fdm_get_fields(ctx,c_loc(F(1,1,1)),c_loc(DF(1,1,1)))
     if (stat==FDM_OK.and.block1_plastic()) stat = fdm_get_plastic(ctx,c_loc(gp(1,1)),c_loc(lam(1,1)))
+    call alloc_w()
+    if (stat==FDM_OK.and.any_pmlx()) stat = fdm_get_pml(ctx,c_loc(ckWx(1,1,1)),c_null_ptr)
+    if (stat==FDM_OK.and.any_pmly()) stat = fdm_get_pml(ctx,c_null_ptr,c_loc(ckWy(1,1,1)))
     off = 0
     do k = 1,nifaces
        if (stat/=FDM_OK) exit
@@ -396,6 +438,8 @@ contains
     real(dp),allocatable,target :: buf(:)
     stat = fdm_set_fields(ctx,c_loc(F(1,1,1)),c_loc(DF(1,1,1)))
     if (stat==FDM_OK.and.block1_plastic()) stat = fdm_set_plastic(ctx,c_loc(gp(1,1)),c_loc(lam(1,1)))
+    if (stat==FDM_OK.and.any_pmlx()) stat = fdm_set_pml(ctx,c_loc(ckWx(1,1,1)),c_null_ptr)
+    if (stat==FDM_OK.and.any_pmly()) stat = fdm_set_pml(ctx,c_null_ptr,c_loc(ckWy(1,1,1)))
     off = 0
     do k = 1,nifaces
        if (stat/=FDM_OK) exit
@@ -500,13 +544,15 @@ contains
   end function fdmap_host_ckpt_init
 
 
-  function fdmap_host_ckpt_files(op,number,F,DF,gp,lam,IF) bind(C,name='fdmap_host_ckpt_files') result(rc)
+  function fdmap_host_ckpt_files(op,number,F,DF,gp,lam,IF,Wx,Wy) bind(C,name='fdmap_host_ckpt_files') &
+       result(rc)
     ! op 0: write, 1: read, 2: delete checkpoint number; F, DF [6][ny][nx],
-    ! gp, lam [ny][nx], IF [18][frictional interface nodes]
+    ! gp, lam [ny][nx], IF [18][frictional interface nodes], Wx, Wy
+    ! [6][ny][nx] (used when a block sets pmlx / pmly)
     integer(c_int32_t),value :: op,number
-    type(c_ptr),value :: F,DF,gp,lam,IF
+    type(c_ptr),value :: F,DF,gp,lam,IF,Wx,Wy
     integer(c_int) :: rc
-    real(dp),pointer :: pF(:,:,:),pDF(:,:,:),pgp(:,:),plam(:,:),pIF(:,:)
+    real(dp),pointer :: pF(:,:,:),pDF(:,:,:),pgp(:,:),plam(:,:),pIF(:,:),pW(:,:,:)
     setup_error = ''
     if (op==2) then
        call files_delete(number)
@@ -519,9 +565,26 @@ contains
     call c_f_pointer(lam,plam,(/ gnx,gny /))
     call c_f_pointer(IF,pIF,(/ max(1,nfric_nodes()),FDM_IF_NFIELDS /))
     if (op==0) then
+       call alloc_w()
+       if (any_pmlx()) then
+          call c_f_pointer(Wx,pW,(/ gnx,gny,6 /))
+          ckWx = pW
+       end if
+       if (any_pmly()) then
+          call c_f_pointer(Wy,pW,(/ gnx,gny,6 /))
+          ckWy = pW
+       end if
        call files_write(number,pF,pDF,pgp,plam,pIF)
     else
        call files_read(number,pF,pDF,pgp,plam,pIF)
+       if (any_pmlx()) then
+          call c_f_pointer(Wx,pW,(/ gnx,gny,6 /))
+          pW = ckWx
+       end if
+       if (any_pmly()) then
+          call c_f_pointer(Wy,pW,(/ gnx,gny,6 /))
+          pW = ckWy
+       end if
     end if
     rc = 0
     if (setup_error/='') rc = -1

@@ -49,6 +49,9 @@ module fdmap_setup
      real(dp) :: rho,G_,cs,cp,M_,lambda,nu,K,gamma,Zs,Zp,mu,beta,b,h,eta
      logical :: het = .false.
      real(dp) :: Cdiss = 0.0_dp
+     ! perfectly matched layer: pmlx, pmly and n, sigma, kappa, mx, px, my, py
+     logical :: pmlx = .false., pmly = .false.
+     real(dp) :: pml(7) = 0.0_dp
      real(dp) :: F0blk(9)
      character(256) :: problem
      type(side_data) :: side(4)
@@ -438,6 +441,7 @@ contains
     end if
     call read_problem_list(input)
     if (setup_error=='') call read_domain(input)
+    if (setup_error=='') call read_pml(input)
     if (setup_error=='') call build_grid()
     if (setup_error=='') call read_blocks(input)
     if (setup_error=='') call read_interfaces(input)
@@ -647,6 +651,46 @@ contains
        gny = gny+HB(1+(iy-1)*nblocks_x)%G%ny
     end do
   end subroutine read_domain
+
+
+  subroutine read_pml(input)
+    ! init_pml (material.f90:152-223): pml_list after each block's marker.
+    ! The reference leaves n unset when the list does not give it; here it
+    ! defaults to 0, the constant-damping layer.
+    integer,intent(in) :: input
+    integer :: i,stat
+    character(256) :: str
+    logical :: found
+    logical :: pmlx,pmly
+    real(dp) :: n,sigma,kappa,mx,px,my,py
+    namelist /pml_list/ pmlx,pmly,n,sigma,kappa,mx,px,my,py
+    do i = 1,nblocks
+       pmlx = .false.
+       pmly = .false.
+       n = 0.0_dp
+       sigma = 0.0_dp
+       kappa = 0.0_dp
+       mx = -huge(0.0_dp)
+       px = huge(0.0_dp)
+       my = -huge(0.0_dp)
+       py = huge(0.0_dp)
+       write(str,'(a,i0,a)') '!---BLOCK',i,'---'
+       call seek_to_string(input,str,found)
+       read(input,nml=pml_list,iostat=stat)
+       if (stat>0) then
+          call fail('Error in pml_list')
+          return
+       end if
+       HB(i)%pmlx = pmlx
+       HB(i)%pmly = pmly
+       HB(i)%pml = (/ n,sigma,kappa,mx,px,my,py /)
+       if (pb_mode==3.and.(pmlx.or.pmly)) then
+          ! set_rates has no layer in mode 3 (time_step.f90:522-547)
+          call fail('pml_list: the perfectly matched layer is available in mode 2 only')
+          return
+       end if
+    end do
+  end subroutine read_pml
 
 
   subroutine build_grid()
@@ -1384,6 +1428,9 @@ contains
          ! artificial dissipation scaled as (wave speed)/(grid spacing)
          ! (domain.f90:316-320)
          cb%Cdiss = HBk%Cdiss*CFL/dt
+         cb%pmlx = merge(1,0,HBk%pmlx)
+         cb%pmly = merge(1,0,HBk%pmly)
+         cb%pml = HBk%pml
          ! initial_stress (fields.f90:1007-1058), default problem: the stress
          ! components that are not fields; in mode 3 the in-plane prestress
          ! that frictional interfaces resolve onto the fault

@@ -47,7 +47,7 @@ def load():
     lib.fdmap_host_run.restype = C.c_int
     lib.fdmap_host_ckpt_init.argtypes = [C.c_char_p]
     lib.fdmap_host_ckpt_init.restype = C.c_int
-    lib.fdmap_host_ckpt_files.argtypes = [C.c_int32, C.c_int32] + [C.c_void_p] * 5
+    lib.fdmap_host_ckpt_files.argtypes = [C.c_int32, C.c_int32] + [C.c_void_p] * 7
     lib.fdmap_host_ckpt_files.restype = C.c_int
     _lib = lib
     return lib
@@ -103,23 +103,26 @@ class HostProblem:
         return sum(P.ifaces[k].p - P.ifaces[k].m + 1 for k in range(P.nifaces)
                    if P.ifaces[k].coupling == capi.COUPLING["friction"])
 
-    def checkpoint_files(self, op, number, F=None, DF=None, gammap=None, lam=None, IF=None):
+    def checkpoint_files(self, op, number, F=None, DF=None, gammap=None, lam=None, IF=None, Wx=None,
+                         Wy=None):
         """op 'write' | 'read' | 'delete' on checkpoint number with host
         arrays F, DF [6, ny, nx], gammap, lam [ny, nx], IF [18, nodes of the
-        frictional interfaces]; 'read' returns them."""
+        frictional interfaces], Wx, Wy [6, ny, nx] (the PML fields, used when
+        a block sets pmlx / pmly); 'read' returns them."""
         lib = load()
         code = {"write": 0, "read": 1, "delete": 2}[op]
         nn = max(1, self.fric_nodes())
         shp = {"F": (6, self.ny, self.nx), "DF": (6, self.ny, self.nx),
                "gammap": (self.ny, self.nx), "lam": (self.ny, self.nx),
-               "IF": (len(capi.IFACE_FIELDS), nn)}
+               "IF": (len(capi.IFACE_FIELDS), nn), "Wx": (6, self.ny, self.nx), "Wy": (6, self.ny, self.nx)}
         arrs = {}
-        for name, v in (("F", F), ("DF", DF), ("gammap", gammap), ("lam", lam), ("IF", IF)):
+        for name, v in (("F", F), ("DF", DF), ("gammap", gammap), ("lam", lam), ("IF", IF), ("Wx", Wx),
+                        ("Wy", Wy)):
             if op == "read" or v is None:
                 arrs[name] = np.zeros(shp[name])
             else:
                 arrs[name] = np.ascontiguousarray(v, dtype=np.float64).reshape(shp[name])
-        ptrs = [arrs[k].ctypes.data for k in ("F", "DF", "gammap", "lam", "IF")]
+        ptrs = [arrs[k].ctypes.data for k in ("F", "DF", "gammap", "lam", "IF", "Wx", "Wy")]
         if lib.fdmap_host_ckpt_files(code, int(number), *ptrs) != 0:
             raise RuntimeError(f"fdmap host: {lib.fdmap_host_error().decode()}")
         return arrs

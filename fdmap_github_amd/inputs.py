@@ -69,7 +69,7 @@ vx vy
 def c2_planar_rs(name="c2", nx=1601, ny=801, h=0.025, nsteps=4000, outputs=True, ninfo=500,
                  law="RFL", plastic=False, curve=None, tau0=70.0, sn0=120.0, overstress=25.0,
                  R=1.5, Vinit=1e-12, a=0.010, b=0.014, sxy0=None, bcT="absorbing",
-                 medium=None, Cdiss=0.0):
+                 medium=None, Cdiss=0.0, pml=None):
     """Two blocks nx x ny coupled across a horizontal fault by SAT; slip law
     with strong velocity weakening; Gaussian overstress at the fault centre.
     curve: optional path of a fault-profile curve file (FP64 x, y, n1, n2),
@@ -78,7 +78,9 @@ def c2_planar_rs(name="c2", nx=1601, ny=801, h=0.025, nsteps=4000, outputs=True,
     Vinit under tau0.  bcT: boundary condition of the top of the upper block
     ('free' for a free surface).  medium: path of a heterogeneous medium file
     (write_medium) that both blocks read.  Cdiss: artificial dissipation of
-    both blocks (operator_list, domain.f90:167-180)."""
+    both blocks (operator_list, domain.f90:167-180).  pml: a perfectly
+    matched layer in both blocks, a dict of pml_list entries (pmlx, pmly, n,
+    sigma, kappa, mx, px, my, py; material.f90:152-223)."""
     Lx = 0.5 * (nx - 1) * h
     Ly = (ny - 1) * h
     psi0 = rs_state(a, 1e-6, tau0, sn0, Vinit)
@@ -86,6 +88,9 @@ def c2_planar_rs(name="c2", nx=1601, ny=801, h=0.025, nsteps=4000, outputs=True,
     c2 = f"bndBfile = '{curve}'," if curve else ""
     mat = f"rho = {_f(RHO)}, cs = {_f(CS)}, cp = {_f(CP)}"
     op = f"&operator_list Cdiss = {_f(Cdiss)} /\n\n" if Cdiss else ""
+    if pml:
+        ent = ", ".join(f"{k} = {('T' if v else 'F') if isinstance(v, bool) else _f(v)}" for k, v in pml.items())
+        op += f"&pml_list {ent} /\n\n"
     if plastic:
         mat = f"response = 'plastic', {mat}, mu = 0.6d0, b = 0d0, beta = 0d0"
     if medium:

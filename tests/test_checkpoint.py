@@ -158,3 +158,26 @@ def test_checkpoint_layout_antiplane(tmp_path):
     r = hp.checkpoint_files("read", 3)
     np.testing.assert_array_equal(r["F"][:3], st["F"])
     np.testing.assert_array_equal(r["IF"], st["IF"])
+
+
+def test_checkpoint_pml_fields(tmp_path):
+    """Wx and Wy (6 planes each) follow F, DF in the field file when a block
+    sets pmlx / pmly (fields.f90:383-384, 406-407), and read back"""
+    name = os.path.join(str(tmp_path), "p")
+    txt = inputs.c2_planar_rs(name=name, nx=41, ny=21, nsteps=10, outputs=False,
+                              pml=dict(pmlx=True, pmly=True, sigma=2.0))
+    hp = host.HostProblem(inputs.write(name + ".in", txt))
+    st = state(hp, 9)
+    del st["gammap"], st["lam"]
+    rng = np.random.default_rng(4)
+    st["Wx"] = rng.standard_normal((6, hp.ny, hp.nx))
+    st["Wy"] = rng.standard_normal((6, hp.ny, hp.nx))
+    hp.checkpoint_files("write", 12, **st)
+    planes = np.fromfile(name + "F.ckpt12", dtype="<f8").reshape(-1, hp.ny, hp.nx)
+    assert planes.shape[0] == 24
+    np.testing.assert_array_equal(planes[:6], st["F"])
+    np.testing.assert_array_equal(planes[12:18], st["Wx"])
+    np.testing.assert_array_equal(planes[18:], st["Wy"])
+    r = hp.checkpoint_files("read", 12)
+    for k in ("F", "DF", "Wx", "Wy"):
+        np.testing.assert_array_equal(r[k], st[k])

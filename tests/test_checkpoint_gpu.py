@@ -127,3 +127,48 @@ def test_abort_file_and_resume(gpu, tmp_path, reference_run):
     assert f"reading  checkpoint at time step {begin}" in r.stdout
     assert outputs(name) == reference_run
     assert ckpt_files(tmp_path, begin) == []
+
+
+def test_pml_resume_continues_bit_for_bit(gpu, tmp_path):
+    """a layered problem keeps Wx, Wy in its field checkpoint: a run resumed
+    from it ends with output files identical to an uninterrupted run's"""
+    layer = dict(pmlx=True, pmly=True, n=2.0, sigma=60.0, mx=-0.4, px=0.4, my=-0.3, py=0.3)
+    nx, ny = 121, 61
+
+    def pb(d, ckpt=""):
+        name = os.path.join(str(d), "p")
+        txt = inputs.c2_planar_rs(name=name, nx=nx, ny=ny, nsteps=NSTEPS, outputs=False, R=0.5, pml=layer)
+        items = "B\nbody\nvx sxy\n0d0 1d10 7\n"
+        txt = txt.replace("!---end:output_list---", items + "!---end:output_list---")
+        if ckpt:
+            txt += f"\n&checkpoint_list {ckpt} /\n"
+        return inputs.write(name + ".in", txt), name
+
+    def outs(name):
+        return {f: open(f"{name}_B_{f}.dat", "rb").read() for f in ("vx", "sxy")}
+
+    full = tmp_path / "full"
+    full.mkdir()
+    p, name = pb(full)
+    run(p)
+    ref = outs(name)
+    res = tmp_path / "res"
+    res.mkdir()
+    p, name = pb(res, "ckpt = T, interval = 25, del_final = F")
+    run(p)
+    assert outs(name) == ref
+    assert os.path.getsize(name + "F.ckpt50") == (12 + 12) * nx * (2 * ny) * 8
+    W = np.fromfile(name + "F.ckpt50", dtype="<f8").reshape(24, 2 * ny, nx)[12:]
+    assert np.abs(W).max() > 0.0
+    rec = nx * 2 * ny * 4
+    k = record_offset(50, 7)
+    for f in ("vx", "sxy"):
+        path = f"{name}_B_{f}.dat"
+        data = bytearray(open(path, "rb").read())
+        data[k * rec:] = b"\xff" * (len(data) - k * rec)
+        open(path, "wb").write(bytes(data))
+    assert outs(name) != ref
+    p, name = pb(res, "ckpt = T, resume = T, interval = 25")
+    r = run(p)
+    assert "reading  checkpoint at time step 50" in r.stdout
+    assert outs(name) == ref

@@ -178,3 +178,34 @@ def test_kinematic_forcing_is_read(tmp_path):
     x = np.ctypeslib.as_array(I.kn_x, shape=(n,))
     h = 0.025
     np.testing.assert_allclose(x, (np.arange(n) - (n - 1) / 2) * h, atol=1e-12)
+
+
+def test_pml_list_is_read(tmp_path):
+    """pml_list after each block's marker (material.f90:152-223); unset
+    limits are -huge/+huge, n defaults to 0 (constant damping)"""
+    layer = dict(pmlx=True, pmly=True, n=2.0, sigma=3.5, kappa=0.25, mx=-0.3, px=0.3, my=-0.2, py=0.2)
+    txt = inputs.c2_planar_rs(name=str(tmp_path / "p"), nx=41, ny=21, nsteps=5, outputs=False, pml=layer)
+    hp = host.HostProblem(inputs.write(str(tmp_path / "p.in"), txt))
+    P = hp.ptr.contents
+    for b in range(2):
+        B = P.blocks[b]
+        assert (B.pmlx, B.pmly) == (1, 1)
+        assert list(B.pml) == [2.0, 3.5, 0.25, -0.3, 0.3, -0.2, 0.2]
+    txt = inputs.c2_planar_rs(name=str(tmp_path / "q"), nx=41, ny=21, nsteps=5, outputs=False,
+                              pml=dict(pmlx=True, sigma=2.0))
+    P = host.HostProblem(inputs.write(str(tmp_path / "q.in"), txt)).ptr.contents
+    big = np.finfo(np.float64).max
+    assert (P.blocks[0].pmlx, P.blocks[0].pmly) == (1, 0)
+    assert list(P.blocks[0].pml) == [0.0, 2.0, 0.0, -big, big, -big, big]
+    txt = inputs.c2_planar_rs(name=str(tmp_path / "r"), nx=41, ny=21, nsteps=5, outputs=False)
+    P = host.HostProblem(inputs.write(str(tmp_path / "r.in"), txt)).ptr.contents
+    assert (P.blocks[0].pmlx, P.blocks[0].pmly, P.blocks[1].pmlx) == (0, 0, 0)
+
+
+def test_pml_refused_in_antiplane_shear(tmp_path):
+    txt = inputs.antiplane_rs(name=str(tmp_path / "m"), nx=21, ny=17, nsteps=5, outputs=False)
+    marker = "!---BLOCK1---"
+    assert marker in txt
+    txt = txt.replace(marker, marker + "\n\n&pml_list pmlx = T, sigma = 1d0 /\n")
+    with pytest.raises(RuntimeError, match="mode 2 only"):
+        host.HostProblem(inputs.write(str(tmp_path / "m.in"), txt))
