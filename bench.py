@@ -226,6 +226,67 @@ def run_c2_variant(args, tmp, het, cdiss):
     print(json.dumps(out), flush=True)
 
 
+def run_c2_pml(args, tmp):
+    """C2 with a graded perfectly matched layer in both blocks (pmlx and
+    pmly, n = 2, from 80 % of the half-widths outwards): every point is a
+    layer point, evaluated by the band kernel's layer path.  Per point per
+    stage the layer kernel reads F (48 B), Wx, Wy (96 B), the coordinates
+    (16 B) and, after stage 1, DF (40 B) and DWx, DWy (96 B); it writes F,
+    DF (96 B), W and its rates (192 B): 448 B at stage 1 and 584 B after, in
+    FP64 (the field planes halve in FP32, the coordinates stay FP64).  The
+    dominant launch is the one over the points four or more from every
+    block edge (bulk_pml<T, FULL, true>), which the profile counts with the
+    interior launches."""
+    from fdmap_github_amd import inputs
+
+    nx, ny = args.nx, args.ny
+    h = 0.025
+    Lx, Ly = 0.5 * (nx - 1) * h, (ny - 1) * h
+    layer = dict(pmlx=True, pmly=True, n=2.0, sigma=50.0 / (0.2 * Ly) ** 2, mx=-0.8 * Lx, px=0.8 * Lx,
+                 my=-0.8 * Ly, py=0.8 * Ly)
+    inpath = inputs.write(os.path.join(tmp, "c2pml.in"),
+                          inputs.c2_planar_rs(name=os.path.join(tmp, "c2pml"), nx=nx, ny=ny,
+                                              nsteps=args.steps + args.warmup + args.prof_steps,
+                                              outputs=False, pml=layer))
+    res = {}
+    for prec in ("fp64",) if args.no_fp32 else ("fp64", "fp32"):
+        r = run_device(inpath, prec, args.steps, args.warmup, args.prof_steps)
+        ns, prof = r["nstage"], r["prof"]
+        b1, b2 = (448.0, 584.0) if prec == "fp64" else (232.0, 300.0)
+        kpts = prof["bulk_points"]
+        assert kpts == 2 * (nx - 8) * (ny - 8)
+        bytes_launch = kpts * (b1 + (ns - 1) * b2) / ns
+        avg_ms = prof["bulk_ms"] / max(1, prof["bulk_launches"])
+        res[prec] = dict(r=r, ups=r["npts"] * args.steps / r["el"], avg_ms=avg_ms, kpts=kpts,
+                         achieved=bytes_launch / (avg_ms * 1e-3) / 1e9, bytes_launch=bytes_launch)
+    r64 = res["fp64"]
+    out = {
+        "metric": METRIC, "value": r64["ups"], "unit": "updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * r64["r"]["el"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (planar fault problem)",
+        "config": {"workload": f"planar rate-and-state rupture with a graded perfectly matched layer "
+                               f"(n 2) in both blocks, 2 blocks x ({nx}x{ny}), RFL slip law, LSRK(5,4) CFL 0.5",
+                   "grid_points": r64["r"]["npts"], "fault_nodes": nx, "dt": r64["r"]["hp"].dt},
+        "roofline": {
+            "bound": "hbm", "kernel": "bulk_pml<double,false,true> (layer points four or more from the block edges)",
+            "kernel_points": r64["kpts"], "achieved": r64["achieved"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS, "traffic": None,
+            "alg_bytes_per_launch": r64["bytes_launch"], "avg_launch_ms": r64["avg_ms"],
+        },
+    }
+    if "fp32" in res:
+        r32 = res["fp32"]
+        out["fp32_bulk"] = {
+            "updates_per_s": r32["ups"], "speedup_vs_fp64": r32["ups"] / r64["ups"],
+            "layer_avg_launch_ms": r32["avg_ms"], "layer_achieved_GBs": r32["achieved"],
+            "rel_l2_fields_vs_fp64": rel_l2(r32["r"]["F"], r64["r"]["F"]),
+            "rel_l2_slip_vs_fp64": rel_l2(r32["r"]["fault"]["D"], r64["r"]["fault"]["D"]),
+            "after_steps": args.steps + args.warmup,
+        }
+    print(json.dumps(out), flush=True)
+
+
 def run_m3(args, tmp):
     """Antiplane shear (mode 3): two blocks of 1601 x 801 points (h = 0.025
     km) side by side with a vertical aging-law fault between them, a free
@@ -502,10 +563,11 @@ def main():
                     help="steps timed after the headline ones with HIP events per launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--config", choices=("c2", "c2het", "c2diss", "m3", "c3", "c4", "c5"), default="c2",
+    ap.add_argument("--config", choices=("c2", "c2het", "c2diss", "c2pml", "m3", "c3", "c4", "c5"),
+                    default="c2",
                     help="c2: planar rupture (the headline); c2het: the same in a "
                          "heterogeneous medium; c2diss: the same with artificial dissipation "
-                         "(Cdiss 0.1); m3: antiplane shear rupture, 2 x (801x1601); "
+                         "(Cdiss 0.1); c2pml: the same inside a graded perfectly matched layer; m3: antiplane shear rupture, 2 x (801x1601); "
                          "c3: rough curvilinear fault with "
                          "off-fault plasticity, 2 x (4001x2001), FP64 and FP32-bulk; c4: 64 "
                          "uneven rupture scenarios advanced as one graph launch per step; c5: "
@@ -527,6 +589,9 @@ def main():
     if args.config in ("c2het", "c2diss"):
         run_c2_variant(args, tmp, het=args.config == "c2het",
                        cdiss=0.1 if args.config == "c2diss" else 0.0)
+        return
+    if args.config == "c2pml":
+        run_c2_pml(args, tmp)
         return
     if args.config == "c3":
         run_c3(args, tmp)

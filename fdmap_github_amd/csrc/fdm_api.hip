@@ -241,6 +241,10 @@ struct fdm_ctx {
   // Wy, DWx, DWy, 6 planes each (field precision, plane stride ps)
   bool pml = false, pml_wx = false, pml_wy = false;
   void* W = nullptr;
+  // the points of the layer blocks, row-major within each block: those four
+  // or more from every edge of their block (lpts[0]) and the rest (lpts[1])
+  int2* lpts[2] = {nullptr, nullptr};
+  int nlpts[2] = {0, 0};
   // 2: plane strain, 6 fields; 3: antiplane shear, 3 fields
   int mode = 2, nF = 6;
   std::vector<double> hnu;  // nu of the medium ([ny][nx]), for restore_df6
@@ -518,15 +522,41 @@ void launch_band(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   dim3 grid((c->nband + 63) / 64);
   if (c->mode == 3)
     hipLaunchKernelGGL(bulk_band3<T>, grid, dim3(64), 0, s, a, c->band, c->nband);
-  else if (full_variant(c, a) && (c->het || c->diss || c->pml))
+  else if (full_variant(c, a) && (c->het || c->diss))
     hipLaunchKernelGGL((bulk_band<T, true, true>), grid, dim3(64), 0, s, a, c->band, c->nband);
   else if (full_variant(c, a))
     hipLaunchKernelGGL((bulk_band<T, true, false>), grid, dim3(64), 0, s, a, c->band, c->nband);
-  else if (c->het || c->diss || c->pml)
+  else if (c->het || c->diss)
     hipLaunchKernelGGL((bulk_band<T, false, true>), grid, dim3(64), 0, s, a, c->band, c->nband);
   else
     hipLaunchKernelGGL((bulk_band<T, false, false>), grid, dim3(64), 0, s, a, c->band, c->nband);
   prof_end(c, ep, 2, s);
+}
+
+// the layer points, after the band on the same stream (their boundary
+// points need this stage's edge values); the inner points are timed with
+// the interior launches, the rest with the band
+template <typename T>
+void launch_layer(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
+  const bool full = full_variant(c, a);
+  if (c->nlpts[0]) {
+    EventPair* ep = prof_begin(c, 0, s);
+    dim3 grid((c->nlpts[0] + 255) / 256);
+    if (full)
+      hipLaunchKernelGGL((bulk_pml<T, true, true>), grid, dim3(256), 0, s, a, c->lpts[0], c->nlpts[0]);
+    else
+      hipLaunchKernelGGL((bulk_pml<T, false, true>), grid, dim3(256), 0, s, a, c->lpts[0], c->nlpts[0]);
+    prof_end(c, ep, 0, s);
+  }
+  if (c->nlpts[1]) {
+    EventPair* ep = prof_begin(c, 2, s);
+    dim3 grid((c->nlpts[1] + 255) / 256);
+    if (full)
+      hipLaunchKernelGGL((bulk_pml<T, true, false>), grid, dim3(256), 0, s, a, c->lpts[1], c->nlpts[1]);
+    else
+      hipLaunchKernelGGL((bulk_pml<T, false, false>), grid, dim3(256), 0, s, a, c->lpts[1], c->nlpts[1]);
+    prof_end(c, ep, 2, s);
+  }
 }
 
 template <typename T>
@@ -646,6 +676,7 @@ int run_stage(fdm_ctx* c, int stage, double t0, double dt, int final_step, int s
     if (rc) return rc;
     HIPCHK(c, hipStreamWaitEvent(c->side, c->evI, 0));
     launch_band<T>(c, c->side, a);
+    launch_layer<T>(c, c->side, a);
     // the previous step's fields are this stage's input until stage 2
     // writes them; both streams have finished them here
     if (c->rec_out && stage == 1 && !final_step && c->cur_n > c->first_n) record_receivers(c, c->side, c->cur_n - 1);
@@ -922,8 +953,8 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   {
     // interior tiles: each block's rectangle [mx+4, px-4] x [my+4, py-4]
     // covered from its corner, x fastest; band: every other point, row-major.
-    // A block with a perfectly matched layer in effect has no tiles: all of
-    // its points go to the band, whose per-point path carries the layer.
+    // A block with a perfectly matched layer in effect has no tiles and no
+    // band: all of its points go to the layer list.
     auto layer = [&](const fdm_block& B) { return (B.pmlx || B.pmly) && !(B.Cdiss > 0.0); };
     std::vector<int4> tl[3];
     for (int b = 0; b < c->nb; b++) {
@@ -938,13 +969,19 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
           for (int i0 = ia; i0 <= ib; i0 += tx) tl[q].push_back(make_int4(b, i0, j0, 0));
       }
     }
-    std::vector<int2> band;
+    std::vector<int2> band, lpts[2];
     for (int j = 0; j < P->ny; j++)
       for (int i = 0; i < P->nx; i++) {
         const fdm_block& B = P->blocks[block_of(c, i, j)];
-        if (layer(B) || i - B.mx < 4 || B.px - i < 4 || j - B.my < 4 || B.py - j < 4)
-          band.push_back(make_int2(i, j));
+        const bool edge = i - B.mx < 4 || B.px - i < 4 || j - B.my < 4 || B.py - j < 4;
+        if (c->mode == 2 && layer(B)) lpts[edge ? 1 : 0].push_back(make_int2(i, j));
+        else if (edge) band.push_back(make_int2(i, j));
       }
+    for (int q = 0; q < 2; q++) {
+      c->nlpts[q] = (int)lpts[q].size();
+      if (c->nlpts[q] && (rc = dupload(c, &c->lpts[q], lpts[q].data(), lpts[q].size() * sizeof(int2))))
+        return fail(rc);
+    }
     for (int q = 0; q < 3; q++) {
       c->ntiles[q] = (int)tl[q].size();
       if (c->ntiles[q] && (rc = dupload(c, &c->tiles[q], tl[q].data(), tl[q].size() * sizeof(int4))))
@@ -1612,7 +1649,7 @@ int fdm_profile_read(fdm_ctx* c, fdm_profile_t* out) {
   out->edge_launches = c->kind_n[1];
   out->band_ms = c->kind_ms[2];
   out->band_launches = c->kind_n[2];
-  out->bulk_points = c->bulk_mode == 2 ? (int64_t)c->ninterior : (int64_t)c->nx * c->ny;
+  out->bulk_points = c->bulk_mode == 2 ? (int64_t)c->ninterior + c->nlpts[0] : (int64_t)c->nx * c->ny;
   return FDM_OK;
 }
 

@@ -653,14 +653,16 @@ __device__ __forceinline__ void pml_yterm(T (&d)[6], T (&dw)[6], const T (&v)[6]
 // one-sided loops.  The block's scalar material is used.  The auxiliary
 // rates are scaled by A and the fields updated in place (fields.f90:
 // 1074-1077, 1181-1184): each lane touches only its own point's W.
-template <typename T>
+// INNER = true compiles the interior form alone (points four or more from
+// every edge of their block, where no SAT applies).
+template <typename T, bool INNER = false>
 __device__ __forceinline__ void pml_rates(const BulkArgs<T>& a, const BlockDev& B, const int i, const int j,
                                           const long long cc, T (&d)[6]) {
   const long long ps = a.ps;
   const int ld = a.ld;
   const T* __restrict__ F = a.Fin;
   const int dl = i - B.mx, dr = B.px - i, db = j - B.my, dtp = B.py - j;
-  if (dl == 0 || dr == 0 || db == 0 || dtp == 0) {
+  if (!INNER && (dl == 0 || dr == 0 || db == 0 || dtp == 0)) {
     T Fe[6];
 #pragma unroll
     for (int l = 0; l < 6; l++) Fe[l] = F[l * ps + cc];
@@ -669,8 +671,8 @@ __device__ __forceinline__ void pml_rates(const BulkArgs<T>& a, const BlockDev& 
     if (db == 0) sat_term<T>(d, Fe, a.sd, B.side_off[FDM_SIDE_B] + dl);
     if (dtp == 0) sat_term<T>(d, Fe, a.sd, B.side_off[FDM_SIDE_T] + dl);
   }
-  const int rx = dl < 4 ? 0 : (dr < 4 ? 2 : 1);
-  const int ry = db < 4 ? 0 : (dtp < 4 ? 2 : 1);
+  const int rx = INNER ? 1 : (dl < 4 ? 0 : (dr < 4 ? 2 : 1));
+  const int ry = INNER ? 1 : (db < 4 ? 0 : (dtp < 4 ? 2 : 1));
   const T hxi = (T)B.phx[3 * ry + rx], hyi = (T)B.phy[3 * ry + rx];
   // damping profile (rates.f90:2048-2049, 2063-2066)
   double sxd = B.pml[1], syd = B.pml[1];
@@ -744,6 +746,7 @@ __device__ __forceinline__ void pml_rates(const BulkArgs<T>& a, const BlockDev& 
       d[3] = d[3] + (hxi * dx[1] + hyi * dy[0]) * cb;
       d[4] = d[4] + (hxi * dx[0] * gamma + hyi * dy[1]) * cm;
     }
+  } else if constexpr (INNER) {
   } else if (ry == 1) {
     // L or R strip: one-sided x, then central y
 #pragma unroll
@@ -1105,6 +1108,29 @@ __global__ void __launch_bounds__(64) bulk_band(BulkArgs<T> a, const int2* __res
   if (k >= npts) return;
   const int2 p = pts[k];
   point_generic<T, FULL, EXT>(a, p.x, p.y);
+}
+
+// Layer kernel: the points of the perfectly matched layer blocks, one lane
+// each, from a list that runs row by row through each block (consecutive
+// lanes read consecutive addresses).  It carries only the layer's code;
+// INNER = true serves the points four or more from every block edge with
+// the interior form alone, which keeps more waves in flight.
+template <typename T, bool FULL, bool INNER>
+__global__ void __launch_bounds__(256) bulk_pml(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= npts) return;
+  const int2 p = pts[k];
+  const int i = p.x, j = p.y;
+  const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
+  const BlockDev& B = a.blocks[bi];
+  const long long cc = (long long)j * a.ld + i;
+  T d[6];
+  scaled_rates<T>(a, cc, d);
+  pml_rates<T, INNER>(a, B, i, j, cc, d);
+  T Fc[6];
+#pragma unroll
+  for (int l = 0; l < 6; l++) Fc[l] = a.Fin[l * a.ps + cc];
+  finish_point<T, FULL>(a, B, bi, i, j, cc, d, Fc);
 }
 
 // Batched band kernel: pts[k] = (i, j, problem, valid), every wave's 64
