@@ -533,30 +533,26 @@ void launch_band(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   prof_end(c, ep, 2, s);
 }
 
+template <typename T, bool INNER>
+void launch_layer_list(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, int q) {
+  EventPair* ep = prof_begin(c, INNER ? 0 : 2, s);
+  dim3 grid((c->nlpts[q] + 255) / 256), block(256);
+  if (full_variant(c, a))
+    hipLaunchKernelGGL((bulk_pml<T, true, INNER, false>), grid, block, 0, s, a, c->lpts[q], c->nlpts[q]);
+  else if (c->nt_store)
+    hipLaunchKernelGGL((bulk_pml<T, false, INNER, true>), grid, block, 0, s, a, c->lpts[q], c->nlpts[q]);
+  else
+    hipLaunchKernelGGL((bulk_pml<T, false, INNER, false>), grid, block, 0, s, a, c->lpts[q], c->nlpts[q]);
+  prof_end(c, ep, INNER ? 0 : 2, s);
+}
+
 // the layer points, after the band on the same stream (their boundary
 // points need this stage's edge values); the inner points are timed with
 // the interior launches, the rest with the band
 template <typename T>
 void launch_layer(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
-  const bool full = full_variant(c, a);
-  if (c->nlpts[0]) {
-    EventPair* ep = prof_begin(c, 0, s);
-    dim3 grid((c->nlpts[0] + 255) / 256);
-    if (full)
-      hipLaunchKernelGGL((bulk_pml<T, true, true>), grid, dim3(256), 0, s, a, c->lpts[0], c->nlpts[0]);
-    else
-      hipLaunchKernelGGL((bulk_pml<T, false, true>), grid, dim3(256), 0, s, a, c->lpts[0], c->nlpts[0]);
-    prof_end(c, ep, 0, s);
-  }
-  if (c->nlpts[1]) {
-    EventPair* ep = prof_begin(c, 2, s);
-    dim3 grid((c->nlpts[1] + 255) / 256);
-    if (full)
-      hipLaunchKernelGGL((bulk_pml<T, true, false>), grid, dim3(256), 0, s, a, c->lpts[1], c->nlpts[1]);
-    else
-      hipLaunchKernelGGL((bulk_pml<T, false, false>), grid, dim3(256), 0, s, a, c->lpts[1], c->nlpts[1]);
-    prof_end(c, ep, 2, s);
-  }
+  if (c->nlpts[0]) launch_layer_list<T, true>(c, s, a, 0);
+  if (c->nlpts[1]) launch_layer_list<T, false>(c, s, a, 1);
 }
 
 template <typename T>

@@ -402,8 +402,8 @@ __device__ __forceinline__ void plastic_point(double G, double K, double mu, dou
 // plastic.f90:8-59).  Fc: the point's fields at the start of the stage.
 // FULL = false leaves out the singular source, the manufactured-solution
 // source and plasticity; the host selects it for launches that need none of
-// them.
-template <typename T, bool FULL>
+// them.  NTS stores F and DF non-temporally.
+template <typename T, bool FULL, bool NTS = false>
 __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDev& B, const int bi,
                                              const int i, const int j, const long long cc,
                                              T (&d)[6], const T (&Fc)[6]) {
@@ -467,10 +467,16 @@ __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDe
       }
     }
 #pragma unroll
-    for (int l = 0; l < 6; l++) a.Fout[l * ps + cc] = f[l];
+    for (int l = 0; l < 6; l++) {
+      if constexpr (NTS) __builtin_nontemporal_store(f[l], &a.Fout[l * ps + cc]);
+      else a.Fout[l * ps + cc] = f[l];
+    }
   }
 #pragma unroll
-  for (int l = 0; l < 6; l++) a.DF[l * ps + cc] = d[l];
+  for (int l = 0; l < 6; l++) {
+    if constexpr (NTS) __builtin_nontemporal_store(d[l], &a.DF[l * ps + cc]);
+    else a.DF[l * ps + cc] = d[l];
+  }
 }
 
 // dissipation terms of rates_dissipation_boundary_mode2 (rates.f90:1485-
@@ -690,13 +696,16 @@ __device__ __forceinline__ void pml_rates(const BulkArgs<T>& a, const BlockDev& 
   T dwx[6], dwy[6];
 #pragma unroll
   for (int l = 0; l < 6; l++) {
-    dwx[l] = (!onx || a.a_zero) ? (T)0.0 : A * W[(12 + l) * ps + cc];
-    dwy[l] = (!ony || a.a_zero) ? (T)0.0 : A * W[(18 + l) * ps + cc];
+    // W and its rates stream through once per stage: non-temporal, so that
+    // they do not displace the field rows neighbouring points reuse
+    dwx[l] = (!onx || a.a_zero) ? (T)0.0 : A * __builtin_nontemporal_load(&W[(12 + l) * ps + cc]);
+    dwy[l] = (!ony || a.a_zero) ? (T)0.0 : A * __builtin_nontemporal_load(&W[(18 + l) * ps + cc]);
   }
+  T wx[6], wy[6];
   if (onx) {
-    T w[6];
+    T* w = wx;
 #pragma unroll
-    for (int l = 0; l < 6; l++) w[l] = W[l * ps + cc];
+    for (int l = 0; l < 6; l++) w[l] = __builtin_nontemporal_load(&W[l * ps + cc]);
     d[0] = d[0] - w[2] / rho;
     d[1] = d[1] - w[3] / rho;
     d[2] = d[2] - w[0] * M;
@@ -706,9 +715,9 @@ __device__ __forceinline__ void pml_rates(const BulkArgs<T>& a, const BlockDev& 
     for (int l = 0; l < 6; l++) dwx[l] = dwx[l] - (sx + kap) * w[l];
   }
   if (ony) {
-    T w[6];
+    T* w = wy;
 #pragma unroll
-    for (int l = 0; l < 6; l++) w[l] = W[(6 + l) * ps + cc];
+    for (int l = 0; l < 6; l++) w[l] = __builtin_nontemporal_load(&W[(6 + l) * ps + cc]);
     d[0] = d[0] - w[3] / rho;
     d[1] = d[1] - w[4] / rho;
     d[2] = d[2] - w[1] * gamma * M;
@@ -797,18 +806,19 @@ __device__ __forceinline__ void pml_rates(const BulkArgs<T>& a, const BlockDev& 
     }
   }
   const T Bdt = (T)a.Bdt;
+  // W + Bdt*DW from the values loaded above (this lane's own point)
   if (onx) {
 #pragma unroll
     for (int l = 0; l < 6; l++) {
-      if (a.do_update) W[l * ps + cc] = W[l * ps + cc] + Bdt * dwx[l];
-      W[(12 + l) * ps + cc] = dwx[l];
+      if (a.do_update) __builtin_nontemporal_store(wx[l] + Bdt * dwx[l], &W[l * ps + cc]);
+      __builtin_nontemporal_store(dwx[l], &W[(12 + l) * ps + cc]);
     }
   }
   if (ony) {
 #pragma unroll
     for (int l = 0; l < 6; l++) {
-      if (a.do_update) W[(6 + l) * ps + cc] = W[(6 + l) * ps + cc] + Bdt * dwy[l];
-      W[(18 + l) * ps + cc] = dwy[l];
+      if (a.do_update) __builtin_nontemporal_store(wy[l] + Bdt * dwy[l], &W[(6 + l) * ps + cc]);
+      __builtin_nontemporal_store(dwy[l], &W[(18 + l) * ps + cc]);
     }
   }
 }
@@ -1114,8 +1124,9 @@ __global__ void __launch_bounds__(64) bulk_band(BulkArgs<T> a, const int2* __res
 // each, from a list that runs row by row through each block (consecutive
 // lanes read consecutive addresses).  It carries only the layer's code;
 // INNER = true serves the points four or more from every block edge with
-// the interior form alone, which keeps more waves in flight.
-template <typename T, bool FULL, bool INNER>
+// the interior form alone, which keeps more waves in flight.  NTS: F and
+// DF stored non-temporally (working set beyond the last-level cache).
+template <typename T, bool FULL, bool INNER, bool NTS>
 __global__ void __launch_bounds__(256) bulk_pml(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= npts) return;
@@ -1130,7 +1141,7 @@ __global__ void __launch_bounds__(256) bulk_pml(BulkArgs<T> a, const int2* __res
   T Fc[6];
 #pragma unroll
   for (int l = 0; l < 6; l++) Fc[l] = a.Fin[l * a.ps + cc];
-  finish_point<T, FULL>(a, B, bi, i, j, cc, d, Fc);
+  finish_point<T, FULL, NTS>(a, B, bi, i, j, cc, d, Fc);
 }
 
 // Batched band kernel: pts[k] = (i, j, problem, valid), every wave's 64
