@@ -215,6 +215,93 @@ vz sxz syz
 """
 
 
+def kali_het(directory, name="kali_het", n=301, nsteps=2000, outputs=True, ninfo=10):
+    """The reference's mode-3 rate-and-state example with fault properties
+    from files (problems/kali-heterogeneous-input.in, files as
+    problems/kali_input.m makes them): two 30 km x 30 km blocks side by side
+    with a vertical aging-law fault between them, a free surface on top; a,
+    b, the state and a fault prestress vary along the fault.  n points per
+    block side (301 in the example).  The example leaves FDmethod at the
+    reference's default, SBP6 (domain.f90:94); this one asks for SBP4, the
+    operator built here.  Writes the three fault files next to the .in text
+    it returns."""
+    os.makedirs(directory, exist_ok=True)
+    y = np.linspace(-30.0, 0.0, n)
+    a = 0.01 + 0.005 * np.sin(2 * np.pi * y / 10)
+    b = 0.016 + 0.004 * np.exp(-0.5 * ((y + 15) / 3) ** 2)
+    rs = [a, b, 1e-6 + 0 * y, 0.6 + 0 * y, 0.05 + 0 * y, 0 * y, 1e10 + 0 * y]
+    fric = os.path.join(directory, "kali.friction")
+    state = os.path.join(directory, "kali.state")
+    pre = os.path.join(directory, "kali.prestress")
+    # consecutive FP64 arrays along the fault (friction.f90:373-489)
+    np.concatenate(rs).astype("<f8").tofile(fric)
+    (0.6 + 0.1 * np.exp(y / 5)).astype("<f8").tofile(state)
+    np.concatenate([np.cos(2 * np.pi * y / 10), np.sin(2 * np.pi * y / 4)]).astype("<f8").tofile(pre)
+    out = ""
+    if outputs:
+        out = """I
+iface1
+V S N Psi D
+0d0 1d10 1
+"""
+    return f"""&problem_list
+ name = '{os.path.join(directory, name)}', ninfo = {ninfo},
+ nt = {nsteps + 1}, CFL = 0.5d0, refine = 1d0 /
+
+&domain_list
+ mode = 3, FDmethod = 'SBP4', nx = {n}, ny = {2 * n},
+ nblocks = 2, nblocks_x = 2, nblocks_y = 1, nifaces = 1 /
+
+!---BLOCK1---
+
+&grid_list
+ nx = {n}, ny = {n}, mgx = 1, mgy = 1, iblock_x = 1, iblock_y = 1,
+ LT = -30d0,  0d0, RT = 0d0,  0d0,
+ LB = -30d0,-30d0, RB = 0d0,-30d0 /
+
+&boundaries_list
+ bcL = 'absorbing', bcR = 'none', bcB = 'absorbing', bcT = 'free' /
+
+!---BLOCK2---
+
+&grid_list
+ nx = {n}, ny = {n}, mgx = {n + 1}, mgy = 1, iblock_x = 2, iblock_y = 1,
+ LT = 0d0,  0d0, RT = 30d0,  0d0,
+ LB = 0d0,-30d0, RB = 30d0,-30d0 /
+
+&boundaries_list
+ bcL = 'none', bcR = 'absorbing', bcB = 'absorbing', bcT = 'free' /
+
+!---ALL BLOCKS---
+
+&material_list
+ rho = 2.7d0, cs = 3.4641d0, cp = 6d0 /
+
+&fields_list
+ sxx0 = -50d0,
+ sxz0 =  30d0 /
+
+!---IFACE1---
+
+&interface_list
+ coupling = 'friction', iblockm = 1, iblockp = 2,
+ direction = 'x', mg = 1, pg = {n} /
+
+&friction_list
+ angle = 90d0, xlockm = -25d0, xlockp = 25d0,
+ friction_law = 'RAL',
+ friction_file = T, filename = '{fric}',
+ stress_file = T, stress_filename = '{pre}',
+ state_file = T, state_filename = '{state}',
+ Psi0 = 0.6d0,
+ rs = 0.01d0,0.016d0,1d-6,0.6d0,0.05d0,
+ ld = 'gaussian',5d0,0d0,1d-9,4d0,-15d0,0d0 /
+
+!---begin:output_list---
+{out}!---end:output_list---
+"""
+
+
 def gp229_sw3(name="gp229", nx=801, nsteps=1000, CFL=1.0, outputs=True, ninfo=10):
     """The reference's antiplane slip-weakening example (gp229/gp229sw3ex1.in):
     two 80 km x 40 km blocks stacked in y with a horizontal fault, prestress

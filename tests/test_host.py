@@ -209,3 +209,24 @@ def test_pml_refused_in_antiplane_shear(tmp_path):
     txt = txt.replace(marker, marker + "\n\n&pml_list pmlx = T, sigma = 1d0 /\n")
     with pytest.raises(RuntimeError, match="mode 2 only"):
         host.HostProblem(inputs.write(str(tmp_path / "m.in"), txt))
+
+
+def test_kali_example_fault_files(tmp_path):
+    """friction_file, stress_file and state_file of the mode-3 example
+    (friction.f90:373-489): a..Vw, S0 then N0, and Psi along the fault"""
+    n = 41
+    txt = inputs.kali_het(str(tmp_path), n=n, nsteps=5, outputs=False)
+    hp = host.HostProblem(inputs.write(str(tmp_path / "k.in"), txt))
+    P = hp.ptr.contents
+    assert P.mode == 3 and P.nifaces == 1
+    I = P.ifaces[0]
+    assert I.direction == 0 and I.p - I.m + 1 == n
+    y = np.linspace(-30.0, 0.0, n)
+    rs = np.ctypeslib.as_array(I.rs, shape=(7, n))
+    np.testing.assert_array_equal(rs, np.fromfile(str(tmp_path / "kali.friction"), dtype="<f8").reshape(7, n))
+    np.testing.assert_allclose(rs[0], 0.01 + 0.005 * np.sin(2 * np.pi * y / 10), rtol=1e-15)
+    pre = np.fromfile(str(tmp_path / "kali.prestress"), dtype="<f8").reshape(2, n)
+    np.testing.assert_array_equal(np.ctypeslib.as_array(I.S0base, shape=(n,)), pre[0])
+    np.testing.assert_array_equal(np.ctypeslib.as_array(I.N0base, shape=(n,)), pre[1])
+    np.testing.assert_array_equal(np.ctypeslib.as_array(I.Psi0, shape=(n,)),
+                                  np.fromfile(str(tmp_path / "kali.state"), dtype="<f8"))

@@ -155,3 +155,28 @@ def test_gp229_forced_slip_weakening_rupture(gpu, tmp_path):
         assert rel_l2(a, b) <= 1e-12
     V = orc.get_iface(0, "V")
     assert V.max() > 0.1 and np.sum(V > 0) > 10  # the forced rupture has grown
+
+
+def test_kali_fault_properties_from_files(gpu, tmp_path):
+    """the reference's mode-3 aging-law example with a, b, the state and a
+    fault prestress read from files (problems/kali-heterogeneous-input.in,
+    files as problems/kali_input.m makes them), at 121 points a side (the
+    example has 301).  Coarser grids and longer runs meet the aging law's
+    stiffness: where the state has dropped, exp((f0 - Psi)/b) V0/L exceeds
+    what LSRK(5,4) integrates stably at dt, the state blows up and the
+    Newton solve fails, as the reference's would; 200 steps here stay well
+    before that (the oracle alone runs to step 397)."""
+    p = inputs.write(str(tmp_path / "kali.in"),
+                     inputs.kali_het(str(tmp_path), n=121, nsteps=200, outputs=False))
+    hp, dev, orc = run_pair(p, 200)
+    e = rel_l2(dev.get_fields()[0], orc.get_fields()[0])
+    print("fields", e)
+    assert e <= 1e-12
+    for f in ("D", "V", "S", "N", "Psi"):
+        a, b = dev.get_iface(0, f), orc.get_iface(0, f)
+        print(f, rel_l2(a, b))
+        assert rel_l2(a, b) <= 1e-12
+    # the properties along the fault are the files' (a varies, so does Psi)
+    rs = np.ctypeslib.as_array(hp.ptr.contents.ifaces[0].rs, shape=(7, 121))
+    assert np.ptp(rs[0]) > 0.009 and np.ptp(orc.get_iface(0, "Psi")) > 0.0
+    assert np.isfinite(orc.get_iface(0, "V")).all()
