@@ -245,6 +245,7 @@ struct fdm_ctx {
   // or more from every edge of their block (lpts[0]) and the rest (lpts[1])
   int2* lpts[2] = {nullptr, nullptr};
   int nlpts[2] = {0, 0};
+  long long nlayer_inner = 0;  // lpts[0] entries that are points (not padding)
   // 2: plane strain, 6 fields; 3: antiplane shear, 3 fields
   int mode = 2, nF = 6;
   std::vector<double> hnu;  // nu of the medium ([ny][nx]), for restore_df6
@@ -970,9 +971,27 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
       for (int i = 0; i < P->nx; i++) {
         const fdm_block& B = P->blocks[block_of(c, i, j)];
         const bool edge = i - B.mx < 4 || B.px - i < 4 || j - B.my < 4 || B.py - j < 4;
-        if (c->mode == 2 && layer(B)) lpts[edge ? 1 : 0].push_back(make_int2(i, j));
-        else if (edge) band.push_back(make_int2(i, j));
+        if (c->mode == 2 && layer(B)) {
+          if (edge) lpts[1].push_back(make_int2(i, j));
+        } else if (edge) {
+          band.push_back(make_int2(i, j));
+        }
       }
+    // inner layer points: each row of a layer block's interior in runs of
+    // 64 that start where the stored planes are 64-element aligned
+    // ((i + coff) % 64 == 0), padded with (-1, -1) entries that do nothing
+    for (int b = 0; b < c->nb; b++) {
+      const fdm_block& B = P->blocks[b];
+      if (c->mode != 2 || !layer(B)) continue;
+      const int ia = B.mx + 4, ib = B.px - 4;
+      if (ia > ib) continue;
+      const int i0 = ia - (ia + c->coff) % 64;
+      const int i1 = ib + 1 + (64 - (ib + 1 + c->coff) % 64) % 64;
+      for (int j = B.my + 4; j <= B.py - 4; j++)
+        for (int i = i0; i < i1; i++)
+          lpts[0].push_back(i >= ia && i <= ib ? make_int2(i, j) : make_int2(-1, -1));
+      c->nlayer_inner += (long long)(ib - ia + 1) * (B.py - B.my - 7);
+    }
     for (int q = 0; q < 2; q++) {
       c->nlpts[q] = (int)lpts[q].size();
       if (c->nlpts[q] && (rc = dupload(c, &c->lpts[q], lpts[q].data(), lpts[q].size() * sizeof(int2))))
@@ -1371,10 +1390,13 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     if ((rc = dupload(c, &c->Y, P->y, np * 8))) return fail(rc);
   }
   if (c->pml_wx || c->pml_wy) {
-    // Wx, Wy start at zero (fields.f90:203-209), and so do their rates
-    const size_t bytes = 24 * (size_t)c->ps * c->esz;
-    if ((rc = dalloc(c, &c->W, bytes))) return fail(rc);
-    HIPCHK(c, hipMemset(c->W, 0, bytes));
+    // Wx, Wy start at zero (fields.f90:203-209), and so do their rates; the
+    // planes carry the field planes' column offset, so that rows align alike
+    const size_t bytes = 24 * (size_t)c->ps * c->esz + 64 * c->esz;
+    void* wb = nullptr;
+    if ((rc = dalloc(c, &wb, bytes))) return fail(rc);
+    HIPCHK(c, hipMemset(wb, 0, bytes));
+    c->W = (char*)wb + (size_t)c->coff * c->esz;
   }
   if (c->any_plastic) {
     size_t np = (size_t)c->nx * c->ny;
@@ -1645,7 +1667,7 @@ int fdm_profile_read(fdm_ctx* c, fdm_profile_t* out) {
   out->edge_launches = c->kind_n[1];
   out->band_ms = c->kind_ms[2];
   out->band_launches = c->kind_n[2];
-  out->bulk_points = c->bulk_mode == 2 ? (int64_t)c->ninterior + c->nlpts[0] : (int64_t)c->nx * c->ny;
+  out->bulk_points = c->bulk_mode == 2 ? (int64_t)c->ninterior + c->nlayer_inner : (int64_t)c->nx * c->ny;
   return FDM_OK;
 }
 

@@ -1122,7 +1122,8 @@ __global__ void __launch_bounds__(64) bulk_band(BulkArgs<T> a, const int2* __res
 
 // Layer kernel: the points of the perfectly matched layer blocks, one lane
 // each, from a list that runs row by row through each block (consecutive
-// lanes read consecutive addresses).  It carries only the layer's code;
+// lanes read consecutive addresses; entries with x < 0 pad the inner list
+// to aligned runs of 64).  It carries only the layer's code;
 // INNER = true serves the points four or more from every block edge with
 // the interior form alone, which keeps more waves in flight.  NTS: F and
 // DF stored non-temporally (working set beyond the last-level cache).
@@ -1131,6 +1132,7 @@ __global__ void __launch_bounds__(256) bulk_pml(BulkArgs<T> a, const int2* __res
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= npts) return;
   const int2 p = pts[k];
+  if (p.x < 0) return;  // padding of an aligned run
   const int i = p.x, j = p.y;
   const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
   const BlockDev& B = a.blocks[bi];
