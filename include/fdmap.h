@@ -223,14 +223,17 @@ typedef struct fdm_problem {
 /* Kernel timing gathered with HIP events on the context's stream. */
 typedef struct fdm_profile_t {
   double bulk_ms;        /* summed duration of the bulk (stencil+RK) launches
-                            that update the block interiors                 */
+                            that update the block interiors: the interior
+                            tiles and, in PML blocks, the layer points four
+                            or more from every block edge                  */
   int64_t bulk_launches;
   double edge_ms;        /* summed duration of edge/interface launches      */
   int64_t edge_launches;
-  int64_t bulk_points;   /* grid points updated by each of those launches   */
+  int64_t bulk_points;   /* grid points updated per stage by those launches */
   double band_ms;        /* summed duration of the launches that update the
-                            points within four of a block edge (0 when the
-                            bulk launch covers every point)                 */
+                            points within four of a block edge, PML blocks'
+                            included (0 when the bulk launch covers every
+                            point)                                         */
   int64_t band_launches;
 } fdm_profile_t;
 
