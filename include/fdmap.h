@@ -1,12 +1,15 @@
 /*
  * fdmap.h -- C ABI of the MI355X time-stepping library for FDMAP problems.
  *
- * The library advances the plane-strain (mode 2) elastic wave equation on a
- * multi-block curvilinear grid with SBP4 first-derivative operators, SAT
- * boundary/interface penalties, rate-and-state friction on block interfaces,
- * a singular moment-tensor source, Drucker-Prager plasticity and low-storage
- * Runge-Kutta time integration.  Everything lives in device memory; host
- * pointers handed to any call are read or written only during that call.
+ * The library advances the plane-strain (mode 2) or antiplane (mode 3)
+ * elastic wave equation on a multi-block curvilinear grid with SBP4
+ * first-derivative operators, SAT boundary/interface penalties, rate-and-
+ * state or slip-weakening friction (with kinematic forcing) on block
+ * interfaces, a singular moment-tensor source, Drucker-Prager plasticity,
+ * heterogeneous media, artificial dissipation, perfectly matched layers and
+ * low-storage Runge-Kutta time integration.  Everything lives in device
+ * memory; host pointers handed to any call are read or written only during
+ * that call.
  *
  * The reference implements this path as Fortran module procedures that take
  * assumed-shape arrays (not C-interoperable).  Each entry point below names
