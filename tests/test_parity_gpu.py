@@ -373,3 +373,29 @@ def test_fp32_bulk_error_against_fp64(gpu):
     assert orc.get_iface(0, "V").max() > 1e-3  # rupture under way
     for k, v in e.items():
         assert v <= 1e-5, (k, v)
+
+
+def test_pml_fp32_bulk_error_against_fp64(gpu):
+    """FP32-bulk with graded layers in both blocks of a prestressed rupture
+    (W in FP32 with the fields) against the FP64 oracle: relative L2 <= 1e-5
+    on fields, slip, slip velocity and W"""
+    layer = dict(x=True, y=True, n=2.0, sigma=300.0, kappa=0.0, mx=-0.8, px=0.8, my=-0.6, py=0.6)
+    kw = dict(nx=96, nys=(40, 40), seed=13, warp=0.03, background=BG, perturb=0.0, law="RFL",
+              fault_overstress=25.0, pml=layer)
+    p32, F = problems.build(precision="fp32", **kw)
+    p64, _ = problems.build(precision="fp64", **kw)
+    dev = Context(p32)
+    orc = OracleContext(p64, threads=4)
+    for c in (dev, orc):
+        c.set_fields(F)
+        c.init_edges(0.0)
+        c.step(0.0, DT, 1, 150)
+    e = {"fields": rel_l2(dev.get_fields()[0], orc.get_fields()[0])}
+    for f in ("D", "V", "S"):
+        e[f] = rel_l2(dev.get_iface(0, f), orc.get_iface(0, f))
+    Wd, Wo = dev.get_pml(), orc.get_pml()
+    e["Wx"], e["Wy"] = rel_l2(Wd[0], Wo[0]), rel_l2(Wd[1], Wo[1])
+    print("FP32-bulk layer vs FP64 relative L2", e)
+    assert orc.get_iface(0, "V").max() > 1e-3 and np.abs(Wo[0]).max() > 0.0
+    for k, v in e.items():
+        assert v <= 1e-5, (k, v)
