@@ -987,10 +987,13 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
       if (ia > ib) continue;
       const int i0 = ia - (ia + c->coff) % 64;
       const int i1 = ib + 1 + (64 - (ib + 1 + c->coff) % 64) % 64;
-      for (int j = B.my + 4; j <= B.py - 4; j++)
+      // count the points as they are pushed: a block with fewer than
+      // nine rows has no inner rows at all
+      for (int j = B.my + 4; j <= B.py - 4; j++) {
         for (int i = i0; i < i1; i++)
           lpts[0].push_back(i >= ia && i <= ib ? make_int2(i, j) : make_int2(-1, -1));
-      c->nlayer_inner += (long long)(ib - ia + 1) * (B.py - B.my - 7);
+        c->nlayer_inner += ib - ia + 1;
+      }
     }
     for (int q = 0; q < 2; q++) {
       c->nlpts[q] = (int)lpts[q].size();

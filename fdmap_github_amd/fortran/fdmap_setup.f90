@@ -774,6 +774,12 @@ contains
          if (stat>0) call fail('Error in material_list')
          if (trim(response)/='elastic'.and.trim(response)/='plastic') &
               call fail('Invalid material response: '//trim(response))
+         ! a plastic block with these flags accumulates the plastic strain
+         ! tensor Ep and plastic work Wp (plastic.f90:51-54) and writes them
+         ! into the field checkpoint (fields.f90:194-200, 381-389); neither
+         ! is built, so the run stops rather than dropping them
+         if (trim(response)=='plastic'.and.(plastic_strain_tensor.or.plastic_work)) &
+              call fail('plastic_strain_tensor and plastic_work are not supported')
          ! the first heterogeneous block reads the medium of the whole grid
          ! (material.f90:80-81)
          if (heterogeneous.and.(.not.allocated(gmrho))) call read_medium(filename)

@@ -52,6 +52,109 @@ def test_c2_setup_closed_forms(tmp_path):
         assert np.all(F[l] == v)
 
 
+def _dense_sbp4(n):
+    """dense SBP4 first derivative (unit spacing) from the published
+    rationals (Strand 1994), independent of the host's tables"""
+    from fractions import Fraction as Fr
+
+    dl = [[Fr(-24, 17), Fr(59, 34), Fr(-4, 17), Fr(-3, 34), 0, 0],
+          [Fr(-1, 2), 0, Fr(1, 2), 0, 0, 0],
+          [Fr(4, 43), Fr(-59, 86), 0, Fr(59, 86), Fr(-4, 43), 0],
+          [Fr(3, 98), 0, Fr(-59, 98), 0, Fr(32, 49), Fr(-4, 49)]]
+    D = np.zeros((n, n))
+    for i in range(4):
+        D[i, :6] = [float(v) for v in dl[i]]
+        D[n - 1 - i, n - 6:] = [-float(v) for v in dl[i][::-1]]
+    for i in range(4, n - 4):
+        D[i, i - 2:i + 3] = [1 / 12, -2 / 3, 0.0, 2 / 3, -1 / 12]
+    return D
+
+
+def test_curved_block_from_curve_files(tmp_path):
+    """A block whose bottom and top edges are read from curve files
+    (read_bnd, grid.f90:646-702: FP64 x, y, n1, n2) and whose left and right
+    edges are straight lines between the corners (line_map,
+    geometry.f90:70-106).  Transfinite interpolation (geometry.f90:153-200;
+    Boyd 2001, pp. 114-115) of such a block is, in closed form,
+    x(i, j) = xB(i) and y(i, j) = (1 - t) yB(i) + t yT(i), t = j/(ny-1),
+    when both curves share their x and the side lines are vertical.  Then:
+    the metrics are the SBP4 derivative of those coordinates (grid.f90:
+    465-534), J = xq yr - xr yq and Ji = 1/J; the side normals are the curve
+    files' on the bottom, their negation on the top, and the lines'
+    outward normals on the left and right (grid.f90:431, 453); the SAT
+    coefficients are H00i c |(xq, yq)| / J (domain.f90:1436-1456)."""
+    nx, ny, Lx, Ly = 41, 27, 0.5, 0.65
+    A1, A2 = 0.04, 0.025
+    x = np.linspace(-Lx, Lx, nx)
+    k = np.pi / Lx
+
+    def curve(y, yp, name):
+        nrm = np.sqrt(1.0 + yp ** 2)
+        n1, n2 = yp / nrm, -1.0 / nrm  # downward unit normal
+        path = str(tmp_path / name)
+        inputs.write_curve(path, x, y, n1, n2)
+        return path, np.vstack([n1, n2])
+
+    yB = A1 * np.sin(k * (x + Lx))
+    yB[0] = yB[-1] = 0.0
+    yT = Ly + A2 * np.sin(2 * k * (x + Lx)) * np.cos(k * x)
+    yT[0] = yT[-1] = Ly
+    pB, nB = curve(yB, A1 * k * np.cos(k * (x + Lx)), "bottom.bin")
+    pT, nT = curve(yT, A2 * (2 * k * np.cos(2 * k * (x + Lx)) * np.cos(k * x)
+                             - k * np.sin(2 * k * (x + Lx)) * np.sin(k * x)), "top.bin")
+    text = f"""&problem_list
+ name = 'cb', nt = 3, CFL = 0.5d0, RKorder = 4 /
+
+&domain_list
+ mode = 2, FDmethod = 'SBP4', nblocks = 1, nx = {nx}, ny = {ny} /
+
+!---BLOCK1---
+
+&grid_list
+ nx = {nx}, ny = {ny}, bndBfile = '{pB}', bndTfile = '{pT}',
+ LB = {-Lx!r}d0,0d0, LT = {-Lx!r}d0,{Ly!r}d0,
+ RB = {Lx!r}d0,0d0, RT = {Lx!r}d0,{Ly!r}d0 /
+
+&boundaries_list bcL = 'absorbing', bcR = 'absorbing', bcB = 'absorbing', bcT = 'free' /
+
+&material_list rho = 2.67d0, cs = 3.464d0, cp = 6d0 /
+
+!---begin:output_list---
+!---end:output_list---
+"""
+    hp = host.HostProblem(inputs.write(str(tmp_path / "cb.in"), text))
+    X, Y = hp.coords("x"), hp.coords("y")
+    t = (np.arange(ny) / (ny - 1))[:, None]
+    np.testing.assert_allclose(X, np.broadcast_to(x, (ny, nx)), rtol=0, atol=1e-15)
+    np.testing.assert_allclose(Y, (1 - t) * yB + t * yT, rtol=0, atol=1e-15)
+    assert np.abs(Y[1:-1] - ((1 - t) * yB[None, :] * 0 + t * Ly)[1:-1]).max() > 0.01  # curved
+    Dq, Dr = _dense_sbp4(nx), _dense_sbp4(ny)
+    want = {"xq": X @ Dq.T, "yq": Y @ Dq.T, "xr": Dr @ X, "yr": Dr @ Y}
+    for name, w in want.items():
+        np.testing.assert_allclose(hp.coords(name), w, rtol=0, atol=1e-14, err_msg=name)
+    xq, xr, yq, yr = (hp.coords(n) for n in ("xq", "xr", "yq", "yr"))
+    J = xq * yr - xr * yq
+    np.testing.assert_allclose(hp.coords("J"), J, rtol=1e-14)
+    np.testing.assert_allclose(hp.coords("Ji"), 1.0 / J, rtol=1e-14)
+    # second-order closure at the edges, fourth order inside: the analytic
+    # slope of the bottom curve is matched within the truncation error
+    hq = 2 * Lx / (nx - 1)
+    np.testing.assert_allclose(yq[0, 4:-4], hq * A1 * k * np.cos(k * (x[4:-4] + Lx)), rtol=0,
+                               atol=5e-4 * A1)
+    b = hp.ptr.contents.blocks[0]
+    normals = [np.ctypeslib.as_array(b.side[s].normal, shape=(2, b.side[s].n)) for s in range(4)]
+    np.testing.assert_allclose(normals[0], np.tile([[-1.0], [0.0]], ny), atol=1e-15)
+    np.testing.assert_allclose(normals[1], np.tile([[1.0], [0.0]], ny), atol=1e-15)
+    np.testing.assert_array_equal(normals[2], nB)
+    np.testing.assert_array_equal(normals[3], -nT)
+    assert np.all(normals[2][1] < 0) and np.all(normals[3][1] > 0)  # outward on both
+    for s, j in ((2, 0), (3, ny - 1)):
+        M = np.ctypeslib.as_array(b.side[s].M, shape=(5, nx))
+        scale = np.sqrt(xq[j] ** 2 + yq[j] ** 2) / J[j]
+        np.testing.assert_allclose(M[3], H00I * 3.464 * scale, rtol=1e-14)
+        np.testing.assert_allclose(M[4], H00I * 6.0 * scale, rtol=1e-14)
+
+
 def test_c1_source_placement(tmp_path):
     p = inputs.write(str(tmp_path / "c1.in"), inputs.c1_pointsource(name="c1", n=41, nsteps=5))
     hp = host.HostProblem(p)
@@ -76,6 +179,27 @@ def test_unsupported_options_fail_loudly(tmp_path, text, msg):
     p = inputs.write(str(tmp_path / "bad.in"), text)
     with pytest.raises(RuntimeError, match=msg):
         host.HostProblem(p)
+
+
+@pytest.mark.parametrize("flag", ["plastic_strain_tensor", "plastic_work"])
+def test_plastic_strain_and_work_flags_are_refused(tmp_path, flag):
+    """a plastic block that asks for the plastic strain tensor or plastic
+    work (plastic.f90:51-54, fields.f90:194-200) stops with a message: the
+    host does not accumulate them, and its field checkpoint would lack the
+    records the reference writes for them (fields.f90:381-389)"""
+    text = inputs.c2_planar_rs(name="pf", nx=41, ny=21, nsteps=5, plastic=True)
+    assert "response = 'plastic'" in text
+    bad = text.replace("response = 'plastic',", f"response = 'plastic', {flag} = T,")
+    p = inputs.write(str(tmp_path / "pf.in"), bad)
+    with pytest.raises(RuntimeError, match="plastic_strain_tensor and plastic_work"):
+        host.HostProblem(p)
+    # the same flags on an elastic block change nothing in the reference
+    # (fields.f90:194 allocates them only for plastic blocks): accepted
+    ok = inputs.c2_planar_rs(name="pf", nx=41, ny=21, nsteps=5).replace(
+        "&material_list ", f"&material_list {flag} = T, ")
+    host.HostProblem(inputs.write(str(tmp_path / "pe.in"), ok))
+    # and the plastic problem without them is accepted
+    host.HostProblem(inputs.write(str(tmp_path / "pp.in"), text))
 
 
 def test_heterogeneous_medium_file(tmp_path):

@@ -270,7 +270,7 @@ def test_plastic_return_onto_yield_surface():
     assert abs(sig1 - sig) < 1e-12 * abs(sig)  # beta = 0: no dilatancy
     assert np.allclose(dev1 / tau1, dev / tau, rtol=1e-12, atol=1e-14)  # radial return
     assert abs(lam - (tau + mu * sig) / (dt * G)) < 1e-10 * lam
-    # plastic.f90 increments gammap by dt*lambda twice (lines 161 and 177)
+    # plastic.f90 increments gammap by dt*lambda twice (lines 129 and 177)
     assert abs(gp - 2.0 * dt * lam) < 1e-15 * gp
     # rates pick up -lambda * G * dev / tau
     assert np.allclose(Ds1, -lam * G * dev / tau, rtol=1e-12, atol=1e-12)
@@ -743,6 +743,64 @@ def test_boundary_hat_states_satisfy_characteristic_conditions(bcs):
             assert np.abs(h["vt"]).max() < tol / Zs.max() and np.abs(h["vn"]).max() < tol / Zp.max()
 
 
+@pytest.mark.parametrize("bcs", [("absorbing-velocity", "free-velocity", "free-velocity",
+                                  "absorbing-velocity"),
+                                 ("free-velocity", "absorbing-velocity", "absorbing-velocity",
+                                  "free-velocity")])
+def test_velocity_boundary_hat_states(bcs):
+    """'absorbing-velocity' and 'free-velocity' (boundaries.f90:333-342)
+    keep the grid's traction in the hat state and change only the velocity,
+    both relative to the side's initial fields F0: absorbing-velocity takes
+    the velocity at which the hat's incoming characteristics vanish,
+    snt + Zs vt = snn + Zp vn = 0; free-velocity the velocity a free surface
+    gives, the one that keeps the outgoing characteristics at zero traction,
+    -Zs vt = snt - Zs vt(grid), -Zp vn = snn - Zp vn(grid).  Neither moves
+    stt (snn is unchanged).  The initial fields are far from zero so that F0
+    matters."""
+    cs, cp, rho = 3.464, 6.0, 2.67
+    bg = np.array([0.3, -0.2, -120.0, 70.0, -110.0, -100.0])
+    prob, F = problems.build(nx=13, nys=(14,), seed=13, warp=0.07, cs=cs, cp=cp, rho=rho,
+                             background=list(bg), perturb=3.0, bcs=bcs)
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.stage1(0.0, 1e-3, 1)
+    _, DF = o.get_fields()
+    G, M = rho * cs ** 2, rho * cp ** 2
+    gamma = (M - 2 * G) / M
+    dx, dy = split_ops(prob)
+    vx, vy, sxx, sxy, syy = F[:5]
+    ref = [(dx(sxx) + dy(sxy)) / rho, (dx(sxy) + dy(syy)) / rho, M * (dx(vx) + gamma * dy(vy)),
+           G * (dx(vy) + dy(vx)), M * (gamma * dx(vx) + dy(vy))]
+    SAT = np.array([DF[l] - ref[l] for l in range(5)])
+    FD = F - bg[:, None, None]
+    ny, nx = prob.ny, prob.nx
+    blk = prob.blocks[0]
+    for s, side in enumerate(blk["sides"]):
+        idx = [(slice(None), 0), (slice(None), nx - 1), (0, slice(None)), (ny - 1, slice(None))][s]
+        n1, n2 = side["normal"]
+        Zs, Zp, gam, Ks, Kp = side["M"]
+        pen = to_nt(SAT[(slice(None),) + idx], n1, n2)
+        vt, vn, stt, snt, snn = to_nt(FD[(slice(None),) + idx], n1, n2)
+        dvt, dvn, dstt, dsnt, dsnn = pen[0] / Ks, pen[1] / Kp, pen[2] / Kp, pen[3] / Ks, pen[4] / Kp
+        k = slice(1, -1)  # corners carry two penalties
+        sc = np.abs(np.concatenate([snn, snt, Zp * vn, Zs * vt])).max()
+        tol = 1e-9 * sc
+        vtol = tol / Zs.max()
+        # the traction and stt are kept
+        assert np.abs(dsnt[k]).max() < tol and np.abs(dsnn[k]).max() < tol, (s, "traction")
+        assert np.abs(dstt[k]).max() < tol, (s, "stt")
+        vt_h, vn_h = (vt + dvt)[k], (vn + dvn)[k]
+        if bcs[s] == "absorbing-velocity":
+            assert np.abs(snt[k] + Zs[k] * vt_h).max() < tol, (s, "S in")
+            assert np.abs(snn[k] + Zp[k] * vn_h).max() < tol, (s, "P in")
+        else:
+            assert np.abs(-Zs[k] * vt_h - (snt - Zs * vt)[k]).max() < tol, (s, "S out")
+            assert np.abs(-Zp[k] * vn_h - (snn - Zp * vn)[k]).max() < tol, (s, "P out")
+        # the velocity changes: the conditions are not vacuous here
+        assert np.abs(dvt[k]).max() > 1e3 * vtol and np.abs(dvn[k]).max() > 1e3 * vtol
+
+
 def test_locked_interface_hat_states():
     """Locked interface (interfaces.f90:715-772): the hat states of the two
     sides have equal velocity and traction, each side keeps its outgoing
@@ -866,6 +924,71 @@ def test_frictional_interface_hat_states_and_law(seed):
     Sf = a * st["N"] * np.arcsinh(st["V"] / (2 * V0) * np.exp(st["Psi"] / a))
     assert np.abs(Sf - st["S"]).max() < 1e4 * 2.3e-16 * 200
     assert (st["V"][k] > 0).any() and (st["N"] > 0).all()
+
+
+def test_frictional_interface_takes_initial_fields_crosswise():
+    """The reference's call at interfaces.f90:795 passes the minus side's
+    initial fields F0m where frictional_interface_mode2 (interfaces.f90:900)
+    declares its plus side's, F0p, and the plus side's where it declares
+    F0m.  With equal prestress on both sides nothing shows.  Here each block
+    rests at its own uniform prestress, the minus block's shear stress 10 MPa
+    below the plus block's, on a flat fault.  Then, in closed form:
+
+    - the fault is loaded by the minus side's resolved prestress S0 = 60:
+      a N asinh(V e^(Psi/a) / 2V0) = 60 - eta V, with eta = Zs/2;
+    - each side's hat state is completed with the other side's initial
+      fields, so the penalties move the minus side's snt by +10 - eta V and
+      vt by (10 - eta V)/Zs, the plus side's snt by -10 - eta V and vt by
+      (10 + eta V)/Zs; normal stress and normal velocity stay.
+
+    Reading the arguments in their declared order would load the fault with
+    70 (a slip velocity about e^8 times larger) and leave every penalty at
+    about eta V."""
+    from scipy.optimize import brentq
+
+    cs, cp, rho = 3.464, 6.0, 2.67
+    bg_p = np.array([0.0, 0.0, -120.0, 70.0, -120.0, -120.0])
+    bg_m = bg_p.copy()
+    bg_m[3] = 60.0
+    prob, F = problems.build(nx=17, nys=(12, 13), seed=35, warp=0.0, cs=cs, cp=cp, rho=rho,
+                             background=list(bg_p), perturb=0.0, law="RFL", fault_overstress=0.0,
+                             bcs=("absorbing",) * 4)
+    F[:, :12] = bg_m[:, None, None]
+    for side in prob.blocks[0]["sides"]:
+        side["F0"] = np.tile(bg_m[:, None], (1, side["F0"].shape[1]))
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.stage1(0.0, 1e-3, 1)
+    _, DF = o.get_fields()
+    V, S, S0, N, Psi = (o.get_iface(0, f) for f in ("V", "S", "S0", "N", "Psi"))
+    itf = prob.ifaces[0]
+    a, V0 = itf["rs"][0], itf["rs"][2]
+    Zs = rho * cs
+    eta = 0.5 * Zs
+    # the load is the minus side's prestress
+    assert np.all(S0 == 60.0) and np.all(N == 120.0)
+    want = np.array([brentq(lambda v: a[k] * N[k] * np.arcsinh(v / (2 * V0[k]) * np.exp(Psi[k] / a[k]))
+                            - (60.0 - eta * v), 0.0, 1.0, xtol=1e-300, rtol=1e-15)
+                     for k in range(len(V))])
+    np.testing.assert_allclose(V, want, rtol=1e-10)
+    np.testing.assert_allclose(S, 60.0 - eta * V, rtol=1e-14)
+    other = brentq(lambda v: a[0] * 120.0 * np.arcsinh(v / (2 * V0[0]) * np.exp(Psi[0] / a[0]))
+                   - (70.0 - eta * v), 0.0, 1.0, xtol=1e-300, rtol=1e-15)
+    assert other > 1e3 * V[0]
+    # penalties of the stage-1 rates at the fault rows (uniform fields: the
+    # operator adds nothing); flat fault, normal (0, 1): t = x, n = y
+    n1, n2 = itf["nhat"]
+    assert np.allclose(n1, 0.0) and np.allclose(n2, 1.0)
+    k = slice(1, -1)
+    for b, j, s, dsnt, dvt in ((0, 11, 3, 10.0 - eta * V, (10.0 - eta * V) / Zs),
+                              (1, 12, 2, -10.0 - eta * V, (10.0 + eta * V) / Zs)):
+        Zs_, Zp_, gam, Ks, Kp = prob.blocks[b]["sides"][s]["M"]
+        pen = to_nt(DF[:5, j], n1, n2)
+        d = [pen[0] / Ks, pen[1] / Kp, pen[2] / Kp, pen[3] / Ks, pen[4] / Kp]
+        np.testing.assert_allclose(d[3][k], dsnt[k], rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(d[0][k], dvt[k], rtol=1e-12, atol=1e-15)
+        assert np.abs(d[1][k]).max() < 1e-12 and np.abs(d[4][k]).max() < 1e-12
 
 
 def _fault_problem(seed, **kw):
