@@ -138,7 +138,7 @@ class fdm_problem(C.Structure):
         ("source", fdm_source),
         ("mat_rho", _dp), ("mat_cs", _dp), ("mat_cp", _dp),
         ("J", _dp),
-        ("mode", C.c_int32), ("pad_mode_", C.c_int32),
+        ("mode", C.c_int32), ("sbp", C.c_int32),
     ]
 
 
@@ -173,6 +173,12 @@ ENTRY_POINTS = [
     ("fdm_sync", C.c_int, [_P]),
     ("fdm_profile_enable", C.c_int, [_P, C.c_int32]),
     ("fdm_profile_read", C.c_int, [_P, C.POINTER(fdm_profile_t)]),
+]
+
+# entry points of include/fdmap.h that only the MI355X library implements:
+# the allocation-free variant of fdm_max_slip_velocity
+NOALLOC_ENTRY_POINTS = [
+    ("fdm_max_slip_velocity_noalloc", C.c_int, [_P, _dp]),
 ]
 
 
@@ -213,7 +219,7 @@ def bind(lib, prefix_from="fdm_", prefix_to="fdm_"):
         fn = getattr(lib, prefix_to + name[len(prefix_from):])
         fn.restype = res
         fn.argtypes = args
-    for name, res, args in ENSEMBLE_ENTRY_POINTS + ADJOINT_ENTRY_POINTS:
+    for name, res, args in NOALLOC_ENTRY_POINTS + ENSEMBLE_ENTRY_POINTS + ADJOINT_ENTRY_POINTS:
         if hasattr(lib, name):
             fn = getattr(lib, name)
             fn.restype = res

@@ -78,7 +78,7 @@ module fdmap_capi
      type(fdm_source) :: source
      type(c_ptr) :: mat_rho = c_null_ptr, mat_cs = c_null_ptr, mat_cp = c_null_ptr
      type(c_ptr) :: J = c_null_ptr
-     integer(c_int32_t) :: mode = 2, pad_mode_ = 0
+     integer(c_int32_t) :: mode = 2, sbp = 4
   end type fdm_problem
 
   type,bind(C) :: fdm_profile_t

@@ -42,10 +42,12 @@ def _f64(a):
 class Problem:
     """One problem in the fdm_problem layout (0-based global indices)."""
 
-    def __init__(self, nx: int, ny: int, rk_order: int = 4, precision: str = "fp64", mode: int = 2):
+    def __init__(self, nx: int, ny: int, rk_order: int = 4, precision: str = "fp64", mode: int = 2,
+                 sbp: int = 4):
         self.nx, self.ny = int(nx), int(ny)
         # 2: plane strain (6 fields); 3: antiplane shear (3 fields)
         self.mode = int(mode)
+        self.sbp = int(sbp)  # first-derivative operator: 2, 4 or 6
         self.rk_order = rk_order
         self.precision = precision
         shape = (self.ny, self.nx)
@@ -119,6 +121,7 @@ class Problem:
             prob.mat_rho = prob.mat_cs = prob.mat_cp = capi.dptr(None)
         prob.J = P(self.J) if self.J is not None else capi.dptr(None)
         prob.mode = self.mode
+        prob.sbp = self.sbp
 
         barr = (capi.fdm_block * max(1, len(self.blocks)))()
         for b, blk in enumerate(self.blocks):

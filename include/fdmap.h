@@ -220,7 +220,11 @@ typedef struct fdm_problem {
      onto the fault (interfaces.f90:859-876).  Mode 3 covers elastic
      homogeneous blocks without sources or dissipation. */
   int32_t mode;
-  int32_t pad_mode_;
+  /* first-derivative operator, FDmethod of domain_list (fd_coeff.f90:
+     41-248): 2 'SBP2', 4 'SBP4', 6 'SBP6'; 0 means 4.  The metrics of the
+     problem and the SAT coefficients (H00i) must come from the same
+     operator. */
+  int32_t sbp;
 } fdm_problem;
 
 /* Kernel timing gathered with HIP events on the context's stream. */
@@ -299,8 +303,20 @@ int fdm_get_pml(fdm_ctx* ctx, double* Wx, double* Wy);
 int fdm_set_pml(fdm_ctx* ctx, const double* Wx, const double* Wy);
 
 /* max over all frictional interfaces of V (check_nucleation,
-   domain.f90:474-499, interfaces.f90:506-517). */
+   domain.f90:474-499, interfaces.f90:506-517), reduced on the device in a
+   buffer that fdm_create allocates with the context. */
 int fdm_max_slip_velocity(fdm_ctx* ctx, double* vmax);
+
+/* The same maximum, bit for bit, from a call that allocates nothing: its
+   partial maxima live in a fixed device array of the library (520 bytes per
+   device, held until the process ends) instead of context memory.  No
+   counterpart in the reference: it serves callers that cannot allocate
+   device memory (tight serving loops, harnesses that forbid allocation).
+   Limit: 16384 frictional interface nodes per context (FDM_EINVAL beyond,
+   use fdm_max_slip_velocity).  Every call on a device shares that device's
+   array, so calls that overlap on one device race; consecutive calls do
+   not depend on each other. */
+int fdm_max_slip_velocity_noalloc(fdm_ctx* ctx, double* vmax);
 
 /* Wait for all queued work. */
 int fdm_sync(fdm_ctx* ctx);

@@ -45,6 +45,10 @@ def load():
     lib.orc_sbp4_tables.restype = None
     lib.orc_diff.argtypes = [C.c_int32, _dp, _dp]
     lib.orc_diff.restype = None
+    lib.orc_diff_op.argtypes = [C.c_int32, C.c_int32, _dp, _dp]
+    lib.orc_diff_op.restype = None
+    lib.orc_sbp_tables.argtypes = [C.c_int32, _dp, _dp, _dp, _dp]
+    lib.orc_sbp_tables.restype = C.c_int32
     lib.orc_diss1d.argtypes = [C.c_int32, _dp, _dp]
     lib.orc_diss1d.restype = None
     lib.orc_rk_coeffs.argtypes = [C.c_int32, _dp, _dp, _dp]
@@ -87,11 +91,29 @@ def sbp4_tables():
     return DI, DL, HL, H.value
 
 
-def diff(f):
+def sbp_tables(order):
+    """(DI, DL, HL, H00i) of 'SBP<order>': DI over -pI..pI, DL [nbst, nbnd]
+    (row r = DL(0..nbnd-1, r)), HL [nbst]"""
+    lib = load()
+    DI = np.empty(7)
+    DL = np.empty(6 * 9)
+    HL = np.empty(6)
+    H = C.c_double()
+    code = lib.orc_sbp_tables(order, capi.dptr(DI), capi.dptr(DL), capi.dptr(HL),
+                              C.cast(C.pointer(H), _dp))
+    assert code, order
+    nbnd, nbst, pI = code // 100, (code // 10) % 10, code % 10
+    return DI[:2 * pI + 1], DL[:nbst * nbnd].reshape(nbst, nbnd), HL[:nbst], H.value
+
+
+def diff(f, order=4):
     lib = load()
     f = np.ascontiguousarray(f, dtype=np.float64)
     out = np.empty_like(f)
-    lib.orc_diff(len(f), capi.dptr(f), capi.dptr(out))
+    if order == 4:
+        lib.orc_diff(len(f), capi.dptr(f), capi.dptr(out))
+    else:
+        lib.orc_diff_op(order, len(f), capi.dptr(f), capi.dptr(out))
     return out
 
 

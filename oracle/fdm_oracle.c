@@ -22,64 +22,148 @@
 #endif
 
 /* ---------------------------------------------------------------------- */
-/* SBP4 tables: the literal 15-digit values of fd_coeff.f90:101-126.       */
+/* SBP tables: the literal 15-digit values of fd_coeff.f90:79-246 for the    */
+/* methods 'SBP2', 'SBP4' and 'SBP6' (init_fd).                             */
 /* ---------------------------------------------------------------------- */
 
-/* DI(-2..2), fd_coeff.f90:101-102 */
-static const double DI4[5] = {0.083333333333333, -0.666666666666667, 0.000000000000000,
-                              0.666666666666667, -0.083333333333333};
-/* DL4[r][s] = DL(s,r), fd_coeff.f90:108-115 */
-static const double DL4[4][6] = {
-    {-1.411764705882353, 1.735294117647059, -0.235294117647059, -0.088235294117647,
-     0.000000000000000, 0.000000000000000},
-    {-0.500000000000000, 0.000000000000000, 0.500000000000000, 0.000000000000000,
-     0.000000000000000, 0.000000000000000},
-    {0.093023255813953, -0.686046511627907, 0.000000000000000, 0.686046511627907,
-     -0.093023255813953, 0.000000000000000},
-    {0.030612244897959, 0.000000000000000, -0.602040816326531, 0.000000000000000,
-     0.653061224489796, -0.081632653061224}};
-/* HL, fd_coeff.f90:126 */
-static const double HL4[4] = {0.354166666666667, 1.229166666666667, 0.895833333333333,
-                              1.020833333333333};
-/* H00i for SBP4, fd_coeff.f90:65 */
-static const double H00I4 = 2.823529411764706;
+typedef struct {
+  int order;            /* 2, 4, 6 */
+  int nbnd, nbst, pI;   /* fd_coeff.f90:56-70: points of a boundary stencil,
+                           boundary stencils, interior half-width */
+  double H00i;
+  double DI[7], AI[7];  /* DI(s), AI(s) at [pI + s] */
+  double DL[6][9];      /* DL[r][s] = DL(s,r) */
+  double AL[6][9];      /* AL[r][s] = AL(s,r) */
+  double HL[6];
+} sbp_t;
 
-#define NBND 6 /* fd_coeff.f90:63 */
-#define NBST 4 /* fd_coeff.f90:64 */
-#define PI_I 2 /* (nI-1)/2 for SBP4 */
+/* 'SBP2', fd_coeff.f90:83-96 */
+static const sbp_t SBP2_OP = {
+    2, 2, 1, 1, 2.0,
+    {-0.5, 0.0, 0.5}, {0.5, -1.0, 0.5},
+    {{-1.0, 1.0}}, {{-1.0, 1.0}}, {0.5}};
 
-/* artificial dissipation AI(-2..2) and AL4[r][s] = AL(s,r),
-   fd_coeff.f90:103-124 */
-static const double AI4[5] = {-0.083333333333333, 0.333333333333333, -0.500000000000000,
-                              0.333333333333333, -0.083333333333333};
-static const double AL4[4][6] = {
-    {-0.470588235294118, 0.941176470588235, -0.470588235294118, 0.000000000000000,
-     0.000000000000000, 0.000000000000000},
-    {0.271186440677966, -0.610169491525424, 0.406779661016949, -0.067796610169492,
-     0.000000000000000, 0.000000000000000},
-    {-0.186046511627907, 0.558139534883721, -0.651162790697674, 0.372093023255814,
-     -0.093023255813953, 0.000000000000000},
-    {0.000000000000000, -0.081632653061224, 0.326530612244898, -0.489795918367347,
-     0.326530612244898, -0.081632653061224}};
+/* 'SBP4', fd_coeff.f90:97-126 */
+static const sbp_t SBP4_OP = {
+    4, 6, 4, 2, 2.823529411764706,
+    {0.083333333333333, -0.666666666666667, 0.000000000000000, 0.666666666666667,
+     -0.083333333333333},
+    {-0.083333333333333, 0.333333333333333, -0.500000000000000, 0.333333333333333,
+     -0.083333333333333},
+    {{-1.411764705882353, 1.735294117647059, -0.235294117647059, -0.088235294117647,
+      0.000000000000000, 0.000000000000000},
+     {-0.500000000000000, 0.000000000000000, 0.500000000000000, 0.000000000000000,
+      0.000000000000000, 0.000000000000000},
+     {0.093023255813953, -0.686046511627907, 0.000000000000000, 0.686046511627907,
+      -0.093023255813953, 0.000000000000000},
+     {0.030612244897959, 0.000000000000000, -0.602040816326531, 0.000000000000000,
+      0.653061224489796, -0.081632653061224}},
+    {{-0.470588235294118, 0.941176470588235, -0.470588235294118, 0.000000000000000,
+      0.000000000000000, 0.000000000000000},
+     {0.271186440677966, -0.610169491525424, 0.406779661016949, -0.067796610169492,
+      0.000000000000000, 0.000000000000000},
+     {-0.186046511627907, 0.558139534883721, -0.651162790697674, 0.372093023255814,
+      -0.093023255813953, 0.000000000000000},
+     {0.000000000000000, -0.081632653061224, 0.326530612244898, -0.489795918367347,
+      0.326530612244898, -0.081632653061224}},
+    {0.354166666666667, 1.229166666666667, 0.895833333333333, 1.020833333333333}};
 
-static inline double DIc(int s) { return DI4[2 + s]; }
-static inline double AIc(int s) { return AI4[2 + s]; }
-/* AL(s,r); AR(-s,-r) = AL(s,r) (fd_coeff.f90:242-246, no sign change) */
-static inline double ALc(int s, int r) { return AL4[r][s]; }
-/* DL(s,r) */
-static inline double DLc(int s, int r) { return DL4[r][s]; }
-/* DR(-s,-r) = -DL(s,r)  (fd_coeff.f90:242-246) */
-static inline double DRc(int s, int r) { return -DL4[r][s]; }
+/* 'SBP6', fd_coeff.f90:128-173 */
+static const sbp_t SBP6_OP = {
+    6, 9, 6, 3, 3.165067037878233,
+    {-0.016666666666667, 0.150000000000000, -0.750000000000000, 0.000000000000000,
+     0.750000000000000, -0.150000000000000, 0.016666666666667},
+    {0.016666666666667, -0.100000000000000, 0.250000000000000, -0.333333333333333,
+     0.250000000000000, -0.100000000000000, 0.016666666666667},
+    {{-1.582533518939116, 2.033378678700676, -0.141512858744873, -0.450398306578272,
+      0.104488069284042, 0.036577936277544, 0.000000000000000, 0.000000000000000,
+      0.000000000000000},
+     {-0.462059195631158, 0.000000000000000, 0.287258622978251, 0.258816087376832,
+      -0.069112065532624, -0.014903449191300, 0.000000000000000, 0.000000000000000,
+      0.000000000000000},
+     {0.071247104721830, -0.636451095137907, 0.000000000000000, 0.606235523609147,
+      -0.022902190275815, -0.018129342917256, 0.000000000000000, 0.000000000000000,
+      0.000000000000000},
+     {0.114713313798970, -0.290087484386815, -0.306681191361148, 0.000000000000000,
+      0.520262285050482, -0.051642265516119, 0.013435342414630, 0.000000000000000,
+      0.000000000000000},
+     {-0.036210680656541, 0.105400944933782, 0.015764336127392, -0.707905442575989,
+      0.000000000000000, 0.769199413962647, -0.164529643265203, 0.018281071473911,
+      0.000000000000000},
+     {-0.011398193015050, 0.020437334208704, 0.011220896474665, 0.063183694641876,
+      -0.691649024426814, 0.000000000000000, 0.739709139060752, -0.147941827812150,
+      0.016437980868017}},
+    {{-0.105502234595941, 0.316506703787823, -0.316506703787823, 0.105502234595941,
+      0.000000000000000, 0.000000000000000, 0.000000000000000, 0.000000000000000,
+      0.000000000000000},
+     {0.071922084408557, -0.227753267293765, 0.251727295429951, -0.107883126612836,
+      0.011987014068093, 0.000000000000000, 0.000000000000000, 0.000000000000000,
+      0.000000000000000},
+     {-0.159350793065290, 0.557727775728513, -0.743637034304685, 0.478052379195869,
+      -0.159350793065290, 0.026558465510882, 0.000000000000000, 0.000000000000000,
+      0.000000000000000},
+     {0.026870684829259, -0.120918081731666, 0.241836163463333, -0.282142190707221,
+      0.201530136219444, -0.080612054487778, 0.013435342414630, 0.000000000000000,
+      0.000000000000000},
+     {0.000000000000000, 0.018281071473911, -0.109686428843468, 0.274216072108671,
+      -0.365621429478228, 0.274216072108671, -0.109686428843468, 0.018281071473911,
+      0.000000000000000},
+     {0.000000000000000, 0.000000000000000, 0.016437980868017, -0.098627885208100,
+      0.246569713020251, -0.328759617360334, 0.246569713020251, -0.098627885208100,
+      0.016437980868017}},
+    {0.315949074074074, 1.390393518518518, 0.627546296296296, 1.240509259259259,
+     0.911689814814815, 1.013912037037037}};
 
-void orc_sbp4_tables(double* DI, double* DL, double* HL, double* H00i) {
-  for (int k = 0; k < 5; k++) DI[k] = DI4[k];
-  for (int r = 0; r < 4; r++)
-    for (int s = 0; s < 6; s++) DL[r * 6 + s] = DL4[r][s];
-  for (int r = 0; r < 4; r++) HL[r] = HL4[r];
-  *H00i = H00I4;
+static const sbp_t* sbp_of(int order) {
+  return order == 2 ? &SBP2_OP : order == 6 ? &SBP6_OP : &SBP4_OP;
 }
 
-/* fd.f90:42-80, endm = endp = true, unit spacing */
+/* The operator of the context being stepped.  Every entry point that
+   evaluates rates sets it from its context (use_op) before any rate is
+   formed; the oracle steps one context at a time. */
+static const sbp_t* g_op = &SBP4_OP;
+
+#define NBND (g_op->nbnd)
+#define NBST (g_op->nbst)
+#define PI_I (g_op->pI)
+
+static inline double DIc(int s) { return g_op->DI[g_op->pI + s]; }
+static inline double AIc(int s) { return g_op->AI[g_op->pI + s]; }
+/* AL(s,r); AR(-s,-r) = AL(s,r) (fd_coeff.f90:242-246, no sign change) */
+static inline double ALc(int s, int r) { return g_op->AL[r][s]; }
+/* DL(s,r) */
+static inline double DLc(int s, int r) { return g_op->DL[r][s]; }
+/* DR(-s,-r) = -DL(s,r)  (fd_coeff.f90:242-246) */
+static inline double DRc(int s, int r) { return -g_op->DL[r][s]; }
+
+/* the tables of one method: DI(-pI..pI), DL[r][s] (nbst x nbnd), HL(0..nbst-1),
+   H00i; returns nbnd*100 + nbst*10 + pI, 0 for an unknown order */
+int32_t orc_sbp_tables(int32_t order, double* DI, double* DL, double* HL, double* H00i) {
+  if (order != 2 && order != 4 && order != 6) return 0;
+  const sbp_t* op = sbp_of(order);
+  for (int k = 0; k < 2 * op->pI + 1; k++) DI[k] = op->DI[k];
+  for (int r = 0; r < op->nbst; r++)
+    for (int s = 0; s < op->nbnd; s++) DL[r * op->nbnd + s] = op->DL[r][s];
+  for (int r = 0; r < op->nbst; r++) HL[r] = op->HL[r];
+  *H00i = op->H00i;
+  return op->nbnd * 100 + op->nbst * 10 + op->pI;
+}
+
+void orc_sbp4_tables(double* DI, double* DL, double* HL, double* H00i) {
+  (void)orc_sbp_tables(4, DI, DL, HL, H00i);
+}
+
+/* fd.f90:42-80, endm = endp = true, unit spacing, with the operator of
+   the given order */
+void orc_diff_op(int32_t order, int32_t n, const double* f, double* Df) {
+  const sbp_t* saved = g_op;
+  g_op = sbp_of(order);
+  orc_diff(n, f, Df);
+  g_op = saved;
+}
+
+/* fd.f90:42-80, endm = endp = true, unit spacing (the current operator,
+   SBP4 outside a step) */
 void orc_diff(int32_t n, const double* f, double* Df) {
   if (n == 1) {
     Df[0] = 0.0;
@@ -101,7 +185,7 @@ void orc_diff(int32_t n, const double* f, double* Df) {
   }
   for (int i = NBST; i <= n - 1 - NBST; i++) {
     double s = 0.0;
-    for (int k = -2; k <= 2; k++) s = s + DIc(k) * f[i + k];
+    for (int k = -PI_I; k <= PI_I; k++) s = s + DIc(k) * f[i + k];
     Df[i] = s;
   }
 }
@@ -122,7 +206,7 @@ void orc_diss1d(int32_t n, const double* f, double* Af) {
   }
   for (int i = NBST; i <= n - 1 - NBST; i++) {
     double s = 0.0;
-    for (int k = -2; k <= 2; k++) s = s + AIc(k) * f[i + k];
+    for (int k = -PI_I; k <= PI_I; k++) s = s + AIc(k) * f[i + k];
     Af[i] = s;
   }
 }
@@ -377,6 +461,7 @@ typedef struct {
 } oiface_t;
 
 struct orc_ctx {
+  const sbp_t* op; /* first-derivative operator (FDmethod) */
   int32_t nx, ny, nb, ni, nstage;
   int32_t mode, nF; /* 2: plane strain, 6 fields; 3: antiplane shear, 3 fields */
   size_t np;
@@ -446,6 +531,21 @@ int orc_create(const fdm_problem* P, orc_ctx** out) {
   c->nstage = P->nstage;
   c->mode = P->mode == 3 ? 3 : 2;
   c->nF = c->mode == 3 ? 3 : 6;
+  if (P->sbp != 0 && P->sbp != 2 && P->sbp != 4 && P->sbp != 6) {
+    snprintf(g_err, sizeof g_err, "orc_create: unknown operator SBP%d", (int)P->sbp);
+    free(c);
+    return FDM_EINVAL;
+  }
+  c->op = sbp_of(P->sbp == 0 ? 4 : P->sbp);
+  for (int b = 0; b < P->nblocks; b++) {
+    const fdm_block* B = &P->blocks[b];
+    if (B->px - B->mx + 1 < c->op->nbnd || B->py - B->my + 1 < c->op->nbnd) {
+      snprintf(g_err, sizeof g_err, "orc_create: block %d is narrower than the operator's %d-point boundary stencil",
+               b, c->op->nbnd);
+      free(c);
+      return FDM_EINVAL;
+    }
+  }
   for (int s = 0; s < P->nstage; s++) {
     c->rkA[s] = P->rkA[s];
     c->rkB[s] = P->rkB[s];
@@ -1638,7 +1738,7 @@ static void rates_dissipation_interior(const sten_t* S, int mx, int px, int my, 
       double rho, G, M, g;
       mat_at(S, cc, &rho, &G, &M, &g);
       for (int l = 0; l < 5; l++) S->DF[l][cc] = S->DF[l][cc] + d[0] * S->F[l][cc];
-      for (int s = 1; s <= 3; s++)
+      for (int s = 1; s <= PI_I; s++)
         for (int l = 0; l < 5; l++) {
           size_t ip = cc + s, im = cc - s, jp = cc + (size_t)s * nx, jm = cc - (size_t)s * nx;
           double ayp = 0.5 * (S->yr[ip] + S->yr[cc]), axp = 0.5 * (S->xr[ip] + S->xr[cc]);
@@ -1681,7 +1781,7 @@ static void rates_interior(const sten_t* S, int mx, int px, int my, int py) {
       size_t cc = (size_t)j * nx + i;
       double rho, G, M, g;
       mat_at(S, cc, &rho, &G, &M, &g);
-      for (int s = 1; s <= 3; s++)
+      for (int s = 1; s <= PI_I; s++)
         for (int l = 0; l < 5; l++) {
           size_t ip = cc + s, im = cc - s, jp = cc + (size_t)s * nx, jm = cc - (size_t)s * nx;
           double ayp = 0.5 * (S->yr[ip] + S->yr[cc]), axp = 0.5 * (S->xr[ip] + S->xr[cc]);
@@ -1871,8 +1971,8 @@ static inline double bscale3(int l, double t, double D, double rho, double G) {
 /* rates_interior_mode3 (rates.f90:8-56): a(s) = DI(s)/rho, b(s) = DI(s)*G;
    the s = 3 coefficient is zero */
 static void rates_interior3(const sten_t* S, int mx, int px, int my, int py) {
-  double coef[3][3];
-  for (int s = 1; s <= 2; s++) {
+  double coef[3][4];
+  for (int s = 1; s <= PI_I; s++) {
     coef[0][s] = DIc(s) / S->rho;
     coef[1][s] = DIc(s) * S->G;
     coef[2][s] = DIc(s) * S->G;
@@ -1882,7 +1982,7 @@ static void rates_interior3(const sten_t* S, int mx, int px, int my, int py) {
   for (int j = my; j <= py; j++)
     for (int i = mx; i <= px; i++) {
       size_t cc = (size_t)j * nx + i;
-      for (int s = 1; s <= 2; s++) {
+      for (int s = 1; s <= PI_I; s++) {
         size_t ip = cc + s, im = cc - s, jp = cc + (size_t)s * nx, jm = cc - (size_t)s * nx;
         double ayp = 0.5 * (S->yr[ip] + S->yr[cc]), axp = 0.5 * (S->xr[ip] + S->xr[cc]);
         double aym = 0.5 * (S->yr[im] + S->yr[cc]), axm = 0.5 * (S->xr[im] + S->xr[cc]);
@@ -2016,8 +2116,8 @@ static void rates_pml(orc_ctx* c, const fdm_block* b, int mx, int px, int my, in
   p.pmly = b->pmly;
   const double kappa = b->pml[2];
   /* interior coefficients a, b, c (rates.f90:2053-2058) */
-  double ca[3], cb[3], cc[3];
-  for (int s = 1; s <= 2; s++) {
+  double ca[4], cb[4], cc[4];
+  for (int s = 1; s <= PI_I; s++) {
     ca[s] = DIc(s) / p.rho;
     cb[s] = DIc(s) * p.G;
     cc[s] = DIc(s) * p.M;
@@ -2053,8 +2153,8 @@ static void rates_pml(orc_ctx* c, const fdm_block* b, int mx, int px, int my, in
       }
       if (qside == 0 && rside == 0) {
         /* interior: x and y differences in one sum (rates.f90:2088-2124);
-           the s = 3 terms carry DI(3) = 0 for SBP4 */
-        for (int s = 1; s <= 2; s++) {
+           terms with a zero coefficient (s > pI) are left out */
+        for (int s = 1; s <= PI_I; s++) {
           double dx[6], dy[6];
           for (int l = 0; l < 6; l++) {
             dx[l] = PF(l, IDX(i + s, j)) - PF(l, IDX(i - s, j));
@@ -2072,23 +2172,23 @@ static void rates_pml(orc_ctx* c, const fdm_block* b, int mx, int px, int my, in
         }
       } else if (rside == 0) {
         /* L or R strip: one-sided x, then central y (rates.f90:2165-2218) */
-        for (int s = 0; s < 6; s++) {
+        for (int s = 0; s < NBND; s++) {
           const size_t k = qside == 1 ? IDX(mgx + s, j) : IDX(pgx - s, j);
           const double D = qside == 1 ? DLc(s, i - mgx) : DRc(s, pgx - i);
           for (int l = 0; l < 6; l++) v[l] = PF(l, k);
           pml_xterm(&p, v, D, d, dwx);
         }
-        for (int s = 1; s <= 2; s++) {
+        for (int s = 1; s <= PI_I; s++) {
           for (int l = 0; l < 6; l++) v[l] = PF(l, IDX(i, j + s)) - PF(l, IDX(i, j - s));
           pml_yterm(&p, v, DIc(s), d, dwy);
         }
       } else if (qside == 0) {
         /* B or T strip: central x, then one-sided y (rates.f90:2287-2335) */
-        for (int s = 1; s <= 2; s++) {
+        for (int s = 1; s <= PI_I; s++) {
           for (int l = 0; l < 6; l++) v[l] = PF(l, IDX(i + s, j)) - PF(l, IDX(i - s, j));
           pml_xterm(&p, v, DIc(s), d, dwx);
         }
-        for (int s = 0; s < 6; s++) {
+        for (int s = 0; s < NBND; s++) {
           const size_t k = rside == 1 ? IDX(i, mgy + s) : IDX(i, pgy - s);
           const double D = rside == 1 ? DLc(s, j - mgy) : DRc(s, pgy - j);
           for (int l = 0; l < 6; l++) v[l] = PF(l, k);
@@ -2096,7 +2196,7 @@ static void rates_pml(orc_ctx* c, const fdm_block* b, int mx, int px, int my, in
         }
       } else {
         /* corner: one-sided x and y, interleaved per s (rates.f90:2399-2441) */
-        for (int s = 0; s < 6; s++) {
+        for (int s = 0; s < NBND; s++) {
           const size_t kx = qside == 1 ? IDX(mgx + s, j) : IDX(pgx - s, j);
           const double Dx = qside == 1 ? DLc(s, i - mgx) : DRc(s, pgx - i);
           for (int l = 0; l < 6; l++) v[l] = PF(l, kx);
@@ -2343,6 +2443,7 @@ static void snapshot_stage1(orc_ctx* c) {
 
 /* time_step_LS (time_step.f90:71-167) */
 static int time_step_LS(orc_ctx* c, double dt, int final_step) {
+  g_op = c->op; /* this context's operator for every rate of the step */
   double t0 = c->t;
   int fail = 0;
   for (int stage = 1; stage <= c->nstage; stage++) {

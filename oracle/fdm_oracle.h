@@ -45,9 +45,16 @@ int orc_set_threads(int32_t nthreads);
    DL[4][6] = DL(0..5, r) row r, HL[4], H00i. */
 void orc_sbp4_tables(double* DI, double* DL, double* HL, double* H00i);
 
+/* Tables of 'SBP2', 'SBP4' or 'SBP6' (order 2, 4, 6; fd_coeff.f90:79-246):
+   DI[2 pI + 1] = DI(-pI..pI), DL[nbst][nbnd] = DL(0..nbnd-1, r) row r,
+   HL[nbst], H00i.  Returns nbnd*100 + nbst*10 + pI (0: unknown order). */
+int32_t orc_sbp_tables(int32_t order, double* DI, double* DL, double* HL, double* H00i);
+
 /* 1-D SBP4 first derivative with unit spacing over n points (fd.f90:42-80
-   with both ends inside the range). */
+   with both ends inside the range); orc_diff_op with the operator of the
+   given order. */
 void orc_diff(int32_t n, const double* f, double* Df);
+void orc_diff_op(int32_t order, int32_t n, const double* f, double* Df);
 void orc_diss1d(int32_t n, const double* f, double* Af);
 
 /* LSRK coefficients (time_step.f90:25-66): returns nstage, fills A, B,

@@ -23,8 +23,12 @@ DL = np.array([
 ])
 
 
-def sbp_diff(f):
-    """fd.f90:42-80 along the last axis (both ends inside the range)."""
+def sbp_diff(f, order=4):
+    """fd.f90:42-80 along the last axis (both ends inside the range), with
+    the operator 'SBP<order>' (the SBP2 and SBP6 tables are the oracle's,
+    pinned in test_oracle.py)."""
+    if order != 4:
+        return _sbp_diff_tables(f, order)
     f = np.asarray(f, dtype=np.float64)
     n = f.shape[-1]
     out = np.empty_like(f)
@@ -47,6 +51,39 @@ def sbp_diff(f):
     return out
 
 
+def _sbp_diff_tables(f, order):
+    from oracle import binding as ob
+
+    DIo, DLo, _, _ = ob.sbp_tables(order)
+    nbst, nbnd = DLo.shape
+    pI = (len(DIo) - 1) // 2
+    f = np.asarray(f, dtype=np.float64)
+    n = f.shape[-1]
+    out = np.empty_like(f)
+    for i in range(nbst):
+        s = np.zeros(f.shape[:-1])
+        for k in range(nbnd):
+            s = s + DLo[i, k] * f[..., k]
+        out[..., i] = s
+    for i in range(nbst):
+        s = np.zeros(f.shape[:-1])
+        for k in range(nbnd):
+            sidx = nbnd - 1 - k
+            s = s + (-DLo[i, sidx]) * f[..., n - 1 - sidx]
+        out[..., n - 1 - i] = s
+    for i in range(nbst, n - nbst):
+        s = np.zeros(f.shape[:-1])
+        for k in range(-pI, pI + 1):
+            s = s + DIo[pI + k] * f[..., i + k]
+        out[..., i] = s
+    return out
+
+
+def h00i(order):
+    """SAT penalty weight H00i of 'SBP<order>' (fd_coeff.f90:56-70)"""
+    return {2: 2.0, 4: H00I, 6: 3.165067037878233}[order]
+
+
 def rs_psi(a, V0, S, N, V):
     """State Psi at which the regularised law gives slip velocity V at shear
     stress S (inverse of strength, friction.f90:1066-1070)."""
@@ -58,7 +95,7 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
           perturb=1e-3, source=False, plastic=None, nbx=1, rk_order=4, outside_frac=0.0,
           fault_overstress=25.0, ld_T=0.0, reg=False, rup_field="V", Vinit=1e-12,
           skempton=0.0, mms=False, medium=None, Cdiss=None, mode=2,
-          inplane=(-120.0, 0.0, -120.0, -120.0), force=None, pml=None):
+          inplane=(-120.0, 0.0, -120.0, -120.0), force=None, pml=None, sbp=4):
     """Blocks stacked in y (len(nys) rows) and nbx columns; interfaces
     between them.  medium: amplitude of a smooth seeded heterogeneous medium
     (rho, cs, cp varying by that relative amount around the given values;
@@ -71,7 +108,8 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
     xR, tmax (distance along the fault: x for horizontal faults, y for
     vertical ones).  pml: a perfectly matched layer for every block (a dict
     of x, y, n, sigma, kappa, mx, px, my, py), or one dict or None per block.
-    Returns (Problem, F0 fields [6,ny,nx])."""
+    sbp: first-derivative operator 'SBP<sbp>' (2, 4, 6) for the metrics, the
+    SAT weights and the rates.  Returns (Problem, F0 fields [6,ny,nx])."""
     rng = np.random.default_rng(seed)
     nyb = list(nys)
     ny = sum(nyb)
@@ -102,7 +140,7 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
         X2 = X + warp * Lx * 0.05 * np.sin(2 * np.pi * Y / Lx + ph[0]) * np.cos(np.pi * X / Lx + ph[1])
         Y2 = Y + warp * Lx * 0.05 * np.sin(2 * np.pi * X / Lx + ph[2]) * np.cos(np.pi * Y / Lx + ph[3])
         X, Y = X2, Y2
-    prob = Problem(NX, ny, rk_order=rk_order, precision=precision, mode=mode)
+    prob = Problem(NX, ny, rk_order=rk_order, precision=precision, mode=mode, sbp=sbp)
     xq = np.empty_like(X)
     xr = np.empty_like(X)
     yq = np.empty_like(X)
@@ -111,10 +149,10 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
         for by in range(len(nyb)):
             sj = slice(ym[by], ym[by] + nyb[by])
             si = slice(xm[bx], xm[bx] + nxb[bx])
-            xq[sj, si] = sbp_diff(X[sj, si])
-            yq[sj, si] = sbp_diff(Y[sj, si])
-            xr[sj, si] = sbp_diff(X[sj, si].T).T
-            yr[sj, si] = sbp_diff(Y[sj, si].T).T
+            xq[sj, si] = sbp_diff(X[sj, si], sbp)
+            yq[sj, si] = sbp_diff(Y[sj, si], sbp)
+            xr[sj, si] = sbp_diff(X[sj, si].T, sbp).T
+            yr[sj, si] = sbp_diff(Y[sj, si].T, sbp).T
     J = xq * yr - xr * yq
     prob.xq, prob.xr, prob.yq, prob.yr = xq, xr, yq, yr
     prob.Ji = 1.0 / J
@@ -167,8 +205,8 @@ def build(nx=48, nys=(24, 24), seed=0, warp=0.02, h=0.025, cs=3.464, cp=6.0, rho
                 M[0] = r_ * s_
                 M[1] = r_ * p_
                 M[2] = 1.0 - 2.0 * (s_ / p_) ** 2
-                M[3] = H00I * s_ * nrm / Jv
-                M[4] = H00I * p_ * nrm / Jv
+                M[3] = h00i(sbp) * s_ * nrm / Jv
+                M[4] = h00i(sbp) * p_ * nrm / Jv
                 F0 = np.tile(bg[:, None], (1, n))
                 outer = ((s == 0 and bx == 0) or (s == 1 and bx == nbx - 1) or
                          (s == 2 and by == 0) or (s == 3 and by == len(nyb) - 1))

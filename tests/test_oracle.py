@@ -96,6 +96,54 @@ def test_sbp_identity_with_tables():
     assert np.abs(Q + Q.T - B).max() < 2e-14
 
 
+# diagonal norms of the published operators: SBP2 (1/2 at the ends), SBP6
+# (Strand 1994; Mattsson and Nordstrom 2004, sixth-order interior with
+# third-order boundary closure)
+HL_EXACT_OP = {2: [Fr(1, 2)],
+               4: HL_EXACT,
+               6: [Fr(13649, 43200), Fr(12013, 8640), Fr(2711, 4320), Fr(5359, 4320),
+                   Fr(7877, 8640), Fr(43801, 43200)]}
+# (interior order, closure order) of each operator
+ORDERS = {2: (2, 1), 4: (4, 2), 6: (6, 3)}
+
+
+@pytest.mark.parametrize("order", [2, 4, 6])
+def test_sbp_tables_are_summation_by_parts(order):
+    """'SBP2', 'SBP4', 'SBP6' (fd_coeff.f90:79-246): the norm is the
+    published one, H00i = 1/H(0,0), the dense operator satisfies
+    H D + (H D)^T = diag(-1, 0, ..., 0, 1) (summation by parts), and it
+    differentiates polynomials exactly through the interior order inside and
+    the closure order in the boundary rows (the tables are 15-digit
+    decimals, hence the tolerances)"""
+    DIo, DLo, HL, H00i = ob.sbp_tables(order)
+    nbst, nbnd = DLo.shape
+    np.testing.assert_allclose(HL, [float(v) for v in HL_EXACT_OP[order]], rtol=0, atol=5e-15)
+    assert abs(H00i - 1.0 / float(HL_EXACT_OP[order][0])) < 5e-15
+    n = 31
+    D = dense_D_op(n, order)
+    h = norm_weights(n, order)
+    Q = np.diag(h) @ D
+    B = np.zeros((n, n))
+    B[0, 0], B[-1, -1] = -1.0, 1.0
+    assert np.abs(Q + Q.T - B).max() < 5e-14
+    pin, pbnd = ORDERS[order]
+    x = np.arange(n, dtype=float) - 7.0
+    for p in range(pin + 1):
+        d = D @ x ** p
+        want = p * x ** (p - 1) if p else np.zeros(n)
+        scale = max(1.0, np.abs(x).max() ** max(p - 1, 0))
+        inner = slice(nbst, n - nbst)
+        assert np.abs(d[inner] - want[inner]).max() < 1e-12 * scale, ("interior", p)
+        if p <= pbnd:
+            assert np.abs(d - want).max() < 1e-12 * scale, ("closure", p)
+    # the closure rows are not exact one degree higher (the order is sharp)
+    d = D @ x ** (pbnd + 1)
+    assert np.abs(d[:nbst] - (pbnd + 1) * x[:nbst] ** pbnd).max() > 1e-6
+    # diff (fd.f90:42-80) applies exactly this matrix
+    f = np.random.default_rng(order).standard_normal(n)
+    np.testing.assert_allclose(ob.diff(f, order), D @ f, rtol=0, atol=1e-13)
+
+
 @pytest.mark.parametrize("n", [9, 12, 31])
 def test_diff_matches_dense_operator(n):
     rng = np.random.default_rng(n)
@@ -323,9 +371,35 @@ def test_mms_sources_are_residuals_of_the_fields(side):
 # ---------------------------------------------------------------------------
 
 
-def split_ops(prob):
-    Dq = dense_D(prob.nx)
-    Dr = dense_D(prob.ny)
+def dense_D_op(n, order):
+    """dense derivative matrix of 'SBP<order>' from the oracle's tables
+    (pinned by test_sbp_tables_are_summation_by_parts)"""
+    if order == 4:
+        return dense_D(n)
+    DIo, DLo, _, _ = ob.sbp_tables(order)
+    nbst, nbnd = DLo.shape
+    pI = (len(DIo) - 1) // 2
+    D = np.zeros((n, n))
+    for i in range(nbst):
+        D[i, :nbnd] = DLo[i]
+        D[n - 1 - i, n - nbnd:] = -DLo[i][::-1]
+    for i in range(nbst, n - nbst):
+        D[i, i - pI:i + pI + 1] = DIo
+    return D
+
+
+def norm_weights(n, order=4):
+    """diagonal of the SBP norm H (unit spacing) of 'SBP<order>'"""
+    _, _, HL, _ = ob.sbp_tables(order)
+    h = np.ones(n)
+    h[:len(HL)] = HL
+    h[n - len(HL):] = HL[::-1]
+    return h
+
+
+def split_ops(prob, order=4):
+    Dq = dense_D_op(prob.nx, order)
+    Dr = dense_D_op(prob.ny, order)
     dq = lambda f: f @ Dq.T
     dr = lambda f: Dr @ f
     sq = lambda a, f: 0.5 * (dq(a * f) + a * dq(f))
@@ -366,6 +440,55 @@ def test_stage1_rates_match_dense_split_operator(seed, warp):
     scale = np.abs(ref[inner]).max(axis=(1, 2), keepdims=True)
     err = np.abs(DF[inner] - ref[inner]) / scale
     assert err.max() < 1e-12, err.max()
+
+
+@pytest.mark.parametrize("order,warp", [(2, 0.08), (6, 0.0), (6, 0.08)])
+def test_stage1_rates_other_operators_match_dense_split_operator(order, warp):
+    """the same split-form check with 'SBP2' and 'SBP6': interior stencils
+    of half-width 1 and 3, closures of 1 x 2 and 6 x 9 points and their
+    corners (rates.f90:384-917 with the operator's tables), the metrics
+    from the same operator"""
+    cs, cp, rho = 3.464, 6.0, 2.67
+    n = 21 if order == 6 else 13
+    prob, F = problems.build(nx=n, nys=(n + 2,), seed=order, warp=warp, cs=cs, cp=cp, rho=rho,
+                             background=[0.1, -0.2, -120.0, 70.0, -110.0, -100.0], perturb=0.5,
+                             bcs=("absorbing", "free", "absorbing", "free"), sbp=order)
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.stage1(0.0, 1e-3, 1)
+    _, DF = o.get_fields()
+    G, M = rho * cs ** 2, rho * cp ** 2
+    lam = M - 2 * G
+    gamma, nu = lam / M, lam / (2 * (lam + G))
+    dx, dy = split_ops(prob, order)
+    vx, vy, sxx, sxy, syy = F[:5]
+    ref = np.empty_like(DF)
+    ref[0] = (dx(sxx) + dy(sxy)) / rho
+    ref[1] = (dx(sxy) + dy(syy)) / rho
+    ref[2] = M * (dx(vx) + gamma * dy(vy))
+    ref[3] = G * (dx(vy) + dy(vx))
+    ref[4] = M * (gamma * dx(vx) + dy(vy))
+    ref[5] = nu * (ref[2] + ref[4])
+    inner = (slice(None), slice(1, -1), slice(1, -1))
+    scale = np.abs(ref[inner]).max(axis=(1, 2), keepdims=True)
+    err = np.abs(DF[inner] - ref[inner]) / scale
+    assert err.max() < 1e-12, err.max()
+    # and mode 3 with the same operator
+    prob3, F3 = problems.build(nx=n, nys=(n + 2,), seed=order + 1, warp=warp, cs=cs, cp=cp,
+                               rho=rho, background=[0.1, 30.0, -5.0], perturb=0.5, mode=3,
+                               bcs=("absorbing", "free", "absorbing", "free"), sbp=order)
+    o3 = OracleContext(prob3)
+    o3.set_fields(F3)
+    o3.init_edges(0.0)
+    o3.stage1(0.0, 1e-3, 1)
+    _, DF3 = o3.get_fields()
+    dx3, dy3 = split_ops(prob3, order)
+    vz, sxz, syz = F3
+    ref3 = np.array([(dx3(sxz) + dy3(syz)) / rho, G * dx3(vz), G * dy3(vz)])
+    inner3 = (slice(None), slice(1, -1), slice(1, -1))
+    scale3 = np.abs(ref3[inner3]).max(axis=(1, 2), keepdims=True)
+    assert (np.abs(DF3[inner3] - ref3[inner3]) / scale3).max() < 1e-12
 
 
 @pytest.mark.parametrize("seed,warp", [(5, 0.0), (6, 0.08)])
@@ -486,15 +609,8 @@ def test_stage1_dissipation_matches_dense_operator(medium):
     assert err.max() < 1e-12, err.max()
 
 
-def energy(prob, F, rho, cs, cp):
-    _, _, HL, _ = ob.sbp4_tables()
-
-    def hw(n):
-        h = np.ones(n)
-        h[:4] = HL
-        h[n - 4:] = HL[::-1]
-        return h
-    W = np.outer(hw(prob.ny), hw(prob.nx)) / prob.Ji
+def energy(prob, F, rho, cs, cp, order=4):
+    W = np.outer(norm_weights(prob.ny, order), norm_weights(prob.nx, order)) / prob.Ji
     G = rho * cs ** 2
     lam = rho * cp ** 2 - 2 * G
     nu = lam / (2 * (lam + G))
@@ -504,19 +620,22 @@ def energy(prob, F, rho, cs, cp):
     return float((W * e).sum())
 
 
-@pytest.mark.parametrize("bcs", [("absorbing",) * 4, ("absorbing", "free", "absorbing", "free")])
-def test_energy_decays_with_absorbing_boundaries(bcs):
+@pytest.mark.parametrize("bcs,order", [(("absorbing",) * 4, 4),
+                                       (("absorbing", "free", "absorbing", "free"), 4),
+                                       (("absorbing", "free", "absorbing", "free"), 6),
+                                       (("absorbing", "free", "absorbing", "free"), 2)])
+def test_energy_decays_with_absorbing_boundaries(bcs, order):
     cs, cp, rho, h = 3.464, 6.0, 2.67, 0.025
     prob, F = problems.build(nx=31, nys=(27,), seed=4, warp=0.06, h=h, cs=cs, cp=cp, rho=rho,
-                             background=[0.0] * 6, perturb=1.0, bcs=bcs)
+                             background=[0.0] * 6, perturb=1.0, bcs=bcs, sbp=order)
     dt = 0.5 * h / np.hypot(cs, cp)
     o = OracleContext(prob)
     o.set_fields(F)
     o.init_edges(0.0)
-    E = [energy(prob, F, rho, cs, cp)]
+    E = [energy(prob, F, rho, cs, cp, order)]
     for n in range(1, 61):
         o.step(0.0, dt, n, 1)
-        E.append(energy(prob, o.get_fields()[0], rho, cs, cp))
+        E.append(energy(prob, o.get_fields()[0], rho, cs, cp, order))
     E = np.array(E)
     assert (np.diff(E) <= 1e-12 * E[0]).all(), np.diff(E).max() / E[0]
     assert E[-1] < 0.95 * E[0]
@@ -653,12 +772,12 @@ def test_kinematic_forcing_moves_the_rupture_tips(mode):
     np.testing.assert_allclose(S[inside], 0.525 * N[inside], rtol=1e-14)
 
 
-def mms_error(n, tend=0.04, warp=0.05):
+def mms_error(n, tend=0.04, warp=0.05, order=4):
     G, cs, cp = 32.0, 3.0, 5.0
     rho = G / cs ** 2
     h = 1.0 / (n - 1)
     prob, F = problems.build(nx=n, nys=(n,), seed=8, warp=warp, h=h, cs=cs, cp=cp, rho=rho,
-                             mms=True, bcs=("mms-sin-w",) * 4)
+                             mms=True, bcs=("mms-sin-w",) * 4, sbp=order)
     nsteps = int(np.ceil(tend / (0.5 * h / np.hypot(cs, cp))))
     dt = tend / nsteps
     o = OracleContext(prob, threads=4)
@@ -674,11 +793,15 @@ def mms_error(n, tend=0.04, warp=0.05):
     return np.sqrt((err ** 2).sum(axis=(1, 2)) * h * h).sum()
 
 
-def test_mms_convergence_rate():
-    e = [mms_error(n) for n in (21, 41, 81)]
+@pytest.mark.parametrize("order,rate", [(4, 2.7), (6, 3.5), (2, 1.7)])
+def test_mms_convergence_rate(order, rate):
+    """the manufactured solution converges at the operator's global rate:
+    about 3 for SBP4 (second-order closures), 4 for SBP6 (third-order
+    closures), 2 for SBP2 (first-order closures)"""
+    e = [mms_error(n, order=order) for n in (21, 41, 81)]
     rates = np.log2(np.array(e[:-1]) / np.array(e[1:]))
-    print("MMS errors", e, "rates", rates)
-    assert rates[-1] > 2.7, rates
+    print("MMS errors", order, e, "rates", rates)
+    assert rates[-1] > rate, rates
 
 
 def to_nt(F, n1, n2):
