@@ -7,9 +7,15 @@ velocity weakening), LSRK(5,4) at CFL 0.5, built from its .in file by the
 Fortran host.  A step is one full LSRK(5,4) step of the whole grid.
 
 Prints one JSON line: FP64 updates/s (value), the roofline of the bulk
-kernel measured with HIP events inside the timed region, the FP32-bulk
-throughput and its error against FP64, and the CPU oracle timed on the
-host cores as the CPU baseline.
+kernel measured with HIP events after the timed region, the FP32-bulk
+throughput and its error against FP64 after the configuration's 4000 steps
+(whatever --steps), and the CPU oracle timed on the host cores as the CPU
+baseline.  Two more legs state the metric's other parts at their
+configurations' own lengths: the rough fault with plasticity (2 x 4001 x
+2001, 3000 steps, FP32-bulk against FP64: speed ratio and error) and the
+adjoint gradient (2 x 801 x 401, 2000 steps: gradient time over forward
+time with square-root checkpointing and with the fault trace kept, and the
+finite-difference check).  --no-legs leaves them out.
 """
 import argparse
 import json
@@ -39,10 +45,12 @@ def rel_l2(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / (d if d > 0 else 1.0))
 
 
-def run_device(inpath, precision, steps, warmup, prof_steps):
+def run_device(inpath, precision, steps, warmup, prof_steps, horizon=None):
     """Time `steps` steps after `warmup` untimed ones (the headline number),
     then `prof_steps` more with HIP events around every launch (kernel
-    durations for the roofline)."""
+    durations for the roofline).  horizon: keep stepping until that many
+    steps in all, and return the fields and fault arrays there too (errors
+    at the configuration's own length, whatever the timed steps)."""
     from fdmap_github_amd import host
     from fdmap_github_amd.device import Context
 
@@ -66,10 +74,19 @@ def run_device(inpath, precision, steps, warmup, prof_steps):
     ctx.sync()
     el_prof = time.perf_counter() - t0
     prof = ctx.profile_read()
+    ctx.profile_enable(False)
     nstage = hp.ptr.contents.nstage
+    done = warmup + steps + prof_steps
+    Fh, fault_h = F, fault
+    if horizon is not None and horizon > done:
+        ctx.step(hp.t, hp.dt, done + 1, horizon - done)
+        done = horizon
+    if horizon is not None:
+        Fh, _ = ctx.get_fields()
+        fault_h = {f: ctx.get_iface(0, f) for f in ("D", "V", "S")}
     ctx.close()
     return dict(hp=hp, npts=npts, el=el, el_prof=el_prof, prof=prof, F=F, fault=fault,
-                nstage=nstage)
+                nstage=nstage, F_h=Fh, fault_h=fault_h, horizon=done)
 
 
 def cpu_baseline(inpath, sample_steps):
@@ -106,47 +123,54 @@ def measured_traffic():
     return None
 
 
-def run_c3(args, tmp):
+def c3_measure(tmp, nx=4001, ny=2001, steps=3000, warmup=0, prof_steps=0):
     """Rough fault: a seeded band-limited self-similar profile (rms amplitude
     1e-2 of the fault length, wavelengths from 20 h to the fault length) as
-    the shared edge of two curvilinear 4001 x 2001 blocks, C2's friction and
+    the shared edge of two curvilinear nx x ny blocks, C2's friction and
     prestress, Drucker-Prager plasticity (cohesion 0, friction 0.6); FP64 and
-    FP32-bulk, the FP32 error stated against FP64."""
+    FP32-bulk over the same steps, the FP32 error stated against FP64 after
+    warmup + steps steps."""
     from fdmap_github_amd import inputs
 
-    nx = 4001 if args.nx == 1601 else args.nx
-    ny = 2001 if args.ny == 801 else args.ny
-    steps = 3000 if args.steps == 4000 else args.steps
+    os.makedirs(tmp, exist_ok=True)
     h = 0.025
     x, y, n1, n2 = inputs.fractal_profile(nx, h, alpha=1e-2, seed=3)
     curve = os.path.join(tmp, "fault.bin")
     inputs.write_curve(curve, x, y, n1, n2)
     inpath = inputs.write(os.path.join(tmp, "c3.in"),
                           inputs.c2_planar_rs(name=os.path.join(tmp, "c3"), nx=nx, ny=ny, h=h,
-                                              nsteps=steps + args.warmup + args.prof_steps,
+                                              nsteps=steps + warmup + prof_steps,
                                               outputs=False, plastic=True, curve=curve))
-    r64 = run_device(inpath, "fp64", steps, args.warmup, args.prof_steps)
-    r32 = run_device(inpath, "fp32", steps, args.warmup, args.prof_steps)
+    r64 = run_device(inpath, "fp64", steps, warmup, prof_steps)
+    r32 = run_device(inpath, "fp32", steps, warmup, prof_steps)
     npts = r64["npts"]
     ups64 = npts * steps / r64["el"]
     ups32 = npts * steps / r32["el"]
+    return dict(r64=r64, r32=r32, npts=npts, ups64=ups64, ups32=ups32, steps=steps, nx=nx, ny=ny,
+                fp32={"updates_per_s": ups32, "speedup_vs_fp64": ups32 / ups64,
+                      "ms_per_step": 1e3 * r32["el"] / steps,
+                      "rel_l2_fields_vs_fp64": rel_l2(r32["F"], r64["F"]),
+                      "rel_l2_slip_vs_fp64": rel_l2(r32["fault"]["D"], r64["fault"]["D"]),
+                      "rel_l2_slip_velocity_vs_fp64": rel_l2(r32["fault"]["V"], r64["fault"]["V"]),
+                      "after_steps": steps + warmup})
+
+
+def run_c3(args, tmp):
+    nx = 4001 if args.nx == 1601 else args.nx
+    ny = 2001 if args.ny == 801 else args.ny
+    steps = 3000 if args.steps == 4000 else args.steps
+    m = c3_measure(tmp, nx, ny, steps, args.warmup, args.prof_steps)
+    r64, r32 = m["r64"], m["r32"]
     out = {
-        "metric": METRIC, "value": ups64, "unit": "updates/s", "n_gpus": 1, "steps": steps,
+        "metric": METRIC, "value": m["ups64"], "unit": "updates/s", "n_gpus": 1, "steps": steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * r64["el"] / steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (rough-fault problem generated as an FDMAP .in file and curve file)",
         "config": {"workload": f"rough curvilinear fault with plasticity, 2 blocks x ({nx}x{ny}), "
                                "self-similar profile alpha 1e-2 seed 3, RFL slip law, "
                                "Drucker-Prager mu 0.6 b 0, LSRK(5,4) CFL 0.5",
-                   "grid_points": npts, "fault_nodes": nx},
-        "fp32_bulk": {
-            "updates_per_s": ups32, "speedup_vs_fp64": ups32 / ups64,
-            "ms_per_step": 1e3 * r32["el"] / steps,
-            "rel_l2_fields_vs_fp64": rel_l2(r32["F"], r64["F"]),
-            "rel_l2_slip_vs_fp64": rel_l2(r32["fault"]["D"], r64["fault"]["D"]),
-            "rel_l2_slip_velocity_vs_fp64": rel_l2(r32["fault"]["V"], r64["fault"]["V"]),
-            "after_steps": steps + args.warmup,
-        },
+                   "grid_points": m["npts"], "fault_nodes": nx},
+        "fp32_bulk": m["fp32"],
         "avg_launch_ms": {prec: {k: r["prof"][k + "_ms"] / max(1, r["prof"][k + "_launches"])
                                  for k in ("bulk", "band", "edge")}
                           for prec, r in (("fp64", r64), ("fp32", r32))},
@@ -436,7 +460,7 @@ def run_c4(args, tmp):
     print(json.dumps(out), flush=True)
 
 
-def run_c5(args, tmp):
+def c5_measure(tmp, steps=2000):
     """Adjoint gradient: the planar rate-and-state rupture (C2 parameters)
     on two blocks of 801 x 401 points with a free surface on top, 2000
     steps; misfit 0.5 dt sum (v - v_target)^2 of vx, vy at 32 free-surface
@@ -452,7 +476,6 @@ def run_c5(args, tmp):
     from fdmap_github_amd.adjoint import FaultInversion
     from fdmap_github_amd.device import Context
 
-    steps = 2000 if args.steps == 4000 else args.steps
     nx, ny, h = 801, 401, 0.025
     rng = np.random.default_rng(5)
     sc = dict(index=0, nx=nx, ny=ny, a=0.010, S0=np.full(nx, 70.0), h=h)
@@ -548,7 +571,11 @@ def run_c5(args, tmp):
         "fd_check": {"eps": eps, **fd},
         "fd_max_rel_err": max(v["rel_err"] for v in fd.values()),
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def run_c5(args, tmp):
+    print(json.dumps(c5_measure(tmp, 2000 if args.steps == 4000 else args.steps)), flush=True)
 
 
 def main():
@@ -563,6 +590,12 @@ def main():
                     help="steps timed after the headline ones with HIP events per launch")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-legs", action="store_true",
+                    help="leave out the C3 (FP32-bulk against FP64 on the rough fault, 3000 "
+                         "steps) and C5 (adjoint gradient time) legs of the default line")
+    ap.add_argument("--horizon", type=int, default=4000,
+                    help="steps after which the FP32-bulk error against FP64 is stated (C2's "
+                         "own length), whatever --steps")
     ap.add_argument("--config", choices=("c2", "c2het", "c2diss", "c2pml", "m3", "c3", "c4", "c5"),
                     default="c2",
                     help="c2: planar rupture (the headline); c2het: the same in a "
@@ -605,9 +638,11 @@ def main():
     inpath = inputs.write(os.path.join(tmp, "c2.in"),
                           inputs.c2_planar_rs(name=os.path.join(tmp, "c2"), nx=args.nx,
                                               ny=args.ny,
-                                              nsteps=args.steps + args.warmup + args.prof_steps,
+                                              nsteps=max(args.horizon,
+                                                         args.steps + args.warmup + args.prof_steps),
                                               outputs=False))
-    r64 = run_device(inpath, "fp64", args.steps, args.warmup, args.prof_steps)
+    horizon = None if args.no_fp32 else args.horizon
+    r64 = run_device(inpath, "fp64", args.steps, args.warmup, args.prof_steps, horizon)
     npts = r64["npts"]
     prof = r64["prof"]
     ns = r64["nstage"]
@@ -663,23 +698,49 @@ def main():
     }
     out["roofline"]["traffic"] = measured_traffic()
     if not args.no_fp32:
-        r32 = run_device(inpath, "fp32", args.steps, args.warmup, args.prof_steps)
+        r32 = run_device(inpath, "fp32", args.steps, args.warmup, args.prof_steps, horizon)
         ups32 = npts * args.steps / r32["el"]
         p32 = r32["prof"]
         c1, c2 = BYTES_STAGE["fp32"]
         avg32 = p32["bulk_ms"] / max(1, p32["bulk_launches"])
         k32 = p32["bulk_points"]
+        # the stored planes of the FP32-bulk context: fields (two buffers),
+        # rates and metrics, 23 planes of 4 bytes
+        hp = r64["hp"]
+        ws = 23 * 4 * hp.ny * (-(-hp.nx // 64) * 64 + 64)
         out["fp32_bulk"] = {
             "updates_per_s": ups32,
             "speedup_vs_fp64": ups32 / ups,
             "ms_per_step": 1e3 * r32["el"] / args.steps,
             "bulk_avg_launch_ms": avg32,
             "bulk_achieved_GBs": k32 * (c1 + (ns - 1) * c2) / ns / (avg32 * 1e-3) / 1e9,
-            "rel_l2_fields_vs_fp64": rel_l2(r32["F"], r64["F"]),
-            "rel_l2_slip_vs_fp64": rel_l2(r32["fault"]["D"], r64["fault"]["D"]),
-            "rel_l2_slip_velocity_vs_fp64": rel_l2(r32["fault"]["V"], r64["fault"]["V"]),
-            "after_steps": args.steps + args.warmup,
+            "working_set_MB": ws / 2 ** 20,
+            "cache_resident": ws < 256 * 2 ** 20,
+            "note": "the FP32 planes fit MI355X's 256 MB Infinity Cache: bulk_achieved_GBs is "
+                    "cache-assisted, not an HBM figure" if ws < 256 * 2 ** 20 else "",
+            "rel_l2_fields_vs_fp64": rel_l2(r32["F_h"], r64["F_h"]),
+            "rel_l2_slip_vs_fp64": rel_l2(r32["fault_h"]["D"], r64["fault_h"]["D"]),
+            "rel_l2_slip_velocity_vs_fp64": rel_l2(r32["fault_h"]["V"], r64["fault_h"]["V"]),
+            "after_steps": r64["horizon"],
         }
+    if not args.no_legs:
+        t0 = time.perf_counter()
+        m = c3_measure(os.path.join(tmp, "c3"))
+        out["c3_rough_fault"] = {
+            "workload": f"rough curvilinear fault with plasticity, 2 blocks x ({m['nx']}x{m['ny']}), "
+                        "FP64 and FP32-bulk over the same 3000 steps",
+            "steps": m["steps"], "fp64_updates_per_s": m["ups64"], **m["fp32"],
+            "leg_s": time.perf_counter() - t0}
+        t0 = time.perf_counter()
+        r5 = c5_measure(os.path.join(tmp, "c5"))
+        out["c5_adjoint"] = {
+            "workload": r5["config"]["workload"],
+            "gradient_over_forward_sqrtN_checkpoints": r5["modes"]["sqrtN"]["ratio_to_forward"],
+            "gradient_over_forward_trace_kept": r5["modes"]["auto"]["ratio_to_forward"],
+            "interval_sqrtN": r5["modes"]["sqrtN"]["interval"],
+            "forward_s": r5["forward_s"], "modes": r5["modes"],
+            "fd_eps": r5["fd_check"]["eps"], "fd_max_rel_err": r5["fd_max_rel_err"],
+            "leg_s": time.perf_counter() - t0}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(inpath, args.cpu_steps)
         out["cpu_baseline"]["gpu_over_cpu"] = ups / out["cpu_baseline"]["value"]

@@ -281,6 +281,15 @@ struct fdm_ctx {
   long long rec_n0 = 0;
   long long cur_n = 0;       // step being launched
   long long first_n = 0;     // first step of the current fdm_step call
+  // the frictional interface nodes (indices into the [.][NI] interface
+  // arrays) and the context's reduction buffer of fdm_max_slip_velocity:
+  // vred[0] the result, vred[1..] one partial maximum per workgroup
+  int* fnode = nullptr;
+  int nfnode = 0;
+  double* vred = nullptr;
+  // the interior kernel forms Ji from the metrics (FP64, when the given Ji
+  // equals 1/(xq yr - xr yq) everywhere; FDM_JI_CALC = 0 turns it off)
+  bool ji_calc = false;
 };
 
 // Non-temporal stores pay when a launch's working set (bytes of the planes
@@ -506,6 +515,8 @@ BulkArgs<T> bulk_args(fdm_ctx* c, double A, double Bdt, int do_update, double t,
   a.mnu = (const T*)c->mat[4];
   a.J = (const T*)c->Jp;
   a.W = (T*)c->W;
+  a.ji_calc = c->ji_calc ? 1 : 0;
+  a.skip_df = 0;
   return a;
 }
 
@@ -646,8 +657,12 @@ int join_side(fdm_ctx* c) {
   return FDM_OK;
 }
 
+// skip_df: the stage is the last of a step that another step of the same
+// call follows, so the interior launch may leave DF unstored (the next
+// stage has A = 0)
 template <typename T>
-int run_stage(fdm_ctx* c, int stage, double t0, double dt, int final_step, int snapshot) {
+int run_stage(fdm_ctx* c, int stage, double t0, double dt, int final_step, int snapshot,
+              int skip_df = 0) {
   double A = c->rkA[stage - 1];
   double Bdt = c->rkB[stage - 1] * dt;
   double t = t0 + c->rkC[stage - 1] * dt;  // D%t during this stage (time_step.f90:137)
@@ -655,8 +670,9 @@ int run_stage(fdm_ctx* c, int stage, double t0, double dt, int final_step, int s
   double srcA = c->src.nplace ? stf_amplitude(c->src, t) : 0.0;
   c->g_rkc = c->rkC[stage - 1];
   int last = (stage == c->nstage);
-  const BulkArgs<T> a =
+  BulkArgs<T> a =
       bulk_args<T>(c, A, Bdt, final_step ? 0 : 1, t, srcA, (!final_step && last) ? 1 : 0, dt);
+  a.skip_df = (skip_df && last && !final_step) ? 1 : 0;
   int rc;
   if (c->bulk_mode == 2) {
     // Interior points need neither this stage's edge values nor this
@@ -702,7 +718,11 @@ int step_impl(fdm_ctx* c, double t, double dt, long long n_first, int nsteps) {
     long long n = n_first + k;
     c->cur_n = n;
     double t0 = t + (double)(n - 1) * dt;  // main.f90:176
-    for (int stage = 1; stage <= c->nstage && !rc; stage++) rc = run_stage<T>(c, stage, t0, dt, 0, stage == 1);
+    // DF after the last stage is read by no later stage (A = 0 at stage 1)
+    // and is stored for the caller only by the call's last step
+    const int skip_df = (k + 1 < nsteps && c->rkA[0] == 0.0) ? 1 : 0;
+    for (int stage = 1; stage <= c->nstage && !rc; stage++)
+      rc = run_stage<T>(c, stage, t0, dt, 0, stage == 1, skip_df);
   }
   int rj = join_side(c);
   if (!rc && !rj && c->rec_out && nsteps > 0) {
@@ -741,6 +761,72 @@ void restore_df6(const fdm_ctx* c, T* h) {
 // step counter after the stages
 __global__ void __launch_bounds__(64) step_counter_kernel(long long* n) {
   if (threadIdx.x == 0) n[0] = n[0] + 1;
+}
+
+// Maximum slip velocity on the device (check_nucleation, domain.f90:474-499;
+// interfaces.f90:506-517): each workgroup of kVmaxWG lanes takes the maximum
+// of V over its share of the frictional nodes, starting from 0 as the
+// reference's maxval over non-negative V does; one more workgroup reduces
+// those partial maxima.  Maxima are exact, so the result does not depend on
+// the order; a NaN never replaces a number, as with std::max.
+constexpr int kVmaxWG = 256;
+
+__device__ __forceinline__ double vmax_tree(double v) {
+  __shared__ double s[kVmaxWG];
+  s[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int w = kVmaxWG / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double x = s[threadIdx.x + w];
+      if (s[threadIdx.x] < x) s[threadIdx.x] = x;
+    }
+    __syncthreads();
+  }
+  return s[0];
+}
+
+__global__ void __launch_bounds__(kVmaxWG) vmax_partial_kernel(const double* __restrict__ V,
+                                                               const int* __restrict__ node, int n,
+                                                               double* __restrict__ part) {
+  const int k = blockIdx.x * kVmaxWG + threadIdx.x;
+  double v = 0.0;
+  if (k < n) {
+    const double x = V[node[k]];
+    v = (v < x) ? x : v;
+  }
+  v = vmax_tree(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(kVmaxWG) vmax_final_kernel(const double* __restrict__ part, int np,
+                                                             double* __restrict__ out) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < np; k += kVmaxWG) {
+    const double x = part[k];
+    v = (v < x) ? x : v;
+  }
+  v = vmax_tree(v);
+  if (threadIdx.x == 0) out[0] = v;
+}
+
+// the two reduction launches on the context's stream with the buffer red
+// (red[0] the result, red[1..np] the partial maxima), then the result to
+// the host
+int vmax_reduce(fdm_ctx* c, double* red, double* vmax) {
+  if (c->nfnode == 0) {
+    *vmax = 0.0;
+    return FDM_OK;
+  }
+  const int np = (c->nfnode + kVmaxWG - 1) / kVmaxWG;
+  const double* V = c->ist + (size_t)FDM_IF_V * c->NI;
+  hipLaunchKernelGGL(vmax_partial_kernel, dim3(np), dim3(kVmaxWG), 0, c->stream, V, c->fnode,
+                     c->nfnode, red + 1);
+  hipLaunchKernelGGL(vmax_final_kernel, dim3(1), dim3(kVmaxWG), 0, c->stream, red + 1, np, red);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(vmax, red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return FDM_OK;
 }
 
 int check_fail(fdm_ctx* c) {
@@ -784,6 +870,10 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     return set_err(nullptr, FDM_EINVAL, "fdm_create: malformed problem description");
   if (P->precision != FDM_FP64 && P->precision != FDM_FP32_BULK)
     return set_err(nullptr, FDM_EINVAL, "fdm_create: unknown precision");
+  if (P->sbp != 0 && P->sbp != 4)
+    return set_err(nullptr, FDM_EINVAL,
+                   "fdm_create: the device kernels implement SBP4 only (FDmethod 'SBP" +
+                       std::to_string(P->sbp) + "' requested)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_err(nullptr, FDM_EDEVICE, "fdm_create: no HIP device available");
@@ -1384,6 +1474,21 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
       HIPCHK(c, hipMemcpy(c->met[q], h.data(), h.size() * 4, hipMemcpyHostToDevice));
     }
   }
+  {
+    // In FP64 the interior kernel may form Ji itself when the given Ji is
+    // 1/(xq yr - xr yq) to the bit (grid.f90:531-534) at every point: the
+    // Ji plane is then not read by that kernel (8 bytes per point and
+    // stage).  This file is compiled without contraction, as the kernel.
+    bool exact = c->precision == FDM_FP64;
+    const size_t np = (size_t)c->nx * c->ny;
+    for (size_t k = 0; k < np && exact; k++) {
+      const double J = P->xq[k] * P->yr[k] - P->xr[k] * P->yq[k];
+      const double ji = 1.0 / J;
+      exact = std::memcmp(&ji, &P->Ji[k], sizeof(double)) == 0;
+    }
+    if (const char* v = getenv("FDM_JI_CALC")) exact = exact && atoi(v) != 0;
+    c->ji_calc = exact;
+  }
   for (int b = 0; b < c->nb; b++) c->any_mms |= (P->blocks[b].mms != 0);
   bool graded = false;
   for (int b = 0; b < c->nb; b++) graded |= (c->hblocks[b].pmlx || c->hblocks[b].pmly) && c->hblocks[b].pml[0] > 0.0;
@@ -1410,6 +1515,19 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   {
     int z = 0;
     if ((rc = dupload(c, &c->fail, &z, sizeof(int)))) return fail(rc);
+  }
+  {
+    // frictional nodes and the reduction buffer of fdm_max_slip_velocity
+    std::vector<int> fn;
+    for (const IfaceDev& I : c->hifaces)
+      if (I.coupling == FDM_COUPLING_FRICTION)
+        for (int k = 0; k < I.n; k++) fn.push_back(I.node_off + k);
+    c->nfnode = (int)fn.size();
+    if (c->nfnode) {
+      if ((rc = dupload(c, &c->fnode, fn.data(), fn.size() * sizeof(int)))) return fail(rc);
+      const size_t np = (fn.size() + kVmaxWG - 1) / kVmaxWG;
+      if ((rc = dalloc(c, &c->vred, (1 + np) * sizeof(double)))) return fail(rc);
+    }
   }
   *out = c;
   return FDM_OK;
@@ -1625,16 +1743,7 @@ int fdm_set_pml(fdm_ctx* c, const double* Wx, const double* Wy) {
 
 int fdm_max_slip_velocity(fdm_ctx* c, double* vmax) {
   if (!c || !vmax) return FDM_EINVAL;
-  double v = 0.0;
-  for (int ii = 0; ii < c->ni; ii++) {
-    if (c->hifaces[ii].coupling != FDM_COUPLING_FRICTION) continue;
-    std::vector<double> h(c->hifaces[ii].n);
-    int rc = iface_copy(c, ii, FDM_IF_V, h.data(), nullptr, c->ist);
-    if (rc) return rc;
-    for (double x : h) v = std::max(v, x);
-  }
-  *vmax = v;
-  return FDM_OK;
+  return vmax_reduce(c, c->vred, vmax);
 }
 
 int fdm_sync(fdm_ctx* c) {

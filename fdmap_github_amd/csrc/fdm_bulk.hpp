@@ -51,6 +51,14 @@ struct BulkArgs {
   // PML auxiliary fields: Wx (planes 0-5), Wy (6-11), DWx (12-17), DWy
   // (18-23), plane stride ps (null without a layer)
   T* W;
+  // lean interior launches: ji_calc forms the point's Ji as 1/(xq yr - xr yq)
+  // from the staged metrics instead of reading the Ji plane (set when the
+  // context's Ji equals that expression bit for bit at every point);
+  // skip_df leaves DF1..5 unstored (the last stage of a step that another
+  // step of the same call follows: the next stage scales DF by A = 0 and
+  // never reads it)
+  int ji_calc;
+  int skip_df;
 };
 
 // Map the hardware workgroup id to a tile.  Workgroups are dispatched to the
@@ -998,7 +1006,8 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     for (int l = 0; l < 5; l++) dfa[l] = ld_off(a.DF + l * ps, oc);
   }
   const T f6 = ld_off(F + 5 * ps, oc);
-  const T Ji = ld_off(a.Ji, oc);
+  T Ji = (T)0.0;
+  if (!a.ji_calc) Ji = ld_off(a.Ji, oc);
   // a heterogeneous medium: the lane's own material (rho, G, M, gamma, nu)
   T mat[5];
   if constexpr (HET) {
@@ -1038,6 +1047,13 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
   }
   __syncthreads();
   if (!valid) return;
+  if (a.ji_calc) {
+    // grid.f90:531-534: J = xq yr - xr yq, Ji = 1/J, from the lane's own
+    // metrics (no contraction: the host's rounding)
+    const T J = sQ[0][threadIdx.y + 2][threadIdx.x] * sR[1][threadIdx.y][threadIdx.x + 2] -
+                sR[0][threadIdx.y][threadIdx.x + 2] * sQ[1][threadIdx.y + 2][threadIdx.x];
+    Ji = (T)1.0 / J;
+  }
   T d[6];
   if (a.a_zero) {
 #pragma unroll
@@ -1087,8 +1103,10 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
 #pragma unroll
       for (int l = 0; l < 6; l++) st_sel<NTS>(a.Fout + l * ps, oc, Fc[l] + Bdt * d[l]);
     }
+    if (!a.skip_df) {
 #pragma unroll
-    for (int l = 0; l < 5; l++) st_sel<NTS>(a.DF + l * ps, oc, d[l]);
+      for (int l = 0; l < 5; l++) st_sel<NTS>(a.DF + l * ps, oc, d[l]);
+    }
   }
 }
 

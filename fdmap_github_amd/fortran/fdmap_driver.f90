@@ -220,7 +220,7 @@ contains
     call wm_c(u,'method','LS','RK')
     call wm_i(u,'order',RKorder,'RK')
     call wm_i(u,'mode',pb_mode)
-    call wm_c(u,'FDmethod','SBP4')
+    call wm_c(u,'FDmethod',merge('SBP2',merge('SBP4','SBP6',sbp_order==4),sbp_order==2))
     call wm_i(u,'nblocks',nblocks)
     call wm_i(u,'nifaces',nifaces)
     call wm_i(u,'nx',gnx)

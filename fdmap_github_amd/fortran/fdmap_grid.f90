@@ -1,5 +1,5 @@
 ! fdmap_grid.f90 -- block grids, boundary curves, transfinite interpolation
-! and SBP4 metrics on the host (FP64).
+! and SBP metrics on the host (FP64).
 !
 ! Restates grid.f90:188-618, geometry.f90:70-200 and fd.f90:42-80 for one
 ! process holding every block: the metrics computed here are the inputs
@@ -11,21 +11,14 @@ module fdmap_grid
   implicit none
 
   integer,parameter :: dp = c_double
-  integer,parameter :: nbnd = 6, nbst = 4
-
-  ! SBP4 tables (fd_coeff.f90:101-115); DR(-s,-r) = -DL(s,r) (:242-246)
-  real(dp),parameter :: DI(-2:2) = (/ 0.083333333333333_dp, -0.666666666666667_dp, &
-       0.000000000000000_dp, 0.666666666666667_dp, -0.083333333333333_dp /)
-  real(dp),parameter :: DL(0:5,0:3) = reshape( (/ &
-       -1.411764705882353_dp,  1.735294117647059_dp, -0.235294117647059_dp, &
-       -0.088235294117647_dp,  0.000000000000000_dp,  0.000000000000000_dp, &
-       -0.500000000000000_dp,  0.000000000000000_dp,  0.500000000000000_dp, &
-        0.000000000000000_dp,  0.000000000000000_dp,  0.000000000000000_dp, &
-        0.093023255813953_dp, -0.686046511627907_dp,  0.000000000000000_dp, &
-        0.686046511627907_dp, -0.093023255813953_dp,  0.000000000000000_dp, &
-        0.030612244897959_dp,  0.000000000000000_dp, -0.602040816326531_dp, &
-        0.000000000000000_dp,  0.653061224489796_dp, -0.081632653061224_dp /), (/6,4/) )
-  real(dp),parameter :: H00i = 2.823529411764706_dp
+  ! the first-derivative operator of the problem (FDmethod, fd_coeff.f90:
+  ! 41-248), set by init_op: points of a boundary stencil, boundary
+  ! stencils, interior half-width, DI(-pI:pI), DL(0:nbnd-1,0:nbst-1) and the
+  ! SAT weight H00i; DR(-s,-r) = -DL(s,r) (:242-246)
+  integer :: nbnd = 6, nbst = 4, pI = 2, sbp_order = 4
+  real(dp) :: DI(-3:3) = 0.0_dp
+  real(dp) :: DL(0:8,0:5) = 0.0_dp
+  real(dp) :: H00i = 2.823529411764706_dp
 
   type :: point
      real(dp) :: x,y
@@ -46,8 +39,65 @@ module fdmap_grid
 
 contains
 
+  subroutine init_op(method,ok)
+    ! init_fd (fd_coeff.f90:41-248) for 'SBP2', 'SBP4' and 'SBP6': the
+    ! reference's 15-digit literals
+    character(*),intent(in) :: method
+    logical,intent(out) :: ok
+    ok = .true.
+    DI = 0.0_dp
+    DL = 0.0_dp
+    select case(method)
+    case('SBP2')
+       sbp_order = 2; nbnd = 2; nbst = 1; pI = 1
+       H00i = 2.0_dp
+       DI(-1:1) = (/ -0.5_dp, 0.0_dp, 0.5_dp /)
+       DL(0:1,0) = (/ -1.0_dp, 1.0_dp /)
+    case('SBP4')
+       sbp_order = 4; nbnd = 6; nbst = 4; pI = 2
+       H00i = 2.823529411764706_dp
+       DI(-2:2) = (/ 0.083333333333333_dp, -0.666666666666667_dp, &
+            0.000000000000000_dp, 0.666666666666667_dp, -0.083333333333333_dp /)
+       DL(0:5,0) = (/ -1.411764705882353_dp,  1.735294117647059_dp, -0.235294117647059_dp, &
+            -0.088235294117647_dp,  0.000000000000000_dp,  0.000000000000000_dp /)
+       DL(0:5,1) = (/ -0.500000000000000_dp,  0.000000000000000_dp,  0.500000000000000_dp, &
+            0.000000000000000_dp,  0.000000000000000_dp,  0.000000000000000_dp /)
+       DL(0:5,2) = (/ 0.093023255813953_dp, -0.686046511627907_dp,  0.000000000000000_dp, &
+            0.686046511627907_dp, -0.093023255813953_dp,  0.000000000000000_dp /)
+       DL(0:5,3) = (/ 0.030612244897959_dp,  0.000000000000000_dp, -0.602040816326531_dp, &
+            0.000000000000000_dp,  0.653061224489796_dp, -0.081632653061224_dp /)
+    case('SBP6')
+       sbp_order = 6; nbnd = 9; nbst = 6; pI = 3
+       H00i = 3.165067037878233_dp
+       DI = (/ -0.016666666666667_dp, 0.150000000000000_dp, -0.750000000000000_dp, &
+            0.000000000000000_dp, 0.750000000000000_dp, -0.150000000000000_dp, &
+            0.016666666666667_dp /)
+       DL(:,0) = (/ -1.582533518939116_dp, 2.033378678700676_dp, -0.141512858744873_dp, &
+            -0.450398306578272_dp, 0.104488069284042_dp, 0.036577936277544_dp, &
+            0.000000000000000_dp, 0.000000000000000_dp, 0.000000000000000_dp /)
+       DL(:,1) = (/ -0.462059195631158_dp, 0.000000000000000_dp, 0.287258622978251_dp, &
+            0.258816087376832_dp, -0.069112065532624_dp, -0.014903449191300_dp, &
+            0.000000000000000_dp, 0.000000000000000_dp, 0.000000000000000_dp /)
+       DL(:,2) = (/ 0.071247104721830_dp, -0.636451095137907_dp, 0.000000000000000_dp, &
+            0.606235523609147_dp, -0.022902190275815_dp, -0.018129342917256_dp, &
+            0.000000000000000_dp, 0.000000000000000_dp, 0.000000000000000_dp /)
+       DL(:,3) = (/ 0.114713313798970_dp, -0.290087484386815_dp, -0.306681191361148_dp, &
+            0.000000000000000_dp, 0.520262285050482_dp, -0.051642265516119_dp, &
+            0.013435342414630_dp, 0.000000000000000_dp, 0.000000000000000_dp /)
+       DL(:,4) = (/ -0.036210680656541_dp, 0.105400944933782_dp, 0.015764336127392_dp, &
+            -0.707905442575989_dp, 0.000000000000000_dp, 0.769199413962647_dp, &
+            -0.164529643265203_dp, 0.018281071473911_dp, 0.000000000000000_dp /)
+       DL(:,5) = (/ -0.011398193015050_dp, 0.020437334208704_dp, 0.011220896474665_dp, &
+            0.063183694641876_dp, -0.691649024426814_dp, 0.000000000000000_dp, &
+            0.739709139060752_dp, -0.147941827812150_dp, 0.016437980868017_dp /)
+    case default
+       ok = .false.
+    end select
+  end subroutine init_op
+
+
   function DR(s,r) result(c)
-    ! DR(s,r) for s = -5..0, r = -3..0
+    ! DR(s,r) for s = -(nbnd-1)..0, r = -(nbst-1)..0
     integer,intent(in) :: s,r
     real(dp) :: c
     c = -DL(-s,-r)
@@ -61,22 +111,22 @@ contains
     real(dp),intent(in) :: f(mg:pg)
     real(dp),intent(out) :: Df(mg:pg)
     integer :: i,k
-    real(dp) :: c(0:nbnd-1)
+    real(dp) :: c(0:8)
     if (mg==pg) then
        Df = 0.0_dp
        return
     end if
     do i = 0,nbst-1
-       Df(mg+i) = dot_product(DL(:,i),f(mg:mg+nbnd-1))
+       Df(mg+i) = dot_product(DL(0:nbnd-1,i),f(mg:mg+nbnd-1))
     end do
     do i = 0,nbst-1
        do k = 0,nbnd-1
           c(k) = DR(k-(nbnd-1),-i)
        end do
-       Df(pg-i) = dot_product(c,f(pg-nbnd+1:pg))
+       Df(pg-i) = dot_product(c(0:nbnd-1),f(pg-nbnd+1:pg))
     end do
     do i = mg+nbst,pg-nbst
-       Df(i) = dot_product(DI,f(i-2:i+2))
+       Df(i) = dot_product(DI(-pI:pI),f(i-pI:i+pI))
     end do
   end subroutine diff
 

@@ -509,7 +509,7 @@ contains
          acoustic_gravity_waves,basal_traction_plane_stress
     real(dp) :: Cdiss
     character(256) :: str
-    logical :: found
+    logical :: found,ok
     integer :: iblock_x,iblock_y,mgx,mgy
     character(256) :: bndLfile,bndRfile,bndBfile,bndTfile
     real(dp) :: bndLmap,bndRmap,bndBmap,bndTmap
@@ -551,7 +551,8 @@ contains
     end if
     if (mode/=2.and.mode/=3) call fail('mode must be 2 (plane strain) or 3 (antiplane shear)')
     pb_mode = mode
-    if (trim(FDmethod)/='SBP4') call fail("only FDmethod = 'SBP4' is supported")
+    call init_op(trim(FDmethod),ok)
+    if (.not.ok) call fail("FDmethod must be 'SBP2', 'SBP4' or 'SBP6' (not '"//trim(FDmethod)//"')")
     if (exact_metric) call fail('exact_metric is not supported')
     if (energy_balance.or.displacement.or.peak.or.acoustic_gravity_waves.or. &
          basal_traction_plane_stress.or.operator_split) &
@@ -579,6 +580,9 @@ contains
        read(input,nml=operator_list,iostat=stat)
        if (stat>0) call fail('Error in operator_list')
        HB(i)%Cdiss = Cdiss
+       ! the dissipation operators are built for SBP4 only
+       if (Cdiss>0.0_dp.and.sbp_order/=4) &
+            call fail("artificial dissipation (Cdiss > 0) needs FDmethod = 'SBP4'")
     end do
 
     do i = 1,nblocks
@@ -637,7 +641,11 @@ contains
          G%mgy = ceiling(dble(G%mgy-G%iblock_y)*refine)+G%iblock_y
          G%pgx = G%mgx+G%nx-1
          G%pgy = G%mgy+G%ny-1
-         if (G%nx<8.or.G%ny<8) call fail('every block needs at least 8 points in x and y for SBP4')
+         if (G%nx<max(8,2*nbst,nbnd).or.G%ny<max(8,2*nbst,nbnd)) then
+            write(str,'(a,i0,a,a)') 'every block needs at least ',max(8,2*nbst,nbnd), &
+                 ' points in x and y for ',trim(FDmethod)
+            call fail(trim(str))
+         end if
        end associate
     end do
 
@@ -687,6 +695,10 @@ contains
        if (pb_mode==3.and.(pmlx.or.pmly)) then
           ! set_rates has no layer in mode 3 (time_step.f90:522-547)
           call fail('pml_list: the perfectly matched layer is available in mode 2 only')
+          return
+       end if
+       if ((pmlx.or.pmly).and.sbp_order/=4) then
+          call fail("pml_list: the perfectly matched layer needs FDmethod = 'SBP4'")
           return
        end if
     end do
@@ -1389,6 +1401,7 @@ contains
     cprob%y = c_loc(gcy(1,1))
     cprob%J = c_loc(gJ(1,1))
     cprob%mode = pb_mode
+    cprob%sbp = sbp_order
     if (allocated(gmrho)) then
        cprob%mat_rho = c_loc(gmrho(1,1))
        cprob%mat_cs = c_loc(gmcs(1,1))

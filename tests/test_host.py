@@ -173,7 +173,7 @@ def test_c1_source_placement(tmp_path):
 
 @pytest.mark.parametrize("text,msg", [
     ("&problem_list nt = 2 /\n&domain_list mode = 4, FDmethod = 'SBP4' /\n", "mode must be 2"),
-    ("&problem_list nt = 2 /\n&domain_list FDmethod = 'SBP6' /\n", "SBP4"),
+    ("&problem_list nt = 2 /\n&domain_list FDmethod = 'WENO' /\n", "SBP2', 'SBP4' or 'SBP6'"),
 ])
 def test_unsupported_options_fail_loudly(tmp_path, text, msg):
     p = inputs.write(str(tmp_path / "bad.in"), text)
@@ -200,6 +200,51 @@ def test_plastic_strain_and_work_flags_are_refused(tmp_path, flag):
     host.HostProblem(inputs.write(str(tmp_path / "pe.in"), ok))
     # and the plastic problem without them is accepted
     host.HostProblem(inputs.write(str(tmp_path / "pp.in"), text))
+
+
+@pytest.mark.parametrize("method,order,h00i", [("SBP6", 6, 3.165067037878233), ("SBP2", 2, 2.0),
+                                               (None, 6, 3.165067037878233)])
+def test_fdmethod_selects_the_operator(tmp_path, method, order, h00i):
+    """FDmethod (domain_list, default 'SBP6', domain.f90:94) selects the
+    operator of the metrics (grid.f90:465-534 via fd.f90:42-80) and of the
+    SAT weights (H00i, domain.f90:1436-1456), and reaches the problem
+    description"""
+    import problems
+
+    # the fault (block 1's top, block 2's bottom) is a curved profile, so the
+    # metrics vary and the operators give different ones
+    x, y, n1, n2 = inputs.fractal_profile(41, 0.025, alpha=2e-2, seed=4)
+    curve = str(tmp_path / "fault.bin")
+    inputs.write_curve(curve, x, y, n1, n2)
+    text = inputs.c2_planar_rs(name="op", nx=41, ny=21, nsteps=5, curve=curve)
+    if method is None:
+        text = text.replace(", FDmethod = 'SBP4'", "")
+        assert "FDmethod" not in text
+    else:
+        text = text.replace("FDmethod = 'SBP4'", f"FDmethod = '{method}'")
+    hp = host.HostProblem(inputs.write(str(tmp_path / "op.in"), text))
+    P = hp.ptr.contents
+    assert P.sbp == order
+    X, Y = hp.coords("x"), hp.coords("y")
+    for b in (0, 1):
+        sj = slice(b * 21, (b + 1) * 21)
+        np.testing.assert_allclose(hp.coords("xq")[sj], problems.sbp_diff(X[sj], order), rtol=0,
+                                   atol=1e-14)
+        np.testing.assert_allclose(hp.coords("yr")[sj], problems.sbp_diff(Y[sj].T, order).T,
+                                   rtol=0, atol=1e-14)
+        np.testing.assert_allclose(hp.coords("yq")[sj], problems.sbp_diff(Y[sj], order), rtol=0,
+                                   atol=1e-14)
+        other = problems.sbp_diff(Y[sj], 4 if order != 4 else 6)
+        assert np.abs(hp.coords("yq")[sj] - other).max() > 1e-6  # the operator matters
+    M = np.ctypeslib.as_array(P.blocks[0].side[3].M, shape=(5, 41))
+    np.testing.assert_allclose(M[3], h00i * 3.464 / 0.025, rtol=1e-12)
+
+
+def test_dissipation_needs_sbp4(tmp_path):
+    text = inputs.c2_planar_rs(name="dx", nx=41, ny=21, nsteps=5, Cdiss=0.1).replace(
+        "FDmethod = 'SBP4'", "FDmethod = 'SBP6'")
+    with pytest.raises(RuntimeError, match="Cdiss > 0"):
+        host.HostProblem(inputs.write(str(tmp_path / "dx.in"), text))
 
 
 def test_heterogeneous_medium_file(tmp_path):

@@ -399,3 +399,22 @@ def test_pml_fp32_bulk_error_against_fp64(gpu):
     assert orc.get_iface(0, "V").max() > 1e-3 and np.abs(Wo[0]).max() > 0.0
     for k, v in e.items():
         assert v <= 1e-5, (k, v)
+
+
+def test_frictional_interface_with_unequal_side_prestress(gpu):
+    """The reference's frictional coupling takes the two sides' initial
+    fields crosswise (interfaces.f90:795 against :900), which shows only
+    when they differ: here the minus block's shear prestress is 10 MPa
+    below the plus block's, and both blocks carry a perturbation."""
+    bg_p = np.array(BG)
+    bg_m = bg_p.copy()
+    bg_m[3] = 60.0
+    prob, F = problems.build(nx=64, nys=(30, 34), seed=36, warp=0.03, background=list(bg_p),
+                             perturb=1e-2, law="RFL", fault_overstress=20.0,
+                             bcs=("absorbing", "absorbing", "absorbing", "free"))
+    F[:, :30] += (bg_m - bg_p)[:, None, None]
+    for side in prob.blocks[0]["sides"]:
+        side["F0"] = np.tile(bg_m[:, None], (1, side["F0"].shape[1]))
+    dev, orc = run_both(prob, F, 40)
+    compare(dev, orc, 1, bit_exact=False)
+    assert np.all(orc.get_iface(0, "S0") < 60.0 + 20.0 + 1e-9)  # the minus side's prestress loads
