@@ -236,8 +236,11 @@ def test_fdmethod_selects_the_operator(tmp_path, method, order, h00i):
                                    atol=1e-14)
         other = problems.sbp_diff(Y[sj], 4 if order != 4 else 6)
         assert np.abs(hp.coords("yq")[sj] - other).max() > 1e-6  # the operator matters
+    # SAT weights of block 1's top (the curved fault): H00i of the operator
+    j = 20
+    xq, yq, J = hp.coords("xq")[j], hp.coords("yq")[j], hp.coords("J")[j]
     M = np.ctypeslib.as_array(P.blocks[0].side[3].M, shape=(5, 41))
-    np.testing.assert_allclose(M[3], h00i * 3.464 / 0.025, rtol=1e-12)
+    np.testing.assert_allclose(M[3], h00i * 3.464 * np.sqrt(xq ** 2 + yq ** 2) / J, rtol=1e-13)
 
 
 def test_dissipation_needs_sbp4(tmp_path):
