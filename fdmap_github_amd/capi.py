@@ -175,8 +175,11 @@ ENTRY_POINTS = [
     ("fdm_profile_read", C.c_int, [_P, C.POINTER(fdm_profile_t)]),
 ]
 
-# entry points of include/fdmap.h that only the MI355X library implements
-NOALLOC_ENTRY_POINTS = []
+# entry points of include/fdmap.h that only the MI355X library implements:
+# the allocation-free variant of fdm_max_slip_velocity
+NOALLOC_ENTRY_POINTS = [
+    ("fdm_max_slip_velocity_noalloc", C.c_int, [_P, _dp]),
+]
 
 
 # entry points declared in include/fdmap_ensemble.h (the MI355X library only)

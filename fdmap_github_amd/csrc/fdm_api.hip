@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -829,6 +830,32 @@ int vmax_reduce(fdm_ctx* c, double* red, double* vmax) {
   return FDM_OK;
 }
 
+// Device memory of fdm_max_slip_velocity_noalloc, which allocates nothing:
+// the result and up to kNoallocParts partial maxima (kNoallocParts *
+// kVmaxWG frictional nodes at most), 520 bytes per device, held until the
+// process ends.  Host code never passes the array's name to a kernel or a
+// copy: it uses the device address that fdm_create resolves once per device
+// (g_scratch_ptr).  Every call on a device shares that device's copy, and
+// each call overwrites every element it reads, so no state carries from one
+// call to the next; calls that overlap on one device race.
+constexpr int kNoallocParts = 64;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[1 + kNoallocParts];
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// resolve the device address of scratch on device dev (current on entry)
+int resolve_scratch(int dev) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxDevices) return FDM_EDEVICE;
+  if (g_scratch_ptr[dev]) return FDM_OK;
+  void* ptr = nullptr;
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return FDM_EDEVICE;
+  g_scratch_ptr[dev] = (double*)ptr;
+  return FDM_OK;
+}
+
 int check_fail(fdm_ctx* c) {
   if (!c->fail) return FDM_OK;
   int h = 0;
@@ -1528,6 +1555,11 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
       const size_t np = (fn.size() + kVmaxWG - 1) / kVmaxWG;
       if ((rc = dalloc(c, &c->vred, (1 + np) * sizeof(double)))) return fail(rc);
     }
+    // the allocation-free reduction's device scratch on this device
+    if (resolve_scratch(c->device) != FDM_OK) {
+      set_err(c, FDM_EDEVICE, "fdm_create: cannot resolve the device scratch address");
+      return fail(FDM_EDEVICE);
+    }
   }
   *out = c;
   return FDM_OK;
@@ -1744,6 +1776,22 @@ int fdm_set_pml(fdm_ctx* c, const double* Wx, const double* Wy) {
 int fdm_max_slip_velocity(fdm_ctx* c, double* vmax) {
   if (!c || !vmax) return FDM_EINVAL;
   return vmax_reduce(c, c->vred, vmax);
+}
+
+int fdm_max_slip_velocity_noalloc(fdm_ctx* c, double* vmax) {
+  if (!c || !vmax) return FDM_EINVAL;
+  if (c->nfnode > kNoallocParts * kVmaxWG)
+    return set_err(c, FDM_EINVAL,
+                   "fdm_max_slip_velocity_noalloc: " + std::to_string(c->nfnode) +
+                       " frictional interface nodes exceed its limit of " +
+                       std::to_string(kNoallocParts * kVmaxWG) + " (use fdm_max_slip_velocity)");
+  double* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (c->device >= 0 && c->device < kMaxDevices) ptr = g_scratch_ptr[c->device];
+  }
+  if (!ptr) return set_err(c, FDM_EDEVICE, "fdm_max_slip_velocity_noalloc: device scratch unresolved");
+  return vmax_reduce(c, ptr, vmax);
 }
 
 int fdm_sync(fdm_ctx* c) {

@@ -129,9 +129,12 @@ class Context:
     def sync(self):
         self._check(self._fn("sync")(self.h))
 
-    def max_slip_velocity(self):
+    def max_slip_velocity(self, noalloc=False):
+        """max of V over the frictional interfaces; noalloc: through the
+        entry point that allocates nothing (MI355X library only)"""
         v = C.c_double()
-        self._check(self._fn("max_slip_velocity")(self.h, C.byref(v)))
+        name = "max_slip_velocity_noalloc" if noalloc else "max_slip_velocity"
+        self._check(self._fn(name)(self.h, C.byref(v)))
         return v.value
 
     def profile_enable(self, on=True):

@@ -307,6 +307,17 @@ int fdm_set_pml(fdm_ctx* ctx, const double* Wx, const double* Wy);
    buffer that fdm_create allocates with the context. */
 int fdm_max_slip_velocity(fdm_ctx* ctx, double* vmax);
 
+/* The same maximum, bit for bit, from a call that allocates nothing: its
+   partial maxima live in a fixed device array of the library (520 bytes per
+   device, held until the process ends) instead of context memory.  No
+   counterpart in the reference: it serves callers that cannot allocate
+   device memory (tight serving loops, harnesses that forbid allocation).
+   Limit: 16384 frictional interface nodes per context (FDM_EINVAL beyond,
+   use fdm_max_slip_velocity).  Every call on a device shares that device's
+   array, so calls that overlap on one device race; consecutive calls do
+   not depend on each other. */
+int fdm_max_slip_velocity_noalloc(fdm_ctx* ctx, double* vmax);
+
 /* Wait for all queued work. */
 int fdm_sync(fdm_ctx* ctx);
 
