@@ -852,9 +852,10 @@ int fdm_adjoint_create(fdm_ctx* c, int32_t nrec, const int32_t* rec_i, const int
   if (!c || !out || nsteps < 1 || nrec < 0 || (nrec && (!rec_i || !rec_j)))
     return adj_err(nullptr, FDM_EINVAL, "fdm_adjoint_create: invalid arguments");
   *out = nullptr;
-  if (c->precision != FDM_FP64 || c->any_plastic || c->het || c->diss || c->pml || c->mode != 2)
+  if (c->precision != FDM_FP64 || c->any_plastic || c->het || c->diss || c->pml || c->mode != 2 ||
+      c->sbp != 4)
     return adj_err(nullptr, FDM_EINVAL,
-                   "fdm_adjoint_create: needs an FP64 plane-strain context without plasticity, "
+                   "fdm_adjoint_create: needs an FP64 plane-strain SBP4 context without plasticity, "
                    "heterogeneous medium, artificial dissipation or PML");
   for (const BlockDev& B : c->hblocks)
     if (B.px - B.mx + 1 < 12 || B.py - B.my + 1 < 12)

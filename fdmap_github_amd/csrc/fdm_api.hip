@@ -10,6 +10,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fdm_bulk.hpp"
@@ -249,6 +250,9 @@ struct fdm_ctx {
   long long nlayer_inner = 0;  // lpts[0] entries that are points (not padding)
   // 2: plane strain, 6 fields; 3: antiplane shear, 3 fields
   int mode = 2, nF = 6;
+  // first-derivative operator SBP2/4/6 (fd_coeff.f90:41-248) and its
+  // closure depth nbst: points nearer a block edge take the one-sided rows
+  int sbp = 4, nbst = 4;
   std::vector<double> hnu;  // nu of the medium ([ny][nx]), for restore_df6
   // bulk kernel: 0 per-point, 2 interior tiles (kTileShape[tile_shape]) + band points
   int bulk_mode = 2;
@@ -528,21 +532,28 @@ bool full_variant(const fdm_ctx* c, const BulkArgs<T>& a) {
   return c->src.nplace > 0 || c->any_mms || a.do_plastic;
 }
 
+template <typename T, int OP>
+void launch_band_op(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid) {
+  if (c->mode == 3)
+    hipLaunchKernelGGL((bulk_band3<T, OP>), grid, dim3(64), 0, s, a, c->band, c->nband);
+  else if (full_variant(c, a) && (c->het || c->diss))
+    hipLaunchKernelGGL((bulk_band<T, true, true, OP>), grid, dim3(64), 0, s, a, c->band, c->nband);
+  else if (full_variant(c, a))
+    hipLaunchKernelGGL((bulk_band<T, true, false, OP>), grid, dim3(64), 0, s, a, c->band, c->nband);
+  else if (c->het || c->diss)
+    hipLaunchKernelGGL((bulk_band<T, false, true, OP>), grid, dim3(64), 0, s, a, c->band, c->nband);
+  else
+    hipLaunchKernelGGL((bulk_band<T, false, false, OP>), grid, dim3(64), 0, s, a, c->band, c->nband);
+}
+
 template <typename T>
 void launch_band(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   if (!c->nband) return;
   EventPair* ep = prof_begin(c, 2, s);
   dim3 grid((c->nband + 63) / 64);
-  if (c->mode == 3)
-    hipLaunchKernelGGL(bulk_band3<T>, grid, dim3(64), 0, s, a, c->band, c->nband);
-  else if (full_variant(c, a) && (c->het || c->diss))
-    hipLaunchKernelGGL((bulk_band<T, true, true>), grid, dim3(64), 0, s, a, c->band, c->nband);
-  else if (full_variant(c, a))
-    hipLaunchKernelGGL((bulk_band<T, true, false>), grid, dim3(64), 0, s, a, c->band, c->nband);
-  else if (c->het || c->diss)
-    hipLaunchKernelGGL((bulk_band<T, false, true>), grid, dim3(64), 0, s, a, c->band, c->nband);
-  else
-    hipLaunchKernelGGL((bulk_band<T, false, false>), grid, dim3(64), 0, s, a, c->band, c->nband);
+  if (c->sbp == 2) launch_band_op<T, 2>(c, s, a, grid);
+  else if (c->sbp == 6) launch_band_op<T, 6>(c, s, a, grid);
+  else launch_band_op<T, 4>(c, s, a, grid);
   prof_end(c, ep, 2, s);
 }
 
@@ -568,6 +579,50 @@ void launch_layer(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   if (c->nlpts[1]) launch_layer_list<T, false>(c, s, a, 1);
 }
 
+template <typename T, int OP>
+void launch_interior3(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid, dim3 block) {
+  const int4* tl = c->tiles[c->tile_shape];
+  if (c->nt_store)
+    hipLaunchKernelGGL((bulk_interior3<T, 64, 4, true, OP>), grid, block, 0, s, a, tl);
+  else
+    hipLaunchKernelGGL((bulk_interior3<T, 64, 4, false, OP>), grid, block, 0, s, a, tl);
+}
+
+// one interior tile variant: FULL (sources, MMS, plasticity), or lean with
+// non-temporal or normal stores
+template <typename T, int TX, int TY, bool HET, bool DISS, int OP, bool JI>
+void launch_interior_v(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid, dim3 block,
+                       bool full) {
+  const int4* tl = c->tiles[c->tile_shape];
+  if (full)
+    hipLaunchKernelGGL((bulk_interior<T, TX, TY, true, false, HET, DISS, OP, JI>), grid, block, 0, s, a, tl);
+  else if (c->nt_store)
+    hipLaunchKernelGGL((bulk_interior<T, TX, TY, false, true, HET, DISS, OP, JI>), grid, block, 0, s, a, tl);
+  else
+    hipLaunchKernelGGL((bulk_interior<T, TX, TY, false, false, HET, DISS, OP, JI>), grid, block, 0, s, a, tl);
+}
+
+// the JI variants (Ji formed in the kernel) exist for FP64 only: fdm_create
+// sets ji_calc in FP64 alone
+template <typename T, int TX, int TY, bool HET, bool DISS, int OP>
+void launch_interior_j(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid, dim3 block,
+                       bool full) {
+  if constexpr (std::is_same<T, double>::value) {
+    if (c->ji_calc) {
+      launch_interior_v<T, TX, TY, HET, DISS, OP, true>(c, s, a, grid, block, full);
+      return;
+    }
+  }
+  launch_interior_v<T, TX, TY, HET, DISS, OP, false>(c, s, a, grid, block, full);
+}
+
+// the 64 x 4 interior tiles of SBP2 or SBP6 (no dissipation)
+template <typename T, int OP, bool HET>
+void launch_interior_op(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid, dim3 block,
+                        bool full) {
+  launch_interior_j<T, 64, 4, HET, false, OP>(c, s, a, grid, block, full);
+}
+
 template <typename T>
 void launch_interior(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   const int q = c->tile_shape;
@@ -580,34 +635,33 @@ void launch_interior(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   if (c->mode == 3) {
     // antiplane shear runs the 64 x 4 tiles (fdm_create keeps tile_shape 0)
     c->df6_stale = false;
-    if (c->nt_store)
-      hipLaunchKernelGGL((bulk_interior3<T, 64, 4, true>), grid, block, 0, s, a, c->tiles[q]);
-    else
-      hipLaunchKernelGGL((bulk_interior3<T, 64, 4, false>), grid, block, 0, s, a, c->tiles[q]);
+    if (c->sbp == 2) launch_interior3<T, 2>(c, s, a, grid, block);
+    else if (c->sbp == 6) launch_interior3<T, 6>(c, s, a, grid, block);
+    else launch_interior3<T, 4>(c, s, a, grid, block);
     prof_end(c, ep, 0, s);
     return;
   }
-#define FDM_INTERIOR(TX_, TY_, HET_, DISS_)                                                          \
-  do {                                                                                               \
-    if (full)                                                                                        \
-      hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, true, false, HET_, DISS_>), grid, block, 0, s, a, \
-                         c->tiles[q]);                                                               \
-    else if (c->nt_store)                                                                            \
-      hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, false, true, HET_, DISS_>), grid, block, 0, s, a, \
-                         c->tiles[q]);                                                               \
-    else                                                                                             \
-      hipLaunchKernelGGL((bulk_interior<T, TX_, TY_, false, false, HET_, DISS_>), grid, block, 0, s, a, \
-                         c->tiles[q]);                                                               \
-  } while (0)
+  if (c->sbp != 4) {
+    // SBP2 and SBP6 run the 64 x 4 tiles (fdm_create and fdm_tune keep
+    // tile_shape 0); neither has dissipation
+    if (c->sbp == 2) {
+      if (c->het) launch_interior_op<T, 2, true>(c, s, a, grid, block, full);
+      else launch_interior_op<T, 2, false>(c, s, a, grid, block, full);
+    } else {
+      if (c->het) launch_interior_op<T, 6, true>(c, s, a, grid, block, full);
+      else launch_interior_op<T, 6, false>(c, s, a, grid, block, full);
+    }
+    prof_end(c, ep, 0, s);
+    return;
+  }
   // a heterogeneous medium and artificial dissipation run the 64 x 4 tiles
   // only (fdm_create and fdm_tune keep tile_shape 0)
-  if (c->het && c->diss) FDM_INTERIOR(64, 4, true, true);
-  else if (c->het) FDM_INTERIOR(64, 4, true, false);
-  else if (c->diss) FDM_INTERIOR(64, 4, false, true);
-  else if (q == 0) FDM_INTERIOR(64, 4, false, false);
-  else if (q == 1) FDM_INTERIOR(64, 8, false, false);
-  else FDM_INTERIOR(128, 4, false, false);
-#undef FDM_INTERIOR
+  if (c->het && c->diss) launch_interior_j<T, 64, 4, true, true, 4>(c, s, a, grid, block, full);
+  else if (c->het) launch_interior_j<T, 64, 4, true, false, 4>(c, s, a, grid, block, full);
+  else if (c->diss) launch_interior_j<T, 64, 4, false, true, 4>(c, s, a, grid, block, full);
+  else if (q == 0) launch_interior_j<T, 64, 4, false, false, 4>(c, s, a, grid, block, full);
+  else if (q == 1) launch_interior_j<T, 64, 8, false, false, 4>(c, s, a, grid, block, full);
+  else launch_interior_j<T, 128, 4, false, false, 4>(c, s, a, grid, block, full);
   prof_end(c, ep, 0, s);
 }
 
@@ -617,10 +671,15 @@ void launch_pointwise(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   EventPair* ep = prof_begin(c, 0, s);
   dim3 block(64, c->tile_y);
   dim3 grid((c->nx + 63) / 64, (c->ny + c->tile_y - 1) / c->tile_y);
-  if (c->mode == 3)
-    hipLaunchKernelGGL(bulk_kernel3<T>, grid, block, 0, s, a);
-  else
-    hipLaunchKernelGGL(bulk_kernel<T>, grid, block, 0, s, a);
+  if (c->mode == 3) {
+    if (c->sbp == 2) hipLaunchKernelGGL((bulk_kernel3<T, 2>), grid, block, 0, s, a);
+    else if (c->sbp == 6) hipLaunchKernelGGL((bulk_kernel3<T, 6>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((bulk_kernel3<T, 4>), grid, block, 0, s, a);
+  } else {
+    if (c->sbp == 2) hipLaunchKernelGGL((bulk_kernel<T, 2>), grid, block, 0, s, a);
+    else if (c->sbp == 6) hipLaunchKernelGGL((bulk_kernel<T, 6>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((bulk_kernel<T, 4>), grid, block, 0, s, a);
+  }
   prof_end(c, ep, 0, s);
 }
 
@@ -749,8 +808,8 @@ void restore_df6(const fdm_ctx* c, T* h) {
   const size_t pe = (size_t)c->ps;
   for (int b = 0; b < c->nb; b++) {
     const BlockDev& B = c->hblocks[b];
-    for (int j = B.my + 4; j <= B.py - 4; j++)
-      for (int i = B.mx + 4; i <= B.px - 4; i++) {
+    for (int j = B.my + c->nbst; j <= B.py - c->nbst; j++)
+      for (int i = B.mx + c->nbst; i <= B.px - c->nbst; i++) {
         const size_t k = (size_t)j * c->ld + i;
         const T nu = c->het ? (T)c->hnu[(size_t)j * c->nx + i] : (T)B.nu;
         h[5 * pe + k] = nu * (h[2 * pe + k] + h[4 * pe + k]);
@@ -897,10 +956,33 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     return set_err(nullptr, FDM_EINVAL, "fdm_create: malformed problem description");
   if (P->precision != FDM_FP64 && P->precision != FDM_FP32_BULK)
     return set_err(nullptr, FDM_EINVAL, "fdm_create: unknown precision");
-  if (P->sbp != 0 && P->sbp != 4)
+  if (P->sbp != 0 && P->sbp != 2 && P->sbp != 4 && P->sbp != 6)
     return set_err(nullptr, FDM_EINVAL,
-                   "fdm_create: the device kernels implement SBP4 only (FDmethod 'SBP" +
-                       std::to_string(P->sbp) + "' requested)");
+                   "fdm_create: FDmethod must be SBP2, SBP4 or SBP6 ('SBP" + std::to_string(P->sbp) +
+                       "' requested)");
+  const int sbp = P->sbp == 0 ? 4 : P->sbp;
+  // closure depth nbst and closure width nbnd (fd_coeff.f90:56-79)
+  const int nbst = sbp == 2 ? 1 : sbp == 4 ? 4 : 6, nbnd = sbp == 2 ? 2 : sbp == 4 ? 6 : 9;
+  if (sbp != 4) {
+    // the reference forms artificial dissipation and the perfectly matched
+    // layer with SBP4's tables (rates.f90:1397-2021, 2024-2619)
+    for (int b = 0; b < P->nblocks; b++)
+      if (P->blocks[b].Cdiss > 0.0 || P->blocks[b].pmlx || P->blocks[b].pmly)
+        return set_err(nullptr, FDM_EINVAL,
+                       "fdm_create: artificial dissipation and the perfectly matched layer need "
+                       "FDmethod SBP4");
+  }
+  for (int b = 0; b < P->nblocks; b++) {
+    // every point of a block lies in at most one closure per direction, and
+    // the closure rows read nbnd points into the block
+    const fdm_block& B = P->blocks[b];
+    const int n = std::min(B.px - B.mx, B.py - B.my) + 1;
+    if (n < std::max(2 * nbst, nbnd))
+      return set_err(nullptr, FDM_EINVAL,
+                     "fdm_create: block " + std::to_string(b) + " has " + std::to_string(n) +
+                         " points across; SBP" + std::to_string(sbp) + " needs at least " +
+                         std::to_string(std::max(2 * nbst, nbnd)));
+  }
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return set_err(nullptr, FDM_EDEVICE, "fdm_create: no HIP device available");
@@ -921,6 +1003,8 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
 
   fdm_ctx* c = new fdm_ctx();
   c->mode = P->mode == 3 ? 3 : 2;
+  c->sbp = sbp;
+  c->nbst = nbst;
   c->nF = c->mode == 3 ? 3 : 6;
   c->nx = P->nx;
   c->ny = P->ny;
@@ -937,12 +1021,12 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   // row pitch: a multiple of 64 elements so every row starts 256/512-byte aligned
   // Column i of a plane is stored at element coff + i, with coff chosen so
   // that the interior tiles of the first block column (which start at column
-  // mx + 4) begin on a 512-byte boundary; every plane pointer below already
+  // mx + nbst) begin on a 512-byte boundary; every plane pointer below already
   // includes the offset.
   {
     int mx0 = P->nblocks > 0 ? P->blocks[0].mx : 0;
     for (int b = 1; b < P->nblocks; b++) mx0 = std::min(mx0, P->blocks[b].mx);
-    c->coff = (64 - (mx0 + 4) % 64) % 64;
+    c->coff = (64 - (mx0 + nbst) % 64) % 64;
   }
   c->ld = (P->nx + c->coff + 63) / 64 * 64;
   c->ps = (long long)c->ld * P->ny;
@@ -1065,7 +1149,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   c->hrowb = rowb;
   c->hblk = blk;
   {
-    // interior tiles: each block's rectangle [mx+4, px-4] x [my+4, py-4]
+    // interior tiles: each block's rectangle [mx+nbst, px-nbst] x [my+nbst, py-nbst]
     // covered from its corner, x fastest; band: every other point, row-major.
     // A block with a perfectly matched layer in effect has no tiles and no
     // band: all of its points go to the layer list.
@@ -1074,7 +1158,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     for (int b = 0; b < c->nb; b++) {
       const fdm_block& B = P->blocks[b];
       if (layer(B)) continue;
-      const int ia = B.mx + 4, ib = B.px - 4, ja = B.my + 4, jb = B.py - 4;
+      const int ia = B.mx + nbst, ib = B.px - nbst, ja = B.my + nbst, jb = B.py - nbst;
       if (ia > ib || ja > jb) continue;
       c->ninterior += (long long)(ib - ia + 1) * (jb - ja + 1);
       for (int q = 0; q < 3; q++) {
@@ -1087,7 +1171,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     for (int j = 0; j < P->ny; j++)
       for (int i = 0; i < P->nx; i++) {
         const fdm_block& B = P->blocks[block_of(c, i, j)];
-        const bool edge = i - B.mx < 4 || B.px - i < 4 || j - B.my < 4 || B.py - j < 4;
+        const bool edge = i - B.mx < nbst || B.px - i < nbst || j - B.my < nbst || B.py - j < nbst;
         if (c->mode == 2 && layer(B)) {
           if (edge) lpts[1].push_back(make_int2(i, j));
         } else if (edge) {
@@ -1158,9 +1242,10 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     D.M = B.M;
     D.gamma = B.gamma;
     D.nu = B.nu;
-    // rates.f90:400-405: a(s) = DI(s)/rho, b(s) = DI(s)*G, c(s) = DI(s)*M
-    const double DIs[2] = {FDM_DI1, FDM_DI2};
-    for (int s = 0; s < 2; s++) {
+    // rates.f90:400-405: a(s) = DI(s)/rho, b(s) = DI(s)*G, c(s) = DI(s)*M;
+    // DI(s) of the operator, zero beyond its half-width (fd_coeff.f90:56-79)
+    const double DIs[3] = {sbp_di_host(sbp, 1), sbp_di_host(sbp, 2), sbp_di_host(sbp, 3)};
+    for (int s = 0; s < 3; s++) {
       D.cI[0][s] = DIs[s] / B.rho;
       D.cI[1][s] = DIs[s] / B.rho;
       D.cI[2][s] = DIs[s] * B.M;
@@ -1431,7 +1516,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   // (6), metrics xq xr yq yr Ji (5); a heterogeneous medium adds rho, G, M,
   // gamma, nu (5)
   c->het = P->mat_rho && P->mat_cs && P->mat_cp;
-  if (c->het || c->diss || c->mode == 3) c->tile_shape = 0;
+  if (c->het || c->diss || c->mode == 3 || c->sbp != 4) c->tile_shape = 0;
   if (c->diss && !P->J) {
     set_err(c, FDM_EINVAL, "fdm_create: artificial dissipation needs the Jacobian J");
     return fail(FDM_EINVAL);
@@ -1834,9 +1919,10 @@ int fdm_profile_read(fdm_ctx* c, fdm_profile_t* out) {
 int fdm_tune(fdm_ctx* c, int32_t knob, int32_t value) {
   if (!c) return FDM_EINVAL;
   if (knob == FDM_TUNE_TILE && value >= 0 && value < 3) {
-    if ((c->het || c->diss) && value != 0)
+    if ((c->het || c->diss || c->mode == 3 || c->sbp != 4) && value != 0)
       return set_err(c, FDM_EINVAL,
-                     "fdm_tune: a heterogeneous medium or artificial dissipation runs 64x4 tiles only");
+                     "fdm_tune: a heterogeneous medium, artificial dissipation, antiplane shear and "
+                     "SBP2/SBP6 run 64x4 tiles only");
     c->tile_shape = value;
     return FDM_OK;
   }
@@ -2080,12 +2166,12 @@ int fdm_ensemble_create(fdm_ctx* const* members, int32_t n, fdm_ensemble** out) 
     if (!c) return FDM_EINVAL;
     const fdm_ctx* c0 = members[0];
     if (c->src.nplace > 0 || c->any_mms || (!c->ld_T.empty() && c->ld_T[0] > 0.0) || c->bulk_mode != 2 ||
-        c->het || c->diss || c->pml || c->mode != 2 || c->nstage != c0->nstage || c->precision != c0->precision ||
-        c->device != c0->device) {
+        c->het || c->diss || c->pml || c->mode != 2 || c->sbp != 4 || c->nstage != c0->nstage ||
+        c->precision != c0->precision || c->device != c0->device) {
       g_create_error =
           "fdm_ensemble_create: members must have no point source, no manufactured solution, "
-          "no load ramp, no heterogeneous medium, no artificial dissipation, no PML, plane strain, the tile kernels, "
-          "and the same RK order, precision and device";
+          "no load ramp, no heterogeneous medium, no artificial dissipation, no PML, plane strain, SBP4, "
+          "the tile kernels, and the same RK order, precision and device";
       return FDM_EINVAL;
     }
     for (int q = 0; q < k; q++)

@@ -51,9 +51,9 @@ struct BulkArgs {
   // PML auxiliary fields: Wx (planes 0-5), Wy (6-11), DWx (12-17), DWy
   // (18-23), plane stride ps (null without a layer)
   T* W;
-  // lean interior launches: ji_calc forms the point's Ji as 1/(xq yr - xr yq)
-  // from the staged metrics instead of reading the Ji plane (set when the
-  // context's Ji equals that expression bit for bit at every point);
+  // ji_calc: the context's Ji equals 1/(xq yr - xr yq) bit for bit at every
+  // point, so the interior tiles may form it from the staged metrics instead
+  // of reading the Ji plane (the host then launches their JI variant);
   // skip_df leaves DF1..5 unstored (the last stage of a step that another
   // step of the same call follows: the next stage scales DF by A = 0 and
   // never reads it)
@@ -197,11 +197,13 @@ __device__ __forceinline__ void interior_term(T* d, const Acc& A, int s, T gamma
   d[l] = d[l] + cI.template scale<l>(t, s, Ji);
 }
 
-// cI(l, s) returns the interior coefficient of field l and offset s
-template <typename T, typename Acc, typename Coef>
+// cI(l, s) returns the interior coefficient of field l and offset s; H is
+// the operator's interior half-width (terms of larger s carry a zero
+// coefficient in the reference's s = 1..3 loop and add nothing)
+template <typename T, typename Acc, typename Coef, int H = 2>
 __device__ __forceinline__ void interior_rates(T* d, const Acc& A, T gamma, const Coef& cI, T Ji) {
 #pragma unroll
-  for (int s = 1; s <= 2; s++) {
+  for (int s = 1; s <= H; s++) {
     const T ayp = (T)0.5 * (A.yr(s) + A.yr(0)), axp = (T)0.5 * (A.xr(s) + A.xr(0));
     const T aym = (T)0.5 * (A.yr(-s) + A.yr(0)), axm = (T)0.5 * (A.xr(-s) + A.xr(0));
     const T byp = (T)0.5 * (A.yq(s) + A.yq(0)), bxp = (T)0.5 * (A.xq(s) + A.xq(0));
@@ -273,12 +275,12 @@ struct BlockCoef {
 // the material of one point of a heterogeneous medium:
 // ")*DI(s)*JacInv/rho", ")*DI(s)*JacInv*M", ")*DI(s)*JacInv*G"
 // (rates_heterogeneous.f90:392-428)
-template <typename T>
+template <typename T, int OP = 4>
 struct PointCoef {
   T rho, G, M;
   template <int l>
   __device__ __forceinline__ T scale(T t, int s, T Ji) const {
-    const T u = t * (s == 1 ? (T)FDM_DI1 : (T)FDM_DI2) * Ji;
+    const T u = t * (T)sbp_di<OP>(s) * Ji;
     if constexpr (l < 2) return u / rho;
     else if constexpr (l == 3) return u * G;
     else return u * M;
@@ -287,7 +289,7 @@ struct PointCoef {
   // dissipative form (rates_heterogeneous.f90:1409-1450)
   template <int l>
   __device__ __forceinline__ T coef(int s) const {
-    const T D = s == 1 ? (T)FDM_DI1 : (T)FDM_DI2;
+    const T D = (T)sbp_di<OP>(s);
     if constexpr (l < 2) return D / rho;
     else if constexpr (l == 3) return D * G;
     else return D * M;
@@ -525,12 +527,13 @@ __device__ __forceinline__ void diss_central(T (&d)[6], const T* __restrict__ F,
 // that all of a point's loads are in flight together: the band is a thin
 // set of points, latency-bound.  EXT = false leaves out the heterogeneous
 // medium and dissipation code (launches of problems with neither).
-template <typename T, bool EXT>
+template <typename T, bool EXT, int OP = 4>
 __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockDev& B, const int i,
                                               const int j, const long long cc, T (&d)[6]) {
   const long long ps = a.ps;
   const int ld = a.ld;
   const T* __restrict__ F = a.Fin;
+  constexpr int NB = Sbp<OP>::nbst, NS = Sbp<OP>::nbnd, HW = Sbp<OP>::half;
   const int dl = i - B.mx, dr = B.px - i, db = j - B.my, dtp = B.py - j;
   // SAT penalties, sides in the order L, R, B, T
   if (dl == 0 || dr == 0 || db == 0 || dtp == 0) {
@@ -552,76 +555,78 @@ __device__ __forceinline__ void closure_rates(const BulkArgs<T>& a, const BlockD
   const bool het = EXT && a.mrho != nullptr;
   const T gamma = het ? a.mgam[cc] : (T)B.gamma, rho = het ? a.mrho[cc] : (T)B.rho;
   const T G = het ? a.mG[cc] : (T)B.G, M = het ? a.mM[cc] : (T)B.M;
-  const bool diss = EXT && B.Cdiss > 0.0;
+  // artificial dissipation is built for SBP4 only (fdm_create refuses it
+  // with the other operators)
+  const bool diss = EXT && OP == 4 && B.Cdiss > 0.0;
   const T Cd = (T)B.Cdiss;
   const T* __restrict__ J = a.J;
-  const bool qone = (dl < 4) || (dr < 4);
-  const bool rone = (db < 4) || (dtp < 4);
+  const bool qone = (dl < NB) || (dr < NB);
+  const bool rone = (db < NB) || (dtp < NB);
   const long long row = (long long)j * ld;
   const long long colm = (long long)B.my * ld + i, colp = (long long)B.py * ld + i;
   if (qone) {
     // one-sided q: L or R strip, or q part of a corner
 #pragma unroll
-    for (int s = 0; s < 6; s++) {
-      long long kq = (dl < 4) ? row + B.mx + s : row + B.px - s;
-      T Dq = (dl < 4) ? (T)kDL[dl][s] : (T)(-kDL[dr][s]);
+    for (int s = 0; s < NS; s++) {
+      long long kq = (dl < NB) ? row + B.mx + s : row + B.px - s;
+      T Dq = (dl < NB) ? (T)sbp_dl<OP>(dl, s) : (T)(-sbp_dl<OP>(dr, s));
       d[0] = d[0] + bscale<T, 0>(q1<T, 0>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
       d[1] = d[1] + bscale<T, 0>(q1<T, 1>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
       d[2] = d[2] + bscale<T, 2>(q1<T, 2>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
       d[3] = d[3] + bscale<T, 1>(q1<T, 3>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
       d[4] = d[4] + bscale<T, 2>(q1<T, 4>(F, yr, xr, ps, kq, cc, gamma), Dq, rho, G, M) * Ji;
-      if (diss) diss_one<T>(d, F, J, ps, kq, (T)((dl < 4) ? kAL[dl][s] : kAL[dr][s]), Ji, Cd);
+      if constexpr (OP == 4) if (diss) diss_one<T>(d, F, J, ps, kq, (T)((dl < 4) ? kAL[dl][s] : kAL[dr][s]), Ji, Cd);
       if (rone) {
         // corner: the r term of the same s follows
-        long long kr = (db < 4) ? colm + (long long)s * ld : colp - (long long)s * ld;
-        T Dr = (db < 4) ? (T)kDL[db][s] : (T)(-kDL[dtp][s]);
+        long long kr = (db < NB) ? colm + (long long)s * ld : colp - (long long)s * ld;
+        T Dr = (db < NB) ? (T)sbp_dl<OP>(db, s) : (T)(-sbp_dl<OP>(dtp, s));
         d[0] = d[0] + bscale<T, 0>(r1<T, 0>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
         d[1] = d[1] + bscale<T, 0>(r1<T, 1>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
         d[2] = d[2] + bscale<T, 2>(r1<T, 2>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
         d[3] = d[3] + bscale<T, 1>(r1<T, 3>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
         d[4] = d[4] + bscale<T, 2>(r1<T, 4>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
-        if (diss) diss_one<T>(d, F, J, ps, kr, (T)((db < 4) ? kAL[db][s] : kAL[dtp][s]), Ji, Cd);
+        if constexpr (OP == 4) if (diss) diss_one<T>(d, F, J, ps, kr, (T)((db < 4) ? kAL[db][s] : kAL[dtp][s]), Ji, Cd);
       }
     }
     if (!rone) {
       // L or R strip: central r
-      if (diss) diss_diag<T>(d, F, ps, cc, Cd);
+      if constexpr (OP == 4) if (diss) diss_diag<T>(d, F, ps, cc, Cd);
 #pragma unroll
-      for (int s = 1; s <= 2; s++) {
-        T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
+      for (int s = 1; s <= HW; s++) {
+        T D = (T)sbp_di<OP>(s);
         long long kp = cc + (long long)s * ld, km = cc - (long long)s * ld;
         d[0] = d[0] + bscale<T, 0>(rc<T, 0>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
         d[1] = d[1] + bscale<T, 0>(rc<T, 1>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
         d[2] = d[2] + bscale<T, 2>(rc<T, 2>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
         d[3] = d[3] + bscale<T, 1>(rc<T, 3>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
         d[4] = d[4] + bscale<T, 2>(rc<T, 4>(F, yq, xq, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
-        if (diss) diss_central<T>(d, F, J, ps, kp, km, (s == 1) ? (T)FDM_AI1 : (T)FDM_AI2, Ji, Cd);
+        if constexpr (OP == 4) if (diss) diss_central<T>(d, F, J, ps, kp, km, (s == 1) ? (T)FDM_AI1 : (T)FDM_AI2, Ji, Cd);
       }
     }
   } else {
     // B or T strip: one-sided r, then central q
 #pragma unroll
-    for (int s = 0; s < 6; s++) {
-      long long kr = (db < 4) ? colm + (long long)s * ld : colp - (long long)s * ld;
-      T Dr = (db < 4) ? (T)kDL[db][s] : (T)(-kDL[dtp][s]);
+    for (int s = 0; s < NS; s++) {
+      long long kr = (db < NB) ? colm + (long long)s * ld : colp - (long long)s * ld;
+      T Dr = (db < NB) ? (T)sbp_dl<OP>(db, s) : (T)(-sbp_dl<OP>(dtp, s));
       d[0] = d[0] + bscale<T, 0>(r1<T, 0>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
       d[1] = d[1] + bscale<T, 0>(r1<T, 1>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
       d[2] = d[2] + bscale<T, 2>(r1<T, 2>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
       d[3] = d[3] + bscale<T, 1>(r1<T, 3>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
       d[4] = d[4] + bscale<T, 2>(r1<T, 4>(F, yq, xq, ps, kr, cc, gamma), Dr, rho, G, M) * Ji;
-      if (diss) diss_one<T>(d, F, J, ps, kr, (T)((db < 4) ? kAL[db][s] : kAL[dtp][s]), Ji, Cd);
+      if constexpr (OP == 4) if (diss) diss_one<T>(d, F, J, ps, kr, (T)((db < 4) ? kAL[db][s] : kAL[dtp][s]), Ji, Cd);
     }
-    if (diss) diss_diag<T>(d, F, ps, cc, Cd);
+    if constexpr (OP == 4) if (diss) diss_diag<T>(d, F, ps, cc, Cd);
 #pragma unroll
-    for (int s = 1; s <= 2; s++) {
-      T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
+    for (int s = 1; s <= HW; s++) {
+      T D = (T)sbp_di<OP>(s);
       long long kp = cc + s, km = cc - s;
       d[0] = d[0] + bscale<T, 0>(qc<T, 0>(F, yr, xr, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
       d[1] = d[1] + bscale<T, 0>(qc<T, 1>(F, yr, xr, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
       d[2] = d[2] + bscale<T, 2>(qc<T, 2>(F, yr, xr, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
       d[3] = d[3] + bscale<T, 1>(qc<T, 3>(F, yr, xr, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
       d[4] = d[4] + bscale<T, 2>(qc<T, 4>(F, yr, xr, ps, kp, km, cc, gamma), D, rho, G, M) * Ji;
-      if (diss) diss_central<T>(d, F, J, ps, kp, km, (s == 1) ? (T)FDM_AI1 : (T)FDM_AI2, Ji, Cd);
+      if constexpr (OP == 4) if (diss) diss_central<T>(d, F, J, ps, kp, km, (s == 1) ? (T)FDM_AI1 : (T)FDM_AI2, Ji, Cd);
     }
   }
 }
@@ -849,33 +854,47 @@ __device__ __forceinline__ void scaled_rates(const BulkArgs<T>& a, const long lo
 // neighbours straight from global memory: the per-point kernel, and the
 // closure points of the tile and marching kernels.  EXT as in closure_rates,
 // and also carries the perfectly matched layer.
-template <typename T, bool FULL, bool EXT>
+template <typename T, bool FULL, bool EXT, int OP = 4>
 __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i, const int j) {
   const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
   const BlockDev& B = a.blocks[bi];
   const long long ps = a.ps;
   const int ld = a.ld;
   const long long cc = (long long)j * ld + i;
+  constexpr int NB = Sbp<OP>::nbst, HW = Sbp<OP>::half;
   T d[6];
   scaled_rates<T>(a, cc, d);
-  if (EXT && (B.pmlx | B.pmly)) {
-    pml_rates<T>(a, B, i, j, cc, d);
-  } else if (i - B.mx >= 4 && B.px - i >= 4 && j - B.my >= 4 && B.py - j >= 4) {
-    const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld, a.J};
-    if (EXT && B.Cdiss > 0.0) {
+  bool done = false;
+  if constexpr (OP == 4) {
+    // the perfectly matched layer and artificial dissipation are built for
+    // SBP4 only (fdm_create refuses them with the other operators)
+    if (EXT && (B.pmlx | B.pmly)) {
+      pml_rates<T>(a, B, i, j, cc, d);
+      done = true;
+    } else if (EXT && B.Cdiss > 0.0 && i - B.mx >= 4 && B.px - i >= 4 && j - B.my >= 4 &&
+               B.py - j >= 4) {
+      const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld, a.J};
       const T dA[3] = {(T)B.dA[0], (T)B.dA[1], (T)B.dA[2]};
       if (a.mrho)
         interior_rates_diss<T>(d, acc, a.mgam[cc], PointCoef<T>{a.mrho[cc], a.mG[cc], a.mM[cc]}, dA,
                                a.Ji[cc]);
       else
         interior_rates_diss<T>(d, acc, (T)B.gamma, BlockCoef<T, const BlockDev&>{B}, dA, a.Ji[cc]);
-    } else if (EXT && a.mrho) {
-      interior_rates<T>(d, acc, a.mgam[cc], PointCoef<T>{a.mrho[cc], a.mG[cc], a.mM[cc]}, a.Ji[cc]);
+      done = true;
+    }
+  }
+  if (done) {
+  } else if (i - B.mx >= NB && B.px - i >= NB && j - B.my >= NB && B.py - j >= NB) {
+    const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld, a.J};
+    if (EXT && a.mrho) {
+      interior_rates<T, GlobalAcc<T>, PointCoef<T, OP>, HW>(
+          d, acc, a.mgam[cc], PointCoef<T, OP>{a.mrho[cc], a.mG[cc], a.mM[cc]}, a.Ji[cc]);
     } else {
-      interior_rates<T>(d, acc, (T)B.gamma, BlockCoef<T, const BlockDev&>{B}, a.Ji[cc]);
+      interior_rates<T, GlobalAcc<T>, BlockCoef<T, const BlockDev&>, HW>(
+          d, acc, (T)B.gamma, BlockCoef<T, const BlockDev&>{B}, a.Ji[cc]);
     }
   } else {
-    closure_rates<T, EXT>(a, B, i, j, cc, d);
+    closure_rates<T, EXT, OP>(a, B, i, j, cc, d);
   }
   T Fc[6];
 #pragma unroll
@@ -884,14 +903,14 @@ __device__ __forceinline__ void point_generic(const BulkArgs<T>& a, const int i,
 }
 
 // per-point kernel: one lane per grid point, 64 x blockDim.y tiles
-template <typename T>
+template <typename T, int OP = 4>
 __global__ void __launch_bounds__(256) bulk_kernel(BulkArgs<T> a) {
   int tx, ty;
   tile_of(a.xcd_swizzle, tx, ty);
   const int i = tx * blockDim.x + threadIdx.x;
   const int j = ty * blockDim.y + threadIdx.y;
   if (i >= a.nx || j >= a.ny) return;
-  point_generic<T, true, true>(a, i, j);
+  point_generic<T, true, true, OP>(a, i, j);
 }
 
 // global load at a 32-bit byte offset from a wave-uniform base, so that the
@@ -918,19 +937,21 @@ __device__ __forceinline__ void st_sel(T* base, unsigned boff, T v) {
 // neighbourhood of the interior kernel: everything from LDS.  sF: F1..F5
 // with a halo of 2 in q and r; sR: xr, yr with a halo in q; sQ: xq, yq with
 // a halo in r.  (x, y): the lane's point in tile coordinates.
-template <typename T, int TX, int TY>
+template <typename T, int TX, int TY, int H = 2>
 struct TileAcc {
-  const T (*sF)[TY + 4][TX + 4];
-  const T (*sR)[TY][TX + 4];
-  const T (*sQ)[TY + 4][TX];
+  const T (*sF)[TY + 2 * H][TX + 2 * H];
+  const T (*sR)[TY][TX + 2 * H];
+  const T (*sQ)[TY + 2 * H][TX];
   int x, y;
-  const T* sJ;  // [TY + 4][TX + 4], with artificial dissipation only
-  __device__ __forceinline__ T f(int k, int di, int dj) const { return sF[k][y + 2 + dj][x + 2 + di]; }
-  __device__ __forceinline__ T j(int di, int dj) const { return sJ[(y + 2 + dj) * (TX + 4) + x + 2 + di]; }
-  __device__ __forceinline__ T xr(int di) const { return sR[0][y][x + 2 + di]; }
-  __device__ __forceinline__ T yr(int di) const { return sR[1][y][x + 2 + di]; }
-  __device__ __forceinline__ T xq(int dj) const { return sQ[0][y + 2 + dj][x]; }
-  __device__ __forceinline__ T yq(int dj) const { return sQ[1][y + 2 + dj][x]; }
+  const T* sJ;  // [TY + 2H][TX + 2H], with artificial dissipation only
+  __device__ __forceinline__ T f(int k, int di, int dj) const { return sF[k][y + H + dj][x + H + di]; }
+  __device__ __forceinline__ T j(int di, int dj) const {
+    return sJ[(y + H + dj) * (TX + 2 * H) + x + H + di];
+  }
+  __device__ __forceinline__ T xr(int di) const { return sR[0][y][x + H + di]; }
+  __device__ __forceinline__ T yr(int di) const { return sR[1][y][x + H + di]; }
+  __device__ __forceinline__ T xq(int dj) const { return sQ[0][y + H + dj][x]; }
+  __device__ __forceinline__ T yq(int dj) const { return sQ[1][y + H + dj][x]; }
 };
 
 typedef const __attribute__((address_space(4))) BlockDev ConstBlockDev;
@@ -945,9 +966,17 @@ typedef const __attribute__((address_space(4))) BlockDev ConstBlockDev;
 // xq/yq with the r halo go to LDS; the lane's own DF, F6 and Ji stay in
 // registers.  Halo indices are clamped into the grid; a clamped value never
 // reaches an active lane.  tiles[k] = (block, i0, j0, 0).
-template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET, bool DISS>
+// JI: form Ji from the staged metrics (a context with ji_calc set) rather
+// than read it; a compile-time switch, so that neither path holds the other's
+// registers
+template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET, bool DISS, int OP = 4,
+          bool JI = false>
 __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 td) {
-  constexpr int HX = TX + 4, HY = TY + 4, NT = TX * TY;
+  // H: the operator's interior half-width, the tile's halo; NB: the
+  // closure's depth, how far the interior rectangle stays from block edges
+  constexpr int H = Sbp<OP>::half, NB = Sbp<OP>::nbst;
+  static_assert(!DISS || OP == 4, "artificial dissipation is built for SBP4 only");
+  constexpr int HX = TX + 2 * H, HY = TY + 2 * H, NT = TX * TY;
   constexpr int NF = (HY * HX + NT - 1) / NT;  // staging passes, F1..F5
   constexpr int NR = (TY * HX + NT - 1) / NT;  // xr, yr
   constexpr int NQ = (HY * TX + NT - 1) / NT;  // xq, yq
@@ -968,7 +997,7 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     const int k = tid + p * NT;
     if (k < HY * HX) {
       const int r = k / HX, c = k - r * HX;
-      const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(max(i0 - 2 + c, 0), nx1);
+      const int gj = min(max(j0 - H + r, 0), ny1), gi = min(max(i0 - H + c, 0), nx1);
       const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
 #pragma unroll
       for (int l = 0; l < 5; l++) vF[p][l] = ld_off(F + l * ps, o);
@@ -980,7 +1009,7 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     const int k = tid + p * NT;
     if (k < TY * HX) {
       const int r = k / HX, c = k - r * HX;
-      const int gj = min(j0 + r, ny1), gi = min(max(i0 - 2 + c, 0), nx1);
+      const int gj = min(j0 + r, ny1), gi = min(max(i0 - H + c, 0), nx1);
       const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
       vR[p][0] = ld_off(a.xr, o);
       vR[p][1] = ld_off(a.yr, o);
@@ -991,14 +1020,14 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     const int k = tid + p * NT;
     if (k < HY * TX) {
       const int r = k / TX, c = k - r * TX;
-      const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(i0 + c, nx1);
+      const int gj = min(max(j0 - H + r, 0), ny1), gi = min(i0 + c, nx1);
       const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
       vQ[p][0] = ld_off(a.xq, o);
       vQ[p][1] = ld_off(a.yq, o);
     }
   }
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
-  const bool valid = i <= B4->px - 4 && j <= B4->py - 4;
+  const bool valid = i <= B4->px - NB && j <= B4->py - NB;
   const unsigned oc = (unsigned)(min(j, ny1) * ld + min(i, nx1)) * (unsigned)sizeof(T);
   T dfa[5];
   if (!a.a_zero) {
@@ -1007,7 +1036,7 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
   }
   const T f6 = ld_off(F + 5 * ps, oc);
   T Ji = (T)0.0;
-  if (!a.ji_calc) Ji = ld_off(a.Ji, oc);
+  if constexpr (!JI) Ji = ld_off(a.Ji, oc);
   // a heterogeneous medium: the lane's own material (rho, G, M, gamma, nu)
   T mat[5];
   if constexpr (HET) {
@@ -1047,11 +1076,11 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
   }
   __syncthreads();
   if (!valid) return;
-  if (a.ji_calc) {
+  if constexpr (JI) {
     // grid.f90:531-534: J = xq yr - xr yq, Ji = 1/J, from the lane's own
     // metrics (no contraction: the host's rounding)
-    const T J = sQ[0][threadIdx.y + 2][threadIdx.x] * sR[1][threadIdx.y][threadIdx.x + 2] -
-                sR[0][threadIdx.y][threadIdx.x + 2] * sQ[1][threadIdx.y + 2][threadIdx.x];
+    const T J = sQ[0][threadIdx.y + H][threadIdx.x] * sR[1][threadIdx.y][threadIdx.x + H] -
+                sR[0][threadIdx.y][threadIdx.x + H] * sQ[1][threadIdx.y + H][threadIdx.x];
     Ji = (T)1.0 / J;
   }
   T d[6];
@@ -1064,7 +1093,7 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     for (int l = 0; l < 5; l++) d[l] = A * dfa[l];
     d[5] = (T)0.0;  // DF6 is overwritten by the szz relation
   }
-  const TileAcc<T, TX, TY> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y, &sJ[0][0]};
+  const TileAcc<T, TX, TY, H> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y, &sJ[0][0]};
   bool diss_done = false;
   if constexpr (DISS) {
     // a block without dissipation keeps the plain expression
@@ -1079,9 +1108,11 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
   }
   if (!diss_done) {
     if constexpr (HET)
-      interior_rates<T>(d, acc, mat[3], PointCoef<T>{mat[0], mat[1], mat[2]}, Ji);
+      interior_rates<T, TileAcc<T, TX, TY, H>, PointCoef<T, OP>, H>(
+          d, acc, mat[3], PointCoef<T, OP>{mat[0], mat[1], mat[2]}, Ji);
     else
-      interior_rates<T>(d, acc, (T)B4->gamma, BlockCoef<T, ConstBlockDev&>{*B4}, Ji);
+      interior_rates<T, TileAcc<T, TX, TY, H>, BlockCoef<T, ConstBlockDev&>, H>(
+          d, acc, (T)B4->gamma, BlockCoef<T, ConstBlockDev&>{*B4}, Ji);
   }
   T Fc[6];
 #pragma unroll
@@ -1112,11 +1143,12 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
 
 // a heterogeneous medium holds five more values per lane, dissipation the
 // Jacobian's staging: three waves per SIMD keep them in registers
-template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET, bool DISS>
+template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET, bool DISS, int OP = 4,
+          bool JI = false>
 __global__ void __launch_bounds__(TX * TY)
-    __attribute__((amdgpu_waves_per_eu(FULL ? 2 : ((HET || DISS) ? 3 : 4))))
+    __attribute__((amdgpu_waves_per_eu(FULL ? 2 : ((HET || DISS || OP == 6) ? 3 : 4))))
 bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
-  interior_tile<T, TX, TY, FULL, NTS, HET, DISS>(a, tiles[linear_tile(a.xcd_swizzle)]);
+  interior_tile<T, TX, TY, FULL, NTS, HET, DISS, OP, JI>(a, tiles[linear_tile(a.xcd_swizzle)]);
 }
 
 // Batched interior kernel: the tiles of many problems in one launch; tile
@@ -1128,14 +1160,14 @@ bulk_interior_batch(const BulkArgs<T>* __restrict__ tab, const int4* __restrict_
   interior_tile<T, TX, TY, FULL, NTS, false, false>(tab[td.w], td);
 }
 
-// Band kernel: the points within four points of their block's edge (one-
-// sided closures, SAT), one lane each, from a list of (i, j).
-template <typename T, bool FULL, bool EXT>
+// Band kernel: the points within the closure depth of their block's edge
+// (one-sided closures, SAT), one lane each, from a list of (i, j).
+template <typename T, bool FULL, bool EXT, int OP = 4>
 __global__ void __launch_bounds__(64) bulk_band(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= npts) return;
   const int2 p = pts[k];
-  point_generic<T, FULL, EXT>(a, p.x, p.y);
+  point_generic<T, FULL, EXT, OP>(a, p.x, p.y);
 }
 
 // Layer kernel: the points of the perfectly matched layer blocks, one lane
@@ -1187,12 +1219,13 @@ __global__ void __launch_bounds__(64) bulk_band_batch(const BulkArgs<T>* __restr
 // plane-strain expressions.
 // ---------------------------------------------------------------------
 
-// interior point: a(s) = DI(s)/rho, b(s) = DI(s)*G (rates.f90:24-28)
-template <typename T, typename Acc>
-__device__ __forceinline__ void interior_rates3(T* d, const Acc& A, const T (&as)[2], const T (&bs)[2],
+// interior point: a(s) = DI(s)/rho, b(s) = DI(s)*G (rates.f90:24-28), s up
+// to the operator's interior half-width H
+template <typename T, typename Acc, int H = 2>
+__device__ __forceinline__ void interior_rates3(T* d, const Acc& A, const T (&as)[3], const T (&bs)[3],
                                                 T Ji) {
 #pragma unroll
-  for (int s = 1; s <= 2; s++) {
+  for (int s = 1; s <= H; s++) {
     const T ayp = (T)0.5 * (A.yr(s) + A.yr(0)), axp = (T)0.5 * (A.xr(s) + A.xr(0));
     const T aym = (T)0.5 * (A.yr(-s) + A.yr(0)), axm = (T)0.5 * (A.xr(-s) + A.xr(0));
     const T byp = (T)0.5 * (A.yq(s) + A.yq(0)), bxp = (T)0.5 * (A.xq(s) + A.xq(0));
@@ -1256,10 +1289,11 @@ __device__ __forceinline__ T bscale3(T t, T D, T rho, T G) {
 }
 
 // SAT (DF = DF+Ks*(Fhat-F), time_step.f90:829-830) and the closures of a
-// point within four points of its block's edge
-template <typename T>
+// point within the closure depth of its block's edge
+template <typename T, int OP = 4>
 __device__ __forceinline__ void closure_rates3(const BulkArgs<T>& a, const BlockDev& B, const int i,
                                                const int j, const long long cc, T (&d)[6]) {
+  constexpr int NB = Sbp<OP>::nbst, NS = Sbp<OP>::nbnd, HW = Sbp<OP>::half;
   const long long ps = a.ps;
   const int ld = a.ld;
   const T* __restrict__ F = a.Fin;
@@ -1288,21 +1322,21 @@ __device__ __forceinline__ void closure_rates3(const BulkArgs<T>& a, const Block
   const T* __restrict__ yr = a.yr;
   const T Ji = a.Ji[cc];
   const T rho = (T)B.rho, G = (T)B.G;
-  const bool qone = (dl < 4) || (dr < 4);
-  const bool rone = (db < 4) || (dtp < 4);
+  const bool qone = (dl < NB) || (dr < NB);
+  const bool rone = (db < NB) || (dtp < NB);
   const long long row = (long long)j * ld;
   const long long colm = (long long)B.my * ld + i, colp = (long long)B.py * ld + i;
   if (qone) {
 #pragma unroll
-    for (int s = 0; s < 6; s++) {
-      const long long kq = (dl < 4) ? row + B.mx + s : row + B.px - s;
-      const T Dq = (dl < 4) ? (T)kDL[dl][s] : (T)(-kDL[dr][s]);
+    for (int s = 0; s < NS; s++) {
+      const long long kq = (dl < NB) ? row + B.mx + s : row + B.px - s;
+      const T Dq = (dl < NB) ? (T)sbp_dl<OP>(dl, s) : (T)(-sbp_dl<OP>(dr, s));
       d[0] = d[0] + bscale3<T, 0>(q1_3<T, 0>(F, yr, xr, ps, kq, cc), Dq, rho, G) * Ji;
       d[1] = d[1] + bscale3<T, 1>(q1_3<T, 1>(F, yr, xr, ps, kq, cc), Dq, rho, G) * Ji;
       d[2] = d[2] + bscale3<T, 2>(q1_3<T, 2>(F, yr, xr, ps, kq, cc), Dq, rho, G) * Ji;
       if (rone) {
-        const long long kr = (db < 4) ? colm + (long long)s * ld : colp - (long long)s * ld;
-        const T Dr = (db < 4) ? (T)kDL[db][s] : (T)(-kDL[dtp][s]);
+        const long long kr = (db < NB) ? colm + (long long)s * ld : colp - (long long)s * ld;
+        const T Dr = (db < NB) ? (T)sbp_dl<OP>(db, s) : (T)(-sbp_dl<OP>(dtp, s));
         d[0] = d[0] + bscale3<T, 0>(r1_3<T, 0>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
         d[1] = d[1] + bscale3<T, 1>(r1_3<T, 1>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
         d[2] = d[2] + bscale3<T, 2>(r1_3<T, 2>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
@@ -1310,8 +1344,8 @@ __device__ __forceinline__ void closure_rates3(const BulkArgs<T>& a, const Block
     }
     if (!rone) {
 #pragma unroll
-      for (int s = 1; s <= 2; s++) {
-        const T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
+      for (int s = 1; s <= HW; s++) {
+        const T D = (T)sbp_di<OP>(s);
         const long long kp = cc + (long long)s * ld, km = cc - (long long)s * ld;
         d[0] = d[0] + bscale3<T, 0>(rc_3<T, 0>(F, yq, xq, ps, kp, km, cc), D, rho, G) * Ji;
         d[1] = d[1] + bscale3<T, 1>(rc_3<T, 1>(F, yq, xq, ps, kp, km, cc), D, rho, G) * Ji;
@@ -1320,16 +1354,16 @@ __device__ __forceinline__ void closure_rates3(const BulkArgs<T>& a, const Block
     }
   } else {
 #pragma unroll
-    for (int s = 0; s < 6; s++) {
-      const long long kr = (db < 4) ? colm + (long long)s * ld : colp - (long long)s * ld;
-      const T Dr = (db < 4) ? (T)kDL[db][s] : (T)(-kDL[dtp][s]);
+    for (int s = 0; s < NS; s++) {
+      const long long kr = (db < NB) ? colm + (long long)s * ld : colp - (long long)s * ld;
+      const T Dr = (db < NB) ? (T)sbp_dl<OP>(db, s) : (T)(-sbp_dl<OP>(dtp, s));
       d[0] = d[0] + bscale3<T, 0>(r1_3<T, 0>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
       d[1] = d[1] + bscale3<T, 1>(r1_3<T, 1>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
       d[2] = d[2] + bscale3<T, 2>(r1_3<T, 2>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
     }
 #pragma unroll
-    for (int s = 1; s <= 2; s++) {
-      const T D = (s == 1) ? (T)FDM_DI1 : (T)FDM_DI2;
+    for (int s = 1; s <= HW; s++) {
+      const T D = (T)sbp_di<OP>(s);
       const long long kp = cc + s, km = cc - s;
       d[0] = d[0] + bscale3<T, 0>(qc_3<T, 0>(F, yr, xr, ps, kp, km, cc), D, rho, G) * Ji;
       d[1] = d[1] + bscale3<T, 1>(qc_3<T, 1>(F, yr, xr, ps, kp, km, cc), D, rho, G) * Ji;
@@ -1340,8 +1374,9 @@ __device__ __forceinline__ void closure_rates3(const BulkArgs<T>& a, const Block
 
 // one grid point of one stage in antiplane shear, neighbours from global
 // memory: A*DF, SAT, rates, update (fields.f90:1117-1133, 1225-1242)
-template <typename T>
+template <typename T, int OP = 4>
 __device__ __forceinline__ void point_generic3(const BulkArgs<T>& a, const int i, const int j) {
+  constexpr int NB = Sbp<OP>::nbst, HW = Sbp<OP>::half;
   const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
   const BlockDev& B = a.blocks[bi];
   const long long ps = a.ps;
@@ -1356,13 +1391,13 @@ __device__ __forceinline__ void point_generic3(const BulkArgs<T>& a, const int i
 #pragma unroll
     for (int l = 0; l < 3; l++) d[l] = A * a.DF[l * ps + cc];
   }
-  if (i - B.mx >= 4 && B.px - i >= 4 && j - B.my >= 4 && B.py - j >= 4) {
+  if (i - B.mx >= NB && B.px - i >= NB && j - B.my >= NB && B.py - j >= NB) {
     const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld, nullptr};
-    const T as[2] = {(T)B.cI[0][0], (T)B.cI[0][1]};
-    const T bs[2] = {(T)B.cI[3][0], (T)B.cI[3][1]};
-    interior_rates3<T>(d, acc, as, bs, a.Ji[cc]);
+    const T as[3] = {(T)B.cI[0][0], (T)B.cI[0][1], (T)B.cI[0][2]};
+    const T bs[3] = {(T)B.cI[3][0], (T)B.cI[3][1], (T)B.cI[3][2]};
+    interior_rates3<T, GlobalAcc<T>, HW>(d, acc, as, bs, a.Ji[cc]);
   } else {
-    closure_rates3<T>(a, B, i, j, cc, d);
+    closure_rates3<T, OP>(a, B, i, j, cc, d);
   }
   if (a.do_update) {
     const T Bdt = (T)a.Bdt;
@@ -1373,30 +1408,32 @@ __device__ __forceinline__ void point_generic3(const BulkArgs<T>& a, const int i
   for (int l = 0; l < 3; l++) a.DF[l * ps + cc] = d[l];
 }
 
-template <typename T>
+template <typename T, int OP = 4>
 __global__ void __launch_bounds__(256) bulk_kernel3(BulkArgs<T> a) {
   int tx, ty;
   tile_of(a.xcd_swizzle, tx, ty);
   const int i = tx * blockDim.x + threadIdx.x;
   const int j = ty * blockDim.y + threadIdx.y;
   if (i >= a.nx || j >= a.ny) return;
-  point_generic3<T>(a, i, j);
+  point_generic3<T, OP>(a, i, j);
 }
 
-template <typename T>
+template <typename T, int OP = 4>
 __global__ void __launch_bounds__(64) bulk_band3(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= npts) return;
   const int2 p = pts[k];
-  point_generic3<T>(a, p.x, p.y);
+  point_generic3<T, OP>(a, p.x, p.y);
 }
 
-// interior tiles in antiplane shear: vz, sxz, syz with the two-point halo,
-// the metrics as in the plane-strain kernel, one barrier
-template <typename T, int TX, int TY, bool NTS>
+// interior tiles in antiplane shear: vz, sxz, syz with a halo of the
+// operator's half-width, the metrics as in the plane-strain kernel, one
+// barrier
+template <typename T, int TX, int TY, bool NTS, int OP = 4>
 __global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4)))
 bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
-  constexpr int HX = TX + 4, HY = TY + 4, NT = TX * TY;
+  constexpr int H = Sbp<OP>::half, NB = Sbp<OP>::nbst;
+  constexpr int HX = TX + 2 * H, HY = TY + 2 * H, NT = TX * TY;
   constexpr int NF = (HY * HX + NT - 1) / NT;
   constexpr int NR = (TY * HX + NT - 1) / NT;
   constexpr int NQ = (HY * TX + NT - 1) / NT;
@@ -1416,7 +1453,7 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
     const int k = tid + p * NT;
     if (k < HY * HX) {
       const int r = k / HX, c = k - r * HX;
-      const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(max(i0 - 2 + c, 0), nx1);
+      const int gj = min(max(j0 - H + r, 0), ny1), gi = min(max(i0 - H + c, 0), nx1);
       const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
 #pragma unroll
       for (int l = 0; l < 3; l++) vF[p][l] = ld_off(F + l * ps, o);
@@ -1427,7 +1464,7 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
     const int k = tid + p * NT;
     if (k < TY * HX) {
       const int r = k / HX, c = k - r * HX;
-      const int gj = min(j0 + r, ny1), gi = min(max(i0 - 2 + c, 0), nx1);
+      const int gj = min(j0 + r, ny1), gi = min(max(i0 - H + c, 0), nx1);
       const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
       vR[p][0] = ld_off(a.xr, o);
       vR[p][1] = ld_off(a.yr, o);
@@ -1438,14 +1475,14 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
     const int k = tid + p * NT;
     if (k < HY * TX) {
       const int r = k / TX, c = k - r * TX;
-      const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(i0 + c, nx1);
+      const int gj = min(max(j0 - H + r, 0), ny1), gi = min(i0 + c, nx1);
       const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
       vQ[p][0] = ld_off(a.xq, o);
       vQ[p][1] = ld_off(a.yq, o);
     }
   }
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
-  const bool valid = i <= B4->px - 4 && j <= B4->py - 4;
+  const bool valid = i <= B4->px - NB && j <= B4->py - NB;
   const unsigned oc = (unsigned)(min(j, ny1) * ld + min(i, nx1)) * (unsigned)sizeof(T);
   T dfa[3];
   if (!a.a_zero) {
@@ -1491,10 +1528,10 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
 #pragma unroll
     for (int l = 0; l < 3; l++) d[l] = A * dfa[l];
   }
-  const TileAcc<T, TX, TY> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y, nullptr};
-  const T as[2] = {(T)B4->cI[0][0], (T)B4->cI[0][1]};
-  const T bs[2] = {(T)B4->cI[3][0], (T)B4->cI[3][1]};
-  interior_rates3<T>(d, acc, as, bs, Ji);
+  const TileAcc<T, TX, TY, H> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y, nullptr};
+  const T as[3] = {(T)B4->cI[0][0], (T)B4->cI[0][1], (T)B4->cI[0][2]};
+  const T bs[3] = {(T)B4->cI[3][0], (T)B4->cI[3][1], (T)B4->cI[3][2]};
+  interior_rates3<T, TileAcc<T, TX, TY, H>, H>(d, acc, as, bs, Ji);
   if (a.do_update) {
     const T Bdt = (T)a.Bdt;
 #pragma unroll

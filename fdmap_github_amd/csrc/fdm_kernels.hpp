@@ -6,7 +6,7 @@
 //                  rate-and-state solve, hats, state rates and the fault RK
 //                  update, all in FP64.
 //   bulk_kernel  : one lane per grid point.  DF <- A*DF, SAT penalties,
-//                  SBP4 stencils with curvilinear metrics, szz overwrite,
+//                  SBP2/4/6 stencils with curvilinear metrics, szz overwrite,
 //                  sources, F <- F + B*dt*DF and (last stage) the
 //                  Drucker-Prager return map.  Templated on the storage
 //                  type T of fields/rates/metrics (double, or float for the
@@ -21,7 +21,7 @@ struct BlockDev {
   int mx, px, my, py;
   int plastic, mms, id, pad;
   double rho, G, M, gamma, nu;
-  double cI[5][2];  // interior coefficients DI(s)/rho, DI(s)*M, DI(s)*G (rates.f90:400-405)
+  double cI[5][3];  // interior coefficients DI(s)/rho, DI(s)*M, DI(s)*G, s = 1..3 (rates.f90:400-405)
   double mu, beta, b, h, eta, K;
   double s0[6];
   int side_off[4], side_n[4], side_bc[4];

@@ -26,6 +26,66 @@ __device__ constexpr double kDL[4][6] = {
     {0.030612244897959, 0.000000000000000, -0.602040816326531, 0.000000000000000,
      0.653061224489796, -0.081632653061224}};
 
+// The first-derivative operators 'SBP2', 'SBP4' and 'SBP6' (init_fd,
+// fd_coeff.f90:41-248): points of a boundary stencil (nbnd), boundary
+// stencils (nbst, the width of each block's closure band) and the interior
+// half-width.  The tables are the reference's 15-digit literals.
+template <int OP>
+struct Sbp;
+template <>
+struct Sbp<2> { static constexpr int nbnd = 2, nbst = 1, half = 1; };
+template <>
+struct Sbp<4> { static constexpr int nbnd = 6, nbst = 4, half = 2; };
+template <>
+struct Sbp<6> { static constexpr int nbnd = 9, nbst = 6, half = 3; };
+
+// 'SBP2' DL (fd_coeff.f90:88-94) and 'SBP6' DL (fd_coeff.f90:137-155),
+// kDL6[r][s] = DL(s,r)
+__device__ constexpr double kDL2[1][2] = {{-1.0, 1.0}};
+__device__ constexpr double kDL6[6][9] = {
+    {-1.582533518939116, 2.033378678700676, -0.141512858744873, -0.450398306578272,
+     0.104488069284042, 0.036577936277544, 0.000000000000000, 0.000000000000000,
+     0.000000000000000},
+    {-0.462059195631158, 0.000000000000000, 0.287258622978251, 0.258816087376832,
+     -0.069112065532624, -0.014903449191300, 0.000000000000000, 0.000000000000000,
+     0.000000000000000},
+    {0.071247104721830, -0.636451095137907, 0.000000000000000, 0.606235523609147,
+     -0.022902190275815, -0.018129342917256, 0.000000000000000, 0.000000000000000,
+     0.000000000000000},
+    {0.114713313798970, -0.290087484386815, -0.306681191361148, 0.000000000000000,
+     0.520262285050482, -0.051642265516119, 0.013435342414630, 0.000000000000000,
+     0.000000000000000},
+    {-0.036210680656541, 0.105400944933782, 0.015764336127392, -0.707905442575989,
+     0.000000000000000, 0.769199413962647, -0.164529643265203, 0.018281071473911,
+     0.000000000000000},
+    {-0.011398193015050, 0.020437334208704, 0.011220896474665, 0.063183694641876,
+     -0.691649024426814, 0.000000000000000, 0.739709139060752, -0.147941827812150,
+     0.016437980868017}};
+
+// DL(s,r) of the operator (s = 0..nbnd-1, r = 0..nbst-1); DR(-s,-r) = -DL(s,r)
+template <int OP>
+__device__ __forceinline__ double sbp_dl(int r, int s) {
+  if constexpr (OP == 2) return kDL2[r][s];
+  else if constexpr (OP == 6) return kDL6[r][s];
+  else return kDL[r][s];
+}
+
+// interior DI(s), s = 1..half (fd_coeff.f90:86, 101-102, 132-134)
+template <int OP>
+__host__ __device__ __forceinline__ double sbp_di(int s) {
+  if constexpr (OP == 2) return 0.5;
+  else if constexpr (OP == 6) return s == 1 ? 0.750000000000000 : (s == 2 ? -0.150000000000000 : 0.016666666666667);
+  else return s == 1 ? FDM_DI1 : FDM_DI2;
+}
+
+// DI(s) of 'SBP<op>' for s = 1..3, zero beyond the operator's half-width
+// (the reference's DI(-3:3) table, fd_coeff.f90:41-50)
+inline double sbp_di_host(int op, int s) {
+  if (op == 2) return s <= Sbp<2>::half ? sbp_di<2>(s) : 0.0;
+  if (op == 6) return s <= Sbp<6>::half ? sbp_di<6>(s) : 0.0;
+  return s <= Sbp<4>::half ? sbp_di<4>(s) : 0.0;
+}
+
 // artificial dissipation AI(0..2) (fd_coeff.f90:103-104) and AL(s,r)
 // (fd_coeff.f90:117-124); AR(-s,-r) = AL(s,r) (fd_coeff.f90:242-246)
 #define FDM_AI0 (-0.500000000000000)

@@ -5,9 +5,10 @@
 ! those of the reference (main.f90:29, domain.f90:84-117, grid.f90:51,
 ! boundaries.f90:32, material.f90:47, fields.f90:90, source.f90:41,
 ! interfaces.f90:44, friction.f90:79); sections are located with the same
-! '!---BLOCKn---' / '!---IFACEn---' markers (io.f90:209-228).  Only the
-! plane-strain (mode 2) SBP4 path is supported; anything else stops with a
-! message naming the unsupported option.
+! '!---BLOCKn---' / '!---IFACEn---' markers (io.f90:209-228).  Plane strain
+! (mode 2) and antiplane shear (mode 3) with FDmethod SBP2, SBP4 or SBP6 are
+! supported; an option the device path does not cover stops with a message
+! naming it.
 
 module fdmap_setup
 

@@ -20,6 +20,12 @@ def _f(v):
     return repr(float(v)).replace("e", "d") if "e" in repr(float(v)) else repr(float(v)) + "d0"
 
 
+def _fdm(fdmethod):
+    """the domain_list entry for FDmethod; None leaves the reference's
+    default (SBP6, domain.f90:94)"""
+    return "" if fdmethod is None else f" FDmethod = '{fdmethod}',"
+
+
 def rs_state(a, V0, tau, sn, V):
     """Psi such that the regularised law slips at V under tau, sn
     (inverse of friction.f90:1066-1070)."""
@@ -215,16 +221,17 @@ vz sxz syz
 """
 
 
-def kali_het(directory, name="kali_het", n=301, nsteps=2000, outputs=True, ninfo=10):
+def kali_het(directory, name="kali_het", n=301, nsteps=2000, outputs=True, ninfo=10,
+             fdmethod=None):
     """The reference's mode-3 rate-and-state example with fault properties
     from files (problems/kali-heterogeneous-input.in, files as
     problems/kali_input.m makes them): two 30 km x 30 km blocks side by side
     with a vertical aging-law fault between them, a free surface on top; a,
     b, the state and a fault prestress vary along the fault.  n points per
     block side (301 in the example).  The example leaves FDmethod at the
-    reference's default, SBP6 (domain.f90:94); this one asks for SBP4, the
-    operator built here.  Writes the three fault files next to the .in text
-    it returns."""
+    reference's default, SBP6 (domain.f90:94), as fdmethod=None does; a
+    name ('SBP2', 'SBP4', 'SBP6') sets it.  Writes the three fault files next
+    to the .in text it returns."""
     os.makedirs(directory, exist_ok=True)
     y = np.linspace(-30.0, 0.0, n)
     a = 0.01 + 0.005 * np.sin(2 * np.pi * y / 10)
@@ -249,7 +256,7 @@ V S N Psi D
  nt = {nsteps + 1}, CFL = 0.5d0, refine = 1d0 /
 
 &domain_list
- mode = 3, FDmethod = 'SBP4', nx = {n}, ny = {2 * n},
+ mode = 3,{_fdm(fdmethod)} nx = {n}, ny = {2 * n},
  nblocks = 2, nblocks_x = 2, nblocks_y = 1, nifaces = 1 /
 
 !---BLOCK1---
@@ -302,15 +309,14 @@ V S N Psi D
 """
 
 
-def gp229_sw3(name="gp229", nx=801, nsteps=1000, CFL=1.0, outputs=True, ninfo=10):
+def gp229_sw3(name="gp229", nx=801, nsteps=1000, CFL=1.0, outputs=True, ninfo=10, fdmethod=None):
     """The reference's antiplane slip-weakening example (gp229/gp229sw3ex1.in):
     two 80 km x 40 km blocks stacked in y with a horizontal fault, prestress
     syy0 = -120, syz0 = 70 MPa, slip-weakening friction with forced rupture
     expansion, and a smooth boxcar lowering the shear stress by 3 MPa for
     x > 10 km.  nx points along the fault (801 in the example), nx/2 + 1
     across each block.  The example leaves FDmethod at the reference's
-    default, SBP6 (domain.f90:94); this one asks for SBP4, the operator
-    built here."""
+    default, SBP6 (domain.f90:94), as fdmethod=None does; a name sets it."""
     ny = nx // 2 + 1
     out = ""
     if outputs:
@@ -327,7 +333,7 @@ vz sxz syz
  name = '{name}', ninfo = {ninfo}, nt = {nsteps + 1}, CFL = {_f(CFL)}, refine = 1d0 /
 
 &domain_list
- mode = 3, FDmethod = 'SBP4', nx = {nx}, ny = {2 * ny}, nblocks = 2, nblocks_x = 1, nblocks_y = 2,
+ mode = 3,{_fdm(fdmethod)} nx = {nx}, ny = {2 * ny}, nblocks = 2, nblocks_x = 1, nblocks_y = 2,
  nifaces = 1 /
 
 !---BLOCK1---

@@ -1,6 +1,7 @@
 """The C-ABI boundary without a GPU: the HIP library loads, exports every
 function include/fdmap.h declares, and the Python descriptors match the
-header's structure layout.  No entry point is called (each one needs a
+header's structure layout.  Apart from fdm_create's argument checks, which
+return before any device call, no entry point is called (each one needs a
 device)."""
 import ctypes as C
 import os
@@ -85,3 +86,24 @@ def test_descriptor_layout_matches_header():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(RuntimeError, match="libfdmap_hip.so not found"):
         capi.load_library(str(tmp_path / "libfdmap_hip.so"))
+
+
+@pytest.mark.parametrize("change,msg", [
+    (dict(sbp=8), "SBP2, SBP4 or SBP6"),
+    (dict(sbp=6, Cdiss=300.0), "need FDmethod SBP4"),
+    (dict(sbp=2, pml=dict(x=True)), "need FDmethod SBP4"),
+    (dict(sbp=6, nys=(11,)), "SBP6 needs at least 12"),
+])
+def test_create_refuses_what_the_operator_cannot_serve(change, msg):
+    """fdm_create validates the operator before it touches a device: the
+    range of FDmethod, dissipation and PML (built on SBP4's tables in the
+    reference, rates.f90:1397-2619) and the smallest block the closures fit"""
+    import problems
+    from fdmap_github_amd.device import Context, FdmError
+    kw = dict(nx=24, nys=(20,), seed=1, coupling="locked")
+    sbp = change.pop("sbp")
+    kw.update(change)
+    prob, _ = problems.build(**kw)
+    prob.sbp = sbp
+    with pytest.raises(FdmError, match=msg):
+        Context(prob)

@@ -140,12 +140,16 @@ def test_antiplane_executable_outputs(gpu, tmp_path):
     assert re.search(r"^mode = +3;$", m, re.M)
 
 
-def test_gp229_forced_slip_weakening_rupture(gpu, tmp_path):
+@pytest.mark.parametrize("fdmethod", [None, "SBP4"])
+def test_gp229_forced_slip_weakening_rupture(gpu, tmp_path, fdmethod):
     """the reference's antiplane slip-weakening example with forced rupture
-    expansion (gp229/gp229sw3ex1.in), at a quarter of its resolution"""
+    expansion (gp229/gp229sw3ex1.in), at a quarter of its resolution; as the
+    example is written (FDmethod left at SBP6) and with SBP4"""
     p = inputs.write(str(tmp_path / "gp.in"),
-                     inputs.gp229_sw3(name=str(tmp_path / "gp"), nx=201, nsteps=150, outputs=False))
+                     inputs.gp229_sw3(name=str(tmp_path / "gp"), nx=201, nsteps=150, outputs=False,
+                                      fdmethod=fdmethod))
     hp, dev, orc = run_pair(p, 150)
+    assert hp.ptr.contents.sbp == (6 if fdmethod is None else 4)
     e = rel_l2(dev.get_fields()[0], orc.get_fields()[0])
     print("fields", e)
     assert e <= 1e-12
@@ -157,7 +161,8 @@ def test_gp229_forced_slip_weakening_rupture(gpu, tmp_path):
     assert V.max() > 0.1 and np.sum(V > 0) > 10  # the forced rupture has grown
 
 
-def test_kali_fault_properties_from_files(gpu, tmp_path):
+@pytest.mark.parametrize("fdmethod", [None, "SBP4"])
+def test_kali_fault_properties_from_files(gpu, tmp_path, fdmethod):
     """the reference's mode-3 aging-law example with a, b, the state and a
     fault prestress read from files (problems/kali-heterogeneous-input.in,
     files as problems/kali_input.m makes them), at 121 points a side (the
@@ -165,10 +170,13 @@ def test_kali_fault_properties_from_files(gpu, tmp_path):
     stiffness: where the state has dropped, exp((f0 - Psi)/b) V0/L exceeds
     what LSRK(5,4) integrates stably at dt, the state blows up and the
     Newton solve fails, as the reference's would; 200 steps here stay well
-    before that (the oracle alone runs to step 397)."""
+    before that (the oracle alone runs to step 397 with SBP4).  As the
+    example is written (FDmethod left at SBP6) and with SBP4."""
     p = inputs.write(str(tmp_path / "kali.in"),
-                     inputs.kali_het(str(tmp_path), n=121, nsteps=200, outputs=False))
+                     inputs.kali_het(str(tmp_path), n=121, nsteps=200, outputs=False,
+                                     fdmethod=fdmethod))
     hp, dev, orc = run_pair(p, 200)
+    assert hp.ptr.contents.sbp == (6 if fdmethod is None else 4)
     e = rel_l2(dev.get_fields()[0], orc.get_fields()[0])
     print("fields", e)
     assert e <= 1e-12

@@ -1,0 +1,161 @@
+"""FDmethod 'SBP2' and 'SBP6' on the HIP path against the oracle
+(init_fd, fd_coeff.f90:41-248; the reference's default operator is SBP6,
+domain.f90:94).
+
+The operator changes the closure depth (1, 4, 6 points), the closure width
+(2, 6, 9 points), the interior half-width (1, 2, 3: the LDS halo of the
+interior tiles) and the SAT weight H00i.  Problems without friction use
+only +,-,*,/ and sqrt, so the device result is bit-identical to the
+oracle's; rate-and-state runs meet relative L2 <= 1e-12 on fields and fault
+arrays (1e-11 on the rates).  Every case also
+runs the one-lane-per-point kernel, which must agree bit for bit with the
+tiles + band split.
+"""
+import numpy as np
+import pytest
+
+import problems
+from fdmap_github_amd.device import Context
+from oracle.binding import OracleContext
+
+pytestmark = pytest.mark.gpu
+
+CS, CP, H = 3.464, 6.0, 0.025
+DT = 0.5 * H / np.sqrt(CS ** 2 + CP ** 2)
+BG = [0.0, 0.0, -120.0, 70.0, -120.0, -120.0]
+BG3 = [0.0, 30.0, 40.0]
+INPLANE = (-120.0, 0.0, -100.0, -110.0)
+FAULT = ("D", "V", "S", "N", "Psi")
+
+
+def rel_l2(a, b):
+    den = np.linalg.norm(b)
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / (den if den > 0 else 1.0)
+
+
+def run(ctx, F, nsteps):
+    ctx.set_fields(F)
+    ctx.init_edges(0.0)
+    ctx.step(0.0, DT, 1, nsteps)
+    Fo, DFo = ctx.get_fields()
+    faults = [{f: ctx.get_iface(k, f) for f in FAULT} for k in range(len(ctx.ifaces_n))]
+    return Fo, DFo, faults
+
+
+def device_run(monkeypatch, prob, F, nsteps, bulk):
+    monkeypatch.setenv("FDM_BULK", bulk)
+    dev = Context(prob)
+    try:
+        return run(dev, F, nsteps)
+    finally:
+        dev.close()
+
+
+def check(monkeypatch, prob, F, nsteps, bit_exact, tol=1e-12, rates_tol=1e-11):
+    ref = run(OracleContext(prob, threads=4), F, nsteps)
+    tiles = device_run(monkeypatch, prob, F, nsteps, "tile")
+    point = device_run(monkeypatch, prob, F, nsteps, "generic")
+    print(f"SBP{prob.sbp}: fields rel L2 {rel_l2(tiles[0], ref[0]):.3e}, "
+          f"rates {rel_l2(tiles[1], ref[1]):.3e}")
+    for k in (0, 1):
+        # the tile + band split and the per-point kernel: same expression
+        # tree per point
+        assert np.array_equal(tiles[k], point[k])
+        if bit_exact:
+            assert np.array_equal(tiles[k], ref[k]), np.abs(tiles[k] - ref[k]).max()
+        # fields at the north-star 1e-12; the rates, which pass through zero
+        # where the rupture front has passed, at 1e-11 as in the other parity
+        # tests
+        assert rel_l2(tiles[k], ref[k]) <= (tol if k == 0 else max(tol, rates_tol))
+    for fd, fp, fo in zip(tiles[2], point[2], ref[2]):
+        for f in FAULT:
+            assert np.array_equal(fd[f], fp[f]), f
+            assert rel_l2(fd[f], fo[f]) <= tol, (f, rel_l2(fd[f], fo[f]))
+    return tiles, ref
+
+
+@pytest.mark.parametrize("sbp", [2, 6])
+def test_single_block_source_bit_exact(gpu, monkeypatch, sbp):
+    """one curved block with a point source, absorbing and free sides;
+    ragged interior tiles in both directions"""
+    prob, F = problems.build(nx=83, nys=(45,), seed=3, warp=0.05, background=BG, perturb=1e-2,
+                             source=True, bcs=("absorbing", "free", "absorbing", "free"), sbp=sbp)
+    check(monkeypatch, prob, F, 25, bit_exact=True)
+
+
+@pytest.mark.parametrize("sbp", [2, 6])
+def test_boundary_conditions_bit_exact(gpu, monkeypatch, sbp):
+    prob, F = problems.build(nx=40, nys=(33,), seed=5, warp=0.08, background=BG, perturb=1e-2,
+                             bcs=("rigid", "absorbing-velocity", "free-velocity", "absorbing-0"),
+                             sbp=sbp)
+    check(monkeypatch, prob, F, 15, bit_exact=True)
+
+
+@pytest.mark.parametrize("sbp", [2, 6])
+def test_locked_2x2_bit_exact(gpu, monkeypatch, sbp):
+    prob, F = problems.build(nx=70, nys=(36, 41), nbx=2, seed=7, warp=0.05, background=BG,
+                             perturb=1e-2, coupling="locked", sbp=sbp)
+    check(monkeypatch, prob, F, 20, bit_exact=True)
+
+
+@pytest.mark.parametrize("sbp", [2, 6])
+def test_heterogeneous_medium_bit_exact(gpu, monkeypatch, sbp):
+    """the point-wise material coefficients DI(s)/rho(i,j) etc.
+    (rates_heterogeneous.f90:392-428) with the operator's DI"""
+    prob, F = problems.build(nx=90, nys=(40, 37), seed=37, warp=0.04, background=BG,
+                             perturb=1e-2, coupling="locked", medium=0.3, sbp=sbp)
+    check(monkeypatch, prob, F, 20, bit_exact=True)
+
+
+@pytest.mark.parametrize("sbp", [2, 6])
+def test_rate_and_state_rupture(gpu, monkeypatch, sbp):
+    prob, F = problems.build(nx=96, nys=(30, 30), seed=11, warp=0.03, background=BG,
+                             perturb=0.0, law="RFL", fault_overstress=25.0, sbp=sbp)
+    _, ref = check(monkeypatch, prob, F, 60, bit_exact=False)
+    assert ref[2][0]["V"].max() > 1e-3  # the rupture is under way
+
+
+@pytest.mark.parametrize("sbp", [2, 6])
+def test_plasticity(gpu, monkeypatch, sbp):
+    """initial state just above the Drucker-Prager yield surface, so the
+    return map runs from the first step"""
+    pl = dict(mu=0.6, beta=0.0, b=0.0, h=0.0, eta=0.0, s0=[0.0] * 6)
+    bg = [0.0, 0.0, -60.0, 44.0, -100.0, -80.0]
+    prob, F = problems.build(nx=64, nys=(24, 24), seed=17, warp=0.03, background=bg,
+                             perturb=0.0, law="RFL", fault_overstress=20.0, plastic=pl, sbp=sbp)
+    check(monkeypatch, prob, F, 30, bit_exact=False)
+
+
+@pytest.mark.parametrize("sbp", [2, 6])
+def test_antiplane_bit_exact(gpu, monkeypatch, sbp):
+    """rates_*_mode3 (rates.f90:8-381) with the operator's tables"""
+    prob, F = problems.build(nx=70, nys=(44,), seed=61, warp=0.05, background=BG3, perturb=1e-2,
+                             mode=3, bcs=("absorbing", "free", "rigid", "free"), sbp=sbp)
+    check(monkeypatch, prob, F, 25, bit_exact=True)
+
+
+@pytest.mark.parametrize("sbp", [2, 6])
+def test_antiplane_rupture(gpu, monkeypatch, sbp):
+    prob, F = problems.build(nx=64, nys=(24, 26), seed=71, warp=0.03, background=[0.0, 0.0, 70.0],
+                             perturb=0.0, law="RFL", fault_overstress=25.0, mode=3,
+                             inplane=INPLANE, sbp=sbp)
+    check(monkeypatch, prob, F, 60, bit_exact=False)
+
+
+@pytest.mark.parametrize("sbp", [2, 6])
+def test_fp32_bulk_error_against_fp64(gpu, sbp):
+    """FP32 fields with the operator: the error of every result against the
+    FP64 device run on the same inputs, at the bound the SBP4 form meets"""
+    kw = dict(nx=96, nys=(40, 40), seed=19, warp=0.03, background=BG, perturb=0.0, law="RFL",
+              fault_overstress=25.0, sbp=sbp)
+    p64, F = problems.build(**kw)
+    p32, _ = problems.build(precision="fp32", **kw)
+    out = [run(Context(p), F, 60) for p in (p64, p32)]
+    for k in (0,):
+        e = rel_l2(out[1][k], out[0][k])
+        print(f"SBP{sbp} fp32 fields rel L2 {e:.3e}")
+        assert e <= 1e-5
+    for f in ("D", "V"):
+        e = rel_l2(out[1][2][0][f], out[0][2][0][f])
+        print(f"SBP{sbp} fp32 {f} rel L2 {e:.3e}")
+        assert e <= 1e-5
