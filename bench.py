@@ -35,9 +35,22 @@ HBM_PEAK_GBS = 8000.0
 # DF(6) + F(6), read 5 metric arrays; the first stage does not read DF
 # (A1 = 0).  FP64: 232 B (184 B at stage 1); FP32-bulk: half.
 BYTES_STAGE = {"fp64": (184.0, 232.0), "fp32": (92.0, 116.0)}
-# what the interior kernel itself must move: DF6 is neither read nor stored
-# (no stage needs it), so 176 B at stage 1 and 216 B after (FP64)
-KERNEL_BYTES_STAGE = {"fp64": (176.0, 216.0), "fp32": (88.0, 108.0)}
+# what the interior kernel itself must move, per point and stage: DF6 is
+# neither read nor stored (no stage needs it); Ji is not read where the
+# kernel forms it from the metrics (ji); the metrics take 4 FP32 deltas
+# instead of 4 FP64 values where that encoding is exact (mc); the last
+# stage of a step that another step of the same call follows does not store
+# DF1..5
+
+
+def kernel_min_bytes(prec, ns, ji=False, mc=False):
+    """average bytes per point of one interior launch over a step's stages"""
+    e = 8.0 if prec == "fp64" else 4.0
+    read = 6 * e + (16.0 if mc else 4 * e) + (0.0 if ji else e)
+    tot = 0.0
+    for s in range(1, ns + 1):
+        tot += read + (5 * e if s > 1 else 0.0) + 6 * e + (5 * e if s < ns else 0.0)
+    return tot / ns
 
 
 def rel_l2(a, b):
@@ -54,9 +67,27 @@ def run_device(inpath, precision, steps, warmup, prof_steps, horizon=None):
     from fdmap_github_amd import host
     from fdmap_github_amd.device import Context
 
+    from fdmap_github_amd import capi
+    from fdmap_github_amd.device import FdmError
+
     hp = host.HostProblem(inpath, precision=precision)
     ctx = Context(hp.ptr)
     npts = hp.nx * hp.ny
+    # the interior kernel's variant: Ji formed in the kernel (FP64, where the
+    # given Ji is 1/(xq yr - xr yq) to the bit) and FP32-delta metrics (on
+    # wherever fdm_create found them exact; setting the knob to 1 fails
+    # where it did not)
+    P = hp.ptr.contents
+    ji = mc = False
+    if precision == "fp64":
+        m = {k: np.ctypeslib.as_array(getattr(P, k), shape=(hp.ny, hp.nx))
+             for k in ("xq", "xr", "yq", "yr", "Ji")}
+        ji = bool(np.array_equal(1.0 / (m["xq"] * m["yr"] - m["xr"] * m["yq"]), m["Ji"]))
+        try:
+            ctx._check(ctx.lib.fdm_tune(ctx.h, capi.FDM_TUNE_METRIC_F32, 1))
+            mc = True
+        except FdmError:
+            mc = False
     ctx.set_fields(hp.fields())
     ctx.init_edges(hp.t)
     ctx.step(hp.t, hp.dt, 1, warmup)
@@ -86,7 +117,7 @@ def run_device(inpath, precision, steps, warmup, prof_steps, horizon=None):
         fault_h = {f: ctx.get_iface(0, f) for f in ("D", "V", "S")}
     ctx.close()
     return dict(hp=hp, npts=npts, el=el, el_prof=el_prof, prof=prof, F=F, fault=fault,
-                nstage=nstage, F_h=Fh, fault_h=fault_h, horizon=done)
+                nstage=nstage, F_h=Fh, fault_h=fault_h, horizon=done, ji=ji, mc=mc)
 
 
 def cpu_baseline(inpath, sample_steps):
@@ -675,9 +706,11 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "bulk_interior<double,64,4,false,true> (SBP4 interior rates + RK update, "
-                      "64x4 LDS tiles over the block interiors, non-temporal stores since the "
-                      "23 field, rate and metric planes exceed 256 MB)",
+            "kernel": f"bulk_interior<double,64,4,false,true,false,false,4,"
+                      f"{str(r64['ji']).lower()},{str(r64['mc']).lower()}> (SBP4 interior rates + "
+                      "RK update, 64x4 LDS tiles over the block interiors, non-temporal stores "
+                      "since the 23 field, rate and metric planes exceed 256 MB; JI: Ji formed "
+                      "from the metrics, MC: metrics read as FP32 deltas)",
             "kernel_points": kpts,
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -685,8 +718,7 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "alg_bytes_per_launch": bytes_launch,
-            "kernel_min_bytes_per_launch": kpts * (KERNEL_BYTES_STAGE["fp64"][0]
-                                                   + (ns - 1) * KERNEL_BYTES_STAGE["fp64"][1]) / ns,
+            "kernel_min_bytes_per_launch": kpts * kernel_min_bytes("fp64", ns, r64["ji"], r64["mc"]),
             "avg_launch_ms": avg_ms,
             "launches": prof["bulk_launches"],
             "ms_per_step_with_events": 1e3 * r64["el_prof"] / args.prof_steps,

@@ -186,6 +186,7 @@ NOALLOC_ENTRY_POINTS = [
 FDM_TUNE_TILE = 0
 FDM_TUNE_SWIZZLE = 1
 FDM_TUNE_NT_STORE = 2
+FDM_TUNE_METRIC_F32 = 3
 TILE_SHAPES = ["64x4", "64x8", "128x4"]
 ENSEMBLE_ENTRY_POINTS = [
     ("fdm_tune", C.c_int, [_P, C.c_int32, C.c_int32]),

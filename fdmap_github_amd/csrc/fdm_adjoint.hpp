@@ -63,6 +63,78 @@ struct AdjBulkArgs {
   double A, Bdt_prev;
 };
 
+// the transposed field couplings of rates_*_mode2 (rates.f90:411-444) for one
+// stencil row: row kk of the q derivative (Q = true; ay, ax from yr, xr) or
+// of the r derivative (Q = false; by, bx from yq, xq, opposite signs)
+template <bool Q>
+__device__ __forceinline__ void adj_row(double (&g)[5], const double (&w)[5], double ay, double ax,
+                                        double gam) {
+  if constexpr (Q) {
+    g[2] += ay * w[0];
+    g[3] -= ax * w[0];
+    g[3] += ay * w[1];
+    g[4] -= ax * w[1];
+    g[0] += ay * w[2];
+    g[1] -= ax * gam * w[2];
+    g[1] += ay * w[3];
+    g[0] -= ax * w[3];
+    g[0] += ay * gam * w[4];
+    g[1] -= ax * w[4];
+  } else {
+    g[2] -= ay * w[0];
+    g[3] += ax * w[0];
+    g[3] -= ay * w[1];
+    g[4] += ax * w[1];
+    g[0] -= ay * w[2];
+    g[1] += ax * gam * w[2];
+    g[1] -= ay * w[3];
+    g[0] += ax * w[3];
+    g[0] -= ay * gam * w[4];
+    g[1] += ax * w[4];
+  }
+}
+
+// the rows of one direction's derivative whose stencil reaches local index
+// c of a block side of n points, in increasing row order: a central index
+// (6 <= c <= n-7) is reached by interior rows c-2..c+2 only (D = DI(c - r),
+// zero at r = c); any other by rows max(0, c-5)..min(n-1, c+5), whose
+// entries come from the closures or DI.  Both loops have a fixed trip count
+// and no branch, so that a lane's loads are in flight together: a row
+// outside the range reads a clamped, valid address and carries D = 0, which
+// adds nothing to g.  base: address of row 0's point, step: between rows.
+template <bool Q>
+__device__ __forceinline__ void adj_dir(const AdjBulkArgs& a, double (&g)[5], const int c, const int n,
+                                        const long long base, const long long step, const long long cc,
+                                        const double (&cl)[5], const double gam) {
+  const double* __restrict__ m1 = Q ? a.yr : a.yq;  // the "y" metric of the pair
+  const double* __restrict__ m2 = Q ? a.xr : a.xq;
+  const double y0 = m1[cc], x0 = m2[cc];
+  const long long ps = a.ps;
+  auto row = [&](const int r, const double D) {
+    const long long kk = base + (long long)r * step;
+    const double Jk = a.Ji[kk];
+    double w[5];
+#pragma unroll
+    for (int l = 0; l < 5; l++) w[l] = a.min[l * ps + kk] * Jk * cl[l];
+    const double ay = 0.5 * (m1[kk] + y0) * D, ax = 0.5 * (m2[kk] + x0) * D;
+    adj_row<Q>(g, w, ay, ax, gam);
+  };
+  if (c >= 6 && c <= n - 7) {
+    row(c - 2, FDM_DI2);
+    row(c - 1, FDM_DI1);
+    row(c + 1, -FDM_DI1);
+    row(c + 2, -FDM_DI2);
+  } else {
+    const int r0 = max(0, c - 5), r1 = min(n - 1, c + 5);
+#pragma unroll
+    for (int t = 0; t < 11; t++) {
+      const int r = min(r0 + t, r1);
+      const double D = (r0 + t <= r1) ? sbp_entry(r, c, n) : 0.0;
+      row(r, D);
+    }
+  }
+}
+
 // R^T mu at (i, j), edge contributions, and the two recurrences
 __device__ __forceinline__ void adj_point(const AdjBulkArgs& a, const int i, const int j) {
   const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
@@ -76,59 +148,9 @@ __device__ __forceinline__ void adj_point(const AdjBulkArgs& a, const int i, con
   const double cl[5] = {1.0 / B.rho, 1.0 / B.rho, B.M, B.G, B.M};
   double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   // q: rows of the block's q-derivative whose stencil reaches column il
-  {
-    const double yr0 = a.yr[cc], xr0 = a.xr[cc];
-    // columns 6..nq-7 are reached by interior rows il-2..il+2 only
-    const bool cen = il >= 6 && il <= nq - 7;
-    const int r0 = cen ? il - 2 : max(0, il - 5), r1 = cen ? il + 2 : min(nq - 1, il + 5);
-    for (int r = r0; r <= r1; r++) {
-      const double D = sbp_entry(r, il, nq);
-      if (D == 0.0) continue;
-      const long long kk = (long long)j * ld + B.mx + r;
-      const double Jk = a.Ji[kk];
-      double w[5];
-#pragma unroll
-      for (int l = 0; l < 5; l++) w[l] = a.min[l * ps + kk] * Jk * cl[l];
-      const double ay = 0.5 * (a.yr[kk] + yr0) * D, ax = 0.5 * (a.xr[kk] + xr0) * D;
-      // field couplings of rates_*_mode2 (rates.f90:411-444), transposed
-      g[2] += ay * w[0];
-      g[3] -= ax * w[0];
-      g[3] += ay * w[1];
-      g[4] -= ax * w[1];
-      g[0] += ay * w[2];
-      g[1] -= ax * gam * w[2];
-      g[1] += ay * w[3];
-      g[0] -= ax * w[3];
-      g[0] += ay * gam * w[4];
-      g[1] -= ax * w[4];
-    }
-  }
+  adj_dir<true>(a, g, il, nq, (long long)j * ld + B.mx, 1, cc, cl, gam);
   // r: rows of the block's r-derivative whose stencil reaches row jl
-  {
-    const double yq0 = a.yq[cc], xq0 = a.xq[cc];
-    const bool cen = jl >= 6 && jl <= nr - 7;
-    const int r0 = cen ? jl - 2 : max(0, jl - 5), r1 = cen ? jl + 2 : min(nr - 1, jl + 5);
-    for (int r = r0; r <= r1; r++) {
-      const double D = sbp_entry(r, jl, nr);
-      if (D == 0.0) continue;
-      const long long kk = (long long)(B.my + r) * ld + i;
-      const double Jk = a.Ji[kk];
-      double w[5];
-#pragma unroll
-      for (int l = 0; l < 5; l++) w[l] = a.min[l * ps + kk] * Jk * cl[l];
-      const double by = 0.5 * (a.yq[kk] + yq0) * D, bx = 0.5 * (a.xq[kk] + xq0) * D;
-      g[2] -= by * w[0];
-      g[3] += bx * w[0];
-      g[3] -= by * w[1];
-      g[4] += bx * w[1];
-      g[0] -= by * w[2];
-      g[1] += bx * gam * w[2];
-      g[1] -= by * w[3];
-      g[0] += bx * w[3];
-      g[0] -= by * gam * w[4];
-      g[1] += bx * w[4];
-    }
-  }
+  adj_dir<false>(a, g, jl, nr, (long long)B.my * ld + i, ld, cc, cl, gam);
   // SAT and hat contributions of the sides this point lies on
   const int NS = a.NS;
   int so[4], nso = 0;

@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -295,6 +296,10 @@ struct fdm_ctx {
   // the interior kernel forms Ji from the metrics (FP64, when the given Ji
   // equals 1/(xq yr - xr yq) everywhere; FDM_JI_CALC = 0 turns it off)
   bool ji_calc = false;
+  // FP32-delta metrics for the interior tiles (fdm_create decides)
+  bool mcomp = false;
+  float* mfp[4] = {nullptr, nullptr, nullptr, nullptr};
+  long long mc_escapes = 0;
 };
 
 // Non-temporal stores pay when a launch's working set (bytes of the planes
@@ -522,6 +527,10 @@ BulkArgs<T> bulk_args(fdm_ctx* c, double A, double Bdt, int do_update, double t,
   a.W = (T*)c->W;
   a.ji_calc = c->ji_calc ? 1 : 0;
   a.skip_df = 0;
+  a.fxq = c->mfp[0];
+  a.fxr = c->mfp[1];
+  a.fyq = c->mfp[2];
+  a.fyr = c->mfp[3];
   return a;
 }
 
@@ -590,24 +599,32 @@ void launch_interior3(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid
 
 // one interior tile variant: FULL (sources, MMS, plasticity), or lean with
 // non-temporal or normal stores
-template <typename T, int TX, int TY, bool HET, bool DISS, int OP, bool JI>
+template <typename T, int TX, int TY, bool HET, bool DISS, int OP, bool JI, bool MC = false>
 void launch_interior_v(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid, dim3 block,
                        bool full) {
   const int4* tl = c->tiles[c->tile_shape];
   if (full)
-    hipLaunchKernelGGL((bulk_interior<T, TX, TY, true, false, HET, DISS, OP, JI>), grid, block, 0, s, a, tl);
+    hipLaunchKernelGGL((bulk_interior<T, TX, TY, true, false, HET, DISS, OP, JI, MC>), grid, block, 0, s, a, tl);
   else if (c->nt_store)
-    hipLaunchKernelGGL((bulk_interior<T, TX, TY, false, true, HET, DISS, OP, JI>), grid, block, 0, s, a, tl);
+    hipLaunchKernelGGL((bulk_interior<T, TX, TY, false, true, HET, DISS, OP, JI, MC>), grid, block, 0, s, a, tl);
   else
-    hipLaunchKernelGGL((bulk_interior<T, TX, TY, false, false, HET, DISS, OP, JI>), grid, block, 0, s, a, tl);
+    hipLaunchKernelGGL((bulk_interior<T, TX, TY, false, false, HET, DISS, OP, JI, MC>), grid, block, 0, s, a, tl);
 }
 
 // the JI variants (Ji formed in the kernel) exist for FP64 only: fdm_create
-// sets ji_calc in FP64 alone
+// sets ji_calc in FP64 alone; so do the MC variants (FP32-delta metrics),
+// for the plain SBP4 64 x 4 tiles
 template <typename T, int TX, int TY, bool HET, bool DISS, int OP>
 void launch_interior_j(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid, dim3 block,
                        bool full) {
   if constexpr (std::is_same<T, double>::value) {
+    if constexpr (TX == 64 && TY == 4 && !HET && !DISS && OP == 4) {
+      if (c->mcomp) {
+        if (c->ji_calc) launch_interior_v<T, TX, TY, HET, DISS, OP, true, true>(c, s, a, grid, block, full);
+        else launch_interior_v<T, TX, TY, HET, DISS, OP, false, true>(c, s, a, grid, block, full);
+        return;
+      }
+    }
     if (c->ji_calc) {
       launch_interior_v<T, TX, TY, HET, DISS, OP, true>(c, s, a, grid, block, full);
       return;
@@ -1601,6 +1618,55 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     if (const char* v = getenv("FDM_JI_CALC")) exact = exact && atoi(v) != 0;
     c->ji_calc = exact;
   }
+  if (c->precision == FDM_FP64 && c->mode == 2 && c->sbp == 4 && c->bulk_mode == 2 && !c->het &&
+      !c->diss) {
+    // FP32-delta metrics: xq, xr, yq, yr of a point stored as d = m - base
+    // in FP32, base = the metric at the first interior point of its block,
+    // kept where base + (double)d reproduces m to the bit and marked NaN
+    // (read from the FP64 plane) elsewhere.  The interior tiles then read 16
+    // instead of 32 bytes of metrics per point; worth it when the escapes
+    // are rare (grids made of nearly affine blocks), so used when at most
+    // 1/64 of the metric values escape.
+    const double* mets[4] = {P->xq, P->xr, P->yq, P->yr};
+    for (int b = 0; b < c->nb; b++) {
+      BlockDev& D = c->hblocks[b];
+      const int i0 = std::min(D.mx + c->nbst, D.px), j0 = std::min(D.my + c->nbst, D.py);
+      for (int q = 0; q < 4; q++) D.mbase[q] = mets[q][(size_t)j0 * c->nx + i0];
+    }
+    const float qnan = std::numeric_limits<float>::quiet_NaN();
+    std::vector<float> h[4];
+    long long esc = 0;
+    for (int q = 0; q < 4; q++) h[q].assign((size_t)c->ps, 0.0f);
+    for (int j = 0; j < c->ny; j++)
+      for (int i = 0; i < c->nx; i++) {
+        const BlockDev& D = c->hblocks[block_of(c, i, j)];
+        for (int q = 0; q < 4; q++) {
+          const double m = mets[q][(size_t)j * c->nx + i];
+          const double d = m - D.mbase[q];
+          const float f = (float)d;
+          const double back = D.mbase[q] + (double)f;
+          const bool ok = std::isfinite(d) && (double)f == d && std::memcmp(&back, &m, sizeof(double)) == 0;
+          h[q][(size_t)j * c->ld + i] = ok ? f : qnan;
+          esc += ok ? 0 : 1;
+        }
+      }
+    bool use = esc * 64 <= 4LL * c->nx * c->ny;
+    if (const char* v = getenv("FDM_METRIC_F32")) use = use && atoi(v) != 0;
+    if (use) {
+      void* mb = nullptr;
+      const size_t fbytes = (size_t)c->ps * 4;
+      if ((rc = dalloc(c, &mb, 4 * fbytes + (size_t)c->coff * 4 + 256))) return fail(rc);
+      char* fb = (char*)mb + (size_t)c->coff * 4;
+      for (int q = 0; q < 4; q++) {
+        c->mfp[q] = (float*)(fb + q * fbytes);
+        HIPCHK(c, hipMemcpy(c->mfp[q], h[q].data(), fbytes, hipMemcpyHostToDevice));
+      }
+      HIPCHK(c, hipMemcpy(c->blocks, c->hblocks.data(), c->hblocks.size() * sizeof(BlockDev),
+                          hipMemcpyHostToDevice));
+      c->mcomp = true;
+      c->mc_escapes = esc;
+    }
+  }
   for (int b = 0; b < c->nb; b++) c->any_mms |= (P->blocks[b].mms != 0);
   bool graded = false;
   for (int b = 0; b < c->nb; b++) graded |= (c->hblocks[b].pmlx || c->hblocks[b].pmly) && c->hblocks[b].pml[0] > 0.0;
@@ -1776,7 +1842,8 @@ int fdm_set_iface(fdm_ctx* c, int32_t ii, int32_t field, const double* in) {
 
 int fdm_get_iface_stage1(fdm_ctx* c, int32_t ii, int32_t field, double* out) {
   if (!out) return FDM_EINVAL;
-  return iface_copy(c, ii, field, out, nullptr, c ? c->ist1 : nullptr);
+  int rc = iface_copy(c, ii, field, out, nullptr, c ? c->ist1 : nullptr);
+  return rc ? rc : check_fail(c);
 }
 
 int fdm_get_plastic(fdm_ctx* c, double* gammap, double* lambda) {
@@ -1786,11 +1853,11 @@ int fdm_get_plastic(fdm_ctx* c, double* gammap, double* lambda) {
   if (!c->any_plastic) {
     if (gammap) std::fill(gammap, gammap + np, 0.0);
     if (lambda) std::fill(lambda, lambda + np, 0.0);
-    return FDM_OK;
+    return check_fail(c);
   }
   if (gammap) HIPCHK(c, hipMemcpy(gammap, c->gammap, np * 8, hipMemcpyDeviceToHost));
   if (lambda) HIPCHK(c, hipMemcpy(lambda, c->lambda, np * 8, hipMemcpyDeviceToHost));
-  return FDM_OK;
+  return check_fail(c);
 }
 
 int fdm_set_plastic(fdm_ctx* c, const double* gammap, const double* lambda) {
@@ -1844,7 +1911,7 @@ int fdm_get_pml(fdm_ctx* c, double* Wx, double* Wy) {
   int rc;
   if (Wx && (rc = pml_copy(c, 0, Wx, nullptr))) return rc;
   if (Wy && (rc = pml_copy(c, 6, Wy, nullptr))) return rc;
-  return FDM_OK;
+  return check_fail(c);
 }
 
 int fdm_set_pml(fdm_ctx* c, const double* Wx, const double* Wy) {
@@ -1860,7 +1927,8 @@ int fdm_set_pml(fdm_ctx* c, const double* Wx, const double* Wy) {
 
 int fdm_max_slip_velocity(fdm_ctx* c, double* vmax) {
   if (!c || !vmax) return FDM_EINVAL;
-  return vmax_reduce(c, c->vred, vmax);
+  const int rc = vmax_reduce(c, c->vred, vmax);
+  return rc ? rc : check_fail(c);
 }
 
 int fdm_max_slip_velocity_noalloc(fdm_ctx* c, double* vmax) {
@@ -1876,7 +1944,8 @@ int fdm_max_slip_velocity_noalloc(fdm_ctx* c, double* vmax) {
     if (c->device >= 0 && c->device < kMaxDevices) ptr = g_scratch_ptr[c->device];
   }
   if (!ptr) return set_err(c, FDM_EDEVICE, "fdm_max_slip_velocity_noalloc: device scratch unresolved");
-  return vmax_reduce(c, ptr, vmax);
+  const int rc = vmax_reduce(c, ptr, vmax);
+  return rc ? rc : check_fail(c);
 }
 
 int fdm_sync(fdm_ctx* c) {
@@ -1932,6 +2001,12 @@ int fdm_tune(fdm_ctx* c, int32_t knob, int32_t value) {
   }
   if (knob == FDM_TUNE_NT_STORE && (value == 0 || value == 1)) {
     c->nt_store = value != 0;
+    return FDM_OK;
+  }
+  if (knob == FDM_TUNE_METRIC_F32 && (value == 0 || value == 1)) {
+    if (value && !c->mfp[0])
+      return set_err(c, FDM_EINVAL, "fdm_tune: this context's metrics have no FP32-delta form");
+    c->mcomp = value != 0;
     return FDM_OK;
   }
   return set_err(c, FDM_EINVAL, "fdm_tune: unknown knob or value");

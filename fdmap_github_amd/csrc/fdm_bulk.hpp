@@ -1,7 +1,9 @@
-// fdm_bulk.hpp -- the fused bulk kernel: RK scaling, SAT penalties, SBP4
+// fdm_bulk.hpp -- the fused bulk kernel: RK scaling, SAT penalties, SBP2/4/6
 // curvilinear stencils (rates.f90:384-936), szz overwrite, sources, RK
 // update and plasticity, one lane per grid point.
 #pragma once
+
+#include <type_traits>
 
 #include "fdm_kernels.hpp"
 
@@ -59,6 +61,12 @@ struct BulkArgs {
   // never reads it)
   int ji_calc;
   int skip_df;
+  // FP32 deltas of xq, xr, yq, yr from their block's mbase (FP64 contexts
+  // whose metrics allow it; read by the interior tiles' MC variant only)
+  const float* fxq;
+  const float* fxr;
+  const float* fyq;
+  const float* fyr;
 };
 
 // Map the hardware workgroup id to a tile.  Workgroups are dispatched to the
@@ -927,6 +935,12 @@ __device__ __forceinline__ void st_off(T* base, unsigned boff, T v) {
 
 // NT: a non-temporal store, for planes that are streamed once per launch
 // when the working set exceeds the last-level cache (no allocation there)
+// a metric value from its FP32 delta: base + d, exact by construction; a
+// NaN delta (a value the encoding cannot hold) reads the FP64 plane
+__device__ __forceinline__ double metric_of(float d, double base, const double* plane, unsigned boff) {
+  return (d == d) ? base + (double)d : ld_off(plane, boff);
+}
+
 template <bool NT, typename T>
 __device__ __forceinline__ void st_sel(T* base, unsigned boff, T v) {
   T* p = reinterpret_cast<T*>(reinterpret_cast<char*>(base) + boff);
@@ -968,14 +982,16 @@ typedef const __attribute__((address_space(4))) BlockDev ConstBlockDev;
 // reaches an active lane.  tiles[k] = (block, i0, j0, 0).
 // JI: form Ji from the staged metrics (a context with ji_calc set) rather
 // than read it; a compile-time switch, so that neither path holds the other's
-// registers
+// registers.  MC: read xq, xr, yq, yr as FP32 deltas from their block's base
+// (FP64 only; the values staged in LDS are the FP64 metrics to the bit).
 template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET, bool DISS, int OP = 4,
-          bool JI = false>
+          bool JI = false, bool MC = false>
 __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 td) {
   // H: the operator's interior half-width, the tile's halo; NB: the
   // closure's depth, how far the interior rectangle stays from block edges
   constexpr int H = Sbp<OP>::half, NB = Sbp<OP>::nbst;
   static_assert(!DISS || OP == 4, "artificial dissipation is built for SBP4 only");
+  static_assert(!MC || std::is_same<T, double>::value, "FP32-delta metrics serve FP64 fields");
   constexpr int HX = TX + 2 * H, HY = TY + 2 * H, NT = TX * TY;
   constexpr int NF = (HY * HX + NT - 1) / NT;  // staging passes, F1..F5
   constexpr int NR = (TY * HX + NT - 1) / NT;  // xr, yr
@@ -991,7 +1007,8 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
   const int ld = a.ld, nx1 = a.nx - 1, ny1 = a.ny - 1;
   const long long ps = a.ps;
   const T* __restrict__ F = a.Fin;
-  T vF[NF][5], vR[NR][2], vQ[NQ][2], vJ[DISS ? NF : 1];
+  T vF[NF][5], vR[MC ? 1 : NR][2], vQ[MC ? 1 : NQ][2], vJ[DISS ? NF : 1];
+  float fR[MC ? NR : 1][2], fQ[MC ? NQ : 1][2];
 #pragma unroll
   for (int p = 0; p < NF; p++) {
     const int k = tid + p * NT;
@@ -1010,9 +1027,15 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     if (k < TY * HX) {
       const int r = k / HX, c = k - r * HX;
       const int gj = min(j0 + r, ny1), gi = min(max(i0 - H + c, 0), nx1);
-      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
-      vR[p][0] = ld_off(a.xr, o);
-      vR[p][1] = ld_off(a.yr, o);
+      if constexpr (MC) {
+        const unsigned o4 = (unsigned)(gj * ld + gi) * 4u;
+        fR[p][0] = ld_off(a.fxr, o4);
+        fR[p][1] = ld_off(a.fyr, o4);
+      } else {
+        const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+        vR[p][0] = ld_off(a.xr, o);
+        vR[p][1] = ld_off(a.yr, o);
+      }
     }
   }
 #pragma unroll
@@ -1021,9 +1044,15 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     if (k < HY * TX) {
       const int r = k / TX, c = k - r * TX;
       const int gj = min(max(j0 - H + r, 0), ny1), gi = min(i0 + c, nx1);
-      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
-      vQ[p][0] = ld_off(a.xq, o);
-      vQ[p][1] = ld_off(a.yq, o);
+      if constexpr (MC) {
+        const unsigned o4 = (unsigned)(gj * ld + gi) * 4u;
+        fQ[p][0] = ld_off(a.fxq, o4);
+        fQ[p][1] = ld_off(a.fyq, o4);
+      } else {
+        const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+        vQ[p][0] = ld_off(a.xq, o);
+        vQ[p][1] = ld_off(a.yq, o);
+      }
     }
   }
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
@@ -1061,8 +1090,15 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     const int k = tid + p * NT;
     if (k < TY * HX) {
       const int r = k / HX, c = k - r * HX;
-      sR[0][r][c] = vR[p][0];
-      sR[1][r][c] = vR[p][1];
+      if constexpr (MC) {
+        const int gj = min(j0 + r, ny1), gi = min(max(i0 - H + c, 0), nx1);
+        const unsigned o = (unsigned)(gj * ld + gi) * 8u;
+        sR[0][r][c] = metric_of(fR[p][0], B4->mbase[1], a.xr, o);
+        sR[1][r][c] = metric_of(fR[p][1], B4->mbase[3], a.yr, o);
+      } else {
+        sR[0][r][c] = vR[p][0];
+        sR[1][r][c] = vR[p][1];
+      }
     }
   }
 #pragma unroll
@@ -1070,8 +1106,15 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
     const int k = tid + p * NT;
     if (k < HY * TX) {
       const int r = k / TX, c = k - r * TX;
-      sQ[0][r][c] = vQ[p][0];
-      sQ[1][r][c] = vQ[p][1];
+      if constexpr (MC) {
+        const int gj = min(max(j0 - H + r, 0), ny1), gi = min(i0 + c, nx1);
+        const unsigned o = (unsigned)(gj * ld + gi) * 8u;
+        sQ[0][r][c] = metric_of(fQ[p][0], B4->mbase[0], a.xq, o);
+        sQ[1][r][c] = metric_of(fQ[p][1], B4->mbase[2], a.yq, o);
+      } else {
+        sQ[0][r][c] = vQ[p][0];
+        sQ[1][r][c] = vQ[p][1];
+      }
     }
   }
   __syncthreads();
@@ -1144,11 +1187,11 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
 // a heterogeneous medium holds five more values per lane, dissipation the
 // Jacobian's staging: three waves per SIMD keep them in registers
 template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET, bool DISS, int OP = 4,
-          bool JI = false>
+          bool JI = false, bool MC = false>
 __global__ void __launch_bounds__(TX * TY)
     __attribute__((amdgpu_waves_per_eu(FULL ? 2 : ((HET || DISS || OP == 6) ? 3 : 4))))
 bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
-  interior_tile<T, TX, TY, FULL, NTS, HET, DISS, OP, JI>(a, tiles[linear_tile(a.xcd_swizzle)]);
+  interior_tile<T, TX, TY, FULL, NTS, HET, DISS, OP, JI, MC>(a, tiles[linear_tile(a.xcd_swizzle)]);
 }
 
 // Batched interior kernel: the tiles of many problems in one launch; tile

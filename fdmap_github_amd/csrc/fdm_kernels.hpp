@@ -41,6 +41,10 @@ struct BlockDev {
   int pmlx, pmly;
   double pml[7];
   double phx[9], phy[9];
+  // FP32-delta metrics (FP64 contexts): xq, xr, yq, yr of a point of this
+  // block are mbase[q] + (double)d with d the stored FP32 delta, exactly; a
+  // NaN delta sends the reader to the FP64 plane
+  double mbase[4];
 };
 
 struct IfaceDev {

@@ -272,7 +272,15 @@ int fdm_init_edges(fdm_ctx* ctx, double t);
 /* nsteps full low-storage RK steps: time_step_LS (time_step.f90:71-167)
    called from the main loop (main.f90:168-231) for steps
    n = n_first .. n_first+nsteps-1, each starting at t + (n-1)*dt
-   (main.f90:176).  Includes the plasticity split step. */
+   (main.f90:176).  Includes the plasticity split step.
+   The call queues the work and returns.  A failed friction Newton solve
+   (the reference stops there, friction.f90:1012-1036) sets a device flag
+   that the next synchronising call (fdm_sync, fdm_init_edges, every
+   fdm_get_*, the slip-velocity maxima) reports as FDM_ENEWTON, with the
+   requested data still copied out; the stages queued after
+   the failing one still run, on the failed state.  A caller that must stop
+   at the failing step, as the reference does, steps one step per call and
+   calls fdm_sync after each. */
 int fdm_step(fdm_ctx* ctx, double t, double dt, int64_t n_first, int32_t nsteps);
 
 /* Final step of the main loop (final_step = .true., time_step.f90:104-133):

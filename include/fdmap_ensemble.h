@@ -22,6 +22,13 @@ extern "C" {
 #define FDM_TUNE_NT_STORE 2 /* non-temporal stores of the interior kernel: 0 or 1
                                (default: 1 when the 23 field, rate and metric
                                planes exceed 256 MB) */
+#define FDM_TUNE_METRIC_F32 3 /* interior tiles read the metrics as FP32 deltas
+                                 from a per-block base (FP64 contexts, plain
+                                 SBP4 64x4 tiles): 0 or 1.  fdm_create turns it
+                                 on where the encoding is exact for all but
+                                 1/64 of the metric values (those are read
+                                 from the FP64 planes); value 1 fails where
+                                 it did not. */
 
 /* Set a launch knob of ctx.  Results do not depend on it.  Set knobs
    before putting ctx into an ensemble (its graphs keep the shapes they were

@@ -1,0 +1,121 @@
+"""FP32-delta metrics of the interior tiles (FDM_TUNE_METRIC_F32).
+
+In FP64 contexts whose blocks are nearly affine, fdm_create stores xq, xr,
+yq, yr a second time as FP32 deltas from a per-block base, where base +
+delta reproduces the FP64 value to the bit, and marks the other values NaN
+so that the kernel reads them from the FP64 planes.  The staged metrics are
+therefore the FP64 ones exactly: every result must be bit-identical with
+and without the encoding, and the oracle comparisons keep their bars.
+"""
+import numpy as np
+import pytest
+
+import problems
+from fdmap_github_amd import capi, host, inputs
+from fdmap_github_amd.device import Context, FdmError
+from oracle.binding import OracleContext
+
+pytestmark = pytest.mark.gpu
+
+CS, CP, H = 3.464, 6.0, 0.025
+DT = 0.5 * H / np.sqrt(CS ** 2 + CP ** 2)
+BG = [0.0, 0.0, -120.0, 70.0, -120.0, -120.0]
+FAULT = ("D", "V", "S", "N", "Psi")
+
+
+def rel_l2(a, b):
+    den = np.linalg.norm(b)
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / (den if den > 0 else 1.0)
+
+
+def set_metric_f32(ctx, on):
+    ctx._check(ctx.lib.fdm_tune(ctx.h, capi.FDM_TUNE_METRIC_F32, 1 if on else 0))
+
+
+def run(ctx, F, nsteps, t=0.0, dt=DT):
+    ctx.set_fields(F)
+    ctx.init_edges(t)
+    ctx.step(t, dt, 1, nsteps)
+    Fo, DFo = ctx.get_fields()
+    faults = [{f: ctx.get_iface(k, f) for f in FAULT} for k in range(len(ctx.ifaces_n))]
+    return Fo, DFo, faults
+
+
+def both_ways(problem, F, nsteps, **kw):
+    out = []
+    for on in (True, False):
+        ctx = Context(problem)
+        set_metric_f32(ctx, on)  # raises where the context has no FP32 form
+        out.append(run(ctx, F, nsteps, **kw))
+        ctx.close()
+    for k in (0, 1):
+        assert np.array_equal(out[0][k], out[1][k])
+    for fa, fb in zip(out[0][2], out[1][2]):
+        for f in FAULT:
+            assert np.array_equal(fa[f], fb[f]), f
+    return out[0]
+
+
+def escapes(P, nx, ny):
+    """metric values of a host-built problem that the encoding cannot hold
+    (the rule of fdm_create, restated)"""
+    n = 0
+    for m in ("xq", "xr", "yq", "yr"):
+        a = np.ctypeslib.as_array(getattr(P, m), shape=(ny, nx))
+        for b in range(P.nblocks):
+            B = P.blocks[b]
+            blk = a[B.my:B.py + 1, B.mx:B.px + 1]
+            base = blk[4, 4]
+            d = blk - base
+            f = d.astype(np.float32).astype(np.float64)
+            n += int((~((f == d) & (base + f == blk))).sum())
+    return n
+
+
+def test_flat_rupture_same_bits_and_oracle(gpu):
+    prob, F = problems.build(nx=203, nys=(61, 77), seed=23, warp=0.0, background=BG, perturb=1e-3,
+                             law="RFL")
+    Fd, DFd, fd = both_ways(prob, F, 40)
+    Fo, DFo, fo = run(OracleContext(prob, threads=4), F, 40)
+    assert rel_l2(Fd, Fo) <= 1e-12
+    for f in FAULT:
+        assert rel_l2(fd[0][f], fo[0][f]) <= 1e-12, f
+
+
+def test_flat_locked_blocks_bit_exact_against_oracle(gpu):
+    prob, F = problems.build(nx=90, nys=(45, 52), nbx=2, seed=29, warp=0.0, background=BG,
+                             perturb=1e-2, coupling="locked", source=True)
+    Fd, DFd, _ = both_ways(prob, F, 25)
+    Fo, DFo, _ = run(OracleContext(prob, threads=4), F, 25)
+    assert np.array_equal(Fd, Fo)
+    assert np.array_equal(DFd, DFo)
+
+
+def test_curved_grid_has_no_fp32_form(gpu):
+    prob, F = problems.build(nx=64, nys=(30, 30), seed=3, warp=0.04, background=BG, perturb=1e-3,
+                             law="RFL")
+    ctx = Context(prob)
+    with pytest.raises(FdmError, match="FP32-delta"):
+        set_metric_f32(ctx, True)
+    set_metric_f32(ctx, False)
+
+
+def test_planar_rupture_input_with_escaped_values(gpu, tmp_path):
+    """the planar rupture input (the benchmark's problem at 201 x 101 points a
+    block): its metrics carry a few values that the encoding cannot hold (x
+    of the grid's middle column is a rounding residue of zero), which the
+    kernel reads from the FP64 planes"""
+    p = inputs.write(str(tmp_path / "c2.in"),
+                     inputs.c2_planar_rs(name=str(tmp_path / "c2"), nx=201, ny=101, nsteps=300,
+                                         outputs=False, R=0.5))
+    hp = host.HostProblem(p, precision="fp64")
+    P = hp.ptr.contents
+    n_esc = escapes(P, hp.nx, hp.ny)
+    print("escaped metric values", n_esc, "of", 4 * hp.nx * hp.ny)
+    assert 0 < n_esc * 64 <= 4 * hp.nx * hp.ny
+    Fd, DFd, fd = both_ways(hp.ptr, hp.fields(), 300, t=hp.t, dt=hp.dt)
+    hp2 = host.HostProblem(p, precision="fp64")
+    Fo, DFo, fo = run(OracleContext(hp2.ptr, threads=4), hp2.fields(), 300, t=hp2.t, dt=hp2.dt)
+    assert rel_l2(Fd, Fo) <= 1e-12
+    for f in FAULT:
+        assert rel_l2(fd[0][f], fo[0][f]) <= 1e-12, f
