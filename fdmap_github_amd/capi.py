@@ -181,6 +181,13 @@ NOALLOC_ENTRY_POINTS = [
     ("fdm_max_slip_velocity_noalloc", C.c_int, [_P, _dp]),
 ]
 
+# output streaming of include/fdmap.h (the MI355X library only)
+FDM_SNAPSHOT_SLOTS = 4
+STREAM_ENTRY_POINTS = [
+    ("fdm_snapshot", C.c_int, [_P, C.c_uint32, C.POINTER(C.c_int32)]),
+    ("fdm_snapshot_read", C.c_int, [_P, C.c_int32, _dp]),
+]
+
 
 # entry points declared in include/fdmap_ensemble.h (the MI355X library only)
 FDM_TUNE_TILE = 0
@@ -220,7 +227,8 @@ def bind(lib, prefix_from="fdm_", prefix_to="fdm_"):
         fn = getattr(lib, prefix_to + name[len(prefix_from):])
         fn.restype = res
         fn.argtypes = args
-    for name, res, args in NOALLOC_ENTRY_POINTS + ENSEMBLE_ENTRY_POINTS + ADJOINT_ENTRY_POINTS:
+    for name, res, args in (NOALLOC_ENTRY_POINTS + STREAM_ENTRY_POINTS + ENSEMBLE_ENTRY_POINTS
+                            + ADJOINT_ENTRY_POINTS):
         if hasattr(lib, name):
             fn = getattr(lib, name)
             fn.restype = res

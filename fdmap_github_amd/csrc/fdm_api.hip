@@ -296,6 +296,16 @@ struct fdm_ctx {
   // the interior kernel forms Ji from the metrics (FP64, when the given Ji
   // equals 1/(xq yr - xr yq) everywhere; FDM_JI_CALC = 0 turns it off)
   bool ji_calc = false;
+  // output streaming: a copy stream and the snapshot slots (device staging,
+  // pinned host copy, the event that ends the host copy)
+  hipStream_t copy = nullptr;
+  struct Snapshot {
+    void* dev = nullptr;
+    void* host = nullptr;
+    hipEvent_t staged = nullptr, done = nullptr;
+    uint32_t mask = 0;
+    bool busy = false;
+  } snap[FDM_SNAPSHOT_SLOTS];
   // FP32-delta metrics for the interior tiles (fdm_create decides)
   bool mcomp = false;
   float* mfp[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -951,6 +961,15 @@ void fdm_destroy(fdm_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->side) (void)hipStreamSynchronize(c->side);
+  if (c->copy) {
+    (void)hipStreamSynchronize(c->copy);
+    (void)hipStreamDestroy(c->copy);
+  }
+  for (auto& sn : c->snap) {
+    if (sn.host) (void)hipHostFree(sn.host);
+    if (sn.staged) (void)hipEventDestroy(sn.staged);
+    if (sn.done) (void)hipEventDestroy(sn.done);
+  }
   for (auto& ep : c->pool) {
     (void)hipEventDestroy(ep.a);
     (void)hipEventDestroy(ep.b);
@@ -1794,6 +1813,66 @@ int fdm_get_fields(fdm_ctx* c, double* F, double* DF) {
     }
   }
   return check_fail(c);
+}
+
+int fdm_snapshot(fdm_ctx* c, uint32_t field_mask, int32_t* slot) {
+  if (!c || !slot || field_mask == 0 || (field_mask >> c->nF) != 0) return FDM_EINVAL;
+  int k = 0;
+  while (k < FDM_SNAPSHOT_SLOTS && c->snap[k].busy) k++;
+  if (k == FDM_SNAPSHOT_SLOTS)
+    return set_err(c, FDM_EINVAL, "fdm_snapshot: all snapshot slots are waiting to be read");
+  auto& sn = c->snap[k];
+  const size_t pb = (size_t)c->ps * c->esz;
+  if (!c->copy) HIPCHK(c, hipStreamCreateWithFlags(&c->copy, hipStreamNonBlocking));
+  if (!sn.dev) {
+    int rc = dalloc(c, &sn.dev, (size_t)c->nF * pb);
+    if (rc) return rc;
+    HIPCHK(c, hipHostMalloc(&sn.host, (size_t)c->nF * pb, hipHostMallocDefault));
+    HIPCHK(c, hipEventCreateWithFlags(&sn.staged, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&sn.done, hipEventDisableTiming));
+  }
+  // the selected planes of the current fields, packed, then to the host on
+  // the copy stream while the main stream goes on
+  int q = 0;
+  for (int l = 0; l < c->nF; l++) {
+    if (!(field_mask >> l & 1u)) continue;
+    HIPCHK(c, hipMemcpyAsync((char*)sn.dev + q * pb, (const char*)c->F[c->cur] + l * pb, pb,
+                             hipMemcpyDeviceToDevice, c->stream));
+    q++;
+  }
+  HIPCHK(c, hipEventRecord(sn.staged, c->stream));
+  HIPCHK(c, hipStreamWaitEvent(c->copy, sn.staged, 0));
+  HIPCHK(c, hipMemcpyAsync(sn.host, sn.dev, q * pb, hipMemcpyDeviceToHost, c->copy));
+  HIPCHK(c, hipEventRecord(sn.done, c->copy));
+  sn.mask = field_mask;
+  sn.busy = true;
+  *slot = k;
+  return FDM_OK;
+}
+
+int fdm_snapshot_read(fdm_ctx* c, int32_t slot, double* out) {
+  if (!c || !out || slot < 0 || slot >= FDM_SNAPSHOT_SLOTS || !c->snap[slot].busy) return FDM_EINVAL;
+  auto& sn = c->snap[slot];
+  HIPCHK(c, hipEventSynchronize(sn.done));
+  const size_t pe = (size_t)c->ps, plane = (size_t)c->nx * c->ny;
+  int q = 0;
+  for (int l = 0; l < c->nF; l++) {
+    if (!(sn.mask >> l & 1u)) continue;
+    double* dst = out + q * plane;
+    if (c->precision == FDM_FP64) {
+      const double* h = (const double*)sn.host + q * pe;
+      for (int j = 0; j < c->ny; j++) memcpy(dst + (size_t)j * c->nx, h + (size_t)j * c->ld, c->nx * 8);
+    } else {
+      const float* h = (const float*)sn.host + q * pe;
+      for (int j = 0; j < c->ny; j++)
+        for (int i = 0; i < c->nx; i++)
+          dst[(size_t)j * c->nx + i] =
+              (double)h[(size_t)j * c->ld + i] + c->hblocks[block_of(c, i, j)].Fb[l];
+    }
+    q++;
+  }
+  sn.busy = false;
+  return FDM_OK;
 }
 
 int fdm_init_edges(fdm_ctx* c, double t) {

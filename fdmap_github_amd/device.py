@@ -77,6 +77,24 @@ class Context:
         self._check(self._fn("get_fields")(self.h, capi.dptr(F), capi.dptr(DF)))
         return F, DF
 
+    def snapshot(self, fields=None):
+        """queue a copy of the current fields (a list of field indices, all by
+        default) to host memory; returns the slot to read"""
+        fields = range(self.nF) if fields is None else fields
+        mask = 0
+        for l in fields:
+            mask |= 1 << int(l)
+        slot = C.c_int32()
+        self._check(self._fn("snapshot")(self.h, mask, C.byref(slot)))
+        return slot.value, sorted(set(int(l) for l in fields))
+
+    def snapshot_read(self, snap):
+        """the fields of a snapshot, [nselected, ny, nx]"""
+        slot, fields = snap
+        out = np.empty((len(fields), self.ny, self.nx))
+        self._check(self._fn("snapshot_read")(self.h, int(slot), capi.dptr(out)))
+        return out
+
     def init_edges(self, t):
         self._check(self._fn("init_edges")(self.h, float(t)))
 

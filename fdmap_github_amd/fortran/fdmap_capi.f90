@@ -15,6 +15,7 @@ module fdmap_capi
   integer(c_int),parameter :: FDM_OK = 0, FDM_EINVAL = -1, FDM_EDEVICE = -2, &
        FDM_ENOMEM = -3, FDM_ENEWTON = -4
   integer(c_int32_t),parameter :: FDM_FP64 = 0, FDM_FP32_BULK = 1
+  integer,parameter :: FDM_SNAPSHOT_SLOTS = 4
 
   integer(c_int32_t),parameter :: FDM_BC_NONE = 0, FDM_BC_ABSORBING = 1, FDM_BC_FREE = 2, &
        FDM_BC_RIGID = 3, FDM_BC_ABSORBING_VELOCITY = 4, FDM_BC_FREE_VELOCITY = 5, &
@@ -121,6 +122,23 @@ module fdmap_capi
        type(c_ptr),value :: ctx, F, DF
        integer(c_int) :: rc
      end function fdm_get_fields
+
+     function fdm_snapshot(ctx,mask,slot) bind(C,name='fdm_snapshot') result(rc)
+       ! mask: uint32_t in C; the field masks used stay below 2**31
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr),value :: ctx
+       integer(c_int32_t),value :: mask
+       integer(c_int32_t) :: slot
+       integer(c_int) :: rc
+     end function fdm_snapshot
+
+     function fdm_snapshot_read(ctx,slot,out) bind(C,name='fdm_snapshot_read') result(rc)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr),value :: ctx
+       integer(c_int32_t),value :: slot
+       type(c_ptr),value :: out
+       integer(c_int) :: rc
+     end function fdm_snapshot_read
 
      function fdm_init_edges(ctx,t) bind(C,name='fdm_init_edges') result(rc)
        import :: c_ptr, c_int, c_double

@@ -379,12 +379,18 @@ contains
     character(*),intent(in) :: infile
     integer(c_int32_t),intent(in) :: precision
     integer :: rc
-    logical :: ok,final_step,due,slipping,need_body,need_plastic,abort
+    logical :: ok,final_step,due,slipping,need_body,need_plastic,abort,deferred
     integer :: n,k,nnext,nrun,stat,nstart
     integer(c_int64_t) :: n64
     real(dp) :: t,wstart,w1,w2,wave,vmax,wprog
-    real(dp),allocatable,target :: F(:,:,:),gp(:,:),lam(:,:)
+    real(dp),allocatable,target :: F(:,:,:),gp(:,:),lam(:,:),Fs(:,:,:)
     character(512) :: str
+    ! body and block outputs in flight: the snapshot slot, step and time of
+    ! each, oldest first (written when read back)
+    integer :: npend
+    integer(c_int32_t) :: slot
+    integer :: pslot(FDM_SNAPSHOT_SLOTS),pn(FDM_SNAPSHOT_SLOTS)
+    real(dp) :: pt(FDM_SNAPSHOT_SLOTS)
 
     rc = 0
     wprog = walltime()
@@ -432,9 +438,10 @@ contains
     nstart = ck%begin+1
     call write_echo()
     call open_outputs(ck%begin)
-    allocate(F(gnx,gny,6),gp(gnx,gny),lam(gnx,gny))
+    allocate(F(gnx,gny,6),gp(gnx,gny),lam(gnx,gny),Fs(gnx,gny,6))
     gp = 0.0_dp
     lam = 0.0_dp
+    npend = 0
     call message('Initialization complete')
 
     wstart = walltime()
@@ -457,8 +464,27 @@ contains
              if (trim(items(k)%field)=='gammap'.or.trim(items(k)%field)=='lambda') need_plastic = .true.
           end if
        end do
-       if (need_body) stat = fdm_get_fields(dev_ctx,c_loc(F(1,1,1)),c_null_ptr)
-       if (need_plastic) stat = fdm_get_plastic(dev_ctx,c_loc(gp(1,1)),c_loc(lam(1,1)))
+       ! body and block outputs stream: a device snapshot of the fields now,
+       ! copied to the host while the next steps run, written when read
+       ! back (at most FDM_SNAPSHOT_SLOTS-1 in flight).  Plastic outputs
+       ! take the synchronous path, after the outputs in flight.
+       deferred = .false.
+       if (need_body.and.(.not.need_plastic)) then
+          if (npend==FDM_SNAPSHOT_SLOTS-1) call flush_outputs(1)
+          stat = fdm_snapshot(dev_ctx,int(2**nF_mode()-1,c_int32_t),slot)
+          if (stat==FDM_OK) then
+             npend = npend+1
+             pslot(npend) = slot
+             pn(npend) = n-1
+             pt(npend) = t
+             deferred = .true.
+          end if
+       end if
+       if ((need_body.or.need_plastic).and.(.not.deferred)) then
+          call flush_outputs(npend)
+          if (need_body) stat = fdm_get_fields(dev_ctx,c_loc(F(1,1,1)),c_null_ptr)
+          if (need_plastic) stat = fdm_get_plastic(dev_ctx,c_loc(gp(1,1)),c_loc(lam(1,1)))
+       end if
        if (final_step) then
           stat = fdm_stage1(dev_ctx,t_init,dt,int(n,c_int64_t))
           nrun = 1
@@ -493,7 +519,9 @@ contains
        end if
        if (due) then
           do k = 1,nitems
-             if (item_due(items(k),n-1)) call write_item(items(k),t,F,gp,lam)
+             if (.not.item_due(items(k),n-1)) cycle
+             if (deferred.and.trim(items(k)%location)/='iface') cycle
+             call write_item(items(k),t,F,gp,lam)
           end do
        end if
        n = n+nrun
@@ -501,6 +529,8 @@ contains
        ! abort is seen at the end of a run of steps
        abort = abort_now(walltime()-wprog)
        if (abort) call message('Aborting now...')
+       ! a checkpoint records the output files as they stand
+       if (ck%ckpt.or.abort) call flush_outputs(npend)
        call ckpt_write_delete(dev_ctx,n-1,abort,ok)
        if (.not.ok) then
           if (setup_error/='') then
@@ -536,6 +566,7 @@ contains
           wave = (walltime()-wstart)/dble(max(1,nt-nstart))
        end if
     end do
+    call flush_outputs(npend)
     stat = fdm_sync(dev_ctx)
     ! the last checkpoint goes once the run has finished (main.f90:233-235)
     if (rc==0.and.(.not.abort).and.ck%del_final) call ckpt_delete(.true.)
@@ -558,6 +589,35 @@ contains
       logical :: w
       w = (a<=x.and.x<=b)
     end function within
+
+    function nF_mode() result(m)
+      integer :: m
+      m = merge(3,6,pb_mode==3)
+    end function nF_mode
+
+    subroutine flush_outputs(m)
+      ! read back the m oldest snapshots and write their body and block items
+      integer,intent(in) :: m
+      integer :: r,kk,st
+      do r = 1,m
+         st = fdm_snapshot_read(dev_ctx,int(pslot(r),c_int32_t),c_loc(Fs(1,1,1)))
+         if (st/=FDM_OK) then
+            call message('Error: '//fdm_error_message(dev_ctx))
+            cycle
+         end if
+         do kk = 1,nitems
+            if (.not.item_due(items(kk),pn(r))) cycle
+            if (trim(items(kk)%location)=='iface') cycle
+            call write_item(items(kk),pt(r),Fs,gp,lam)
+         end do
+      end do
+      if (m>0) then
+         pslot(1:npend-m) = pslot(m+1:npend)
+         pn(1:npend-m) = pn(m+1:npend)
+         pt(1:npend-m) = pt(m+1:npend)
+         npend = npend-m
+      end if
+    end subroutine flush_outputs
   end function run_problem
 
 

@@ -264,6 +264,26 @@ const char* fdm_last_error(const fdm_ctx* ctx);
 int fdm_set_fields(fdm_ctx* ctx, const double* F, const double* DF);
 int fdm_get_fields(fdm_ctx* ctx, double* F, double* DF);
 
+/* Output streaming.  The reference writes its body and block outputs from
+   host arrays at every output step, with the time stepping waiting on it
+   (domain.f90:502-894, output.f90).  Here fdm_snapshot queues, after the
+   work queued so far, a device copy of the fields selected by field_mask
+   (bit l: field l of the mode, vx vy sxx sxy syy szz, or vz sxz syz) into a
+   staging slot of the context, then an asynchronous copy of that slot to
+   pinned host memory on a copy stream of its own, and returns the slot in
+   *slot without waiting: later fdm_step calls run while the copy moves.
+   fdm_snapshot_read waits for that slot's copy and writes the selected
+   fields to out, [nselected][ny][nx] FP64 in field order, as fdm_get_fields
+   would have returned them at the snapshot (FP32-bulk adds each block's
+   background); the slot is then free.  It does not wait for later work, so
+   it does not report a failed Newton solve (fdm_sync does).  There are
+   FDM_SNAPSHOT_SLOTS slots; fdm_snapshot fails with FDM_EINVAL when all are
+   taken.  A slot's device and pinned host buffers (all fields of the mode)
+   are allocated at its first use and kept until fdm_destroy. */
+#define FDM_SNAPSHOT_SLOTS 4
+int fdm_snapshot(fdm_ctx* ctx, uint32_t field_mask, int32_t* slot);
+int fdm_snapshot_read(fdm_ctx* ctx, int32_t slot, double* out);
+
 /* Initial interface solve with dt = 0 at time t: prepare_edges +
    enforce_edge_conditions(D,0d0) at the end of init_domain
    (domain.f90:346-356), including set_rates_friction. */

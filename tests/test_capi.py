@@ -30,7 +30,8 @@ def test_header_declares_the_entry_points():
                      "fdm_get_iface"):
         assert required in names
     # the Python binding knows every declared entry point, and nothing else
-    assert sorted(n for n, _, _ in capi.ENTRY_POINTS + capi.NOALLOC_ENTRY_POINTS) == names
+    assert sorted(n for n, _, _ in capi.ENTRY_POINTS + capi.NOALLOC_ENTRY_POINTS
+                  + capi.STREAM_ENTRY_POINTS) == names
     assert sorted(n for n, _, _ in capi.ENSEMBLE_ENTRY_POINTS) == declared_functions(ENSEMBLE_HEADER)
     assert sorted(n for n, _, _ in capi.ADJOINT_ENTRY_POINTS) == declared_functions(ADJOINT_HEADER)
 
