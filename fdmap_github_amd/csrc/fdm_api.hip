@@ -598,13 +598,26 @@ void launch_layer(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   if (c->nlpts[1]) launch_layer_list<T, false>(c, s, a, 1);
 }
 
-template <typename T, int OP>
-void launch_interior3(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid, dim3 block) {
+template <typename T, int OP, bool HET, bool DISS>
+void launch_interior3_v(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid, dim3 block) {
   const int4* tl = c->tiles[c->tile_shape];
   if (c->nt_store)
-    hipLaunchKernelGGL((bulk_interior3<T, 64, 4, true, OP>), grid, block, 0, s, a, tl);
+    hipLaunchKernelGGL((bulk_interior3<T, 64, 4, true, OP, HET, DISS>), grid, block, 0, s, a, tl);
   else
-    hipLaunchKernelGGL((bulk_interior3<T, 64, 4, false, OP>), grid, block, 0, s, a, tl);
+    hipLaunchKernelGGL((bulk_interior3<T, 64, 4, false, OP, HET, DISS>), grid, block, 0, s, a, tl);
+}
+
+template <typename T, int OP>
+void launch_interior3(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid, dim3 block) {
+  if constexpr (OP == 4) {
+    if (c->diss) {
+      if (c->het) launch_interior3_v<T, OP, true, true>(c, s, a, grid, block);
+      else launch_interior3_v<T, OP, false, true>(c, s, a, grid, block);
+      return;
+    }
+  }
+  if (c->het) launch_interior3_v<T, OP, true, false>(c, s, a, grid, block);
+  else launch_interior3_v<T, OP, false, false>(c, s, a, grid, block);
 }
 
 // one interior tile variant: FULL (sources, MMS, plasticity), or lean with
@@ -1027,14 +1040,13 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   if (P->mode != 0 && P->mode != 2 && P->mode != 3)
     return set_err(nullptr, FDM_EINVAL, "fdm_create: mode must be 2 or 3");
   if (P->mode == 3) {
-    bool unsupported = P->source.nplace > 0 || P->mat_rho;
+    bool unsupported = P->source.nplace > 0;
     for (int b = 0; b < P->nblocks; b++)
-      unsupported |= P->blocks[b].plastic || P->blocks[b].mms || P->blocks[b].Cdiss > 0.0 ||
-                     P->blocks[b].pmlx || P->blocks[b].pmly;
+      unsupported |= P->blocks[b].plastic || P->blocks[b].mms || P->blocks[b].pmlx || P->blocks[b].pmly;
     if (unsupported)
       return set_err(nullptr, FDM_EINVAL,
-                     "fdm_create: mode 3 covers elastic homogeneous blocks without sources, "
-                     "plasticity, manufactured solutions, dissipation or PML");
+                     "fdm_create: mode 3 covers elastic blocks without sources, plasticity, "
+                     "manufactured solutions or PML");
   }
 
   fdm_ctx* c = new fdm_ctx();

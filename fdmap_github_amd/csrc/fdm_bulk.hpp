@@ -498,31 +498,33 @@ __device__ __forceinline__ void finish_point(const BulkArgs<T>& a, const BlockDe
 }
 
 // dissipation terms of rates_dissipation_boundary_mode2 (rates.f90:1485-
-// 2021) for fields 1..5 (the szz relation overwrites the sixth):
+// 2021) for fields 1..5 (the szz relation overwrites the sixth), NF = 5; the
+// same terms of rates_dissipation_boundary_mode3 (rates.f90:1002-1394) for
+// vz, sxz, syz, NF = 3:
 // one-sided "(Jac*F)*A*JacInv*Cdiss", the strips' diagonal "F*AI(0)*Cdiss",
 // central "(Jac(+)*F(+)+Jac(-)*F(-))*AI(s)*JacInv*Cdiss"
-template <typename T>
+template <typename T, int NF = 5>
 __device__ __forceinline__ void diss_one(T (&d)[6], const T* __restrict__ F, const T* __restrict__ J,
                                          long long ps, long long k, T A, T Ji, T Cd) {
   const T Jk = J[k];
 #pragma unroll
-  for (int l = 0; l < 5; l++) d[l] = d[l] + Jk * F[l * ps + k] * A * Ji * Cd;
+  for (int l = 0; l < NF; l++) d[l] = d[l] + Jk * F[l * ps + k] * A * Ji * Cd;
 }
 
-template <typename T>
+template <typename T, int NF = 5>
 __device__ __forceinline__ void diss_diag(T (&d)[6], const T* __restrict__ F, long long ps,
                                           long long cc, T Cd) {
 #pragma unroll
-  for (int l = 0; l < 5; l++) d[l] = d[l] + F[l * ps + cc] * (T)FDM_AI0 * Cd;
+  for (int l = 0; l < NF; l++) d[l] = d[l] + F[l * ps + cc] * (T)FDM_AI0 * Cd;
 }
 
-template <typename T>
+template <typename T, int NF = 5>
 __device__ __forceinline__ void diss_central(T (&d)[6], const T* __restrict__ F, const T* __restrict__ J,
                                              long long ps, long long kp, long long km, T A, T Ji,
                                              T Cd) {
   const T Jp = J[kp], Jm = J[km];
 #pragma unroll
-  for (int l = 0; l < 5; l++) d[l] = d[l] + (Jp * F[l * ps + kp] + Jm * F[l * ps + km]) * A * Ji * Cd;
+  for (int l = 0; l < NF; l++) d[l] = d[l] + (Jp * F[l * ps + kp] + Jm * F[l * ps + km]) * A * Ji * Cd;
 }
 
 // SAT penalties and one-sided stencils of a point within four points of a
@@ -935,6 +937,14 @@ __device__ __forceinline__ void st_off(T* base, unsigned boff, T v) {
 
 // NT: a non-temporal store, for planes that are streamed once per launch
 // when the working set exceeds the last-level cache (no allocation there)
+// a load at a byte offset, non-temporal when NT
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_sel(const T* base, unsigned boff) {
+  const T* p = reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + boff);
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
 // a metric value from its FP32 delta: base + d, exact by construction; a
 // NaN delta (a value the encoding cannot hold) reads the FP64 plane
 __device__ __forceinline__ double metric_of(float d, double base, const double* plane, unsigned boff) {
@@ -1058,12 +1068,15 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
   const bool valid = i <= B4->px - NB && j <= B4->py - NB;
   const unsigned oc = (unsigned)(min(j, ny1) * ld + min(i, nx1)) * (unsigned)sizeof(T);
+  // the lane's own DF1..5 and F6 are read once per launch: with the
+  // non-temporal stores (planes beyond the last-level cache) they are read
+  // non-temporally too
   T dfa[5];
   if (!a.a_zero) {
 #pragma unroll
-    for (int l = 0; l < 5; l++) dfa[l] = ld_off(a.DF + l * ps, oc);
+    for (int l = 0; l < 5; l++) dfa[l] = ld_sel<NTS>(a.DF + l * ps, oc);
   }
-  const T f6 = ld_off(F + 5 * ps, oc);
+  const T f6 = ld_sel<NTS>(F + 5 * ps, oc);
   T Ji = (T)0.0;
   if constexpr (!JI) Ji = ld_off(a.Ji, oc);
   // a heterogeneous medium: the lane's own material (rho, G, M, gamma, nu)
@@ -1263,10 +1276,13 @@ __global__ void __launch_bounds__(64) bulk_band_batch(const BulkArgs<T>* __restr
 // ---------------------------------------------------------------------
 
 // interior point: a(s) = DI(s)/rho, b(s) = DI(s)*G (rates.f90:24-28), s up
-// to the operator's interior half-width H
-template <typename T, typename Acc, int H = 2>
+// to the operator's interior half-width H.  HET: a heterogeneous medium
+// (rates_h_interior_mode3, rates_heterogeneous.f90:8-48), ")*DI(s)*
+// JacInv/rho" and ")*DI(s)*G*JacInv" with the point's rho and G, as(s) then
+// holding DI(s), bs(s) unused.
+template <typename T, typename Acc, int H = 2, bool HET = false>
 __device__ __forceinline__ void interior_rates3(T* d, const Acc& A, const T (&as)[3], const T (&bs)[3],
-                                                T Ji) {
+                                                T Ji, T rho = (T)0, T G = (T)0) {
 #pragma unroll
   for (int s = 1; s <= H; s++) {
     const T ayp = (T)0.5 * (A.yr(s) + A.yr(0)), axp = (T)0.5 * (A.xr(s) + A.xr(0));
@@ -1275,11 +1291,48 @@ __device__ __forceinline__ void interior_rates3(T* d, const Acc& A, const T (&as
     const T bym = (T)0.5 * (A.yq(-s) + A.yq(0)), bxm = (T)0.5 * (A.xq(-s) + A.xq(0));
     const T t1 = ayp * A.f(1, s, 0) - axp * A.f(2, s, 0) - aym * A.f(1, -s, 0) + axm * A.f(2, -s, 0) -
                  byp * A.f(1, 0, s) + bxp * A.f(2, 0, s) + bym * A.f(1, 0, -s) - bxm * A.f(2, 0, -s);
-    d[0] = d[0] + t1 * as[s - 1] * Ji;
+    if constexpr (HET) d[0] = d[0] + t1 * as[s - 1] * Ji / rho;
+    else d[0] = d[0] + t1 * as[s - 1] * Ji;
     const T t2 = ayp * A.f(0, s, 0) - aym * A.f(0, -s, 0) - byp * A.f(0, 0, s) + bym * A.f(0, 0, -s);
-    d[1] = d[1] + t2 * bs[s - 1] * Ji;
+    if constexpr (HET) d[1] = d[1] + t2 * as[s - 1] * G * Ji;
+    else d[1] = d[1] + t2 * bs[s - 1] * Ji;
     const T t3 = -(axp * A.f(0, s, 0)) + axm * A.f(0, -s, 0) + bxp * A.f(0, 0, s) - bxm * A.f(0, 0, -s);
-    d[2] = d[2] + t3 * bs[s - 1] * Ji;
+    if constexpr (HET) d[2] = d[2] + t3 * as[s - 1] * G * Ji;
+    else d[2] = d[2] + t3 * bs[s - 1] * Ji;
+  }
+}
+
+// interior point with artificial dissipation (rates_dissipation_interior_
+// mode3, rates.f90:939-999; rates_h_dissipation_interior_mode3,
+// rates_heterogeneous.f90:920-978): c(0) F first, then per s and field
+// (X)*a(s)*JacInv + c(s)*JacInv*(Y), Y the J-weighted neighbours of the
+// field itself; a(s) as interior_rates3, or with a medium ")*DI(s)/rho*
+// JacInv" and ")*DI(s)*G*JacInv" (as[] then holds DI(s)).  SBP4 only.
+template <typename T, typename Acc, bool HET = false>
+__device__ __forceinline__ void interior_rates3_diss(T* d, const Acc& A, const T (&as)[3], const T (&bs)[3],
+                                                     const T (&dA)[3], T Ji, T rho = (T)0, T G = (T)0) {
+#pragma unroll
+  for (int l = 0; l < 3; l++) d[l] = d[l] + dA[0] * A.f(l, 0, 0);
+#pragma unroll
+  for (int s = 1; s <= 2; s++) {
+    const T ayp = (T)0.5 * (A.yr(s) + A.yr(0)), axp = (T)0.5 * (A.xr(s) + A.xr(0));
+    const T aym = (T)0.5 * (A.yr(-s) + A.yr(0)), axm = (T)0.5 * (A.xr(-s) + A.xr(0));
+    const T byp = (T)0.5 * (A.yq(s) + A.yq(0)), bxp = (T)0.5 * (A.xq(s) + A.xq(0));
+    const T bym = (T)0.5 * (A.yq(-s) + A.yq(0)), bxm = (T)0.5 * (A.xq(-s) + A.xq(0));
+    T x[3];
+    x[0] = ayp * A.f(1, s, 0) - axp * A.f(2, s, 0) - aym * A.f(1, -s, 0) + axm * A.f(2, -s, 0) -
+           byp * A.f(1, 0, s) + bxp * A.f(2, 0, s) + bym * A.f(1, 0, -s) - bxm * A.f(2, 0, -s);
+    x[1] = ayp * A.f(0, s, 0) - aym * A.f(0, -s, 0) - byp * A.f(0, 0, s) + bym * A.f(0, 0, -s);
+    x[2] = -(axp * A.f(0, s, 0)) + axm * A.f(0, -s, 0) + bxp * A.f(0, 0, s) - bxm * A.f(0, 0, -s);
+#pragma unroll
+    for (int l = 0; l < 3; l++) {
+      const T y = A.j(s, 0) * A.f(l, s, 0) + A.j(-s, 0) * A.f(l, -s, 0) + A.j(0, s) * A.f(l, 0, s) +
+                  A.j(0, -s) * A.f(l, 0, -s);
+      T u;
+      if constexpr (HET) u = l == 0 ? x[l] * as[s - 1] / rho * Ji : x[l] * as[s - 1] * G * Ji;
+      else u = x[l] * (l == 0 ? as[s - 1] : bs[s - 1]) * Ji;
+      d[l] = d[l] + u + dA[s] * Ji * y;
+    }
   }
 }
 
@@ -1364,11 +1417,19 @@ __device__ __forceinline__ void closure_rates3(const BulkArgs<T>& a, const Block
   const T* __restrict__ yq = a.yq;
   const T* __restrict__ yr = a.yr;
   const T Ji = a.Ji[cc];
-  const T rho = (T)B.rho, G = (T)B.G;
+  // a heterogeneous medium: the point's rho and G in the same expressions
+  // (rates_h_boundary_mode3, rates_heterogeneous.f90:51-372)
+  const T rho = a.mrho ? a.mrho[cc] : (T)B.rho, G = a.mrho ? a.mG[cc] : (T)B.G;
   const bool qone = (dl < NB) || (dr < NB);
   const bool rone = (db < NB) || (dtp < NB);
   const long long row = (long long)j * ld;
   const long long colm = (long long)B.my * ld + i, colp = (long long)B.py * ld + i;
+  // artificial dissipation (rates_dissipation_boundary_mode3, rates.f90:
+  // 1002-1394; SBP4 only): after each one-sided or central term, and the
+  // strips' diagonal between their two loops; corners have none
+  const bool diss = OP == 4 && B.Cdiss > 0.0;
+  const T* __restrict__ Jp = a.J;
+  const T Cd = (T)B.Cdiss;
   if (qone) {
 #pragma unroll
     for (int s = 0; s < NS; s++) {
@@ -1377,15 +1438,21 @@ __device__ __forceinline__ void closure_rates3(const BulkArgs<T>& a, const Block
       d[0] = d[0] + bscale3<T, 0>(q1_3<T, 0>(F, yr, xr, ps, kq, cc), Dq, rho, G) * Ji;
       d[1] = d[1] + bscale3<T, 1>(q1_3<T, 1>(F, yr, xr, ps, kq, cc), Dq, rho, G) * Ji;
       d[2] = d[2] + bscale3<T, 2>(q1_3<T, 2>(F, yr, xr, ps, kq, cc), Dq, rho, G) * Ji;
+      if constexpr (OP == 4)
+        if (diss) diss_one<T, 3>(d, F, Jp, ps, kq, (T)((dl < 4) ? kAL[dl][s] : kAL[dr][s]), Ji, Cd);
       if (rone) {
         const long long kr = (db < NB) ? colm + (long long)s * ld : colp - (long long)s * ld;
         const T Dr = (db < NB) ? (T)sbp_dl<OP>(db, s) : (T)(-sbp_dl<OP>(dtp, s));
         d[0] = d[0] + bscale3<T, 0>(r1_3<T, 0>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
         d[1] = d[1] + bscale3<T, 1>(r1_3<T, 1>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
         d[2] = d[2] + bscale3<T, 2>(r1_3<T, 2>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
+        if constexpr (OP == 4)
+          if (diss) diss_one<T, 3>(d, F, Jp, ps, kr, (T)((db < 4) ? kAL[db][s] : kAL[dtp][s]), Ji, Cd);
       }
     }
     if (!rone) {
+      if constexpr (OP == 4)
+        if (diss) diss_diag<T, 3>(d, F, ps, cc, Cd);
 #pragma unroll
       for (int s = 1; s <= HW; s++) {
         const T D = (T)sbp_di<OP>(s);
@@ -1393,6 +1460,8 @@ __device__ __forceinline__ void closure_rates3(const BulkArgs<T>& a, const Block
         d[0] = d[0] + bscale3<T, 0>(rc_3<T, 0>(F, yq, xq, ps, kp, km, cc), D, rho, G) * Ji;
         d[1] = d[1] + bscale3<T, 1>(rc_3<T, 1>(F, yq, xq, ps, kp, km, cc), D, rho, G) * Ji;
         d[2] = d[2] + bscale3<T, 2>(rc_3<T, 2>(F, yq, xq, ps, kp, km, cc), D, rho, G) * Ji;
+        if constexpr (OP == 4)
+          if (diss) diss_central<T, 3>(d, F, Jp, ps, kp, km, (s == 1) ? (T)FDM_AI1 : (T)FDM_AI2, Ji, Cd);
       }
     }
   } else {
@@ -1403,7 +1472,11 @@ __device__ __forceinline__ void closure_rates3(const BulkArgs<T>& a, const Block
       d[0] = d[0] + bscale3<T, 0>(r1_3<T, 0>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
       d[1] = d[1] + bscale3<T, 1>(r1_3<T, 1>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
       d[2] = d[2] + bscale3<T, 2>(r1_3<T, 2>(F, yq, xq, ps, kr, cc), Dr, rho, G) * Ji;
+      if constexpr (OP == 4)
+        if (diss) diss_one<T, 3>(d, F, Jp, ps, kr, (T)((db < 4) ? kAL[db][s] : kAL[dtp][s]), Ji, Cd);
     }
+    if constexpr (OP == 4)
+      if (diss) diss_diag<T, 3>(d, F, ps, cc, Cd);
 #pragma unroll
     for (int s = 1; s <= HW; s++) {
       const T D = (T)sbp_di<OP>(s);
@@ -1411,6 +1484,8 @@ __device__ __forceinline__ void closure_rates3(const BulkArgs<T>& a, const Block
       d[0] = d[0] + bscale3<T, 0>(qc_3<T, 0>(F, yr, xr, ps, kp, km, cc), D, rho, G) * Ji;
       d[1] = d[1] + bscale3<T, 1>(qc_3<T, 1>(F, yr, xr, ps, kp, km, cc), D, rho, G) * Ji;
       d[2] = d[2] + bscale3<T, 2>(qc_3<T, 2>(F, yr, xr, ps, kp, km, cc), D, rho, G) * Ji;
+      if constexpr (OP == 4)
+        if (diss) diss_central<T, 3>(d, F, Jp, ps, kp, km, (s == 1) ? (T)FDM_AI1 : (T)FDM_AI2, Ji, Cd);
     }
   }
 }
@@ -1434,11 +1509,34 @@ __device__ __forceinline__ void point_generic3(const BulkArgs<T>& a, const int i
 #pragma unroll
     for (int l = 0; l < 3; l++) d[l] = A * a.DF[l * ps + cc];
   }
-  if (i - B.mx >= NB && B.px - i >= NB && j - B.my >= NB && B.py - j >= NB) {
+  const bool inner = i - B.mx >= NB && B.px - i >= NB && j - B.my >= NB && B.py - j >= NB;
+  bool done = false;
+  if constexpr (OP == 4) {
+    if (inner && B.Cdiss > 0.0) {
+      const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld, a.J};
+      const T dA[3] = {(T)B.dA[0], (T)B.dA[1], (T)B.dA[2]};
+      if (a.mrho) {
+        const T D[3] = {(T)FDM_DI1, (T)FDM_DI2, (T)0};
+        interior_rates3_diss<T, GlobalAcc<T>, true>(d, acc, D, D, dA, a.Ji[cc], a.mrho[cc], a.mG[cc]);
+      } else {
+        const T as[3] = {(T)B.cI[0][0], (T)B.cI[0][1], (T)B.cI[0][2]};
+        const T bs[3] = {(T)B.cI[3][0], (T)B.cI[3][1], (T)B.cI[3][2]};
+        interior_rates3_diss<T, GlobalAcc<T>>(d, acc, as, bs, dA, a.Ji[cc]);
+      }
+      done = true;
+    }
+  }
+  if (done) {
+  } else if (inner) {
     const GlobalAcc<T> acc{a.Fin, a.xq, a.xr, a.yq, a.yr, ps, cc, ld, nullptr};
-    const T as[3] = {(T)B.cI[0][0], (T)B.cI[0][1], (T)B.cI[0][2]};
-    const T bs[3] = {(T)B.cI[3][0], (T)B.cI[3][1], (T)B.cI[3][2]};
-    interior_rates3<T, GlobalAcc<T>, HW>(d, acc, as, bs, a.Ji[cc]);
+    if (a.mrho) {
+      const T D[3] = {(T)sbp_di<OP>(1), HW > 1 ? (T)sbp_di<OP>(2) : (T)0, HW > 2 ? (T)sbp_di<OP>(3) : (T)0};
+      interior_rates3<T, GlobalAcc<T>, HW, true>(d, acc, D, D, a.Ji[cc], a.mrho[cc], a.mG[cc]);
+    } else {
+      const T as[3] = {(T)B.cI[0][0], (T)B.cI[0][1], (T)B.cI[0][2]};
+      const T bs[3] = {(T)B.cI[3][0], (T)B.cI[3][1], (T)B.cI[3][2]};
+      interior_rates3<T, GlobalAcc<T>, HW>(d, acc, as, bs, a.Ji[cc]);
+    }
   } else {
     closure_rates3<T, OP>(a, B, i, j, cc, d);
   }
@@ -1471,10 +1569,13 @@ __global__ void __launch_bounds__(64) bulk_band3(BulkArgs<T> a, const int2* __re
 
 // interior tiles in antiplane shear: vz, sxz, syz with a halo of the
 // operator's half-width, the metrics as in the plane-strain kernel, one
-// barrier
-template <typename T, int TX, int TY, bool NTS, int OP = 4>
-__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(4)))
+// barrier.  HET: a heterogeneous medium, the lane's own rho and G.  DISS:
+// artificial dissipation in some block (SBP4), the Jacobian staged with the
+// halo of F; a block without it keeps the plain expression.
+template <typename T, int TX, int TY, bool NTS, int OP = 4, bool HET = false, bool DISS = false>
+__global__ void __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(DISS ? 3 : 4)))
 bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
+  static_assert(!DISS || OP == 4, "artificial dissipation is built for SBP4 only");
   constexpr int H = Sbp<OP>::half, NB = Sbp<OP>::nbst;
   constexpr int HX = TX + 2 * H, HY = TY + 2 * H, NT = TX * TY;
   constexpr int NF = (HY * HX + NT - 1) / NT;
@@ -1483,6 +1584,7 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
   __shared__ T sF[3][HY][HX];
   __shared__ T sR[2][TY][HX];
   __shared__ T sQ[2][HY][TX];
+  __shared__ T sJ[DISS ? HY : 1][DISS ? HX : 1];
   const int4 td = tiles[linear_tile(a.xcd_swizzle)];
   const int bu = td.x, i0 = td.y, j0 = td.z;
   ConstBlockDev* B4 = (ConstBlockDev*)a.blocks + bu;
@@ -1490,7 +1592,7 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
   const int ld = a.ld, nx1 = a.nx - 1, ny1 = a.ny - 1;
   const long long ps = a.ps;
   const T* __restrict__ F = a.Fin;
-  T vF[NF][3], vR[NR][2], vQ[NQ][2];
+  T vF[NF][3], vR[NR][2], vQ[NQ][2], vJ[DISS ? NF : 1];
 #pragma unroll
   for (int p = 0; p < NF; p++) {
     const int k = tid + p * NT;
@@ -1500,6 +1602,7 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
       const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
 #pragma unroll
       for (int l = 0; l < 3; l++) vF[p][l] = ld_off(F + l * ps, o);
+      if constexpr (DISS) vJ[p] = ld_off(a.J, o);
     }
   }
 #pragma unroll
@@ -1533,6 +1636,11 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
     for (int l = 0; l < 3; l++) dfa[l] = ld_off(a.DF + l * ps, oc);
   }
   const T Ji = ld_off(a.Ji, oc);
+  T mrho = (T)0, mG = (T)0;
+  if constexpr (HET) {
+    mrho = ld_off(a.mrho, oc);
+    mG = ld_off(a.mG, oc);
+  }
 #pragma unroll
   for (int p = 0; p < NF; p++) {
     const int k = tid + p * NT;
@@ -1540,6 +1648,7 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
       const int r = k / HX, c = k - r * HX;
 #pragma unroll
       for (int l = 0; l < 3; l++) sF[l][r][c] = vF[p][l];
+      if constexpr (DISS) sJ[r][c] = vJ[p];
     }
   }
 #pragma unroll
@@ -1571,10 +1680,31 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
 #pragma unroll
     for (int l = 0; l < 3; l++) d[l] = A * dfa[l];
   }
-  const TileAcc<T, TX, TY, H> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y, nullptr};
-  const T as[3] = {(T)B4->cI[0][0], (T)B4->cI[0][1], (T)B4->cI[0][2]};
-  const T bs[3] = {(T)B4->cI[3][0], (T)B4->cI[3][1], (T)B4->cI[3][2]};
-  interior_rates3<T, TileAcc<T, TX, TY, H>, H>(d, acc, as, bs, Ji);
+  const TileAcc<T, TX, TY, H> acc{sF, sR, sQ, (int)threadIdx.x, (int)threadIdx.y, &sJ[0][0]};
+  bool diss_done = false;
+  if constexpr (DISS) {
+    if (B4->Cdiss > 0.0) {
+      const T dA[3] = {(T)B4->dA[0], (T)B4->dA[1], (T)B4->dA[2]};
+      if constexpr (HET) {
+        const T D[3] = {(T)FDM_DI1, (T)FDM_DI2, (T)0};
+        interior_rates3_diss<T, TileAcc<T, TX, TY, H>, true>(d, acc, D, D, dA, Ji, mrho, mG);
+      } else {
+        const T as[3] = {(T)B4->cI[0][0], (T)B4->cI[0][1], (T)B4->cI[0][2]};
+        const T bs[3] = {(T)B4->cI[3][0], (T)B4->cI[3][1], (T)B4->cI[3][2]};
+        interior_rates3_diss<T, TileAcc<T, TX, TY, H>>(d, acc, as, bs, dA, Ji);
+      }
+      diss_done = true;
+    }
+  }
+  if (diss_done) {
+  } else if constexpr (HET) {
+    const T D[3] = {(T)sbp_di<OP>(1), H > 1 ? (T)sbp_di<OP>(2) : (T)0, H > 2 ? (T)sbp_di<OP>(3) : (T)0};
+    interior_rates3<T, TileAcc<T, TX, TY, H>, H, true>(d, acc, D, D, Ji, mrho, mG);
+  } else {
+    const T as[3] = {(T)B4->cI[0][0], (T)B4->cI[0][1], (T)B4->cI[0][2]};
+    const T bs[3] = {(T)B4->cI[3][0], (T)B4->cI[3][1], (T)B4->cI[3][2]};
+    interior_rates3<T, TileAcc<T, TX, TY, H>, H>(d, acc, as, bs, Ji);
+  }
   if (a.do_update) {
     const T Bdt = (T)a.Bdt;
 #pragma unroll

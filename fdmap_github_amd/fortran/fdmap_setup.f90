@@ -870,8 +870,8 @@ contains
             ! fields.f90:167-174, 803-809
             if (index(problem,'mms')>0.or.index(problem,'inplane')>0) &
                  call fail("fields problem '"//trim(problem)//"' is not available in mode 3")
-            if (trim(response)=='plastic'.or.heterogeneous.or.HBk%Cdiss>0.0_dp) &
-                 call fail('mode 3 covers elastic homogeneous blocks without dissipation')
+            if (trim(response)=='plastic') &
+                 call fail('mode 3 covers elastic blocks (no plastic response)')
             if (setup_error/='') return
             F0 = (/ vz0,sxz0,syz0,0.0_dp,0.0_dp,0.0_dp /)
             FP(:,1) = (/ P1%vz,P1%sxz,P1%syz,0.0_dp,0.0_dp,0.0_dp /)

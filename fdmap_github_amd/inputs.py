@@ -157,12 +157,17 @@ D V S N Psi
 
 def antiplane_rs(name="m3", nx=301, ny=301, h=0.1, nsteps=1000, outputs=True, law="RAL",
                  sxx0=-50.0, sxz0=30.0, a=0.010, b=0.016, L=0.05, Psi0=0.6, ld_S=5.0,
-                 ld_T=0.0, ld_width=4.0, ninfo=100):
+                 ld_T=0.0, ld_width=4.0, ninfo=100, medium=None, fdmethod="SBP4"):
     """Antiplane shear (mode 3) rupture after the reference's mode-3 example
     (problems/kali-heterogeneous-input.in): two blocks side by side with a
     vertical rate-and-state fault between them, a free surface on top,
     absorbing elsewhere; the fault normal stress is -sxx0 and the shear
-    prestress sxz0, with a Gaussian shear load ld_S at mid-depth."""
+    prestress sxz0, with a Gaussian shear load ld_S at mid-depth.  medium:
+    path of a heterogeneous medium file (write_medium) of the whole grid
+    that both blocks read."""
+    mat = f"rho = {_f(RHO)}, cs = {_f(CS)}, cp = {_f(CP)}"
+    if medium:
+        mat = f"{mat}, heterogeneous = T, filename = '{medium}'"
     Lx = (nx - 1) * h
     Ly = (ny - 1) * h
     out = ""
@@ -181,7 +186,7 @@ vz sxz syz
  name = '{name}', nt = {nsteps + 1}, CFL = 0.5d0, RKorder = 4, ninfo = {ninfo} /
 
 &domain_list
- mode = 3, FDmethod = 'SBP4', nblocks_x = 2, nblocks_y = 1, nx = {2 * nx}, ny = {ny} /
+ mode = 3,{_fdm(fdmethod)} nblocks_x = 2, nblocks_y = 1, nx = {2 * nx}, ny = {ny} /
 
 !---BLOCK1---
 
@@ -203,7 +208,7 @@ vz sxz syz
 
 !---ALL BLOCKS---
 
-&material_list rho = {_f(RHO)}, cs = {_f(CS)}, cp = {_f(CP)} /
+&material_list {mat} /
 
 &fields_list sxx0 = {_f(sxx0)}, sxz0 = {_f(sxz0)} /
 

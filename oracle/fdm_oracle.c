@@ -593,11 +593,10 @@ int orc_create(const fdm_problem* P, orc_ctx** out) {
   for (int ib = 0; ib < c->nb; ib++) {
     const fdm_block* b = &P->blocks[ib];
     c->B[ib].b = *b;
-    if (c->mode == 3 && (b->plastic || b->mms || b->Cdiss > 0.0 || P->mat_rho ||
-                         P->source.nplace > 0 || b->pmlx || b->pmly)) {
+    if (c->mode == 3 && (b->plastic || b->mms || P->source.nplace > 0 || b->pmlx || b->pmly)) {
       snprintf(g_err, sizeof g_err,
-               "orc_create: mode 3 covers elastic homogeneous blocks without sources, "
-               "dissipation or PML");
+               "orc_create: mode 3 covers elastic blocks without sources, plasticity, "
+               "manufactured solutions or PML");
       orc_destroy(c);
       return FDM_EINVAL;
     }
@@ -1827,6 +1826,20 @@ static inline void diss_central(const sten_t* S, size_t kp, size_t km, size_t cc
     S->DF[l][cc] = S->DF[l][cc] + (S->J[kp] * S->F[l][kp] + S->J[km] * S->F[l][km]) * AIc(s) *
                                       S->Ji[cc] * S->Cdiss;
 }
+/* the same terms for the three antiplane fields (rates_dissipation_boundary_
+   mode3, rates.f90:1002-1394) */
+static inline void diss_one3(const sten_t* S, size_t k, size_t cc, double A) {
+  for (int l = 0; l < 3; l++)
+    S->DF[l][cc] = S->DF[l][cc] + S->J[k] * S->F[l][k] * A * S->Ji[cc] * S->Cdiss;
+}
+static inline void diss_diag3(const sten_t* S, size_t cc) {
+  for (int l = 0; l < 3; l++) S->DF[l][cc] = S->DF[l][cc] + S->F[l][cc] * AIc(0) * S->Cdiss;
+}
+static inline void diss_central3(const sten_t* S, size_t kp, size_t km, size_t cc, int s) {
+  for (int l = 0; l < 3; l++)
+    S->DF[l][cc] = S->DF[l][cc] + (S->J[kp] * S->F[l][kp] + S->J[km] * S->F[l][km]) * AIc(s) *
+                                      S->Ji[cc] * S->Cdiss;
+}
 
 /* rates_boundary_mode2, rates.f90:453-917.  qside: 0 none (central q),
    1 left one-sided, 2 right one-sided; rside likewise for r.  (mx,px,my,py)
@@ -1969,8 +1982,17 @@ static inline double bscale3(int l, double t, double D, double rho, double G) {
 }
 
 /* rates_interior_mode3 (rates.f90:8-56): a(s) = DI(s)/rho, b(s) = DI(s)*G;
-   the s = 3 coefficient is zero */
+   the s = 3 coefficient is zero.  With a heterogeneous medium,
+   rates_h_interior_mode3 (rates_heterogeneous.f90:8-48): the point's rho
+   and G, as ")*DI(s)*JacInv(i,j)/rho(i,j)" and ")*DI(s)*G(i,j)*JacInv(i,j)",
+   evaluated left to right. */
+static void rates_dissipation_interior3(const sten_t* S, int mx, int px, int my, int py);
+
 static void rates_interior3(const sten_t* S, int mx, int px, int my, int py) {
+  if (S->Cdiss > 0.0) {
+    rates_dissipation_interior3(S, mx, px, my, py);
+    return;
+  }
   double coef[3][4];
   for (int s = 1; s <= PI_I; s++) {
     coef[0][s] = DIc(s) / S->rho;
@@ -1991,57 +2013,130 @@ static void rates_interior3(const sten_t* S, int mx, int px, int my, int py) {
         const double *F1 = S->F[0], *F2 = S->F[1], *F3 = S->F[2];
         double t1 = ayp * F2[ip] - axp * F3[ip] - aym * F2[im] + axm * F3[im] - byp * F2[jp] +
                     bxp * F3[jp] + bym * F2[jm] - bxm * F3[jm];
-        S->DF[0][cc] = S->DF[0][cc] + t1 * coef[0][s] * S->Ji[cc];
         double t2 = ayp * F1[ip] - aym * F1[im] - byp * F1[jp] + bym * F1[jm];
-        S->DF[1][cc] = S->DF[1][cc] + t2 * coef[1][s] * S->Ji[cc];
         double t3 = -(axp * F1[ip]) + axm * F1[im] + bxp * F1[jp] - bxm * F1[jm];
-        S->DF[2][cc] = S->DF[2][cc] + t3 * coef[2][s] * S->Ji[cc];
+        if (S->prho) {
+          const double D = DIc(s), r = S->prho[cc], G = S->pG[cc];
+          S->DF[0][cc] = S->DF[0][cc] + t1 * D * S->Ji[cc] / r;
+          S->DF[1][cc] = S->DF[1][cc] + t2 * D * G * S->Ji[cc];
+          S->DF[2][cc] = S->DF[2][cc] + t3 * D * G * S->Ji[cc];
+        } else {
+          S->DF[0][cc] = S->DF[0][cc] + t1 * coef[0][s] * S->Ji[cc];
+          S->DF[1][cc] = S->DF[1][cc] + t2 * coef[1][s] * S->Ji[cc];
+          S->DF[2][cc] = S->DF[2][cc] + t3 * coef[2][s] * S->Ji[cc];
+        }
       }
     }
 }
 
 /* rates_boundary_mode3 (rates.f90:59-381), the loop structure of the mode-2
-   routine */
+   routine; with a heterogeneous medium rates_h_boundary_mode3
+   (rates_heterogeneous.f90:51-372) takes the point's rho and G in the same
+   expressions */
 static void rates_boundary3(const sten_t* S, int mx, int px, int my, int py, int qside,
                             int rside, int bx0, int bx1, int by0, int by1) {
   int nx = S->nx;
-  double rho = S->rho, G = S->G;
+  const int diss = S->Cdiss > 0.0;
   for (int j = my; j <= py; j++)
     for (int i = mx; i <= px; i++) {
       size_t cc = (size_t)j * nx + i;
       double Ji = S->Ji[cc];
+      const double rho = S->prho ? S->prho[cc] : S->rho, G = S->pG ? S->pG[cc] : S->G;
       if (qside) {
         for (int s = 0; s < NBND; s++) {
           size_t kq = qside == 1 ? (size_t)j * nx + (bx0 + s) : (size_t)j * nx + (bx1 - s);
           double Dq = qside == 1 ? DLc(s, i - bx0) : DRc(s, bx1 - i);
+          double Aq = qside == 1 ? ALc(s, i - bx0) : ALc(s, bx1 - i);
           for (int l = 0; l < 3; l++)
             S->DF[l][cc] = S->DF[l][cc] + bscale3(l, q1m3(S, l, kq, cc), Dq, rho, G) * Ji;
+          if (diss) diss_one3(S, kq, cc, Aq);
           if (rside) {
             size_t kr = rside == 1 ? (size_t)(by0 + s) * nx + i : (size_t)(by1 - s) * nx + i;
             double Dr = rside == 1 ? DLc(s, j - by0) : DRc(s, by1 - j);
+            double Ar = rside == 1 ? ALc(s, j - by0) : ALc(s, by1 - j);
             for (int l = 0; l < 3; l++)
               S->DF[l][cc] = S->DF[l][cc] + bscale3(l, r1m3(S, l, kr, cc), Dr, rho, G) * Ji;
+            if (diss) diss_one3(S, kr, cc, Ar);
           }
         }
-        if (!rside)
+        if (!rside) {
+          if (diss) diss_diag3(S, cc);
           for (int s = 1; s <= PI_I; s++) {
             size_t jp = cc + (size_t)s * nx, jm = cc - (size_t)s * nx;
             for (int l = 0; l < 3; l++)
               S->DF[l][cc] =
                   S->DF[l][cc] + bscale3(l, rcm3(S, l, jp, jm, cc), DIc(s), rho, G) * Ji;
+            if (diss) diss_central3(S, jp, jm, cc, s);
           }
+        }
       } else {
         for (int s = 0; s < NBND; s++) {
           size_t kr = rside == 1 ? (size_t)(by0 + s) * nx + i : (size_t)(by1 - s) * nx + i;
           double Dr = rside == 1 ? DLc(s, j - by0) : DRc(s, by1 - j);
+          double Ar = rside == 1 ? ALc(s, j - by0) : ALc(s, by1 - j);
           for (int l = 0; l < 3; l++)
             S->DF[l][cc] = S->DF[l][cc] + bscale3(l, r1m3(S, l, kr, cc), Dr, rho, G) * Ji;
+          if (diss) diss_one3(S, kr, cc, Ar);
         }
+        if (diss) diss_diag3(S, cc);
         for (int s = 1; s <= PI_I; s++) {
           size_t ip = cc + s, im = cc - s;
           for (int l = 0; l < 3; l++)
             S->DF[l][cc] =
                 S->DF[l][cc] + bscale3(l, qcm3(S, l, ip, im, cc), DIc(s), rho, G) * Ji;
+          if (diss) diss_central3(S, ip, im, cc, s);
+        }
+      }
+    }
+}
+
+/* rates_dissipation_interior_mode3 (rates.f90:939-999) and its heterogeneous
+   form (rates_heterogeneous.f90:920-978): c(0) F first, then per s and
+   field (X)*a(s)*JacInv + c(s)*JacInv*(Y), with a(s) = DI(s)/rho for vz and
+   DI(s)*G for the stresses, or ")*DI(s)/rho(i,j)*JacInv" and
+   ")*DI(s)*G(i,j)*JacInv" with a medium; Y the J-weighted neighbours of
+   the field itself.  The s = 3 terms are zero for SBP4 (the only operator
+   with dissipation). */
+static void rates_dissipation_interior3(const sten_t* S, int mx, int px, int my, int py) {
+  double c[4], a[3][4];
+  c[0] = 2.0 * AIc(0) * S->Cdiss;
+  for (int s = 1; s <= 3; s++) {
+    c[s] = s <= PI_I ? AIc(s) * S->Cdiss : 0.0;
+    const double D = s <= PI_I ? DIc(s) : 0.0;
+    a[0][s] = D / S->rho;
+    a[1][s] = D * S->G;
+    a[2][s] = D * S->G;
+  }
+  int nx = S->nx;
+  const double* J = S->J;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+  for (int j = my; j <= py; j++)
+    for (int i = mx; i <= px; i++) {
+      size_t cc = (size_t)j * nx + i;
+      const double *F1 = S->F[0], *F2 = S->F[1], *F3 = S->F[2];
+      for (int l = 0; l < 3; l++) S->DF[l][cc] = S->DF[l][cc] + c[0] * S->F[l][cc];
+      for (int s = 1; s <= PI_I; s++) {
+        size_t ip = cc + s, im = cc - s, jp = cc + (size_t)s * nx, jm = cc - (size_t)s * nx;
+        double ayp = 0.5 * (S->yr[ip] + S->yr[cc]), axp = 0.5 * (S->xr[ip] + S->xr[cc]);
+        double aym = 0.5 * (S->yr[im] + S->yr[cc]), axm = 0.5 * (S->xr[im] + S->xr[cc]);
+        double byp = 0.5 * (S->yq[jp] + S->yq[cc]), bxp = 0.5 * (S->xq[jp] + S->xq[cc]);
+        double bym = 0.5 * (S->yq[jm] + S->yq[cc]), bxm = 0.5 * (S->xq[jm] + S->xq[cc]);
+        double x[3];
+        x[0] = ayp * F2[ip] - axp * F3[ip] - aym * F2[im] + axm * F3[im] - byp * F2[jp] +
+               bxp * F3[jp] + bym * F2[jm] - bxm * F3[jm];
+        x[1] = ayp * F1[ip] - aym * F1[im] - byp * F1[jp] + bym * F1[jm];
+        x[2] = -(axp * F1[ip]) + axm * F1[im] + bxp * F1[jp] - bxm * F1[jm];
+        for (int l = 0; l < 3; l++) {
+          const double* Fl = S->F[l];
+          double y = J[ip] * Fl[ip] + J[im] * Fl[im] + J[jp] * Fl[jp] + J[jm] * Fl[jm];
+          double u;
+          if (S->prho) {
+            const double D = DIc(s);
+            u = l == 0 ? x[l] * D / S->prho[cc] * S->Ji[cc] : x[l] * D * S->pG[cc] * S->Ji[cc];
+          } else {
+            u = x[l] * a[l][s] * S->Ji[cc];
+          }
+          S->DF[l][cc] = S->DF[l][cc] + u + c[s] * S->Ji[cc] * y;
         }
       }
     }

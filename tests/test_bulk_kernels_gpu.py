@@ -62,6 +62,11 @@ CASES = {
     # antiplane shear with a fault
     "antiplane": dict(nx=203, nys=(61, 77), seed=47, warp=0.04, background=[0.0, 0.0, 70.0],
                       perturb=1e-3, law="RFL", mode=3, inplane=(-120.0, 0.0, -100.0, -110.0)),
+    # antiplane shear in a heterogeneous medium, dissipation in one block row
+    "antiplane_medium_dissipation": dict(nx=203, nys=(61, 77), seed=49, warp=0.04,
+                                         background=[0.0, 0.0, 70.0], perturb=1e-3, law="RFL",
+                                         mode=3, inplane=(-120.0, 0.0, -100.0, -110.0),
+                                         medium=0.3, Cdiss=[300.0, 0.0]),
     # plasticity and a point source: the full finishing path
     "plastic_source": dict(nx=130, nys=(40, 44), seed=31, warp=0.02,
                            background=[0.0, 0.0, -60.0, 44.0, -100.0, -80.0], perturb=0.0,

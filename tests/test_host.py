@@ -324,7 +324,7 @@ def test_antiplane_problem(tmp_path):
     txt = open(p).read()
     bad = txt.replace("rho = 2.67d0", "response = 'plastic', rho = 2.67d0")
     assert bad != txt
-    with pytest.raises(RuntimeError, match="mode 3 covers"):
+    with pytest.raises(RuntimeError, match="mode 3 covers elastic blocks"):
         host.HostProblem(inputs.write(str(tmp_path / "b1.in"), bad))
     bad = txt.replace("bcB = 'absorbing', bcT = 'free' /", "bcB = 'absorbing-velocity', bcT = 'free' /", 1)
     assert bad != txt

@@ -116,6 +116,28 @@ def test_antiplane_rupture_from_input_file(gpu, tmp_path):
     assert orc.get_iface(0, "V").max() > 1e-3  # rupture nucleated
 
 
+def test_antiplane_rupture_in_a_medium_from_input_file(gpu, tmp_path):
+    """the antiplane rupture with a heterogeneous medium file of the whole
+    grid (material.f90:226-321): the host hands rho, cs, cp of every point
+    and the edge impedances at each node; HIP against the oracle"""
+    nx, ny, h = 61, 61, 0.1
+    rho, cs, cp = inputs.smooth_medium(2 * nx, ny, h, amp=0.15, seed=4)
+    med = inputs.write_medium(str(tmp_path / "m3.medium"), rho, cs, cp)
+    p = inputs.write(str(tmp_path / "m3h.in"),
+                     inputs.antiplane_rs(name=str(tmp_path / "m3h"), nx=nx, ny=ny, h=h, nsteps=250,
+                                         outputs=False, medium=med))
+    hp, dev, orc = run_pair(p, 250)
+    assert hp.ptr.contents.mat_rho
+    e = rel_l2(dev.get_fields()[0], orc.get_fields()[0])
+    print("fields", e)
+    assert e <= 1e-12
+    for f in ("D", "V", "S", "N", "Psi"):
+        a, b = dev.get_iface(0, f), orc.get_iface(0, f)
+        print(f, rel_l2(a, b))
+        assert rel_l2(a, b) <= 1e-12
+    assert orc.get_iface(0, "V").max() > 1e-3
+
+
 def test_antiplane_executable_outputs(gpu, tmp_path):
     nsteps = 60
     p = inputs.write(str(tmp_path / "m3x.in"),

@@ -609,6 +609,38 @@ def test_stage1_dissipation_matches_dense_operator(medium):
     assert err.max() < 1e-12, err.max()
 
 
+@pytest.mark.parametrize("medium", [None, 0.3])
+def test_stage1_dissipation_mode3_matches_dense_operator(medium):
+    """rates_dissipation_*_mode3 (rates.f90:939-1394, rates_heterogeneous.f90:
+    920-1373): the antiplane rates with Cdiss exceed those without by
+    Cdiss Ji (A_q + A_r)(J F) in each block, for vz, sxz and syz"""
+    kw = dict(nx=26, nys=(17, 19), seed=5, warp=0.06, background=[0.1, 30.0, -20.0],
+              perturb=0.5, coupling="locked", medium=medium, mode=3,
+              inplane=(-120.0, 0.0, -100.0, -110.0))
+    out = []
+    for Cd in (0.0, 2000.0):
+        prob, F = problems.build(Cdiss=Cd, **kw)
+        o = OracleContext(prob)
+        o.set_fields(F)
+        o.init_edges(0.0)
+        o.stage1(0.0, 1e-3, 1)
+        out.append(o.get_fields()[1])
+    diff = out[1] - out[0]
+    JF = prob.J[None] * F
+    ref = np.zeros_like(diff)
+    for b in prob.blocks:
+        sj, si = slice(b["my"], b["py"] + 1), slice(b["mx"], b["px"] + 1)
+        for l in range(3):
+            blk = JF[l, sj, si]
+            aq = np.array([ob.diss1d(row) for row in blk])
+            ar = np.array([ob.diss1d(col) for col in blk.T]).T
+            ref[l, sj, si] = 2000.0 * prob.Ji[sj, si] * (aq + ar)
+    scale = np.abs(out[0]).max(axis=(1, 2), keepdims=True)
+    err = np.abs(diff - ref) / scale
+    assert np.all(np.abs(ref).max(axis=(1, 2)) > 0.05 * scale[:, 0, 0])
+    assert err.max() < 1e-12, err.max()
+
+
 def energy(prob, F, rho, cs, cp, order=4):
     W = np.outer(norm_weights(prob.ny, order), norm_weights(prob.nx, order)) / prob.Ji
     G = rho * cs ** 2
@@ -666,6 +698,57 @@ def test_stage1_rates_mode3_match_dense_split_operator(seed, warp):
     scale = np.abs(ref[inner]).max(axis=(1, 2), keepdims=True)
     err = np.abs(DF[inner] - ref[inner]) / scale
     assert err.max() < 1e-12, err.max()
+
+
+@pytest.mark.parametrize("seed,warp", [(13, 0.0), (14, 0.08)])
+def test_stage1_rates_mode3_heterogeneous_match_dense_split_operator(seed, warp):
+    """rates_h_interior_mode3 / rates_h_boundary_mode3 (rates_heterogeneous.
+    f90:8-372): the point's rho and G scale the same split-form operator,
+    DF1 = (dx sxz + dy syz)/rho, DF2 = G dx vz, DF3 = G dy vz"""
+    cs, cp, rho = 3.464, 6.0, 2.67
+    prob, F = problems.build(nx=13, nys=(15,), seed=seed, warp=warp, cs=cs, cp=cp, rho=rho,
+                             background=[0.1, 30.0, -20.0], perturb=0.5, mode=3,
+                             bcs=("absorbing", "free", "rigid", "free"), medium=0.3)
+    o = OracleContext(prob)
+    o.set_fields(F)
+    o.init_edges(0.0)
+    o.stage1(0.0, 1e-3, 1)
+    _, DF = o.get_fields()
+    r, s_, _ = prob.medium
+    assert np.ptp(r) > 0.1 * rho and np.ptp(s_) > 0.1 * cs
+    G = r * s_ ** 2
+    dx, dy = split_ops(prob)
+    vz, sxz, syz = F
+    ref = np.array([(dx(sxz) + dy(syz)) / r, G * dx(vz), G * dy(vz)])
+    inner = (slice(None), slice(1, -1), slice(1, -1))
+    scale = np.abs(ref[inner]).max(axis=(1, 2), keepdims=True)
+    err = np.abs(DF[inner] - ref[inner]) / scale
+    assert err.max() < 1e-12, err.max()
+
+
+def test_mode3_uniform_medium_matches_homogeneous_material():
+    """antiplane shear with a medium file holding the block's own constants:
+    the heterogeneous path differs from the homogeneous one only in
+    rounding order, through a rupture"""
+    cs, cp, rho = 3.464, 6.0, 2.67
+    kw = dict(nx=40, nys=(20, 24), seed=9, warp=0.03, cs=cs, cp=cp, rho=rho,
+              background=[0.0, 0.0, 70.0], perturb=1e-2, law="RFL", fault_overstress=25.0,
+              mode=3, inplane=(-120.0, 0.0, -100.0, -110.0))
+    out = []
+    for uniform in (False, True):
+        prob, F = problems.build(**kw)
+        if uniform:
+            prob.medium = (np.full((prob.ny, prob.nx), rho), np.full((prob.ny, prob.nx), cs),
+                           np.full((prob.ny, prob.nx), cp))
+        o = OracleContext(prob)
+        o.set_fields(F)
+        o.init_edges(0.0)
+        o.step(0.0, 1e-3, 1, 40)
+        out.append((o.get_fields()[0], o.get_iface(0, "V")))
+    a, b = out
+    assert np.abs(a[0] - b[0]).max() / np.abs(a[0]).max() < 1e-13
+    assert np.abs(a[1] - b[1]).max() / np.abs(a[1]).max() < 1e-12
+    assert not np.array_equal(a[0], b[0])  # the two paths round differently
 
 
 def energy3(prob, F, rho, cs):

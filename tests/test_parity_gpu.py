@@ -302,6 +302,47 @@ def test_antiplane_locked_2x2_bit_exact(gpu):
     compare(dev, orc, 0, bit_exact=True)
 
 
+@pytest.mark.parametrize("sbp", [4, 6])
+def test_antiplane_heterogeneous_medium_bit_exact(gpu, sbp):
+    """rates_h_interior_mode3 / rates_h_boundary_mode3 (rates_heterogeneous.
+    f90:8-372): the point's rho and G in the interior tiles and the band,
+    locked 2 x 2 blocks, bit for bit"""
+    prob, F = problems.build(nx=70, nys=(40, 37), nbx=2, seed=63, warp=0.04, background=BG3,
+                             perturb=1e-2, coupling="locked", mode=3, inplane=INPLANE,
+                             medium=0.3, sbp=sbp)
+    dev, orc = run_both(prob, F, 20)
+    compare(dev, orc, 0, bit_exact=True)
+
+
+@pytest.mark.parametrize("medium", [None, 0.3])
+def test_antiplane_dissipation_bit_exact(gpu, medium):
+    """rates_dissipation_*_mode3 (rates.f90:939-1394) and the heterogeneous
+    forms (rates_heterogeneous.f90:920-1373): dissipation in one of two
+    block rows, locked, with and without a medium, bit for bit"""
+    prob, F = problems.build(nx=70, nys=(40, 37), seed=67, warp=0.04, background=BG3,
+                             perturb=1e-2, coupling="locked", mode=3, inplane=INPLANE,
+                             medium=medium, Cdiss=[300.0, 0.0])
+    dev, orc = run_both(prob, F, 20)
+    compare(dev, orc, 0, bit_exact=True)
+
+
+def test_antiplane_dissipation_rupture(gpu):
+    prob, F = problems.build(nx=96, nys=(30, 34), seed=69, warp=0.03, background=[0.0, 0.0, 70.0],
+                             perturb=0.0, law="RFL", fault_overstress=25.0, mode=3,
+                             inplane=INPLANE, Cdiss=300.0, medium=0.2)
+    dev, orc = run_both(prob, F, 60)
+    compare(dev, orc, 1, bit_exact=False)
+
+
+def test_antiplane_heterogeneous_rupture(gpu):
+    prob, F = problems.build(nx=96, nys=(30, 34), seed=65, warp=0.03, background=[0.0, 0.0, 70.0],
+                             perturb=0.0, law="RFL", fault_overstress=25.0, mode=3,
+                             inplane=INPLANE, medium=0.2)
+    dev, orc = run_both(prob, F, 60)
+    compare(dev, orc, 1, bit_exact=False)
+    assert orc.get_iface(0, "V").max() > 1e-3
+
+
 @pytest.mark.parametrize("law", ["RFL", "RAL", "SL", "SW", "frictionless"])
 def test_antiplane_friction_laws(gpu, law):
     """frictional_interface_mode3 (interfaces.f90:806-897)"""
