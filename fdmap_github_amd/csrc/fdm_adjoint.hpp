@@ -176,13 +176,79 @@ __global__ void __launch_bounds__(256) adj_bulk_kernel(AdjBulkArgs a) {
   adj_point(a, i, j);
 }
 
-// points within six of their block's edge (closure rows reach them, or SAT)
+// The gathers of one direction with the direction a run-time choice (q:
+// columns of row j, r: rows of column i): 11 rows from max(0, c-5), D = 0
+// where the stencil does not reach (a central point's rows included, which
+// leaves its sum unchanged), the r direction's signs folded into ay and ax
+// (exact, -(a*b) = (-a)*b), so both lanes of a pair run one code path.
+__device__ __forceinline__ void adj_dir_any(const AdjBulkArgs& a, double (&g)[5], const int c, const int n,
+                                            const long long base, const long long step,
+                                            const long long cc, const double (&cl)[5], const double gam,
+                                            const bool q) {
+  const double* __restrict__ m1 = q ? a.yr : a.yq;
+  const double* __restrict__ m2 = q ? a.xr : a.xq;
+  const double sg = q ? 1.0 : -1.0;
+  const double y0 = m1[cc], x0 = m2[cc];
+  const long long ps = a.ps;
+  const int r0 = max(0, c - 5), r1 = min(n - 1, c + 5);
+#pragma unroll
+  for (int t = 0; t < 11; t++) {
+    const int r = min(r0 + t, r1);
+    const double D = (r0 + t <= r1) ? sbp_entry(r, c, n) : 0.0;
+    const long long kk = base + (long long)r * step;
+    const double Jk = a.Ji[kk];
+    double w[5];
+#pragma unroll
+    for (int l = 0; l < 5; l++) w[l] = a.min[l * ps + kk] * Jk * cl[l];
+    const double ay = sg * (0.5 * (m1[kk] + y0) * D), ax = sg * (0.5 * (m2[kk] + x0) * D);
+    adj_row<true>(g, w, ay, ax, gam);
+  }
+}
+
+// points within six of their block's edge (closure rows reach them, or
+// SAT), two lanes per point: the even lane gathers the q-derivative rows,
+// the odd lane the r-derivative rows (so each lane's chain of dependent
+// loads is half as long, and twice the waves are in flight); the pair adds
+// its sums, and the even lane adds the edge contributions and finishes the
+// point as adj_point does
 __global__ void __launch_bounds__(64) adj_band_kernel(AdjBulkArgs a, const int2* __restrict__ pts,
                                                       int npts) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= npts) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = t >> 1;
+  if (k >= npts) return;  // both lanes of a pair leave together
+  const bool q = (t & 1) == 0;
   const int2 p = pts[k];
-  adj_point(a, p.x, p.y);
+  const int i = p.x, j = p.y;
+  const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
+  const BlockDev& B = a.blocks[bi];
+  const int nq = B.px - B.mx + 1, nr = B.py - B.my + 1;
+  const int il = i - B.mx, jl = j - B.my;
+  const long long ps = a.ps;
+  const int ld = a.ld;
+  const long long cc = (long long)j * ld + i;
+  const double gam = B.gamma;
+  const double cl[5] = {1.0 / B.rho, 1.0 / B.rho, B.M, B.G, B.M};
+  double g[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  adj_dir_any(a, g, q ? il : jl, q ? nq : nr, q ? (long long)j * ld + B.mx : (long long)B.my * ld + i,
+              q ? 1 : ld, cc, cl, gam, q);
+#pragma unroll
+  for (int l = 0; l < 5; l++) g[l] = g[l] + __shfl_xor(g[l], 1);
+  if (!q) return;
+  const int NS = a.NS;
+  int so[4], nso = 0;
+  if (il == 0) so[nso++] = B.side_off[FDM_SIDE_L] + jl;
+  if (il == nq - 1) so[nso++] = B.side_off[FDM_SIDE_R] + jl;
+  if (jl == 0) so[nso++] = B.side_off[FDM_SIDE_B] + il;
+  if (jl == nr - 1) so[nso++] = B.side_off[FDM_SIDE_T] + il;
+  for (int m = 0; m < nso; m++)
+#pragma unroll
+    for (int l = 0; l < 5; l++) g[l] += a.gE[l * NS + so[m]];
+#pragma unroll
+  for (int l = 0; l < 5; l++) {
+    const double lo = a.lin[l * ps + cc] + g[l];
+    a.lout[l * ps + cc] = lo;
+    a.mout[l * ps + cc] = a.A * a.min[l * ps + cc] + a.Bdt_prev * lo;
+  }
 }
 
 // Interior adjoint kernel over TX x TY tiles of each block's rectangle
@@ -839,8 +905,8 @@ int adj_stage(fdm_adjoint* a, int n, int s, int& cur) {
       hipLaunchKernelGGL(adj_edge_kernel, dim3((c->ntasks + 63) / 64), dim3(64), 0, c->side, e);
     ADJCHK(a, hipStreamWaitEvent(c->side, c->evI, 0));
     if (a->nband)
-      hipLaunchKernelGGL(adj_band_kernel, dim3((a->nband + 63) / 64), dim3(64), 0, c->side, b, a->band,
-                         a->nband);
+      hipLaunchKernelGGL(adj_band_kernel, dim3((2 * a->nband + 63) / 64), dim3(64), 0, c->side, b,
+                         a->band, a->nband);
     ADJCHK(a, hipEventRecord(c->evB, c->side));
     ADJCHK(a, hipEventRecord(c->evI, c->stream));
   } else {
