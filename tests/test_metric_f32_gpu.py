@@ -91,6 +91,18 @@ def test_flat_locked_blocks_bit_exact_against_oracle(gpu):
     assert np.array_equal(DFd, DFo)
 
 
+@pytest.mark.parametrize("medium,Cdiss", [(0.3, None), (None, [300.0, 0.0]), (0.3, 300.0)])
+def test_flat_medium_and_dissipation_bit_exact_against_oracle(gpu, medium, Cdiss):
+    """the medium and dissipation variants of the interior tiles read the
+    metrics the same way"""
+    prob, F = problems.build(nx=90, nys=(45, 52), seed=33, warp=0.0, background=BG,
+                             perturb=1e-2, coupling="locked", medium=medium, Cdiss=Cdiss)
+    Fd, DFd, _ = both_ways(prob, F, 20)
+    Fo, DFo, _ = run(OracleContext(prob, threads=4), F, 20)
+    assert np.array_equal(Fd, Fo)
+    assert np.array_equal(DFd, DFo)
+
+
 def test_curved_grid_has_no_fp32_form(gpu):
     prob, F = problems.build(nx=64, nys=(30, 30), seed=3, warp=0.04, background=BG, perturb=1e-3,
                              law="RFL")
