@@ -140,11 +140,12 @@ def cpu_baseline(inpath, sample_steps):
                        f"({el:.1f} s wall on {threads} threads)")
 
 
-def measured_traffic():
-    """HBM bytes per bulk launch from the committed counter summary (two
-    rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, each scaled by what the
-    counter reads for a plain copy of known size)."""
-    path = os.path.join(ROOT, "profiles", "bulk_interior_c2_fp64_traffic.md")
+def measured_traffic(name="bulk_interior_c2_fp64"):
+    """HBM bytes per launch of the dominant kernel from the committed counter
+    summary profiles/<name>_traffic.md (two rocprofv3 --pmc runs, FETCH_SIZE
+    and WRITE_SIZE, each scaled by what the counter reads for a plain copy of
+    known size)."""
+    path = os.path.join(ROOT, "profiles", name + "_traffic.md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -259,10 +260,12 @@ def run_c2_variant(args, tmp, het, cdiss):
                    "dt": r64["r"]["hp"].dt},
         "roofline": {
             "bound": "hbm",
-            "kernel": f"bulk_interior<double,64,4,false,NT,{str(bool(het)).lower()},"
-                      f"{str(bool(cdiss)).lower()}>",
+            "kernel": f"bulk_interior<double,64,4,false,true,{str(bool(het)).lower()},"
+                      f"{str(bool(cdiss)).lower()},4,{str(r64['r']['ji']).lower()},"
+                      f"{str(r64['r']['mc']).lower()}>",
             "kernel_points": r64["kpts"], "achieved": r64["achieved"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS,
+            "traffic": measured_traffic(f"bulk_interior_{name}_fp64"),
             "alg_bytes_per_launch": r64["bytes_launch"], "avg_launch_ms": r64["avg_ms"],
         },
     }
@@ -326,7 +329,8 @@ def run_c2_pml(args, tmp):
         "roofline": {
             "bound": "hbm", "kernel": "bulk_pml<double,false,true> (layer points four or more from the block edges)",
             "kernel_points": r64["kpts"], "achieved": r64["achieved"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS,
+            "traffic": measured_traffic("bulk_pml_c2pml_fp64"),
             "alg_bytes_per_launch": r64["bytes_launch"], "avg_launch_ms": r64["avg_ms"],
         },
     }
@@ -382,7 +386,8 @@ def run_m3(args, tmp):
         "roofline": {
             "bound": "hbm", "kernel": "bulk_interior3<double,64,4,NT>",
             "kernel_points": r64["kpts"], "achieved": r64["achieved"], "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS,
+            "traffic": measured_traffic("bulk_interior3_m3_fp64"),
             "alg_bytes_per_launch": r64["bytes_launch"], "avg_launch_ms": r64["avg_ms"],
         },
     }
