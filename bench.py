@@ -284,6 +284,65 @@ def run_c2_variant(args, tmp, het, cdiss):
     print(json.dumps(out), flush=True)
 
 
+def run_c2_sbp6(args, tmp):
+    """C2 with the reference's default operator, SBP6 (domain.f90:94): the
+    same grid, fault and byte count per point (the interior stencil reads a
+    halo of 3 instead of 2 from LDS; the closures reach 6 points in
+    instead of 4, so the interior tiles start 6 points from the block
+    edges and the band is wider)."""
+    from fdmap_github_amd import inputs
+
+    nx, ny = args.nx, args.ny
+    inpath = inputs.write(os.path.join(tmp, "c2sbp6.in"),
+                          inputs.c2_planar_rs(name=os.path.join(tmp, "c2sbp6"), nx=nx, ny=ny,
+                                              nsteps=args.steps + args.warmup + args.prof_steps,
+                                              outputs=False, fdmethod="SBP6"))
+    res = {}
+    for prec in ("fp64",) if args.no_fp32 else ("fp64", "fp32"):
+        r = run_device(inpath, prec, args.steps, args.warmup, args.prof_steps)
+        ns, prof = r["nstage"], r["prof"]
+        b1, b2 = BYTES_STAGE[prec]
+        kpts = prof["bulk_points"]
+        bytes_launch = kpts * (b1 + (ns - 1) * b2) / ns
+        avg_ms = prof["bulk_ms"] / max(1, prof["bulk_launches"])
+        res[prec] = dict(r=r, ups=r["npts"] * args.steps / r["el"], avg_ms=avg_ms, kpts=kpts,
+                         achieved=bytes_launch / (avg_ms * 1e-3) / 1e9, bytes_launch=bytes_launch)
+    r64 = res["fp64"]
+    out = {
+        "metric": METRIC, "value": r64["ups"], "unit": "updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * r64["r"]["el"] / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic (planar fault problem)",
+        "config": {"workload": f"planar rate-and-state rupture on the SBP6 operator, 2 blocks x "
+                               f"({nx}x{ny}), RFL slip law, LSRK(5,4) CFL 0.5",
+                   "grid_points": r64["r"]["npts"], "fault_nodes": nx, "dt": r64["r"]["hp"].dt},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"bulk_interior<double,64,4,false,true,false,false,6,{str(r64['r']['ji']).lower()},"
+                      f"{str(r64['r']['mc']).lower()}>",
+            "kernel_points": r64["kpts"], "achieved": r64["achieved"], "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": r64["achieved"] / HBM_PEAK_GBS,
+            "traffic": measured_traffic("bulk_interior_c2sbp6_fp64"),
+            "alg_bytes_per_launch": r64["bytes_launch"], "avg_launch_ms": r64["avg_ms"],
+            "edge_kernel_avg_ms": prof["edge_ms"] / max(1, prof["edge_launches"]),
+            "band_kernel_avg_ms": prof["band_ms"] / max(1, prof["band_launches"]),
+        },
+    }
+    if "fp32" in res:
+        r32 = res["fp32"]
+        out["fp32_bulk"] = {
+            "updates_per_s": r32["ups"], "speedup_vs_fp64": r32["ups"] / r64["ups"],
+            "ms_per_step": 1e3 * r32["r"]["el"] / args.steps,
+            "bulk_avg_launch_ms": r32["avg_ms"], "bulk_achieved_GBs": r32["achieved"],
+            "rel_l2_fields_vs_fp64": rel_l2(r32["r"]["F"], r64["r"]["F"]),
+            "rel_l2_slip_vs_fp64": rel_l2(r32["r"]["fault"]["D"], r64["r"]["fault"]["D"]),
+            "rel_l2_slip_velocity_vs_fp64": rel_l2(r32["r"]["fault"]["V"],
+                                                   r64["r"]["fault"]["V"]),
+            "after_steps": args.steps + args.warmup,
+        }
+    print(json.dumps(out), flush=True)
+
+
 def run_c2_pml(args, tmp):
     """C2 with a graded perfectly matched layer in both blocks (pmlx and
     pmly, n = 2, from 80 % of the half-widths outwards): every point is a
@@ -632,11 +691,14 @@ def main():
     ap.add_argument("--horizon", type=int, default=4000,
                     help="steps after which the FP32-bulk error against FP64 is stated (C2's "
                          "own length), whatever --steps")
-    ap.add_argument("--config", choices=("c2", "c2het", "c2diss", "c2pml", "m3", "c3", "c4", "c5"),
+    ap.add_argument("--config", choices=("c2", "c2het", "c2diss", "c2pml", "c2sbp6", "m3", "c3", "c4",
+                                         "c5"),
                     default="c2",
                     help="c2: planar rupture (the headline); c2het: the same in a "
                          "heterogeneous medium; c2diss: the same with artificial dissipation "
-                         "(Cdiss 0.1); c2pml: the same inside a graded perfectly matched layer; m3: antiplane shear rupture, 2 x (801x1601); "
+                         "(Cdiss 0.1); c2pml: the same inside a graded perfectly matched layer; "
+                         "c2sbp6: the same on the SBP6 operator; m3: antiplane shear rupture, "
+                         "2 x (801x1601); "
                          "c3: rough curvilinear fault with "
                          "off-fault plasticity, 2 x (4001x2001), FP64 and FP32-bulk; c4: 64 "
                          "uneven rupture scenarios advanced as one graph launch per step; c5: "
@@ -661,6 +723,9 @@ def main():
         return
     if args.config == "c2pml":
         run_c2_pml(args, tmp)
+        return
+    if args.config == "c2sbp6":
+        run_c2_sbp6(args, tmp)
         return
     if args.config == "c3":
         run_c3(args, tmp)

@@ -636,12 +636,12 @@ void launch_interior_v(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 gri
 
 // the JI variants (Ji formed in the kernel) exist for FP64 only: fdm_create
 // sets ji_calc in FP64 alone; so do the MC variants (FP32-delta metrics),
-// for the SBP4 64 x 4 tiles
+// for the 64 x 4 tiles of every operator
 template <typename T, int TX, int TY, bool HET, bool DISS, int OP>
 void launch_interior_j(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid, dim3 block,
                        bool full) {
   if constexpr (std::is_same<T, double>::value) {
-    if constexpr (TX == 64 && TY == 4 && OP == 4) {
+    if constexpr (TX == 64 && TY == 4) {
       if (c->mcomp) {
         if (c->ji_calc) launch_interior_v<T, TX, TY, HET, DISS, OP, true, true>(c, s, a, grid, block, full);
         else launch_interior_v<T, TX, TY, HET, DISS, OP, false, true>(c, s, a, grid, block, full);
@@ -1649,7 +1649,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     if (const char* v = getenv("FDM_JI_CALC")) exact = exact && atoi(v) != 0;
     c->ji_calc = exact;
   }
-  if (c->precision == FDM_FP64 && c->mode == 2 && c->sbp == 4 && c->bulk_mode == 2) {
+  if (c->precision == FDM_FP64 && c->mode == 2 && c->bulk_mode == 2) {
     // FP32-delta metrics: xq, xr, yq, yr of a point stored as d = m - base
     // in FP32, base = the metric at the first interior point of its block,
     // kept where base + (double)d reproduces m to the bit and marked NaN

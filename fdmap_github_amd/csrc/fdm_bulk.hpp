@@ -1235,7 +1235,10 @@ __global__ void __launch_bounds__(64) bulk_band(BulkArgs<T> a, const int2* __res
 // DF stored non-temporally (working set beyond the last-level cache).
 template <typename T, bool FULL, bool INNER, bool NTS>
 __global__ void __launch_bounds__(256) bulk_pml(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  // the list runs row by row: with the XCD swizzle each XCD takes a
+  // contiguous run of it, so the rows a stencil shares are fetched into one
+  // L2 rather than into up to five
+  const int k = linear_tile(a.xcd_swizzle) * blockDim.x + threadIdx.x;
   if (k >= npts) return;
   const int2 p = pts[k];
   if (p.x < 0) return;  // padding of an aligned run

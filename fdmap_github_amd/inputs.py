@@ -75,7 +75,7 @@ vx vy
 def c2_planar_rs(name="c2", nx=1601, ny=801, h=0.025, nsteps=4000, outputs=True, ninfo=500,
                  law="RFL", plastic=False, curve=None, tau0=70.0, sn0=120.0, overstress=25.0,
                  R=1.5, Vinit=1e-12, a=0.010, b=0.014, sxy0=None, bcT="absorbing",
-                 medium=None, Cdiss=0.0, pml=None):
+                 medium=None, Cdiss=0.0, pml=None, fdmethod="SBP4"):
     """Two blocks nx x ny coupled across a horizontal fault by SAT; slip law
     with strong velocity weakening; Gaussian overstress at the fault centre.
     curve: optional path of a fault-profile curve file (FP64 x, y, n1, n2),
@@ -86,7 +86,8 @@ def c2_planar_rs(name="c2", nx=1601, ny=801, h=0.025, nsteps=4000, outputs=True,
     (write_medium) that both blocks read.  Cdiss: artificial dissipation of
     both blocks (operator_list, domain.f90:167-180).  pml: a perfectly
     matched layer in both blocks, a dict of pml_list entries (pmlx, pmly, n,
-    sigma, kappa, mx, px, my, py; material.f90:152-223)."""
+    sigma, kappa, mx, px, my, py; material.f90:152-223).  fdmethod: the
+    first-derivative operator (None: the reference's default, SBP6)."""
     Lx = 0.5 * (nx - 1) * h
     Ly = (ny - 1) * h
     psi0 = rs_state(a, 1e-6, tau0, sn0, Vinit)
@@ -113,7 +114,7 @@ D V S N Psi
  name = '{name}', nt = {nsteps + 1}, CFL = 0.5d0, RKorder = 4, ninfo = {ninfo} /
 
 &domain_list
- mode = 2, FDmethod = 'SBP4', nblocks_x = 1, nblocks_y = 2, nx = {nx}, ny = {2 * ny} /
+ mode = 2,{_fdm(fdmethod)} nblocks_x = 1, nblocks_y = 2, nx = {nx}, ny = {2 * ny} /
 
 !---BLOCK1---
 

@@ -24,7 +24,7 @@ extern "C" {
                                planes exceed 256 MB) */
 #define FDM_TUNE_METRIC_F32 3 /* interior tiles read the metrics as FP32 deltas
                                  from a per-block base (FP64 plane-strain
-                                 contexts, SBP4 64x4 tiles): 0 or 1.  fdm_create turns it
+                                 contexts, 64x4 tiles): 0 or 1.  fdm_create turns it
                                  on where the encoding is exact for all but
                                  1/64 of the metric values (those are read
                                  from the FP64 planes); value 1 fails where

@@ -1,6 +1,6 @@
 """FP32-delta metrics of the interior tiles (FDM_TUNE_METRIC_F32).
 
-In FP64 contexts whose blocks are nearly affine, fdm_create stores xq, xr,
+In FP64 contexts whose blocks are nearly affine (any of SBP2/4/6), fdm_create stores xq, xr,
 yq, yr a second time as FP32 deltas from a per-block base, where base +
 delta reproduces the FP64 value to the bit, and marks the other values NaN
 so that the kernel reads them from the FP64 planes.  The staged metrics are
@@ -101,6 +101,29 @@ def test_flat_medium_and_dissipation_bit_exact_against_oracle(gpu, medium, Cdiss
     Fo, DFo, _ = run(OracleContext(prob, threads=4), F, 20)
     assert np.array_equal(Fd, Fo)
     assert np.array_equal(DFd, DFo)
+
+
+@pytest.mark.parametrize("sbp", [2, 6])
+def test_flat_sbp2_sbp6_bit_exact_against_oracle(gpu, sbp):
+    """the SBP2 and SBP6 interior tiles (halo 1 and 3, the interior
+    rectangle 1 and 6 points in) read the metrics the same way"""
+    prob, F = problems.build(nx=90, nys=(45, 52), seed=37, warp=0.0, background=BG,
+                             perturb=1e-2, coupling="locked", sbp=sbp)
+    Fd, DFd, _ = both_ways(prob, F, 20)
+    Fo, DFo, _ = run(OracleContext(prob, threads=4), F, 20)
+    assert np.array_equal(Fd, Fo)
+    assert np.array_equal(DFd, DFo)
+
+
+@pytest.mark.parametrize("sbp", [2, 6])
+def test_flat_rupture_sbp2_sbp6(gpu, sbp):
+    prob, F = problems.build(nx=203, nys=(61, 77), seed=41, warp=0.0, background=BG, perturb=1e-3,
+                             law="RFL", sbp=sbp)
+    Fd, DFd, fd = both_ways(prob, F, 40)
+    Fo, DFo, fo = run(OracleContext(prob, threads=4), F, 40)
+    assert rel_l2(Fd, Fo) <= 1e-12
+    for f in FAULT:
+        assert rel_l2(fd[0][f], fo[0][f]) <= 1e-12, f
 
 
 def test_curved_grid_has_no_fp32_form(gpu):
