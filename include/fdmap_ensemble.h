@@ -17,14 +17,16 @@ extern "C" {
 #endif
 
 /* launch knobs of one context */
-#define FDM_TUNE_TILE 0    /* interior tile shape: 0 = 64x4, 1 = 64x8, 2 = 128x4 */
+#define FDM_TUNE_TILE 0    /* interior tile shape: 0 = 64x4, 1 = 64x8, 2 = 128x4
+                              (SBP4 in a uniform medium without dissipation;
+                              SBP6 there: 0 or 1, default 1; else 0 only) */
 #define FDM_TUNE_SWIZZLE 1 /* XCD-contiguous tile order: 0 or 1 */
 #define FDM_TUNE_NT_STORE 2 /* non-temporal stores of the interior kernel: 0 or 1
                                (default: 1 when the 23 field, rate and metric
                                planes exceed 256 MB) */
 #define FDM_TUNE_METRIC_F32 3 /* interior tiles read the metrics as FP32 deltas
                                  from a per-block base (FP64 plane-strain
-                                 contexts, 64x4 tiles): 0 or 1.  fdm_create turns it
+                                 contexts, 64x4 and 64x8 tiles): 0 or 1.  fdm_create turns it
                                  on where the encoding is exact for all but
                                  1/64 of the metric values (those are read
                                  from the FP64 planes); value 1 fails where

@@ -98,6 +98,17 @@ def test_locked_2x2_bit_exact(gpu, monkeypatch, sbp):
     check(monkeypatch, prob, F, 20, bit_exact=True)
 
 
+@pytest.mark.parametrize("tile", ["64x4", "64x8"])
+def test_sbp6_tile_shapes_bit_exact(gpu, monkeypatch, tile):
+    """SBP6 in a uniform medium runs 64 x 8 tiles by default; both shapes
+    match the oracle and the per-point kernel bit for bit (ragged tiles in
+    both directions, two block rows and columns)"""
+    monkeypatch.setenv("FDM_TILE", tile)
+    prob, F = problems.build(nx=97, nys=(37, 43), nbx=2, seed=43, warp=0.05, background=BG,
+                             perturb=1e-2, coupling="locked", sbp=6)
+    check(monkeypatch, prob, F, 20, bit_exact=True)
+
+
 @pytest.mark.parametrize("sbp", [2, 6])
 def test_heterogeneous_medium_bit_exact(gpu, monkeypatch, sbp):
     """the point-wise material coefficients DI(s)/rho(i,j) etc.
