@@ -278,41 +278,38 @@ __global__ void __launch_bounds__(TX * TY) adj_interior_kernel(AdjBulkArgs a,
   const int tid = threadIdx.y * TX + threadIdx.x;
   const int ld = a.ld, nx1 = a.nx - 1, ny1 = a.ny - 1;
   const long long ps = a.ps;
-  // every load of the tile is issued before the LDS stores and the barrier
+  // every load of the tile is issued before the LDS stores and the barrier;
+  // every lane loads in every pass (past the end: the last element again),
+  // so that no load sits under a lane condition (whose result the compiler
+  // would copy out of the branch after waiting for all earlier loads)
   double vW[NF][6], vR[NR][2], vQ[NQ][2];
 #pragma unroll
   for (int p = 0; p < NF; p++) {
-    const int k = tid + p * NT;
-    if (k < HY * HX) {
-      const int r = k / HX, c = k - r * HX;
-      const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(max(i0 - 2 + c, 0), nx1);
-      const unsigned o = (unsigned)(gj * ld + gi) * 8u;
-      vW[p][5] = ld_off(a.Ji, o);
+    const int k = min(tid + p * NT, HY * HX - 1);
+    const int r = k / HX, c = k - r * HX;
+    const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(max(i0 - 2 + c, 0), nx1);
+    const unsigned o = (unsigned)(gj * ld + gi) * 8u;
+    vW[p][5] = ld_off(a.Ji, o);
 #pragma unroll
-      for (int l = 0; l < 5; l++) vW[p][l] = ld_off(a.min + l * ps, o);
-    }
+    for (int l = 0; l < 5; l++) vW[p][l] = ld_off(a.min + l * ps, o);
   }
 #pragma unroll
   for (int p = 0; p < NR; p++) {
-    const int k = tid + p * NT;
-    if (k < TY * HX) {
-      const int r = k / HX, c = k - r * HX;
-      const int gj = min(j0 + r, ny1), gi = min(max(i0 - 2 + c, 0), nx1);
-      const unsigned o = (unsigned)(gj * ld + gi) * 8u;
-      vR[p][0] = ld_off(a.xr, o);
-      vR[p][1] = ld_off(a.yr, o);
-    }
+    const int k = min(tid + p * NT, TY * HX - 1);
+    const int r = k / HX, c = k - r * HX;
+    const int gj = min(j0 + r, ny1), gi = min(max(i0 - 2 + c, 0), nx1);
+    const unsigned o = (unsigned)(gj * ld + gi) * 8u;
+    vR[p][0] = ld_off(a.xr, o);
+    vR[p][1] = ld_off(a.yr, o);
   }
 #pragma unroll
   for (int p = 0; p < NQ; p++) {
-    const int k = tid + p * NT;
-    if (k < HY * TX) {
-      const int r = k / TX, c = k - r * TX;
-      const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(i0 + c, nx1);
-      const unsigned o = (unsigned)(gj * ld + gi) * 8u;
-      vQ[p][0] = ld_off(a.xq, o);
-      vQ[p][1] = ld_off(a.yq, o);
-    }
+    const int k = min(tid + p * NT, HY * TX - 1);
+    const int r = k / TX, c = k - r * TX;
+    const int gj = min(max(j0 - 2 + r, 0), ny1), gi = min(i0 + c, nx1);
+    const unsigned o = (unsigned)(gj * ld + gi) * 8u;
+    vQ[p][0] = ld_off(a.xq, o);
+    vQ[p][1] = ld_off(a.yq, o);
   }
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
   const bool valid = i <= B4->px - 6 && j <= B4->py - 6;

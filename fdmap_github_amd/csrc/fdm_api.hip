@@ -274,6 +274,16 @@ struct fdm_ctx {
   bool graph_mode = false;
   double g_tinit = 0.0, g_dt = 0.0, g_rkc = 0.0;
   long long ninterior = 0;
+  // band split (plane strain, SBP4, uniform medium, no dissipation; FP64 by
+  // default): the band points on no block edge (iband) join the interior
+  // launch; the second stream's band kernel takes the edge points (eband)
+  bool band_split = false;
+  // lean plane-strain band points in a uniform medium: one lane per (point,
+  // field) (bulk_band_fl; FP64 by default, FDM_BAND_FL=0|1 overrides)
+  bool band_fl = true;
+  int2* iband = nullptr;
+  int2* eband = nullptr;
+  int niband = 0, neband = 0;
   // adjoint forward passes: fault trajectories ([nstage][3][NI] per step,
   // step n at trace + (n - trace_n0) nstage 3 NI), the slot of the stage
   // being launched, and vx, vy at receivers after every step (step n at
@@ -541,6 +551,9 @@ BulkArgs<T> bulk_args(fdm_ctx* c, double A, double Bdt, int do_update, double t,
   a.fxr = c->mfp[1];
   a.fyq = c->mfp[2];
   a.fyr = c->mfp[3];
+  a.ibpts = c->iband;
+  a.nib = c->niband;
+  a.nibwg = 0;  // set by the interior launch that carries them
   return a;
 }
 
@@ -553,23 +566,30 @@ bool full_variant(const fdm_ctx* c, const BulkArgs<T>& a) {
 
 template <typename T, int OP>
 void launch_band_op(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid) {
+  // with the band split, the edge points only
+  const int2* pts = c->band_split ? c->eband : c->band;
+  const int n = c->band_split ? c->neband : c->nband;
   if (c->mode == 3)
-    hipLaunchKernelGGL((bulk_band3<T, OP>), grid, dim3(64), 0, s, a, c->band, c->nband);
+    hipLaunchKernelGGL((bulk_band3<T, OP>), grid, dim3(64), 0, s, a, pts, n);
   else if (full_variant(c, a) && (c->het || c->diss))
-    hipLaunchKernelGGL((bulk_band<T, true, true, OP>), grid, dim3(64), 0, s, a, c->band, c->nband);
+    hipLaunchKernelGGL((bulk_band<T, true, true, OP>), grid, dim3(64), 0, s, a, pts, n);
   else if (full_variant(c, a))
-    hipLaunchKernelGGL((bulk_band<T, true, false, OP>), grid, dim3(64), 0, s, a, c->band, c->nband);
+    hipLaunchKernelGGL((bulk_band<T, true, false, OP>), grid, dim3(64), 0, s, a, pts, n);
   else if (c->het || c->diss)
-    hipLaunchKernelGGL((bulk_band<T, false, true, OP>), grid, dim3(64), 0, s, a, c->band, c->nband);
+    hipLaunchKernelGGL((bulk_band<T, false, true, OP>), grid, dim3(64), 0, s, a, pts, n);
+  else if (c->band_fl)
+    // one lane per (point, field), twelve points per wave
+    hipLaunchKernelGGL((bulk_band_fl<T, OP>), dim3((n + 11) / 12), dim3(64), 0, s, a, pts, n);
   else
-    hipLaunchKernelGGL((bulk_band<T, false, false, OP>), grid, dim3(64), 0, s, a, c->band, c->nband);
+    hipLaunchKernelGGL((bulk_band<T, false, false, OP>), grid, dim3(64), 0, s, a, pts, n);
 }
 
 template <typename T>
 void launch_band(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
-  if (!c->nband) return;
+  const int n = c->band_split ? c->neband : c->nband;
+  if (!n) return;
   EventPair* ep = prof_begin(c, 2, s);
-  dim3 grid((c->nband + 63) / 64);
+  dim3 grid((n + 63) / 64);
   if (c->sbp == 2) launch_band_op<T, 2>(c, s, a, grid);
   else if (c->sbp == 6) launch_band_op<T, 6>(c, s, a, grid);
   else launch_band_op<T, 4>(c, s, a, grid);
@@ -708,9 +728,17 @@ void launch_interior(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
   if (c->het && c->diss) launch_interior_j<T, 64, 4, true, true, 4>(c, s, a, grid, block, full);
   else if (c->het) launch_interior_j<T, 64, 4, true, false, 4>(c, s, a, grid, block, full);
   else if (c->diss) launch_interior_j<T, 64, 4, false, true, 4>(c, s, a, grid, block, full);
-  else if (q == 0) launch_interior_j<T, 64, 4, false, false, 4>(c, s, a, grid, block, full);
-  else if (q == 1) launch_interior_j<T, 64, 8, false, false, 4>(c, s, a, grid, block, full);
-  else launch_interior_j<T, 128, 4, false, false, 4>(c, s, a, grid, block, full);
+  else {
+    // the inner band points ride ahead of the tiles (band split), one lane
+    // per point
+    BulkArgs<T> b = a;
+    const int nt = kTileShape[q][0] * kTileShape[q][1];
+    b.nibwg = c->band_split ? (c->niband + nt - 1) / nt : 0;
+    grid.x += b.nibwg;
+    if (q == 0) launch_interior_j<T, 64, 4, false, false, 4>(c, s, b, grid, block, full);
+    else if (q == 1) launch_interior_j<T, 64, 8, false, false, 4>(c, s, b, grid, block, full);
+    else launch_interior_j<T, 128, 4, false, false, 4>(c, s, b, grid, block, full);
+  }
   prof_end(c, ep, 0, s);
 }
 
@@ -1117,8 +1145,11 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     return set_err(nullptr, FDM_EDEVICE, "fdm_create: hipStreamCreate failed");
   }
   if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evI, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evB, hipEventDisableTiming) != hipSuccess ||
+      // the stage marks order work on this device only: no system-scope
+      // fence (an event recorded with one costs about 3 us of GPU time per
+      // stage on the stream that records it)
+      hipEventCreateWithFlags(&c->evI, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evB, hipEventDisableTiming | hipEventDisableSystemFence) != hipSuccess ||
       hipEventCreateWithFlags(&c->evJ, hipEventDisableTiming) != hipSuccess) {
     fdm_destroy(c);
     return set_err(nullptr, FDM_EDEVICE, "fdm_create: stream/event creation failed");
@@ -1738,6 +1769,33 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     int z = 0;
     if ((rc = dupload(c, &c->fail, &z, sizeof(int)))) return fail(rc);
   }
+  // Both band forms pay off in FP64, where the interior launch is long
+  // enough to carry the inner band points and the field lanes shorten the
+  // second stream; in FP32-bulk (an interior launch half as long) the one
+  // band kernel of one lane per point keeps the step shortest.
+  c->band_fl = c->precision == FDM_FP64;
+  if (const char* e = getenv("FDM_BAND_FL")) c->band_fl = atoi(e) != 0;
+  if (c->bulk_mode == 2 && c->mode == 2 && c->sbp == 4 && !c->het && !c->diss && c->nband) {
+    // band split: the band points on no block edge need no edge values of
+    // the stage and join the interior launch (bulk_interior); the second
+    // stream's band kernel takes the edge points (FDM_BAND_SPLIT=0|1
+    // overrides the default)
+    bool use = c->precision == FDM_FP64;
+    if (const char* e = getenv("FDM_BAND_SPLIT")) use = atoi(e) != 0;
+    if (use) {
+      std::vector<int2> ib, eb;
+      for (const int2& p : c->hband) {
+        const fdm_block& B = P->blocks[block_of(c, p.x, p.y)];
+        const bool edge = p.x == B.mx || p.x == B.px || p.y == B.my || p.y == B.py;
+        (edge ? eb : ib).push_back(p);
+      }
+      c->niband = (int)ib.size();
+      c->neband = (int)eb.size();
+      if (c->niband && (rc = dupload(c, &c->iband, ib.data(), ib.size() * sizeof(int2)))) return fail(rc);
+      if (c->neband && (rc = dupload(c, &c->eband, eb.data(), eb.size() * sizeof(int2)))) return fail(rc);
+      c->band_split = true;
+    }
+  }
   {
     // frictional nodes and the reduction buffer of fdm_max_slip_velocity
     std::vector<int> fn;
@@ -2085,7 +2143,10 @@ int fdm_profile_read(fdm_ctx* c, fdm_profile_t* out) {
   out->edge_launches = c->kind_n[1];
   out->band_ms = c->kind_ms[2];
   out->band_launches = c->kind_n[2];
-  out->bulk_points = c->bulk_mode == 2 ? (int64_t)c->ninterior + c->nlayer_inner : (int64_t)c->nx * c->ny;
+  // the interior launch's points: tiles, inner layer points and, with the
+  // band split, the inner band points
+  out->bulk_points = c->bulk_mode == 2 ? (int64_t)c->ninterior + c->nlayer_inner + c->niband
+                                       : (int64_t)c->nx * c->ny;
   return FDM_OK;
 }
 

@@ -67,6 +67,11 @@ struct BulkArgs {
   const float* fxr;
   const float* fyq;
   const float* fyr;
+  // inner band points: the band points on no block edge (they need no edge
+  // values of the stage), updated by the first nibwg workgroups of the
+  // interior launch, one lane per point (nib points; none: nibwg = 0)
+  const int2* ibpts;
+  int nib, nibwg;
 };
 
 // Map the hardware workgroup id to a tile.  Workgroups are dispatched to the
@@ -90,10 +95,11 @@ __device__ __forceinline__ void tile_of(int swz, int& tx, int& ty) {
 // Linear tile index of this workgroup in a 1-D grid: with the swizzle, the
 // workgroups dispatched to XCD x (ids congruent to x mod 8) take a
 // contiguous range of the tile list.
-__device__ __forceinline__ int linear_tile(int swz) {
-  const int bid = blockIdx.x;
+__device__ __forceinline__ int linear_tile(int swz, int off = 0) {
+  // off: workgroups ahead of the tiles (the interior launch's inner band)
+  const int bid = blockIdx.x - off;
   if (!swz) return bid;
-  const int nwg = gridDim.x;
+  const int nwg = gridDim.x - off;
   const int q = nwg >> 3, r = nwg & 7, x = bid & 7, k = bid >> 3;
   return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
 }
@@ -946,9 +952,15 @@ __device__ __forceinline__ T ld_sel(const T* base, unsigned boff) {
 }
 
 // a metric value from its FP32 delta: base + d, exact by construction; a
-// NaN delta (a value the encoding cannot hold) reads the FP64 plane
+// NaN delta (a value the encoding cannot hold) reads the FP64 plane.  That
+// rare load completes inside its branch (s_waitcnt vmcnt(0)): left in
+// flight, it would make the compiler wait after the branch, in every wave,
+// for all of the tile's loads (the lane's DF and F6 among them).
 __device__ __forceinline__ double metric_of(float d, double base, const double* plane, unsigned boff) {
-  return (d == d) ? base + (double)d : ld_off(plane, boff);
+  if (d == d) return base + (double)d;
+  const double v = ld_off(plane, boff);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt and lgkmcnt unchanged
+  return v;
 }
 
 template <bool NT, typename T>
@@ -1019,50 +1031,49 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
   const T* __restrict__ F = a.Fin;
   T vF[NF][5], vR[MC ? 1 : NR][2], vQ[MC ? 1 : NQ][2], vJ[DISS ? NF : 1];
   float fR[MC ? NR : 1][2], fQ[MC ? NQ : 1][2];
+  // Every lane loads in every pass: a lane past the end of the staged
+  // block (in the last pass) reloads its last element, and only the LDS
+  // stores below are conditional.  A load under a lane condition makes the
+  // compiler copy its result out of the branch, which waits for every load
+  // issued before it.
 #pragma unroll
   for (int p = 0; p < NF; p++) {
-    const int k = tid + p * NT;
-    if (k < HY * HX) {
-      const int r = k / HX, c = k - r * HX;
-      const int gj = min(max(j0 - H + r, 0), ny1), gi = min(max(i0 - H + c, 0), nx1);
-      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+    const int k = min(tid + p * NT, HY * HX - 1);
+    const int r = k / HX, c = k - r * HX;
+    const int gj = min(max(j0 - H + r, 0), ny1), gi = min(max(i0 - H + c, 0), nx1);
+    const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
 #pragma unroll
-      for (int l = 0; l < 5; l++) vF[p][l] = ld_off(F + l * ps, o);
-      if constexpr (DISS) vJ[p] = ld_off(a.J, o);
-    }
+    for (int l = 0; l < 5; l++) vF[p][l] = ld_off(F + l * ps, o);
+    if constexpr (DISS) vJ[p] = ld_off(a.J, o);
   }
 #pragma unroll
   for (int p = 0; p < NR; p++) {
-    const int k = tid + p * NT;
-    if (k < TY * HX) {
-      const int r = k / HX, c = k - r * HX;
-      const int gj = min(j0 + r, ny1), gi = min(max(i0 - H + c, 0), nx1);
-      if constexpr (MC) {
-        const unsigned o4 = (unsigned)(gj * ld + gi) * 4u;
-        fR[p][0] = ld_off(a.fxr, o4);
-        fR[p][1] = ld_off(a.fyr, o4);
-      } else {
-        const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
-        vR[p][0] = ld_off(a.xr, o);
-        vR[p][1] = ld_off(a.yr, o);
-      }
+    const int k = min(tid + p * NT, TY * HX - 1);
+    const int r = k / HX, c = k - r * HX;
+    const int gj = min(j0 + r, ny1), gi = min(max(i0 - H + c, 0), nx1);
+    if constexpr (MC) {
+      const unsigned o4 = (unsigned)(gj * ld + gi) * 4u;
+      fR[p][0] = ld_off(a.fxr, o4);
+      fR[p][1] = ld_off(a.fyr, o4);
+    } else {
+      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+      vR[p][0] = ld_off(a.xr, o);
+      vR[p][1] = ld_off(a.yr, o);
     }
   }
 #pragma unroll
   for (int p = 0; p < NQ; p++) {
-    const int k = tid + p * NT;
-    if (k < HY * TX) {
-      const int r = k / TX, c = k - r * TX;
-      const int gj = min(max(j0 - H + r, 0), ny1), gi = min(i0 + c, nx1);
-      if constexpr (MC) {
-        const unsigned o4 = (unsigned)(gj * ld + gi) * 4u;
-        fQ[p][0] = ld_off(a.fxq, o4);
-        fQ[p][1] = ld_off(a.fyq, o4);
-      } else {
-        const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
-        vQ[p][0] = ld_off(a.xq, o);
-        vQ[p][1] = ld_off(a.yq, o);
-      }
+    const int k = min(tid + p * NT, HY * TX - 1);
+    const int r = k / TX, c = k - r * TX;
+    const int gj = min(max(j0 - H + r, 0), ny1), gi = min(i0 + c, nx1);
+    if constexpr (MC) {
+      const unsigned o4 = (unsigned)(gj * ld + gi) * 4u;
+      fQ[p][0] = ld_off(a.fxq, o4);
+      fQ[p][1] = ld_off(a.fyq, o4);
+    } else {
+      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+      vQ[p][0] = ld_off(a.xq, o);
+      vQ[p][1] = ld_off(a.yq, o);
     }
   }
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
@@ -1197,6 +1208,157 @@ __device__ __forceinline__ void interior_tile(const BulkArgs<T>& a, const int4 t
   }
 }
 
+// Field-lane forms of the coupling terms above: the field's F(UA), F(UB),
+// gamma flags and material factor are per-lane values (fld_* of lane field
+// l), and each product rounds as the templated form does.
+struct FldRt {
+  int ua, ub, mat;
+  bool ga, gb;
+};
+
+__device__ __forceinline__ FldRt fld_rt(int l) {
+  FldRt f;
+  f.ua = l == 0 ? 2 : (l == 1 ? 3 : (l == 3 ? 1 : 0));
+  f.ub = l == 0 ? 3 : (l == 1 ? 4 : (l == 3 ? 0 : 1));
+  f.mat = (l <= 1) ? 0 : (l == 3 ? 1 : 2);
+  f.ga = l == 4;
+  f.gb = l == 2;
+  return f;
+}
+
+template <typename T>
+__device__ __forceinline__ T prod_rt(T m, T f, T gamma, bool g) {
+  const T mf = m * f;
+  return g ? mf * gamma : mf;
+}
+
+template <typename T>
+__device__ __forceinline__ T bscale_rt(T t, T D, T rho, T G, T M, int mat) {
+  const T td = t * D;
+  return mat == 0 ? td / rho : (mat == 1 ? td * G : td * M);
+}
+
+// Band kernel with one lane per (point, field) for plane strain in a
+// uniform medium without dissipation, lean stages (no sources, MMS or
+// plastic update).  Twelve points share a wave, five lanes each (lanes 60-63
+// idle); the lane of field l evaluates DF(l) alone, with the terms of
+// closure_rates in their order, so that it holds a fifth of the stencil and
+// its loads are in flight together.  The field-2 lane also forms szz, DF(6)
+// = nu (DF(3) + DF(5)), from the field-4 lane two lanes up.  Every lane runs
+// to the shuffle (idle lanes on a clamped point) and only active ones store.
+template <typename T, int OP = 4>
+__device__ __forceinline__ void band_fl_wave(const BulkArgs<T>& a, const int2* __restrict__ pts, int npts,
+                                             int wave, int lane) {
+  constexpr int NB = Sbp<OP>::nbst, NS = Sbp<OP>::nbnd, HW = Sbp<OP>::half;
+  const int grp = lane / 5, l = min(lane - 5 * grp, 4);
+  const int k = wave * 12 + grp;
+  const bool active = lane < 60 && k < npts;
+  const int2 p = pts[min(k, npts - 1)];
+  const int i = p.x, j = p.y;
+  const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
+  const BlockDev& B = a.blocks[bi];
+  const long long ps = a.ps;
+  const int ld = a.ld;
+  const long long cc = (long long)j * ld + i;
+  const T* __restrict__ F = a.Fin;
+  const FldRt fr = fld_rt(l);
+  T d = a.a_zero ? (T)0.0 : (T)a.A * a.DF[l * ps + cc];
+  const int dl = i - B.mx, dr = B.px - i, db = j - B.my, dtp = B.py - j;
+  if (dl == 0 || dr == 0 || db == 0 || dtp == 0) {
+    // SAT penalties (all six fields, this lane's kept), sides L, R, B, T
+    T Fe[6], dd[6];
+#pragma unroll
+    for (int m = 0; m < 6; m++) {
+      Fe[m] = F[m * ps + cc];
+      dd[m] = d;
+    }
+    if (dl == 0) sat_term<T>(dd, Fe, a.sd, B.side_off[FDM_SIDE_L] + db);
+    if (dr == 0) sat_term<T>(dd, Fe, a.sd, B.side_off[FDM_SIDE_R] + db);
+    if (db == 0) sat_term<T>(dd, Fe, a.sd, B.side_off[FDM_SIDE_B] + dl);
+    if (dtp == 0) sat_term<T>(dd, Fe, a.sd, B.side_off[FDM_SIDE_T] + dl);
+    d = l == 0 ? dd[0] : (l == 1 ? dd[1] : (l == 2 ? dd[2] : (l == 3 ? dd[3] : dd[4])));
+  }
+  const T* __restrict__ xq = a.xq;
+  const T* __restrict__ xr = a.xr;
+  const T* __restrict__ yq = a.yq;
+  const T* __restrict__ yr = a.yr;
+  const T Ji = a.Ji[cc];
+  const T gamma = (T)B.gamma, rho = (T)B.rho, G = (T)B.G, M = (T)B.M;
+  const T* __restrict__ Fa = F + fr.ua * ps;
+  const T* __restrict__ Fb = F + fr.ub * ps;
+  const bool qone = (dl < NB) || (dr < NB);
+  const bool rone = (db < NB) || (dtp < NB);
+  const long long row = (long long)j * ld;
+  const long long colm = (long long)B.my * ld + i, colp = (long long)B.py * ld + i;
+  // q1: ay F(UA) - ax F(UB); r1: -by F(UA) + bx F(UB) (closure_rates)
+  if (qone) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const long long kq = (dl < NB) ? row + B.mx + s : row + B.px - s;
+      const T Dq = (dl < NB) ? (T)sbp_dl<OP>(dl, s) : (T)(-sbp_dl<OP>(dr, s));
+      const T ay = (T)0.5 * (yr[kq] + yr[cc]), ax = (T)0.5 * (xr[kq] + xr[cc]);
+      const T tq = prod_rt(ay, Fa[kq], gamma, fr.ga) - prod_rt(ax, Fb[kq], gamma, fr.gb);
+      d = d + bscale_rt(tq, Dq, rho, G, M, fr.mat) * Ji;
+      if (rone) {
+        const long long kr = (db < NB) ? colm + (long long)s * ld : colp - (long long)s * ld;
+        const T Dr = (db < NB) ? (T)sbp_dl<OP>(db, s) : (T)(-sbp_dl<OP>(dtp, s));
+        const T by = (T)0.5 * (yq[kr] + yq[cc]), bx = (T)0.5 * (xq[kr] + xq[cc]);
+        const T tr = -prod_rt(by, Fa[kr], gamma, fr.ga) + prod_rt(bx, Fb[kr], gamma, fr.gb);
+        d = d + bscale_rt(tr, Dr, rho, G, M, fr.mat) * Ji;
+      }
+    }
+    if (!rone) {
+      // L or R strip: central r (rc)
+#pragma unroll
+      for (int s = 1; s <= HW; s++) {
+        const T D = (T)sbp_di<OP>(s);
+        const long long kp = cc + (long long)s * ld, km = cc - (long long)s * ld;
+        const T byp = (T)0.5 * (yq[kp] + yq[cc]), bxp = (T)0.5 * (xq[kp] + xq[cc]);
+        const T bym = (T)0.5 * (yq[km] + yq[cc]), bxm = (T)0.5 * (xq[km] + xq[cc]);
+        const T t = -prod_rt(byp, Fa[kp], gamma, fr.ga) + prod_rt(bxp, Fb[kp], gamma, fr.gb) +
+                    prod_rt(bym, Fa[km], gamma, fr.ga) - prod_rt(bxm, Fb[km], gamma, fr.gb);
+        d = d + bscale_rt(t, D, rho, G, M, fr.mat) * Ji;
+      }
+    }
+  } else {
+    // B or T strip: one-sided r, then central q (qc)
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const long long kr = (db < NB) ? colm + (long long)s * ld : colp - (long long)s * ld;
+      const T Dr = (db < NB) ? (T)sbp_dl<OP>(db, s) : (T)(-sbp_dl<OP>(dtp, s));
+      const T by = (T)0.5 * (yq[kr] + yq[cc]), bx = (T)0.5 * (xq[kr] + xq[cc]);
+      const T tr = -prod_rt(by, Fa[kr], gamma, fr.ga) + prod_rt(bx, Fb[kr], gamma, fr.gb);
+      d = d + bscale_rt(tr, Dr, rho, G, M, fr.mat) * Ji;
+    }
+#pragma unroll
+    for (int s = 1; s <= HW; s++) {
+      const T D = (T)sbp_di<OP>(s);
+      const long long kp = cc + s, km = cc - s;
+      const T ayp = (T)0.5 * (yr[kp] + yr[cc]), axp = (T)0.5 * (xr[kp] + xr[cc]);
+      const T aym = (T)0.5 * (yr[km] + yr[cc]), axm = (T)0.5 * (xr[km] + xr[cc]);
+      const T t = prod_rt(ayp, Fa[kp], gamma, fr.ga) - prod_rt(axp, Fb[kp], gamma, fr.gb) -
+                  prod_rt(aym, Fa[km], gamma, fr.ga) + prod_rt(axm, Fb[km], gamma, fr.gb);
+      d = d + bscale_rt(t, D, rho, G, M, fr.mat) * Ji;
+    }
+  }
+  // finish_point (lean): szz, the field update, the rates
+  const T d4 = __shfl(d, (lane + 2) & 63, 64);
+  if (!active) return;
+  const T Bdt = (T)a.Bdt;
+  if (a.do_update) a.Fout[l * ps + cc] = F[l * ps + cc] + Bdt * d;
+  a.DF[l * ps + cc] = d;
+  if (l == 2) {
+    const T d5 = (T)B.nu * (d + d4);
+    if (a.do_update) a.Fout[5 * ps + cc] = F[5 * ps + cc] + Bdt * d5;
+    a.DF[5 * ps + cc] = d5;
+  }
+}
+
+template <typename T, int OP = 4>
+__global__ void __launch_bounds__(64) bulk_band_fl(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
+  band_fl_wave<T, OP>(a, pts, npts, blockIdx.x, threadIdx.x);
+}
+
 // a heterogeneous medium holds five more values per lane, dissipation the
 // Jacobian's staging: three waves per SIMD keep them in registers
 template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET, bool DISS, int OP = 4,
@@ -1204,7 +1366,24 @@ template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET, bool DISS, 
 __global__ void __launch_bounds__(TX * TY)
     __attribute__((amdgpu_waves_per_eu(FULL ? 2 : ((HET || DISS || OP == 6) ? 3 : 4))))
 bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
-  interior_tile<T, TX, TY, FULL, NTS, HET, DISS, OP, JI, MC>(a, tiles[linear_tile(a.xcd_swizzle)]);
+  if constexpr (!HET && !DISS && OP == 4) {
+    // The first nibwg workgroups update the inner band points, one lane per
+    // point.  They need only what the tiles need (the previous stage's band
+    // and interior), so they ride in this launch rather than on the second
+    // stream after the edge values; dispatched first, their closures (bound
+    // by load latency) overlap the tiles.
+    if ((int)blockIdx.x < a.nibwg) {
+      // one lane per point (five lanes per point, as in bulk_band_fl, made
+      // this launch slower: 600 workgroups ahead of the tiles instead of 112)
+      const int k = (int)blockIdx.x * (TX * TY) + (int)(threadIdx.y * TX + threadIdx.x);
+      if (k < a.nib) {
+        const int2 p = a.ibpts[k];
+        point_generic<T, FULL, false, OP>(a, p.x, p.y);
+      }
+      return;
+    }
+  }
+  interior_tile<T, TX, TY, FULL, NTS, HET, DISS, OP, JI, MC>(a, tiles[linear_tile(a.xcd_swizzle, a.nibwg)]);
 }
 
 // Batched interior kernel: the tiles of many problems in one launch; tile
@@ -1215,6 +1394,7 @@ bulk_interior_batch(const BulkArgs<T>* __restrict__ tab, const int4* __restrict_
   const int4 td = tiles[linear_tile(swz)];
   interior_tile<T, TX, TY, FULL, NTS, false, false>(tab[td.w], td);
 }
+
 
 // Band kernel: the points within the closure depth of their block's edge
 // (one-sided closures, SAT), one lane each, from a list of (i, j).
@@ -1596,39 +1776,35 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
   const long long ps = a.ps;
   const T* __restrict__ F = a.Fin;
   T vF[NF][3], vR[NR][2], vQ[NQ][2], vJ[DISS ? NF : 1];
+  // every lane loads in every pass (past the end: the last element again);
+  // only the LDS stores are conditional, as in the plane-strain tile
 #pragma unroll
   for (int p = 0; p < NF; p++) {
-    const int k = tid + p * NT;
-    if (k < HY * HX) {
-      const int r = k / HX, c = k - r * HX;
-      const int gj = min(max(j0 - H + r, 0), ny1), gi = min(max(i0 - H + c, 0), nx1);
-      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+    const int k = min(tid + p * NT, HY * HX - 1);
+    const int r = k / HX, c = k - r * HX;
+    const int gj = min(max(j0 - H + r, 0), ny1), gi = min(max(i0 - H + c, 0), nx1);
+    const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
 #pragma unroll
-      for (int l = 0; l < 3; l++) vF[p][l] = ld_off(F + l * ps, o);
-      if constexpr (DISS) vJ[p] = ld_off(a.J, o);
-    }
+    for (int l = 0; l < 3; l++) vF[p][l] = ld_off(F + l * ps, o);
+    if constexpr (DISS) vJ[p] = ld_off(a.J, o);
   }
 #pragma unroll
   for (int p = 0; p < NR; p++) {
-    const int k = tid + p * NT;
-    if (k < TY * HX) {
-      const int r = k / HX, c = k - r * HX;
-      const int gj = min(j0 + r, ny1), gi = min(max(i0 - H + c, 0), nx1);
-      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
-      vR[p][0] = ld_off(a.xr, o);
-      vR[p][1] = ld_off(a.yr, o);
-    }
+    const int k = min(tid + p * NT, TY * HX - 1);
+    const int r = k / HX, c = k - r * HX;
+    const int gj = min(j0 + r, ny1), gi = min(max(i0 - H + c, 0), nx1);
+    const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+    vR[p][0] = ld_off(a.xr, o);
+    vR[p][1] = ld_off(a.yr, o);
   }
 #pragma unroll
   for (int p = 0; p < NQ; p++) {
-    const int k = tid + p * NT;
-    if (k < HY * TX) {
-      const int r = k / TX, c = k - r * TX;
-      const int gj = min(max(j0 - H + r, 0), ny1), gi = min(i0 + c, nx1);
-      const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
-      vQ[p][0] = ld_off(a.xq, o);
-      vQ[p][1] = ld_off(a.yq, o);
-    }
+    const int k = min(tid + p * NT, HY * TX - 1);
+    const int r = k / TX, c = k - r * TX;
+    const int gj = min(max(j0 - H + r, 0), ny1), gi = min(i0 + c, nx1);
+    const unsigned o = (unsigned)(gj * ld + gi) * (unsigned)sizeof(T);
+    vQ[p][0] = ld_off(a.xq, o);
+    vQ[p][1] = ld_off(a.yq, o);
   }
   const int i = i0 + threadIdx.x, j = j0 + threadIdx.y;
   const bool valid = i <= B4->px - NB && j <= B4->py - NB;

@@ -1,7 +1,10 @@
 """The bulk kernels (interior LDS tiles plus band points, or one lane per
 point everywhere) evaluate the same expression tree per point, so on the
 same inputs they must give bit-identical fields, rates and fault state, in
-FP64 and in FP32-bulk, with either kind of store (plain or non-temporal).  Grids here are large enough for interior tiles,
+FP64 and in FP32-bulk, with either kind of store (plain or non-temporal), and
+with the band points off the block edges updated by the interior launch or by
+the band kernel, and the band kernel with one lane per point or per (point,
+field).  Grids here are large enough for interior tiles,
 ragged tile edges in both directions and several block columns.
 """
 import numpy as np
@@ -18,12 +21,22 @@ BG = [0.0, 0.0, -120.0, 70.0, -120.0, -120.0]
 MODES = [("generic", {}), ("tile", {"FDM_TILE": "64x8"}), ("tile", {"FDM_TILE": "64x4"}),
          ("tile", {"FDM_TILE": "128x4"}), ("tile", {"FDM_TILE": "64x8", "FDM_XCD_SWIZZLE": "0"}),
          ("tile", {"FDM_TILE": "64x4", "FDM_NT_STORE": "1"}),
-         ("tile", {"FDM_TILE": "128x4", "FDM_NT_STORE": "1"})]
+         ("tile", {"FDM_TILE": "128x4", "FDM_NT_STORE": "1"}),
+         # every band point in the second stream's band kernel, none in the
+         # interior launch
+         ("tile", {"FDM_TILE": "64x4", "FDM_BAND_SPLIT": "0"}),
+         ("tile", {"FDM_TILE": "64x8", "FDM_BAND_SPLIT": "0", "FDM_NT_STORE": "1"}),
+         # one lane per band point rather than per (point, field), and both
+         # forms whatever the precision's default
+         ("tile", {"FDM_TILE": "64x4", "FDM_BAND_FL": "0"}),
+         ("tile", {"FDM_TILE": "64x4", "FDM_BAND_FL": "0", "FDM_BAND_SPLIT": "0"}),
+         ("tile", {"FDM_TILE": "64x4", "FDM_BAND_FL": "1", "FDM_BAND_SPLIT": "1"}),
+         ("tile", {"FDM_TILE": "64x8", "FDM_BAND_FL": "1", "FDM_BAND_SPLIT": "0"})]
 
 
 def run_mode(monkeypatch, prob, F, nsteps, mode, env):
     monkeypatch.setenv("FDM_BULK", mode)
-    for k in ("FDM_TILE", "FDM_XCD_SWIZZLE", "FDM_NT_STORE"):
+    for k in ("FDM_TILE", "FDM_XCD_SWIZZLE", "FDM_NT_STORE", "FDM_BAND_SPLIT", "FDM_BAND_FL"):
         monkeypatch.delenv(k, raising=False)
     for k, v in env.items():
         monkeypatch.setenv(k, v)
