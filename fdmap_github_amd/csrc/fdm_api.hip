@@ -1769,11 +1769,12 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
     int z = 0;
     if ((rc = dupload(c, &c->fail, &z, sizeof(int)))) return fail(rc);
   }
-  // Both band forms pay off in FP64, where the interior launch is long
-  // enough to carry the inner band points and the field lanes shorten the
-  // second stream; in FP32-bulk (an interior launch half as long) the one
-  // band kernel of one lane per point keeps the step shortest.
-  c->band_fl = c->precision == FDM_FP64;
+  // Both band forms pay off in FP64 with SBP4, where the interior launch is
+  // long enough to carry the inner band points and the field lanes shorten
+  // the second stream; in FP32-bulk (an interior launch half as long) and
+  // with SBP6 (whose field lanes took longer) the band kernel of one lane
+  // per point keeps the step shortest.
+  c->band_fl = c->precision == FDM_FP64 && c->sbp == 4;
   if (const char* e = getenv("FDM_BAND_FL")) c->band_fl = atoi(e) != 0;
   if (c->bulk_mode == 2 && c->mode == 2 && c->sbp == 4 && !c->het && !c->diss && c->nband) {
     // band split: the band points on no block edge need no edge values of
