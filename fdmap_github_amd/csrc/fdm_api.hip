@@ -281,6 +281,10 @@ struct fdm_ctx {
   // lean plane-strain band points in a uniform medium: one lane per (point,
   // field) (bulk_band_fl; FP64 by default, FDM_BAND_FL=0|1 overrides)
   bool band_fl = true;
+  // antiplane band points without dissipation: one lane per (point, field)
+  // (bulk_band3_fl) when FDM_BAND_FL3=1; off by default, since it lengthened
+  // the C2-sized antiplane step (0.380 against 0.366 ms in FP64)
+  bool band_fl3 = false;
   int2* iband = nullptr;
   int2* eband = nullptr;
   int niband = 0, neband = 0;
@@ -569,7 +573,10 @@ void launch_band_op(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid) 
   // with the band split, the edge points only
   const int2* pts = c->band_split ? c->eband : c->band;
   const int n = c->band_split ? c->neband : c->nband;
-  if (c->mode == 3)
+  if (c->mode == 3 && c->band_fl3 && !c->diss)
+    // one lane per (point, field), 21 points per wave
+    hipLaunchKernelGGL((bulk_band3_fl<T, OP>), dim3((n + 20) / 21), dim3(64), 0, s, a, pts, n);
+  else if (c->mode == 3)
     hipLaunchKernelGGL((bulk_band3<T, OP>), grid, dim3(64), 0, s, a, pts, n);
   else if (full_variant(c, a) && (c->het || c->diss))
     hipLaunchKernelGGL((bulk_band<T, true, true, OP>), grid, dim3(64), 0, s, a, pts, n);
@@ -1776,6 +1783,7 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   // per point keeps the step shortest.
   c->band_fl = c->precision == FDM_FP64 && c->sbp == 4;
   if (const char* e = getenv("FDM_BAND_FL")) c->band_fl = atoi(e) != 0;
+  if (const char* e = getenv("FDM_BAND_FL3")) c->band_fl3 = atoi(e) != 0;
   if (c->bulk_mode == 2 && c->mode == 2 && c->sbp == 4 && !c->het && !c->diss && c->nband) {
     // band split: the band points on no block edge need no edge values of
     // the stage and join the interior launch (bulk_interior); the second

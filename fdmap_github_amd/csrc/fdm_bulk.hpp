@@ -1742,6 +1742,115 @@ __global__ void __launch_bounds__(256) bulk_kernel3(BulkArgs<T> a) {
   point_generic3<T, OP>(a, i, j);
 }
 
+// Antiplane band kernel with one lane per (point, field) for blocks
+// without dissipation: 21 points per wave, three lanes each (lane 63
+// idle).  The lane of field l evaluates DF(l) alone with closure_rates3's
+// terms in their order (a medium's rho and G per point as there); the three
+// fields are not coupled after the rates, so each lane finishes its own.
+// A point of a dissipative block is left to bulk_band3 (the host sends
+// only blocks without dissipation here).
+template <typename T, int OP = 4>
+__global__ void __launch_bounds__(64) bulk_band3_fl(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
+  constexpr int NB = Sbp<OP>::nbst, NS = Sbp<OP>::nbnd, HW = Sbp<OP>::half;
+  const int lane = threadIdx.x;
+  const int grp = lane / 3, l = lane - 3 * grp;
+  const int k = blockIdx.x * 21 + grp;
+  if (lane >= 63 || k >= npts) return;
+  const int2 p = pts[k];
+  const int i = p.x, j = p.y;
+  const int bi = a.blk[a.rowb[j] * a.nbx + a.colb[i]];
+  const BlockDev& B = a.blocks[bi];
+  const long long ps = a.ps;
+  const int ld = a.ld;
+  const long long cc = (long long)j * ld + i;
+  const T* __restrict__ F = a.Fin;
+  T d = a.a_zero ? (T)0.0 : (T)a.A * a.DF[l * ps + cc];
+  const int dl = i - B.mx, dr = B.px - i, db = j - B.my, dtp = B.py - j;
+  if (dl == 0 || dr == 0 || db == 0 || dtp == 0) {
+    const T Fe = F[l * ps + cc];
+    const int NSd = a.sd.NS;
+    const int sides[4] = {dl == 0 ? B.side_off[FDM_SIDE_L] + db : -1,
+                          dr == 0 ? B.side_off[FDM_SIDE_R] + db : -1,
+                          db == 0 ? B.side_off[FDM_SIDE_B] + dl : -1,
+                          dtp == 0 ? B.side_off[FDM_SIDE_T] + dl : -1};
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int so = sides[e];
+      if (so < 0) continue;
+      const T Ks = (T)a.sd.M[3 * NSd + so];
+      d = d + Ks * ((T)a.sd.Fhat[l * NSd + so] - Fe);
+    }
+  }
+  const T* __restrict__ xq = a.xq;
+  const T* __restrict__ xr = a.xr;
+  const T* __restrict__ yq = a.yq;
+  const T* __restrict__ yr = a.yr;
+  const T Ji = a.Ji[cc];
+  const T rho = a.mrho ? a.mrho[cc] : (T)B.rho, G = a.mrho ? a.mG[cc] : (T)B.G;
+  // field 0 (vz) pairs sxz with the y-metrics and syz with the x-metrics;
+  // fields 1 and 2 (sxz, syz) take vz alone
+  const T* __restrict__ Fa = F + (l == 0 ? ps : 0);
+  const T* __restrict__ Fb = F + 2 * ps;
+  const bool qone = (dl < NB) || (dr < NB);
+  const bool rone = (db < NB) || (dtp < NB);
+  const long long row = (long long)j * ld;
+  const long long colm = (long long)B.my * ld + i, colp = (long long)B.py * ld + i;
+  // q1_3, r1_3, qc_3, rc_3 and bscale3 of field l
+  auto q1 = [&](long long kk) {
+    const T ay = (T)0.5 * (yr[kk] + yr[cc]), ax = (T)0.5 * (xr[kk] + xr[cc]);
+    return l == 0 ? ay * Fa[kk] - ax * Fb[kk] : (l == 1 ? ay * Fa[kk] : -(ax * Fa[kk]));
+  };
+  auto r1 = [&](long long kk) {
+    const T by = (T)0.5 * (yq[kk] + yq[cc]), bx = (T)0.5 * (xq[kk] + xq[cc]);
+    return l == 0 ? -(by * Fa[kk]) + bx * Fb[kk] : (l == 1 ? -(by * Fa[kk]) : bx * Fa[kk]);
+  };
+  auto sc = [&](T t, T D) { return l == 0 ? t * D / rho : t * D * G; };
+  if (qone) {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const long long kq = (dl < NB) ? row + B.mx + s : row + B.px - s;
+      const T Dq = (dl < NB) ? (T)sbp_dl<OP>(dl, s) : (T)(-sbp_dl<OP>(dr, s));
+      d = d + sc(q1(kq), Dq) * Ji;
+      if (rone) {
+        const long long kr = (db < NB) ? colm + (long long)s * ld : colp - (long long)s * ld;
+        const T Dr = (db < NB) ? (T)sbp_dl<OP>(db, s) : (T)(-sbp_dl<OP>(dtp, s));
+        d = d + sc(r1(kr), Dr) * Ji;
+      }
+    }
+    if (!rone) {
+#pragma unroll
+      for (int s = 1; s <= HW; s++) {
+        const T D = (T)sbp_di<OP>(s);
+        const long long kp = cc + (long long)s * ld, km = cc - (long long)s * ld;
+        const T byp = (T)0.5 * (yq[kp] + yq[cc]), bxp = (T)0.5 * (xq[kp] + xq[cc]);
+        const T bym = (T)0.5 * (yq[km] + yq[cc]), bxm = (T)0.5 * (xq[km] + xq[cc]);
+        const T t = l == 0 ? -(byp * Fa[kp]) + bxp * Fb[kp] + bym * Fa[km] - bxm * Fb[km]
+                           : (l == 1 ? -(byp * Fa[kp]) + bym * Fa[km] : bxp * Fa[kp] - bxm * Fa[km]);
+        d = d + sc(t, D) * Ji;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const long long kr = (db < NB) ? colm + (long long)s * ld : colp - (long long)s * ld;
+      const T Dr = (db < NB) ? (T)sbp_dl<OP>(db, s) : (T)(-sbp_dl<OP>(dtp, s));
+      d = d + sc(r1(kr), Dr) * Ji;
+    }
+#pragma unroll
+    for (int s = 1; s <= HW; s++) {
+      const T D = (T)sbp_di<OP>(s);
+      const long long kp = cc + s, km = cc - s;
+      const T ayp = (T)0.5 * (yr[kp] + yr[cc]), axp = (T)0.5 * (xr[kp] + xr[cc]);
+      const T aym = (T)0.5 * (yr[km] + yr[cc]), axm = (T)0.5 * (xr[km] + xr[cc]);
+      const T t = l == 0 ? ayp * Fa[kp] - axp * Fb[kp] - aym * Fa[km] + axm * Fb[km]
+                         : (l == 1 ? ayp * Fa[kp] - aym * Fa[km] : -(axp * Fa[kp]) + axm * Fa[km]);
+      d = d + sc(t, D) * Ji;
+    }
+  }
+  if (a.do_update) a.Fout[l * ps + cc] = F[l * ps + cc] + (T)a.Bdt * d;
+  a.DF[l * ps + cc] = d;
+}
+
 template <typename T, int OP = 4>
 __global__ void __launch_bounds__(64) bulk_band3(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
