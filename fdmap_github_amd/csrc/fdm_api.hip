@@ -285,6 +285,11 @@ struct fdm_ctx {
   // (bulk_band3_fl) when FDM_BAND_FL3=1; off by default, since it lengthened
   // the C2-sized antiplane step (0.380 against 0.366 ms in FP64)
   bool band_fl3 = false;
+  // plane strain in a medium or with dissipation: field lanes too
+  // (bulk_band_fl<..., EXT>): by default only in FP32-bulk in a medium
+  // without dissipation, the one case they shortened (FDM_BAND_FL_EXT
+  // overrides)
+  bool band_fl_ext = false;
   int2* iband = nullptr;
   int2* eband = nullptr;
   int niband = 0, neband = 0;
@@ -582,6 +587,8 @@ void launch_band_op(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a, dim3 grid) 
     hipLaunchKernelGGL((bulk_band<T, true, true, OP>), grid, dim3(64), 0, s, a, pts, n);
   else if (full_variant(c, a))
     hipLaunchKernelGGL((bulk_band<T, true, false, OP>), grid, dim3(64), 0, s, a, pts, n);
+  else if ((c->het || c->diss) && c->band_fl_ext)
+    hipLaunchKernelGGL((bulk_band_fl<T, OP, true>), dim3((n + 11) / 12), dim3(64), 0, s, a, pts, n);
   else if (c->het || c->diss)
     hipLaunchKernelGGL((bulk_band<T, false, true, OP>), grid, dim3(64), 0, s, a, pts, n);
   else if (c->band_fl)
@@ -1784,6 +1791,8 @@ int fdm_create(const fdm_problem* P, fdm_ctx** out) {
   c->band_fl = c->precision == FDM_FP64 && c->sbp == 4;
   if (const char* e = getenv("FDM_BAND_FL")) c->band_fl = atoi(e) != 0;
   if (const char* e = getenv("FDM_BAND_FL3")) c->band_fl3 = atoi(e) != 0;
+  c->band_fl_ext = c->precision == FDM_FP32_BULK && c->het && !c->diss;
+  if (const char* e = getenv("FDM_BAND_FL_EXT")) c->band_fl_ext = atoi(e) != 0;
   if (c->bulk_mode == 2 && c->mode == 2 && c->sbp == 4 && !c->het && !c->diss && c->nband) {
     // band split: the band points on no block edge need no edge values of
     // the stage and join the interior launch (bulk_interior); the second

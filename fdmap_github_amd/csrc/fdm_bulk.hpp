@@ -1246,7 +1246,7 @@ __device__ __forceinline__ T bscale_rt(T t, T D, T rho, T G, T M, int mat) {
 // its loads are in flight together.  The field-2 lane also forms szz, DF(6)
 // = nu (DF(3) + DF(5)), from the field-4 lane two lanes up.  Every lane runs
 // to the shuffle (idle lanes on a clamped point) and only active ones store.
-template <typename T, int OP = 4>
+template <typename T, int OP = 4, bool EXT = false>
 __device__ __forceinline__ void band_fl_wave(const BulkArgs<T>& a, const int2* __restrict__ pts, int npts,
                                              int wave, int lane) {
   constexpr int NB = Sbp<OP>::nbst, NS = Sbp<OP>::nbnd, HW = Sbp<OP>::half;
@@ -1283,7 +1283,15 @@ __device__ __forceinline__ void band_fl_wave(const BulkArgs<T>& a, const int2* _
   const T* __restrict__ yq = a.yq;
   const T* __restrict__ yr = a.yr;
   const T Ji = a.Ji[cc];
-  const T gamma = (T)B.gamma, rho = (T)B.rho, G = (T)B.G, M = (T)B.M;
+  // EXT: a heterogeneous medium (the point's material) and artificial
+  // dissipation (SBP4 only), as in closure_rates
+  const bool het = EXT && a.mrho != nullptr;
+  const T gamma = het ? a.mgam[cc] : (T)B.gamma, rho = het ? a.mrho[cc] : (T)B.rho;
+  const T G = het ? a.mG[cc] : (T)B.G, M = het ? a.mM[cc] : (T)B.M;
+  const bool diss = EXT && OP == 4 && B.Cdiss > 0.0;
+  const T Cd = (T)B.Cdiss;
+  const T* __restrict__ Jp = a.J;
+  const T* __restrict__ Fl = F + l * ps;
   const T* __restrict__ Fa = F + fr.ua * ps;
   const T* __restrict__ Fb = F + fr.ub * ps;
   const bool qone = (dl < NB) || (dr < NB);
@@ -1299,16 +1307,22 @@ __device__ __forceinline__ void band_fl_wave(const BulkArgs<T>& a, const int2* _
       const T ay = (T)0.5 * (yr[kq] + yr[cc]), ax = (T)0.5 * (xr[kq] + xr[cc]);
       const T tq = prod_rt(ay, Fa[kq], gamma, fr.ga) - prod_rt(ax, Fb[kq], gamma, fr.gb);
       d = d + bscale_rt(tq, Dq, rho, G, M, fr.mat) * Ji;
+      if constexpr (EXT && OP == 4)
+        if (diss) d = d + Jp[kq] * Fl[kq] * (T)((dl < 4) ? kAL[dl][s] : kAL[dr][s]) * Ji * Cd;
       if (rone) {
         const long long kr = (db < NB) ? colm + (long long)s * ld : colp - (long long)s * ld;
         const T Dr = (db < NB) ? (T)sbp_dl<OP>(db, s) : (T)(-sbp_dl<OP>(dtp, s));
         const T by = (T)0.5 * (yq[kr] + yq[cc]), bx = (T)0.5 * (xq[kr] + xq[cc]);
         const T tr = -prod_rt(by, Fa[kr], gamma, fr.ga) + prod_rt(bx, Fb[kr], gamma, fr.gb);
         d = d + bscale_rt(tr, Dr, rho, G, M, fr.mat) * Ji;
+        if constexpr (EXT && OP == 4)
+          if (diss) d = d + Jp[kr] * Fl[kr] * (T)((db < 4) ? kAL[db][s] : kAL[dtp][s]) * Ji * Cd;
       }
     }
     if (!rone) {
       // L or R strip: central r (rc)
+      if constexpr (EXT && OP == 4)
+        if (diss) d = d + Fl[cc] * (T)FDM_AI0 * Cd;
 #pragma unroll
       for (int s = 1; s <= HW; s++) {
         const T D = (T)sbp_di<OP>(s);
@@ -1318,6 +1332,8 @@ __device__ __forceinline__ void band_fl_wave(const BulkArgs<T>& a, const int2* _
         const T t = -prod_rt(byp, Fa[kp], gamma, fr.ga) + prod_rt(bxp, Fb[kp], gamma, fr.gb) +
                     prod_rt(bym, Fa[km], gamma, fr.ga) - prod_rt(bxm, Fb[km], gamma, fr.gb);
         d = d + bscale_rt(t, D, rho, G, M, fr.mat) * Ji;
+        if constexpr (EXT && OP == 4)
+          if (diss) d = d + (Jp[kp] * Fl[kp] + Jp[km] * Fl[km]) * ((s == 1) ? (T)FDM_AI1 : (T)FDM_AI2) * Ji * Cd;
       }
     }
   } else {
@@ -1329,7 +1345,11 @@ __device__ __forceinline__ void band_fl_wave(const BulkArgs<T>& a, const int2* _
       const T by = (T)0.5 * (yq[kr] + yq[cc]), bx = (T)0.5 * (xq[kr] + xq[cc]);
       const T tr = -prod_rt(by, Fa[kr], gamma, fr.ga) + prod_rt(bx, Fb[kr], gamma, fr.gb);
       d = d + bscale_rt(tr, Dr, rho, G, M, fr.mat) * Ji;
+      if constexpr (EXT && OP == 4)
+        if (diss) d = d + Jp[kr] * Fl[kr] * (T)((db < 4) ? kAL[db][s] : kAL[dtp][s]) * Ji * Cd;
     }
+    if constexpr (EXT && OP == 4)
+      if (diss) d = d + Fl[cc] * (T)FDM_AI0 * Cd;
 #pragma unroll
     for (int s = 1; s <= HW; s++) {
       const T D = (T)sbp_di<OP>(s);
@@ -1339,6 +1359,8 @@ __device__ __forceinline__ void band_fl_wave(const BulkArgs<T>& a, const int2* _
       const T t = prod_rt(ayp, Fa[kp], gamma, fr.ga) - prod_rt(axp, Fb[kp], gamma, fr.gb) -
                   prod_rt(aym, Fa[km], gamma, fr.ga) + prod_rt(axm, Fb[km], gamma, fr.gb);
       d = d + bscale_rt(t, D, rho, G, M, fr.mat) * Ji;
+      if constexpr (EXT && OP == 4)
+        if (diss) d = d + (Jp[kp] * Fl[kp] + Jp[km] * Fl[km]) * ((s == 1) ? (T)FDM_AI1 : (T)FDM_AI2) * Ji * Cd;
     }
   }
   // finish_point (lean): szz, the field update, the rates
@@ -1348,15 +1370,15 @@ __device__ __forceinline__ void band_fl_wave(const BulkArgs<T>& a, const int2* _
   if (a.do_update) a.Fout[l * ps + cc] = F[l * ps + cc] + Bdt * d;
   a.DF[l * ps + cc] = d;
   if (l == 2) {
-    const T d5 = (T)B.nu * (d + d4);
+    const T d5 = (a.mnu ? a.mnu[cc] : (T)B.nu) * (d + d4);
     if (a.do_update) a.Fout[5 * ps + cc] = F[5 * ps + cc] + Bdt * d5;
     a.DF[5 * ps + cc] = d5;
   }
 }
 
-template <typename T, int OP = 4>
+template <typename T, int OP = 4, bool EXT = false>
 __global__ void __launch_bounds__(64) bulk_band_fl(BulkArgs<T> a, const int2* __restrict__ pts, int npts) {
-  band_fl_wave<T, OP>(a, pts, npts, blockIdx.x, threadIdx.x);
+  band_fl_wave<T, OP, EXT>(a, pts, npts, blockIdx.x, threadIdx.x);
 }
 
 // a heterogeneous medium holds five more values per lane, dissipation the

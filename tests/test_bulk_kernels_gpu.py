@@ -33,13 +33,15 @@ MODES = [("generic", {}), ("tile", {"FDM_TILE": "64x8"}), ("tile", {"FDM_TILE": 
          ("tile", {"FDM_TILE": "64x4", "FDM_BAND_FL": "1", "FDM_BAND_SPLIT": "1"}),
          ("tile", {"FDM_TILE": "64x8", "FDM_BAND_FL": "1", "FDM_BAND_SPLIT": "0"}),
          # antiplane band points one lane per (point, field) rather than per point
-         ("tile", {"FDM_TILE": "64x4", "FDM_BAND_FL3": "1"})]
+         ("tile", {"FDM_TILE": "64x4", "FDM_BAND_FL3": "1"}),
+         # the same for plane strain in a medium or with dissipation
+         ("tile", {"FDM_TILE": "64x4", "FDM_BAND_FL_EXT": "1"})]
 
 
 def run_mode(monkeypatch, prob, F, nsteps, mode, env):
     monkeypatch.setenv("FDM_BULK", mode)
     for k in ("FDM_TILE", "FDM_XCD_SWIZZLE", "FDM_NT_STORE", "FDM_BAND_SPLIT", "FDM_BAND_FL",
-              "FDM_BAND_FL3"):
+              "FDM_BAND_FL3", "FDM_BAND_FL_EXT"):
         monkeypatch.delenv(k, raising=False)
     for k, v in env.items():
         monkeypatch.setenv(k, v)
