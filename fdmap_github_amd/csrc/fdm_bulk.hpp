@@ -1388,7 +1388,7 @@ template <typename T, int TX, int TY, bool FULL, bool NTS, bool HET, bool DISS, 
 __global__ void __launch_bounds__(TX * TY)
     __attribute__((amdgpu_waves_per_eu(FULL ? 2 : ((HET || DISS || OP == 6) ? 3 : 4))))
 bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
-  if constexpr (!HET && !DISS && OP == 4) {
+  if constexpr (!HET && OP == 4) {
     // The first nibwg workgroups update the inner band points, one lane per
     // point.  They need only what the tiles need (the previous stage's band
     // and interior), so they ride in this launch rather than on the second
@@ -1400,7 +1400,7 @@ bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
       const int k = (int)blockIdx.x * (TX * TY) + (int)(threadIdx.y * TX + threadIdx.x);
       if (k < a.nib) {
         const int2 p = a.ibpts[k];
-        point_generic<T, FULL, false, OP>(a, p.x, p.y);
+        point_generic<T, FULL, DISS, OP>(a, p.x, p.y);
       }
       return;
     }
