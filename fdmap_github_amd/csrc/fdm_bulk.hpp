@@ -1892,6 +1892,22 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
   static_assert(!DISS || OP == 4, "artificial dissipation is built for SBP4 only");
   constexpr int H = Sbp<OP>::half, NB = Sbp<OP>::nbst;
   constexpr int HX = TX + 2 * H, HY = TY + 2 * H, NT = TX * TY;
+  if constexpr (!HET && !DISS && OP == 4) {
+    // the inner band points ahead of the tiles, one lane per point, as in
+    // the plane-strain launch (band split, on request).  This path takes
+    // the FP64 kernel from 76 to 120 VGPRs (six waves per SIMD to four);
+    // with the split off, the four-wave kernel made the antiplane step
+    // shorter (0.356 against 0.366 ms on the C2-size grid, FP32-bulk 0.215
+    // against 0.221).
+    if ((int)blockIdx.x < a.nibwg) {
+      const int k = (int)blockIdx.x * NT + (int)(threadIdx.y * TX + threadIdx.x);
+      if (k < a.nib) {
+        const int2 p = a.ibpts[k];
+        point_generic3<T, OP>(a, p.x, p.y);
+      }
+      return;
+    }
+  }
   constexpr int NF = (HY * HX + NT - 1) / NT;
   constexpr int NR = (TY * HX + NT - 1) / NT;
   constexpr int NQ = (HY * TX + NT - 1) / NT;
@@ -1899,7 +1915,7 @@ bulk_interior3(BulkArgs<T> a, const int4* __restrict__ tiles) {
   __shared__ T sR[2][TY][HX];
   __shared__ T sQ[2][HY][TX];
   __shared__ T sJ[DISS ? HY : 1][DISS ? HX : 1];
-  const int4 td = tiles[linear_tile(a.xcd_swizzle)];
+  const int4 td = tiles[linear_tile(a.xcd_swizzle, a.nibwg)];
   const int bu = td.x, i0 = td.y, j0 = td.z;
   ConstBlockDev* B4 = (ConstBlockDev*)a.blocks + bu;
   const int tid = threadIdx.y * TX + threadIdx.x;
