@@ -569,6 +569,10 @@ BulkArgs<T> bulk_args(fdm_ctx* c, double A, double Bdt, int do_update, double t,
   a.ibpts = c->iband;
   a.nib = c->niband;
   a.nibwg = 0;  // set by the interior launch that carries them
+  a.rcc = nullptr;
+  a.rout = nullptr;
+  a.nrec = 0;
+  a.nrecwg = 0;
   return a;
 }
 
@@ -760,7 +764,8 @@ void launch_interior(fdm_ctx* c, hipStream_t s, const BulkArgs<T>& a) {
     BulkArgs<T> b = a;
     const int nt = kTileShape[q][0] * kTileShape[q][1];
     b.nibwg = c->band_split ? (c->niband + nt - 1) / nt : 0;
-    grid.x += b.nibwg;
+    b.nrecwg = (a.nrec + nt - 1) / nt;
+    grid.x += b.nrecwg + b.nibwg;
     if (c->diss) launch_interior_j<T, 64, 4, false, true, 4>(c, s, b, grid, block, full);
     else if (q == 0) launch_interior_j<T, 64, 4, false, false, 4>(c, s, b, grid, block, full);
     else if (q == 1) launch_interior_j<T, 64, 8, false, false, 4>(c, s, b, grid, block, full);
@@ -837,6 +842,17 @@ int run_stage(fdm_ctx* c, int stage, double t0, double dt, int final_step, int s
   BulkArgs<T> a =
       bulk_args<T>(c, A, Bdt, final_step ? 0 : 1, t, srcA, (!final_step && last) ? 1 : 0, dt);
   a.skip_df = (skip_df && last && !final_step) ? 1 : 0;
+  // receivers of the previous step: in stage 1's interior launch where its
+  // kernel carries them (FP64 plane strain, SBP4, uniform medium), else a
+  // launch of their own
+  const bool rec_now = c->rec_out && stage == 1 && !final_step && c->cur_n > c->first_n;
+  const bool rec_fused = rec_now && std::is_same<T, double>::value && c->bulk_mode == 2 &&
+                         c->mode == 2 && c->sbp == 4 && !c->het && c->ntiles[c->tile_shape] > 0;
+  if (rec_fused) {
+    a.rcc = c->rec_cc;
+    a.rout = c->rec_out + (size_t)(c->cur_n - 1 - c->rec_n0) * c->nrec * 2;
+    a.nrec = c->nrec;
+  }
   int rc;
   if (c->bulk_mode == 2) {
     // Interior points need neither this stage's edge values nor this
@@ -856,7 +872,7 @@ int run_stage(fdm_ctx* c, int stage, double t0, double dt, int final_step, int s
     launch_layer<T>(c, c->side, a);
     // the previous step's fields are this stage's input until stage 2
     // writes them; both streams have finished them here
-    if (c->rec_out && stage == 1 && !final_step && c->cur_n > c->first_n) record_receivers(c, c->side, c->cur_n - 1);
+    if (rec_now && !rec_fused) record_receivers(c, c->side, c->cur_n - 1);
     HIPCHK(c, hipEventRecord(c->evB, c->side));
     HIPCHK(c, hipEventRecord(c->evI, c->stream));
   } else {

@@ -72,6 +72,12 @@ struct BulkArgs {
   // interior launch, one lane per point (nib points; none: nibwg = 0)
   const int2* ibpts;
   int nib, nibwg;
+  // receivers: vx, vy of Fin at the nrec points rcc into rout (FP64
+  // plane-strain contexts), by nrecwg workgroups ahead of the inner band
+  // points in stage 1 of a step; none: nrecwg = 0
+  const long long* rcc;
+  double* rout;
+  int nrec, nrecwg;
 };
 
 // Map the hardware workgroup id to a tile.  Workgroups are dispatched to the
@@ -1394,10 +1400,24 @@ bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
     // and interior), so they ride in this launch rather than on the second
     // stream after the edge values; dispatched first, their closures (bound
     // by load latency) overlap the tiles.
-    if ((int)blockIdx.x < a.nibwg) {
+    if constexpr (std::is_same<T, double>::value) {
+      if ((int)blockIdx.x < a.nrecwg) {
+        // the receivers of the previous step's fields (this stage's input,
+        // which the launch does not write)
+        const int r = (int)blockIdx.x * (TX * TY) + (int)(threadIdx.y * TX + threadIdx.x);
+        if (r < a.nrec) {
+          const long long q = a.rcc[r];
+          a.rout[2 * r] = a.Fin[q];
+          a.rout[2 * r + 1] = a.Fin[a.ps + q];
+        }
+        return;
+      }
+    }
+    const int wb = (int)blockIdx.x - a.nrecwg;
+    if (wb < a.nibwg) {
       // one lane per point (five lanes per point, as in bulk_band_fl, made
       // this launch slower: 600 workgroups ahead of the tiles instead of 112)
-      const int k = (int)blockIdx.x * (TX * TY) + (int)(threadIdx.y * TX + threadIdx.x);
+      const int k = wb * (TX * TY) + (int)(threadIdx.y * TX + threadIdx.x);
       if (k < a.nib) {
         const int2 p = a.ibpts[k];
         point_generic<T, FULL, DISS, OP>(a, p.x, p.y);
@@ -1405,7 +1425,8 @@ bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
       return;
     }
   }
-  interior_tile<T, TX, TY, FULL, NTS, HET, DISS, OP, JI, MC>(a, tiles[linear_tile(a.xcd_swizzle, a.nibwg)]);
+  interior_tile<T, TX, TY, FULL, NTS, HET, DISS, OP, JI, MC>(
+      a, tiles[linear_tile(a.xcd_swizzle, a.nrecwg + a.nibwg)]);
 }
 
 // Batched interior kernel: the tiles of many problems in one launch; tile
