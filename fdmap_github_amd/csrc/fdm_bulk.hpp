@@ -1413,7 +1413,8 @@ bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
         return;
       }
     }
-    const int wb = (int)blockIdx.x - a.nrecwg;
+    // receivers ride in FP64 launches only; the FP32 kernel keeps its offsets
+    const int wb = (int)blockIdx.x - (std::is_same<T, double>::value ? a.nrecwg : 0);
     if (wb < a.nibwg) {
       // one lane per point (five lanes per point, as in bulk_band_fl, made
       // this launch slower: 600 workgroups ahead of the tiles instead of 112)
@@ -1426,7 +1427,7 @@ bulk_interior(BulkArgs<T> a, const int4* __restrict__ tiles) {
     }
   }
   interior_tile<T, TX, TY, FULL, NTS, HET, DISS, OP, JI, MC>(
-      a, tiles[linear_tile(a.xcd_swizzle, a.nrecwg + a.nibwg)]);
+      a, tiles[linear_tile(a.xcd_swizzle, (std::is_same<T, double>::value ? a.nrecwg : 0) + a.nibwg)]);
 }
 
 // Batched interior kernel: the tiles of many problems in one launch; tile
